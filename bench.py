@@ -1,0 +1,157 @@
+#!/usr/bin/env python3
+"""k-space throughput of the MI355X engine on the n_wf=32 workload.
+
+One step = the whole k-loop of core_worker (mep_run_mesh) over the 64^3
+Monkhorst-Pack mesh of a seeded n_wf=32, NR=343 model (8 sites x 2 orbitals
+x 2 spins, non-collinear exchange, hoppings N(0,1) exp(-|R|/1.5) eV): k-point
+generation, phase tile, Fourier GEMM (FP64 MFMA), batched eigensolves,
+velocity rotation, Fermi-Dirac weights, AHC + velo/ohc, Kubo MEP, Niu MEP and
+the device reduction, ending with the sums on the host.  X(R) is uploaded
+before the timed region.
+
+Prints ONE JSON line on stdout; diagnostics go to stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+AU_TO_EV = 27.211385
+PEAK_FP64_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
+PEAK_HBM_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def workload():
+    from mepinterp_amd import models
+    m = models.make_c2(seed=2)
+    p = dict(do_ahc=True, do_kubo=True, do_mep=True, n_eF=1, eF_min=0.0, eF_max=0.0, n_hw=1,
+             hw_min=0.0, hw_max=0.0, T_kelvin=300.0, eta=1e-3 / AU_TO_EV, valence_bands=16)
+    return m, p, (64, 64, 64)
+
+
+def cpu_baseline(m, p, mp, seconds=12.0):
+    """The oracle (FP64, reference loop structure) on the host cores, over a
+    fixed seeded sample of the same mesh's k-points."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    threads = max(1, min(16, os.cpu_count() or 1))
+    om = O.Model(m["R_frac"], m["H"], a_latt=m["a_latt"], kubo_tol=1e-5)
+    rp = O.RunParams(do_ahc=p["do_ahc"], do_kubo=p["do_kubo"], do_mep=p["do_mep"], n_eF=p["n_eF"],
+                     eF_min=p["eF_min"], eF_max=p["eF_max"], n_hw=p["n_hw"], T_kelvin=p["T_kelvin"],
+                     eta=p["eta"], valence_bands=p["valence_bands"], n_threads=threads)
+    mesh = O.mp_mesh(mp)
+    rng = np.random.default_rng(1234)
+    probe = mesh[:, rng.choice(mesh.shape[1], size=threads * 2, replace=False)]
+    t0 = time.perf_counter()
+    O.accumulate(om, rp, probe)
+    per_k = (time.perf_counter() - t0) / probe.shape[1]
+    nk = int(max(threads * 4, min(mesh.shape[1], seconds / max(per_k, 1e-9))))
+    nk = (nk // threads) * threads
+    sample = mesh[:, rng.choice(mesh.shape[1], size=nk, replace=False)]
+    t0 = time.perf_counter()
+    O.accumulate(om, rp, sample)
+    dt = time.perf_counter() - t0
+    return {"value": nk / dt, "unit": "k-pts/s", "cores": threads, "kind": "port",
+            "sample": f"{nk} seeded random k-points of the 64^3 mesh, same jobs, "
+                      f"{dt:.1f} s on {threads} host threads (OpenMP over k)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
+              "(k-blocks would be sharded across GPUs with one final allreduce of the sums)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from mepinterp_amd import capi, host
+    m, p, mp = workload()
+    nk = int(np.prod(mp))
+    n, nR = m["H"].shape[0], m["H"].shape[2]
+    eng = capi.Engine(m["R_frac"], m["H"], a_latt=m["a_latt"], kubo_tol=1e-5)
+    rp = host.run_params(p)
+    for _ in range(args.warmup):
+        eng.run_mesh(mp, rp)
+    times = []
+    sums = None
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        sums = eng.run_mesh(mp, rp)
+        times.append(time.perf_counter() - t0)
+    t_med = statistics.median(times)
+    value = nk / t_med
+    log(f"[bench] step times (s): {['%.4f' % t for t in times]}")
+
+    # kernel split from HIP events on the engine's stream (separate pass)
+    eng.set_profiling(True)
+    eng.reset_times()
+    eng.run_mesh(mp, rp)
+    split = {}
+    for name in ("kgen", "phase", "ft_gemm", "eigh", "velo", "bandsum", "reduce"):
+        ms, cnt = eng.kernel_time(name)
+        split[name] = (ms, cnt)
+        log(f"[bench] {name:8s} total {ms:9.3f} ms over {cnt} launches")
+    eng.set_profiling(False)
+    gemm_ms, gemm_cnt = split["ft_gemm"]
+    n_mat = 4
+    flops_per_k = 8.0 * nR * n_mat * n * n
+    achieved = flops_per_k * nk / (gemm_ms * 1e-3) / 1e12  # TFLOP/s over all launches
+    out = host.finalize(sums, p, mp, m["a_latt"])
+    log(f"[bench] sigma_xy = {out['ahc_tens'][0, 1]:.10e}  alpha_Niu_xx = {out['mep_tens'][0, 0]:.6e}")
+
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_baseline(m, p, mp, seconds=args.cpu_seconds)
+            log(f"[bench] cpu baseline {cpu}")
+        except Exception as e:  # the baseline is reported, never the product
+            log(f"[bench] cpu baseline failed: {e}")
+    res = {
+        "metric": "k-pts/s (interp+eigh+Berry) at n_wf=32",
+        "value": value,
+        "unit": "k-pts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t_med * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "n_wf=32 NR=343 ({-3..3}^3) seeded exchange model, 64^3 MP mesh, "
+                               "AHC+velo/ohc (n_hw=1), Kubo MEP, Niu MEP (16 valence), T=300 K, "
+                               "E_F=0, FP64 path",
+                   "n_wf": n, "n_R": nR, "mp_grid": list(mp), "n_k": nk,
+                   "kernel_ms": {k: v[0] for k, v in split.items()}},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "kernel": "ft_gemm_f64_kernel",
+                     "launch_ms": gemm_ms / max(gemm_cnt, 1)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(res), flush=True)
+    eng.close()
+
+
+if __name__ == "__main__":
+    main()

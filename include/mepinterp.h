@@ -1,0 +1,134 @@
+/*
+ * mepinterp.h -- C-ABI of the MI355X (gfx950) k-space engine for MEPinterp.
+ *
+ * The reference (cMax347/MEPinterp) has no C-ABI: its boundary is a set of
+ * Fortran module procedures.  Each entry point below replaces one of them;
+ * the Fortran host in mepinterp_amd/fortran binds these through
+ * iso_c_binding and keeps the reference's call signatures.
+ *
+ * Conventions
+ *   - every array is column-major in the reference's index order, Cartesian
+ *     index fastest: V(3,n,n), Om(3,3,n,n), e_k(n,nk), kpts(3,nk), ...
+ *   - complex arrays are interleaved (re,im) doubles, binary-compatible with
+ *     Fortran complex(c_double_complex) and C double _Complex
+ *   - energies in Hartree, lengths in Bohr, k in fractional coordinates
+ *   - every function returns MEP_OK (0) or a negative MEP_ERR_* code;
+ *     mep_last_error() describes the last failure of the calling thread
+ *   - one GPU, one HIP stream per context; a context is not re-entrant
+ */
+#ifndef MEPINTERP_H
+#define MEPINTERP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MEP_OK 0
+#define MEP_ERR_ARG (-1)         /* invalid argument / unsupported size     */
+#define MEP_ERR_HIP (-2)         /* HIP runtime error                       */
+#define MEP_ERR_EIG (-3)         /* eigensolver did not converge (zheevd info != 0) */
+#define MEP_ERR_NOMEM (-4)       /* device allocation failed                */
+
+/* model flags (mep_ctx_create) */
+#define MEP_F_GAUGE_TRAFO 1      /* wannInterp%doGaugeTrafo                 */
+#define MEP_F_CART_VELO 2        /* wannInterp%use_cart_velo                */
+#define MEP_F_TEXTBOOK 4         /* PRB 74 195118 Eqs. 21-31 as printed instead
+                                    of the reference's evaluation order (only
+                                    differs with a position operator)       */
+#define MEP_F_APPLY_DEG 8        /* divide X(R) by deg(R); the reference reads
+                                    deg but never applies it                */
+
+/* band-sum jobs (mep_run_params.jobs) */
+#define MEP_JOB_AHC 1            /* jobs%do_ahc : kubo_ahc_tens, velo_ahc_tens, kubo_ohc_tens */
+#define MEP_JOB_KUBO 2           /* jobs%do_kubo: kubo_mep_IC/LC/CS          */
+#define MEP_JOB_MEP 4            /* jobs%do_mep : mep_niu_IC/LC/CS           */
+
+/* arithmetic of the k-space pipeline */
+#define MEP_PREC_FP64 0          /* FP64 MFMA Fourier GEMM, FP64 eigensolve  */
+
+typedef struct mep_ctx mep_ctx;
+
+typedef struct {
+  int32_t jobs;                  /* MEP_JOB_* bits                           */
+  int32_t n_eF;                  /* Fermi%N_eF                               */
+  double eF_min, eF_max;         /* Fermi%eF_min/max  (Hartree)              */
+  int32_t n_hw;                  /* Laser%N_hw (>= 1)                        */
+  double hw_min, hw_max;         /* Laser%hw_min/max  (Hartree)              */
+  double T_kelvin;               /* Fermi%Tkelvin                            */
+  double eta;                    /* Fermi%eta_smearing (Hartree)             */
+  int32_t valence_bands;         /* MEP%valence_bands                        */
+  int32_t precision;             /* MEP_PREC_*                               */
+} mep_run_params;
+
+/* Raw k-sums (not normalised; all Fermi levels kept).  Host buffers owned
+   by the caller; a NULL member is skipped.                                  */
+typedef struct {
+  double *Ne;          /* (n_eF)              sum_k N_el(E_F)               */
+  double *ahc;         /* (3,3,n_eF)          sum_k kubo_ahc_tens           */
+  double *ahc_velo;    /* (2,3,3,n_hw,n_eF)   sum_k velo_ahc_tens (complex) */
+  double *ohc;         /* (2,3,3,n_hw,n_eF)   sum_k kubo_ohc_tens (complex) */
+  double *kubo_ic;     /* (3,3,n_eF)          sum_k kubo_mep_IC             */
+  double *kubo_lc;     /* (3,3,n_eF)          sum_k kubo_mep_LC             */
+  double *kubo_cs;     /* (3,3,n_eF)          sum_k kubo_mep_CS             */
+  double *niu_ic;      /* (3,3,valence_bands) sum_k mep_niu_IC              */
+  double *niu_lc;      /* (3,3,valence_bands) sum_k mep_niu_LC              */
+  double *niu_cs;      /* (3,3,valence_bands) sum_k mep_niu_CS              */
+  int64_t n_k;         /* out: number of k-points summed                    */
+} mep_sums;
+
+/* Upload the real-space model once.  Replaces the model half of
+   read_tb_basis (file_io.f90:800-860) + kspace_allocator
+   (k_space.f90:219-236).
+     R_frac(3,n_R); H_real(n,n,n_R) [Hartree]; r_real(3,n,n,n_R) [Bohr] or
+     NULL; wf_centers(3,n) or NULL (NULL = Wannier convention, given =
+     tight-binding convention, wann_interp.f90:192-280); deg(n_R) or NULL;
+     a_latt(3,3) with a_latt(i,:) = a_i (input_paras.f90:213-216).           */
+int mep_ctx_create(int64_t n_wf, int64_t n_R, const double *R_frac, const double *wf_centers,
+                   const double *H_real, const double *r_real, const int64_t *deg,
+                   const double *a_latt, double kubo_tol, int32_t flags, mep_ctx **ctx);
+int mep_ctx_destroy(mep_ctx *ctx);
+const char *mep_last_error(void);
+
+/* get_wann_interp (wann_interp.f90:56-108) for nk k-points at once.
+     kpts(3,nk) -> e_k(n,nk), V_ka(3,n,n,nk) or NULL,
+     A_ka(3,n,n,nk), Om_kab(3,3,n,n,nk) (only with a position operator;
+     otherwise NULL).                                                        */
+int mep_interp_batch(mep_ctx *ctx, int64_t nk, const double *kpts, double *e_k, double *V_ka,
+                     double *A_ka, double *Om_kab);
+
+/* The band sums of core.f90:187-230 over nk k-points whose e_k, V_ka (and
+   A_ka, Om_kab) the caller provides: fd_stat + fd_get_N_el
+   (statistics.f90:65-124), kubo_ahc_tens / velo_ahc_tens / kubo_ohc_tens
+   (kubo.f90:37-198), kubo_mep_IC/LC/CS (kubo_mep.f90:16-149),
+   mep_niu_IC/LC/CS (mep_niu.f90:20-159).  With nk = 1 the outputs are the
+   reference functions' results.                                             */
+int mep_bandsum_batch(mep_ctx *ctx, int64_t nk, const double *e_k, const double *V_ka,
+                      const double *A_ka, const double *Om_kab, const mep_run_params *p,
+                      mep_sums *out);
+
+/* The whole k-loop of core_worker (core.f90:165-270) over the
+   Monkhorst-Pack mesh mp_grid (k_space.f90:341-375), on the device; the
+   raw sums come back to the host.  fermi_selector and normalize_k_int
+   (core.f90:317-603) stay with the caller.                                  */
+int mep_run_mesh(mep_ctx *ctx, const int32_t mp_grid[3], const mep_run_params *p, mep_sums *out);
+
+/* Same over an explicit k list (3,nk) with optional weights(nk) (NULL = 1):
+   sub-mesh refinement and sampled baselines.                                */
+int mep_run_kpts(mep_ctx *ctx, int64_t nk, const double *kpts, const double *weights,
+                 const mep_run_params *p, mep_sums *out);
+
+/* Kernel timing with HIP events on the context's stream (off by default).
+   name: "phase", "ft_gemm", "eigh", "velo", "bandsum", "reduce", "kgen".   */
+int mep_set_profiling(mep_ctx *ctx, int32_t on);
+int mep_get_kernel_time(mep_ctx *ctx, const char *name, double *total_ms, int64_t *launches);
+int mep_reset_kernel_times(mep_ctx *ctx);
+
+/* Largest k-block the context processes per pass (device workspace bound). */
+int mep_set_kblock(mep_ctx *ctx, int64_t kblock);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

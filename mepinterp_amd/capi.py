@@ -1,0 +1,226 @@
+"""ctypes binding of the C-ABI in include/mepinterp.h (libmepinterp_amd.so).
+
+This is the Python-side counterpart of the Fortran iso_c_binding module: it
+loads the in-tree gfx950 library and fails loudly when it is missing.  There
+is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmepinterp_amd.so")
+
+MEP_OK = 0
+MEP_F_GAUGE_TRAFO = 1
+MEP_F_CART_VELO = 2
+MEP_F_TEXTBOOK = 4
+MEP_F_APPLY_DEG = 8
+MEP_JOB_AHC = 1
+MEP_JOB_KUBO = 2
+MEP_JOB_MEP = 4
+MEP_PREC_FP64 = 0
+
+# every symbol include/mepinterp.h declares
+EXPORTS = ("mep_ctx_create", "mep_ctx_destroy", "mep_last_error", "mep_interp_batch",
+           "mep_bandsum_batch", "mep_run_mesh", "mep_run_kpts", "mep_set_profiling",
+           "mep_get_kernel_time", "mep_reset_kernel_times", "mep_set_kblock")
+
+c_dp = C.POINTER(C.c_double)
+
+
+class RunParams(C.Structure):
+    _fields_ = [("jobs", C.c_int32), ("n_eF", C.c_int32), ("eF_min", C.c_double),
+                ("eF_max", C.c_double), ("n_hw", C.c_int32), ("hw_min", C.c_double),
+                ("hw_max", C.c_double), ("T_kelvin", C.c_double), ("eta", C.c_double),
+                ("valence_bands", C.c_int32), ("precision", C.c_int32)]
+
+
+class Sums(C.Structure):
+    _fields_ = [("Ne", c_dp), ("ahc", c_dp), ("ahc_velo", c_dp), ("ohc", c_dp),
+                ("kubo_ic", c_dp), ("kubo_lc", c_dp), ("kubo_cs", c_dp),
+                ("niu_ic", c_dp), ("niu_lc", c_dp), ("niu_cs", c_dp), ("n_k", C.c_int64)]
+
+
+_lib = None
+
+
+class MepError(RuntimeError):
+    pass
+
+
+def load(path: str = LIB_PATH):
+    """Load libmepinterp_amd.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise MepError(f"{path} is missing: build it with `python -m mepinterp_amd.build`")
+    lib = C.CDLL(path)
+    lib.mep_last_error.restype = C.c_char_p
+    for name in EXPORTS:
+        if name != "mep_last_error":
+            getattr(lib, name).restype = C.c_int
+    lib.mep_ctx_create.argtypes = [C.c_int64, C.c_int64, c_dp, c_dp, c_dp, c_dp,
+                                   C.POINTER(C.c_int64), c_dp, C.c_double, C.c_int32,
+                                   C.POINTER(C.c_void_p)]
+    lib.mep_ctx_destroy.argtypes = [C.c_void_p]
+    lib.mep_interp_batch.argtypes = [C.c_void_p, C.c_int64, c_dp, c_dp, c_dp, c_dp, c_dp]
+    lib.mep_bandsum_batch.argtypes = [C.c_void_p, C.c_int64, c_dp, c_dp, c_dp, c_dp,
+                                      C.POINTER(RunParams), C.POINTER(Sums)]
+    lib.mep_run_mesh.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(RunParams),
+                                 C.POINTER(Sums)]
+    lib.mep_run_kpts.argtypes = [C.c_void_p, C.c_int64, c_dp, c_dp, C.POINTER(RunParams),
+                                 C.POINTER(Sums)]
+    lib.mep_set_profiling.argtypes = [C.c_void_p, C.c_int32]
+    lib.mep_get_kernel_time.argtypes = [C.c_void_p, C.c_char_p, c_dp, C.POINTER(C.c_int64)]
+    lib.mep_reset_kernel_times.argtypes = [C.c_void_p]
+    lib.mep_set_kblock.argtypes = [C.c_void_p, C.c_int64]
+    _lib = lib
+    return lib
+
+
+def _p(a):
+    if a is None:
+        return None
+    return a.ctypes.data_as(c_dp)
+
+
+def _check(rc, what):
+    if rc != MEP_OK:
+        msg = load().mep_last_error().decode(errors="replace")
+        raise MepError(f"{what} failed (rc={rc}): {msg}")
+
+
+def _f(a, dtype):
+    return None if a is None else np.asfortranarray(a, dtype=dtype)
+
+
+class Engine:
+    """A model resident on the GPU (one mep_ctx).
+
+    Arguments follow the reference's arrays: R_frac (3,nR), H (n,n,nR)
+    Hartree, r (3,n,n,nR) Bohr or None, wf_centers (3,n) or None, a_latt
+    with rows a_i (Bohr).
+    """
+
+    def __init__(self, R_frac, H, r=None, wf_centers=None, a_latt=None, kubo_tol=1e-5,
+                 gauge_trafo=True, cart_velo=False, textbook=False, deg=None, apply_deg=False):
+        lib = load()
+        self.R = _f(R_frac, np.float64)
+        self.H = _f(H, np.complex128)
+        self.r = _f(r, np.complex128)
+        self.tau = _f(wf_centers, np.float64)
+        self.a_latt = np.asfortranarray(np.eye(3) if a_latt is None else a_latt, dtype=np.float64)
+        self.n = self.H.shape[0]
+        self.nR = self.H.shape[2]
+        self.pos = self.r is not None
+        flags = (MEP_F_GAUGE_TRAFO if gauge_trafo else 0) | (MEP_F_CART_VELO if cart_velo else 0) \
+            | (MEP_F_TEXTBOOK if textbook else 0) | (MEP_F_APPLY_DEG if apply_deg else 0)
+        d = None if deg is None else np.ascontiguousarray(deg, dtype=np.int64)
+        ctx = C.c_void_p()
+        rc = lib.mep_ctx_create(self.n, self.nR, _p(self.R), _p(self.tau), _p(self.H), _p(self.r),
+                                None if d is None else d.ctypes.data_as(C.POINTER(C.c_int64)),
+                                _p(self.a_latt), float(kubo_tol), flags, C.byref(ctx))
+        _check(rc, "mep_ctx_create")
+        self.ctx = ctx
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            load().mep_ctx_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- get_wann_interp over a batch ------------------------------------
+    def interp(self, kpts, velo=True):
+        k = np.asfortranarray(kpts, dtype=np.float64).reshape(3, -1, order="F")
+        nk, n = k.shape[1], self.n
+        e = np.zeros((n, nk), order="F")
+        V = np.zeros((3, n, n, nk), np.complex128, order="F") if velo else None
+        A = np.zeros((3, n, n, nk), np.complex128, order="F") if (velo and self.pos) else None
+        Om = np.zeros((3, 3, n, n, nk), np.complex128, order="F") if (velo and self.pos) else None
+        rc = load().mep_interp_batch(self.ctx, nk, _p(k), _p(e), _p(V), _p(A), _p(Om))
+        _check(rc, "mep_interp_batch")
+        return e, V, A, Om
+
+    @staticmethod
+    def params(jobs=MEP_JOB_AHC, n_eF=1, eF_min=0.0, eF_max=0.0, n_hw=1, hw_min=0.0, hw_max=0.0,
+               T_kelvin=0.0, eta=0.0, valence_bands=1, precision=MEP_PREC_FP64):
+        return RunParams(int(jobs), int(n_eF), float(eF_min), float(eF_max), int(max(1, n_hw)),
+                         float(hw_min), float(hw_max), float(T_kelvin), float(eta),
+                         int(valence_bands), int(precision))
+
+    @staticmethod
+    def _sums_buffers(p: RunParams):
+        nef, nhw, nv = p.n_eF, p.n_hw, max(1, p.valence_bands)
+        out = {"Ne": np.zeros(nef), "ahc": np.zeros((3, 3, nef), order="F"),
+               "ahc_velo": np.zeros((3, 3, nhw, nef), np.complex128, order="F"),
+               "ohc": np.zeros((3, 3, nhw, nef), np.complex128, order="F"),
+               "kubo_ic": np.zeros((3, 3, nef), order="F"),
+               "kubo_lc": np.zeros((3, 3, nef), order="F"),
+               "kubo_cs": np.zeros((3, 3, nef), order="F"),
+               "niu_ic": np.zeros((3, 3, nv), order="F"),
+               "niu_lc": np.zeros((3, 3, nv), order="F"),
+               "niu_cs": np.zeros((3, 3, nv), order="F")}
+        s = Sums()
+        for k in out:
+            setattr(s, k, _p(out[k]))
+        return out, s
+
+    def bandsum(self, e, V, p: RunParams, A=None, Om=None):
+        e = np.asfortranarray(e, dtype=np.float64).reshape(self.n, -1, order="F")
+        V = np.asfortranarray(V, dtype=np.complex128)
+        out, s = self._sums_buffers(p)
+        rc = load().mep_bandsum_batch(self.ctx, e.shape[1], _p(e), _p(V), _p(_f(A, np.complex128)),
+                                      _p(_f(Om, np.complex128)), C.byref(p), C.byref(s))
+        _check(rc, "mep_bandsum_batch")
+        out["n_k"] = s.n_k
+        return out
+
+    def run_mesh(self, mp_grid, p: RunParams):
+        mp = (C.c_int32 * 3)(*[int(x) for x in mp_grid])
+        out, s = self._sums_buffers(p)
+        rc = load().mep_run_mesh(self.ctx, mp, C.byref(p), C.byref(s))
+        _check(rc, "mep_run_mesh")
+        out["n_k"] = s.n_k
+        return out
+
+    def run_kpts(self, kpts, p: RunParams, weights=None):
+        k = np.asfortranarray(kpts, dtype=np.float64).reshape(3, -1, order="F")
+        w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+        out, s = self._sums_buffers(p)
+        rc = load().mep_run_kpts(self.ctx, k.shape[1], _p(k), _p(w), C.byref(p), C.byref(s))
+        _check(rc, "mep_run_kpts")
+        out["n_k"] = s.n_k
+        return out
+
+    # -- timing ------------------------------------------------------------
+    def set_profiling(self, on=True):
+        _check(load().mep_set_profiling(self.ctx, 1 if on else 0), "mep_set_profiling")
+
+    def reset_times(self):
+        _check(load().mep_reset_kernel_times(self.ctx), "mep_reset_kernel_times")
+
+    def kernel_time(self, name):
+        ms = C.c_double()
+        cnt = C.c_int64()
+        _check(load().mep_get_kernel_time(self.ctx, name.encode(), C.byref(ms), C.byref(cnt)),
+               "mep_get_kernel_time")
+        return ms.value, cnt.value
+
+    def set_kblock(self, kb):
+        _check(load().mep_set_kblock(self.ctx, int(kb)), "mep_set_kblock")

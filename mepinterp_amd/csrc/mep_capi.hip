@@ -1,0 +1,561 @@
+// mep_capi.hip -- the C-ABI of include/mepinterp.h: model upload, k-block
+// workspace, the per-block pipeline (phase -> Fourier GEMM -> eigensolve ->
+// velocity -> band sums -> reduction) and kernel timing.
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/mepinterp.h"
+#include "mep_common.hpp"
+#include "mep_kernels.hpp"
+
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(x)                                                                       \
+  do {                                                                                  \
+    hipError_t e_ = (x);                                                                \
+    if (e_ != hipSuccess)                                                               \
+      return fail(MEP_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_));         \
+  } while (0)
+
+#define CHK(x)              \
+  do {                      \
+    int r_ = (x);           \
+    if (r_ != MEP_OK) return r_; \
+  } while (0)
+
+enum Kind { K_KGEN = 0, K_PHASE, K_GEMM, K_EIGH, K_VELO, K_BANDSUM, K_REDUCE, K_NKIND };
+static const char* kKindName[K_NKIND] = {"kgen", "phase", "ft_gemm", "eigh", "velo", "bandsum", "reduce"};
+
+struct EvPair {
+  hipEvent_t a, b;
+  int kind;
+};
+
+size_t bandsum_lds_bytes(const BandArgs& b);
+void launch_fill_zero(hipStream_t s, double* p, int64_t n);
+
+struct mep_ctx {
+  int64_t n = 0, nR = 0;
+  int32_t flags = 0;
+  bool pos = false, tb = false;
+  double al[9];
+  double kubo_tol = 0;
+  double cmat[9];
+  int nmat = 0;
+  int64_t ncols = 0;
+  double* dR = nullptr;
+  double* dtau = nullptr;
+  zd* dX = nullptr;
+  // k-block workspace
+  int64_t kcap = 0, kblock_req = 0;
+  double* dk = nullptr;
+  double* dw = nullptr;
+  zd* dP = nullptr;
+  zd* dHk = nullptr;
+  double* dE = nullptr;
+  zd* dU = nullptr;
+  zd* dV = nullptr;
+  zd* dA = nullptr;
+  zd* dOm = nullptr;
+  int* dinfo = nullptr;
+  double* dpart = nullptr;
+  int64_t part_cap = 0;
+  double* dacc = nullptr;
+  int64_t acc_cap = 0;
+  double* def = nullptr;
+  double* dhw = nullptr;
+  int64_t ef_cap = 0, hw_cap = 0;
+  hipStream_t st = nullptr;
+  // profiling
+  bool prof = false;
+  std::vector<EvPair> pending;
+  std::vector<hipEvent_t> pool;
+  double tms[K_NKIND] = {0};
+  int64_t tcnt[K_NKIND] = {0};
+};
+
+// ---------------------------------------------------------------------------
+// timing helpers
+// ---------------------------------------------------------------------------
+static hipEvent_t getev(mep_ctx* c) {
+  if (!c->pool.empty()) {
+    hipEvent_t e = c->pool.back();
+    c->pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  hipEventCreate(&e);
+  return e;
+}
+
+template <class F>
+static void timed(mep_ctx* c, int kind, F&& f) {
+  if (!c->prof) {
+    f();
+    return;
+  }
+  hipEvent_t a = getev(c), b = getev(c);
+  hipEventRecord(a, c->st);
+  f();
+  hipEventRecord(b, c->st);
+  c->pending.push_back({a, b, kind});
+}
+
+static void resolve_times(mep_ctx* c) {
+  for (auto& p : c->pending) {
+    hipEventSynchronize(p.b);
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, p.a, p.b);
+    c->tms[p.kind] += ms;
+    c->tcnt[p.kind] += 1;
+    c->pool.push_back(p.a);
+    c->pool.push_back(p.b);
+  }
+  c->pending.clear();
+}
+
+// ---------------------------------------------------------------------------
+// workspace
+// ---------------------------------------------------------------------------
+static void free_ws(mep_ctx* c) {
+  hipFree(c->dk); hipFree(c->dw); hipFree(c->dP); hipFree(c->dHk); hipFree(c->dE);
+  hipFree(c->dU); hipFree(c->dV); hipFree(c->dA); hipFree(c->dOm);
+  c->dk = c->dw = nullptr; c->dP = c->dHk = c->dU = c->dV = c->dA = c->dOm = nullptr;
+  c->dE = nullptr;
+  c->kcap = 0;
+}
+
+static int64_t bytes_per_k(const mep_ctx* c) {
+  int64_t n2 = c->n * c->n;
+  int64_t b = 3 * 8 + 8 + c->nR * 16 + (int64_t)c->nmat * n2 * 16 + c->n * 8 + n2 * 16 + 3 * n2 * 16;
+  if (c->pos) b += 12 * n2 * 16;
+  return b;
+}
+
+static int ensure_ws(mep_ctx* c, int64_t nk_total) {
+  int64_t want = c->kblock_req > 0 ? c->kblock_req : (int64_t)(3.0e9 / (double)bytes_per_k(c));
+  if (want > nk_total) want = nk_total;
+  want = ((want + 63) / 64) * 64;
+  if (want < 64) want = 64;
+  if (want <= c->kcap) return MEP_OK;
+  free_ws(c);
+  int64_t n2 = c->n * c->n;
+  HIPCHK(hipMalloc(&c->dk, sizeof(double) * 3 * want));
+  HIPCHK(hipMalloc(&c->dw, sizeof(double) * want));
+  HIPCHK(hipMalloc(&c->dP, sizeof(zd) * c->nR * want));
+  HIPCHK(hipMalloc(&c->dHk, sizeof(zd) * c->nmat * n2 * want));
+  HIPCHK(hipMalloc(&c->dE, sizeof(double) * c->n * want));
+  HIPCHK(hipMalloc(&c->dU, sizeof(zd) * n2 * want));
+  HIPCHK(hipMalloc(&c->dV, sizeof(zd) * 3 * n2 * want));
+  if (c->pos) {
+    HIPCHK(hipMalloc(&c->dA, sizeof(zd) * 3 * n2 * want));
+    HIPCHK(hipMalloc(&c->dOm, sizeof(zd) * 9 * n2 * want));
+  }
+  c->kcap = want;
+  return MEP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+extern "C" const char* mep_last_error(void) { return g_err.c_str(); }
+
+extern "C" int mep_ctx_create(int64_t n_wf, int64_t n_R, const double* R_frac,
+                              const double* wf_centers, const double* H_real, const double* r_real,
+                              const int64_t* deg, const double* a_latt, double kubo_tol,
+                              int32_t flags, mep_ctx** out) {
+  if (!out) return fail(MEP_ERR_ARG, "mep_ctx_create: ctx pointer is NULL");
+  *out = nullptr;
+  if (n_wf <= 0 || n_R <= 0 || !R_frac || !H_real || !a_latt)
+    return fail(MEP_ERR_ARG, "mep_ctx_create: n_wf, n_R, R_frac, H_real and a_latt are required");
+  if (n_wf > 32) return fail(MEP_ERR_ARG, "mep_ctx_create: n_wf > 32 is not supported");
+  if (r_real && n_wf > 16)
+    return fail(MEP_ERR_ARG, "mep_ctx_create: a position operator needs n_wf <= 16");
+  if (deg && (flags & MEP_F_APPLY_DEG))
+    for (int64_t s = 0; s < n_R; ++s)
+      if (deg[s] <= 0) return fail(MEP_ERR_ARG, "mep_ctx_create: deg(R) must be positive");
+  int dev_count = 0;
+  if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count <= 0)
+    return fail(MEP_ERR_HIP, "mep_ctx_create: no HIP device");
+
+  mep_ctx* c = new mep_ctx();
+  c->n = n_wf;
+  c->nR = n_R;
+  c->flags = flags;
+  c->pos = (r_real != nullptr);
+  c->tb = (wf_centers != nullptr);
+  c->kubo_tol = kubo_tol;
+  for (int i = 0; i < 9; ++i) c->al[i] = a_latt[i];
+  // c(dR) = 2 pi dR (internal) or intern_to_cart = matmul(a_latt, dR)
+  // (wann_interp.f90:215-233, 387-399); row-major cmat[3i+j] = a_latt(i,j)
+  const double two_pi = 2.0 * 3.14159265358979323846;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      c->cmat[3 * i + j] = (flags & MEP_F_CART_VELO) ? a_latt[i + 3 * j] : (i == j ? two_pi : 0.0);
+  c->nmat = c->pos ? 10 : 4;
+  const int64_t n = n_wf, n2 = n * n;
+  c->ncols = c->nmat * n2;
+
+  // X(R): columns [H | i c_a H | r_a | i (c_a r_b - c_b r_a)]
+  std::vector<zd> X((size_t)(n_R * c->ncols));
+  const zd* H = reinterpret_cast<const zd*>(H_real);
+  const zd* r = reinterpret_cast<const zd*>(r_real);
+  for (int64_t s = 0; s < n_R; ++s) {
+    double cv[3];
+    for (int a = 0; a < 3; ++a)
+      cv[a] = c->cmat[3 * a] * R_frac[3 * s] + c->cmat[3 * a + 1] * R_frac[3 * s + 1] +
+              c->cmat[3 * a + 2] * R_frac[3 * s + 2];
+    double sc = (deg && (flags & MEP_F_APPLY_DEG)) ? 1.0 / (double)deg[s] : 1.0;
+    zd* Xs = &X[(size_t)(s * c->ncols)];
+    for (int64_t o = 0; o < n2; ++o) {
+      zd h = zscale(H[o + n2 * s], sc);
+      Xs[o] = h;
+      for (int a = 0; a < 3; ++a) Xs[(1 + a) * n2 + o] = zmuli(zscale(h, cv[a]));
+      if (c->pos) {
+        const zd* rr = &r[3 * (o + n2 * s)];
+        zd ra[3] = {zscale(rr[0], sc), zscale(rr[1], sc), zscale(rr[2], sc)};
+        for (int a = 0; a < 3; ++a) Xs[(4 + a) * n2 + o] = ra[a];
+        const int AA[3] = {1, 2, 0}, BB[3] = {2, 0, 1};
+        for (int q = 0; q < 3; ++q) {
+          int a = AA[q], b = BB[q];
+          Xs[(7 + q) * n2 + o] = zmuli(zsub(zscale(ra[b], cv[a]), zscale(ra[a], cv[b])));
+        }
+      }
+    }
+  }
+  auto bad = [&](hipError_t e, const char* what) {
+    g_err = std::string(what) + ": " + hipGetErrorString(e);
+    mep_ctx_destroy(c);
+    return MEP_ERR_HIP;
+  };
+  hipError_t e;
+  if ((e = hipStreamCreate(&c->st)) != hipSuccess) return bad(e, "hipStreamCreate");
+  if ((e = hipMalloc(&c->dR, sizeof(double) * 3 * n_R)) != hipSuccess) return bad(e, "hipMalloc R");
+  if ((e = hipMemcpy(c->dR, R_frac, sizeof(double) * 3 * n_R, hipMemcpyHostToDevice)) != hipSuccess)
+    return bad(e, "upload R");
+  if (c->tb) {
+    if ((e = hipMalloc(&c->dtau, sizeof(double) * 3 * n)) != hipSuccess) return bad(e, "hipMalloc tau");
+    if ((e = hipMemcpy(c->dtau, wf_centers, sizeof(double) * 3 * n, hipMemcpyHostToDevice)) != hipSuccess)
+      return bad(e, "upload tau");
+  }
+  if ((e = hipMalloc(&c->dX, sizeof(zd) * X.size())) != hipSuccess) return bad(e, "hipMalloc X");
+  if ((e = hipMemcpy(c->dX, X.data(), sizeof(zd) * X.size(), hipMemcpyHostToDevice)) != hipSuccess)
+    return bad(e, "upload X");
+  if ((e = hipMalloc(&c->dinfo, sizeof(int))) != hipSuccess) return bad(e, "hipMalloc info");
+  if ((e = hipMemset(c->dinfo, 0, sizeof(int))) != hipSuccess) return bad(e, "memset info");
+  *out = c;
+  return MEP_OK;
+}
+
+extern "C" int mep_ctx_destroy(mep_ctx* c) {
+  if (!c) return MEP_OK;
+  if (c->st) hipStreamSynchronize(c->st);
+  resolve_times(c);
+  free_ws(c);
+  hipFree(c->dR); hipFree(c->dtau); hipFree(c->dX); hipFree(c->dinfo);
+  hipFree(c->dpart); hipFree(c->dacc); hipFree(c->def); hipFree(c->dhw);
+  for (auto e : c->pool) hipEventDestroy(e);
+  if (c->st) hipStreamDestroy(c->st);
+  delete c;
+  return MEP_OK;
+}
+
+extern "C" int mep_set_kblock(mep_ctx* c, int64_t kb) {
+  if (!c || kb < 0) return fail(MEP_ERR_ARG, "mep_set_kblock: bad argument");
+  c->kblock_req = kb;
+  if (kb > 0 && kb < c->kcap) free_ws(c);
+  return MEP_OK;
+}
+
+extern "C" int mep_set_profiling(mep_ctx* c, int32_t on) {
+  if (!c) return fail(MEP_ERR_ARG, "mep_set_profiling: ctx is NULL");
+  c->prof = on != 0;
+  return MEP_OK;
+}
+
+extern "C" int mep_reset_kernel_times(mep_ctx* c) {
+  if (!c) return fail(MEP_ERR_ARG, "mep_reset_kernel_times: ctx is NULL");
+  hipStreamSynchronize(c->st);
+  resolve_times(c);
+  for (int i = 0; i < K_NKIND; ++i) {
+    c->tms[i] = 0;
+    c->tcnt[i] = 0;
+  }
+  return MEP_OK;
+}
+
+extern "C" int mep_get_kernel_time(mep_ctx* c, const char* name, double* ms, int64_t* cnt) {
+  if (!c || !name) return fail(MEP_ERR_ARG, "mep_get_kernel_time: bad argument");
+  HIPCHK(hipStreamSynchronize(c->st));
+  resolve_times(c);
+  for (int i = 0; i < K_NKIND; ++i)
+    if (strcmp(name, kKindName[i]) == 0) {
+      if (ms) *ms = c->tms[i];
+      if (cnt) *cnt = c->tcnt[i];
+      return MEP_OK;
+    }
+  return fail(MEP_ERR_ARG, std::string("mep_get_kernel_time: unknown kernel ") + name);
+}
+
+// ---------------------------------------------------------------------------
+// pipeline pieces
+// ---------------------------------------------------------------------------
+static KGeom geom(const mep_ctx* c) {
+  KGeom g;
+  g.kpts = c->dk;
+  g.tau = c->tb ? c->dtau : nullptr;
+  for (int i = 0; i < 9; ++i) g.cmat[i] = c->cmat[i];
+  return g;
+}
+
+static int interp_block(mep_ctx* c, int64_t nb, bool velo) {
+  const int64_t n = c->n, n2 = n * n;
+  timed(c, K_PHASE, [&] { launch_phase(c->st, nb, (int)c->nR, c->dk, c->dR, c->dP); });
+  timed(c, K_GEMM, [&] {
+    launch_ft_gemm_f64(c->st, (int)nb, (int)c->ncols, (int)c->nR, c->dP, c->dX, c->dHk);
+  });
+  KGeom g = geom(c);
+  timed(c, K_EIGH, [&] {
+    launch_eigh(c->st, (int)n, nb, c->dHk, c->nmat * n2, g, c->dE, c->dU, c->dinfo);
+  });
+  if (velo) {
+    VeloArgs va;
+    va.n = (int)n;
+    va.nmat = c->nmat;
+    va.gauge_trafo = (c->flags & MEP_F_GAUGE_TRAFO) ? 1 : 0;
+    va.pos_op = c->pos ? 1 : 0;
+    va.textbook = (c->flags & MEP_F_TEXTBOOK) ? 1 : 0;
+    va.kubo_tol = c->kubo_tol;
+    timed(c, K_VELO, [&] {
+      launch_velo(c->st, va, nb, c->dHk, c->nmat * n2, g, c->dU, c->dE, c->dV, c->dA, c->dOm);
+    });
+  }
+  HIPCHK(hipGetLastError());
+  return MEP_OK;
+}
+
+static int check_info(mep_ctx* c) {
+  int info = 0;
+  HIPCHK(hipMemcpyAsync(&info, c->dinfo, sizeof(int), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  if (info) {
+    hipMemsetAsync(c->dinfo, 0, sizeof(int), c->st);
+    hipStreamSynchronize(c->st);
+    return fail(MEP_ERR_EIG, "eigensolver did not converge (zheevd info != 0 equivalent)");
+  }
+  return MEP_OK;
+}
+
+static void linspace(double lo, double hi, int n, std::vector<double>& out) {
+  // get_linspace (matrix_math.f90:74-100)
+  out.assign(n > 1 ? n : 1, lo);
+  if (n > 1) {
+    double d = (hi - lo) / (double)(n - 1);
+    for (int i = 0; i < n; ++i) out[i] = lo + (double)i * d;
+  }
+}
+
+struct BandSetup {
+  BandArgs b;
+  AccLayout L;
+  int nblocks;
+};
+
+static int band_setup(mep_ctx* c, const mep_run_params* p, int64_t kblock, BandSetup& bs) {
+  if (!p) return fail(MEP_ERR_ARG, "run params are NULL");
+  if (p->n_eF < 1 || p->n_hw < 1) return fail(MEP_ERR_ARG, "n_eF and n_hw must be >= 1");
+  if ((p->jobs & MEP_JOB_MEP) && (p->valence_bands < 1 || p->valence_bands > c->n))
+    return fail(MEP_ERR_ARG, "valence_bands must be in [1, n_wf] for the Niu MEP tensor");
+  if (p->precision != MEP_PREC_FP64) return fail(MEP_ERR_ARG, "unknown precision");
+  BandArgs& b = bs.b;
+  b.n = (int)c->n;
+  b.jobs = p->jobs;
+  b.n_ef = p->n_eF;
+  b.n_hw = p->n_hw;
+  b.n_val = p->valence_bands > 0 ? p->valence_bands : 1;
+  b.T_kelvin = p->T_kelvin;
+  b.eta = p->eta;
+  b.kubo_tol = c->kubo_tol;
+  b.pos_op = c->pos ? 1 : 0;
+  std::vector<double> ef, hw;
+  linspace(p->eF_min, p->eF_max, p->n_eF, ef);
+  linspace(p->hw_min, p->hw_max, p->n_hw, hw);
+  if ((int64_t)ef.size() > c->ef_cap) {
+    hipFree(c->def);
+    HIPCHK(hipMalloc(&c->def, sizeof(double) * ef.size()));
+    c->ef_cap = ef.size();
+  }
+  if ((int64_t)hw.size() > c->hw_cap) {
+    hipFree(c->dhw);
+    HIPCHK(hipMalloc(&c->dhw, sizeof(double) * hw.size()));
+    c->hw_cap = hw.size();
+  }
+  HIPCHK(hipMemcpy(c->def, ef.data(), sizeof(double) * ef.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->dhw, hw.data(), sizeof(double) * hw.size(), hipMemcpyHostToDevice));
+  b.ef = c->def;
+  b.hw = c->dhw;
+  if (bandsum_lds_bytes(b) > 160 * 1024)
+    return fail(MEP_ERR_ARG, "band-sum accumulators exceed the 160 KiB LDS of one workgroup "
+                             "(reduce N_hw * N_eF)");
+  bs.L = acc_layout(b);
+  bs.nblocks = (int)(kblock < 1024 ? kblock : 1024);
+  int64_t need = (int64_t)bs.nblocks * bs.L.total;
+  if (need > c->part_cap) {
+    hipFree(c->dpart);
+    HIPCHK(hipMalloc(&c->dpart, sizeof(double) * need));
+    c->part_cap = need;
+  }
+  if (bs.L.total > c->acc_cap) {
+    hipFree(c->dacc);
+    HIPCHK(hipMalloc(&c->dacc, sizeof(double) * bs.L.total));
+    c->acc_cap = bs.L.total;
+  }
+  launch_fill_zero(c->st, c->dacc, bs.L.total);
+  HIPCHK(hipGetLastError());
+  return MEP_OK;
+}
+
+static int band_block(mep_ctx* c, BandSetup& bs, int64_t nb, const double* w) {
+  int nblk = (int)(nb < bs.nblocks ? nb : bs.nblocks);
+  timed(c, K_BANDSUM, [&] {
+    launch_bandsum(c->st, bs.b, nb, c->dE, c->dV, c->dA, c->dOm, w, nblk, c->dpart);
+  });
+  timed(c, K_REDUCE, [&] { launch_reduce_parts(c->st, nblk, bs.L.total, c->dpart, c->dacc); });
+  HIPCHK(hipGetLastError());
+  return MEP_OK;
+}
+
+static int unpack(mep_ctx* c, const BandSetup& bs, const mep_run_params* p, int64_t nk,
+                  mep_sums* out) {
+  std::vector<double> h(bs.L.total);
+  HIPCHK(hipMemcpyAsync(h.data(), c->dacc, sizeof(double) * bs.L.total, hipMemcpyDeviceToHost,
+                        c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  resolve_times(c);
+  if (!out) return MEP_OK;
+  const int64_t nef = p->n_eF, nhw = p->n_hw, nv = bs.b.n_val;
+  auto cp = [&](double* dst, int64_t off, int64_t len) {
+    if (dst) memcpy(dst, h.data() + off, sizeof(double) * len);
+  };
+  cp(out->Ne, bs.L.Ne, nef);
+  cp(out->ahc, bs.L.ahc, 9 * nef);
+  cp(out->ahc_velo, bs.L.velo, 18 * nhw * nef);
+  cp(out->ohc, bs.L.ohc, 18 * nhw * nef);
+  cp(out->kubo_ic, bs.L.kic, 9 * nef);
+  cp(out->kubo_lc, bs.L.klc, 9 * nef);
+  cp(out->kubo_cs, bs.L.kcs, 9 * nef);
+  if (p->jobs & MEP_JOB_MEP) {
+    cp(out->niu_ic, bs.L.nic, 9 * nv);
+    cp(out->niu_lc, bs.L.nlc, 9 * nv);
+    cp(out->niu_cs, bs.L.ncs, 9 * nv);
+  }
+  out->n_k = nk;
+  return MEP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// entry points
+// ---------------------------------------------------------------------------
+extern "C" int mep_interp_batch(mep_ctx* c, int64_t nk, const double* kpts, double* e_k,
+                                double* V_ka, double* A_ka, double* Om_kab) {
+  if (!c || nk < 0 || (nk > 0 && (!kpts || !e_k)))
+    return fail(MEP_ERR_ARG, "mep_interp_batch: bad argument");
+  if (nk == 0) return MEP_OK;
+  if (c->pos && V_ka && (!A_ka || !Om_kab))
+    return fail(MEP_ERR_ARG, "mep_interp_batch: a position operator needs A_ka and Om_kab");
+  CHK(ensure_ws(c, nk));
+  const int64_t n = c->n, n2 = n * n;
+  for (int64_t k0 = 0; k0 < nk; k0 += c->kcap) {
+    int64_t nb = (nk - k0 < c->kcap) ? nk - k0 : c->kcap;
+    HIPCHK(hipMemcpyAsync(c->dk, kpts + 3 * k0, sizeof(double) * 3 * nb, hipMemcpyHostToDevice,
+                          c->st));
+    CHK(interp_block(c, nb, V_ka != nullptr));
+    HIPCHK(hipMemcpyAsync(e_k + n * k0, c->dE, sizeof(double) * n * nb, hipMemcpyDeviceToHost,
+                          c->st));
+    if (V_ka)
+      HIPCHK(hipMemcpyAsync(V_ka + 2 * 3 * n2 * k0, c->dV, sizeof(zd) * 3 * n2 * nb,
+                            hipMemcpyDeviceToHost, c->st));
+    if (V_ka && c->pos) {
+      HIPCHK(hipMemcpyAsync(A_ka + 2 * 3 * n2 * k0, c->dA, sizeof(zd) * 3 * n2 * nb,
+                            hipMemcpyDeviceToHost, c->st));
+      HIPCHK(hipMemcpyAsync(Om_kab + 2 * 9 * n2 * k0, c->dOm, sizeof(zd) * 9 * n2 * nb,
+                            hipMemcpyDeviceToHost, c->st));
+    }
+    CHK(check_info(c));
+  }
+  HIPCHK(hipStreamSynchronize(c->st));
+  resolve_times(c);
+  return MEP_OK;
+}
+
+extern "C" int mep_bandsum_batch(mep_ctx* c, int64_t nk, const double* e_k, const double* V_ka,
+                                 const double* A_ka, const double* Om_kab,
+                                 const mep_run_params* p, mep_sums* out) {
+  if (!c || nk < 0 || (nk > 0 && (!e_k || !V_ka)))
+    return fail(MEP_ERR_ARG, "mep_bandsum_batch: bad argument");
+  if (c->pos && (!A_ka || !Om_kab))
+    return fail(MEP_ERR_ARG, "mep_bandsum_batch: a position operator needs A_ka and Om_kab");
+  CHK(ensure_ws(c, nk > 0 ? nk : 1));
+  BandSetup bs;
+  CHK(band_setup(c, p, c->kcap, bs));
+  const int64_t n = c->n, n2 = n * n;
+  for (int64_t k0 = 0; k0 < nk; k0 += c->kcap) {
+    int64_t nb = (nk - k0 < c->kcap) ? nk - k0 : c->kcap;
+    HIPCHK(hipMemcpyAsync(c->dE, e_k + n * k0, sizeof(double) * n * nb, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->dV, V_ka + 2 * 3 * n2 * k0, sizeof(zd) * 3 * n2 * nb,
+                          hipMemcpyHostToDevice, c->st));
+    if (c->pos) {
+      HIPCHK(hipMemcpyAsync(c->dA, A_ka + 2 * 3 * n2 * k0, sizeof(zd) * 3 * n2 * nb,
+                            hipMemcpyHostToDevice, c->st));
+      HIPCHK(hipMemcpyAsync(c->dOm, Om_kab + 2 * 9 * n2 * k0, sizeof(zd) * 9 * n2 * nb,
+                            hipMemcpyHostToDevice, c->st));
+    }
+    CHK(band_block(c, bs, nb, nullptr));
+  }
+  return unpack(c, bs, p, nk, out);
+}
+
+static int run_common(mep_ctx* c, int64_t nk, const int32_t* mp, const double* kpts,
+                      const double* weights, const mep_run_params* p, mep_sums* out) {
+  CHK(ensure_ws(c, nk));
+  BandSetup bs;
+  CHK(band_setup(c, p, c->kcap, bs));
+  for (int64_t k0 = 0; k0 < nk; k0 += c->kcap) {
+    int64_t nb = (nk - k0 < c->kcap) ? nk - k0 : c->kcap;
+    if (mp) {
+      int m3[3] = {mp[0], mp[1], mp[2]};
+      timed(c, K_KGEN, [&] { launch_mp_kpts(c->st, k0, nb, m3, c->dk); });
+    } else {
+      HIPCHK(hipMemcpyAsync(c->dk, kpts + 3 * k0, sizeof(double) * 3 * nb, hipMemcpyHostToDevice,
+                            c->st));
+    }
+    if (weights)
+      HIPCHK(hipMemcpyAsync(c->dw, weights + k0, sizeof(double) * nb, hipMemcpyHostToDevice, c->st));
+    CHK(interp_block(c, nb, true));
+    CHK(band_block(c, bs, nb, weights ? c->dw : nullptr));
+  }
+  CHK(check_info(c));
+  return unpack(c, bs, p, nk, out);
+}
+
+extern "C" int mep_run_mesh(mep_ctx* c, const int32_t mp_grid[3], const mep_run_params* p,
+                            mep_sums* out) {
+  if (!c || !mp_grid) return fail(MEP_ERR_ARG, "mep_run_mesh: bad argument");
+  if (mp_grid[0] < 1 || mp_grid[1] < 1 || mp_grid[2] < 1)
+    return fail(MEP_ERR_ARG, "mep_run_mesh: mp_grid entries must be >= 1");
+  int64_t nk = (int64_t)mp_grid[0] * mp_grid[1] * mp_grid[2];
+  return run_common(c, nk, mp_grid, nullptr, nullptr, p, out);
+}
+
+extern "C" int mep_run_kpts(mep_ctx* c, int64_t nk, const double* kpts, const double* weights,
+                            const mep_run_params* p, mep_sums* out) {
+  if (!c || nk < 1 || !kpts) return fail(MEP_ERR_ARG, "mep_run_kpts: bad argument");
+  return run_common(c, nk, nullptr, kpts, weights, p, out);
+}

@@ -1,0 +1,48 @@
+// mep_common.hpp -- shared device helpers for the MEPinterp k-space kernels
+// (gfx950 / CDNA4, wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define MEP_WAVE 64
+
+// Complex FP64 value, interleaved (re, im): binary-compatible with Fortran
+// complex(c_double_complex) and C double _Complex.
+struct zd {
+  double re, im;
+};
+
+__host__ __device__ inline zd zmk(double r, double i) { return zd{r, i}; }
+__host__ __device__ inline zd zadd(zd a, zd b) { return zd{a.re + b.re, a.im + b.im}; }
+__host__ __device__ inline zd zsub(zd a, zd b) { return zd{a.re - b.re, a.im - b.im}; }
+__host__ __device__ inline zd zmul(zd a, zd b) {
+  return zd{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+// conj(a) * b
+__host__ __device__ inline zd zcmul(zd a, zd b) {
+  return zd{a.re * b.re + a.im * b.im, a.re * b.im - a.im * b.re};
+}
+__host__ __device__ inline zd zconj(zd a) { return zd{a.re, -a.im}; }
+__host__ __device__ inline zd zscale(zd a, double s) { return zd{a.re * s, a.im * s}; }
+__host__ __device__ inline zd zmuli(zd a) { return zd{-a.im, a.re}; }  // i*a
+__host__ __device__ inline double zabs2(zd a) { return a.re * a.re + a.im * a.im; }
+// a / b  (textbook formula; denominators here are well scaled)
+__host__ __device__ inline zd zdiv(zd a, zd b) {
+  double d = b.re * b.re + b.im * b.im;
+  return zd{(a.re * b.re + a.im * b.im) / d, (a.im * b.re - a.re * b.im) / d};
+}
+
+// Wave-wide FP64 sum (all 64 lanes receive the result).
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, MEP_WAVE);
+  return v;
+}
+
+// Index helpers in the reference's column-major layout (0-based).
+__host__ __device__ inline int64_t ix3(int a, int n, int m, int N) {
+  return a + 3 * ((int64_t)n + (int64_t)N * m);
+}
+__host__ __device__ inline int64_t ix33(int a, int b, int n, int m, int N) {
+  return a + 3 * (b + 3 * ((int64_t)n + (int64_t)N * m));
+}

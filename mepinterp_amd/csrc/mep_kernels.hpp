@@ -1,0 +1,50 @@
+// mep_kernels.hpp -- launchers of the k-space kernels (host side).
+#pragma once
+#include "mep_common.hpp"
+
+// Per-k geometry handed to the consumers of the Fourier GEMM output.
+struct KGeom {
+  const double* kpts;  // (3, nk) fractional k of the block
+  const double* tau;   // (3, n) Wannier centres (tight-binding convention) or null
+  double cmat[9];      // c(dR) = cmat * dR, row-major 3x3 (2*pi*I, or a_latt)
+};
+
+void launch_mp_kpts(hipStream_t s, int64_t k0, int64_t nk, const int mp[3], double* kpts);
+void launch_phase(hipStream_t s, int64_t nk, int nR, const double* kpts, const double* R, zd* P);
+void launch_ft_gemm_f64(hipStream_t s, int M, int N, int K, const zd* P, const zd* X, zd* C);
+
+// Batched Hermitian eigensolve of H(k) (upper triangle), n <= 64.
+// E (nk, n) ascending; U (nk, n, n) column-major eigenvectors.
+void launch_eigh(hipStream_t s, int n, int64_t nk, const zd* Hk, int64_t strideHk,
+                 const KGeom& g, double* E, zd* U, int* info);
+
+// Velocity (and, with a position operator, connection and curvature) in
+// the Hamiltonian gauge, written in the reference layout V(3,n,n) per k.
+struct VeloArgs {
+  int n, nmat;
+  int gauge_trafo;
+  int pos_op;
+  int textbook;
+  double kubo_tol;
+};
+void launch_velo(hipStream_t s, const VeloArgs& va, int64_t nk, const zd* Hk, int64_t strideHk,
+                 const KGeom& g, const zd* U, const double* E, zd* V, zd* A, zd* Om);
+
+// Band sums over a batch of k with per-block partial accumulators.
+struct BandArgs {
+  int n;
+  int jobs;  // MEP_JOB_* bits
+  int n_ef, n_hw, n_val;
+  double T_kelvin, eta, kubo_tol;
+  const double* ef;  // (n_ef) device
+  const double* hw;  // (n_hw) device
+  int pos_op;
+};
+// layout of one accumulator record (doubles)
+struct AccLayout {
+  int64_t Ne, ahc, velo, ohc, kic, klc, kcs, nic, nlc, ncs, total;
+};
+AccLayout acc_layout(const BandArgs& b);
+void launch_bandsum(hipStream_t s, const BandArgs& b, int64_t nk, const double* E, const zd* V,
+                    const zd* A, const zd* Om, const double* w, int nblocks, double* part);
+void launch_reduce_parts(hipStream_t s, int nblocks, int64_t len, const double* part, double* acc);

@@ -1,0 +1,317 @@
+// mep_velo.hip -- velocity (and connection / curvature) in the Hamiltonian
+// gauge, one k-point per wavefront.
+//
+// Replaces W_to_H_gaugeTRAFO + get_velo (wann_interp.f90:100-101, 406-658).
+// Inputs per k: the Fourier GEMM output in the Wannier gauge, stacked as
+//   mat 0: H, 1-3: dH_a, [4-6: A_a, 7-9: Omega_yz, Omega_zx, Omega_xy]
+// plus U (eigenvectors) and E.  The tight-binding phase of the reference's
+// convention (wann_interp.f90:192-241) is applied on load:
+//   X'_nm = e^{2 pi i k.(tau_m - tau_n)} X_nm,
+//   dH'_a = ph (dH_a + i c_a(tau_m - tau_n) H),
+//   Om'_ab = ph (Om_ab + i c_a(dtau) A_b - i c_b(dtau) A_a).
+// Output: V (3,n,n) per k in the reference layout; with a position operator
+// also A^(H) (3,n,n) and Omega^(H) (3,3,n,n).
+#include "mep_common.hpp"
+#include "mep_kernels.hpp"
+
+struct TBPhase {
+  double kx, ky, kz;
+  const double* kpts;
+  const double* tau;
+  double cm[9];
+  __device__ inline TBPhase at(int64_t k) const {
+    TBPhase r = *this;
+    r.kx = kpts[3 * k];
+    r.ky = kpts[3 * k + 1];
+    r.kz = kpts[3 * k + 2];
+    return r;
+  }
+  __device__ inline zd phase(int n, int m, double dtau[3]) const {
+    dtau[0] = tau[3 * m] - tau[3 * n];
+    dtau[1] = tau[3 * m + 1] - tau[3 * n + 1];
+    dtau[2] = tau[3 * m + 2] - tau[3 * n + 2];
+    const double two_pi = 2.0 * 3.14159265358979323846;
+    double ang = two_pi * (kx * dtau[0] + ky * dtau[1] + kz * dtau[2]);
+    double sn, cs;
+    sincos(ang, &sn, &cs);
+    return zd{cs, sn};
+  }
+  // c_a(dt) = sum_b cm[a][b] dt_b, with a selected without indexing cm
+  __device__ inline double c(int a, const double dt[3]) const {
+    double c0 = cm[0] * dt[0] + cm[1] * dt[1] + cm[2] * dt[2];
+    double c1 = cm[3] * dt[0] + cm[4] * dt[1] + cm[5] * dt[2];
+    double c2 = cm[6] * dt[0] + cm[7] * dt[1] + cm[8] * dt[2];
+    return a == 0 ? c0 : (a == 1 ? c1 : c2);
+  }
+};
+
+// element (n, m) of mat `mat` after the TB epilogue (mats 0..9 as above)
+__device__ inline zd load_tb(const zd* X, int n, int N, int row, int col, int mat, const TBPhase& tb) {
+  const int64_t N2 = (int64_t)N * N;
+  const int64_t o = row + (int64_t)N * col;
+  zd x = X[mat * N2 + o];
+  if (!tb.tau) return x;
+  double dt[3];
+  zd ph = tb.phase(row, col, dt);
+  if (mat >= 1 && mat <= 3) {
+    zd h = X[o];
+    x = zadd(x, zmuli(zscale(h, tb.c(mat - 1, dt))));
+  } else if (mat >= 7) {
+    // Omega_ab with (a,b) = (1,2), (2,0), (0,1)
+    int a = (mat == 7) ? 1 : (mat == 8) ? 2 : 0;
+    int b = (mat == 7) ? 2 : (mat == 8) ? 0 : 1;
+    zd Ab = X[(4 + b) * N2 + o], Aa = X[(4 + a) * N2 + o];
+    x = zadd(x, zmuli(zsub(zscale(Ab, tb.c(a, dt)), zscale(Aa, tb.c(b, dt)))));
+  }
+  return zmul(ph, x);
+}
+
+// --------------------------------------------------------------------------
+// no position operator: V_a = U^H dH_a U  (or dH_a without gauge transform)
+// --------------------------------------------------------------------------
+template <int NMAX>
+__global__ __launch_bounds__(64) void velo_kernel(int n, int64_t nk, const zd* __restrict__ Hk,
+                                                  int64_t strideHk, TBPhase tb0, int gauge_trafo,
+                                                  const zd* __restrict__ U, zd* __restrict__ V) {
+  constexpr int LD = NMAX + 1;
+  __shared__ zd Us[NMAX * LD];
+  __shared__ zd Ds[NMAX * LD];
+  const int t = threadIdx.x;
+  for (int64_t k = blockIdx.x; k < nk; k += gridDim.x) {
+    const TBPhase tb = tb0.at(k);
+    const zd* X = Hk + k * strideHk;
+    zd* Vk = V + k * 3 * (int64_t)n * n;
+    if (gauge_trafo) {
+      for (int idx = t; idx < n * n; idx += 64) {
+        int i = idx % n, j = idx / n;
+        Us[i + LD * j] = U[k * (int64_t)n * n + idx];
+      }
+    }
+    for (int a = 0; a < 3; ++a) {
+      __syncthreads();
+      for (int idx = t; idx < n * n; idx += 64) {
+        int i = idx % n, j = idx / n;
+        Ds[i + LD * j] = load_tb(X, 0, n, i, j, 1 + a, tb);
+      }
+      __syncthreads();
+      if (gauge_trafo) {
+        zd T[NMAX], Hb[NMAX];
+#pragma unroll
+        for (int i = 0; i < NMAX; ++i) T[i] = zd{0.0, 0.0};
+        if (t < n) {
+          for (int l = 0; l < n; ++l) {
+            zd ul = Us[l + LD * t];
+#pragma unroll
+            for (int i = 0; i < NMAX; ++i)
+              if (i < n) T[i] = zadd(T[i], zmul(Ds[i + LD * l], ul));
+          }
+        }
+#pragma unroll
+        for (int i = 0; i < NMAX; ++i) {
+          zd s = zd{0.0, 0.0};
+          if (i < n && t < n) {
+#pragma unroll
+            for (int l = 0; l < NMAX; ++l)
+              if (l < n) s = zadd(s, zcmul(Us[l + LD * i], T[l]));
+          }
+          Hb[i] = s;
+        }
+        __syncthreads();
+        if (t < n) {
+#pragma unroll
+          for (int i = 0; i < NMAX; ++i)
+            if (i < n) Ds[i + LD * t] = Hb[i];
+        }
+        __syncthreads();
+      }
+      for (int idx = t; idx < n * n; idx += 64) {
+        int i = idx % n, j = idx / n;
+        Vk[a + 3 * (int64_t)idx] = Ds[i + LD * j];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// --------------------------------------------------------------------------
+// position operator present (n <= 16): PRB 74 195118 Eqs. 21-31 with the
+// reference's evaluation order (rotate_gauge :502-550, get_gauge_covar_deriv
+// :565-617, conn_gaugeTrafo :620-633, curv_gaugeTrafo :636-658, get_velo
+// :406-444).  textbook != 0 selects the printed equations instead.
+// --------------------------------------------------------------------------
+#define PM 16
+#define PLD (PM + 1)
+
+// C = op(A) * B over the n x n leading blocks of LDS matrices (ld PLD)
+__device__ inline void lds_mm(int n, const zd* A, bool adjA, const zd* B, zd* C) {
+  for (int idx = threadIdx.x; idx < n * n; idx += 64) {
+    int i = idx % n, j = idx / n;
+    zd s = zd{0.0, 0.0};
+    for (int l = 0; l < n; ++l) {
+      zd aa = adjA ? zconj(A[l + PLD * i]) : A[i + PLD * l];
+      s = zadd(s, zmul(aa, B[l + PLD * j]));
+    }
+    C[i + PLD * j] = s;
+  }
+}
+
+__global__ __launch_bounds__(64) void velo_pos_kernel(int n, int64_t nk, const zd* __restrict__ Hk,
+                                                      int64_t strideHk, TBPhase tb, int gauge_trafo,
+                                                      int textbook, double kubo_tol,
+                                                      const zd* __restrict__ U,
+                                                      const double* __restrict__ E,
+                                                      zd* __restrict__ V, zd* __restrict__ Aout,
+                                                      zd* __restrict__ Omout) {
+  __shared__ zd Us[PM * PLD], T1[PM * PLD], T2[PM * PLD];
+  __shared__ zd Hb[3][PM * PLD], Ab[3][PM * PLD], Dm[3][PM * PLD], Om[3][PM * PLD];
+  __shared__ zd Abar[3][PM * PLD];
+  __shared__ double Es[PM];
+  const int t = threadIdx.x;
+  const int N = n;
+  for (int64_t k = blockIdx.x; k < nk; k += gridDim.x) {
+    const TBPhase tk = tb.at(k);
+    const zd* X = Hk + k * strideHk;
+    const int64_t NN = (int64_t)n * n;
+    for (int idx = t; idx < n * n; idx += 64) {
+      int i = idx % n, j = idx / n;
+      Us[i + PLD * j] = U[k * NN + idx];
+      for (int a = 0; a < 3; ++a) {
+        Hb[a][i + PLD * j] = load_tb(X, 0, N, i, j, 1 + a, tk);
+        Ab[a][i + PLD * j] = load_tb(X, 0, N, i, j, 4 + a, tk);
+        Om[a][i + PLD * j] = load_tb(X, 0, N, i, j, 7 + a, tk);
+      }
+    }
+    if (t < n) Es[t] = E[k * n + t];
+    __syncthreads();
+    if (gauge_trafo) {
+      for (int a = 0; a < 3; ++a) {
+        // Hbar = U^H dH U ; tmp1 = dH U
+        lds_mm(n, Hb[a], false, Us, T1);
+        __syncthreads();
+        lds_mm(n, Us, true, T1, Hb[a]);
+        __syncthreads();
+        if (textbook) {
+          lds_mm(n, Ab[a], false, Us, T2);
+          __syncthreads();
+          lds_mm(n, Us, true, T2, Ab[a]);
+        } else {
+          // stale temporary: A = (U^H (dH U)) U
+          lds_mm(n, Us, true, T1, T2);
+          __syncthreads();
+          lds_mm(n, T2, false, Us, Ab[a]);
+        }
+        __syncthreads();
+        // Omega_bar = U^H Omega U
+        lds_mm(n, Us, true, Om[a], T2);
+        __syncthreads();
+        lds_mm(n, T2, false, Us, Om[a]);
+        __syncthreads();
+      }
+      // D_a,ij = Hbar_a,ij / (E_j - E_i) for i != j and |E_j - E_i| > kubo_tol
+      for (int idx = t; idx < n * n; idx += 64) {
+        int i = idx % n, j = idx / n;
+        double de = Es[j] - Es[i];
+        for (int a = 0; a < 3; ++a) {
+          zd dv = zd{0.0, 0.0};
+          if (i != j && fabs(de) > kubo_tol)
+            dv = zd{Hb[a][i + PLD * j].re / de, Hb[a][i + PLD * j].im / de};
+          Dm[a][i + PLD * j] = dv;
+          Abar[a][i + PLD * j] = Ab[a][i + PLD * j];
+        }
+      }
+      __syncthreads();
+      // A^(H) = A + i D
+      for (int idx = t; idx < n * n; idx += 64) {
+        int i = idx % n, j = idx / n;
+        for (int a = 0; a < 3; ++a)
+          Ab[a][i + PLD * j] = zadd(Ab[a][i + PLD * j], zmuli(Dm[a][i + PLD * j]));
+      }
+      __syncthreads();
+      // Omega^(H)_ab = Om_ab - [D_a, A_b] + [D_b, A_a] - i [D_a, D_b]
+      // for (a,b) in {(1,2), (2,0), (0,1)}; A is A^(H) (reference) or Abar
+      for (int c = 0; c < 3; ++c) {
+        int a = (c == 0) ? 1 : (c == 1) ? 2 : 0;
+        int b = (c == 0) ? 2 : (c == 1) ? 0 : 1;
+        zd(*Asrc)[PM * PLD] = textbook ? Abar : Ab;
+        for (int idx = t; idx < n * n; idx += 64) {
+          int i = idx % n, j = idx / n;
+          zd s = Om[c][i + PLD * j];
+          zd c1 = zd{0, 0}, c2 = zd{0, 0}, c3 = zd{0, 0};
+          for (int l = 0; l < n; ++l) {
+            // [D_a, A_b]_ij
+            c1 = zadd(c1, zsub(zmul(Dm[a][i + PLD * l], Asrc[b][l + PLD * j]),
+                               zmul(Asrc[b][i + PLD * l], Dm[a][l + PLD * j])));
+            // [D_b, A_a]_ij
+            c2 = zadd(c2, zsub(zmul(Dm[b][i + PLD * l], Asrc[a][l + PLD * j]),
+                               zmul(Asrc[a][i + PLD * l], Dm[b][l + PLD * j])));
+            // [D_a, D_b]_ij
+            c3 = zadd(c3, zsub(zmul(Dm[a][i + PLD * l], Dm[b][l + PLD * j]),
+                               zmul(Dm[b][i + PLD * l], Dm[a][l + PLD * j])));
+          }
+          s = zsub(s, c1);
+          s = zadd(s, c2);
+          s = zsub(s, zmuli(c3));
+          T1[i + PLD * j] = s;
+        }
+        __syncthreads();
+        for (int idx = t; idx < n * n; idx += 64) {
+          int i = idx % n, j = idx / n;
+          Om[c][i + PLD * j] = T1[i + PLD * j];
+        }
+        __syncthreads();
+      }
+    }
+    // velocity
+    for (int idx = t; idx < n * n; idx += 64) {
+      int i = idx % n, j = idx / n;
+      for (int a = 0; a < 3; ++a) {
+        zd v = Hb[a][i + PLD * j];
+        if (i != j) {
+          // V_nm = H_nm - i (E_m - E_n) A_nm
+          const zd* Asel = (gauge_trafo && textbook) ? Abar[a] : Ab[a];
+          zd ce = zd{0.0, Es[j] - Es[i]};
+          v = zsub(v, zmul(ce, Asel[i + PLD * j]));
+        }
+        V[k * 3 * NN + a + 3 * (int64_t)idx] = v;
+        Aout[k * 3 * NN + a + 3 * (int64_t)idx] = Ab[a][i + PLD * j];
+      }
+      // Omega (3,3,n,n): antisymmetric, zero diagonal in (a,b)
+      zd* O = Omout + k * 9 * NN + 9 * (int64_t)idx;
+      zd oyz = Om[0][i + PLD * j], ozx = Om[1][i + PLD * j], oxy = Om[2][i + PLD * j];
+      O[0] = zd{0, 0};
+      O[1 + 3 * 1] = zd{0, 0};
+      O[2 + 3 * 2] = zd{0, 0};
+      O[1 + 3 * 2] = oyz;
+      O[2 + 3 * 1] = zsub(zd{0, 0}, oyz);
+      O[2 + 3 * 0] = ozx;
+      O[0 + 3 * 2] = zsub(zd{0, 0}, ozx);
+      O[0 + 3 * 1] = oxy;
+      O[1 + 3 * 0] = zsub(zd{0, 0}, oxy);
+    }
+    __syncthreads();
+  }
+}
+
+void launch_velo(hipStream_t s, const VeloArgs& va, int64_t nk, const zd* Hk, int64_t strideHk,
+                 const KGeom& g, const zd* U, const double* E, zd* V, zd* A, zd* Om) {
+  unsigned grid = (unsigned)(nk < 65536 ? nk : 65536);
+  TBPhase tb;
+  tb.kx = tb.ky = tb.kz = 0.0;
+  tb.kpts = g.kpts;
+  tb.tau = g.tau;
+  for (int i = 0; i < 9; ++i) tb.cm[i] = g.cmat[i];
+  if (va.pos_op) {
+    hipLaunchKernelGGL(velo_pos_kernel, dim3(grid), dim3(64), 0, s, va.n, nk, Hk, strideHk, tb,
+                       va.gauge_trafo, va.textbook, va.kubo_tol, U, E, V, A, Om);
+    return;
+  }
+  if (va.n <= 8)
+    hipLaunchKernelGGL(velo_kernel<8>, dim3(grid), dim3(64), 0, s, va.n, nk, Hk, strideHk, tb,
+                       va.gauge_trafo, U, V);
+  else if (va.n <= 16)
+    hipLaunchKernelGGL(velo_kernel<16>, dim3(grid), dim3(64), 0, s, va.n, nk, Hk, strideHk, tb,
+                       va.gauge_trafo, U, V);
+  else
+    hipLaunchKernelGGL(velo_kernel<32>, dim3(grid), dim3(64), 0, s, va.n, nk, Hk, strideHk, tb,
+                       va.gauge_trafo, U, V);
+}

@@ -1,0 +1,124 @@
+"""Seeded synthetic tight-binding models for the k-space workloads.
+
+All generators return ``dict(R_frac (3,nR), H (n,n,nR) Hartree, r (3,n,n,nR)
+Bohr or None, wf_centers or None, a_latt (3,3) Bohr)`` with H(-R) = H(R)^+
+enforced exactly, so H(k) is Hermitian for every k.  Energies are generated
+in eV and divided by 27.211385 as the reference's reader does
+(file_io.f90:741); positions in Angstrom divided by 0.52917721092
+(file_io.f90:790).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+AU_TO_EV = 27.211385
+AU_TO_ANGSTROM = 0.52917721092
+
+
+def cube_R(L: int):
+    """R in {-L..L}^3, lexicographic with the first component slowest
+    (the order of the fixtures' _hr.dat: (-1,-1,-1), (-1,-1,0), ...)."""
+    g = np.arange(-L, L + 1)
+    R = np.array([[x, y, z] for x in g for y in g for z in g], dtype=np.float64).T
+    return np.asfortranarray(R)
+
+
+def _hermitian_pairs(R, blocks_pos, zero_block):
+    """Fill X(R) for all R from the values on R > 0 (lexicographic) and
+    R = 0, with X(-R) = X(R)^+ (last two axes)."""
+    nR = R.shape[1]
+    idx = {tuple(R[:, s].astype(int)): s for s in range(nR)}
+    out = np.zeros(zero_block.shape + (nR,), dtype=np.complex128)
+    for s in range(nR):
+        key = tuple(R[:, s].astype(int))
+        if key == (0, 0, 0):
+            out[..., s] = zero_block
+        elif key > (0, 0, 0):
+            out[..., s] = blocks_pos[s]
+            out[..., idx[tuple(-x for x in key)]] = np.conj(np.swapaxes(blocks_pos[s], -1, -2))
+    return out
+
+
+def make_c1(seed: int = 1, n: int = 8, a: float = 6.0, with_r: bool = True):
+    """n_wf=8, R in {-1,0,1}^3 (deg 1).  H(0) real diagonal U(-2,2) eV;
+    hoppings complex N(0,1) exp(-|R|) eV; r_a(R) complex N(0,0.1) A,
+    Hermitian-symmetrised the same way."""
+    rng = np.random.default_rng(seed)
+    R = cube_R(1)
+    nR = R.shape[1]
+    Hpos, rpos = {}, {}
+    for s in range(nR):
+        if tuple(R[:, s].astype(int)) > (0, 0, 0):
+            rn = np.linalg.norm(R[:, s])
+            Hpos[s] = (rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))) * np.exp(-rn)
+            rpos[s] = (rng.normal(scale=0.1, size=(3, n, n)) + 1j * rng.normal(scale=0.1, size=(3, n, n)))
+    H0 = np.diag(rng.uniform(-2.0, 2.0, size=n)).astype(np.complex128)
+    H = _hermitian_pairs(R, Hpos, H0) / AU_TO_EV
+    r = None
+    if with_r:
+        X = rng.normal(scale=0.1, size=(3, n, n)) + 1j * rng.normal(scale=0.1, size=(3, n, n))
+        r0 = 0.5 * (X + np.conj(np.swapaxes(X, -1, -2)))
+        r = _hermitian_pairs(R, rpos, r0) / AU_TO_ANGSTROM
+    return dict(R_frac=R, H=np.asfortranarray(H),
+                r=None if r is None else np.asfortranarray(r), wf_centers=None,
+                a_latt=a * np.eye(3), name="c1_n8_nr27")
+
+
+_PAULI = np.array([[[0, 1], [1, 0]], [[0, -1j], [1j, 0]], [[1, 0], [0, -1]]], dtype=np.complex128)
+
+
+def make_c2(seed: int = 2, n_sites: int = 8, n_orb: int = 2, L: int = 3, decay: float = 1.5,
+            m_ex: float = 1.0, a: float = 6.0):
+    """n_wf = n_sites x n_orb x 2 spins (32 by default) with a seeded
+    non-collinear exchange -|m| m_s.sigma (x) P_s on every site, 3q-like
+    (tetrahedral) directions; R in {-L..L}^3, hoppings N(0,1) exp(-|R|/decay)
+    eV.  Basis index = (site*n_orb + orb)*2 + spin."""
+    rng = np.random.default_rng(seed)
+    n = n_sites * n_orb * 2
+    R = cube_R(L)
+    nR = R.shape[1]
+    Hpos = {}
+    for s in range(nR):
+        if tuple(R[:, s].astype(int)) > (0, 0, 0):
+            rn = np.linalg.norm(R[:, s])
+            Hpos[s] = (rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))) * np.exp(-rn / decay)
+    X = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    H0 = 0.5 * (X + X.conj().T)
+    tet = np.array([[1, 1, 1], [1, -1, -1], [-1, 1, -1], [-1, -1, 1]], dtype=np.float64) / np.sqrt(3)
+    for site in range(n_sites):
+        m = tet[site % 4].copy()
+        if site >= 4:  # rotate the second tetrahedron by a seeded angle
+            th = rng.uniform(0, np.pi)
+            c, s_ = np.cos(th), np.sin(th)
+            m = np.array([c * m[0] - s_ * m[1], s_ * m[0] + c * m[1], m[2]])
+        ms = -m_ex * np.einsum("a,aij->ij", m, _PAULI)
+        for orb in range(n_orb):
+            i0 = (site * n_orb + orb) * 2
+            H0[i0:i0 + 2, i0:i0 + 2] += ms
+    H = _hermitian_pairs(R, Hpos, H0) / AU_TO_EV
+    return dict(R_frac=R, H=np.asfortranarray(H), r=None, wf_centers=None,
+                a_latt=a * np.eye(3), name=f"c2_n{n}_nr{nR}")
+
+
+def make_random(seed: int, n: int, L: int = 1, decay: float = 1.0, with_r: bool = False,
+                tb: bool = False, a: float = 6.0):
+    """Generic seeded Hermitian model for parity tests (any n, cube {-L..L}^3)."""
+    rng = np.random.default_rng(seed)
+    R = cube_R(L)
+    nR = R.shape[1]
+    Hpos, rpos = {}, {}
+    for s in range(nR):
+        if tuple(R[:, s].astype(int)) > (0, 0, 0):
+            rn = np.linalg.norm(R[:, s])
+            Hpos[s] = (rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))) * np.exp(-rn / decay)
+            rpos[s] = rng.normal(scale=0.1, size=(3, n, n)) + 1j * rng.normal(scale=0.1, size=(3, n, n))
+    X = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    H = _hermitian_pairs(R, Hpos, 0.5 * (X + X.conj().T)) / AU_TO_EV
+    r = None
+    if with_r:
+        Y = rng.normal(scale=0.1, size=(3, n, n)) + 1j * rng.normal(scale=0.1, size=(3, n, n))
+        r = _hermitian_pairs(R, rpos, 0.5 * (Y + np.conj(np.swapaxes(Y, -1, -2)))) / AU_TO_ANGSTROM
+    tau = rng.uniform(0, 1, size=(3, n)) if tb else None
+    return dict(R_frac=R, H=np.asfortranarray(H), r=None if r is None else np.asfortranarray(r),
+                wf_centers=None if tau is None else np.asfortranarray(tau),
+                a_latt=a * np.eye(3), name=f"rand_n{n}_L{L}")
