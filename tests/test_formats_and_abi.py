@@ -1,0 +1,165 @@
+"""Boundary formats (NPY, _hr.dat, input.cfg) and the C-ABI library's
+exports -- CPU only, no GPU call."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import FEMN, ROOT
+
+
+def test_npy_header_bytes_match_fixtures():
+    from mepinterp_amd.formats import npy_header
+    with open(os.path.join(FEMN, "delta0.900", "ahc_tens.npy"), "rb") as f:
+        raw = f.read()
+    h = npy_header("<f8", (3, 3))
+    assert raw[:len(h)] == h and len(h) == 82
+    with open(os.path.join(FEMN, "delta0.900", "ahcVELO_sol09.npy"), "rb") as f:
+        raw = f.read()
+    h = npy_header("<c16", (3, 3, 121))
+    assert raw[:len(h)] == h
+
+
+@pytest.mark.parametrize("shape,cplx", [((3, 3), False), ((3, 3, 121), True), ((3, 3, 7), False),
+                                        ((3, 3, 256, 4), True), ((5,), False)])
+def test_npy_writer_roundtrip(tmp_path, shape, cplx):
+    from mepinterp_amd.formats import write_npy
+    rng = np.random.default_rng(0)
+    a = rng.normal(size=shape) + (1j * rng.normal(size=shape) if cplx else 0)
+    p = tmp_path / "x.npy"
+    write_npy(str(p), a)
+    b = np.load(str(p))
+    np.testing.assert_array_equal(a, b)
+    raw = p.read_bytes()
+    hlen = int.from_bytes(raw[8:12], "little")
+    assert raw[6:8] == b"\x02\x00" and (12 + hlen) % 16 == 2  # (10 + hlen) % 16 == 0
+
+
+def test_fixture_written_back_identically(tmp_path):
+    """Re-writing the fixture's array gives the fixture's bytes."""
+    from mepinterp_amd.formats import write_npy
+    src = os.path.join(FEMN, "delta0.900", "ahcVELO_sol09.npy")
+    write_npy(str(tmp_path / "v.npy"), np.load(src))
+    assert (tmp_path / "v.npy").read_bytes() == open(src, "rb").read()
+
+
+def test_hr_roundtrip(tmp_path):
+    from mepinterp_amd import models
+    from mepinterp_amd.formats import read_hr_dat, write_hr_dat
+    m = models.make_random(seed=1, n=4, L=1)
+    p = tmp_path / "x_hr.dat"
+    write_hr_dat(str(p), m["R_frac"], m["H"])
+    R, H, deg = read_hr_dat(str(p))
+    np.testing.assert_array_equal(R, m["R_frac"])
+    big = np.abs(m["H"]) * 27.211385 >= 1e-7
+    np.testing.assert_allclose(H[big], m["H"][big], rtol=1e-15)
+    assert np.all(deg == 1)
+
+
+CFG = """# input file for TB model from New J Physics 12, 053032 (2010)
+# generated on 10:00AM on February 26, 2019
+
+[jobs]
+    plot_bands=False
+    debug_mode=False
+    R_vect_float=False
+    do_write_velo=False
+    do_mep=True
+    do_kubo=False
+    do_ahc=True
+    do_opt=False
+    do_gyro=False
+
+[unitCell]
+    a1= 5.90389713 0.0 3.56440913
+    a2= -2.95194856 5.11292489 3.56440913
+    a3= -2.95194856 -5.11292489 3.56440913
+    a0= 1
+
+[wannBase]
+    seed_name= FeMn
+    N_wf= 8
+    wf_centers_x=  0.0 0.5 0.5 0.0 0.0 0.5 0.5 0.0
+    wf_centers_y=  0.0 0.5 0.0 0.5 0.0 0.5 0.0 0.5
+    wf_centers_z=  0.0 0.0 0.5 0.5 0.0 0.0 0.5 0.5
+
+[wannInterp]
+    use_cart_velo= False
+    doGaugeTrafo= True
+    mp_grid= 200 200 200
+
+[Fermi]
+    N_eF= 1
+    eF_min= 0
+    eF_max= 0
+    Tkelvin= 0
+    eta_smearing= 0.1
+    kuboTol= 1e-05
+
+[MEP]
+    valence_bands= 2
+    do_write_mep_bands= True
+
+[Laser]
+    N_hw= 121
+    hw_min= 0
+    hw_max= 6
+    laser_phase= 1
+"""
+
+
+def test_input_cfg_reference_layout(tmp_path):
+    """The layout tb_input_writer.py:107-188 writes."""
+    from mepinterp_amd.formats import read_input_cfg
+    p = tmp_path / "input.cfg"
+    p.write_text(CFG)
+    c = read_input_cfg(str(p))
+    assert c["do_mep"] and c["do_ahc"] and not c["do_kubo"]
+    assert c["seed_name"] == "FeMn" and c["N_wf"] == 8
+    assert c["mp_grid"] == [200, 200, 200]
+    assert c["N_hw"] == 121 and abs(c["hw_max"] * 27.211385 - 6.0) < 1e-14
+    assert abs(c["eta"] * 27.211385 - 0.1) < 1e-15 and c["kubo_tol"] == 1e-5
+    np.testing.assert_array_equal(c["a_latt"][1], [-2.95194856, 5.11292489, 3.56440913])
+    np.testing.assert_array_equal(c["wf_centers"][:, 3], [0.0, 0.5, 0.5])
+
+
+def test_input_cfg_fortran_logicals(tmp_path):
+    from mepinterp_amd.formats import read_input_cfg
+    text = CFG.replace("do_mep=True", "do_mep=.true.").replace("do_ahc=True", "do_ahc = T") \
+        .replace("do_kubo=False", "do_kubo= .false.")
+    p = tmp_path / "input.cfg"
+    p.write_text(text)
+    c = read_input_cfg(str(p))
+    assert c["do_mep"] and c["do_ahc"] and not c["do_kubo"]
+
+
+def _declared_functions():
+    with open(os.path.join(ROOT, "include", "mepinterp.h")) as f:
+        txt = f.read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:int|const char \*)\s*\*?\s*(mep_\w+)\s*\(", txt, re.M)))
+
+
+def test_capi_library_exports_every_declared_symbol():
+    from mepinterp_amd import capi
+    declared = _declared_functions()
+    assert set(declared) == set(capi.EXPORTS)
+    lib = capi.load()  # dlopen only; no GPU call
+    for name in declared:
+        assert hasattr(lib, name), name
+    out = subprocess.run(["nm", "-D", "--defined-only", capi.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    syms = {line.split()[-1] for line in out.splitlines() if line.strip()}
+    for name in declared:
+        assert name in syms, name
+
+
+def test_capi_has_no_oracle_dependency():
+    """The product library must not link or name the oracle."""
+    from mepinterp_amd import capi
+    out = subprocess.run(["nm", "-D", capi.LIB_PATH], capture_output=True, text=True).stdout
+    assert "orc_" not in out
+    ldd = subprocess.run(["ldd", capi.LIB_PATH], capture_output=True, text=True).stdout
+    assert "mep_oracle" not in ldd

@@ -1,0 +1,286 @@
+"""The oracle against what the mathematics fixes: LAPACK's eigensolver,
+finite differences, closed forms, exact invariants and the Chern number of
+a textbook model.  Each check is chosen so that a plausible slip (wrong
+sign, transposed index, dropped term) fails it."""
+import numpy as np
+import pytest
+
+from conftest import AU_TO_EV, oracle_model, rand_kpts
+
+
+def _herm(n, seed):
+    rng = np.random.default_rng(seed)
+    X = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    return 0.5 * (X + X.conj().T)
+
+
+# ---------------------------------------------------------------- eigensolver
+
+@pytest.mark.parametrize("n", [1, 2, 3, 8, 17, 32, 40])
+def test_zheev_matches_lapack(n):
+    import oracle as O
+    A = _herm(n, n)
+    w, U = O.zheev(A)
+    w_ref = np.linalg.eigh(A)[0]  # LAPACK zheevd
+    assert np.max(np.abs(w - w_ref)) < 1e-13 * max(1.0, np.abs(w_ref).max())
+    assert np.all(np.diff(w) >= 0)
+    np.testing.assert_allclose(U.conj().T @ A @ U, np.diag(w), atol=1e-12 * np.abs(A).max())
+    np.testing.assert_allclose(U.conj().T @ U, np.eye(n), atol=1e-13)
+    # phase convention: largest component of each column is real positive
+    idx = np.argmax(np.abs(U), axis=0)
+    piv = U[idx, np.arange(n)]
+    assert np.all(np.abs(piv.imag) < 1e-14) and np.all(piv.real > 0)
+
+
+def test_zheev_reads_upper_triangle_only():
+    import oracle as O
+    A = _herm(12, 3)
+    B = A.copy()
+    B[np.tril_indices(12, -1)] = 123.0 + 7j  # garbage below the diagonal
+    B[np.diag_indices(12)] += 5j             # imaginary diagonal is ignored
+    w1, U1 = O.zheev(A)
+    w2, U2 = O.zheev(B)
+    np.testing.assert_array_equal(w1, w2)
+    np.testing.assert_array_equal(U1, U2)
+
+
+def test_zheev_degenerate_and_diagonal():
+    import oracle as O
+    rng = np.random.default_rng(0)
+    Q = np.linalg.qr(rng.normal(size=(6, 6)) + 1j * rng.normal(size=(6, 6)))[0]
+    ev = np.array([-1.0, -1.0, 0.5, 0.5, 0.5, 2.0])
+    A = Q @ np.diag(ev) @ Q.conj().T
+    w, U = O.zheev(A)
+    assert np.max(np.abs(w - ev)) < 1e-14 * 4
+    np.testing.assert_allclose(U.conj().T @ A @ U, np.diag(w), atol=1e-13)
+    w, U = O.zheev(np.diag([3.0, -2.0, 1.0]).astype(complex))
+    np.testing.assert_array_equal(w, [-2.0, 1.0, 3.0])
+
+
+# ---------------------------------------------------------------- Fourier sums
+
+@pytest.mark.parametrize("tb,cart", [(False, False), (True, False), (False, True), (True, True)])
+def test_ft_derivative_is_dH_dk(tb, cart):
+    """H_ka = dH/dk: central differences of H(k) along fractional k_b give
+    sum_a H_ka c_a-weights (internal: dH/dk_b = H_kb; cartesian velocity:
+    dH/dk_b = sum_a a_latt(a, b)... handled through the chain rule)."""
+    import oracle as O
+    from mepinterp_amd import models
+    m = models.make_random(seed=4, n=6, L=1, tb=tb)
+    m["a_latt"] = np.array([[5.0, 0.3, 0.0], [0.2, 4.5, 0.1], [0.0, 0.4, 6.0]])
+    om = oracle_model(m, use_cart_velo=cart)
+    k = np.array([0.13, -0.27, 0.31])
+    _, Hka, _, _ = O.ft_R_to_k(om, k)
+    h = 1e-6
+    dH = []
+    for b in range(3):
+        e = np.zeros(3)
+        e[b] = h
+        Hp = O.ft_R_to_k(om, k + e, grad=False)[0]
+        Hm = O.ft_R_to_k(om, k - e, grad=False)[0]
+        dH.append((Hp - Hm) / (2 * h))
+    dH = np.array(dH)
+    if not cart:
+        np.testing.assert_allclose(Hka, dH, atol=1e-8 * np.abs(dH).max())
+    else:
+        # H_ka = sum_b M_ab dH/dk_b with c = a_latt . dR, phase 2pi k.dR
+        # => dH/dk_b = 2pi * sum ... ; i.e. Hka_a = sum_b a_latt(a,b)/(2pi) dH/dk_b
+        M = m["a_latt"] / (2 * np.pi)
+        np.testing.assert_allclose(Hka, np.einsum("ab,bnm->anm", M, dH),
+                                   atol=1e-8 * np.abs(Hka).max())
+
+
+def test_ft_hermitian_and_periodic():
+    import oracle as O
+    from mepinterp_amd import models
+    m = models.make_random(seed=9, n=7, L=2)
+    om = oracle_model(m)
+    k = np.array([0.21, 0.05, -0.44])
+    Hk, Hka, _, _ = O.ft_R_to_k(om, k)
+    np.testing.assert_allclose(Hk, Hk.conj().T, atol=1e-15)
+    for a in range(3):
+        np.testing.assert_allclose(Hka[a], Hka[a].conj().T, atol=1e-14)
+    Hk2 = O.ft_R_to_k(om, k + np.array([1.0, -2.0, 3.0]), grad=False)[0]
+    np.testing.assert_allclose(Hk2, Hk, atol=1e-12)
+
+
+def test_ft_tight_binding_is_diagonal_gauge_of_wannier():
+    """H^TB(k) = D^+ H^W(k) D with D = diag(e^{2 pi i k.tau_n})."""
+    import oracle as O
+    from mepinterp_amd import models
+    m = models.make_random(seed=2, n=5, L=1, tb=True)
+    k = np.array([0.3, -0.1, 0.2])
+    Htb = O.ft_R_to_k(oracle_model(m), k, grad=False)[0]
+    mw = dict(m)
+    mw["wf_centers"] = None
+    Hw = O.ft_R_to_k(oracle_model(mw), k, grad=False)[0]
+    d = np.exp(2j * np.pi * (k @ m["wf_centers"]))
+    np.testing.assert_allclose(Htb, np.conj(d)[:, None] * Hw * d[None, :], atol=1e-14)
+
+
+# ---------------------------------------------------------------- velocities
+
+def test_hellmann_feynman():
+    """V_nn = dE_n/dk_a (no position operator, gauge transformation on)."""
+    import oracle as O
+    from mepinterp_amd import models
+    m = models.make_random(seed=12, n=9, L=1)
+    om = oracle_model(m)
+    k = np.array([0.11, 0.23, -0.37])
+    e, V, _, _ = O.get_wann_interp(om, k)
+    h = 1e-6
+    for a in range(3):
+        dk = np.zeros(3)
+        dk[a] = h
+        ep = O.get_wann_interp(om, k + dk, velo=False)[0]
+        em = O.get_wann_interp(om, k - dk, velo=False)[0]
+        np.testing.assert_allclose(np.diag(V[a]).real, (ep - em) / (2 * h),
+                                   atol=1e-7 * np.abs(np.diag(V[a])).max())
+        assert np.max(np.abs(np.diag(V[a]).imag)) < 1e-14
+
+
+def test_position_operator_reference_vs_textbook():
+    """With r(R) the reference's velocity V_nm = 2 Hbar_nm - i dE A~_nm
+    (stale temporary, ordering) differs from PRB 74 195118 Eq. 31, which is
+    Hermitian; without r both reduce to V = Hbar."""
+    import oracle as O
+    from mepinterp_amd import models
+    m = models.make_c1(seed=3)
+    k = np.array([0.1, 0.2, 0.3])
+    _, Vref, _, _ = O.get_wann_interp(oracle_model(m), k)
+    _, Vtb, _, _ = O.get_wann_interp(oracle_model(m, textbook=True), k)
+    herm_tb = np.abs(Vtb - np.conj(np.transpose(Vtb, (0, 2, 1)))).max()
+    herm_ref = np.abs(Vref - np.conj(np.transpose(Vref, (0, 2, 1)))).max()
+    assert herm_tb < 1e-12 * np.abs(Vtb).max()
+    assert herm_ref > 1e-3 * np.abs(Vref).max()
+    m0 = dict(m)
+    m0["r"] = None
+    _, V0, _, _ = O.get_wann_interp(oracle_model(m0), k)
+    _, V0t, _, _ = O.get_wann_interp(oracle_model(m0, textbook=True), k)
+    np.testing.assert_array_equal(V0, V0t)
+
+
+# ---------------------------------------------------------------- band sums
+
+def _qwz(mass):
+    """Qi-Wu-Zhang 2-band Chern insulator in the kx-ky plane (no z hopping):
+    H = sin kx sx + sin ky sy + (m + cos kx + cos ky) sz, k = 2 pi k_frac."""
+    sx = np.array([[0, 1], [1, 0]], complex)
+    sy = np.array([[0, -1j], [1j, 0]])
+    sz = np.array([[1, 0], [0, -1]], complex)
+    Rs, Hs = [], []
+    Rs.append((0, 0, 0)); Hs.append(mass * sz)
+    for d in ((1, 0, 0), (0, 1, 0)):
+        s = sx if d[0] else sy
+        Rs.append(d); Hs.append(s / (2j) + sz / 2)
+        Rs.append(tuple(-x for x in d)); Hs.append(-s / (2j) + sz / 2)
+    R = np.asfortranarray(np.array(Rs, float).T)
+    H = np.asfortranarray(np.stack(Hs, axis=2))
+    return dict(R_frac=R, H=H, r=None, wf_centers=None, a_latt=np.eye(3))
+
+
+@pytest.mark.parametrize("mass,chern", [(1.0, -1), (-1.0, 1), (3.0, 0)])
+def test_chern_number_of_qwz(mass, chern):
+    """-sigma_xy * V_cell / (2 pi) is the Chern number of the filled band."""
+    import oracle as O
+    m = _qwz(mass)
+    p = O.RunParams(do_ahc=True, n_eF=1, T_kelvin=0.0, valence_bands=1)
+    mp = (48, 48, 1)
+    out = O.finalize(O.accumulate(oracle_model(m), p, O.mp_mesh(mp)), p, mp, m["a_latt"])
+    c = -out["ahc_tens"][0, 1] * O.unit_vol(m["a_latt"]) / (2 * np.pi)
+    assert abs(c - chern) < 1e-6
+
+
+def test_kubo_mep_equals_niu_for_insulator():
+    """T -> 0 with E_F in a gap: the Fermi-weighted Kubo MEP sums equal the
+    Niu sums over the occupied bands (same formulas, different guards)."""
+    import oracle as O
+    from mepinterp_amd import models
+    m = models.make_random(seed=31, n=6, L=1)
+    k = rand_kpts(20, seed=4)
+    om = oracle_model(m, kubo_tol=1e-300)
+    for ik in range(k.shape[1]):
+        e, V, _, _ = O.get_wann_interp(om, k[:, ik])
+        nocc = 3
+        ef = np.array([0.5 * (e[nocc - 1] + e[nocc])])
+        fd = O.fd_distrib(e, ef, 0.0)
+        np.testing.assert_allclose(O.kubo_mep_ic(e, V, fd)[:, :, 0],
+                                   O.mep_niu_ic(V, e, nocc, 1e-300).sum(axis=2), rtol=1e-12,
+                                   atol=1e-12 * np.abs(O.kubo_mep_ic(e, V, fd)).max())
+        np.testing.assert_allclose(O.kubo_mep_lc(e, V, fd)[:, :, 0],
+                                   O.mep_niu_lc(V, e, nocc, 1e-300).sum(axis=2), rtol=1e-12,
+                                   atol=1e-12 * np.abs(O.kubo_mep_lc(e, V, fd)).max())
+
+
+def test_ic_sum_as_matrix_products():
+    """kubo_mep_IC through V-matrix products (a different evaluation route
+    than the reference's 27-term loop): F2_ij = -sum_kl eps_jkl Re T_ikl,
+    T_ikl = sum_{n0} f_n0 sum_{n,m != n0} a_i(n) V_k(n,m) b_l(m)."""
+    import oracle as O
+    from mepinterp_amd import models
+    m = models.make_random(seed=41, n=7, L=1)
+    e, V, _, _ = O.get_wann_interp(oracle_model(m), np.array([0.2, 0.1, -0.3]))
+    fd = O.fd_distrib(e, [0.0], 300.0)
+    eps = np.zeros((3, 3, 3))
+    for (i, j, k), s in {(0, 1, 2): 1, (1, 2, 0): 1, (2, 0, 1): 1, (0, 2, 1): -1,
+                         (2, 1, 0): -1, (1, 0, 2): -1}.items():
+        eps[i, j, k] = s
+    F2 = np.zeros((3, 3))
+    n = e.size
+    for n0 in range(n):
+        mask = np.arange(n) != n0
+        d = e[n0] - e
+        a = np.where(mask, V[:, n0, :] / np.where(mask, d, 1) ** 2, 0)       # a_i(n)
+        b = np.where(mask, V[:, :, n0] / np.where(mask, d, 1), 0)            # b_l(m)
+        T = np.einsum("in,knm,lm->ikl", a, V, b)
+        F2 += -np.einsum("jkl,ikl->ij", eps, T.real) * fd[0, n0]
+    np.testing.assert_allclose(O.kubo_mep_ic(e, V, fd)[:, :, 0], F2, rtol=1e-11,
+                               atol=1e-12 * np.abs(F2).max())
+
+
+def test_ahc_antisymmetric_and_hw0_limit():
+    """sigma_ij = -sigma_ji, and velo_ahc_tens at hw = 0, eta -> 0 equals
+    kubo_ahc_tens (closed form: f_n - f_m over (E_n - E_m)^2)."""
+    import oracle as O
+    from mepinterp_amd import models
+    m = models.make_random(seed=5, n=6, L=1)
+    e, V, _, _ = O.get_wann_interp(oracle_model(m), np.array([0.3, 0.1, 0.2]))
+    fd = O.fd_distrib(e, [0.0], 0.0)
+    ahc = O.kubo_ahc_tens(e, V, fd)[:, :, 0]
+    np.testing.assert_allclose(ahc, -ahc.T, atol=1e-12 * np.abs(ahc).max())
+    velo = O.velo_ahc_tens(e, V, [0.0], fd, 1e-12)[:, :, 0, 0]
+    np.testing.assert_allclose(velo.real, ahc, rtol=1e-8, atol=1e-10 * np.abs(ahc).max())
+
+
+def test_fermi_dirac_closed_form():
+    import oracle as O
+    kb = 8.6173303 * 1e-5 / AU_TO_EV  # constants.f90:22
+    ef = np.array([-0.01, 0.0, 0.02])
+    f = O.fd_stat(0.005, ef, 300.0)
+    np.testing.assert_allclose(f, 1.0 / (1.0 + np.exp((0.005 - ef) / (kb * 300.0))), rtol=1e-15)
+    np.testing.assert_array_equal(O.fd_stat(0.0, ef, 0.0), [0.0, 0.0, 1.0])  # strict E < E_F
+    np.testing.assert_array_equal(O.fd_stat(0.0, ef, 0.005), [0.0, 0.0, 1.0])  # T <= 0.01 K
+
+
+# ---------------------------------------------------------------- k mesh
+
+def test_mp_mesh_closed_form_and_order():
+    import oracle as O
+    mp = (4, 3, 2)
+    k = O.mp_mesh(mp)
+    pos = 0
+    for z in range(1, 3):
+        for y in range(1, 4):
+            for x in range(1, 5):
+                q = np.array([x, y, z], float)
+                N = np.array(mp, float)
+                np.testing.assert_array_equal(k[:, pos], (2 * q - N - 1) / (2 * N))
+                pos += 1
+    # cubic meshes: get_kpt_idx equals the visiting position
+    assert [O.kpt_idx((3, 3, 3), x, y, z) for z in (1, 2, 3) for y in (1, 2, 3)
+            for x in (1, 2, 3)] == list(range(1, 28))
+    # non-cubic: the reference's index collides (k_space.f90:345)
+    idx = [O.kpt_idx(mp, x, y, z) for z in (1, 2) for y in (1, 2, 3) for x in (1, 2, 3, 4)]
+    assert len(set(idx)) < len(idx)
+    # the mesh is symmetric (no Gamma for even N)
+    assert np.all(np.abs(k).max(axis=1) < 0.5)
