@@ -103,10 +103,12 @@ int mep_interp_batch(mep_ctx *ctx, int64_t nk, const double *kpts, double *e_k, 
    (statistics.f90:65-124), kubo_ahc_tens / velo_ahc_tens / kubo_ohc_tens
    (kubo.f90:37-198), kubo_mep_IC/LC/CS (kubo_mep.f90:16-149),
    mep_niu_IC/LC/CS (mep_niu.f90:20-159).  With nk = 1 the outputs are the
-   reference functions' results.                                             */
+   reference functions' results.  fd_distrib(n_eF,n,nk) supplies the
+   occupations as the reference's kubo_* arguments do; NULL = fd_stat of
+   e_k at the p->n_eF Fermi levels and p->T_kelvin.                          */
 int mep_bandsum_batch(mep_ctx *ctx, int64_t nk, const double *e_k, const double *V_ka,
-                      const double *A_ka, const double *Om_kab, const mep_run_params *p,
-                      mep_sums *out);
+                      const double *A_ka, const double *Om_kab, const double *fd_distrib,
+                      const mep_run_params *p, mep_sums *out);
 
 /* The whole k-loop of core_worker (core.f90:165-270) over the
    Monkhorst-Pack mesh mp_grid (k_space.f90:341-375), on the device; the

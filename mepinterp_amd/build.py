@@ -45,6 +45,10 @@ def build_lib(verbose: bool = False) -> str:
     subprocess.run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp],
                    check=True)
     os.replace(tmp, LIB)
+    # calibration copy kernel for the HBM-traffic counters (profiling only)
+    subprocess.run([HIPCC, "--offload-arch=gfx950", "-O2", os.path.join(CSRC, "copy_calib.hip"),
+                    "-o", os.path.join(bdir, "copy_calib")], check=True,
+                   stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
     return LIB
 
 

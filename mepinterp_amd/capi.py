@@ -69,7 +69,7 @@ def load(path: str = LIB_PATH):
                                    C.POINTER(C.c_void_p)]
     lib.mep_ctx_destroy.argtypes = [C.c_void_p]
     lib.mep_interp_batch.argtypes = [C.c_void_p, C.c_int64, c_dp, c_dp, c_dp, c_dp, c_dp]
-    lib.mep_bandsum_batch.argtypes = [C.c_void_p, C.c_int64, c_dp, c_dp, c_dp, c_dp,
+    lib.mep_bandsum_batch.argtypes = [C.c_void_p, C.c_int64, c_dp, c_dp, c_dp, c_dp, c_dp,
                                       C.POINTER(RunParams), C.POINTER(Sums)]
     lib.mep_run_mesh.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(RunParams),
                                  C.POINTER(Sums)]
@@ -181,12 +181,13 @@ class Engine:
             setattr(s, k, _p(out[k]))
         return out, s
 
-    def bandsum(self, e, V, p: RunParams, A=None, Om=None):
+    def bandsum(self, e, V, p: RunParams, A=None, Om=None, fd=None):
         e = np.asfortranarray(e, dtype=np.float64).reshape(self.n, -1, order="F")
         V = np.asfortranarray(V, dtype=np.complex128)
         out, s = self._sums_buffers(p)
         rc = load().mep_bandsum_batch(self.ctx, e.shape[1], _p(e), _p(V), _p(_f(A, np.complex128)),
-                                      _p(_f(Om, np.complex128)), C.byref(p), C.byref(s))
+                                      _p(_f(Om, np.complex128)), _p(_f(fd, np.float64)),
+                                      C.byref(p), C.byref(s))
         _check(rc, "mep_bandsum_batch")
         out["n_k"] = s.n_k
         return out

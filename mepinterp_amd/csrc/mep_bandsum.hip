@@ -72,9 +72,13 @@ __global__ __launch_bounds__(64) void bandsum_kernel(BandArgs b, double Tsmear, 
     if (t < n) Es[t] = E[k * n + t];
     __syncthreads();
     for (int idx = t; idx < nef * n; idx += 64) {
-      int ef = idx % nef, nb = idx / nef;
-      double e = Es[nb], f = b.ef[ef];
-      fd[idx] = finiteT ? 1.0 / (1.0 + exp((e - f) / Tsmear)) : ((e < f) ? 1.0 : 0.0);
+      if (b.fd_in) {
+        fd[idx] = b.fd_in[k * (int64_t)nef * n + idx];
+      } else {
+        int ef = idx % nef, nb = idx / nef;
+        double e = Es[nb], f = b.ef[ef];
+        fd[idx] = finiteT ? 1.0 / (1.0 + exp((e - f) / Tsmear)) : ((e < f) ? 1.0 : 0.0);
+      }
     }
     __syncthreads();
     // ---- electron count --------------------------------------------------

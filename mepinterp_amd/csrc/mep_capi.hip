@@ -72,6 +72,8 @@ struct mep_ctx {
   double* def = nullptr;
   double* dhw = nullptr;
   int64_t ef_cap = 0, hw_cap = 0;
+  double* dfd = nullptr;
+  int64_t fd_cap = 0;
   hipStream_t st = nullptr;
   // profiling
   bool prof = false;
@@ -260,7 +262,7 @@ extern "C" int mep_ctx_destroy(mep_ctx* c) {
   resolve_times(c);
   free_ws(c);
   hipFree(c->dR); hipFree(c->dtau); hipFree(c->dX); hipFree(c->dinfo);
-  hipFree(c->dpart); hipFree(c->dacc); hipFree(c->def); hipFree(c->dhw);
+  hipFree(c->dpart); hipFree(c->dacc); hipFree(c->def); hipFree(c->dhw); hipFree(c->dfd);
   for (auto e : c->pool) hipEventDestroy(e);
   if (c->st) hipStreamDestroy(c->st);
   delete c;
@@ -401,6 +403,7 @@ static int band_setup(mep_ctx* c, const mep_run_params* p, int64_t kblock, BandS
   HIPCHK(hipMemcpy(c->dhw, hw.data(), sizeof(double) * hw.size(), hipMemcpyHostToDevice));
   b.ef = c->def;
   b.hw = c->dhw;
+  b.fd_in = nullptr;
   if (bandsum_lds_bytes(b) > 160 * 1024)
     return fail(MEP_ERR_ARG, "band-sum accumulators exceed the 160 KiB LDS of one workgroup "
                              "(reduce N_hw * N_eF)");
@@ -497,7 +500,8 @@ extern "C" int mep_interp_batch(mep_ctx* c, int64_t nk, const double* kpts, doub
 
 extern "C" int mep_bandsum_batch(mep_ctx* c, int64_t nk, const double* e_k, const double* V_ka,
                                  const double* A_ka, const double* Om_kab,
-                                 const mep_run_params* p, mep_sums* out) {
+                                 const double* fd_distrib, const mep_run_params* p,
+                                 mep_sums* out) {
   if (!c || nk < 0 || (nk > 0 && (!e_k || !V_ka)))
     return fail(MEP_ERR_ARG, "mep_bandsum_batch: bad argument");
   if (c->pos && (!A_ka || !Om_kab))
@@ -505,9 +509,19 @@ extern "C" int mep_bandsum_batch(mep_ctx* c, int64_t nk, const double* e_k, cons
   CHK(ensure_ws(c, nk > 0 ? nk : 1));
   BandSetup bs;
   CHK(band_setup(c, p, c->kcap, bs));
-  const int64_t n = c->n, n2 = n * n;
+  const int64_t n = c->n, n2 = n * n, nef = p->n_eF;
+  if (fd_distrib && nef * n * c->kcap > c->fd_cap) {
+    hipFree(c->dfd);
+    c->dfd = nullptr;
+    HIPCHK(hipMalloc(&c->dfd, sizeof(double) * nef * n * c->kcap));
+    c->fd_cap = nef * n * c->kcap;
+  }
+  if (fd_distrib) bs.b.fd_in = c->dfd;
   for (int64_t k0 = 0; k0 < nk; k0 += c->kcap) {
     int64_t nb = (nk - k0 < c->kcap) ? nk - k0 : c->kcap;
+    if (fd_distrib)
+      HIPCHK(hipMemcpyAsync(c->dfd, fd_distrib + nef * n * k0, sizeof(double) * nef * n * nb,
+                            hipMemcpyHostToDevice, c->st));
     HIPCHK(hipMemcpyAsync(c->dE, e_k + n * k0, sizeof(double) * n * nb, hipMemcpyHostToDevice, c->st));
     HIPCHK(hipMemcpyAsync(c->dV, V_ka + 2 * 3 * n2 * k0, sizeof(zd) * 3 * n2 * nb,
                           hipMemcpyHostToDevice, c->st));

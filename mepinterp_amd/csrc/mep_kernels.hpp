@@ -38,6 +38,7 @@ struct BandArgs {
   double T_kelvin, eta, kubo_tol;
   const double* ef;  // (n_ef) device
   const double* hw;  // (n_hw) device
+  const double* fd_in;  // (n_ef, n, nk) device occupations, or null (fd_stat)
   int pos_op;
 };
 // layout of one accumulator record (doubles)

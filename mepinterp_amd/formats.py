@@ -87,6 +87,47 @@ def write_hr_dat(path: str, R_frac, H_hartree, deg=None, comment="written by mep
                             f"  {z.real: .17e}  {z.imag: .17e}\n")
 
 
+def write_r_dat(path: str, R_frac, r_bohr, comment="written by mepinterp_amd"):
+    """Position operator file read by the Fortran host's read_r_file: a
+    comment line, n_wf, then rows ``R1 R2 R3 m n Re(x) Im(x) Re(y) Im(y)
+    Re(z) Im(z)`` in Angstrom, R in the order of the _hr.dat file."""
+    r = np.asarray(r_bohr) * AU_TO_ANGSTROM
+    n = r.shape[1]
+    with open(path, "w") as f:
+        f.write(comment.strip() + "\n")
+        f.write(f"{n:11d}\n")
+        for s in range(r.shape[3]):
+            R = np.rint(np.asarray(R_frac)[:, s]).astype(np.int64)
+            for m in range(n):
+                for nn in range(n):
+                    v = r[:, m, nn, s]
+                    f.write(f"{R[0]:5d}{R[1]:5d}{R[2]:5d}{m + 1:5d}{nn + 1:5d}" +
+                            "".join(f"  {c.real: .17e}  {c.imag: .17e}" for c in v) + "\n")
+
+
+def read_r_dat(path: str, R_frac, R_float: bool = False):
+    """Read the file write_r_dat writes; returns r (3,n,n,nR) in Bohr."""
+    with open(path) as f:
+        f.readline()
+        n = int(f.readline().split()[0])
+        body = np.loadtxt(f, dtype=np.float64, ndmin=2)
+    nR = np.asarray(R_frac).shape[1]
+    if body.shape[0] != nR * n * n:
+        raise ValueError("[read_r_dat]: wrong number of rows")
+    r = np.zeros((3, n, n, nR), dtype=np.complex128, order="F")
+    rows = body.reshape(nR, n * n, body.shape[1])
+    for s in range(nR):
+        blk = rows[s]
+        R = blk[0, 0:3] if R_float else np.trunc(blk[0, 0:3])
+        if np.max(np.abs(R - np.asarray(R_frac)[:, s])) > 1e-14:
+            raise ValueError("[read_r_dat]: different R order in _hr.dat and _r.dat")
+        m = blk[:, 3].astype(np.int64) - 1
+        nn = blk[:, 4].astype(np.int64) - 1
+        for a in range(3):
+            r[a, m, nn, s] = blk[:, 5 + 2 * a] + 1j * blk[:, 6 + 2 * a]
+    return r / AU_TO_ANGSTROM
+
+
 def read_latt(path: str):
     """Read a ``.latt`` file (comment line + 3 rows a_i in Bohr)."""
     rows = []
@@ -111,7 +152,7 @@ def parse_cfg_text(text: str) -> dict:
     out = {}
     section = ""
     for raw in text.splitlines():
-        line = raw.split("#", 1)[0].split("!", 1)[0].strip()
+        line = raw.split("#", 1)[0].split(";", 1)[0].strip()
         if not line:
             continue
         m = re.match(r"^\[(.+)\]$", line)

@@ -20,7 +20,7 @@ import os
 import numpy as np
 
 from . import capi
-from .formats import AU_TO_EV, read_hr_dat, read_input_cfg, write_npy
+from .formats import AU_TO_EV, read_hr_dat, read_input_cfg, read_r_dat, write_npy
 
 
 def unit_vol(a_latt):
@@ -127,10 +127,11 @@ def core_worker(run_dir: str = "."):
     """core.f90:79-298 on the GPU: read ./input.cfg and w90files/<seed>_hr.dat,
     integrate over the MP mesh, write out/*.npy and out/occ.dat."""
     cfg = read_input_cfg(os.path.join(run_dir, "input.cfg"))
-    R, H, deg = read_hr_dat(os.path.join(run_dir, "w90files", cfg["seed_name"] + "_hr.dat"),
-                            R_float=cfg["R_vect_float"])
+    base = os.path.join(run_dir, "w90files", cfg["seed_name"])
+    R, H, deg = read_hr_dat(base + "_hr.dat", R_float=cfg["R_vect_float"])
+    r = read_r_dat(base + "_r.dat", R, cfg["R_vect_float"]) if os.path.exists(base + "_r.dat") else None
     p = params_from_cfg(cfg)
-    with capi.Engine(R, H, wf_centers=cfg["wf_centers"], a_latt=cfg["a_latt"],
+    with capi.Engine(R, H, r=r, wf_centers=cfg["wf_centers"], a_latt=cfg["a_latt"],
                      kubo_tol=cfg["kubo_tol"], gauge_trafo=cfg["do_gauge_trafo"],
                      cart_velo=cfg["use_cart_velo"]) as eng:
         sums = eng.run_mesh(cfg["mp_grid"], run_params(p))

@@ -1,0 +1,126 @@
+"""The Fortran drop-in (mepinterp_amd/fortran) on the GPU against the oracle:
+the mepInterp executable on an input.cfg run directory, and the per-k
+interface (get_wann_interp, kubo_*, kubo_mep_*, mep_niu_*) called with the
+reference's argument lists from dropin_check."""
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import AU_TO_EV, FEMN, FEMN_CENTERS, ROOT, oracle_model, rand_kpts
+
+pytestmark = pytest.mark.gpu
+
+FDIR = os.path.join(ROOT, "mepinterp_amd", "fortran", "_build")
+
+
+def _need(exe):
+    path = os.path.join(FDIR, exe)
+    if not os.path.exists(path):
+        pytest.fail(f"{path} is not built (make -C mepinterp_amd/fortran)")
+    return path
+
+
+def _cfg(tmp, seed, a_latt, centers, mp, T, n_ef, n_hw, nval, eta_ev=0.1, jobs=("ahc", "mep", "kubo")):
+    from mepinterp_amd.formats import write_input_cfg
+    write_input_cfg(os.path.join(tmp, "input.cfg"), dict(
+        do_ahc="ahc" in jobs, do_mep="mep" in jobs, do_kubo="kubo" in jobs,
+        a_latt=a_latt, seed_name=seed, wf_centers=centers, mp_grid=mp, N_eF=n_ef,
+        eF_min_eV=-0.2, eF_max_eV=0.2, T_kelvin=T, eta_eV=eta_ev, kubo_tol=1e-5,
+        valence_bands=nval, do_write_mep_bands=True, N_hw=n_hw, hw_min_eV=0.0, hw_max_eV=3.0))
+
+
+def test_mepinterp_executable_femn(tmp_path):
+    import oracle as O
+    from mepinterp_amd.formats import read_hr_dat, read_input_cfg, read_latt
+    exe = _need("mepInterp")
+    tmp = str(tmp_path)
+    os.makedirs(os.path.join(tmp, "w90files"))
+    shutil.copy(os.path.join(FEMN, "delta0.900", "FeMn_hr.dat"), os.path.join(tmp, "w90files"))
+    latt = read_latt(os.path.join(FEMN, "delta0.900", "FeMn_d0.9.latt"))
+    _cfg(tmp, "FeMn", latt, FEMN_CENTERS, (12, 12, 12), 0.0, 1, 25, 2, jobs=("ahc", "mep"))
+    r = subprocess.run([exe], cwd=tmp, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    cfg = read_input_cfg(os.path.join(tmp, "input.cfg"))
+    R, H, _ = read_hr_dat(os.path.join(tmp, "w90files", "FeMn_hr.dat"))
+    m = O.Model(R, H, wf_centers=cfg["wf_centers"], a_latt=cfg["a_latt"], kubo_tol=1e-5)
+    p = O.RunParams(do_ahc=True, do_mep=True, n_eF=1, eF_min=cfg["eF_min"], eF_max=cfg["eF_max"],
+                    n_hw=cfg["N_hw"], hw_min=cfg["hw_min"], hw_max=cfg["hw_max"],
+                    T_kelvin=0.0, eta=cfg["eta"], valence_bands=2)
+    o = O.finalize(O.accumulate(m, p, O.mp_mesh((12, 12, 12))), p, (12, 12, 12), cfg["a_latt"])
+    out = os.path.join(tmp, "out")
+    ahc = np.load(os.path.join(out, "ahc_tens.npy"))
+    assert abs(ahc[0, 1] - o["ahc_tens"][0, 1]) < 1e-8 * abs(o["ahc_tens"][0, 1])
+    velo = np.load(os.path.join(out, "ahcVELO.npy"))
+    assert np.max(np.abs(velo - o["ahcVELO"])) < 1e-8 * np.max(np.abs(o["ahcVELO"]))
+    hw = np.load(os.path.join(out, "hw_lst.npy"))
+    assert hw.shape == (25,) and abs(hw[-1] - 3.0) < 1e-12
+    mep = np.load(os.path.join(out, "mep_tens.npy"))
+    assert mep.shape == (3, 3) and np.max(np.abs(mep - o["mep_tens"])) < 1e-12
+    assert np.load(os.path.join(out, "mep_bands.npy")).shape == (3, 3, 2)
+    # header bytes exactly as the reference's files
+    with open(os.path.join(out, "ahc_tens.npy"), "rb") as f:
+        head = f.read(82)
+    with open(os.path.join(FEMN, "delta0.900", "ahc_tens.npy"), "rb") as f:
+        assert head == f.read(82)
+    assert os.path.exists(os.path.join(out, "occ.dat"))
+
+
+@pytest.mark.parametrize("kind", ["tb", "pos"])
+def test_fortran_per_k_interface(tmp_path, kind):
+    import oracle as O
+    from mepinterp_amd import models
+    from mepinterp_amd.formats import read_hr_dat, read_input_cfg, read_r_dat, write_hr_dat, \
+        write_r_dat
+    exe = _need("dropin_check")
+    tmp = str(tmp_path)
+    os.makedirs(os.path.join(tmp, "w90files"))
+    if kind == "tb":
+        mdl = models.make_random(seed=77, n=10, L=1, tb=True)
+        nval, T = 4, 300.0
+    else:
+        mdl = models.make_c1(seed=5)
+        nval, T = 4, 300.0
+    write_hr_dat(os.path.join(tmp, "w90files", "m_hr.dat"), mdl["R_frac"], mdl["H"])
+    if mdl["r"] is not None:
+        write_r_dat(os.path.join(tmp, "w90files", "m_r.dat"), mdl["R_frac"], mdl["r"])
+    _cfg(tmp, "m", mdl["a_latt"], mdl["wf_centers"], (2, 2, 2), T, 2, 3, nval, eta_ev=0.05)
+    k = rand_kpts(5, seed=11)
+    with open(os.path.join(tmp, "kpts.txt"), "w") as f:
+        f.write(f"{k.shape[1]}\n")
+        for i in range(k.shape[1]):
+            f.write(" ".join(repr(float(x)) for x in k[:, i]) + "\n")
+    r = subprocess.run([exe], cwd=tmp, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    cfg = read_input_cfg(os.path.join(tmp, "input.cfg"))
+    R, H, _ = read_hr_dat(os.path.join(tmp, "w90files", "m_hr.dat"))
+    rr = read_r_dat(os.path.join(tmp, "w90files", "m_r.dat"), R) if mdl["r"] is not None else None
+    om = O.Model(R, H, r=rr, wf_centers=cfg["wf_centers"], a_latt=cfg["a_latt"], kubo_tol=1e-5)
+    out = os.path.join(tmp, "out")
+    n = H.shape[0]
+    e_f = np.load(os.path.join(out, "dropin_e.npy"))
+    V_f = np.load(os.path.join(out, "dropin_V.npy")).reshape(3, n, n, -1, order="F")
+    ef = O.linspace(cfg["eF_min"], cfg["eF_max"], cfg["N_eF"])
+    hw = O.linspace(cfg["hw_min"], cfg["hw_max"], cfg["N_hw"])
+    got = {key: np.load(os.path.join(out, f"dropin_{key}.npy")) for key in
+           ("ahc", "velo", "ohc", "kic", "klc", "kcs", "nic", "nlc", "ncs")}
+    for ik in range(k.shape[1]):
+        e, V, A, Om = O.get_wann_interp(om, k[:, ik])
+        assert np.max(np.abs(e_f[:, ik] - e)) < 1e-10 / AU_TO_EV
+        assert np.max(np.abs(V_f[..., ik] - V)) < 1e-8 * np.abs(V).max()
+        fd = O.fd_distrib(e, ef, T)
+        want = {"ahc": O.kubo_ahc_tens(e, V, fd),
+                "velo": O.velo_ahc_tens(e, V, hw, fd, cfg["eta"]).reshape(3, 3, -1, order="F"),
+                "ohc": O.kubo_ohc_tens(e, V, hw, fd, cfg["eta"]).reshape(3, 3, -1, order="F"),
+                "kic": O.kubo_mep_ic(e, V, fd), "klc": O.kubo_mep_lc(e, V, fd),
+                "kcs": O.kubo_mep_cs(A, Om, fd),
+                "nic": O.mep_niu_ic(V, e, nval, 1e-5), "nlc": O.mep_niu_lc(V, e, nval, 1e-5),
+                "ncs": O.mep_niu_cs(A, Om, nval)}
+        for key, w in want.items():
+            per = w.shape[2]
+            g = got[key][:, :, ik * per:(ik + 1) * per] if key not in ("velo", "ohc") \
+                else got[key][..., ik]
+            scale = max(np.abs(w).max(), 1e-300)
+            assert np.max(np.abs(g - w.reshape(g.shape, order="F"))) <= 1e-8 * scale, (key, ik)
