@@ -65,6 +65,7 @@ struct mep_ctx {
   zd* dA = nullptr;
   zd* dOm = nullptr;
   int* dinfo = nullptr;
+  double* dscr = nullptr;  // eigensolver records
   double* dpart = nullptr;
   int64_t part_cap = 0;
   double* dacc = nullptr;
@@ -128,7 +129,8 @@ static void resolve_times(mep_ctx* c) {
 // ---------------------------------------------------------------------------
 static void free_ws(mep_ctx* c) {
   hipFree(c->dk); hipFree(c->dw); hipFree(c->dP); hipFree(c->dHk); hipFree(c->dE);
-  hipFree(c->dU); hipFree(c->dV); hipFree(c->dA); hipFree(c->dOm);
+  hipFree(c->dU); hipFree(c->dV); hipFree(c->dA); hipFree(c->dOm); hipFree(c->dscr);
+  c->dscr = nullptr;
   c->dk = c->dw = nullptr; c->dP = c->dHk = c->dU = c->dV = c->dA = c->dOm = nullptr;
   c->dE = nullptr;
   c->kcap = 0;
@@ -138,6 +140,7 @@ static int64_t bytes_per_k(const mep_ctx* c) {
   int64_t n2 = c->n * c->n;
   int64_t b = 3 * 8 + 8 + c->nR * 16 + (int64_t)c->nmat * n2 * 16 + c->n * 8 + n2 * 16 + 3 * n2 * 16;
   if (c->pos) b += 12 * n2 * 16;
+  b += (int64_t)eigh_scratch_bytes((int)c->n, 1);
   return b;
 }
 
@@ -156,6 +159,7 @@ static int ensure_ws(mep_ctx* c, int64_t nk_total) {
   HIPCHK(hipMalloc(&c->dE, sizeof(double) * c->n * want));
   HIPCHK(hipMalloc(&c->dU, sizeof(zd) * n2 * want));
   HIPCHK(hipMalloc(&c->dV, sizeof(zd) * 3 * n2 * want));
+  HIPCHK(hipMalloc(&c->dscr, eigh_scratch_bytes((int)c->n, want)));
   if (c->pos) {
     HIPCHK(hipMalloc(&c->dA, sizeof(zd) * 3 * n2 * want));
     HIPCHK(hipMalloc(&c->dOm, sizeof(zd) * 9 * n2 * want));
@@ -325,7 +329,7 @@ static int interp_block(mep_ctx* c, int64_t nb, bool velo) {
   });
   KGeom g = geom(c);
   timed(c, K_EIGH, [&] {
-    launch_eigh(c->st, (int)n, nb, c->dHk, c->nmat * n2, g, c->dE, c->dU, c->dinfo);
+    launch_eigh(c->st, (int)n, nb, c->dHk, c->nmat * n2, g, c->dE, c->dU, c->dinfo, c->dscr);
   });
   if (velo) {
     VeloArgs va;
@@ -482,9 +486,13 @@ extern "C" int mep_interp_batch(mep_ctx* c, int64_t nk, const double* kpts, doub
     CHK(interp_block(c, nb, V_ka != nullptr));
     HIPCHK(hipMemcpyAsync(e_k + n * k0, c->dE, sizeof(double) * n * nb, hipMemcpyDeviceToHost,
                           c->st));
-    if (V_ka)
-      HIPCHK(hipMemcpyAsync(V_ka + 2 * 3 * n2 * k0, c->dV, sizeof(zd) * 3 * n2 * nb,
+    if (V_ka) {
+      // planar -> reference layout through the (now free) Fourier buffer
+      launch_v_layout(c->st, (int)n, nb, c->dV, c->dHk, true);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipMemcpyAsync(V_ka + 2 * 3 * n2 * k0, c->dHk, sizeof(zd) * 3 * n2 * nb,
                             hipMemcpyDeviceToHost, c->st));
+    }
     if (V_ka && c->pos) {
       HIPCHK(hipMemcpyAsync(A_ka + 2 * 3 * n2 * k0, c->dA, sizeof(zd) * 3 * n2 * nb,
                             hipMemcpyDeviceToHost, c->st));
@@ -523,8 +531,10 @@ extern "C" int mep_bandsum_batch(mep_ctx* c, int64_t nk, const double* e_k, cons
       HIPCHK(hipMemcpyAsync(c->dfd, fd_distrib + nef * n * k0, sizeof(double) * nef * n * nb,
                             hipMemcpyHostToDevice, c->st));
     HIPCHK(hipMemcpyAsync(c->dE, e_k + n * k0, sizeof(double) * n * nb, hipMemcpyHostToDevice, c->st));
-    HIPCHK(hipMemcpyAsync(c->dV, V_ka + 2 * 3 * n2 * k0, sizeof(zd) * 3 * n2 * nb,
+    HIPCHK(hipMemcpyAsync(c->dHk, V_ka + 2 * 3 * n2 * k0, sizeof(zd) * 3 * n2 * nb,
                           hipMemcpyHostToDevice, c->st));
+    launch_v_layout(c->st, (int)n, nb, c->dHk, c->dV, false);
+    HIPCHK(hipGetLastError());
     if (c->pos) {
       HIPCHK(hipMemcpyAsync(c->dA, A_ka + 2 * 3 * n2 * k0, sizeof(zd) * 3 * n2 * nb,
                             hipMemcpyHostToDevice, c->st));

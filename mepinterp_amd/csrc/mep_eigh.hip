@@ -5,44 +5,67 @@
 // triangle of H(k) is read, eigenvalues ascending, eigenvectors in columns.
 //
 // Supported sizes: n <= 32 (the caller checks).
-// Method (per k, matrix resident in LDS, lane i <-> row i):
-//   1. Householder reduction A = Q T Q^H (rank-2 trailing updates); the
-//      reflectors are kept in the annihilated columns of A.
-//   2. Diagonal phases P make the subdiagonal real: T = P T_r P^H.
-//   3. Implicit-shift QL on T_r (scalar recurrence evaluated redundantly by
-//      every lane; each rotation updates one row of Z per lane).
-//   4. U = Q P Z: lane j keeps column j of P Z in registers and applies the
-//      reflectors in reverse order; columns are written in ascending order.
-//   5. Gauge convention shared with the oracle: each eigenvector's
-//      largest-modulus component is made real and positive.
+// Three kernels per k-block:
+//   eigh_house (a team of NMAX lanes per k, lane i <-> row i, matrix in LDS):
+//      Householder reduction A = Q T Q^H with rank-2 trailing updates; the
+//      reflectors are stored in U's slot of the k-point, the diagonal phases
+//      P that make the subdiagonal real (T = P T_r P^H), d, e and tau in a
+//      side record.
+//   eigh_ql (one LANE per k-point, 64 k-points per wavefront): the implicit
+//      QL recurrence on T_r (EISPACK tql2 method) with d, e in LDS columns;
+//      every sweep's Givens rotations (c_i, s_i) are appended to a per-k
+//      record in HBM; eigenvalues are ranked and written in ascending order.
+//   eigh_vec (a team per k): replays the rotations on Z (LDS, lane r <-> row
+//      r), then U = Q P Z: lane j keeps column j of P Z in registers and
+//      applies the reflectors in reverse order.
+//   Gauge convention shared with the oracle: each eigenvector's
+//   largest-modulus component is made real and positive.
 #include "mep_common.hpp"
 #include "mep_kernels.hpp"
 
-template <int NMAX>
-__global__ __launch_bounds__(64) void eigh_kernel(int n, int64_t nk, const zd* __restrict__ Hk,
-                                                  int64_t strideHk, const double* __restrict__ kpts,
-                                                  const double* __restrict__ tau,
-                                                  double* __restrict__ E, zd* __restrict__ U,
-                                                  int* __restrict__ info) {
-  constexpr int LDA = NMAX;
-  constexpr int LDZ = NMAX + 1;
-  __shared__ zd a[NMAX * NMAX];
-  __shared__ double Zt[NMAX * LDZ];  // Zt[j*LDZ + i] = Z(i, j)
-  __shared__ zd vv[NMAX], ww[NMAX], sub[NMAX], ph[NMAX];
-  __shared__ double tauv[NMAX], d[NMAX], e[NMAX];
-  const int t = threadIdx.x;
-  const double two_pi = 2.0 * 3.14159265358979323846;
-  const double EPS = 2.220446049250313e-16;
+// NMAX lanes form a team that owns one k-point; a wavefront runs 64/NMAX
+// teams side by side.  Team reductions use width-NMAX shuffles.
+template <int W>
+__device__ inline double team_sum(double v) {
+#pragma unroll
+  for (int o = W / 2; o >= 1; o >>= 1) v += __shfl_xor(v, o, W);
+  return v;
+}
 
-  for (int64_t k = blockIdx.x; k < nk; k += gridDim.x) {
-    const zd* H = Hk + k * strideHk;
-    const double kx = kpts[3 * k], ky = kpts[3 * k + 1], kz = kpts[3 * k + 2];
+// side record per k-point: d[n] e[n] tau[n] ph[2n] (doubles)
+__host__ __device__ inline int64_t side_len(int n) { return 5 * (int64_t)n; }
+// rotation record per k-point (pairs of doubles): capacity
+__host__ __device__ inline int64_t rot_cap(int n) { return 4 * (int64_t)n * n + 4 * (int64_t)n + 8; }
+
+template <int NMAX>
+__global__ __launch_bounds__(64) void eigh_house_kernel(int n, int64_t nk,
+                                                        const zd* __restrict__ Hk, int64_t strideHk,
+                                                        const double* __restrict__ kpts,
+                                                        const double* __restrict__ tau,
+                                                        zd* __restrict__ U,
+                                                        double* __restrict__ side) {
+  constexpr int TEAMS = 64 / NMAX;
+  constexpr int LDA = NMAX;
+  __shared__ zd a_s[TEAMS][NMAX * NMAX];
+  __shared__ zd vv_s[TEAMS][NMAX], ww_s[TEAMS][NMAX], sub_s[TEAMS][NMAX];
+  __shared__ double tauv_s[TEAMS][NMAX];
+  const int lane = threadIdx.x;
+  const int tm = lane / NMAX, t = lane % NMAX;
+  zd* a = a_s[tm];
+  zd *vv = vv_s[tm], *ww = ww_s[tm], *sub = sub_s[tm];
+  double* tauv = tauv_s[tm];
+  const double two_pi = 2.0 * 3.14159265358979323846;
+
+  for (int64_t kb = (int64_t)blockIdx.x * TEAMS; kb < nk; kb += (int64_t)gridDim.x * TEAMS) {
+    const int64_t k = kb + tm;
+    const bool valid = k < nk;
     // ---- load the upper triangle (tight-binding phase applied) ----------
-    for (int idx = t; idx < n * n; idx += 64) {
+    for (int idx = t; idx < n * n; idx += NMAX) {
       int i = idx % n, j = idx / n;
       if (i <= j) {
-        zd h = H[i + (int64_t)n * j];
-        if (tau && i != j) {
+        zd h = valid ? Hk[k * strideHk + i + (int64_t)n * j] : zd{0.0, 0.0};
+        if (valid && tau && i != j) {
+          const double kx = kpts[3 * k], ky = kpts[3 * k + 1], kz = kpts[3 * k + 2];
           double ang = two_pi * (kx * (tau[3 * j] - tau[3 * i]) + ky * (tau[3 * j + 1] - tau[3 * i + 1]) +
                                  kz * (tau[3 * j + 2] - tau[3 * i + 2]));
           double sn, cs;
@@ -58,48 +81,42 @@ __global__ __launch_bounds__(64) void eigh_kernel(int n, int64_t nk, const zd* _
       }
     }
     __syncthreads();
-
-    // ---- 1. Householder tridiagonalisation ------------------------------
+    // ---- Householder (an already reduced column gets tau = 0, v = 0) ----
     for (int kc = 0; kc + 2 < n; ++kc) {
       const int m = n - kc - 1;
-      zd x = (t < m) ? a[(kc + 1 + t) + LDA * kc] : zd{0.0, 0.0};
-      double tail2 = wave_sum((t >= 1 && t < m) ? zabs2(x) : 0.0);
-      zd x0 = zd{__shfl(x.re, 0, 64), __shfl(x.im, 0, 64)};
-      double tk = 0.0;
-      zd alpha = x0;
-      zd v = zd{0.0, 0.0};
-      if (tail2 > 0.0) {
-        double ax02 = zabs2(x0);
-        double xn = sqrt(tail2 + ax02);
-        double ax0 = sqrt(ax02);
-        alpha = (ax0 > 0.0) ? zscale(x0, -xn / ax0) : zd{-xn, 0.0};
-        v = x;
-        if (t == 0) v = zsub(x, alpha);
-        double vn2 = wave_sum((t < m) ? zabs2(v) : 0.0);
-        tk = 2.0 / vn2;
-        if (t < NMAX) vv[t] = (t < m) ? v : zd{0.0, 0.0};
-        __syncthreads();
-        zd p = zd{0.0, 0.0};
-        if (t < m) {
-          const zd* row = &a[kc + 1 + t + LDA * (kc + 1)];
-          for (int j = 0; j < m; ++j) p = zadd(p, zmul(row[LDA * j], vv[j]));
-          p = zscale(p, tk);
-        }
-        double K = 0.5 * tk * wave_sum((t < m) ? (v.re * p.re + v.im * p.im) : 0.0);
-        zd w = zsub(p, zscale(v, K));
-        if (t < NMAX) ww[t] = (t < m) ? w : zd{0.0, 0.0};
-        __syncthreads();
-        if (t < m) {
-          zd* row = &a[kc + 1 + t + LDA * (kc + 1)];
-          for (int j = 0; j < m; ++j) {
-            zd vj = vv[j], wj = ww[j];
-            zd upd = zadd(zmul(v, zconj(wj)), zmul(w, zconj(vj)));
-            row[LDA * j] = zsub(row[LDA * j], upd);
-          }
+      const zd x = (t < m) ? a[(kc + 1 + t) + LDA * kc] : zd{0.0, 0.0};
+      const double tail2 = team_sum<NMAX>((t >= 1 && t < m) ? zabs2(x) : 0.0);
+      const zd x0 = zd{__shfl(x.re, 0, NMAX), __shfl(x.im, 0, NMAX)};
+      const bool refl = tail2 > 0.0;
+      const double ax02 = zabs2(x0);
+      const double xn = sqrt(tail2 + ax02);
+      const double ax0 = sqrt(ax02);
+      const zd alpha = !refl ? x0 : ((ax0 > 0.0) ? zscale(x0, -xn / ax0) : zd{-xn, 0.0});
+      zd v = refl ? x : zd{0.0, 0.0};
+      if (t == 0 && refl) v = zsub(x, alpha);
+      const double vn2 = team_sum<NMAX>((t < m) ? zabs2(v) : 0.0);
+      const double tk = refl ? 2.0 / vn2 : 0.0;
+      vv[t] = (t < m) ? v : zd{0.0, 0.0};
+      __syncthreads();
+      zd p = zd{0.0, 0.0};
+      if (t < m) {
+        const zd* row = &a[kc + 1 + t + LDA * (kc + 1)];
+        for (int j = 0; j < m; ++j) p = zadd(p, zmul(row[LDA * j], vv[j]));
+        p = zscale(p, tk);
+      }
+      const double K = 0.5 * tk * team_sum<NMAX>((t < m) ? (v.re * p.re + v.im * p.im) : 0.0);
+      const zd w = zsub(p, zscale(v, K));
+      ww[t] = (t < m) ? w : zd{0.0, 0.0};
+      __syncthreads();
+      if (t < m && refl) {
+        zd* row = &a[kc + 1 + t + LDA * (kc + 1)];
+        for (int j = 0; j < m; ++j) {
+          const zd vj = vv[j], wj = ww[j];
+          const zd upd = zadd(zmul(v, zconj(wj)), zmul(w, zconj(vj)));
+          row[LDA * j] = zsub(row[LDA * j], upd);
         }
       }
-      // keep the reflector in the annihilated column, the subdiagonal aside
-      if (t < m) a[(kc + 1 + t) + LDA * kc] = (tail2 > 0.0) ? v : zd{0.0, 0.0};
+      if (t < m) a[(kc + 1 + t) + LDA * kc] = v;
       if (t == 0) {
         sub[kc] = alpha;
         tauv[kc] = tk;
@@ -110,102 +127,195 @@ __global__ __launch_bounds__(64) void eigh_kernel(int n, int64_t nk, const zd* _
       sub[n - 2] = a[(n - 1) + LDA * (n - 2)];
       tauv[n - 2] = 0.0;
     }
-    if (t < n) d[t] = a[t + LDA * t].re;
-    for (int idx = t; idx < NMAX * LDZ; idx += 64) {
-      int j = idx / LDZ, i = idx % LDZ;
-      Zt[idx] = (i == j) ? 1.0 : 0.0;
-    }
     __syncthreads();
-    // ---- 2. real tridiagonal by diagonal phases -------------------------
-    if (t == 0) {
-      ph[0] = zd{1.0, 0.0};
-      for (int i = 0; i + 1 < n; ++i) {
-        zd s = sub[i];
-        double as = sqrt(zabs2(s));
-        e[i] = as;
-        ph[i + 1] = (as > 0.0) ? zmul(ph[i], zscale(s, 1.0 / as)) : ph[i];
+    if (valid) {
+      double* sd = side + k * side_len(n);
+      // reflectors (strict lower triangle) into U's slot
+      zd* Uk = U + k * (int64_t)n * n;
+      for (int idx = t; idx < n * n; idx += NMAX) Uk[idx] = a[idx % n + LDA * (idx / n)];
+      if (t < n) {
+        sd[t] = a[t + LDA * t].re;  // d
+        sd[2 * n + t] = (t < n - 1) ? tauv[t] : 0.0;
       }
-      e[n - 1] = 0.0;
+      if (t == 0) {
+        // phases and the real subdiagonal (serial recurrence)
+        zd ph = zd{1.0, 0.0};
+        sd[3 * n] = 1.0;
+        sd[3 * n + 1] = 0.0;
+        for (int i = 0; i + 1 < n; ++i) {
+          const zd s_ = sub[i];
+          const double as = sqrt(zabs2(s_));
+          sd[n + i] = as;
+          if (as > 0.0) ph = zmul(ph, zscale(s_, 1.0 / as));
+          sd[3 * n + 2 * (i + 1)] = ph.re;
+          sd[3 * n + 2 * (i + 1) + 1] = ph.im;
+        }
+        sd[n + n - 1] = 0.0;
+      }
     }
     __syncthreads();
-    // ---- 3. implicit QL; every lane evaluates the same scalar recurrence --
-    int fail = 0;
-    for (int l = 0; l < n && !fail; ++l) {
-      int iter = 0;
-      for (;;) {
-        int m = l;
-        for (; m < n - 1; ++m) {
-          double dd = fabs(d[m]) + fabs(d[m + 1]);
-          if (fabs(e[m]) <= EPS * dd) break;
-        }
-        if (m == l) break;
-        if (++iter > 60) {
-          fail = 1;
-          break;
-        }
-        double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
-        double r = hypot(g, 1.0);
-        g = d[m] - d[l] + e[l] / (g + copysign(r, g));
-        double s = 1.0, c = 1.0, p = 0.0;
-        bool underflow = false;
-        for (int i = m - 1; i >= l; --i) {
-          double f = s * e[i], b = c * e[i];
-          r = hypot(f, g);
-          e[i + 1] = r;
-          if (r == 0.0) {
-            d[i + 1] -= p;
-            e[m] = 0.0;
-            underflow = true;
+  }
+}
+
+// One lane per k-point: implicit QL on the real tridiagonal (d, e) held in
+// LDS columns; rotations are appended to rot[k]; ascending eigenvalues go to
+// E.  rot record: per sweep a header (ilow, m) followed by the m - ilow
+// pairs (c_i, s_i) for i = m-1 down to ilow; a header with m < 0 ends it.
+template <int NMAX>
+__global__ __launch_bounds__(64) void eigh_ql_kernel(int n, int64_t nk,
+                                                     double* __restrict__ side,
+                                                     double* __restrict__ rot,
+                                                     double* __restrict__ E,
+                                                     int* __restrict__ info) {
+  __shared__ double ds[NMAX * 64], es[NMAX * 64];
+  const int lane = threadIdx.x;
+  const double EPS = 2.220446049250313e-16;
+  const int64_t cap = rot_cap(n);
+  for (int64_t kb = (int64_t)blockIdx.x * 64; kb < nk; kb += (int64_t)gridDim.x * 64) {
+    const int64_t k = kb + lane;
+    if (k < nk) {
+      const double* sd = side + k * side_len(n);
+      for (int i = 0; i < n; ++i) {
+        ds[i * 64 + lane] = sd[i];
+        es[i * 64 + lane] = sd[n + i];
+      }
+      double* rk = rot + k * 2 * cap;
+      int64_t pos = 0;
+      int fail = 0;
+      for (int l = 0; l < n && !fail; ++l) {
+        int iter = 0;
+        for (;;) {
+          int m = l;
+          for (; m < n - 1; ++m) {
+            const double dd = fabs(ds[m * 64 + lane]) + fabs(ds[(m + 1) * 64 + lane]);
+            if (fabs(es[m * 64 + lane]) <= EPS * dd) break;
+          }
+          if (m == l) break;
+          if (++iter > 60 || pos + 2 + (m - l) >= cap) {
+            fail = 1;
             break;
           }
-          s = f / r;
-          c = g / r;
-          g = d[i + 1] - p;
-          r = (d[i] - g) * s + 2.0 * c * b;
-          p = s * r;
-          d[i + 1] = g + p;
-          g = c * r - b;
-          if (t < n) {
-            double zf = Zt[(i + 1) * LDZ + t];
-            double zi = Zt[i * LDZ + t];
-            Zt[(i + 1) * LDZ + t] = s * zi + c * zf;
-            Zt[i * LDZ + t] = c * zi - s * zf;
+          const double dl = ds[l * 64 + lane], el = es[l * 64 + lane];
+          double g = (ds[(l + 1) * 64 + lane] - dl) / (2.0 * el);
+          double r = sqrt(g * g + 1.0);
+          g = ds[m * 64 + lane] - dl + el / (g + copysign(r, g));
+          double s = 1.0, c = 1.0, p = 0.0;
+          const int64_t hdr = pos;
+          pos += 1;
+          int ilow = l;
+          bool underflow = false;
+          for (int i = m - 1; i >= l; --i) {
+            const double ei = es[i * 64 + lane];
+            const double f = s * ei, b = c * ei;
+            r = sqrt(f * f + g * g);
+            es[(i + 1) * 64 + lane] = r;
+            if (r == 0.0) {
+              ds[(i + 1) * 64 + lane] -= p;
+              es[m * 64 + lane] = 0.0;
+              underflow = true;
+              ilow = i + 1;
+              break;
+            }
+            const double rr = 1.0 / r;
+            s = f * rr;
+            c = g * rr;
+            g = ds[(i + 1) * 64 + lane] - p;
+            r = (ds[i * 64 + lane] - g) * s + 2.0 * c * b;
+            p = s * r;
+            ds[(i + 1) * 64 + lane] = g + p;
+            g = c * r - b;
+            rk[2 * pos] = c;
+            rk[2 * pos + 1] = s;
+            pos += 1;
           }
+          rk[2 * hdr] = (double)ilow;
+          rk[2 * hdr + 1] = (double)m;
+          if (underflow) continue;
+          ds[l * 64 + lane] -= p;
+          es[l * 64 + lane] = g;
+          es[m * 64 + lane] = 0.0;
         }
-        if (underflow) continue;
-        d[l] -= p;
-        e[l] = g;
-        e[m] = 0.0;
       }
+      rk[2 * pos] = 0.0;
+      rk[2 * pos + 1] = -1.0;
+      if (fail) atomicOr(info, 1);
+      // ascending order: rank of each eigenvalue (ties by index)
+      for (int j = 0; j < n; ++j) {
+        const double dj = ds[j * 64 + lane];
+        int rnk = 0;
+        for (int i = 0; i < n; ++i) {
+          const double di = ds[i * 64 + lane];
+          rnk += (di < dj) || (di == dj && i < j);
+        }
+        E[k * n + rnk] = dj;
+        es[j * 64 + lane] = (double)rnk;  // e is no longer needed: keep the rank
+      }
+      double* sdw = side + k * side_len(n);
+      for (int j = 0; j < n; ++j) sdw[n + j] = es[j * 64 + lane];  // rank over e's slot
     }
-    if (fail && t == 0) atomicOr(info, 1);
+  }
+}
+
+// A team per k-point: replay the rotations on Z (LDS), then U = Q P Z.
+template <int NMAX>
+__global__ __launch_bounds__(64) void eigh_vec_kernel(int n, int64_t nk,
+                                                      const double* __restrict__ side,
+                                                      const double* __restrict__ rot,
+                                                      zd* __restrict__ U) {
+  constexpr int TEAMS = 64 / NMAX;
+  constexpr int LDZ = NMAX + 1;
+  __shared__ double Zt_s[TEAMS][NMAX * LDZ];  // Zt[col*LDZ + row]
+  const int lane = threadIdx.x;
+  const int tm = lane / NMAX, t = lane % NMAX;
+  double* Zt = Zt_s[tm];
+  const int64_t cap = rot_cap(n);
+  for (int64_t kb = (int64_t)blockIdx.x * TEAMS; kb < nk; kb += (int64_t)gridDim.x * TEAMS) {
+    const int64_t k = kb + tm;
+    const bool valid = k < nk;
+    for (int idx = t; idx < NMAX * LDZ; idx += NMAX) Zt[idx] = ((idx / LDZ) == (idx % LDZ)) ? 1.0 : 0.0;
     __syncthreads();
-    // ---- ascending rank of each eigenvalue ------------------------------
-    int rnk = 0;
-    double dj = 0.0;
-    if (t < n) {
-      dj = d[t];
-      for (int i = 0; i < n; ++i) {
-        double di = d[i];
-        rnk += (di < dj) || (di == dj && i < t);
+    if (valid && t < n) {
+      const double* rk = rot + k * 2 * cap;
+      int64_t pos = 0;
+      for (;;) {
+        const int ilow = (int)rk[2 * pos];
+        const int m = (int)rk[2 * pos + 1];
+        pos += 1;
+        if (m < 0) break;
+        // rotation i mixes columns (i, i+1); carry column i between steps
+        double carry = Zt[m * LDZ + t];
+        for (int i = m - 1; i >= ilow; --i) {
+          const double c = rk[2 * pos], s = rk[2 * pos + 1];
+          pos += 1;
+          const double zi = Zt[i * LDZ + t];
+          Zt[(i + 1) * LDZ + t] = s * zi + c * carry;
+          carry = c * zi - s * carry;
+        }
+        Zt[ilow * LDZ + t] = carry;
       }
     }
-    // ---- 4. back-transformation U = Q P Z (lane j <-> column j) ----------
+    __syncthreads();
+    // ---- back-transformation (lane j <-> column j) -----------------------
+    const double* sd = side + (valid ? k : 0) * side_len(n);
+    const zd* Uk = U + (valid ? k : 0) * (int64_t)n * n;  // reflectors
     zd W[NMAX];
 #pragma unroll
     for (int i = 0; i < NMAX; ++i)
-      W[i] = (i < n && t < n) ? zscale(ph[i], Zt[t * LDZ + i]) : zd{0.0, 0.0};
-    for (int kc = n - 3; kc >= 0; --kc) {
-      double tk = tauv[kc];
-      if (tk == 0.0) continue;
-      zd s = zd{0.0, 0.0};
+      W[i] = (valid && i < n && t < n)
+                 ? zscale(zd{sd[3 * n + 2 * i], sd[3 * n + 2 * i + 1]}, Zt[t * LDZ + i])
+                 : zd{0.0, 0.0};
+    if (valid) {
+      for (int kc = n - 3; kc >= 0; --kc) {
+        const double tk = sd[2 * n + kc];
+        zd sacc = zd{0.0, 0.0};
 #pragma unroll
-      for (int i = 0; i < NMAX; ++i)
-        if (i > kc && i < n) s = zadd(s, zcmul(a[i + LDA * kc], W[i]));
-      s = zscale(s, tk);
+        for (int i = 0; i < NMAX; ++i)
+          if (i > kc && i < n) sacc = zadd(sacc, zcmul(Uk[i + (int64_t)n * kc], W[i]));
+        sacc = zscale(sacc, tk);
 #pragma unroll
-      for (int i = 0; i < NMAX; ++i)
-        if (i > kc && i < n) W[i] = zsub(W[i], zmul(a[i + LDA * kc], s));
+        for (int i = 0; i < NMAX; ++i)
+          if (i > kc && i < n) W[i] = zsub(W[i], zmul(Uk[i + (int64_t)n * kc], sacc));
+      }
     }
     // gauge convention: largest-modulus component (first on ties) real > 0
     {
@@ -213,40 +323,55 @@ __global__ __launch_bounds__(64) void eigh_kernel(int n, int64_t nk, const zd* _
       zd u = zd{1.0, 0.0};
 #pragma unroll
       for (int i = 0; i < NMAX; ++i) {
-        double m2 = zabs2(W[i]);
+        const double m2 = zabs2(W[i]);
         if (i < n && m2 > best) {
           best = m2;
           u = W[i];
         }
       }
-      double au = sqrt(zabs2(u));
+      const double au = sqrt(zabs2(u));
       if (au > 0.0) {
-        zd phs = zd{u.re / au, -u.im / au};
+        const zd phs = zd{u.re / au, -u.im / au};
 #pragma unroll
         for (int i = 0; i < NMAX; ++i) W[i] = zmul(W[i], phs);
       }
     }
-    if (t < n) {
-      zd* Uk = U + k * (int64_t)n * n + (int64_t)n * rnk;
+    const int rnk = (valid && t < n) ? (int)sd[n + t] : 0;
+    __syncthreads();  // every lane has read the reflectors before U is written
+    if (valid && t < n) {
+      zd* Uo = U + k * (int64_t)n * n + (int64_t)n * rnk;
 #pragma unroll
       for (int i = 0; i < NMAX; ++i)
-        if (i < n) Uk[i] = W[i];
-      E[k * n + rnk] = dj;
+        if (i < n) Uo[i] = W[i];
     }
     __syncthreads();
   }
 }
 
+size_t eigh_scratch_bytes(int n, int64_t nk) {
+  return sizeof(double) * (size_t)nk * (size_t)(side_len(n) + 2 * rot_cap(n));
+}
+
 void launch_eigh(hipStream_t s, int n, int64_t nk, const zd* Hk, int64_t strideHk, const KGeom& g,
-                 double* E, zd* U, int* info) {
-  unsigned grid = (unsigned)(nk < 65536 ? nk : 65536);
-  if (n <= 8)
-    hipLaunchKernelGGL(eigh_kernel<8>, dim3(grid), dim3(64), 0, s, n, nk, Hk, strideHk, g.kpts,
-                       g.tau, E, U, info);
-  else if (n <= 16)
-    hipLaunchKernelGGL(eigh_kernel<16>, dim3(grid), dim3(64), 0, s, n, nk, Hk, strideHk, g.kpts,
-                       g.tau, E, U, info);
-  else
-    hipLaunchKernelGGL(eigh_kernel<32>, dim3(grid), dim3(64), 0, s, n, nk, Hk, strideHk, g.kpts,
-                       g.tau, E, U, info);
+                 double* E, zd* U, int* info, double* scratch) {
+  double* side = scratch;
+  double* rot = scratch + nk * side_len(n);
+  const int teams = n <= 8 ? 8 : n <= 16 ? 4 : 2;
+  int64_t blocks = (nk + teams - 1) / teams;
+  unsigned grid = (unsigned)(blocks < 65536 ? blocks : 65536);
+  int64_t qb = (nk + 63) / 64;
+  unsigned qgrid = (unsigned)(qb < 65536 ? qb : 65536);
+#define MEP_EIGH_LAUNCH(NM)                                                                      \
+  hipLaunchKernelGGL(eigh_house_kernel<NM>, dim3(grid), dim3(64), 0, s, n, nk, Hk, strideHk,     \
+                     g.kpts, g.tau, U, side);                                                    \
+  hipLaunchKernelGGL(eigh_ql_kernel<NM>, dim3(qgrid), dim3(64), 0, s, n, nk, side, rot, E, info); \
+  hipLaunchKernelGGL(eigh_vec_kernel<NM>, dim3(grid), dim3(64), 0, s, n, nk, side, rot, U);
+  if (n <= 8) {
+    MEP_EIGH_LAUNCH(8)
+  } else if (n <= 16) {
+    MEP_EIGH_LAUNCH(16)
+  } else {
+    MEP_EIGH_LAUNCH(32)
+  }
+#undef MEP_EIGH_LAUNCH
 }

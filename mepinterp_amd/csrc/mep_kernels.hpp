@@ -13,13 +13,15 @@ void launch_mp_kpts(hipStream_t s, int64_t k0, int64_t nk, const int mp[3], doub
 void launch_phase(hipStream_t s, int64_t nk, int nR, const double* kpts, const double* R, zd* P);
 void launch_ft_gemm_f64(hipStream_t s, int M, int N, int K, const zd* P, const zd* X, zd* C);
 
-// Batched Hermitian eigensolve of H(k) (upper triangle), n <= 64.
-// E (nk, n) ascending; U (nk, n, n) column-major eigenvectors.
+// Batched Hermitian eigensolve of H(k) (upper triangle), n <= 32.
+// E (nk, n) ascending; U (nk, n, n) column-major eigenvectors.  scratch
+// holds eigh_scratch_bytes(n, nk) bytes (side records + rotation records).
+size_t eigh_scratch_bytes(int n, int64_t nk);
 void launch_eigh(hipStream_t s, int n, int64_t nk, const zd* Hk, int64_t strideHk,
-                 const KGeom& g, double* E, zd* U, int* info);
+                 const KGeom& g, double* E, zd* U, int* info, double* scratch);
 
 // Velocity (and, with a position operator, connection and curvature) in
-// the Hamiltonian gauge, written in the reference layout V(3,n,n) per k.
+// the Hamiltonian gauge.  V is written planar per k: V[k][a][i + n j].
 struct VeloArgs {
   int n, nmat;
   int gauge_trafo;
@@ -30,7 +32,11 @@ struct VeloArgs {
 void launch_velo(hipStream_t s, const VeloArgs& va, int64_t nk, const zd* Hk, int64_t strideHk,
                  const KGeom& g, const zd* U, const double* E, zd* V, zd* A, zd* Om);
 
-// Band sums over a batch of k with per-block partial accumulators.
+// V layout conversion: planar [a][i + n j] <-> reference (3,n,n), per k.
+void launch_v_layout(hipStream_t s, int n, int64_t nk, const zd* src, zd* dst, bool to_ref);
+
+// Band sums over a batch of k with per-block partial accumulators
+// (V planar).
 struct BandArgs {
   int n;
   int jobs;  // MEP_JOB_* bits

@@ -9,10 +9,11 @@
 //   X'_nm = e^{2 pi i k.(tau_m - tau_n)} X_nm,
 //   dH'_a = ph (dH_a + i c_a(tau_m - tau_n) H),
 //   Om'_ab = ph (Om_ab + i c_a(dtau) A_b - i c_b(dtau) A_a).
-// Output: V (3,n,n) per k in the reference layout; with a position operator
-// also A^(H) (3,n,n) and Omega^(H) (3,3,n,n).
+// Output: V per k planar, V[k][a][i + n j]; with a position operator also
+// A^(H) (3,n,n) and Omega^(H) (3,3,n,n) in the reference layout.
 #include "mep_common.hpp"
 #include "mep_kernels.hpp"
+#include "mep_mfma.hpp"
 
 struct TBPhase {
   double kx, ky, kz;
@@ -67,70 +68,69 @@ __device__ inline zd load_tb(const zd* X, int n, int N, int row, int col, int ma
 }
 
 // --------------------------------------------------------------------------
-// no position operator: V_a = U^H dH_a U  (or dH_a without gauge transform)
+// no position operator: V_a = U^H dH_a U (6 complex n x n products per k on
+// FP64 MFMA, 4 waves), or V_a = dH_a without the gauge transformation.
+// V is written planar: V[k][a][i + n*j].
 // --------------------------------------------------------------------------
-template <int NMAX>
-__global__ __launch_bounds__(64) void velo_kernel(int n, int64_t nk, const zd* __restrict__ Hk,
-                                                  int64_t strideHk, TBPhase tb0, int gauge_trafo,
-                                                  const zd* __restrict__ U, zd* __restrict__ V) {
-  constexpr int LD = NMAX + 1;
-  __shared__ zd Us[NMAX * LD];
-  __shared__ zd Ds[NMAX * LD];
+__global__ __launch_bounds__(256) void velo_mfma_kernel(int n, int64_t nk, const zd* __restrict__ Hk,
+                                                        int64_t strideHk, TBPhase tb0,
+                                                        int gauge_trafo, const zd* __restrict__ U,
+                                                        zd* __restrict__ V) {
+  constexpr int LD = 33;
+  __shared__ zd Us[32 * LD], Ds[32 * LD], Ts[32 * LD];
   const int t = threadIdx.x;
+  const int n2 = n * n;
   for (int64_t k = blockIdx.x; k < nk; k += gridDim.x) {
     const TBPhase tb = tb0.at(k);
     const zd* X = Hk + k * strideHk;
-    zd* Vk = V + k * 3 * (int64_t)n * n;
-    if (gauge_trafo) {
-      for (int idx = t; idx < n * n; idx += 64) {
-        int i = idx % n, j = idx / n;
-        Us[i + LD * j] = U[k * (int64_t)n * n + idx];
-      }
-    }
+    zd* Vk = V + k * 3 * (int64_t)n2;
+    if (gauge_trafo)
+      for (int idx = t; idx < n2; idx += 256) Us[idx % n + LD * (idx / n)] = U[k * (int64_t)n2 + idx];
     for (int a = 0; a < 3; ++a) {
-      __syncthreads();
-      for (int idx = t; idx < n * n; idx += 64) {
+      for (int idx = t; idx < n2; idx += 256) {
         int i = idx % n, j = idx / n;
         Ds[i + LD * j] = load_tb(X, 0, n, i, j, 1 + a, tb);
       }
       __syncthreads();
       if (gauge_trafo) {
-        zd T[NMAX], Hb[NMAX];
-#pragma unroll
-        for (int i = 0; i < NMAX; ++i) T[i] = zd{0.0, 0.0};
-        if (t < n) {
-          for (int l = 0; l < n; ++l) {
-            zd ul = Us[l + LD * t];
-#pragma unroll
-            for (int i = 0; i < NMAX; ++i)
-              if (i < n) T[i] = zadd(T[i], zmul(Ds[i + LD * l], ul));
-          }
-        }
-#pragma unroll
-        for (int i = 0; i < NMAX; ++i) {
-          zd s = zd{0.0, 0.0};
-          if (i < n && t < n) {
-#pragma unroll
-            for (int l = 0; l < NMAX; ++l)
-              if (l < n) s = zadd(s, zcmul(Us[l + LD * i], T[l]));
-          }
-          Hb[i] = s;
-        }
+        zgemm_tiles<false>(n, Ds, LD, BPlain{Us, LD}, Ts, LD);  // T = dH U
         __syncthreads();
-        if (t < n) {
-#pragma unroll
-          for (int i = 0; i < NMAX; ++i)
-            if (i < n) Ds[i + LD * t] = Hb[i];
-        }
+        zgemm_tiles<true>(n, Us, LD, BPlain{Ts, LD}, Ds, LD);   // V_a = U^H T
         __syncthreads();
       }
-      for (int idx = t; idx < n * n; idx += 64) {
-        int i = idx % n, j = idx / n;
-        Vk[a + 3 * (int64_t)idx] = Ds[i + LD * j];
-      }
+      for (int idx = t; idx < n2; idx += 256) Vk[a * (int64_t)n2 + idx] = Ds[idx % n + LD * (idx / n)];
+      __syncthreads();
     }
-    __syncthreads();
   }
+}
+
+// V (3,n,n) per k: planar [a][i + n j]  <->  reference (a fastest)
+__global__ void v_planar_to_ref(int n, int64_t nk, const zd* __restrict__ Vp, zd* __restrict__ Vr) {
+  int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t per = 3 * (int64_t)n * n;
+  if (t >= nk * per) return;
+  int64_t k = t / per, r = t % per;
+  int a = (int)(r % 3);
+  int64_t ij = r / 3;
+  Vr[t] = Vp[k * per + a * (int64_t)n * n + ij];
+}
+__global__ void v_ref_to_planar(int n, int64_t nk, const zd* __restrict__ Vr, zd* __restrict__ Vp) {
+  int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t per = 3 * (int64_t)n * n, n2 = (int64_t)n * n;
+  if (t >= nk * per) return;
+  int64_t k = t / per, r = t % per;
+  int a = (int)(r / n2);
+  int64_t ij = r % n2;
+  Vp[t] = Vr[k * per + a + 3 * ij];
+}
+
+void launch_v_layout(hipStream_t s, int n, int64_t nk, const zd* src, zd* dst, bool to_ref) {
+  int64_t tot = nk * 3 * (int64_t)n * n;
+  unsigned nb = (unsigned)((tot + 255) / 256);
+  if (to_ref)
+    hipLaunchKernelGGL(v_planar_to_ref, dim3(nb), dim3(256), 0, s, n, nk, src, dst);
+  else
+    hipLaunchKernelGGL(v_ref_to_planar, dim3(nb), dim3(256), 0, s, n, nk, src, dst);
 }
 
 // --------------------------------------------------------------------------
@@ -272,7 +272,7 @@ __global__ __launch_bounds__(64) void velo_pos_kernel(int n, int64_t nk, const z
           zd ce = zd{0.0, Es[j] - Es[i]};
           v = zsub(v, zmul(ce, Asel[i + PLD * j]));
         }
-        V[k * 3 * NN + a + 3 * (int64_t)idx] = v;
+        V[k * 3 * NN + a * NN + idx] = v;
         Aout[k * 3 * NN + a + 3 * (int64_t)idx] = Ab[a][i + PLD * j];
       }
       // Omega (3,3,n,n): antisymmetric, zero diagonal in (a,b)
@@ -305,13 +305,6 @@ void launch_velo(hipStream_t s, const VeloArgs& va, int64_t nk, const zd* Hk, in
                        va.gauge_trafo, va.textbook, va.kubo_tol, U, E, V, A, Om);
     return;
   }
-  if (va.n <= 8)
-    hipLaunchKernelGGL(velo_kernel<8>, dim3(grid), dim3(64), 0, s, va.n, nk, Hk, strideHk, tb,
-                       va.gauge_trafo, U, V);
-  else if (va.n <= 16)
-    hipLaunchKernelGGL(velo_kernel<16>, dim3(grid), dim3(64), 0, s, va.n, nk, Hk, strideHk, tb,
-                       va.gauge_trafo, U, V);
-  else
-    hipLaunchKernelGGL(velo_kernel<32>, dim3(grid), dim3(64), 0, s, va.n, nk, Hk, strideHk, tb,
-                       va.gauge_trafo, U, V);
+  hipLaunchKernelGGL(velo_mfma_kernel, dim3(grid), dim3(256), 0, s, va.n, nk, Hk, strideHk, tb,
+                     va.gauge_trafo, U, V);
 }
