@@ -1,27 +1,37 @@
 // mep_bandsum.hip -- Fermi-Dirac weights and the Berry / Kubo / MEP band
-// sums of one batch of k-points, reduced into per-block partial records.
+// sums of one batch of k-points, accumulated into per-workgroup partial
+// records (reduced once per run by reduce_parts_kernel).
 //
-// One k-point per 256-thread workgroup (grid-stride over k; fixed k -> block
-// map and fixed-order in-block reductions, so results are deterministic).
-// V (planar, 3 x n x n) is staged in LDS as three padded column-major
-// planes.  Rows of the reference it replaces:
+// One k-point at a time per 256-thread workgroup (grid-stride over k; fixed
+// k -> workgroup map and fixed-order reductions, so results are
+// deterministic).  Rows of the reference it replaces:
 //   d1_fd_stat, d1_fd_get_N_el          statistics.f90:65-124
 //   kubo_ahc_tens, velo_ahc_tens,
 //   kubo_ohc_tens                       kubo.f90:37-198
 //   kubo_mep_CS / LC / IC               kubo_mep.f90:16-149
 //   mep_niu_CS / LC / IC                mep_niu.f90:20-159
-// The itinerant (IC) triple sums of every band n0 are diagonals of matrix
-// products: sum_{n,m} a_i(n0,n) V_k(n,m) B_l(m,n0) = [A_i (V_k B_l)](n0,n0)
-// with A_i(n0,n) = V_i(n0,n)/(E0-En)^2 and B_l(m,n0) = V_l(m,n0)/(E0-Em);
-// the six products W_kl = V_k B_l (k != l) run on FP64 MFMA.  mep_niu's
-// guard |(E0-En)^2 (E0-Em)| > kubo_tol excludes, for fixed (n0, n), one
-// contiguous window of m around n0 (energies are sorted); those terms are
-// subtracted from W (or the kept ones summed directly when the window
-// covers most bands).
+//
+// Per k the workgroup stages in LDS the three velocity planes V_a (padded
+// column-major) and the antisymmetric reciprocal table
+// r(m, n) = 1 / (E_n - E_m) (0 on the diagonal), so no band sum divides.
+//   pair phase    thread (n0, sub) walks n = sub, sub + S, ...: the Berry
+//                 curvature of n0, the local (LC) MEP terms and, for a single
+//                 (hw, E_F), the velo/ohc pair sums;
+//   itinerant     C_j = V_k B_l - V_l B_k, (j,k,l) cyclic, with
+//                 B_l(m, n0) = V_l(m, n0) r(m, n0): three complex n x n
+//                 products on FP64 MFMA; the IC tensor of band n0 is the
+//                 diagonal -Re[A_i C_j](n0, n0) with
+//                 A_i(n0, n) = V_i(n0, n) r(n, n0)^2, contracted straight
+//                 from the accumulator registers.  mep_niu's guard
+//                 |(E0-En)^2 (E0-Em)| > kubo_tol excludes, for fixed (n0, n),
+//                 one contiguous window of m around n0 (energies are
+//                 sorted); those terms are subtracted from C (or the kept
+//                 ones summed directly when the window covers most bands).
 #include "mep_common.hpp"
 #include "mep_kernels.hpp"
 #include "../../include/mepinterp.h"
-#include "mep_mfma.hpp"
+
+typedef double mep_v4d __attribute__((ext_vector_type(4)));
 
 AccLayout acc_layout(const BandArgs& b) {
   AccLayout L;
@@ -40,327 +50,416 @@ AccLayout acc_layout(const BandArgs& b) {
   return L;
 }
 
-// ordered Cartesian pairs (k,l), k != l, grouped so that for output index j
-// pairs 2j (+1) and 2j+1 (-1) carry eps_jkl
-__device__ constexpr int pk(int p) { return p == 0 ? 1 : p == 1 ? 2 : p == 2 ? 2 : p == 3 ? 0 : p == 4 ? 0 : 1; }
-__device__ constexpr int pl(int p) { return p == 0 ? 2 : p == 1 ? 1 : p == 2 ? 0 : p == 3 ? 2 : p == 4 ? 1 : 0; }
-
-// B_l(m, n0) = V_l(m, n0) / (E_n0 - E_m), zero for m = n0
-struct BScaled {
-  const zd* V;
-  int ld;
-  const double* E;
-  __device__ inline zd operator()(int m, int n0) const {
-    if (m == n0) return zd{0.0, 0.0};
-    double d = E[n0] - E[m];
-    zd v = V[m + ld * n0];
-    return zd{v.re / d, v.im / d};
-  }
-};
-
 constexpr int BS_THREADS = 256;
 
-// LDS plan (doubles): V planes 2*3*PL | W 2*PL | E NMAX | fd nef*NMAX |
-// red RED | windows (int16) 2*NMAX*NMAX/4 | acc L.total
+// LDS plan: V planes (zd) 3*PL | r table PL | E NMAX | fd nef*NMAX |
+// Q NQ*NMAX | red NMAX | acc total      (all doubles after the V planes)
 template <int NMAX>
 struct BsLds {
   static constexpr int LD = NMAX + 1;
   static constexpr int PL = NMAX * LD;
-  static constexpr int RED = 27 * NMAX > 36 * (256 / 9) ? 27 * NMAX : 36 * (256 / 9);
+  static constexpr int NT = NMAX / 16;  // 16-row tiles per matrix
+  // Q rows: the pair phase uses 63 (Omega 9, F3 9, F3 guarded 9, velo/ohc
+  // 36), the itinerant phase 18 per row tile (F2, F2 guarded)
+  static constexpr int NQ = (18 * NT > 63) ? 18 * NT : 63;
   static size_t bytes(int nef, int64_t total) {
-    return sizeof(double) * (2 * 3 * (size_t)PL + 2 * (size_t)PL + NMAX + (size_t)nef * NMAX + RED +
-                             (2 * NMAX * NMAX + 3) / 4 + (size_t)total);
+    return sizeof(zd) * 3 * (size_t)PL +
+           sizeof(double) * ((size_t)PL + NMAX + (size_t)nef * NMAX + (size_t)NQ * NMAX + NMAX +
+                             (size_t)total);
   }
 };
 
 template <int NMAX>
-__global__ __launch_bounds__(BS_THREADS) void bandsum_kernel(BandArgs b, double Tsmear, AccLayout L,
-                                                             int64_t nk,
-                                                             const double* __restrict__ E,
-                                                             const zd* __restrict__ V,
-                                                             const zd* __restrict__ A,
-                                                             const zd* __restrict__ Om,
-                                                             const double* __restrict__ w,
-                                                             double* __restrict__ part) {
-  using S = BsLds<NMAX>;
-  constexpr int LD = S::LD, PL = S::PL;
+__global__ __launch_bounds__(BS_THREADS, 2) void bandsum_kernel(BandArgs b, double Tsmear,
+                                                                AccLayout L, int64_t nk,
+                                                                const double* __restrict__ E,
+                                                                const zd* __restrict__ V,
+                                                                const zd* __restrict__ A,
+                                                                const zd* __restrict__ Om,
+                                                                const double* __restrict__ w,
+                                                                double* __restrict__ part) {
+  using S_ = BsLds<NMAX>;
+  constexpr int LD = S_::LD, PL = S_::PL, NQ = S_::NQ;
+  constexpr int SUB = BS_THREADS / NMAX;  // threads per band in the pair phase
   extern __shared__ double smem[];
   const int n = b.n, nef = b.n_ef, nhw = b.n_hw, nval = b.n_val;
   zd* Vs = reinterpret_cast<zd*>(smem);
-  zd* Ws = Vs + 3 * PL;
-  double* Es = reinterpret_cast<double*>(Ws + PL);
+  double* Rs = reinterpret_cast<double*>(Vs + 3 * PL);
+  double* Es = Rs + PL;
   double* fd = Es + NMAX;
-  double* red = fd + (int64_t)nef * NMAX;
-  short* win = reinterpret_cast<short*>(red + S::RED);  // [n0][nn] lo, hi
-  double* acc = red + S::RED + (2 * NMAX * NMAX + 3) / 4;
+  double* Q = fd + (int64_t)nef * NMAX;
+  double* red = Q + NQ * NMAX;
+  double* acc = red + NMAX;
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   const int n2 = n * n;
   const bool finiteT = b.T_kelvin > 1e-2;
+  const bool doAHC = (b.jobs & MEP_JOB_AHC) != 0;
   const bool doK = (b.jobs & MEP_JOB_KUBO) != 0;
   const bool doNjob = (b.jobs & MEP_JOB_MEP) != 0;
+  const bool pmode = doAHC && (nhw * nef == 1);
+  const double tol = b.kubo_tol;
   for (int64_t c = t; c < L.total; c += BS_THREADS) acc[c] = 0.0;
+
+  const zd wq = zd{b.hw[0], b.eta};
+  const zd w2 = zmul(wq, wq);
+
+  const int p0 = t / SUB, psub = t % SUB;  // pair phase: band p0, sub-lane psub
 
   for (int64_t k = blockIdx.x; k < nk; k += gridDim.x) {
     const double wk = w ? w[k] : 1.0;
     __syncthreads();
+    // ---- stage V (planar [a][i + n j]) and E ------------------------------
     const zd* Vk = V + k * 3 * (int64_t)n2;
     for (int idx = t; idx < 3 * n2; idx += BS_THREADS) {
-      int a = idx / n2, r = idx - a * n2;
+      const int a = idx / n2, r = idx - a * n2;
       Vs[a * PL + r % n + LD * (r / n)] = Vk[idx];
     }
     if (t < n) Es[t] = E[k * n + t];
     __syncthreads();
+    // ---- r(m, c) = 1 / (E_c - E_m); occupations ---------------------------
+    for (int idx = t; idx < n2; idx += BS_THREADS) {
+      const int m = idx % n, c = idx / n;
+      Rs[m + LD * c] = (m == c) ? 0.0 : 1.0 / (Es[c] - Es[m]);
+    }
     for (int idx = t; idx < nef * n; idx += BS_THREADS) {
       if (b.fd_in) {
         fd[idx] = b.fd_in[k * (int64_t)nef * n + idx];
       } else {
-        int ef = idx % nef, nb = idx / nef;
-        double e = Es[nb], f = b.ef[ef];
+        const int ef = idx % nef, nb = idx / nef;
+        const double e = Es[nb], f = b.ef[ef];
         fd[idx] = finiteT ? 1.0 / (1.0 + exp((e - f) / Tsmear)) : ((e < f) ? 1.0 : 0.0);
       }
     }
     __syncthreads();
-    // ---- electron count (serial over bands: fixed order) ---------------
-    if (t < nef) {
-      double s = 0.0;
-      for (int nb = 0; nb < n; ++nb) s += fd[t + nef * nb];
-      acc[L.Ne + t] += wk * s;
-    }
-    // ---- anomalous Hall conductivity, hw -> 0 -----------------------------
-    if (b.jobs & MEP_JOB_AHC) {
-      if (t < 9 * n) {
-        const int nb = t % n, q = t / n, i = q % 3, j = q / 3;
-        double cv = 0.0;
-        for (int np = 0; np < n; ++np) {
-          if (np == nb) continue;
-          double de = Es[np] - Es[nb];
-          zd p = zmul(Vs[i * PL + nb + LD * np], Vs[j * PL + np + LD * nb]);
-          cv -= 2.0 * p.im / (de * de);
-        }
-        red[q * NMAX + nb] = cv;
-      }
-      __syncthreads();
-      if (t < 9 * nef) {
-        const int q = t % 9, ef = t / 9;
-        double s = 0.0;
-        for (int nb = 0; nb < n; ++nb) s -= red[q * NMAX + nb] * fd[ef + nef * nb];
-        acc[L.ahc + q + 9 * ef] += wk * s;
-      }
-      __syncthreads();
-      // ---- hw-resolved Hall (velo_ahc) and optical (kubo_ohc) -------------
-      if (nhw * nef == 1) {
-        // thread (q, g): component q = i + 3j, ordered pairs p = g, g + 28, ...
-        constexpr int NG = BS_THREADS / 9;
-        const int q = t % 9, g = t / 9;
-        const int i = q % 3, j = q / 3;
-        double vre = 0.0, vim_ = 0.0, ore = 0.0, oim = 0.0;
-        if (g < NG) {
-          const zd wq = zd{b.hw[0], b.eta};
-          const zd w2 = zmul(wq, wq);
-          for (int p = g; p < n2; p += NG) {
-            const int x = p % n, y = p / n;
-            if (x == y) continue;
-            const double dE = Es[x] - Es[y];
-            const double dfv = fd[x] - fd[y];
-            const zd denv = zd{dE * dE - w2.re, -w2.im};
-            const zd deno = zmul(zd{dE, 0.0}, zd{dE - b.hw[0], -b.eta});
-            double vim = zmul(Vs[i * PL + x + LD * y], Vs[j * PL + y + LD * x]).im;
-            zd q1 = zdiv(zd{vim, 0.0}, denv);
-            vre += dfv * q1.re;
-            vim_ += dfv * q1.im;
-            zd vo = zmul(Vs[i * PL + y + LD * x], Vs[j * PL + x + LD * y]);
-            zd q2 = zmuli(zscale(zdiv(vo, deno), dfv));
-            ore += q2.re;
-            oim += q2.im;
-          }
-          red[(4 * q + 0) * NG + g] = vre;
-          red[(4 * q + 1) * NG + g] = vim_;
-          red[(4 * q + 2) * NG + g] = ore;
-          red[(4 * q + 3) * NG + g] = oim;
-        }
-        __syncthreads();
-        if (t < 36) {
-          const int qq = t / 4, c = t % 4;
-          double s = 0.0;
-          for (int gg = 0; gg < NG; ++gg) s += red[t * NG + gg];
-          const int64_t dst = (c < 2 ? L.velo : L.ohc) + 2 * qq + (c & 1);
-          acc[dst] += wk * s;
-        }
-      } else {
-        // hw-parallel: thread h owns every accumulator entry of frequency h
-        for (int h = t; h < nhw; h += BS_THREADS) {
-          const zd wq = zd{b.hw[h], b.eta};
-          const zd w2 = zmul(wq, wq);
-          for (int x = 0; x < n; ++x)
-            for (int y = 0; y < n; ++y) {
-              if (x == y) continue;
-              const double dE = Es[x] - Es[y];
-              const zd denv = zd{dE * dE - w2.re, -w2.im};
-              const zd deno = zmul(zd{dE, 0.0}, zd{dE - b.hw[h], -b.eta});
+
+    // ==== pair phase =======================================================
+    {
+      double om[9], f3[9], f3m[9];
 #pragma unroll
-              for (int j = 0; j < 3; ++j)
-#pragma unroll
-                for (int i = 0; i < 3; ++i) {
-                  const int q = i + 3 * j;
-                  double vim = zmul(Vs[i * PL + x + LD * y], Vs[j * PL + y + LD * x]).im;
-                  zd vo = zmul(Vs[i * PL + y + LD * x], Vs[j * PL + x + LD * y]);
-                  zd q1 = zdiv(zd{vim, 0.0}, denv);
-                  zd q2o = zdiv(vo, deno);
-                  for (int ef = 0; ef < nef; ++ef) {
-                    const double dfv = fd[ef + nef * x] - fd[ef + nef * y];
-                    double* av = acc + L.velo + 2 * 9 * (h + (int64_t)nhw * ef);
-                    double* ao = acc + L.ohc + 2 * 9 * (h + (int64_t)nhw * ef);
-                    av[2 * q] += wk * (dfv * q1.re);
-                    av[2 * q + 1] += wk * (dfv * q1.im);
-                    zd q2 = zmuli(zscale(q2o, dfv));
-                    ao[2 * q] += wk * q2.re;
-                    ao[2 * q + 1] += wk * q2.im;
-                  }
-                }
-            }
-        }
-      }
-      __syncthreads();
-    }
-    // ---- magnetoelectric polarisability ----------------------------------
-    if (doK || doNjob) {
-      // exclusion windows of mep_niu's guard |(E0-En)^2 (E0-Em)| > kubo_tol:
-      // energies are sorted, so for fixed (n0, n) the excluded m form one
-      // contiguous index range around n0
-      if (doNjob) {
-        for (int idx = t; idx < nval * n; idx += BS_THREADS) {
-          const int n0 = idx % nval, nn = idx / nval;
-          const double E0 = Es[n0], dn = E0 - Es[nn];
-          const double dn2 = dn * dn;
-          int lo = n0, hi = n0;
-          while (lo > 0 && !(fabs(dn2 * (E0 - Es[lo - 1])) > b.kubo_tol)) --lo;
-          while (hi < n - 1 && !(fabs(dn2 * (E0 - Es[hi + 1])) > b.kubo_tol)) ++hi;
-          win[2 * (n0 + NMAX * nn)] = (short)lo;
-          win[2 * (n0 + NMAX * nn) + 1] = (short)hi;
-        }
-      }
-      const int n0 = t % n, ii = t / n;  // thread (n0, i) for t < 3n
-      const bool act = t < 3 * n;
-      const bool doN = doNjob && act && n0 < nval;
-      const double E0 = act ? Es[n0] : 0.0;
-      // F2_ij = -sum_kl eps_jkl Re T_ikl: pair p contributes to j = p/2 with
-      // sign -1 (even p) or +1 (odd p); LC likewise with the opposite sign
-      double F2[3] = {0.0, 0.0, 0.0}, F2m[3] = {0.0, 0.0, 0.0};
-      double F3[3] = {0.0, 0.0, 0.0}, F3m[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-      for (int p = 0; p < 6; ++p) {
-        const double sg = (p & 1) ? 1.0 : -1.0;
-        __syncthreads();
-        // W = V_k B_l  (itinerant sum of all n0 at once)
-        zgemm_tiles<false>(n, Vs + pk(p) * PL, LD, BScaled{Vs + pl(p) * PL, LD, Es}, Ws, LD);
-        __syncthreads();
-        if (act) {
-          const zd* Vi = Vs + ii * PL;
-          const zd* Vk_ = Vs + pk(p) * PL;
-          const BScaled bl{Vs + pl(p) * PL, LD, Es};
-          double tr = 0.0, trm = 0.0;
-          for (int nn = 0; nn < n; ++nn) {
-            if (nn == n0) continue;
-            const double dn = E0 - Es[nn];
-            const double dn2 = dn * dn;
-            const zd vi = Vi[n0 + LD * nn];
-            const zd a = zd{vi.re / dn2, vi.im / dn2};
-            const zd wnm = Ws[nn + LD * n0];
-            tr += a.re * wnm.re - a.im * wnm.im;
-            if (doN) {
-              const int lo = win[2 * (n0 + NMAX * nn)], hi = win[2 * (n0 + NMAX * nn) + 1];
-              if (lo == 0 && hi == n - 1) continue;  // every m excluded
-              zd kept = zd{0.0, 0.0};
-              if (2 * (hi - lo + 1) > n) {
-                for (int m = 0; m < lo; ++m) kept = zadd(kept, zmul(Vk_[nn + LD * m], bl(m, n0)));
-                for (int m = hi + 1; m < n; ++m) kept = zadd(kept, zmul(Vk_[nn + LD * m], bl(m, n0)));
-              } else {
-                zd ex = zd{0.0, 0.0};
-                for (int m = lo; m <= hi; ++m) ex = zadd(ex, zmul(Vk_[nn + LD * m], bl(m, n0)));
-                kept = zsub(wnm, ex);
-              }
-              trm += a.re * kept.re - a.im * kept.im;
-            }
-          }
-          F2[p / 2] += sg * tr;
-          F2m[p / 2] += sg * trm;
-        }
-      }
-      // local (LC) terms
+      for (int q = 0; q < 9; ++q) om[q] = f3[q] = f3m[q] = 0.0;
+      const bool act = p0 < n;
+      const bool doN = doNjob && act && p0 < nval;
       if (act) {
+        const double E0 = Es[p0];
         zd z0[3];
 #pragma unroll
-        for (int l = 0; l < 3; ++l) z0[l] = Vs[l * PL + n0 + LD * n0];
-        for (int nn = 0; nn < n; ++nn) {
-          if (nn == n0) continue;
-          const double dn = E0 - Es[nn];
-          const double den3 = dn * dn * dn;
-          const bool keep3 = doN && (fabs(den3) > b.kubo_tol);
-          const zd xi = Vs[ii * PL + n0 + LD * nn];
-          zd yk[3];
+        for (int l = 0; l < 3; ++l) z0[l] = Vs[l * PL + p0 + LD * p0];
+        for (int nn = psub; nn < n; nn += SUB) {
+          if (nn == p0) continue;
+          zd x[3], y[3];
 #pragma unroll
-          for (int q = 0; q < 3; ++q) yk[q] = Vs[q * PL + nn + LD * n0];
+          for (int a = 0; a < 3; ++a) {
+            x[a] = Vs[a * PL + p0 + LD * nn];  // V_a(n0, n)
+            y[a] = Vs[a * PL + nn + LD * p0];  // V_a(n, n0)
+          }
+          const double r = Rs[nn + LD * p0];  // 1 / (E0 - En)
+          const double r2 = r * r, r3 = r2 * r;
+          if (doAHC) {
+            // Omega_n0,ij = -2 Im(V_i(n0,n) V_j(n,n0)) / (En - E0)^2
 #pragma unroll
-          for (int p = 0; p < 6; ++p) {
-            const double r = zmul(xi, zmul(yk[pk(p)], z0[pl(p)])).re / den3;
-            const double sg = (p & 1) ? -1.0 : 1.0;
-            F3[p / 2] += sg * r;
-            if (keep3) F3m[p / 2] += sg * r;
+            for (int j = 0; j < 3; ++j)
+#pragma unroll
+              for (int i = 0; i < 3; ++i) om[i + 3 * j] -= 2.0 * zmul(x[i], y[j]).im * r2;
+          }
+          if (doK || doN) {
+            // F3_ij = eps_jkl Re(V_i(n0,n) V_k(n,n0) V_l(n0,n0)) / (E0-En)^3
+            zd cr[3];
+            cr[0] = zsub(zmul(y[1], z0[2]), zmul(y[2], z0[1]));
+            cr[1] = zsub(zmul(y[2], z0[0]), zmul(y[0], z0[2]));
+            cr[2] = zsub(zmul(y[0], z0[1]), zmul(y[1], z0[0]));
+            const double dn = E0 - Es[nn];
+            const bool keep3 = doN && (fabs(dn * dn * dn) > tol);
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+#pragma unroll
+              for (int i = 0; i < 3; ++i) {
+                const double v = zmul(x[i], cr[j]).re * r3;
+                f3[i + 3 * j] += v;
+                if (keep3) f3m[i + 3 * j] += v;
+              }
+          }
+        }
+      }
+      // reduce over the SUB lanes of band p0 (consecutive lanes of one wave)
+#pragma unroll
+      for (int q = 0; q < 9; ++q) {
+#pragma unroll
+        for (int o = 1; o < SUB; o <<= 1) {
+          om[q] += __shfl_xor(om[q], o, 64);
+          f3[q] += __shfl_xor(f3[q], o, 64);
+          f3m[q] += __shfl_xor(f3m[q], o, 64);
+        }
+      }
+      if (act && psub == 0) {
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+          Q[q * NMAX + p0] = om[q];
+          Q[(9 + q) * NMAX + p0] = f3[q];
+          Q[(18 + q) * NMAX + p0] = f3m[q];
+        }
+      }
+    }
+    if (pmode) {
+      // ordered pairs (x = n0, y = n) of velo_ahc_tens / kubo_ohc_tens for
+      // the single (hw, E_F); per-band sums go to Q rows 27..62
+      double pv[36];
+#pragma unroll
+      for (int q = 0; q < 36; ++q) pv[q] = 0.0;
+      if (p0 < n) {
+        const double E0 = Es[p0];
+        for (int nn = psub; nn < n; nn += SUB) {
+          if (nn == p0) continue;
+          zd x[3], y[3];
+#pragma unroll
+          for (int a = 0; a < 3; ++a) {
+            x[a] = Vs[a * PL + p0 + LD * nn];
+            y[a] = Vs[a * PL + nn + LD * p0];
+          }
+          const double dE = E0 - Es[nn];
+          const double dfv = fd[p0] - fd[nn];
+          const zd denv = zd{dE * dE - w2.re, -w2.im};
+          const zd deno = zmul(zd{dE, 0.0}, zd{dE - wq.re, -wq.im});
+          const double iv = 1.0 / zabs2(denv), io = 1.0 / zabs2(deno);
+          const zd rv = zd{denv.re * iv * dfv, -denv.im * iv * dfv};  // dfv / denv
+          const zd ro = zd{deno.re * io * dfv, -deno.im * io * dfv};  // dfv / deno
+#pragma unroll
+          for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+              const int q = i + 3 * j;
+              const double vim = zmul(x[i], y[j]).im;
+              pv[2 * q] += vim * rv.re;
+              pv[2 * q + 1] += vim * rv.im;
+              const zd o = zmuli(zmul(zmul(y[i], x[j]), ro));
+              pv[18 + 2 * q] += o.re;
+              pv[18 + 2 * q + 1] += o.im;
+            }
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < 36; ++q) {
+#pragma unroll
+        for (int o = 1; o < SUB; o <<= 1) pv[q] += __shfl_xor(pv[q], o, 64);
+      }
+      if (p0 < n && psub == 0) {
+#pragma unroll
+        for (int q = 0; q < 36; ++q) Q[(27 + q) * NMAX + p0] = pv[q];
+      }
+    }
+    __syncthreads();
+    // ---- per-k sums of the pair phase --------------------------------------
+    for (int ef = t; ef < nef; ef += BS_THREADS) {
+      double s = 0.0;
+      for (int nb = 0; nb < n; ++nb) s += fd[ef + nef * nb];
+      acc[L.Ne + ef] += wk * s;
+    }
+    for (int it = t; it < 18 * nef; it += BS_THREADS) {
+      const int q = it % 9, h = (it / 9) % 2, ef = it / 18;
+      if (h == 0 && doAHC) {
+        double s = 0.0;
+        for (int nb = 0; nb < n; ++nb) s -= Q[q * NMAX + nb] * fd[ef + nef * nb];
+        acc[L.ahc + q + 9 * ef] += wk * s;
+      }
+      if (h == 1 && doK) {
+        double s = 0.0;
+        for (int nb = 0; nb < n; ++nb) s += Q[(9 + q) * NMAX + nb] * fd[ef + nef * nb];
+        acc[L.klc + q + 9 * ef] += wk * s;
+      }
+    }
+    if (doNjob)
+      for (int it = t; it < 9 * nval; it += BS_THREADS) {
+        const int q = it % 9, n0 = it / 9;
+        acc[L.nlc + q + 9 * n0] += wk * Q[(18 + q) * NMAX + n0];
+      }
+    if (pmode && t < 36) {
+      double s = 0.0;
+      for (int nb = 0; nb < n; ++nb) s += Q[(27 + t) * NMAX + nb];
+      acc[(t < 18 ? L.velo + t : L.ohc + (t - 18))] += wk * s;
+    }
+    // ---- hw-resolved Hall and optical conductivity for several (hw, E_F) ---
+    if (doAHC && !pmode) {
+      // thread h owns every accumulator entry of frequency h
+      for (int h = t; h < nhw; h += BS_THREADS) {
+        const zd wh = zd{b.hw[h], b.eta};
+        const zd wh2 = zmul(wh, wh);
+        for (int x = 0; x < n; ++x)
+          for (int y = 0; y < n; ++y) {
+            if (x == y) continue;
+            const double dE = Es[x] - Es[y];
+            const zd denv = zd{dE * dE - wh2.re, -wh2.im};
+            const zd deno = zmul(zd{dE, 0.0}, zd{dE - wh.re, -wh.im});
+            const double iv = 1.0 / zabs2(denv), io = 1.0 / zabs2(deno);
+            const zd rv = zd{denv.re * iv, -denv.im * iv};
+            const zd ro = zd{deno.re * io, -deno.im * io};
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+#pragma unroll
+              for (int i = 0; i < 3; ++i) {
+                const int q = i + 3 * j;
+                const double vim = zmul(Vs[i * PL + x + LD * y], Vs[j * PL + y + LD * x]).im;
+                const zd q2o = zmul(zmul(Vs[i * PL + y + LD * x], Vs[j * PL + x + LD * y]), ro);
+                for (int ef = 0; ef < nef; ++ef) {
+                  const double dfv = wk * (fd[ef + nef * x] - fd[ef + nef * y]);
+                  double* av = acc + L.velo + 2 * 9 * (h + (int64_t)nhw * ef);
+                  double* ao = acc + L.ohc + 2 * 9 * (h + (int64_t)nhw * ef);
+                  av[2 * q] += vim * rv.re * dfv;
+                  av[2 * q + 1] += vim * rv.im * dfv;
+                  const zd q2 = zmuli(zscale(q2o, dfv));
+                  ao[2 * q] += q2.re;
+                  ao[2 * q + 1] += q2.im;
+                }
+              }
+          }
+      }
+    }
+
+    // ==== itinerant (IC) terms on MFMA ======================================
+    if (doK || doNjob) {
+      __syncthreads();  // the pair-phase sums above have read Q
+      const int nt = (n + 15) / 16;
+      const int ntile = 3 * nt * nt;
+      for (int tile = wv; tile < ntile; tile += 4) {
+        const int j = tile / (nt * nt), tr = (tile / nt) % nt, tc = tile % nt;
+        const int kk_ = (j + 1) % 3, ll_ = (j + 2) % 3;
+        const zd* Vk_ = Vs + kk_ * PL;
+        const zd* Vl_ = Vs + ll_ * PL;
+        const int r0 = 16 * tr, c0 = 16 * tc;
+        const int ar = r0 + (lane & 15), bc = c0 + (lane & 15), kq = lane >> 4;
+        mep_v4d re0 = {0, 0, 0, 0}, im0 = {0, 0, 0, 0}, re1 = {0, 0, 0, 0}, im1 = {0, 0, 0, 0};
+        for (int k0 = 0; k0 < n; k0 += 4) {
+          const int m = k0 + kq;
+          zd a0 = zd{0.0, 0.0}, a1 = zd{0.0, 0.0}, b0 = zd{0.0, 0.0}, b1 = zd{0.0, 0.0};
+          if (m < n) {
+            if (ar < n) {
+              a0 = Vk_[ar + LD * m];
+              a1 = Vl_[ar + LD * m];
+            }
+            if (bc < n) {
+              const double rr = Rs[m + LD * bc];
+              b0 = zscale(Vl_[m + LD * bc], rr);
+              b1 = zscale(Vk_[m + LD * bc], -rr);
+            }
+          }
+          re0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0.re, b0.re, re0, 0, 0, 0);
+          im0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0.re, b0.im, im0, 0, 0, 0);
+          re1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1.re, b1.re, re1, 0, 0, 0);
+          im1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1.re, b1.im, im1, 0, 0, 0);
+          re0 = __builtin_amdgcn_mfma_f64_16x16x4f64(-a0.im, b0.im, re0, 0, 0, 0);
+          im0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0.im, b0.re, im0, 0, 0, 0);
+          re1 = __builtin_amdgcn_mfma_f64_16x16x4f64(-a1.im, b1.im, re1, 0, 0, 0);
+          im1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1.im, b1.re, im1, 0, 0, 0);
+        }
+        // contraction with A_i(n0, n) = V_i(n0, n) r(n, n0)^2, n0 = column
+        const int n0 = bc;
+        const bool colok = n0 < n;
+        const bool doN = doNjob && colok && n0 < nval;
+        double f2[3] = {0.0, 0.0, 0.0}, f2m[3] = {0.0, 0.0, 0.0};
+        const double E0 = colok ? Es[n0] : 0.0;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int row = r0 + kq + 4 * v;
+          if (!colok || row >= n || row == n0) continue;
+          const zd cv = zd{re0[v] + re1[v], im0[v] + im1[v]};
+          const double rr = Rs[row + LD * n0];
+          const double rr2 = rr * rr;
+          zd ai[3];
+#pragma unroll
+          for (int i = 0; i < 3; ++i) {
+            ai[i] = zscale(Vs[i * PL + n0 + LD * row], rr2);
+            f2[i] -= ai[i].re * cv.re - ai[i].im * cv.im;
+          }
+          if (doN) {
+            const double dn = E0 - Es[row];
+            const double dn2 = dn * dn;
+            int lo = n0, hi = n0;
+            while (lo > 0 && !(fabs(dn2 * (E0 - Es[lo - 1])) > tol)) --lo;
+            while (hi < n - 1 && !(fabs(dn2 * (E0 - Es[hi + 1])) > tol)) ++hi;
+            if (lo == 0 && hi == n - 1) continue;  // every m excluded
+            zd kept = zd{0.0, 0.0};
+            auto term = [&](int mm) {
+              const double rm = Rs[mm + LD * n0];
+              const zd bl = zscale(Vl_[mm + LD * n0], rm), bk = zscale(Vk_[mm + LD * n0], rm);
+              return zsub(zmul(Vk_[row + LD * mm], bl), zmul(Vl_[row + LD * mm], bk));
+            };
+            if (2 * (hi - lo + 1) > n) {
+              for (int mm = 0; mm < lo; ++mm) kept = zadd(kept, term(mm));
+              for (int mm = hi + 1; mm < n; ++mm) kept = zadd(kept, term(mm));
+            } else {
+              zd ex = zd{0.0, 0.0};
+              for (int mm = lo; mm <= hi; ++mm) ex = zadd(ex, term(mm));
+              kept = zsub(cv, ex);
+            }
+#pragma unroll
+            for (int i = 0; i < 3; ++i) f2m[i] -= ai[i].re * kept.re - ai[i].im * kept.im;
+          }
+        }
+        // sum the 4 row groups of the tile (lanes l, l^16, l^32, l^48)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          f2[i] += __shfl_xor(f2[i], 16, 64);
+          f2[i] += __shfl_xor(f2[i], 32, 64);
+          f2m[i] += __shfl_xor(f2m[i], 16, 64);
+          f2m[i] += __shfl_xor(f2m[i], 32, 64);
+        }
+        if (kq == 0 && colok) {
+#pragma unroll
+          for (int i = 0; i < 3; ++i) {
+            Q[(tr * 18 + i + 3 * j) * NMAX + n0] = f2[i];
+            Q[(tr * 18 + 9 + i + 3 * j) * NMAX + n0] = f2m[i];
           }
         }
       }
       __syncthreads();
-      if (doK) {
-        if (act)
-          for (int j = 0; j < 3; ++j) {
-            red[(ii + 3 * j) * NMAX + n0] = F2[j];
-            red[(9 + ii + 3 * j) * NMAX + n0] = F3[j];
+      if (doK)
+        for (int it = t; it < 9 * nef; it += BS_THREADS) {
+          const int q = it % 9, ef = it / 9;
+          double s = 0.0;
+          for (int nb = 0; nb < n; ++nb) {
+            double f2 = 0.0;
+            for (int tr = 0; tr < nt; ++tr) f2 += Q[(tr * 18 + q) * NMAX + nb];
+            s += f2 * fd[ef + nef * nb];
           }
-        __syncthreads();
-        if (t < 18 * nef) {
-          const int q = t % 18, ef = t / 18;
+          acc[L.kic + q + 9 * ef] += wk * s;
+        }
+      if (doNjob)
+        for (int it = t; it < 9 * nval; it += BS_THREADS) {
+          const int q = it % 9, n0 = it / 9;
           double s = 0.0;
-          for (int m0 = 0; m0 < n; ++m0) s += red[q * NMAX + m0] * fd[ef + nef * m0];
-          if (q < 9) acc[L.kic + q + 9 * ef] += wk * s;
-          else acc[L.klc + (q - 9) + 9 * ef] += wk * s;
+          for (int tr = 0; tr < nt; ++tr) s += Q[(tr * 18 + 9 + q) * NMAX + n0];
+          acc[L.nic + q + 9 * n0] += wk * s;
         }
-      }
-      if (doN) {
-        for (int j = 0; j < 3; ++j) {
-          acc[L.nic + ii + 3 * j + 9 * n0] += wk * F2m[j];
-          acc[L.nlc + ii + 3 * j + 9 * n0] += wk * F3m[j];
-        }
-      }
-      // ---- Chern-Simons term (needs A and Omega) --------------------------
-      if (b.pos_op && A && Om) {
-        __syncthreads();
-        if (t < n) {
-          const int64_t NN = (int64_t)n2;
-          const zd* Ak = A + k * 3 * NN + 3 * ((int64_t)t + (int64_t)n * t);
-          const zd* Ok = Om + k * 9 * NN + 9 * ((int64_t)t + (int64_t)n * t);
-          zd om[3];
-          om[0] = zsub(Ok[1 + 3 * 2], Ok[2 + 3 * 1]);
-          om[1] = zsub(Ok[2 + 3 * 0], Ok[0 + 3 * 2]);
-          om[2] = zsub(Ok[0 + 3 * 1], Ok[1 + 3 * 0]);
-          zd dp = zd{0.0, 0.0};
+    }
+    // ---- Chern-Simons term (needs A and Omega) ------------------------------
+    if ((doK || doNjob) && b.pos_op && A && Om) {
+      __syncthreads();
+      if (t < n) {
+        const int64_t NN = (int64_t)n2;
+        const zd* Ak = A + k * 3 * NN + 3 * ((int64_t)t + (int64_t)n * t);
+        const zd* Ok = Om + k * 9 * NN + 9 * ((int64_t)t + (int64_t)n * t);
+        zd omc[3];
+        omc[0] = zsub(Ok[1 + 3 * 2], Ok[2 + 3 * 1]);
+        omc[1] = zsub(Ok[2 + 3 * 0], Ok[0 + 3 * 2]);
+        omc[2] = zsub(Ok[0 + 3 * 1], Ok[1 + 3 * 0]);
+        zd dp = zd{0.0, 0.0};
 #pragma unroll
-          for (int a = 0; a < 3; ++a) dp = zadd(dp, zcmul(Ak[a], om[a]));
-          red[t] = dp.re;
-        }
-        __syncthreads();
-        if (doK && t < nef) {
-          double s = 0.0;
-          for (int m0 = 0; m0 < n; ++m0) s += 0.5 * fd[t + nef * m0] * red[m0];
-          for (int i = 0; i < 3; ++i) acc[L.kcs + i + 3 * i + 9 * t] += wk * s;
-        }
-        if (doNjob && t < nval && t < n)
-          for (int i = 0; i < 3; ++i) acc[L.ncs + i + 3 * i + 9 * t] += wk * (0.5 * red[t]);
+        for (int a = 0; a < 3; ++a) dp = zadd(dp, zcmul(Ak[a], omc[a]));
+        red[t] = dp.re;
       }
+      __syncthreads();
+      if (doK)
+        for (int ef = t; ef < nef; ef += BS_THREADS) {
+          double s = 0.0;
+          for (int m0 = 0; m0 < n; ++m0) s += 0.5 * fd[ef + nef * m0] * red[m0];
+          for (int i = 0; i < 3; ++i) acc[L.kcs + i + 3 * i + 9 * ef] += wk * s;
+        }
+      if (doNjob)
+        for (int n0 = t; n0 < nval && n0 < n; n0 += BS_THREADS)
+          for (int i = 0; i < 3; ++i) acc[L.ncs + i + 3 * i + 9 * n0] += wk * (0.5 * red[n0]);
     }
   }
   __syncthreads();
   double* out = part + (int64_t)blockIdx.x * L.total;
-  for (int64_t c = t; c < L.total; c += BS_THREADS) out[c] = acc[c];
+  for (int64_t c = t; c < L.total; c += BS_THREADS) out[c] += acc[c];
 }
 
 __global__ void reduce_parts_kernel(int nblocks, int64_t len, const double* __restrict__ part,

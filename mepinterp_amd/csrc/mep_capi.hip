@@ -425,6 +425,7 @@ static int band_setup(mep_ctx* c, const mep_run_params* p, int64_t kblock, BandS
     c->acc_cap = bs.L.total;
   }
   launch_fill_zero(c->st, c->dacc, bs.L.total);
+  launch_fill_zero(c->st, c->dpart, need);
   HIPCHK(hipGetLastError());
   return MEP_OK;
 }
@@ -434,13 +435,17 @@ static int band_block(mep_ctx* c, BandSetup& bs, int64_t nb, const double* w) {
   timed(c, K_BANDSUM, [&] {
     launch_bandsum(c->st, bs.b, nb, c->dE, c->dV, c->dA, c->dOm, w, nblk, c->dpart);
   });
-  timed(c, K_REDUCE, [&] { launch_reduce_parts(c->st, nblk, bs.L.total, c->dpart, c->dacc); });
   HIPCHK(hipGetLastError());
   return MEP_OK;
 }
 
 static int unpack(mep_ctx* c, const BandSetup& bs, const mep_run_params* p, int64_t nk,
                   mep_sums* out) {
+  // the partial records accumulate over every k-block of the run
+  timed(c, K_REDUCE, [&] {
+    launch_reduce_parts(c->st, bs.nblocks, bs.L.total, c->dpart, c->dacc);
+  });
+  HIPCHK(hipGetLastError());
   std::vector<double> h(bs.L.total);
   HIPCHK(hipMemcpyAsync(h.data(), c->dacc, sizeof(double) * bs.L.total, hipMemcpyDeviceToHost,
                         c->st));

@@ -190,6 +190,23 @@ def test_run_mesh_matches_oracle(name):
     _compare_sums(g, o, rel=1e-8, keys=keys, atol=atol)
 
 
+@pytest.mark.parametrize("n,nef,nhw", [(20, 1, 1), (32, 1, 1), (29, 17, 3)])
+def test_run_mesh_large_bands_matches_oracle(n, nef, nhw):
+    """n_wf > 16 (32-band kernel variants), a single (hw, E_F) pair-sum path
+    and more E_F points than one pass of the workgroup covers."""
+    import oracle as O
+    from mepinterp_amd import capi, models
+    m = models.make_random(seed=40 + n, n=n, L=1)
+    mp = (4, 3, 3)
+    jobs = capi.MEP_JOB_AHC | capi.MEP_JOB_KUBO | capi.MEP_JOB_MEP
+    kw = dict(n_eF=nef, eF_min=-0.02, eF_max=0.02, n_hw=nhw, hw_min=0.0, hw_max=0.1,
+              T_kelvin=300.0, eta=0.004, valence_bands=n // 2)
+    with _engine(m) as eng:
+        g = eng.run_mesh(mp, _params(jobs, **kw))
+    o = _oracle_sums(oracle_model(m), O.mp_mesh(mp), jobs, **kw)
+    _compare_sums(g, o, rel=1e-8)
+
+
 def test_run_kpts_weights_and_blocks():
     """Weighted explicit k list, processed in several k-blocks."""
     from mepinterp_amd import capi, models
