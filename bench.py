@@ -29,6 +29,12 @@ AU_TO_EV = 27.211385
 PEAK_FP64_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
 PEAK_HBM_GBS = 8000.0
 
+# HBM bytes per ft_gemm_f64_kernel launch of this workload (14,563 k on
+# average), from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of this script,
+# scaled by the counters' readings for a plain 1 GiB copy (x2.0 / x1.0):
+# 1.842 GB read + 0.954 GB written (profiles/kspace_n32_64cubed_fp64.md).
+FT_GEMM_TRAFFIC_BYTES = 1.842e9 + 0.954e9
+
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
@@ -115,6 +121,22 @@ def main():
     n_mat = 4
     flops_per_k = 8.0 * nR * n_mat * n * n
     achieved = flops_per_k * nk / (gemm_ms * 1e-3) / 1e12  # TFLOP/s over all launches
+    # the other stages against their own bounds (algorithmic work per k)
+    stages = {}
+    velo_ms = split["velo"][0]
+    stages["velo"] = {"bound": "mfma", "flop_per_k": 48.0 * n ** 3,
+                      "achieved_tflops": 48.0 * n ** 3 * nk / (velo_ms * 1e-3) / 1e12}
+    bs_ms = split["bandsum"][0]
+    bs_bytes = 48.0 * n * n + 8.0 * n
+    stages["bandsum"] = {"bound": "mfma", "flop_per_k": 48.0 * n ** 3,
+                         "achieved_tflops": 48.0 * n ** 3 * nk / (bs_ms * 1e-3) / 1e12,
+                         "hbm_bytes_per_k": bs_bytes,
+                         "achieved_gbs": bs_bytes * nk / (bs_ms * 1e-3) / 1e9,
+                         "peak_gbs": PEAK_HBM_GBS}
+    stages["eigh"] = {"bound": "latency", "ms": split["eigh"][0]}
+    for v in stages.values():
+        if "achieved_tflops" in v:
+            v["frac_mfma"] = v["achieved_tflops"] / PEAK_FP64_TFLOPS
     out = host.finalize(sums, p, mp, m["a_latt"])
     log(f"[bench] sigma_xy = {out['ahc_tens'][0, 1]:.10e}  alpha_Niu_xx = {out['mep_tens'][0, 0]:.6e}")
 
@@ -144,9 +166,11 @@ def main():
                    "n_wf": n, "n_R": nR, "mp_grid": list(mp), "n_k": nk,
                    "kernel_ms": {k: v[0] for k, v in split.items()}},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
+                     "traffic": FT_GEMM_TRAFFIC_BYTES,
                      "kernel": "ft_gemm_f64_kernel",
-                     "launch_ms": gemm_ms / max(gemm_cnt, 1)},
+                     "launch_ms": gemm_ms / max(gemm_cnt, 1),
+                     "stages": stages},
         "cpu_baseline": cpu,
     }
     print(json.dumps(res), flush=True)
