@@ -29,11 +29,11 @@ AU_TO_EV = 27.211385
 PEAK_FP64_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
 PEAK_HBM_GBS = 8000.0
 
-# HBM bytes per ft_gemm_f64_kernel launch of this workload (14,563 k on
-# average), from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of this script,
-# scaled by the counters' readings for a plain 1 GiB copy (x2.0 / x1.0):
-# 1.842 GB read + 0.954 GB written (profiles/kspace_n32_64cubed_fp64.md).
-FT_GEMM_TRAFFIC_BYTES = 1.842e9 + 0.954e9
+# HBM bytes per k-point moved by ft_gemm_f64_kernel on this workload, from
+# rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of this script scaled by the
+# counters' readings for a plain 1 GiB copy (x2.0 / x1.0): 1.842 GB read +
+# 0.954 GB written per launch of 14,563 k (profiles/kspace_n32_64cubed_fp64.md).
+FT_GEMM_TRAFFIC_BYTES_PER_K = (1.842e9 + 0.954e9) / 14563.0
 
 
 def log(*a):
@@ -167,7 +167,7 @@ def main():
                    "kernel_ms": {k: v[0] for k, v in split.items()}},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
-                     "traffic": FT_GEMM_TRAFFIC_BYTES,
+                     "traffic": FT_GEMM_TRAFFIC_BYTES_PER_K * nk / max(gemm_cnt, 1),
                      "kernel": "ft_gemm_f64_kernel",
                      "launch_ms": gemm_ms / max(gemm_cnt, 1),
                      "stages": stages},

@@ -145,8 +145,15 @@ static int64_t bytes_per_k(const mep_ctx* c) {
 }
 
 static int ensure_ws(mep_ctx* c, int64_t nk_total) {
-  int64_t want = c->kblock_req > 0 ? c->kblock_req : (int64_t)(3.0e9 / (double)bytes_per_k(c));
+  // default: about 16 GB of work buffers per k-block (HBM holds 288 GB),
+  // split evenly over the blocks of this run
+  int64_t want = c->kblock_req > 0 ? c->kblock_req : (int64_t)(16.0e9 / (double)bytes_per_k(c));
+  if (want < 1) want = 1;
   if (want > nk_total) want = nk_total;
+  if (c->kblock_req <= 0 && want < nk_total) {
+    const int64_t nblk = (nk_total + want - 1) / want;
+    want = (nk_total + nblk - 1) / nblk;
+  }
   want = ((want + 63) / 64) * 64;
   if (want < 64) want = 64;
   if (want <= c->kcap) return MEP_OK;

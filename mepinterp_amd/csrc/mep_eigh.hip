@@ -32,9 +32,10 @@ __device__ inline double team_sum(double v) {
   return v;
 }
 
-// side record per k-point: d[n] e[n] tau[n] ph[2n] (doubles)
-__host__ __device__ inline int64_t side_len(int n) { return 5 * (int64_t)n; }
-// rotation record per k-point (pairs of doubles): capacity
+// side record per k-point (doubles): d[n] e[n] tau[n] ph[2n] count
+__host__ __device__ inline int64_t side_len(int n) { return 5 * (int64_t)n + 1; }
+// rotation record per k-point: capacity in rotations; each rotation is a
+// (c, s) pair of doubles plus its column index (one byte)
 __host__ __device__ inline int64_t rot_cap(int n) { return 4 * (int64_t)n * n + 4 * (int64_t)n + 8; }
 
 template <int NMAX>
@@ -158,13 +159,15 @@ __global__ __launch_bounds__(64) void eigh_house_kernel(int n, int64_t nk,
 }
 
 // One lane per k-point: implicit QL on the real tridiagonal (d, e) held in
-// LDS columns; rotations are appended to rot[k]; ascending eigenvalues go to
-// E.  rot record: per sweep a header (ilow, m) followed by the m - ilow
-// pairs (c_i, s_i) for i = m-1 down to ilow; a header with m < 0 ends it.
+// LDS columns.  Every Givens rotation G(i, i+1; c, s) of every sweep is
+// appended, in application order, to the k-point's flat stream
+// rot[k][q] = (c, s), col[k][q] = i; the count goes to the side record.
+// Ascending eigenvalues go to E, each band's rank to the side record.
 template <int NMAX>
 __global__ __launch_bounds__(64) void eigh_ql_kernel(int n, int64_t nk,
                                                      double* __restrict__ side,
                                                      double* __restrict__ rot,
+                                                     uint8_t* __restrict__ rcol,
                                                      double* __restrict__ E,
                                                      int* __restrict__ info) {
   __shared__ double ds[NMAX * 64], es[NMAX * 64];
@@ -180,6 +183,7 @@ __global__ __launch_bounds__(64) void eigh_ql_kernel(int n, int64_t nk,
         es[i * 64 + lane] = sd[n + i];
       }
       double* rk = rot + k * 2 * cap;
+      uint8_t* ck = rcol + k * cap;
       int64_t pos = 0;
       int fail = 0;
       for (int l = 0; l < n && !fail; ++l) {
@@ -191,7 +195,7 @@ __global__ __launch_bounds__(64) void eigh_ql_kernel(int n, int64_t nk,
             if (fabs(es[m * 64 + lane]) <= EPS * dd) break;
           }
           if (m == l) break;
-          if (++iter > 60 || pos + 2 + (m - l) >= cap) {
+          if (++iter > 60 || pos + (m - l) > cap) {
             fail = 1;
             break;
           }
@@ -200,44 +204,44 @@ __global__ __launch_bounds__(64) void eigh_ql_kernel(int n, int64_t nk,
           double r = sqrt(g * g + 1.0);
           g = ds[m * 64 + lane] - dl + el / (g + copysign(r, g));
           double s = 1.0, c = 1.0, p = 0.0;
-          const int64_t hdr = pos;
-          pos += 1;
-          int ilow = l;
           bool underflow = false;
+          double ei = es[(m - 1) * 64 + lane], di = ds[(m - 1) * 64 + lane];
+          double di1 = ds[m * 64 + lane];
           for (int i = m - 1; i >= l; --i) {
-            const double ei = es[i * 64 + lane];
+            // operands of the next step are not touched by this one
+            const double ein = (i > l) ? es[(i - 1) * 64 + lane] : 0.0;
+            const double din = (i > l) ? ds[(i - 1) * 64 + lane] : 0.0;
             const double f = s * ei, b = c * ei;
             r = sqrt(f * f + g * g);
             es[(i + 1) * 64 + lane] = r;
             if (r == 0.0) {
-              ds[(i + 1) * 64 + lane] -= p;
+              ds[(i + 1) * 64 + lane] = di1 - p;
               es[m * 64 + lane] = 0.0;
               underflow = true;
-              ilow = i + 1;
               break;
             }
             const double rr = 1.0 / r;
             s = f * rr;
             c = g * rr;
-            g = ds[(i + 1) * 64 + lane] - p;
-            r = (ds[i * 64 + lane] - g) * s + 2.0 * c * b;
+            g = di1 - p;
+            r = (di - g) * s + 2.0 * c * b;
             p = s * r;
             ds[(i + 1) * 64 + lane] = g + p;
             g = c * r - b;
             rk[2 * pos] = c;
             rk[2 * pos + 1] = s;
+            ck[pos] = (uint8_t)i;
             pos += 1;
+            ei = ein;
+            di1 = di;
+            di = din;
           }
-          rk[2 * hdr] = (double)ilow;
-          rk[2 * hdr + 1] = (double)m;
           if (underflow) continue;
           ds[l * 64 + lane] -= p;
           es[l * 64 + lane] = g;
           es[m * 64 + lane] = 0.0;
         }
       }
-      rk[2 * pos] = 0.0;
-      rk[2 * pos + 1] = -1.0;
       if (fail) atomicOr(info, 1);
       // ascending order: rank of each eigenvalue (ties by index)
       for (int j = 0; j < n; ++j) {
@@ -252,69 +256,114 @@ __global__ __launch_bounds__(64) void eigh_ql_kernel(int n, int64_t nk,
       }
       double* sdw = side + k * side_len(n);
       for (int j = 0; j < n; ++j) sdw[n + j] = es[j * 64 + lane];  // rank over e's slot
+      sdw[5 * n] = (double)pos;
     }
   }
 }
 
-// A team per k-point: replay the rotations on Z (LDS), then U = Q P Z.
+// A team per k-point.  Z starts as the identity in LDS (lane r <-> row r);
+// the rotation stream is staged through LDS one team-width chunk at a time,
+// the next chunk's loads in flight while the current one is applied.  Then
+// U = Q P Z: lane j takes column j of P Z into registers, the packed
+// reflectors are staged into the Z space, and applied in reverse order.
 template <int NMAX>
 __global__ __launch_bounds__(64) void eigh_vec_kernel(int n, int64_t nk,
                                                       const double* __restrict__ side,
                                                       const double* __restrict__ rot,
+                                                      const uint8_t* __restrict__ rcol,
                                                       zd* __restrict__ U) {
   constexpr int TEAMS = 64 / NMAX;
   constexpr int LDZ = NMAX + 1;
-  __shared__ double Zt_s[TEAMS][NMAX * LDZ];  // Zt[col*LDZ + row]
+  __shared__ double Zt_s[TEAMS][NMAX * LDZ];  // Zt[col*LDZ + row]; later the reflectors
+  __shared__ double cs_s[TEAMS][2 * NMAX];
+  __shared__ int ix_s[TEAMS][NMAX];
   const int lane = threadIdx.x;
   const int tm = lane / NMAX, t = lane % NMAX;
   double* Zt = Zt_s[tm];
+  double* csb = cs_s[tm];
+  int* ixb = ix_s[tm];
   const int64_t cap = rot_cap(n);
   for (int64_t kb = (int64_t)blockIdx.x * TEAMS; kb < nk; kb += (int64_t)gridDim.x * TEAMS) {
     const int64_t k = kb + tm;
     const bool valid = k < nk;
-    for (int idx = t; idx < NMAX * LDZ; idx += NMAX) Zt[idx] = ((idx / LDZ) == (idx % LDZ)) ? 1.0 : 0.0;
-    __syncthreads();
-    if (valid && t < n) {
-      const double* rk = rot + k * 2 * cap;
-      int64_t pos = 0;
-      for (;;) {
-        const int ilow = (int)rk[2 * pos];
-        const int m = (int)rk[2 * pos + 1];
-        pos += 1;
-        if (m < 0) break;
-        // rotation i mixes columns (i, i+1); carry column i between steps
-        double carry = Zt[m * LDZ + t];
-        for (int i = m - 1; i >= ilow; --i) {
-          const double c = rk[2 * pos], s = rk[2 * pos + 1];
-          pos += 1;
-          const double zi = Zt[i * LDZ + t];
-          Zt[(i + 1) * LDZ + t] = s * zi + c * carry;
-          carry = c * zi - s * carry;
-        }
-        Zt[ilow * LDZ + t] = carry;
+    const double* sd = side + (valid ? k : 0) * side_len(n);
+#pragma unroll
+    for (int c = 0; c < NMAX; ++c) Zt[c * LDZ + t] = (c == t) ? 1.0 : 0.0;
+    const int64_t Q = valid ? (int64_t)sd[5 * n] : 0;
+    const double* rk = rot + (valid ? k : 0) * 2 * cap;
+    const uint8_t* ck = rcol + (valid ? k : 0) * cap;
+    double nc = 0.0, ns = 0.0;
+    int ni = 0;
+    if (t < Q) {
+      nc = rk[2 * t];
+      ns = rk[2 * t + 1];
+      ni = ck[t];
+    }
+    int prev = -2;
+    double carry = 0.0;
+    for (int64_t q0 = 0; q0 < Q; q0 += NMAX) {
+      __builtin_amdgcn_wave_barrier();
+      csb[2 * t] = nc;
+      csb[2 * t + 1] = ns;
+      ixb[t] = ni;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      const int64_t qn = q0 + NMAX + t;
+      if (qn < Q) {
+        nc = rk[2 * qn];
+        ns = rk[2 * qn + 1];
+        ni = ck[qn];
+      }
+      const int cnt = (Q - q0 < NMAX) ? (int)(Q - q0) : NMAX;
+      for (int r = 0; r < cnt; ++r) {
+        const double c = csb[2 * r], s = csb[2 * r + 1];
+        const int i = ixb[r];
+        const double zi = Zt[i * LDZ + t];
+        const double zi1 = (prev == i + 1) ? carry : Zt[(i + 1) * LDZ + t];
+        Zt[(i + 1) * LDZ + t] = s * zi + c * zi1;
+        carry = c * zi - s * zi1;
+        Zt[i * LDZ + t] = carry;
+        prev = i;
       }
     }
-    __syncthreads();
-    // ---- back-transformation (lane j <-> column j) -----------------------
-    const double* sd = side + (valid ? k : 0) * side_len(n);
-    const zd* Uk = U + (valid ? k : 0) * (int64_t)n * n;  // reflectors
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // ---- lane j <-> column j of P Z ----------------------------------------
     zd W[NMAX];
 #pragma unroll
     for (int i = 0; i < NMAX; ++i)
       W[i] = (valid && i < n && t < n)
                  ? zscale(zd{sd[3 * n + 2 * i], sd[3 * n + 2 * i + 1]}, Zt[t * LDZ + i])
                  : zd{0.0, 0.0};
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // ---- packed reflectors (strict lower triangle) into the Z space ---------
+    zd* Vp = reinterpret_cast<zd*>(Zt);
+    const zd* Uk = U + (valid ? k : 0) * (int64_t)n * n;
+    if (valid)
+      for (int kc = 0; kc + 2 < n; ++kc) {
+        const int off = kc * (n - 1) - (kc * (kc - 1)) / 2;
+        const int r = kc + 1 + t;
+        if (r < n) Vp[off + t] = Uk[r + (int64_t)n * kc];
+      }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     if (valid) {
       for (int kc = n - 3; kc >= 0; --kc) {
+        const int off = kc * (n - 1) - (kc * (kc - 1)) / 2 - (kc + 1);  // Vp[off + i], i > kc
         const double tk = sd[2 * n + kc];
         zd sacc = zd{0.0, 0.0};
 #pragma unroll
         for (int i = 0; i < NMAX; ++i)
-          if (i > kc && i < n) sacc = zadd(sacc, zcmul(Uk[i + (int64_t)n * kc], W[i]));
+          if (i > kc && i < n) sacc = zadd(sacc, zcmul(Vp[off + i], W[i]));
         sacc = zscale(sacc, tk);
 #pragma unroll
         for (int i = 0; i < NMAX; ++i)
-          if (i > kc && i < n) W[i] = zsub(W[i], zmul(Uk[i + (int64_t)n * kc], sacc));
+          if (i > kc && i < n) W[i] = zsub(W[i], zmul(Vp[off + i], sacc));
       }
     }
     // gauge convention: largest-modulus component (first on ties) real > 0
@@ -337,25 +386,27 @@ __global__ __launch_bounds__(64) void eigh_vec_kernel(int n, int64_t nk,
       }
     }
     const int rnk = (valid && t < n) ? (int)sd[n + t] : 0;
-    __syncthreads();  // every lane has read the reflectors before U is written
     if (valid && t < n) {
       zd* Uo = U + k * (int64_t)n * n + (int64_t)n * rnk;
 #pragma unroll
       for (int i = 0; i < NMAX; ++i)
         if (i < n) Uo[i] = W[i];
     }
-    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
 }
 
 size_t eigh_scratch_bytes(int n, int64_t nk) {
-  return sizeof(double) * (size_t)nk * (size_t)(side_len(n) + 2 * rot_cap(n));
+  return (size_t)nk * (sizeof(double) * (size_t)(side_len(n) + 2 * rot_cap(n)) + (size_t)rot_cap(n));
 }
 
 void launch_eigh(hipStream_t s, int n, int64_t nk, const zd* Hk, int64_t strideHk, const KGeom& g,
                  double* E, zd* U, int* info, double* scratch) {
   double* side = scratch;
   double* rot = scratch + nk * side_len(n);
+  uint8_t* rcol = reinterpret_cast<uint8_t*>(rot + nk * 2 * rot_cap(n));
   const int teams = n <= 8 ? 8 : n <= 16 ? 4 : 2;
   int64_t blocks = (nk + teams - 1) / teams;
   unsigned grid = (unsigned)(blocks < 65536 ? blocks : 65536);
@@ -364,8 +415,9 @@ void launch_eigh(hipStream_t s, int n, int64_t nk, const zd* Hk, int64_t strideH
 #define MEP_EIGH_LAUNCH(NM)                                                                      \
   hipLaunchKernelGGL(eigh_house_kernel<NM>, dim3(grid), dim3(64), 0, s, n, nk, Hk, strideHk,     \
                      g.kpts, g.tau, U, side);                                                    \
-  hipLaunchKernelGGL(eigh_ql_kernel<NM>, dim3(qgrid), dim3(64), 0, s, n, nk, side, rot, E, info); \
-  hipLaunchKernelGGL(eigh_vec_kernel<NM>, dim3(grid), dim3(64), 0, s, n, nk, side, rot, U);
+  hipLaunchKernelGGL(eigh_ql_kernel<NM>, dim3(qgrid), dim3(64), 0, s, n, nk, side, rot, rcol, E,     \
+                     info);                                                                      \
+  hipLaunchKernelGGL(eigh_vec_kernel<NM>, dim3(grid), dim3(64), 0, s, n, nk, side, rot, rcol, U);
   if (n <= 8) {
     MEP_EIGH_LAUNCH(8)
   } else if (n <= 16) {

@@ -18,6 +18,15 @@
  *   o(i,j,ef)        -> o[i + 3*(j + 3*ef)]
  *   o(i,j,hw,ef)     -> o[i + 3*(j + 3*(hw + nHW*ef))]
  * Energies are Hartree, lengths Bohr, as inside the reference.
+ *
+ * Parity unpinned: the position-operator gauge transformation in the
+ * reference's evaluation order (wann_interp.f90:502-658: stale-temporary
+ * connection, A^(H) before get_velo, curvature from A^(H)), the Chern-Simons
+ * terms built on it (kubo_mep_CS, mep_niu_CS) and V_nn with a position
+ * operator (never assigned by the reference) are restated from the source
+ * only; no fixture or identity fixes their values.  Every other function is
+ * checked against LAPACK, finite differences, closed forms, the FeMn
+ * fixtures or a second independent formula (tests/test_oracle_*.py).
  */
 #ifndef MEP_ORACLE_H
 #define MEP_ORACLE_H
