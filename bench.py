@@ -29,11 +29,12 @@ AU_TO_EV = 27.211385
 PEAK_FP64_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
 PEAK_HBM_GBS = 8000.0
 
-# HBM bytes per k-point moved by ft_gemm_f64_kernel on this workload, from
+# HBM bytes per k-point moved by the kernels on this workload, from
 # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of this script scaled by the
-# counters' readings for a plain 1 GiB copy (x2.0 / x1.0): 1.842 GB read +
-# 0.954 GB written per launch of 14,563 k (profiles/kspace_n32_64cubed_fp64.md).
+# counters' readings for a plain 1 GiB copy (x2.0 / x1.0)
+# (profiles/kspace_n32_64cubed_fp64.md).
 FT_GEMM_TRAFFIC_BYTES_PER_K = (1.842e9 + 0.954e9) / 14563.0
+BANDSUM_TRAFFIC_BYTES_PER_K = (0.724e9 + 0.004e9) / 14563.0
 
 
 def log(*a):
@@ -117,26 +118,36 @@ def main():
         split[name] = (ms, cnt)
         log(f"[bench] {name:8s} total {ms:9.3f} ms over {cnt} launches")
     eng.set_profiling(False)
-    gemm_ms, gemm_cnt = split["ft_gemm"]
-    n_mat = 4
-    flops_per_k = 8.0 * nR * n_mat * n * n
-    achieved = flops_per_k * nk / (gemm_ms * 1e-3) / 1e12  # TFLOP/s over all launches
-    # the other stages against their own bounds (algorithmic work per k)
+    # per-stage rooflines (algorithmic work per k of the algorithm each kernel runs)
+    launches = max(split["bandsum"][1], 1)
+    k_per_launch = nk / launches
+    n_up = n * (n + 1) // 2
+    K_ft = nR  # R paired with -R: cos rows + sin rows + the R = 0 row
+    gemm_flop_k = 2.0 * K_ft * 2 * 4 * n_up  # real GEMM over (re, im) of 4 upper triangles
+    survey_flop_k = 8.0 * nR * 4 * n * n       # complex full-matrix count (8 NR n_mat n^2)
+    bs_flop_k = 48.0 * n ** 3                   # 3 products C_j = V_k B_l - V_l B_k (2 each)
+    bs_bytes_k = 48.0 * n * n + 8.0 * n         # V (c128) + E (f64) read once
+    velo_flop_k = 48.0 * n ** 3                 # U^H (dH_a U) for a = x, y, z
     stages = {}
-    velo_ms = split["velo"][0]
-    stages["velo"] = {"bound": "mfma", "flop_per_k": 48.0 * n ** 3,
-                      "achieved_tflops": 48.0 * n ** 3 * nk / (velo_ms * 1e-3) / 1e12}
-    bs_ms = split["bandsum"][0]
-    bs_bytes = 48.0 * n * n + 8.0 * n
-    stages["bandsum"] = {"bound": "mfma", "flop_per_k": 48.0 * n ** 3,
-                         "achieved_tflops": 48.0 * n ** 3 * nk / (bs_ms * 1e-3) / 1e12,
-                         "hbm_bytes_per_k": bs_bytes,
-                         "achieved_gbs": bs_bytes * nk / (bs_ms * 1e-3) / 1e9,
-                         "peak_gbs": PEAK_HBM_GBS}
+    for name, fl in (("ft_gemm", gemm_flop_k), ("velo", velo_flop_k), ("bandsum", bs_flop_k)):
+        ms, cnt = split[name]
+        ach = fl * nk / (ms * 1e-3) / 1e12
+        stages[name] = {"bound": "mfma", "flop_per_k": fl, "achieved_tflops": ach,
+                        "frac": ach / PEAK_FP64_TFLOPS, "launch_ms": ms / max(cnt, 1)}
+    stages["ft_gemm"]["survey_flop_per_k"] = survey_flop_k
+    stages["ft_gemm"]["traffic_per_launch"] = FT_GEMM_TRAFFIC_BYTES_PER_K * k_per_launch
+    stages["bandsum"]["hbm_bytes_per_k"] = bs_bytes_k
+    stages["bandsum"]["achieved_gbs"] = bs_bytes_k * nk / (split["bandsum"][0] * 1e-3) / 1e9
+    stages["bandsum"]["traffic_per_launch"] = BANDSUM_TRAFFIC_BYTES_PER_K * k_per_launch
     stages["eigh"] = {"bound": "latency", "ms": split["eigh"][0]}
-    for v in stages.values():
-        if "achieved_tflops" in v:
-            v["frac_mfma"] = v["achieved_tflops"] / PEAK_FP64_TFLOPS
+    # the dominant single kernel (eigh is three kernels)
+    dom = max(("ft_gemm", "velo", "bandsum"), key=lambda nm: split[nm][0])
+    d = stages[dom]
+    roofline = {"bound": "mfma", "achieved": d["achieved_tflops"], "peak": PEAK_FP64_TFLOPS,
+                "unit": "TFLOP/s", "frac": d["frac"], "traffic": d.get("traffic_per_launch"),
+                "kernel": {"ft_gemm": "ft_gemm_f64_kernel", "velo": "velo_mfma_kernel",
+                           "bandsum": "bandsum_kernel<32>"}[dom],
+                "launch_ms": d["launch_ms"], "stages": stages}
     out = host.finalize(sums, p, mp, m["a_latt"])
     log(f"[bench] sigma_xy = {out['ahc_tens'][0, 1]:.10e}  alpha_Niu_xx = {out['mep_tens'][0, 0]:.6e}")
 
@@ -165,12 +176,7 @@ def main():
                                "E_F=0, FP64 path",
                    "n_wf": n, "n_R": nR, "mp_grid": list(mp), "n_k": nk,
                    "kernel_ms": {k: v[0] for k, v in split.items()}},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
-                     "traffic": FT_GEMM_TRAFFIC_BYTES_PER_K * nk / max(gemm_cnt, 1),
-                     "kernel": "ft_gemm_f64_kernel",
-                     "launch_ms": gemm_ms / max(gemm_cnt, 1),
-                     "stages": stages},
+        "roofline": roofline,
         "cpu_baseline": cpu,
     }
     print(json.dumps(res), flush=True)

@@ -1,6 +1,7 @@
 // mep_capi.hip -- the C-ABI of include/mepinterp.h: model upload, k-block
 // workspace, the per-block pipeline (phase -> Fourier GEMM -> eigensolve ->
 // velocity -> band sums -> reduction) and kernel timing.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -49,15 +50,20 @@ struct mep_ctx {
   double kubo_tol = 0;
   double cmat[9];
   int nmat = 0;
-  int64_t ncols = 0;
-  double* dR = nullptr;
+  int packed = 0;       // upper triangles only (no position operator)
+  int64_t np = 0;       // complex elements stored per matrix
+  int64_t ncols = 0;    // complex columns of the GEMM output per k
+  int64_t hk_per_k = 0; // complex elements of the Hk slot per k
+  int Kh = 0, has_zero = 0;
+  int64_t K = 0;        // GEMM depth: 2 Kh + has_zero
+  double* dR = nullptr;  // (3, Kh) R of the paired entries
   double* dtau = nullptr;
-  zd* dX = nullptr;
+  double* dX = nullptr;  // (K, 2 ncols) real coefficients
   // k-block workspace
   int64_t kcap = 0, kblock_req = 0;
   double* dk = nullptr;
   double* dw = nullptr;
-  zd* dP = nullptr;
+  double* dP = nullptr;  // (K, kblock) real phase tile
   zd* dHk = nullptr;
   double* dE = nullptr;
   zd* dU = nullptr;
@@ -131,14 +137,14 @@ static void free_ws(mep_ctx* c) {
   hipFree(c->dk); hipFree(c->dw); hipFree(c->dP); hipFree(c->dHk); hipFree(c->dE);
   hipFree(c->dU); hipFree(c->dV); hipFree(c->dA); hipFree(c->dOm); hipFree(c->dscr);
   c->dscr = nullptr;
-  c->dk = c->dw = nullptr; c->dP = c->dHk = c->dU = c->dV = c->dA = c->dOm = nullptr;
+  c->dk = c->dw = c->dP = nullptr; c->dHk = c->dU = c->dV = c->dA = c->dOm = nullptr;
   c->dE = nullptr;
   c->kcap = 0;
 }
 
 static int64_t bytes_per_k(const mep_ctx* c) {
   int64_t n2 = c->n * c->n;
-  int64_t b = 3 * 8 + 8 + c->nR * 16 + (int64_t)c->nmat * n2 * 16 + c->n * 8 + n2 * 16 + 3 * n2 * 16;
+  int64_t b = 3 * 8 + 8 + c->K * 8 + c->hk_per_k * 16 + c->n * 8 + n2 * 16 + 3 * n2 * 16;
   if (c->pos) b += 12 * n2 * 16;
   b += (int64_t)eigh_scratch_bytes((int)c->n, 1);
   return b;
@@ -161,8 +167,8 @@ static int ensure_ws(mep_ctx* c, int64_t nk_total) {
   int64_t n2 = c->n * c->n;
   HIPCHK(hipMalloc(&c->dk, sizeof(double) * 3 * want));
   HIPCHK(hipMalloc(&c->dw, sizeof(double) * want));
-  HIPCHK(hipMalloc(&c->dP, sizeof(zd) * c->nR * want));
-  HIPCHK(hipMalloc(&c->dHk, sizeof(zd) * c->nmat * n2 * want));
+  HIPCHK(hipMalloc(&c->dP, sizeof(double) * c->K * want));
+  HIPCHK(hipMalloc(&c->dHk, sizeof(zd) * c->hk_per_k * want));
   HIPCHK(hipMalloc(&c->dE, sizeof(double) * c->n * want));
   HIPCHK(hipMalloc(&c->dU, sizeof(zd) * n2 * want));
   HIPCHK(hipMalloc(&c->dV, sizeof(zd) * 3 * n2 * want));
@@ -213,33 +219,102 @@ extern "C" int mep_ctx_create(int64_t n_wf, int64_t n_R, const double* R_frac,
     for (int j = 0; j < 3; ++j)
       c->cmat[3 * i + j] = (flags & MEP_F_CART_VELO) ? a_latt[i + 3 * j] : (i == j ? two_pi : 0.0);
   c->nmat = c->pos ? 10 : 4;
+  c->packed = c->pos ? 0 : 1;
   const int64_t n = n_wf, n2 = n * n;
-  c->ncols = c->nmat * n2;
+  c->np = c->packed ? n * (n + 1) / 2 : n2;
+  c->ncols = c->nmat * c->np;
+  // the Hk slot also serves as staging for V in the reference layout
+  c->hk_per_k = c->ncols > 3 * n2 ? c->ncols : 3 * n2;
 
-  // X(R): columns [H | i c_a H | r_a | i (c_a r_b - c_b r_a)]
-  std::vector<zd> X((size_t)(n_R * c->ncols));
+  // pair R with -R (exact match); R = 0 entries form the constant row
+  std::vector<int64_t> partner(n_R, -1);
+  std::vector<char> is_zero(n_R, 0);
+  for (int64_t s = 0; s < n_R; ++s)
+    is_zero[s] = (R_frac[3 * s] == 0.0 && R_frac[3 * s + 1] == 0.0 && R_frac[3 * s + 2] == 0.0);
+  for (int64_t s = 0; s < n_R; ++s) {
+    if (is_zero[s] || partner[s] >= 0) continue;
+    for (int64_t t = s + 1; t < n_R; ++t)
+      if (!is_zero[t] && partner[t] < 0 && R_frac[3 * t] == -R_frac[3 * s] &&
+          R_frac[3 * t + 1] == -R_frac[3 * s + 1] && R_frac[3 * t + 2] == -R_frac[3 * s + 2]) {
+        partner[s] = t;
+        partner[t] = s;
+        break;
+      }
+  }
+  std::vector<int64_t> ent;  // entries: unpaired R, or the first of a pair
+  bool anyzero = false;
+  for (int64_t s = 0; s < n_R; ++s) {
+    if (is_zero[s]) {
+      anyzero = true;
+      continue;
+    }
+    if (partner[s] < 0 || s < partner[s]) ent.push_back(s);
+  }
+  c->Kh = (int)ent.size();
+  c->has_zero = anyzero ? 1 : 0;
+  c->K = 2 * (int64_t)c->Kh + c->has_zero;
+
+  // X_s: complex coefficients of R_s, columns [H | i c_a H | r_a |
+  // i (c_a r_b - c_b r_a)], element (i, j) at p(i, j)
   const zd* H = reinterpret_cast<const zd*>(H_real);
   const zd* r = reinterpret_cast<const zd*>(r_real);
-  for (int64_t s = 0; s < n_R; ++s) {
+  auto build_x = [&](int64_t s, std::vector<zd>& Xs) {
     double cv[3];
     for (int a = 0; a < 3; ++a)
       cv[a] = c->cmat[3 * a] * R_frac[3 * s] + c->cmat[3 * a + 1] * R_frac[3 * s + 1] +
               c->cmat[3 * a + 2] * R_frac[3 * s + 2];
     double sc = (deg && (flags & MEP_F_APPLY_DEG)) ? 1.0 / (double)deg[s] : 1.0;
-    zd* Xs = &X[(size_t)(s * c->ncols)];
     for (int64_t o = 0; o < n2; ++o) {
+      const int64_t i = o % n, j = o / n;
+      if (c->packed && i > j) continue;
+      const int64_t p = c->packed ? i + j * (j + 1) / 2 : o;
       zd h = zscale(H[o + n2 * s], sc);
-      Xs[o] = h;
-      for (int a = 0; a < 3; ++a) Xs[(1 + a) * n2 + o] = zmuli(zscale(h, cv[a]));
+      Xs[p] = h;
+      for (int a = 0; a < 3; ++a) Xs[(1 + a) * c->np + p] = zmuli(zscale(h, cv[a]));
       if (c->pos) {
         const zd* rr = &r[3 * (o + n2 * s)];
         zd ra[3] = {zscale(rr[0], sc), zscale(rr[1], sc), zscale(rr[2], sc)};
-        for (int a = 0; a < 3; ++a) Xs[(4 + a) * n2 + o] = ra[a];
+        for (int a = 0; a < 3; ++a) Xs[(4 + a) * c->np + p] = ra[a];
         const int AA[3] = {1, 2, 0}, BB[3] = {2, 0, 1};
         for (int q = 0; q < 3; ++q) {
           int a = AA[q], b = BB[q];
-          Xs[(7 + q) * n2 + o] = zmuli(zsub(zscale(ra[b], cv[a]), zscale(ra[a], cv[b])));
+          Xs[(7 + q) * c->np + p] = zmuli(zsub(zscale(ra[b], cv[a]), zscale(ra[a], cv[b])));
         }
+      }
+    }
+  };
+  // real rows: cos row e: X_s + X_t, sin row Kh + e: i (X_s - X_t), ones row
+  // (R = 0): sum of the R = 0 coefficients; (re, im) interleaved columns
+  const int64_t NC = c->ncols;
+  std::vector<double> X((size_t)(c->K * 2 * NC), 0.0);
+  std::vector<double> Rh((size_t)(3 * (c->Kh > 0 ? c->Kh : 1)), 0.0);
+  std::vector<zd> xs(NC), xt(NC);
+  for (int e = 0; e < c->Kh; ++e) {
+    const int64_t s = ent[e], t = partner[s];
+    for (int a = 0; a < 3; ++a) Rh[3 * e + a] = R_frac[3 * s + a];
+    std::fill(xs.begin(), xs.end(), zd{0.0, 0.0});
+    std::fill(xt.begin(), xt.end(), zd{0.0, 0.0});
+    build_x(s, xs);
+    if (t >= 0) build_x(t, xt);
+    double* rc = &X[(size_t)(e * 2 * NC)];
+    double* rs = &X[(size_t)((c->Kh + e) * 2 * NC)];
+    for (int64_t q = 0; q < NC; ++q) {
+      const zd sum = zadd(xs[q], xt[q]), dif = zmuli(zsub(xs[q], xt[q]));
+      rc[2 * q] = sum.re;
+      rc[2 * q + 1] = sum.im;
+      rs[2 * q] = dif.re;
+      rs[2 * q + 1] = dif.im;
+    }
+  }
+  if (c->has_zero) {
+    double* r0 = &X[(size_t)(2 * c->Kh * 2 * NC)];
+    for (int64_t s = 0; s < n_R; ++s) {
+      if (!is_zero[s]) continue;
+      std::fill(xs.begin(), xs.end(), zd{0.0, 0.0});
+      build_x(s, xs);
+      for (int64_t q = 0; q < NC; ++q) {
+        r0[2 * q] += xs[q].re;
+        r0[2 * q + 1] += xs[q].im;
       }
     }
   }
@@ -250,16 +325,16 @@ extern "C" int mep_ctx_create(int64_t n_wf, int64_t n_R, const double* R_frac,
   };
   hipError_t e;
   if ((e = hipStreamCreate(&c->st)) != hipSuccess) return bad(e, "hipStreamCreate");
-  if ((e = hipMalloc(&c->dR, sizeof(double) * 3 * n_R)) != hipSuccess) return bad(e, "hipMalloc R");
-  if ((e = hipMemcpy(c->dR, R_frac, sizeof(double) * 3 * n_R, hipMemcpyHostToDevice)) != hipSuccess)
+  if ((e = hipMalloc(&c->dR, sizeof(double) * Rh.size())) != hipSuccess) return bad(e, "hipMalloc R");
+  if ((e = hipMemcpy(c->dR, Rh.data(), sizeof(double) * Rh.size(), hipMemcpyHostToDevice)) != hipSuccess)
     return bad(e, "upload R");
   if (c->tb) {
     if ((e = hipMalloc(&c->dtau, sizeof(double) * 3 * n)) != hipSuccess) return bad(e, "hipMalloc tau");
     if ((e = hipMemcpy(c->dtau, wf_centers, sizeof(double) * 3 * n, hipMemcpyHostToDevice)) != hipSuccess)
       return bad(e, "upload tau");
   }
-  if ((e = hipMalloc(&c->dX, sizeof(zd) * X.size())) != hipSuccess) return bad(e, "hipMalloc X");
-  if ((e = hipMemcpy(c->dX, X.data(), sizeof(zd) * X.size(), hipMemcpyHostToDevice)) != hipSuccess)
+  if ((e = hipMalloc(&c->dX, sizeof(double) * X.size())) != hipSuccess) return bad(e, "hipMalloc X");
+  if ((e = hipMemcpy(c->dX, X.data(), sizeof(double) * X.size(), hipMemcpyHostToDevice)) != hipSuccess)
     return bad(e, "upload X");
   if ((e = hipMalloc(&c->dinfo, sizeof(int))) != hipSuccess) return bad(e, "hipMalloc info");
   if ((e = hipMemset(c->dinfo, 0, sizeof(int))) != hipSuccess) return bad(e, "memset info");
@@ -330,24 +405,27 @@ static KGeom geom(const mep_ctx* c) {
 
 static int interp_block(mep_ctx* c, int64_t nb, bool velo) {
   const int64_t n = c->n, n2 = n * n;
-  timed(c, K_PHASE, [&] { launch_phase(c->st, nb, (int)c->nR, c->dk, c->dR, c->dP); });
+  timed(c, K_PHASE, [&] { launch_phase(c->st, nb, c->Kh, c->has_zero, c->dk, c->dR, c->dP); });
   timed(c, K_GEMM, [&] {
-    launch_ft_gemm_f64(c->st, (int)nb, (int)c->ncols, (int)c->nR, c->dP, c->dX, c->dHk);
+    launch_ft_gemm_f64(c->st, (int)nb, (int)(2 * c->ncols), (int)c->K, c->dP, c->dX,
+                       reinterpret_cast<double*>(c->dHk));
   });
   KGeom g = geom(c);
   timed(c, K_EIGH, [&] {
-    launch_eigh(c->st, (int)n, nb, c->dHk, c->nmat * n2, g, c->dE, c->dU, c->dinfo, c->dscr);
+    launch_eigh(c->st, (int)n, nb, c->dHk, c->ncols, c->packed, g, c->dE, c->dU, c->dinfo,
+                c->dscr);
   });
   if (velo) {
     VeloArgs va;
     va.n = (int)n;
     va.nmat = c->nmat;
+    va.packed = c->packed;
     va.gauge_trafo = (c->flags & MEP_F_GAUGE_TRAFO) ? 1 : 0;
     va.pos_op = c->pos ? 1 : 0;
     va.textbook = (c->flags & MEP_F_TEXTBOOK) ? 1 : 0;
     va.kubo_tol = c->kubo_tol;
     timed(c, K_VELO, [&] {
-      launch_velo(c->st, va, nb, c->dHk, c->nmat * n2, g, c->dU, c->dE, c->dV, c->dA, c->dOm);
+      launch_velo(c->st, va, nb, c->dHk, c->ncols, g, c->dU, c->dE, c->dV, c->dA, c->dOm);
     });
   }
   HIPCHK(hipGetLastError());

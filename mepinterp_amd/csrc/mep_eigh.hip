@@ -41,6 +41,7 @@ __host__ __device__ inline int64_t rot_cap(int n) { return 4 * (int64_t)n * n + 
 template <int NMAX>
 __global__ __launch_bounds__(64) void eigh_house_kernel(int n, int64_t nk,
                                                         const zd* __restrict__ Hk, int64_t strideHk,
+                                                        int packed,
                                                         const double* __restrict__ kpts,
                                                         const double* __restrict__ tau,
                                                         zd* __restrict__ U,
@@ -64,7 +65,8 @@ __global__ __launch_bounds__(64) void eigh_house_kernel(int n, int64_t nk,
     for (int idx = t; idx < n * n; idx += NMAX) {
       int i = idx % n, j = idx / n;
       if (i <= j) {
-        zd h = valid ? Hk[k * strideHk + i + (int64_t)n * j] : zd{0.0, 0.0};
+        const int64_t o = packed ? i + ((int64_t)j * (j + 1)) / 2 : i + (int64_t)n * j;
+        zd h = valid ? Hk[k * strideHk + o] : zd{0.0, 0.0};
         if (valid && tau && i != j) {
           const double kx = kpts[3 * k], ky = kpts[3 * k + 1], kz = kpts[3 * k + 2];
           double ang = two_pi * (kx * (tau[3 * j] - tau[3 * i]) + ky * (tau[3 * j + 1] - tau[3 * i + 1]) +
@@ -402,8 +404,8 @@ size_t eigh_scratch_bytes(int n, int64_t nk) {
   return (size_t)nk * (sizeof(double) * (size_t)(side_len(n) + 2 * rot_cap(n)) + (size_t)rot_cap(n));
 }
 
-void launch_eigh(hipStream_t s, int n, int64_t nk, const zd* Hk, int64_t strideHk, const KGeom& g,
-                 double* E, zd* U, int* info, double* scratch) {
+void launch_eigh(hipStream_t s, int n, int64_t nk, const zd* Hk, int64_t strideHk, int packed,
+                 const KGeom& g, double* E, zd* U, int* info, double* scratch) {
   double* side = scratch;
   double* rot = scratch + nk * side_len(n);
   uint8_t* rcol = reinterpret_cast<uint8_t*>(rot + nk * 2 * rot_cap(n));
@@ -414,7 +416,7 @@ void launch_eigh(hipStream_t s, int n, int64_t nk, const zd* Hk, int64_t strideH
   unsigned qgrid = (unsigned)(qb < 65536 ? qb : 65536);
 #define MEP_EIGH_LAUNCH(NM)                                                                      \
   hipLaunchKernelGGL(eigh_house_kernel<NM>, dim3(grid), dim3(64), 0, s, n, nk, Hk, strideHk,     \
-                     g.kpts, g.tau, U, side);                                                    \
+                     packed, g.kpts, g.tau, U, side);                                            \
   hipLaunchKernelGGL(eigh_ql_kernel<NM>, dim3(qgrid), dim3(64), 0, s, n, nk, side, rot, rcol, E,     \
                      info);                                                                      \
   hipLaunchKernelGGL(eigh_vec_kernel<NM>, dim3(grid), dim3(64), 0, s, n, nk, side, rot, rcol, U);

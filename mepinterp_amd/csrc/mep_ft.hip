@@ -1,5 +1,5 @@
 // mep_ft.hip -- Fourier sums H(k), dH(k), A(k), Omega(k) as one batched
-// complex GEMM  C[k, j] = sum_R P[k, R] * X[R, j]  on gfx950 MFMA.
+// real GEMM on gfx950 FP64 MFMA.
 //
 // Replaces the per-k R-loop of FT_R_to_k (wann_interp.f90:151-384).  The
 // k-independent factors i*c_a(R) (velocity) and i*(c_a r_b - c_b r_a)
@@ -7,10 +7,19 @@
 // tight-binding phase e^{2 pi i k.(tau_m - tau_n)} is a per-element epilogue
 // applied by the consumers (mep_eigh.hip, mep_velo.hip).
 //
-// Layouts (complex interleaved, row-major over the GEMM):
-//   P  [M = k-block][K = nR]     phase tile, written by phase_kernel
-//   X  [K = nR][N = nmat*n*n]    column j = mat*n*n + row + n*col
-//   C  [M][N]                    per k: nmat column-major n x n matrices
+// R and -R are paired at upload:  e^{i phi} X_R + e^{-i phi} X_-R
+//   = cos(phi) (X_R + X_-R) + sin(phi) i (X_R - X_-R),
+// so the phase tile is real (cos rows, sin rows, and a row of ones for
+// R = 0) and the sum is a real GEMM against the interleaved (re, im) columns
+// of the combined coefficients: half the flops of the complex product, and
+// the output is already the interleaved complex H(k).  Without a position
+// operator only the upper triangle of each (Hermitian) matrix is produced.
+//
+// Layouts (doubles, row-major over the GEMM):
+//   Pt [K][M]       phase tile, transposed (k-points contiguous)
+//   X  [K][N]       N = 2 * nmat * np: (re, im) of matrix mat, element p
+//                   (p = i + j(j+1)/2 for i <= j packed, i + n j full)
+//   C  [M][N]       per k the nmat matrices, complex interleaved
 #include "mep_common.hpp"
 #include "mep_kernels.hpp"
 
@@ -30,112 +39,123 @@ __global__ void mp_kpts_kernel(int64_t k0, int64_t nk, int n1, int n2, int n3,
   kpts[3 * t + 2] = (2.0 * (double)(z + 1) - (double)n3 - 1.0) / (2.0 * (double)n3);
 }
 
-// P[k][R] = exp(i * 2pi * (k . R))  (wann_interp.f90:258-260 angle order)
-__global__ void phase_kernel(int64_t nk, int nR, const double* __restrict__ kpts,
-                             const double* __restrict__ R, zd* __restrict__ P) {
+// Pt[e][k] = cos(2 pi k.R_e), Pt[Kh + e][k] = sin(2 pi k.R_e) for the
+// paired entries e < Kh (wann_interp.f90:258-260 angle); Pt[2 Kh][k] = 1
+// when the model has R = 0.
+__global__ void phase_real_kernel(int64_t nk, int Kh, int has_zero, const double* __restrict__ kpts,
+                                  const double* __restrict__ Rh, double* __restrict__ Pt) {
   int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (t >= nk * nR) return;
-  int64_t k = t / nR;
-  int s = (int)(t - k * nR);
+  if (t >= nk * (int64_t)(Kh + has_zero)) return;
+  const int e = (int)(t / nk);
+  const int64_t k = t - (int64_t)e * nk;
+  if (e == Kh) {
+    Pt[(int64_t)(2 * Kh) * nk + k] = 1.0;
+    return;
+  }
   const double two_pi = 2.0 * 3.14159265358979323846;
-  double ang = two_pi * (kpts[3 * k] * R[3 * s] + kpts[3 * k + 1] * R[3 * s + 1] +
-                         kpts[3 * k + 2] * R[3 * s + 2]);
+  const double ang = two_pi * (kpts[3 * k] * Rh[3 * e] + kpts[3 * k + 1] * Rh[3 * e + 1] +
+                               kpts[3 * k + 2] * Rh[3 * e + 2]);
   double sn, cs;
   sincos(ang, &sn, &cs);
-  P[t] = zd{cs, sn};
+  Pt[(int64_t)e * nk + k] = cs;
+  Pt[(int64_t)(Kh + e) * nk + k] = sn;
 }
 
 // --------------------------------------------------------------------------
-// FP64 complex GEMM on v_mfma_f64_16x16x4_f64.
-// Block: 256 threads = 4 waves in a 2x2 grid, block tile 64 (k) x 64 (cols)
-// complex; each wave owns 32x32 = 2x2 MFMA tiles, with separate real and
-// imaginary accumulators (C_re += Ar Br - Ai Bi, C_im += Ar Bi + Ai Br).
-// Operand fragments (16x4 A, 4x16 B): lane l holds row/col l%16 at k l/16;
-// accumulators of the f64 instruction: element v of lane l is row l/16 + 4v,
-// col l%16 (measured on gfx950; the f32/bf16 16x16 forms use 4*(l/16) + v).
+// FP64 real GEMM C = Pt^T X on v_mfma_f64_16x16x4_f64.
+// Block: 256 threads = 4 waves in a 2x2 grid, block tile 128 (k-points) x
+// 128 (columns), K staged 16 at a time through double-buffered LDS with the
+// next chunk's global loads in flight during the MFMAs.  Each wave owns
+// 64 x 64 = 4 x 4 MFMA tiles.  Operand fragments (16x4 A, 4x16 B): lane l
+// holds row / col l%16 at k l/16; accumulator element v of lane l is row
+// l/16 + 4v, col l%16 (measured on gfx950).
+// Tiles are mapped XCD-aware: workgroups go round-robin over the 8 XCDs,
+// so each XCD receives a contiguous range of tiles, i.e. all column tiles
+// of a few k-row tiles, and the phase rows they share stay in its L2.
 // --------------------------------------------------------------------------
 typedef double v4d __attribute__((ext_vector_type(4)));
 
-#define FT_BM 64
-#define FT_BN 64
-#define FT_KC 16
+#define DG_BM 128
+#define DG_BN 128
+#define DG_KC 16
+#define DG_LD (DG_BM + 16)  // row pitch: 4 k-rows of a fragment read hit distinct banks
 
-__global__ __launch_bounds__(256) void ft_gemm_f64_kernel(
-    int M, int N, int K, const zd* __restrict__ P, const zd* __restrict__ X,
-    zd* __restrict__ C) {
-  __shared__ double As_re[FT_KC][FT_BM], As_im[FT_KC][FT_BM];
-  __shared__ double Bs_re[FT_KC][FT_BN], Bs_im[FT_KC][FT_BN];
+__global__ __launch_bounds__(256, 2) void ft_gemm_f64_kernel(int M, int N, int K,
+                                                             const double* __restrict__ Pt,
+                                                             const double* __restrict__ X,
+                                                             double* __restrict__ C, int tiles_n,
+                                                             int tiles_total) {
+  __shared__ double As[2][DG_KC][DG_LD], Bs[2][DG_KC][DG_LD];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int wr = wave >> 1, wc = wave & 1;
   const int lr = lane & 15, lk = lane >> 4;
-  const int64_t m0 = (int64_t)blockIdx.y * FT_BM;
-  const int64_t n0 = (int64_t)blockIdx.x * FT_BN;
+  // XCD-aware tile index
+  const int b = blockIdx.x;
+  const int xcd = b & 7, loc = b >> 3;
+  const int base = tiles_total >> 3, rem = tiles_total & 7;
+  const int L = xcd * base + (xcd < rem ? xcd : rem) + loc;
+  const int64_t m0 = (int64_t)(L / tiles_n) * DG_BM;
+  const int64_t n0 = (int64_t)(L % tiles_n) * DG_BN;
 
-  v4d acc_re[2][2], acc_im[2][2];
+  v4d acc[4][4];
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int b = 0; b < 2; ++b) {
-      acc_re[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
-      acc_im[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
-    }
+    for (int j = 0; j < 4; ++j) acc[i][j] = v4d{0.0, 0.0, 0.0, 0.0};
 
-  for (int k0 = 0; k0 < K; k0 += FT_KC) {
-    // stage A: 64 rows x 16 k  (consecutive threads -> consecutive R of a row)
+  // staging map: thread -> (kk = tid/128 + 2 it, col = tid%128), 8 per operand
+  const int sc = tid & 127, sk = tid >> 7;
+  double ra[8], rb[8];
+  auto gload = [&](int k0) {
 #pragma unroll
-    for (int it = 0; it < (FT_BM * FT_KC) / 256; ++it) {
-      int idx = tid + 256 * it;
-      int row = idx / FT_KC, kk = idx % FT_KC;
-      zd v = zd{0.0, 0.0};
-      if (m0 + row < M && k0 + kk < K) v = P[(m0 + row) * (int64_t)K + k0 + kk];
-      As_re[kk][row] = v.re;
-      As_im[kk][row] = v.im;
+    for (int it = 0; it < 8; ++it) {
+      const int kk = k0 + sk + 2 * it;
+      ra[it] = (kk < K && m0 + sc < M) ? Pt[(int64_t)kk * M + m0 + sc] : 0.0;
+      rb[it] = (kk < K && n0 + sc < N) ? X[(int64_t)kk * N + n0 + sc] : 0.0;
     }
-    // stage B: 16 k x 64 cols (consecutive threads -> consecutive columns)
+  };
+  auto lstore = [&](int buf) {
 #pragma unroll
-    for (int it = 0; it < (FT_BN * FT_KC) / 256; ++it) {
-      int idx = tid + 256 * it;
-      int kk = idx / FT_BN, col = idx % FT_BN;
-      zd v = zd{0.0, 0.0};
-      if (n0 + col < N && k0 + kk < K) v = X[(int64_t)(k0 + kk) * N + n0 + col];
-      Bs_re[kk][col] = v.re;
-      Bs_im[kk][col] = v.im;
+    for (int it = 0; it < 8; ++it) {
+      As[buf][sk + 2 * it][sc] = ra[it];
+      Bs[buf][sk + 2 * it][sc] = rb[it];
     }
-    __syncthreads();
+  };
+  const int nch = (K + DG_KC - 1) / DG_KC;
+  gload(0);
+  lstore(0);
+  __syncthreads();
+  for (int ch = 0; ch < nch; ++ch) {
+    const int buf = ch & 1;
+    if (ch + 1 < nch) gload((ch + 1) * DG_KC);
 #pragma unroll
-    for (int kq = 0; kq < FT_KC / 4; ++kq) {
+    for (int kq = 0; kq < DG_KC / 4; ++kq) {
       const int kk = kq * 4 + lk;
-      double ar[2], ai[2], nai[2], br[2], bi[2];
+      double a[4], bb[4];
 #pragma unroll
-      for (int a = 0; a < 2; ++a) {
-        ar[a] = As_re[kk][32 * wr + 16 * a + lr];
-        ai[a] = As_im[kk][32 * wr + 16 * a + lr];
-        nai[a] = -ai[a];
-        br[a] = Bs_re[kk][32 * wc + 16 * a + lr];
-        bi[a] = Bs_im[kk][32 * wc + 16 * a + lr];
+      for (int i = 0; i < 4; ++i) {
+        a[i] = As[buf][kk][64 * wr + 16 * i + lr];
+        bb[i] = Bs[buf][kk][64 * wc + 16 * i + lr];
       }
 #pragma unroll
-      for (int a = 0; a < 2; ++a)
+      for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int b = 0; b < 2; ++b) {
-          acc_re[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[a], br[b], acc_re[a][b], 0, 0, 0);
-          acc_im[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[a], bi[b], acc_im[a][b], 0, 0, 0);
-          acc_re[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai[a], bi[b], acc_re[a][b], 0, 0, 0);
-          acc_im[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[a], br[b], acc_im[a][b], 0, 0, 0);
-        }
+        for (int j = 0; j < 4; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], bb[j], acc[i][j], 0, 0, 0);
     }
+    if (ch + 1 < nch) lstore(buf ^ 1);
     __syncthreads();
   }
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int b = 0; b < 2; ++b)
+    for (int j = 0; j < 4; ++j)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        int64_t row = m0 + 32 * wr + 16 * a + lk + 4 * v;
-        int64_t col = n0 + 32 * wc + 16 * b + lr;
-        if (row < M && col < N) C[row * (int64_t)N + col] = zd{acc_re[a][b][v], acc_im[a][b][v]};
+        const int64_t row = m0 + 64 * wr + 16 * i + lk + 4 * v;
+        const int64_t col = n0 + 64 * wc + 16 * j + lr;
+        if (row < M && col < N) C[row * (int64_t)N + col] = acc[i][j][v];
       }
 }
 
@@ -148,13 +168,18 @@ void launch_mp_kpts(hipStream_t s, int64_t k0, int64_t nk, const int mp[3], doub
                      mp[2], kpts);
 }
 
-void launch_phase(hipStream_t s, int64_t nk, int nR, const double* kpts, const double* R, zd* P) {
-  int64_t tot = nk * nR;
+void launch_phase(hipStream_t s, int64_t nk, int Kh, int has_zero, const double* kpts,
+                  const double* Rh, double* Pt) {
+  int64_t tot = nk * (int64_t)(Kh + has_zero);
   int64_t nb = (tot + 255) / 256;
-  hipLaunchKernelGGL(phase_kernel, dim3((unsigned)nb), dim3(256), 0, s, nk, nR, kpts, R, P);
+  hipLaunchKernelGGL(phase_real_kernel, dim3((unsigned)nb), dim3(256), 0, s, nk, Kh, has_zero, kpts,
+                     Rh, Pt);
 }
 
-void launch_ft_gemm_f64(hipStream_t s, int M, int N, int K, const zd* P, const zd* X, zd* C) {
-  dim3 grid((N + FT_BN - 1) / FT_BN, (M + FT_BM - 1) / FT_BM);
-  hipLaunchKernelGGL(ft_gemm_f64_kernel, grid, dim3(256), 0, s, M, N, K, P, X, C);
+void launch_ft_gemm_f64(hipStream_t s, int M, int N, int K, const double* Pt, const double* X,
+                        double* C) {
+  const int tiles_m = (M + DG_BM - 1) / DG_BM, tiles_n = (N + DG_BN - 1) / DG_BN;
+  const int tiles = tiles_m * tiles_n;
+  hipLaunchKernelGGL(ft_gemm_f64_kernel, dim3((unsigned)tiles), dim3(256), 0, s, M, N, K, Pt, X, C,
+                     tiles_n, tiles);
 }

@@ -10,20 +10,26 @@ struct KGeom {
 };
 
 void launch_mp_kpts(hipStream_t s, int64_t k0, int64_t nk, const int mp[3], double* kpts);
-void launch_phase(hipStream_t s, int64_t nk, int nR, const double* kpts, const double* R, zd* P);
-void launch_ft_gemm_f64(hipStream_t s, int M, int N, int K, const zd* P, const zd* X, zd* C);
+// real phase tile Pt[K][nk] (cos rows, sin rows of the Kh paired R, and a
+// row of ones when has_zero) and the real GEMM C[M][N] = Pt^T X
+void launch_phase(hipStream_t s, int64_t nk, int Kh, int has_zero, const double* kpts,
+                  const double* Rh, double* Pt);
+void launch_ft_gemm_f64(hipStream_t s, int M, int N, int K, const double* Pt, const double* X,
+                        double* C);
 
 // Batched Hermitian eigensolve of H(k) (upper triangle), n <= 32.
 // E (nk, n) ascending; U (nk, n, n) column-major eigenvectors.  scratch
 // holds eigh_scratch_bytes(n, nk) bytes (side records + rotation records).
+// packed: H(k) holds the upper triangle only (element i <= j at i + j(j+1)/2).
 size_t eigh_scratch_bytes(int n, int64_t nk);
-void launch_eigh(hipStream_t s, int n, int64_t nk, const zd* Hk, int64_t strideHk,
+void launch_eigh(hipStream_t s, int n, int64_t nk, const zd* Hk, int64_t strideHk, int packed,
                  const KGeom& g, double* E, zd* U, int* info, double* scratch);
 
 // Velocity (and, with a position operator, connection and curvature) in
 // the Hamiltonian gauge.  V is written planar per k: V[k][a][i + n j].
 struct VeloArgs {
   int n, nmat;
+  int packed;  // upper-triangle matrices (no position operator)
   int gauge_trafo;
   int pos_op;
   int textbook;

@@ -46,22 +46,31 @@ struct TBPhase {
   }
 };
 
-// element (n, m) of mat `mat` after the TB epilogue (mats 0..9 as above)
-__device__ inline zd load_tb(const zd* X, int n, int N, int row, int col, int mat, const TBPhase& tb) {
-  const int64_t N2 = (int64_t)N * N;
-  const int64_t o = row + (int64_t)N * col;
-  zd x = X[mat * N2 + o];
+// Wannier-gauge element (row, col) of matrix `mat`: full column-major
+// matrices of N x N, or (packed) the upper triangles of Hermitian matrices
+// with (i <= j) at i + j(j+1)/2 and the lower half as conjugates
+__device__ inline zd load_w(const zd* X, int N, int packed, int row, int col, int mat) {
+  if (!packed) return X[(int64_t)mat * N * N + row + (int64_t)N * col];
+  const int64_t NP = ((int64_t)N * (N + 1)) / 2;
+  if (row <= col) return X[mat * NP + row + ((int64_t)col * (col + 1)) / 2];
+  return zconj(X[mat * NP + col + ((int64_t)row * (row + 1)) / 2]);
+}
+
+// element (row, col) of mat `mat` after the TB epilogue (mats 0..9 as above)
+__device__ inline zd load_tb(const zd* X, int packed, int N, int row, int col, int mat,
+                             const TBPhase& tb) {
+  zd x = load_w(X, N, packed, row, col, mat);
   if (!tb.tau) return x;
   double dt[3];
   zd ph = tb.phase(row, col, dt);
   if (mat >= 1 && mat <= 3) {
-    zd h = X[o];
+    zd h = load_w(X, N, packed, row, col, 0);
     x = zadd(x, zmuli(zscale(h, tb.c(mat - 1, dt))));
   } else if (mat >= 7) {
     // Omega_ab with (a,b) = (1,2), (2,0), (0,1)
     int a = (mat == 7) ? 1 : (mat == 8) ? 2 : 0;
     int b = (mat == 7) ? 2 : (mat == 8) ? 0 : 1;
-    zd Ab = X[(4 + b) * N2 + o], Aa = X[(4 + a) * N2 + o];
+    zd Ab = load_w(X, N, packed, row, col, 4 + b), Aa = load_w(X, N, packed, row, col, 4 + a);
     x = zadd(x, zmuli(zsub(zscale(Ab, tb.c(a, dt)), zscale(Aa, tb.c(b, dt)))));
   }
   return zmul(ph, x);
@@ -73,7 +82,7 @@ __device__ inline zd load_tb(const zd* X, int n, int N, int row, int col, int ma
 // V is written planar: V[k][a][i + n*j].
 // --------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void velo_mfma_kernel(int n, int64_t nk, const zd* __restrict__ Hk,
-                                                        int64_t strideHk, TBPhase tb0,
+                                                        int64_t strideHk, int packed, TBPhase tb0,
                                                         int gauge_trafo, const zd* __restrict__ U,
                                                         zd* __restrict__ V) {
   constexpr int LD = 33;
@@ -89,7 +98,7 @@ __global__ __launch_bounds__(256) void velo_mfma_kernel(int n, int64_t nk, const
     for (int a = 0; a < 3; ++a) {
       for (int idx = t; idx < n2; idx += 256) {
         int i = idx % n, j = idx / n;
-        Ds[i + LD * j] = load_tb(X, 0, n, i, j, 1 + a, tb);
+        Ds[i + LD * j] = load_tb(X, packed, n, i, j, 1 + a, tb);
       }
       __syncthreads();
       if (gauge_trafo) {
@@ -305,6 +314,6 @@ void launch_velo(hipStream_t s, const VeloArgs& va, int64_t nk, const zd* Hk, in
                        va.gauge_trafo, va.textbook, va.kubo_tol, U, E, V, A, Om);
     return;
   }
-  hipLaunchKernelGGL(velo_mfma_kernel, dim3(grid), dim3(256), 0, s, va.n, nk, Hk, strideHk, tb,
-                     va.gauge_trafo, U, V);
+  hipLaunchKernelGGL(velo_mfma_kernel, dim3(grid), dim3(256), 0, s, va.n, nk, Hk, strideHk,
+                     va.packed, tb, va.gauge_trafo, U, V);
 }
