@@ -14,9 +14,11 @@
 // Per k the workgroup stages in LDS the three velocity planes V_a (padded
 // column-major) and the antisymmetric reciprocal table
 // r(m, n) = 1 / (E_n - E_m) (0 on the diagonal), so no band sum divides.
-//   pair phase    thread (n0, sub) walks n = sub, sub + S, ...: the Berry
-//                 curvature of n0, the local (LC) MEP terms and, for a single
-//                 (hw, E_F), the velo/ohc pair sums;
+//   pair phase    lane (n0, part) of every wave walks n = part, part + P,
+//                 ...; wave 0 sums the Berry curvature of n0 (and finds the
+//                 Niu exclusion windows), wave 1 the local (LC) MEP terms,
+//                 waves 2-3 (single (hw, E_F)) the velo/ohc pair sums; the
+//                 band sums are wavefront shuffle reductions;
 //   itinerant     C_j = V_k B_l - V_l B_k, (j,k,l) cyclic, with
 //                 B_l(m, n0) = V_l(m, n0) r(m, n0): three complex n x n
 //                 products on FP64 MFMA; the IC tensor of band n0 is the
@@ -53,21 +55,92 @@ AccLayout acc_layout(const BandArgs& b) {
 constexpr int BS_THREADS = 256;
 
 // LDS plan: V planes (zd) 3*PL | r table PL | E NMAX | fd nef*NMAX |
-// Q NQ*NMAX | red NMAX | acc total      (all doubles after the V planes)
+// F2 guarded NT*9*NMAX | IC tile sums 9*NT*NT*3*nef | red NMAX | acc total |
+// Niu windows 2*NMAX*NMAX bytes              (doubles after the V planes)
 template <int NMAX>
 struct BsLds {
   static constexpr int LD = NMAX + 1;
   static constexpr int PL = NMAX * LD;
   static constexpr int NT = NMAX / 16;  // 16-row tiles per matrix
-  // Q rows: the pair phase uses 63 (Omega 9, F3 9, F3 guarded 9, velo/ohc
-  // 36), the itinerant phase 18 per row tile (F2, F2 guarded)
-  static constexpr int NQ = (18 * NT > 63) ? 18 * NT : 63;
+  static size_t doubles(int nef, int64_t total) {
+    return (size_t)PL + NMAX + (size_t)nef * NMAX + (size_t)NT * 9 * NMAX +
+           (size_t)9 * NT * NT * nef + NMAX + (size_t)total;
+  }
   static size_t bytes(int nef, int64_t total) {
-    return sizeof(zd) * 3 * (size_t)PL +
-           sizeof(double) * ((size_t)PL + NMAX + (size_t)nef * NMAX + (size_t)NQ * NMAX + NMAX +
-                             (size_t)total);
+    return sizeof(zd) * 3 * (size_t)PL + sizeof(double) * doubles(nef, total) +
+           2 * (size_t)NMAX * NMAX;
   }
 };
+
+// sum over the NMAX lanes of each lane group (every lane receives it)
+template <int NMAX>
+__device__ inline double band_sum(double v) {
+#pragma unroll
+  for (int o = 1; o < NMAX; o <<= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+// sum over the lane groups (offsets NMAX, 2 NMAX, ... < 64)
+template <int NMAX>
+__device__ inline double part_sum(double v) {
+#pragma unroll
+  for (int o = NMAX; o < 64; o <<= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// velo_ahc_tens / kubo_ohc_tens pair sums at a single (hw, E_F) for the
+// components q = QB .. QB+QN-1 (q = i + 3j), accumulated over every pair by
+// one wave and added by lane 0:
+//   velo_q += (f0 - fn) Im(V_i(n0,n) V_j(n,n0)) / ((E0-En)^2 - (hw + i eta)^2)
+//   ohc_q  += i (f0 - fn) V_i(n,n0) V_j(n0,n) / ((E0-En)(E0-En - hw - i eta))
+template <int NMAX, int QB, int QN>
+__device__ inline void pair_sums(int n, int p0, int part, bool act, const zd* Vs, const double* Es,
+                                 const double* fd, zd wq, zd w2, double wk, double* av,
+                                 double* ao) {
+  constexpr int LD = NMAX + 1, PL = NMAX * LD, PARTS = 64 / NMAX;
+  double pv[4 * QN];
+#pragma unroll
+  for (int q = 0; q < 4 * QN; ++q) pv[q] = 0.0;
+  if (act) {
+    const double E0 = Es[p0];
+    for (int nn = part; nn < n; nn += PARTS) {
+      if (nn == p0) continue;
+      zd x[3], y[3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        x[a] = Vs[a * PL + p0 + LD * nn];
+        y[a] = Vs[a * PL + nn + LD * p0];
+      }
+      const double dE = E0 - Es[nn];
+      const double dfv = fd[p0] - fd[nn];
+      const zd denv = zd{dE * dE - w2.re, -w2.im};
+      const zd deno = zmul(zd{dE, 0.0}, zd{dE - wq.re, -wq.im});
+      const double iv = 1.0 / zabs2(denv), io = 1.0 / zabs2(deno);
+      const zd rv = zd{denv.re * iv * dfv, -denv.im * iv * dfv};  // dfv / denv
+      const zd ro = zd{deno.re * io * dfv, -deno.im * io * dfv};  // dfv / deno
+#pragma unroll
+      for (int u = 0; u < QN; ++u) {
+        const int q = QB + u, i = q % 3, j = q / 3;
+        const double vim = zmul(x[i], y[j]).im;
+        pv[2 * u] += vim * rv.re;
+        pv[2 * u + 1] += vim * rv.im;
+        const zd o = zmuli(zmul(zmul(y[i], x[j]), ro));
+        pv[2 * QN + 2 * u] += o.re;
+        pv[2 * QN + 2 * u + 1] += o.im;
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 4 * QN; ++q) pv[q] = band_sum<64>(pv[q]);
+  if (threadIdx.x % 64 == 0)
+#pragma unroll
+    for (int u = 0; u < QN; ++u) {
+      const int q = QB + u;
+      av[2 * q] += wk * pv[2 * u];
+      av[2 * q + 1] += wk * pv[2 * u + 1];
+      ao[2 * q] += wk * pv[2 * QN + 2 * u];
+      ao[2 * q + 1] += wk * pv[2 * QN + 2 * u + 1];
+    }
+}
 
 template <int NMAX>
 __global__ __launch_bounds__(BS_THREADS, 2) void bandsum_kernel(BandArgs b, double Tsmear,
@@ -79,17 +152,19 @@ __global__ __launch_bounds__(BS_THREADS, 2) void bandsum_kernel(BandArgs b, doub
                                                                 const double* __restrict__ w,
                                                                 double* __restrict__ part) {
   using S_ = BsLds<NMAX>;
-  constexpr int LD = S_::LD, PL = S_::PL, NQ = S_::NQ;
-  constexpr int SUB = BS_THREADS / NMAX;  // threads per band in the pair phase
+  constexpr int LD = S_::LD, PL = S_::PL, NT = S_::NT;
+  constexpr int PARTS = 64 / NMAX;
   extern __shared__ double smem[];
   const int n = b.n, nef = b.n_ef, nhw = b.n_hw, nval = b.n_val;
   zd* Vs = reinterpret_cast<zd*>(smem);
   double* Rs = reinterpret_cast<double*>(Vs + 3 * PL);
   double* Es = Rs + PL;
   double* fd = Es + NMAX;
-  double* Q = fd + (int64_t)nef * NMAX;
-  double* red = Q + NQ * NMAX;
+  double* QF = fd + (int64_t)nef * NMAX;        // [NT][9][NMAX] guarded IC per band
+  double* TS = QF + NT * 9 * NMAX;              // [3*NT*NT tiles][3][nef] Kubo IC tile sums
+  double* red = TS + 9 * NT * NT * nef;
   double* acc = red + NMAX;
+  uint8_t* win = reinterpret_cast<uint8_t*>(acc + L.total);  // [n0][n] lo, hi
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   const int n2 = n * n;
   const bool finiteT = b.T_kelvin > 1e-2;
@@ -102,66 +177,118 @@ __global__ __launch_bounds__(BS_THREADS, 2) void bandsum_kernel(BandArgs b, doub
 
   const zd wq = zd{b.hw[0], b.eta};
   const zd w2 = zmul(wq, wq);
-
-  const int p0 = t / SUB, psub = t % SUB;  // pair phase: band p0, sub-lane psub
+  const int p0 = lane % NMAX, pp = lane / NMAX;  // pair phase: band p0, part pp
+  const bool pact = p0 < n;
 
   for (int64_t k = blockIdx.x; k < nk; k += gridDim.x) {
     const double wk = w ? w[k] : 1.0;
     __syncthreads();
     // ---- stage V (planar [a][i + n j]) and E ------------------------------
     const zd* Vk = V + k * 3 * (int64_t)n2;
-    for (int idx = t; idx < 3 * n2; idx += BS_THREADS) {
-      const int a = idx / n2, r = idx - a * n2;
-      Vs[a * PL + r % n + LD * (r / n)] = Vk[idx];
+    {
+      const int i = t % NMAX;
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+        for (int j = t / NMAX; j < n; j += BS_THREADS / NMAX)
+          if (i < n) Vs[a * PL + i + LD * j] = Vk[a * n2 + i + n * j];
     }
     if (t < n) Es[t] = E[k * n + t];
     __syncthreads();
     // ---- r(m, c) = 1 / (E_c - E_m); occupations ---------------------------
-    for (int idx = t; idx < n2; idx += BS_THREADS) {
-      const int m = idx % n, c = idx / n;
-      Rs[m + LD * c] = (m == c) ? 0.0 : 1.0 / (Es[c] - Es[m]);
+    {
+      const int m = t % NMAX;
+      for (int c = t / NMAX; c < n; c += BS_THREADS / NMAX)
+        if (m < n) Rs[m + LD * c] = (m == c) ? 0.0 : 1.0 / (Es[c] - Es[m]);
     }
     for (int idx = t; idx < nef * n; idx += BS_THREADS) {
       if (b.fd_in) {
         fd[idx] = b.fd_in[k * (int64_t)nef * n + idx];
       } else {
-        const int ef = idx % nef, nb = idx / nef;
+        const int ef = (nef == 1) ? 0 : idx % nef, nb = (nef == 1) ? idx : idx / nef;
         const double e = Es[nb], f = b.ef[ef];
         fd[idx] = finiteT ? 1.0 / (1.0 + exp((e - f) / Tsmear)) : ((e < f) ? 1.0 : 0.0);
       }
     }
     __syncthreads();
 
-    // ==== pair phase =======================================================
-    {
-      double om[9], f3[9], f3m[9];
+    // ==== pair phase: lane (p0, pp) of every wave walks n = pp, pp + PARTS,
+    // ...; wave 0: electron count, Berry curvature and mep_niu's exclusion
+    // windows; wave 1: local MEP terms; waves 2-3: velo/ohc pair sums
+    if (wv == 0) {
+      if (t < 64 && pp == 0)
+        for (int ef = 0; ef < nef; ++ef) {
+          const double s = band_sum<NMAX>(pact ? fd[ef + nef * p0] : 0.0);
+          if (lane == 0) acc[L.Ne + ef] += wk * s;
+        }
+      double om[9];
 #pragma unroll
-      for (int q = 0; q < 9; ++q) om[q] = f3[q] = f3m[q] = 0.0;
-      const bool act = p0 < n;
-      const bool doN = doNjob && act && p0 < nval;
-      if (act) {
-        const double E0 = Es[p0];
-        zd z0[3];
-#pragma unroll
-        for (int l = 0; l < 3; ++l) z0[l] = Vs[l * PL + p0 + LD * p0];
-        for (int nn = psub; nn < n; nn += SUB) {
+      for (int q = 0; q < 9; ++q) om[q] = 0.0;
+      const bool wn = doNjob && pact && p0 < nval;
+      const double E0 = pact ? Es[p0] : 0.0;
+      if (pact)
+        for (int nn = pp; nn < n; nn += PARTS) {
           if (nn == p0) continue;
-          zd x[3], y[3];
-#pragma unroll
-          for (int a = 0; a < 3; ++a) {
-            x[a] = Vs[a * PL + p0 + LD * nn];  // V_a(n0, n)
-            y[a] = Vs[a * PL + nn + LD * p0];  // V_a(n, n0)
+          if (wn) {
+            // mep_niu's guard |(E0-En)^2 (E0-Em)| > kubo_tol: energies are
+            // sorted, so the excluded m of (n0, n) form one range around n0
+            const double dn = E0 - Es[nn];
+            const double dn2 = dn * dn;
+            int lo = p0, hi = p0;
+            while (lo > 0 && !(fabs(dn2 * (E0 - Es[lo - 1])) > tol)) --lo;
+            while (hi < n - 1 && !(fabs(dn2 * (E0 - Es[hi + 1])) > tol)) ++hi;
+            win[2 * (p0 * NMAX + nn)] = (uint8_t)lo;
+            win[2 * (p0 * NMAX + nn) + 1] = (uint8_t)hi;
           }
-          const double r = Rs[nn + LD * p0];  // 1 / (E0 - En)
-          const double r2 = r * r, r3 = r2 * r;
           if (doAHC) {
+            zd x[3], y[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+              x[a] = Vs[a * PL + p0 + LD * nn];  // V_a(n0, n)
+              y[a] = Vs[a * PL + nn + LD * p0];  // V_a(n, n0)
+            }
+            const double r = Rs[nn + LD * p0];  // 1 / (E0 - En)
+            const double r2 = r * r;
             // Omega_n0,ij = -2 Im(V_i(n0,n) V_j(n,n0)) / (En - E0)^2
 #pragma unroll
             for (int j = 0; j < 3; ++j)
 #pragma unroll
               for (int i = 0; i < 3; ++i) om[i + 3 * j] -= 2.0 * zmul(x[i], y[j]).im * r2;
           }
-          if (doK || doN) {
+        }
+      if (doAHC) {
+        // kubo_ahc_tens: -sum_n0 f(E_F, n0) Omega_n0
+#pragma unroll
+        for (int q = 0; q < 9; ++q) om[q] = part_sum<NMAX>(om[q]);
+        for (int ef = 0; ef < nef; ++ef) {
+          const double f = (pact && pp == 0) ? fd[ef + nef * p0] : 0.0;
+#pragma unroll
+          for (int q = 0; q < 9; ++q) {
+            const double s = band_sum<64>(f * om[q]);
+            if (lane == 0) acc[L.ahc + q + 9 * ef] -= wk * s;
+          }
+        }
+      }
+    } else if (wv == 1) {
+      if (doK || doNjob) {
+        double f3[9], f3m[9];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) f3[q] = f3m[q] = 0.0;
+        const bool doN = doNjob && pact && p0 < nval;
+        if (pact) {
+          const double E0 = Es[p0];
+          zd z0[3];
+#pragma unroll
+          for (int l = 0; l < 3; ++l) z0[l] = Vs[l * PL + p0 + LD * p0];
+          for (int nn = pp; nn < n; nn += PARTS) {
+            if (nn == p0) continue;
+            zd x[3], y[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+              x[a] = Vs[a * PL + p0 + LD * nn];
+              y[a] = Vs[a * PL + nn + LD * p0];
+            }
+            const double r = Rs[nn + LD * p0];
+            const double r3 = r * r * r;
             // F3_ij = eps_jkl Re(V_i(n0,n) V_k(n,n0) V_l(n0,n0)) / (E0-En)^3
             zd cr[3];
             cr[0] = zsub(zmul(y[1], z0[2]), zmul(y[2], z0[1]));
@@ -179,102 +306,30 @@ __global__ __launch_bounds__(BS_THREADS, 2) void bandsum_kernel(BandArgs b, doub
               }
           }
         }
-      }
-      // reduce over the SUB lanes of band p0 (consecutive lanes of one wave)
-#pragma unroll
-      for (int q = 0; q < 9; ++q) {
-#pragma unroll
-        for (int o = 1; o < SUB; o <<= 1) {
-          om[q] += __shfl_xor(om[q], o, 64);
-          f3[q] += __shfl_xor(f3[q], o, 64);
-          f3m[q] += __shfl_xor(f3m[q], o, 64);
-        }
-      }
-      if (act && psub == 0) {
 #pragma unroll
         for (int q = 0; q < 9; ++q) {
-          Q[q * NMAX + p0] = om[q];
-          Q[(9 + q) * NMAX + p0] = f3[q];
-          Q[(18 + q) * NMAX + p0] = f3m[q];
+          f3[q] = part_sum<NMAX>(f3[q]);
+          f3m[q] = part_sum<NMAX>(f3m[q]);
         }
-      }
-    }
-    if (pmode) {
-      // ordered pairs (x = n0, y = n) of velo_ahc_tens / kubo_ohc_tens for
-      // the single (hw, E_F); per-band sums go to Q rows 27..62
-      double pv[36];
+        if (doN && pp == 0)
 #pragma unroll
-      for (int q = 0; q < 36; ++q) pv[q] = 0.0;
-      if (p0 < n) {
-        const double E0 = Es[p0];
-        for (int nn = psub; nn < n; nn += SUB) {
-          if (nn == p0) continue;
-          zd x[3], y[3];
+          for (int q = 0; q < 9; ++q) acc[L.nlc + q + 9 * p0] += wk * f3m[q];
+        if (doK)
+          for (int ef = 0; ef < nef; ++ef) {
+            const double f = (pact && pp == 0) ? fd[ef + nef * p0] : 0.0;
 #pragma unroll
-          for (int a = 0; a < 3; ++a) {
-            x[a] = Vs[a * PL + p0 + LD * nn];
-            y[a] = Vs[a * PL + nn + LD * p0];
-          }
-          const double dE = E0 - Es[nn];
-          const double dfv = fd[p0] - fd[nn];
-          const zd denv = zd{dE * dE - w2.re, -w2.im};
-          const zd deno = zmul(zd{dE, 0.0}, zd{dE - wq.re, -wq.im});
-          const double iv = 1.0 / zabs2(denv), io = 1.0 / zabs2(deno);
-          const zd rv = zd{denv.re * iv * dfv, -denv.im * iv * dfv};  // dfv / denv
-          const zd ro = zd{deno.re * io * dfv, -deno.im * io * dfv};  // dfv / deno
-#pragma unroll
-          for (int j = 0; j < 3; ++j)
-#pragma unroll
-            for (int i = 0; i < 3; ++i) {
-              const int q = i + 3 * j;
-              const double vim = zmul(x[i], y[j]).im;
-              pv[2 * q] += vim * rv.re;
-              pv[2 * q + 1] += vim * rv.im;
-              const zd o = zmuli(zmul(zmul(y[i], x[j]), ro));
-              pv[18 + 2 * q] += o.re;
-              pv[18 + 2 * q + 1] += o.im;
+            for (int q = 0; q < 9; ++q) {
+              const double s = band_sum<64>(f * f3[q]);
+              if (lane == 0) acc[L.klc + q + 9 * ef] += wk * s;
             }
-        }
+          }
       }
-#pragma unroll
-      for (int q = 0; q < 36; ++q) {
-#pragma unroll
-        for (int o = 1; o < SUB; o <<= 1) pv[q] += __shfl_xor(pv[q], o, 64);
-      }
-      if (p0 < n && psub == 0) {
-#pragma unroll
-        for (int q = 0; q < 36; ++q) Q[(27 + q) * NMAX + p0] = pv[q];
-      }
-    }
-    __syncthreads();
-    // ---- per-k sums of the pair phase --------------------------------------
-    for (int ef = t; ef < nef; ef += BS_THREADS) {
-      double s = 0.0;
-      for (int nb = 0; nb < n; ++nb) s += fd[ef + nef * nb];
-      acc[L.Ne + ef] += wk * s;
-    }
-    for (int it = t; it < 18 * nef; it += BS_THREADS) {
-      const int q = it % 9, h = (it / 9) % 2, ef = it / 18;
-      if (h == 0 && doAHC) {
-        double s = 0.0;
-        for (int nb = 0; nb < n; ++nb) s -= Q[q * NMAX + nb] * fd[ef + nef * nb];
-        acc[L.ahc + q + 9 * ef] += wk * s;
-      }
-      if (h == 1 && doK) {
-        double s = 0.0;
-        for (int nb = 0; nb < n; ++nb) s += Q[(9 + q) * NMAX + nb] * fd[ef + nef * nb];
-        acc[L.klc + q + 9 * ef] += wk * s;
-      }
-    }
-    if (doNjob)
-      for (int it = t; it < 9 * nval; it += BS_THREADS) {
-        const int q = it % 9, n0 = it / 9;
-        acc[L.nlc + q + 9 * n0] += wk * Q[(18 + q) * NMAX + n0];
-      }
-    if (pmode && t < 36) {
-      double s = 0.0;
-      for (int nb = 0; nb < n; ++nb) s += Q[(27 + t) * NMAX + nb];
-      acc[(t < 18 ? L.velo + t : L.ohc + (t - 18))] += wk * s;
+    } else if (pmode) {
+      // wave 2: components q = 0..4, wave 3: q = 5..8 (q = i + 3j)
+      if (wv == 2)
+        pair_sums<NMAX, 0, 5>(n, p0, pp, pact, Vs, Es, fd, wq, w2, wk, acc + L.velo, acc + L.ohc);
+      else
+        pair_sums<NMAX, 5, 4>(n, p0, pp, pact, Vs, Es, fd, wq, w2, wk, acc + L.velo, acc + L.ohc);
     }
     // ---- hw-resolved Hall and optical conductivity for several (hw, E_F) ---
     if (doAHC && !pmode) {
@@ -315,7 +370,7 @@ __global__ __launch_bounds__(BS_THREADS, 2) void bandsum_kernel(BandArgs b, doub
 
     // ==== itinerant (IC) terms on MFMA ======================================
     if (doK || doNjob) {
-      __syncthreads();  // the pair-phase sums above have read Q
+      __syncthreads();  // the Niu windows are complete
       const int nt = (n + 15) / 16;
       const int ntile = 3 * nt * nt;
       for (int tile = wv; tile < ntile; tile += 4) {
@@ -354,7 +409,6 @@ __global__ __launch_bounds__(BS_THREADS, 2) void bandsum_kernel(BandArgs b, doub
         const bool colok = n0 < n;
         const bool doN = doNjob && colok && n0 < nval;
         double f2[3] = {0.0, 0.0, 0.0}, f2m[3] = {0.0, 0.0, 0.0};
-        const double E0 = colok ? Es[n0] : 0.0;
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           const int row = r0 + kq + 4 * v;
@@ -369,11 +423,7 @@ __global__ __launch_bounds__(BS_THREADS, 2) void bandsum_kernel(BandArgs b, doub
             f2[i] -= ai[i].re * cv.re - ai[i].im * cv.im;
           }
           if (doN) {
-            const double dn = E0 - Es[row];
-            const double dn2 = dn * dn;
-            int lo = n0, hi = n0;
-            while (lo > 0 && !(fabs(dn2 * (E0 - Es[lo - 1])) > tol)) --lo;
-            while (hi < n - 1 && !(fabs(dn2 * (E0 - Es[hi + 1])) > tol)) ++hi;
+            const int lo = win[2 * (n0 * NMAX + row)], hi = win[2 * (n0 * NMAX + row) + 1];
             if (lo == 0 && hi == n - 1) continue;  // every m excluded
             zd kept = zd{0.0, 0.0};
             auto term = [&](int mm) {
@@ -386,7 +436,8 @@ __global__ __launch_bounds__(BS_THREADS, 2) void bandsum_kernel(BandArgs b, doub
               for (int mm = hi + 1; mm < n; ++mm) kept = zadd(kept, term(mm));
             } else {
               zd ex = zd{0.0, 0.0};
-              for (int mm = lo; mm <= hi; ++mm) ex = zadd(ex, term(mm));
+              for (int mm = lo; mm <= hi; ++mm)
+                if (mm != n0) ex = zadd(ex, term(mm));
               kept = zsub(cv, ex);
             }
 #pragma unroll
@@ -401,31 +452,33 @@ __global__ __launch_bounds__(BS_THREADS, 2) void bandsum_kernel(BandArgs b, doub
           f2m[i] += __shfl_xor(f2m[i], 16, 64);
           f2m[i] += __shfl_xor(f2m[i], 32, 64);
         }
-        if (kq == 0 && colok) {
+        if (kq == 0 && colok)
 #pragma unroll
-          for (int i = 0; i < 3; ++i) {
-            Q[(tr * 18 + i + 3 * j) * NMAX + n0] = f2[i];
-            Q[(tr * 18 + 9 + i + 3 * j) * NMAX + n0] = f2m[i];
+          for (int i = 0; i < 3; ++i) QF[(tr * 9 + i + 3 * j) * NMAX + n0] = f2m[i];
+        // kubo_mep_IC: f-weighted sum over the tile's 16 bands
+        if (doK)
+          for (int ef = 0; ef < nef; ++ef) {
+            const double f = (colok && kq == 0) ? fd[ef + nef * n0] : 0.0;
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+              const double s = band_sum<16>(f * f2[i]);
+              if (lane == 0) TS[(tile * 3 + i) * nef + ef] = s;
+            }
           }
-        }
       }
       __syncthreads();
       if (doK)
         for (int it = t; it < 9 * nef; it += BS_THREADS) {
-          const int q = it % 9, ef = it / 9;
+          const int q = it % 9, ef = it / 9, i = q % 3, j = q / 3;
           double s = 0.0;
-          for (int nb = 0; nb < n; ++nb) {
-            double f2 = 0.0;
-            for (int tr = 0; tr < nt; ++tr) f2 += Q[(tr * 18 + q) * NMAX + nb];
-            s += f2 * fd[ef + nef * nb];
-          }
+          for (int tt = 0; tt < nt * nt; ++tt) s += TS[((j * nt * nt + tt) * 3 + i) * nef + ef];
           acc[L.kic + q + 9 * ef] += wk * s;
         }
       if (doNjob)
         for (int it = t; it < 9 * nval; it += BS_THREADS) {
           const int q = it % 9, n0 = it / 9;
           double s = 0.0;
-          for (int tr = 0; tr < nt; ++tr) s += Q[(tr * 18 + 9 + q) * NMAX + n0];
+          for (int tr = 0; tr < nt; ++tr) s += QF[(tr * 9 + q) * NMAX + n0];
           acc[L.nic + q + 9 * n0] += wk * s;
         }
     }
