@@ -21,7 +21,8 @@
 //                 band sums are wavefront shuffle reductions;
 //   itinerant     C_j = V_k B_l - V_l B_k, (j,k,l) cyclic, with
 //                 B_l(m, n0) = V_l(m, n0) r(m, n0): three complex n x n
-//                 products on FP64 MFMA; the IC tensor of band n0 is the
+//                 products on FP64 MFMA (three real products per complex
+//                 one); the IC tensor of band n0 is the
 //                 diagonal -Re[A_i C_j](n0, n0) with
 //                 A_i(n0, n) = V_i(n0, n) r(n, n0)^2, contracted straight
 //                 from the accumulator registers.  mep_niu's guard
@@ -200,6 +201,20 @@ __global__ __launch_bounds__(BS_THREADS, 2) void bandsum_kernel(BandArgs b, doub
       for (int c = t / NMAX; c < n; c += BS_THREADS / NMAX)
         if (m < n) Rs[m + LD * c] = (m == c) ? 0.0 : 1.0 / (Es[c] - Es[m]);
     }
+    if (doNjob)
+      for (int idx = t; idx < nval * NMAX; idx += BS_THREADS) {
+        // mep_niu's guard |(E0-En)^2 (E0-Em)| > kubo_tol: energies are
+        // sorted, so the excluded m of (n0, n) form one range around n0
+        const int nn = idx % NMAX, n0 = idx / NMAX;
+        if (nn >= n || nn == n0) continue;
+        const double E0 = Es[n0], dn = E0 - Es[nn];
+        const double dn2 = dn * dn;
+        int lo = n0, hi = n0;
+        while (lo > 0 && !(fabs(dn2 * (E0 - Es[lo - 1])) > tol)) --lo;
+        while (hi < n - 1 && !(fabs(dn2 * (E0 - Es[hi + 1])) > tol)) ++hi;
+        win[2 * (n0 * NMAX + nn)] = (uint8_t)lo;
+        win[2 * (n0 * NMAX + nn) + 1] = (uint8_t)hi;
+      }
     for (int idx = t; idx < nef * n; idx += BS_THREADS) {
       if (b.fd_in) {
         fd[idx] = b.fd_in[k * (int64_t)nef * n + idx];
@@ -212,8 +227,9 @@ __global__ __launch_bounds__(BS_THREADS, 2) void bandsum_kernel(BandArgs b, doub
     __syncthreads();
 
     // ==== pair phase: lane (p0, pp) of every wave walks n = pp, pp + PARTS,
-    // ...; wave 0: electron count, Berry curvature and mep_niu's exclusion
-    // windows; wave 1: local MEP terms; waves 2-3: velo/ohc pair sums
+    // ...; wave 0: electron count and Berry curvature; wave 1: local MEP
+    // terms; waves 2-3: velo/ohc pair sums.  Each wave then goes straight on
+    // to its itinerant tiles.
     if (wv == 0) {
       if (t < 64 && pp == 0)
         for (int ef = 0; ef < nef; ++ef) {
@@ -223,37 +239,22 @@ __global__ __launch_bounds__(BS_THREADS, 2) void bandsum_kernel(BandArgs b, doub
       double om[9];
 #pragma unroll
       for (int q = 0; q < 9; ++q) om[q] = 0.0;
-      const bool wn = doNjob && pact && p0 < nval;
-      const double E0 = pact ? Es[p0] : 0.0;
-      if (pact)
+      if (pact && doAHC)
         for (int nn = pp; nn < n; nn += PARTS) {
           if (nn == p0) continue;
-          if (wn) {
-            // mep_niu's guard |(E0-En)^2 (E0-Em)| > kubo_tol: energies are
-            // sorted, so the excluded m of (n0, n) form one range around n0
-            const double dn = E0 - Es[nn];
-            const double dn2 = dn * dn;
-            int lo = p0, hi = p0;
-            while (lo > 0 && !(fabs(dn2 * (E0 - Es[lo - 1])) > tol)) --lo;
-            while (hi < n - 1 && !(fabs(dn2 * (E0 - Es[hi + 1])) > tol)) ++hi;
-            win[2 * (p0 * NMAX + nn)] = (uint8_t)lo;
-            win[2 * (p0 * NMAX + nn) + 1] = (uint8_t)hi;
+          zd x[3], y[3];
+#pragma unroll
+          for (int a = 0; a < 3; ++a) {
+            x[a] = Vs[a * PL + p0 + LD * nn];  // V_a(n0, n)
+            y[a] = Vs[a * PL + nn + LD * p0];  // V_a(n, n0)
           }
-          if (doAHC) {
-            zd x[3], y[3];
+          const double r = Rs[nn + LD * p0];  // 1 / (E0 - En)
+          const double r2 = r * r;
+          // Omega_n0,ij = -2 Im(V_i(n0,n) V_j(n,n0)) / (En - E0)^2
 #pragma unroll
-            for (int a = 0; a < 3; ++a) {
-              x[a] = Vs[a * PL + p0 + LD * nn];  // V_a(n0, n)
-              y[a] = Vs[a * PL + nn + LD * p0];  // V_a(n, n0)
-            }
-            const double r = Rs[nn + LD * p0];  // 1 / (E0 - En)
-            const double r2 = r * r;
-            // Omega_n0,ij = -2 Im(V_i(n0,n) V_j(n,n0)) / (En - E0)^2
+          for (int j = 0; j < 3; ++j)
 #pragma unroll
-            for (int j = 0; j < 3; ++j)
-#pragma unroll
-              for (int i = 0; i < 3; ++i) om[i + 3 * j] -= 2.0 * zmul(x[i], y[j]).im * r2;
-          }
+            for (int i = 0; i < 3; ++i) om[i + 3 * j] -= 2.0 * zmul(x[i], y[j]).im * r2;
         }
       if (doAHC) {
         // kubo_ahc_tens: -sum_n0 f(E_F, n0) Omega_n0
@@ -370,7 +371,6 @@ __global__ __launch_bounds__(BS_THREADS, 2) void bandsum_kernel(BandArgs b, doub
 
     // ==== itinerant (IC) terms on MFMA ======================================
     if (doK || doNjob) {
-      __syncthreads();  // the Niu windows are complete
       const int nt = (n + 15) / 16;
       const int ntile = 3 * nt * nt;
       for (int tile = wv; tile < ntile; tile += 4) {
@@ -380,8 +380,12 @@ __global__ __launch_bounds__(BS_THREADS, 2) void bandsum_kernel(BandArgs b, doub
         const zd* Vl_ = Vs + ll_ * PL;
         const int r0 = 16 * tr, c0 = 16 * tc;
         const int ar = r0 + (lane & 15), bc = c0 + (lane & 15), kq = lane >> 4;
-        mep_v4d re0 = {0, 0, 0, 0}, im0 = {0, 0, 0, 0}, re1 = {0, 0, 0, 0}, im1 = {0, 0, 0, 0};
-        for (int k0 = 0; k0 < n; k0 += 4) {
+        // C = sum_m A(., m) B(m, .) with three real products per complex
+        // one: P1 = Ar Br, P2 = Ai Bi, P3 = (Ar + Ai)(Br + Bi);
+        // Re C = P1 - P2, Im C = P3 - P1 - P2
+        mep_v4d p1 = {0, 0, 0, 0}, p2 = {0, 0, 0, 0}, p3 = {0, 0, 0, 0};
+#pragma unroll
+        for (int k0 = 0; k0 < NMAX; k0 += 4) {
           const int m = k0 + kq;
           zd a0 = zd{0.0, 0.0}, a1 = zd{0.0, 0.0}, b0 = zd{0.0, 0.0}, b1 = zd{0.0, 0.0};
           if (m < n) {
@@ -395,14 +399,12 @@ __global__ __launch_bounds__(BS_THREADS, 2) void bandsum_kernel(BandArgs b, doub
               b1 = zscale(Vk_[m + LD * bc], -rr);
             }
           }
-          re0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0.re, b0.re, re0, 0, 0, 0);
-          im0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0.re, b0.im, im0, 0, 0, 0);
-          re1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1.re, b1.re, re1, 0, 0, 0);
-          im1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1.re, b1.im, im1, 0, 0, 0);
-          re0 = __builtin_amdgcn_mfma_f64_16x16x4f64(-a0.im, b0.im, re0, 0, 0, 0);
-          im0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0.im, b0.re, im0, 0, 0, 0);
-          re1 = __builtin_amdgcn_mfma_f64_16x16x4f64(-a1.im, b1.im, re1, 0, 0, 0);
-          im1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1.im, b1.re, im1, 0, 0, 0);
+          p1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0.re, b0.re, p1, 0, 0, 0);
+          p2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0.im, b0.im, p2, 0, 0, 0);
+          p3 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0.re + a0.im, b0.re + b0.im, p3, 0, 0, 0);
+          p1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1.re, b1.re, p1, 0, 0, 0);
+          p2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1.im, b1.im, p2, 0, 0, 0);
+          p3 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1.re + a1.im, b1.re + b1.im, p3, 0, 0, 0);
         }
         // contraction with A_i(n0, n) = V_i(n0, n) r(n, n0)^2, n0 = column
         const int n0 = bc;
@@ -413,7 +415,7 @@ __global__ __launch_bounds__(BS_THREADS, 2) void bandsum_kernel(BandArgs b, doub
         for (int v = 0; v < 4; ++v) {
           const int row = r0 + kq + 4 * v;
           if (!colok || row >= n || row == n0) continue;
-          const zd cv = zd{re0[v] + re1[v], im0[v] + im1[v]};
+          const zd cv = zd{p1[v] - p2[v], p3[v] - p1[v] - p2[v]};
           const double rr = Rs[row + LD * n0];
           const double rr2 = rr * rr;
           zd ai[3];
