@@ -149,6 +149,30 @@ def main():
                            "bandsum": "bandsum_kernel<32>"}[dom],
                 "launch_ms": d["launch_ms"], "stages": stages}
     out = host.finalize(sums, p, mp, m["a_latt"])
+    # the mixed-precision path on the same mesh: throughput and error per
+    # quantity against this FP64 run
+    rpm = host.run_params(p)
+    rpm.precision = capi.MEP_PREC_MIXED
+    eng.run_mesh(mp, rpm)
+    tm = []
+    msums = None
+    for _ in range(max(1, min(args.steps, 3))):
+        t0 = time.perf_counter()
+        msums = eng.run_mesh(mp, rpm)
+        tm.append(time.perf_counter() - t0)
+    mout = host.finalize(msums, p, mp, m["a_latt"])
+
+    def relerr(a, b):
+        a, b = np.asarray(a), np.asarray(b)
+        return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+    mixed = {"k_pts_s": nk / statistics.median(tm), "ms_per_step": statistics.median(tm) * 1e3,
+             "speedup_vs_fp64": t_med / statistics.median(tm),
+             "rel_err_vs_fp64": {"sigma_xy": abs(mout["ahc_tens"][0, 1] - out["ahc_tens"][0, 1])
+                                 / abs(out["ahc_tens"][0, 1]),
+                                 "ahc_tens": relerr(mout["ahc_tens"], out["ahc_tens"]),
+                                 "mep_tens_niu": relerr(mout["mep_tens"], out["mep_tens"]),
+                                 "kubo_mep_tens": relerr(mout["kubo_mep_tens"], out["kubo_mep_tens"])}}
+    log(f"[bench] mixed precision {mixed}")
     log(f"[bench] sigma_xy = {out['ahc_tens'][0, 1]:.10e}  alpha_Niu_xx = {out['mep_tens'][0, 0]:.6e}")
 
     cpu = None
@@ -178,6 +202,7 @@ def main():
                    "kernel_ms": {k: v[0] for k, v in split.items()}},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "mixed_precision": mixed,
     }
     print(json.dumps(res), flush=True)
     eng.close()

@@ -47,6 +47,11 @@ extern "C" {
 
 /* arithmetic of the k-space pipeline */
 #define MEP_PREC_FP64 0          /* FP64 MFMA Fourier GEMM, FP64 eigensolve  */
+#define MEP_PREC_MIXED 1         /* FP32 MFMA Fourier GEMM and FP32 eigensolve;
+                                    FP64 Rayleigh-quotient refinement of every
+                                    eigenvalue and Rayleigh-Ritz of groups
+                                    closer than 1e-3 eV; FP64 velocities, band
+                                    sums and k-sums (no position operator)  */
 
 typedef struct mep_ctx mep_ctx;
 
@@ -122,7 +127,8 @@ int mep_run_kpts(mep_ctx *ctx, int64_t nk, const double *kpts, const double *wei
                  const mep_run_params *p, mep_sums *out);
 
 /* Kernel timing with HIP events on the context's stream (off by default).
-   name: "phase", "ft_gemm", "eigh", "velo", "bandsum", "reduce", "kgen".   */
+   name: "phase", "ft_gemm", "eigh", "refine", "velo", "bandsum", "reduce",
+   "kgen".                                                                   */
 int mep_set_profiling(mep_ctx *ctx, int32_t on);
 int mep_get_kernel_time(mep_ctx *ctx, const char *name, double *total_ms, int64_t *launches);
 int mep_reset_kernel_times(mep_ctx *ctx);

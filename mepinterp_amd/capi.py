@@ -23,6 +23,7 @@ MEP_JOB_AHC = 1
 MEP_JOB_KUBO = 2
 MEP_JOB_MEP = 4
 MEP_PREC_FP64 = 0
+MEP_PREC_MIXED = 1
 
 # every symbol include/mepinterp.h declares
 EXPORTS = ("mep_ctx_create", "mep_ctx_destroy", "mep_last_error", "mep_interp_batch",

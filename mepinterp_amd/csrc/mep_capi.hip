@@ -31,8 +31,9 @@ static int fail(int code, const std::string& msg) {
     if (r_ != MEP_OK) return r_; \
   } while (0)
 
-enum Kind { K_KGEN = 0, K_PHASE, K_GEMM, K_EIGH, K_VELO, K_BANDSUM, K_REDUCE, K_NKIND };
-static const char* kKindName[K_NKIND] = {"kgen", "phase", "ft_gemm", "eigh", "velo", "bandsum", "reduce"};
+enum Kind { K_KGEN = 0, K_PHASE, K_GEMM, K_EIGH, K_REFINE, K_VELO, K_BANDSUM, K_REDUCE, K_NKIND };
+static const char* kKindName[K_NKIND] = {"kgen", "phase", "ft_gemm", "eigh", "refine", "velo",
+                                         "bandsum", "reduce"};
 
 struct EvPair {
   hipEvent_t a, b;
@@ -59,6 +60,7 @@ struct mep_ctx {
   double* dR = nullptr;  // (3, Kh) R of the paired entries
   double* dtau = nullptr;
   double* dX = nullptr;  // (K, 2 ncols) real coefficients
+  float* dX32 = nullptr;  // the same in FP32 (mixed-precision path)
   // k-block workspace
   int64_t kcap = 0, kblock_req = 0;
   double* dk = nullptr;
@@ -336,6 +338,15 @@ extern "C" int mep_ctx_create(int64_t n_wf, int64_t n_R, const double* R_frac,
   if ((e = hipMalloc(&c->dX, sizeof(double) * X.size())) != hipSuccess) return bad(e, "hipMalloc X");
   if ((e = hipMemcpy(c->dX, X.data(), sizeof(double) * X.size(), hipMemcpyHostToDevice)) != hipSuccess)
     return bad(e, "upload X");
+  if (!c->pos) {
+    std::vector<float> X32(X.size());
+    for (size_t q = 0; q < X.size(); ++q) X32[q] = (float)X[q];
+    if ((e = hipMalloc(&c->dX32, sizeof(float) * X32.size())) != hipSuccess)
+      return bad(e, "hipMalloc X32");
+    if ((e = hipMemcpy(c->dX32, X32.data(), sizeof(float) * X32.size(), hipMemcpyHostToDevice)) !=
+        hipSuccess)
+      return bad(e, "upload X32");
+  }
   if ((e = hipMalloc(&c->dinfo, sizeof(int))) != hipSuccess) return bad(e, "hipMalloc info");
   if ((e = hipMemset(c->dinfo, 0, sizeof(int))) != hipSuccess) return bad(e, "memset info");
   *out = c;
@@ -347,7 +358,7 @@ extern "C" int mep_ctx_destroy(mep_ctx* c) {
   if (c->st) hipStreamSynchronize(c->st);
   resolve_times(c);
   free_ws(c);
-  hipFree(c->dR); hipFree(c->dtau); hipFree(c->dX); hipFree(c->dinfo);
+  hipFree(c->dR); hipFree(c->dtau); hipFree(c->dX); hipFree(c->dX32); hipFree(c->dinfo);
   hipFree(c->dpart); hipFree(c->dacc); hipFree(c->def); hipFree(c->dhw); hipFree(c->dfd);
   for (auto e : c->pool) hipEventDestroy(e);
   if (c->st) hipStreamDestroy(c->st);
@@ -403,31 +414,57 @@ static KGeom geom(const mep_ctx* c) {
   return g;
 }
 
-static int interp_block(mep_ctx* c, int64_t nb, bool velo) {
-  const int64_t n = c->n, n2 = n * n;
+// near-degenerate threshold of the mixed-precision refinement: 1e-3 eV
+static const double kMixedGroupHartree = 1e-3 / 27.211385;
+
+static int interp_block(mep_ctx* c, int64_t nb, bool velo, int prec = MEP_PREC_FP64) {
+  const int64_t n = c->n;
+  KGeom g = geom(c);
+  VeloArgs va;
+  va.n = (int)n;
+  va.nmat = c->nmat;
+  va.packed = c->packed;
+  va.gauge_trafo = (c->flags & MEP_F_GAUGE_TRAFO) ? 1 : 0;
+  va.pos_op = c->pos ? 1 : 0;
+  va.textbook = (c->flags & MEP_F_TEXTBOOK) ? 1 : 0;
+  va.kubo_tol = c->kubo_tol;
+  if (prec == MEP_PREC_MIXED) {
+    // FP32 phase tile, FP32 GEMM, FP32 eigensolve (U in V's slot), FP64
+    // refinement (U -> dU), FP64 velocities from the FP32 dH
+    float* P32 = reinterpret_cast<float*>(c->dP);
+    zf* H32 = reinterpret_cast<zf*>(c->dHk);
+    zf* U32 = reinterpret_cast<zf*>(c->dV);
+    timed(c, K_PHASE, [&] { launch_phase_f32(c->st, nb, c->Kh, c->has_zero, c->dk, c->dR, P32); });
+    timed(c, K_GEMM, [&] {
+      launch_ft_gemm_f32(c->st, (int)nb, (int)(2 * c->ncols), (int)c->K, P32, c->dX32,
+                         reinterpret_cast<float*>(H32));
+    });
+    timed(c, K_EIGH, [&] {
+      launch_eigh_f32(c->st, (int)n, nb, H32, c->ncols, c->packed, g, c->dE, U32, c->dinfo,
+                      c->dscr);
+    });
+    timed(c, K_REFINE, [&] {
+      launch_refine(c->st, (int)n, nb, H32, c->ncols, c->packed, g, kMixedGroupHartree, U32, c->dE,
+                    c->dU);
+    });
+    if (velo)
+      timed(c, K_VELO, [&] { launch_velo_h32(c->st, va, nb, H32, c->ncols, g, c->dU, c->dV); });
+    HIPCHK(hipGetLastError());
+    return MEP_OK;
+  }
   timed(c, K_PHASE, [&] { launch_phase(c->st, nb, c->Kh, c->has_zero, c->dk, c->dR, c->dP); });
   timed(c, K_GEMM, [&] {
     launch_ft_gemm_f64(c->st, (int)nb, (int)(2 * c->ncols), (int)c->K, c->dP, c->dX,
                        reinterpret_cast<double*>(c->dHk));
   });
-  KGeom g = geom(c);
   timed(c, K_EIGH, [&] {
     launch_eigh(c->st, (int)n, nb, c->dHk, c->ncols, c->packed, g, c->dE, c->dU, c->dinfo,
                 c->dscr);
   });
-  if (velo) {
-    VeloArgs va;
-    va.n = (int)n;
-    va.nmat = c->nmat;
-    va.packed = c->packed;
-    va.gauge_trafo = (c->flags & MEP_F_GAUGE_TRAFO) ? 1 : 0;
-    va.pos_op = c->pos ? 1 : 0;
-    va.textbook = (c->flags & MEP_F_TEXTBOOK) ? 1 : 0;
-    va.kubo_tol = c->kubo_tol;
+  if (velo)
     timed(c, K_VELO, [&] {
       launch_velo(c->st, va, nb, c->dHk, c->ncols, g, c->dU, c->dE, c->dV, c->dA, c->dOm);
     });
-  }
   HIPCHK(hipGetLastError());
   return MEP_OK;
 }
@@ -464,7 +501,10 @@ static int band_setup(mep_ctx* c, const mep_run_params* p, int64_t kblock, BandS
   if (p->n_eF < 1 || p->n_hw < 1) return fail(MEP_ERR_ARG, "n_eF and n_hw must be >= 1");
   if ((p->jobs & MEP_JOB_MEP) && (p->valence_bands < 1 || p->valence_bands > c->n))
     return fail(MEP_ERR_ARG, "valence_bands must be in [1, n_wf] for the Niu MEP tensor");
-  if (p->precision != MEP_PREC_FP64) return fail(MEP_ERR_ARG, "unknown precision");
+  if (p->precision != MEP_PREC_FP64 && p->precision != MEP_PREC_MIXED)
+    return fail(MEP_ERR_ARG, "unknown precision");
+  if (p->precision == MEP_PREC_MIXED && c->pos)
+    return fail(MEP_ERR_ARG, "the mixed-precision path does not take a position operator");
   BandArgs& b = bs.b;
   b.n = (int)c->n;
   b.jobs = p->jobs;
@@ -652,7 +692,7 @@ static int run_common(mep_ctx* c, int64_t nk, const int32_t* mp, const double* k
     }
     if (weights)
       HIPCHK(hipMemcpyAsync(c->dw, weights + k0, sizeof(double) * nb, hipMemcpyHostToDevice, c->st));
-    CHK(interp_block(c, nb, true));
+    CHK(interp_block(c, nb, true, p->precision));
     CHK(band_block(c, bs, nb, weights ? c->dw : nullptr));
   }
   CHK(check_info(c));

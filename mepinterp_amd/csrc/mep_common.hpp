@@ -32,6 +32,39 @@ __host__ __device__ inline zd zdiv(zd a, zd b) {
   return zd{(a.re * b.re + a.im * b.im) / d, (a.im * b.re - a.re * b.im) / d};
 }
 
+// Complex FP32 value, interleaved (re, im).
+struct zf {
+  float re, im;
+};
+__host__ __device__ inline zf zadd(zf a, zf b) { return zf{a.re + b.re, a.im + b.im}; }
+__host__ __device__ inline zf zsub(zf a, zf b) { return zf{a.re - b.re, a.im - b.im}; }
+__host__ __device__ inline zf zmul(zf a, zf b) {
+  return zf{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+__host__ __device__ inline zf zcmul(zf a, zf b) {
+  return zf{a.re * b.re + a.im * b.im, a.re * b.im - a.im * b.re};
+}
+__host__ __device__ inline zf zconj(zf a) { return zf{a.re, -a.im}; }
+__host__ __device__ inline zf zscale(zf a, float s) { return zf{a.re * s, a.im * s}; }
+__host__ __device__ inline zf zmuli(zf a) { return zf{-a.im, a.re}; }
+__host__ __device__ inline float zabs2(zf a) { return a.re * a.re + a.im * a.im; }
+
+// complex type and machine epsilon of a real type
+template <class R>
+struct cplx_of;
+template <>
+struct cplx_of<double> {
+  using type = zd;
+  static constexpr double eps = 2.220446049250313e-16;
+};
+template <>
+struct cplx_of<float> {
+  using type = zf;
+  static constexpr float eps = 1.1920929e-07f;
+};
+__device__ inline void sin_cos(double a, double* s, double* c) { sincos(a, s, c); }
+__device__ inline void sin_cos(float a, float* s, float* c) { sincosf(a, s, c); }
+
 // Wave-wide FP64 sum (all 64 lanes receive the result).
 __device__ inline double wave_sum(double v) {
 #pragma unroll

@@ -61,6 +61,27 @@ __global__ void phase_real_kernel(int64_t nk, int Kh, int has_zero, const double
   Pt[(int64_t)(Kh + e) * nk + k] = sn;
 }
 
+// the same tile rounded to FP32 (angles and sin/cos in FP64)
+__global__ void phase_real_f32_kernel(int64_t nk, int Kh, int has_zero,
+                                      const double* __restrict__ kpts,
+                                      const double* __restrict__ Rh, float* __restrict__ Pt) {
+  int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= nk * (int64_t)(Kh + has_zero)) return;
+  const int e = (int)(t / nk);
+  const int64_t k = t - (int64_t)e * nk;
+  if (e == Kh) {
+    Pt[(int64_t)(2 * Kh) * nk + k] = 1.0f;
+    return;
+  }
+  const double two_pi = 2.0 * 3.14159265358979323846;
+  const double ang = two_pi * (kpts[3 * k] * Rh[3 * e] + kpts[3 * k + 1] * Rh[3 * e + 1] +
+                               kpts[3 * k + 2] * Rh[3 * e + 2]);
+  double sn, cs;
+  sincos(ang, &sn, &cs);
+  Pt[(int64_t)e * nk + k] = (float)cs;
+  Pt[(int64_t)(Kh + e) * nk + k] = (float)sn;
+}
+
 // --------------------------------------------------------------------------
 // FP64 real GEMM C = Pt^T X on v_mfma_f64_16x16x4_f64.
 // Block: 256 threads = 4 waves in a 2x2 grid, block tile 128 (k-points) x
@@ -160,6 +181,90 @@ __global__ __launch_bounds__(256, 2) void ft_gemm_f64_kernel(int M, int N, int K
 }
 
 // --------------------------------------------------------------------------
+// FP32 real GEMM C = Pt^T X on v_mfma_f32_16x16x4_f32 (FP32 accumulation),
+// same tiling and XCD mapping as the FP64 kernel.  Accumulator element v of
+// lane l is row 4(l/16) + v, col l%16 (measured on gfx950).
+// --------------------------------------------------------------------------
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256, 2) void ft_gemm_f32_kernel(int M, int N, int K,
+                                                             const float* __restrict__ Pt,
+                                                             const float* __restrict__ X,
+                                                             float* __restrict__ C, int tiles_n,
+                                                             int tiles_total) {
+  __shared__ float As[2][DG_KC][DG_LD], Bs[2][DG_KC][DG_LD];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wr = wave >> 1, wc = wave & 1;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int b = blockIdx.x;
+  const int xcd = b & 7, loc = b >> 3;
+  const int base = tiles_total >> 3, rem = tiles_total & 7;
+  const int L = xcd * base + (xcd < rem ? xcd : rem) + loc;
+  const int64_t m0 = (int64_t)(L / tiles_n) * DG_BM;
+  const int64_t n0 = (int64_t)(L % tiles_n) * DG_BN;
+
+  v4f acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+
+  const int sc = tid & 127, sk = tid >> 7;
+  float ra[8], rb[8];
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int it = 0; it < 8; ++it) {
+      const int kk = k0 + sk + 2 * it;
+      ra[it] = (kk < K && m0 + sc < M) ? Pt[(int64_t)kk * M + m0 + sc] : 0.0f;
+      rb[it] = (kk < K && n0 + sc < N) ? X[(int64_t)kk * N + n0 + sc] : 0.0f;
+    }
+  };
+  auto lstore = [&](int buf) {
+#pragma unroll
+    for (int it = 0; it < 8; ++it) {
+      As[buf][sk + 2 * it][sc] = ra[it];
+      Bs[buf][sk + 2 * it][sc] = rb[it];
+    }
+  };
+  const int nch = (K + DG_KC - 1) / DG_KC;
+  gload(0);
+  lstore(0);
+  __syncthreads();
+  for (int ch = 0; ch < nch; ++ch) {
+    const int buf = ch & 1;
+    if (ch + 1 < nch) gload((ch + 1) * DG_KC);
+#pragma unroll
+    for (int kq = 0; kq < DG_KC / 4; ++kq) {
+      const int kk = kq * 4 + lk;
+      float a[4], bb[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        a[i] = As[buf][kk][64 * wr + 16 * i + lr];
+        bb[i] = Bs[buf][kk][64 * wc + 16 * i + lr];
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i], bb[j], acc[i][j], 0, 0, 0);
+    }
+    if (ch + 1 < nch) lstore(buf ^ 1);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int64_t row = m0 + 64 * wr + 16 * i + 4 * lk + v;
+        const int64_t col = n0 + 64 * wc + 16 * j + lr;
+        if (row < M && col < N) C[row * (int64_t)N + col] = acc[i][j][v];
+      }
+}
+
+// --------------------------------------------------------------------------
 // launchers
 // --------------------------------------------------------------------------
 void launch_mp_kpts(hipStream_t s, int64_t k0, int64_t nk, const int mp[3], double* kpts) {
@@ -181,5 +286,21 @@ void launch_ft_gemm_f64(hipStream_t s, int M, int N, int K, const double* Pt, co
   const int tiles_m = (M + DG_BM - 1) / DG_BM, tiles_n = (N + DG_BN - 1) / DG_BN;
   const int tiles = tiles_m * tiles_n;
   hipLaunchKernelGGL(ft_gemm_f64_kernel, dim3((unsigned)tiles), dim3(256), 0, s, M, N, K, Pt, X, C,
+                     tiles_n, tiles);
+}
+
+void launch_phase_f32(hipStream_t s, int64_t nk, int Kh, int has_zero, const double* kpts,
+                      const double* Rh, float* Pt) {
+  int64_t tot = nk * (int64_t)(Kh + has_zero);
+  int64_t nb = (tot + 255) / 256;
+  hipLaunchKernelGGL(phase_real_f32_kernel, dim3((unsigned)nb), dim3(256), 0, s, nk, Kh, has_zero,
+                     kpts, Rh, Pt);
+}
+
+void launch_ft_gemm_f32(hipStream_t s, int M, int N, int K, const float* Pt, const float* X,
+                        float* C) {
+  const int tiles_m = (M + DG_BM - 1) / DG_BM, tiles_n = (N + DG_BN - 1) / DG_BN;
+  const int tiles = tiles_m * tiles_n;
+  hipLaunchKernelGGL(ft_gemm_f32_kernel, dim3((unsigned)tiles), dim3(256), 0, s, M, N, K, Pt, X, C,
                      tiles_n, tiles);
 }

@@ -16,6 +16,11 @@ void launch_phase(hipStream_t s, int64_t nk, int Kh, int has_zero, const double*
                   const double* Rh, double* Pt);
 void launch_ft_gemm_f64(hipStream_t s, int M, int N, int K, const double* Pt, const double* X,
                         double* C);
+// mixed precision: FP32 phase tile and FP32 GEMM (FP32 MFMA accumulation)
+void launch_phase_f32(hipStream_t s, int64_t nk, int Kh, int has_zero, const double* kpts,
+                      const double* Rh, float* Pt);
+void launch_ft_gemm_f32(hipStream_t s, int M, int N, int K, const float* Pt, const float* X,
+                        float* C);
 
 // Batched Hermitian eigensolve of H(k) (upper triangle), n <= 32.
 // E (nk, n) ascending; U (nk, n, n) column-major eigenvectors.  scratch
@@ -24,6 +29,9 @@ void launch_ft_gemm_f64(hipStream_t s, int M, int N, int K, const double* Pt, co
 size_t eigh_scratch_bytes(int n, int64_t nk);
 void launch_eigh(hipStream_t s, int n, int64_t nk, const zd* Hk, int64_t strideHk, int packed,
                  const KGeom& g, double* E, zd* U, int* info, double* scratch);
+// the same in FP32 arithmetic (FP32 H(k) and U; eigenvalues written as FP64)
+void launch_eigh_f32(hipStream_t s, int n, int64_t nk, const zf* Hk, int64_t strideHk, int packed,
+                     const KGeom& g, double* E, zf* U, int* info, double* scratch);
 
 // Velocity (and, with a position operator, connection and curvature) in
 // the Hamiltonian gauge.  V is written planar per k: V[k][a][i + n j].
@@ -37,6 +45,15 @@ struct VeloArgs {
 };
 void launch_velo(hipStream_t s, const VeloArgs& va, int64_t nk, const zd* Hk, int64_t strideHk,
                  const KGeom& g, const zd* U, const double* E, zd* V, zd* A, zd* Om);
+
+// mixed precision: dH_a from the FP32 Fourier sum (no position operator),
+// U and V in FP64
+void launch_velo_h32(hipStream_t s, const VeloArgs& va, int64_t nk, const zf* Hk,
+                     int64_t strideHk, const KGeom& g, const zd* U, zd* V);
+// FP64 refinement of FP32 eigenpairs: Rayleigh quotients for every band,
+// Rayleigh-Ritz inside groups closer than dE_thr (Hartree); U out in FP64.
+void launch_refine(hipStream_t s, int n, int64_t nk, const zf* Hk, int64_t strideHk, int packed,
+                   const KGeom& g, double dE_thr, const zf* U32, double* E, zd* U);
 
 // V layout conversion: planar [a][i + n j] <-> reference (3,n,n), per k.
 void launch_v_layout(hipStream_t s, int n, int64_t nk, const zd* src, zd* dst, bool to_ref);

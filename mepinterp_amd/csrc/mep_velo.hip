@@ -48,16 +48,21 @@ struct TBPhase {
 
 // Wannier-gauge element (row, col) of matrix `mat`: full column-major
 // matrices of N x N, or (packed) the upper triangles of Hermitian matrices
-// with (i <= j) at i + j(j+1)/2 and the lower half as conjugates
-__device__ inline zd load_w(const zd* X, int N, int packed, int row, int col, int mat) {
-  if (!packed) return X[(int64_t)mat * N * N + row + (int64_t)N * col];
+// with (i <= j) at i + j(j+1)/2 and the lower half as conjugates.  HT is the
+// stored complex type (FP64, or FP32 from the mixed-precision GEMM).
+__device__ inline zd up64(zd x) { return x; }
+__device__ inline zd up64(zf x) { return zd{(double)x.re, (double)x.im}; }
+template <class HT>
+__device__ inline zd load_w(const HT* X, int N, int packed, int row, int col, int mat) {
+  if (!packed) return up64(X[(int64_t)mat * N * N + row + (int64_t)N * col]);
   const int64_t NP = ((int64_t)N * (N + 1)) / 2;
-  if (row <= col) return X[mat * NP + row + ((int64_t)col * (col + 1)) / 2];
-  return zconj(X[mat * NP + col + ((int64_t)row * (row + 1)) / 2]);
+  if (row <= col) return up64(X[mat * NP + row + ((int64_t)col * (col + 1)) / 2]);
+  return zconj(up64(X[mat * NP + col + ((int64_t)row * (row + 1)) / 2]));
 }
 
 // element (row, col) of mat `mat` after the TB epilogue (mats 0..9 as above)
-__device__ inline zd load_tb(const zd* X, int packed, int N, int row, int col, int mat,
+template <class HT>
+__device__ inline zd load_tb(const HT* X, int packed, int N, int row, int col, int mat,
                              const TBPhase& tb) {
   zd x = load_w(X, N, packed, row, col, mat);
   if (!tb.tau) return x;
@@ -81,7 +86,8 @@ __device__ inline zd load_tb(const zd* X, int packed, int N, int row, int col, i
 // FP64 MFMA, 4 waves), or V_a = dH_a without the gauge transformation.
 // V is written planar: V[k][a][i + n*j].
 // --------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void velo_mfma_kernel(int n, int64_t nk, const zd* __restrict__ Hk,
+template <class HT>
+__global__ __launch_bounds__(256) void velo_mfma_kernel(int n, int64_t nk, const HT* __restrict__ Hk,
                                                         int64_t strideHk, int packed, TBPhase tb0,
                                                         int gauge_trafo, const zd* __restrict__ U,
                                                         zd* __restrict__ V) {
@@ -91,7 +97,7 @@ __global__ __launch_bounds__(256) void velo_mfma_kernel(int n, int64_t nk, const
   const int n2 = n * n;
   for (int64_t k = blockIdx.x; k < nk; k += gridDim.x) {
     const TBPhase tb = tb0.at(k);
-    const zd* X = Hk + k * strideHk;
+    const HT* X = Hk + k * strideHk;
     zd* Vk = V + k * 3 * (int64_t)n2;
     if (gauge_trafo)
       for (int idx = t; idx < n2; idx += 256) Us[idx % n + LD * (idx / n)] = U[k * (int64_t)n2 + idx];
@@ -314,6 +320,18 @@ void launch_velo(hipStream_t s, const VeloArgs& va, int64_t nk, const zd* Hk, in
                        va.gauge_trafo, va.textbook, va.kubo_tol, U, E, V, A, Om);
     return;
   }
-  hipLaunchKernelGGL(velo_mfma_kernel, dim3(grid), dim3(256), 0, s, va.n, nk, Hk, strideHk,
+  hipLaunchKernelGGL(velo_mfma_kernel<zd>, dim3(grid), dim3(256), 0, s, va.n, nk, Hk, strideHk,
+                     va.packed, tb, va.gauge_trafo, U, V);
+}
+
+void launch_velo_h32(hipStream_t s, const VeloArgs& va, int64_t nk, const zf* Hk,
+                     int64_t strideHk, const KGeom& g, const zd* U, zd* V) {
+  unsigned grid = (unsigned)(nk < 65536 ? nk : 65536);
+  TBPhase tb;
+  tb.kx = tb.ky = tb.kz = 0.0;
+  tb.kpts = g.kpts;
+  tb.tau = g.tau;
+  for (int i = 0; i < 9; ++i) tb.cm[i] = g.cmat[i];
+  hipLaunchKernelGGL(velo_mfma_kernel<zf>, dim3(grid), dim3(256), 0, s, va.n, nk, Hk, strideHk,
                      va.packed, tb, va.gauge_trafo, U, V);
 }

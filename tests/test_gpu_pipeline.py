@@ -245,3 +245,34 @@ def test_femn_sigma_xy_matches_oracle_32():
     assert _rel(gout["ahcVELO"], oout["ahcVELO"]) < 1e-9
     # the FeMn 3Q state has alpha = 0 by symmetry: both sides are rounding noise
     assert np.max(np.abs(gout["mep_tens"] - oout["mep_tens"])) < 1e-12
+
+
+def test_mixed_precision_close_to_fp64():
+    """MEP_PREC_MIXED (FP32 Fourier GEMM and eigensolve, FP64 refinement,
+    velocities and sums) against the FP64 path on the same mesh.  The FP32
+    Fourier sum limits the agreement to ~1e-6 relative on this model; the
+    refined eigenvalues are FP64 eigenvalues of the FP32 H(k)."""
+    from mepinterp_amd import capi, host, models
+    m = models.make_random(seed=3, n=16, L=1)
+    mp = (6, 6, 6)
+    p = dict(do_ahc=True, do_kubo=True, do_mep=True, n_eF=1, n_hw=1, T_kelvin=300.0,
+             eta=0.004, valence_bands=8)
+    with _engine(m) as eng:
+        r64 = eng.run_mesh(mp, host.run_params(p))
+        rp = host.run_params(p)
+        rp.precision = capi.MEP_PREC_MIXED
+        r32 = eng.run_mesh(mp, rp)
+    for key in ("ahc", "ahc_velo", "ohc", "kubo_ic", "kubo_lc", "niu_ic", "niu_lc"):
+        assert _rel(r32[key], r64[key]) < 2e-5, f"{key}: {_rel(r32[key], r64[key]):.2e}"
+    assert _rel(r32["Ne"], r64["Ne"]) < 1e-8
+
+
+def test_mixed_precision_rejects_position_operator():
+    from mepinterp_amd import capi, host, models
+    m = models.make_c1(seed=1)
+    p = dict(do_ahc=True, n_eF=1, n_hw=1, T_kelvin=300.0, valence_bands=4)
+    with _engine(m) as eng:
+        rp = host.run_params(p)
+        rp.precision = capi.MEP_PREC_MIXED
+        with pytest.raises(capi.MepError):
+            eng.run_mesh((2, 2, 2), rp)
