@@ -270,6 +270,73 @@ def write_input_cfg(path: str, p: dict):
 
 
 # --------------------------------------------------------------------------
+# band-structure mode files (file_io.f90:57-143, 146-185, 451-478)
+# --------------------------------------------------------------------------
+
+def read_kpts(path: str):
+    """./kpts as written by kptsgen.pl: first line "num_kpts <x>", then one
+    line "kx ky kz weight" per k-point (read_kptsgen_pl_file) -> (3, nk)."""
+    with open(path) as f:
+        first = f.readline().split()
+        nk = int(first[0])
+        k = np.zeros((3, nk))
+        for i in range(nk):
+            v = f.readline().split()
+            k[:, i] = [float(x) for x in v[:3]]
+    return k
+
+
+def write_kpts(path: str, kpts):
+    k = np.asarray(kpts, dtype=np.float64)
+    with open(path, "w") as f:
+        f.write(f"{k.shape[1]} 1.0\n")
+        for i in range(k.shape[1]):
+            f.write(f"{float(k[0, i])!r} {float(k[1, i])!r} {float(k[2, i])!r} 1.0\n")
+
+
+def write_en_binary(path: str, e):
+    """raw/enK.<k>: the band energies as raw little-endian doubles."""
+    np.asarray(e, dtype="<f8").tofile(path)
+
+
+def read_en_binary(path: str):
+    return np.fromfile(path, dtype="<f8")
+
+
+def write_en_global(path: str, kpts, e):
+    """out/eBands.dat (write_en_global): three header lines, then per k and
+    band "k-index  kx ky kz  E" with formats (i7,a) (3f16.8) (a,f18.8)."""
+    k = np.asarray(kpts)
+    with open(path, "w") as f:
+        f.write(" # energies interpolated by MEPinterp program, written by mepInterp (MI355X)\n")
+        f.write(" # first 3 columns give the relative k-mesh, 4th column are the enegies\n")
+        f.write(" # Kpt_idx  K_x (frac)       K_y (frac)        K_z (frac)       Energy (Hartree)  \n")
+        for q in range(k.shape[1]):
+            for band in range(e.shape[0]):
+                f.write(f"{q + 1:7d}\t" + "".join(f"{k[x, q]:16.8f}" for x in range(3)) +
+                        f"\t{e[band, q]:18.8f}\n")
+
+
+def read_en_global(path: str):
+    """-> (k-index, kpts (nk*nb, 3), E) rows of out/eBands.dat."""
+    rows = [ln.split() for ln in open(path) if ln.strip() and not ln.lstrip().startswith("#")]
+    a = np.array([[float(x) for x in r] for r in rows])
+    return a[:, 0].astype(int), a[:, 1:4], a[:, 4]
+
+
+def read_velo(path: str, n: int):
+    """out//velo/velo_Vka.<k> (write_velo) -> V (3, n, n) complex."""
+    V = np.zeros((3, n, n), np.complex128)
+    for ln in open(path):
+        t = ln.split()
+        if not t or t[0].startswith("#"):
+            continue
+        x, i, j = int(t[0]) - 1, int(t[1]) - 1, int(t[2]) - 1
+        V[x, i, j] = float(t[3]) + 1j * float(t[4])
+    return V
+
+
+# --------------------------------------------------------------------------
 # NPY v2.0 as written by NPY-for-Fortran (fixture header bytes)
 # --------------------------------------------------------------------------
 

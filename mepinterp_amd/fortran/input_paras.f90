@@ -239,8 +239,23 @@ contains
     init_parameters = .true.
   end function init_parameters
 
+  ! mkdir -p through the C library (no child process)
   subroutine my_mkdir(dir)
+    use iso_c_binding, only: c_int, c_char, c_null_char
     character(len=*), intent(in) :: dir
-    call execute_command_line("mkdir -p ./" // dir)
+    interface
+      integer(c_int) function c_mkdir(path, mode) bind(C, name="mkdir")
+        import :: c_int, c_char
+        character(kind=c_char), intent(in) :: path(*)
+        integer(c_int), value :: mode
+      end function c_mkdir
+    end interface
+    character(len=:), allocatable :: d
+    integer :: i
+    integer(c_int) :: rc
+    d = "./" // trim(dir)
+    do i = 2, len(d)
+      if (d(i:i) == "/" .or. i == len(d)) rc = c_mkdir(d(1:i) // c_null_char, int(o'755', c_int))
+    end do
   end subroutine my_mkdir
 end module input_paras

@@ -10,6 +10,8 @@ same steps, same names, same outputs.
 * ``write_outputs``         the NPY writers of file_io.f90:186-359
 * ``core_worker``           core.f90:79-298: input.cfg -> w90files/<seed>_hr.dat
                             -> device k-loop -> out/*.npy
+* ``band_worker``           band_calc.f90:30-121: ./kpts -> raw/enK.*,
+                            out/eBands.dat (and out//velo/velo_Vka.*)
 * ``get_wann_interp`` ...   per-k functions with the reference's argument
                             order, on the GPU (nk = 1 batches)
 """
@@ -20,7 +22,8 @@ import os
 import numpy as np
 
 from . import capi
-from .formats import AU_TO_EV, read_hr_dat, read_input_cfg, read_r_dat, write_npy
+from .formats import (AU_TO_EV, read_hr_dat, read_input_cfg, read_kpts, read_r_dat,
+                      write_en_binary, write_en_global, write_npy)
 
 
 def unit_vol(a_latt):
@@ -138,6 +141,26 @@ def core_worker(run_dir: str = "."):
     out = finalize(sums, p, cfg["mp_grid"], cfg["a_latt"])
     write_outputs(out, os.path.join(run_dir, "out"), p)
     return out
+
+
+def band_worker(run_dir: str = "."):
+    """band_calc.f90:30-121 on the GPU: interpolate the k-points of ./kpts in
+    one batch, write raw/enK.<k> and out/eBands.dat."""
+    cfg = read_input_cfg(os.path.join(run_dir, "input.cfg"))
+    base = os.path.join(run_dir, "w90files", cfg["seed_name"])
+    R, H, _ = read_hr_dat(base + "_hr.dat", R_float=cfg["R_vect_float"])
+    r = read_r_dat(base + "_r.dat", R, cfg["R_vect_float"]) if os.path.exists(base + "_r.dat") else None
+    k = read_kpts(os.path.join(run_dir, "kpts"))
+    with capi.Engine(R, H, r=r, wf_centers=cfg["wf_centers"], a_latt=cfg["a_latt"],
+                     kubo_tol=cfg["kubo_tol"], gauge_trafo=cfg["do_gauge_trafo"],
+                     cart_velo=cfg["use_cart_velo"]) as eng:
+        e, V, _, _ = eng.interp(k, velo=r is not None)
+    os.makedirs(os.path.join(run_dir, "raw"), exist_ok=True)
+    os.makedirs(os.path.join(run_dir, "out"), exist_ok=True)
+    for q in range(k.shape[1]):
+        write_en_binary(os.path.join(run_dir, "raw", f"enK.{q + 1:07d}"), e[:, q])
+    write_en_global(os.path.join(run_dir, "out", "eBands.dat"), k, e)
+    return k, e
 
 
 # ---------------------------------------------------------------------------

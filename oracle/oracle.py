@@ -63,7 +63,13 @@ def lib():
 
 
 def _p(a):
-    return a.ctypes.data_as(c_dp) if a is not None else None
+    """Pointer to an array's data; strided views are refused (the C side
+    reads contiguous memory)."""
+    if a is None:
+        return None
+    if not (a.flags.c_contiguous or a.flags.f_contiguous):
+        raise ValueError("oracle: array argument must be contiguous")
+    return a.ctypes.data_as(c_dp)
 
 
 class Model:
@@ -151,7 +157,7 @@ def ft_R_to_k(model: Model, kpt, grad=True, pos=None):
     pos = (model.r is not None) if pos is None else pos
     A = np.zeros((3, n, n), np.complex128, order="F") if pos else None
     Om = np.zeros((3, 3, n, n), np.complex128, order="F") if pos else None
-    k = np.asarray(kpt, dtype=np.float64)
+    k = np.ascontiguousarray(kpt, dtype=np.float64)
     lib().orc_ft_R_to_k(C.byref(mc), _p(k), _p(Hk), _p(Hka), _p(A), _p(Om))
     return Hk, Hka, A, Om
 
@@ -165,7 +171,7 @@ def get_wann_interp(model: Model, kpt, velo=True):
     pos = model.r is not None
     A = np.zeros((3, n, n), np.complex128, order="F") if pos else None
     Om = np.zeros((3, 3, n, n), np.complex128, order="F") if pos else None
-    k = np.asarray(kpt, dtype=np.float64)
+    k = np.ascontiguousarray(kpt, dtype=np.float64)
     info = lib().orc_get_wann_interp(C.byref(mc), _p(k), _p(e), _p(V), _p(A), _p(Om))
     if info:
         raise RuntimeError(f"eigensolver failed, info={info}")

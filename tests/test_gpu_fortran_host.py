@@ -124,3 +124,33 @@ def test_fortran_per_k_interface(tmp_path, kind):
                 else got[key][..., ik]
             scale = max(np.abs(w).max(), 1e-300)
             assert np.max(np.abs(g - w.reshape(g.shape, order="F"))) <= 1e-8 * scale, (key, ik)
+
+
+def test_mepinterp_band_structure_mode(tmp_path):
+    """jobs%plot_bands: ./kpts -> raw/enK.*, out/eBands.dat and (with
+    jobs%do_write_velo) out//velo/velo_Vka.* against the oracle."""
+    import oracle as O
+    from mepinterp_amd import formats as F
+    from mepinterp_amd import models
+    exe = _need("mepInterp")
+    tmp = str(tmp_path)
+    os.makedirs(os.path.join(tmp, "w90files"))
+    m = models.make_random(seed=31, n=10, L=1, tb=True)
+    F.write_hr_dat(os.path.join(tmp, "w90files", "bnd_hr.dat"), m["R_frac"], m["H"])
+    k = np.stack([np.linspace(-0.5, 0.5, 17), np.linspace(0.0, 0.3, 17), np.zeros(17)])
+    F.write_kpts(os.path.join(tmp, "kpts"), k)
+    F.write_input_cfg(os.path.join(tmp, "input.cfg"), dict(
+        plot_bands=True, do_write_velo=True, a_latt=m["a_latt"], seed_name="bnd",
+        wf_centers=m["wf_centers"], mp_grid=(1, 1, 1), T_kelvin=0.0, valence_bands=5))
+    r = subprocess.run([exe], cwd=tmp, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    R, H, _ = F.read_hr_dat(os.path.join(tmp, "w90files", "bnd_hr.dat"))
+    om = O.Model(R, H, wf_centers=m["wf_centers"], a_latt=m["a_latt"])
+    idx, kk, ee = F.read_en_global(os.path.join(tmp, "out", "eBands.dat"))
+    for q in range(k.shape[1]):
+        eo, Vo, _, _ = O.get_wann_interp(om, k[:, q])
+        e = F.read_en_binary(os.path.join(tmp, "raw", f"enK.{q + 1:07d}"))
+        assert np.max(np.abs(e - eo)) < 1e-10 / AU_TO_EV
+        assert np.allclose(ee[idx == q + 1], eo, atol=6e-9)
+        V = F.read_velo(os.path.join(tmp, "out", "velo", f"velo_Vka.{q + 1:07d}"), 10)
+        assert np.max(np.abs(V - Vo)) < 1e-7
