@@ -44,6 +44,9 @@ extern "C" {
 #define MEP_JOB_AHC 1            /* jobs%do_ahc : kubo_ahc_tens, velo_ahc_tens, kubo_ohc_tens */
 #define MEP_JOB_KUBO 2           /* jobs%do_kubo: kubo_mep_IC/LC/CS          */
 #define MEP_JOB_MEP 4            /* jobs%do_mep : mep_niu_IC/LC/CS           */
+#define MEP_JOB_OPT 8            /* jobs%do_opt : kubo_opt_tens (needs the position
+                                    operator) and photo_2nd_cond            */
+#define MEP_JOB_GYRO 16          /* jobs%do_gyro: get_gyro_C / D / Dw         */
 
 /* arithmetic of the k-space pipeline */
 #define MEP_PREC_FP64 0          /* FP64 MFMA Fourier GEMM, FP64 eigensolve  */
@@ -81,6 +84,12 @@ typedef struct {
   double *niu_lc;      /* (3,3,valence_bands) sum_k mep_niu_LC              */
   double *niu_cs;      /* (3,3,valence_bands) sum_k mep_niu_CS              */
   int64_t n_k;         /* out: number of k-points summed                    */
+  double *opt_s;       /* (2,3,3,n_hw,n_eF)   sum_k kubo_opt_tens opt_symm  */
+  double *opt_a;       /* (2,3,3,n_hw,n_eF)   sum_k kubo_opt_tens opt_asymm */
+  double *photo;       /* (2,3,3,3,n_hw,n_eF) sum_k photo_2nd_cond          */
+  double *gyro_C;      /* (2,3,3,n_eF)        sum_k get_gyro_C              */
+  double *gyro_D;      /* (2,3,3,n_eF)        sum_k get_gyro_D              */
+  double *gyro_Dw;     /* (2,3,3,n_hw,n_eF)   sum_k get_gyro_Dw             */
 } mep_sums;
 
 /* Upload the real-space model once.  Replaces the model half of

@@ -22,6 +22,8 @@ MEP_F_APPLY_DEG = 8
 MEP_JOB_AHC = 1
 MEP_JOB_KUBO = 2
 MEP_JOB_MEP = 4
+MEP_JOB_OPT = 8
+MEP_JOB_GYRO = 16
 MEP_PREC_FP64 = 0
 MEP_PREC_MIXED = 1
 
@@ -43,7 +45,9 @@ class RunParams(C.Structure):
 class Sums(C.Structure):
     _fields_ = [("Ne", c_dp), ("ahc", c_dp), ("ahc_velo", c_dp), ("ohc", c_dp),
                 ("kubo_ic", c_dp), ("kubo_lc", c_dp), ("kubo_cs", c_dp),
-                ("niu_ic", c_dp), ("niu_lc", c_dp), ("niu_cs", c_dp), ("n_k", C.c_int64)]
+                ("niu_ic", c_dp), ("niu_lc", c_dp), ("niu_cs", c_dp), ("n_k", C.c_int64),
+                ("opt_s", c_dp), ("opt_a", c_dp), ("photo", c_dp), ("gyro_C", c_dp),
+                ("gyro_D", c_dp), ("gyro_Dw", c_dp)]
 
 
 _lib = None
@@ -176,7 +180,13 @@ class Engine:
                "kubo_cs": np.zeros((3, 3, nef), order="F"),
                "niu_ic": np.zeros((3, 3, nv), order="F"),
                "niu_lc": np.zeros((3, 3, nv), order="F"),
-               "niu_cs": np.zeros((3, 3, nv), order="F")}
+               "niu_cs": np.zeros((3, 3, nv), order="F"),
+               "opt_s": np.zeros((3, 3, nhw, nef), np.complex128, order="F"),
+               "opt_a": np.zeros((3, 3, nhw, nef), np.complex128, order="F"),
+               "photo": np.zeros((3, 3, 3, nhw, nef), np.complex128, order="F"),
+               "gyro_C": np.zeros((3, 3, nef), np.complex128, order="F"),
+               "gyro_D": np.zeros((3, 3, nef), np.complex128, order="F"),
+               "gyro_Dw": np.zeros((3, 3, nhw, nef), np.complex128, order="F")}
         s = Sums()
         for k in out:
             setattr(s, k, _p(out[k]))

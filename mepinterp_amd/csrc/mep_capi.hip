@@ -493,6 +493,8 @@ static void linspace(double lo, double hi, int n, std::vector<double>& out) {
 struct BandSetup {
   BandArgs b;
   AccLayout L;
+  ExtLayout X;  // optical / photogalvanic / gyrotropic records (when asked)
+  bool ext = false;
   int nblocks;
 };
 
@@ -537,19 +539,24 @@ static int band_setup(mep_ctx* c, const mep_run_params* p, int64_t kblock, BandS
     return fail(MEP_ERR_ARG, "band-sum accumulators exceed the 160 KiB LDS of one workgroup "
                              "(reduce N_hw * N_eF)");
   bs.L = acc_layout(b);
+  bs.ext = (p->jobs & (MEP_JOB_OPT | MEP_JOB_GYRO)) != 0;
+  bs.X = ext_layout(b);
+  if (bs.ext && (c->n > 32 || p->n_eF > 16))
+    return fail(MEP_ERR_ARG, "optical / gyrotropic sums: n_wf <= 32 and n_eF <= 16");
+  const int64_t xtot = bs.ext ? bs.X.total : 0;
   bs.nblocks = (int)(kblock < 1024 ? kblock : 1024);
-  int64_t need = (int64_t)bs.nblocks * bs.L.total;
+  int64_t need = (int64_t)bs.nblocks * (bs.L.total + xtot);
   if (need > c->part_cap) {
     hipFree(c->dpart);
     HIPCHK(hipMalloc(&c->dpart, sizeof(double) * need));
     c->part_cap = need;
   }
-  if (bs.L.total > c->acc_cap) {
+  if (bs.L.total + xtot > c->acc_cap) {
     hipFree(c->dacc);
-    HIPCHK(hipMalloc(&c->dacc, sizeof(double) * bs.L.total));
-    c->acc_cap = bs.L.total;
+    HIPCHK(hipMalloc(&c->dacc, sizeof(double) * (bs.L.total + xtot)));
+    c->acc_cap = bs.L.total + xtot;
   }
-  launch_fill_zero(c->st, c->dacc, bs.L.total);
+  launch_fill_zero(c->st, c->dacc, bs.L.total + xtot);
   launch_fill_zero(c->st, c->dpart, need);
   HIPCHK(hipGetLastError());
   return MEP_OK;
@@ -559,6 +566,9 @@ static int band_block(mep_ctx* c, BandSetup& bs, int64_t nb, const double* w) {
   int nblk = (int)(nb < bs.nblocks ? nb : bs.nblocks);
   timed(c, K_BANDSUM, [&] {
     launch_bandsum(c->st, bs.b, nb, c->dE, c->dV, c->dA, c->dOm, w, nblk, c->dpart);
+    if (bs.ext)
+      launch_extsum(c->st, bs.b, nb, c->dE, c->dV, c->dA, c->dOm, w, nblk,
+                    c->dpart + (int64_t)bs.nblocks * bs.L.total);
   });
   HIPCHK(hipGetLastError());
   return MEP_OK;
@@ -569,10 +579,14 @@ static int unpack(mep_ctx* c, const BandSetup& bs, const mep_run_params* p, int6
   // the partial records accumulate over every k-block of the run
   timed(c, K_REDUCE, [&] {
     launch_reduce_parts(c->st, bs.nblocks, bs.L.total, c->dpart, c->dacc);
+    if (bs.ext)
+      launch_reduce_parts(c->st, bs.nblocks, bs.X.total, c->dpart + (int64_t)bs.nblocks * bs.L.total,
+                          c->dacc + bs.L.total);
   });
   HIPCHK(hipGetLastError());
-  std::vector<double> h(bs.L.total);
-  HIPCHK(hipMemcpyAsync(h.data(), c->dacc, sizeof(double) * bs.L.total, hipMemcpyDeviceToHost,
+  const int64_t xtot = bs.ext ? bs.X.total : 0;
+  std::vector<double> h(bs.L.total + xtot);
+  HIPCHK(hipMemcpyAsync(h.data(), c->dacc, sizeof(double) * (bs.L.total + xtot), hipMemcpyDeviceToHost,
                         c->st));
   HIPCHK(hipStreamSynchronize(c->st));
   resolve_times(c);
@@ -592,6 +606,16 @@ static int unpack(mep_ctx* c, const BandSetup& bs, const mep_run_params* p, int6
     cp(out->niu_ic, bs.L.nic, 9 * nv);
     cp(out->niu_lc, bs.L.nlc, 9 * nv);
     cp(out->niu_cs, bs.L.ncs, 9 * nv);
+  }
+  if (p->jobs & MEP_JOB_OPT) {
+    cp(out->opt_s, bs.L.total + bs.X.opt_s, 18 * nhw * nef);
+    cp(out->opt_a, bs.L.total + bs.X.opt_a, 18 * nhw * nef);
+    cp(out->photo, bs.L.total + bs.X.photo, 54 * nhw * nef);
+  }
+  if (p->jobs & MEP_JOB_GYRO) {
+    cp(out->gyro_C, bs.L.total + bs.X.gyro_C, 18 * nef);
+    cp(out->gyro_D, bs.L.total + bs.X.gyro_D, 18 * nef);
+    cp(out->gyro_Dw, bs.L.total + bs.X.gyro_Dw, 18 * nhw * nef);
   }
   out->n_k = nk;
   return MEP_OK;

@@ -1,0 +1,218 @@
+// mep_extsum.hip -- the optical, photogalvanic and gyrotropic band sums of
+// the k-loop (core.f90:235-253), accumulated into per-workgroup partial
+// records like the main band sums.
+//   kubo_opt_tens        kubo.f90:204-405   (wann guide 12.10-12.15; needs A)
+//   photo_2nd_cond       photo.f90:16-86    (Yang et al. PRB 97 241118 Eq. 1)
+//   get_gyro_C / D / Dw  gyro.f90:153-300   (Souza et al. PRB 97 035158)
+//   d1_fd_stat_deriv     statistics.f90:160-196 (1/(2 + 2 cosh) of the
+//                        occupation itself, as the reference evaluates it)
+// One k-point at a time per 256-thread workgroup; every output element is
+// owned by one thread, which adds its per-k value to the workgroup's record
+// (fixed order, deterministic).  V is staged in LDS (planar, padded), A and
+// Omega are read in the reference layout (position operator only).
+#include "mep_common.hpp"
+#include "mep_kernels.hpp"
+#include "../../include/mepinterp.h"
+
+ExtLayout ext_layout(const BandArgs& b) {
+  ExtLayout X;
+  const int64_t hf = (int64_t)b.n_hw * b.n_ef;
+  int64_t o = 0;
+  X.opt_s = o;   o += 18 * hf;
+  X.opt_a = o;   o += 18 * hf;
+  X.photo = o;   o += 54 * hf;
+  X.gyro_C = o;  o += 18 * (int64_t)b.n_ef;
+  X.gyro_D = o;  o += 18 * (int64_t)b.n_ef;
+  X.gyro_Dw = o; o += 18 * hf;
+  X.total = o;
+  return X;
+}
+
+constexpr int XS_THREADS = 256;
+constexpr int XS_NMAX = 32;
+constexpr int XS_LD = XS_NMAX + 1;
+constexpr int XS_PL = XS_NMAX * XS_LD;
+
+__device__ inline int levi3(int a, int b, int c) {
+  return (a == b || b == c || a == c) ? 0 : (((b - a + 3) % 3 == 1) ? 1 : -1);
+}
+
+__global__ __launch_bounds__(XS_THREADS) void extsum_kernel(BandArgs b, double Tsmear, ExtLayout X,
+                                                           int64_t nk, const double* __restrict__ E,
+                                                           const zd* __restrict__ V,
+                                                           const zd* __restrict__ A,
+                                                           const zd* __restrict__ Om,
+                                                           const double* __restrict__ w,
+                                                           double* __restrict__ part) {
+  __shared__ zd Vs[3 * XS_PL];
+  __shared__ double Es[XS_NMAX];
+  __shared__ double fd[XS_NMAX * 16];   // (n_ef <= 16, band)
+  __shared__ double fdd[XS_NMAX * 16];  // fd_stat_deriv
+  __shared__ zd omg[3 * XS_NMAX];       // omega_c(n) of get_gyro_D
+  const int n = b.n, nef = b.n_ef, nhw = b.n_hw, n2 = n * n;
+  const int t = threadIdx.x;
+  const bool finiteT = b.T_kelvin > 1e-2;
+  const bool doOpt = (b.jobs & MEP_JOB_OPT) != 0, doGyro = (b.jobs & MEP_JOB_GYRO) != 0;
+  const bool pos = b.pos_op && A && Om;
+  const double pi = 3.14159265358979323846;
+  const double eta = b.eta;
+  double* rec = part + (int64_t)blockIdx.x * X.total;
+  auto Aat = [&](int64_t k, int a, int i, int j) {  // A_a(i, j) of k, reference layout
+    return A[k * 3 * (int64_t)n2 + a + 3 * ((int64_t)i + (int64_t)n * j)];
+  };
+  for (int64_t k = blockIdx.x; k < nk; k += gridDim.x) {
+    const double wk = w ? w[k] : 1.0;
+    __syncthreads();
+    const zd* Vk = V + k * 3 * (int64_t)n2;
+    for (int idx = t; idx < 3 * n2; idx += XS_THREADS) {
+      const int a = idx / n2, r = idx - a * n2;
+      Vs[a * XS_PL + r % n + XS_LD * (r / n)] = Vk[idx];
+    }
+    if (t < n) Es[t] = E[k * n + t];
+    __syncthreads();
+    for (int idx = t; idx < nef * n; idx += XS_THREADS) {
+      double f;
+      if (b.fd_in) {
+        f = b.fd_in[k * (int64_t)nef * n + idx];
+      } else {
+        const int ef = idx % nef, nb = idx / nef;
+        const double e = Es[nb], ff = b.ef[ef];
+        f = finiteT ? 1.0 / (1.0 + exp((e - ff) / Tsmear)) : ((e < ff) ? 1.0 : 0.0);
+      }
+      fd[idx] = f;
+      fdd[idx] = 1.0 / (2.0 + 2.0 * cosh(f));
+    }
+    // omega_c(n) = sum_ab eps_abc curv_ab(n): Omega_ab(n,n) with a position
+    // operator, else get_kubo_curv (wann_interp.f90:113-146)
+    if (doGyro && t < n) {
+      zd curv[9];
+      if (pos) {
+        const zd* On = Om + k * 9 * (int64_t)n2 + 9 * ((int64_t)t + (int64_t)n * t);
+        for (int q = 0; q < 9; ++q) curv[q] = On[q];
+      } else {
+        for (int q = 0; q < 9; ++q) curv[q] = zd{0.0, 0.0};
+        for (int np = 0; np < n; ++np) {
+          if (np == t) continue;
+          const double dEE = (Es[np] - Es[t]) * (Es[np] - Es[t]);
+          if (fabs(dEE) < b.kubo_tol) continue;
+          for (int j = 0; j < 3; ++j)
+            for (int a = 0; a < 3; ++a)
+              curv[a + 3 * j].re +=
+                  zmul(Vs[a * XS_PL + t + XS_LD * np], Vs[j * XS_PL + np + XS_LD * t]).im / dEE;
+        }
+        for (int q = 0; q < 9; ++q) curv[q] = zscale(curv[q], -2.0);
+      }
+      for (int c = 0; c < 3; ++c) {
+        zd s = zd{0.0, 0.0};
+        for (int bb = 0; bb < 3; ++bb)
+          for (int a = 0; a < 3; ++a) {
+            const int l = levi3(a, bb, c);
+            if (l) s = zadd(s, zscale(curv[a + 3 * bb], (double)l));
+          }
+        omg[c + 3 * t] = s;
+      }
+    }
+    __syncthreads();
+    // ---- kubo_opt_tens: item (a, b, hw, ef) ----------------------------------
+    if (doOpt && pos)
+      for (int it = t; it < 9 * nhw * nef; it += XS_THREADS) {
+        const int q = it % 9, h = (it / 9) % nhw, ef = it / (9 * nhw);
+        const int a = q % 3, bb = q / 3;
+        zd herm = zd{0.0, 0.0}, aherm = zd{0.0, 0.0};
+        const double hwv = b.hw[h];
+        for (int m = 0; m < n; ++m)
+          for (int nn = 0; nn < n; ++nn) {
+            if (nn == m) continue;
+            const double dE = Es[m] - Es[nn];
+            const double df = fd[ef + nef * m] - fd[ef + nef * nn];
+            const zd amn = zscale(zmul(Aat(k, a, nn, m), Aat(k, bb, m, nn)), dE);
+            // delta_broad: Im 1/(pi (dE - i eta) - pi hw)
+            const zd dsm = zd{pi * (dE - hwv), -pi * eta};
+            const double dbr = -dsm.im / zabs2(dsm);
+            herm = zsub(herm, zscale(amn, pi * dbr * df));
+            aherm = zadd(aherm, zmuli(zscale(amn, df / (dE - hwv))));
+          }
+        const int64_t o = 2 * (q + 9 * (h + (int64_t)nhw * ef));
+        rec[X.opt_s + o] += wk * herm.re;       // (12.14) Re herm + i Im aherm
+        rec[X.opt_s + o + 1] += wk * aherm.im;
+        rec[X.opt_a + o] += wk * aherm.re;      // (12.15) Re aherm + i Im herm
+        rec[X.opt_a + o + 1] += wk * herm.im;
+      }
+    // ---- photo_2nd_cond: item (a, b, c, hw, ef) ------------------------------
+    if (doOpt)
+      for (int it = t; it < 27 * nhw * nef; it += XS_THREADS) {
+        const int q = it % 27, h = (it / 27) % nhw, ef = it / (27 * nhw);
+        const int a = q % 3, bb = (q / 3) % 3, c = q / 9;
+        const double hwv = b.hw[h];
+        zd acc = zd{0.0, 0.0};
+        for (int nn = 0; nn < n; ++nn)
+          for (int m = 0; m < n; ++m) {
+            const zd dEnm = zd{Es[nn] - Es[m], -eta};
+            for (int l = 0; l < n; ++l) {
+              if (l == nn) continue;
+              const double df = fd[ef + nef * l] - fd[ef + nef * nn];
+              const zd vvv = zmul(Vs[a * XS_PL + nn + XS_LD * l],
+                                  zmul(Vs[bb * XS_PL + l + XS_LD * m], Vs[c * XS_PL + m + XS_LD * nn]));
+              zd g = zd{0.0, 0.0};
+              for (int om = -1; om <= 1; om += 2) {
+                const zd dEhw = zd{Es[nn] - Es[l] + om * hwv, -2.0 * eta};
+                g = zadd(g, zdiv(vvv, zmul(dEnm, dEhw)));
+              }
+              acc = zadd(acc, zscale(g, df / (hwv * hwv)));
+            }
+          }
+        const int64_t o = 2 * (q + 27 * (h + (int64_t)nhw * ef));
+        rec[X.photo + o] += wk * acc.re;
+        rec[X.photo + o + 1] += wk * acc.im;
+      }
+    // ---- gyrotropic C, D: item (a, b, ef); Dw: item (a, b, hw, ef) -----------
+    if (doGyro) {
+      for (int it = t; it < 9 * nef; it += XS_THREADS) {
+        const int q = it % 9, ef = it / 9, a = q % 3, bb = q / 3;
+        zd c = zd{0.0, 0.0}, d = zd{0.0, 0.0};
+        for (int nn = 0; nn < n; ++nn) {
+          const zd va = Vs[a * XS_PL + nn + XS_LD * nn], vb = Vs[bb * XS_PL + nn + XS_LD * nn];
+          const double f = fdd[ef + nef * nn];
+          c = zadd(c, zscale(zmul(va, vb), f));
+          d = zsub(d, zscale(zmul(va, omg[bb + 3 * nn]), f));
+        }
+        rec[X.gyro_C + 2 * it] += wk * c.re;
+        rec[X.gyro_C + 2 * it + 1] += wk * c.im;
+        rec[X.gyro_D + 2 * it] += wk * d.re;
+        rec[X.gyro_D + 2 * it + 1] += wk * d.im;
+      }
+      if (pos)
+        for (int it = t; it < 9 * nhw * nef; it += XS_THREADS) {
+          const int q = it % 9, h = (it / 9) % nhw, ef = it / (9 * nhw);
+          const int a = q % 3, bb = q / 3;
+          const double hwv = b.hw[h];
+          zd dw = zd{0.0, 0.0};
+          for (int nn = 0; nn < n; ++nn) {
+            // get_curvTilde_hw (gyro.f90:271-300), component bb
+            double omt = 0.0;
+            for (int m = 0; m < n; ++m) {
+              if (m == nn) continue;
+              const double ww = (Es[m] - Es[nn]) * (Es[m] - Es[nn]);
+              const int b1 = (bb + 1) % 3, b2 = (bb + 2) % 3;
+              const zd cr = zsub(zmul(Aat(k, b1, nn, m), Aat(k, b2, m, nn)),
+                                 zmul(Aat(k, b2, nn, m), Aat(k, b1, m, nn)));
+              omt -= cr.im * ww / (ww - hwv);
+            }
+            const zd va = Vs[a * XS_PL + nn + XS_LD * nn];
+            dw = zsub(dw, zscale(va, omt * fdd[ef + nef * nn]));
+          }
+          const int64_t o = 2 * (q + 9 * (h + (int64_t)nhw * ef));
+          rec[X.gyro_Dw + o] += wk * dw.re;
+          rec[X.gyro_Dw + o + 1] += wk * dw.im;
+        }
+    }
+  }
+}
+
+void launch_extsum(hipStream_t s, const BandArgs& b, int64_t nk, const double* E, const zd* V,
+                   const zd* A, const zd* Om, const double* w, int nblocks, double* part) {
+  ExtLayout X = ext_layout(b);
+  const double kB = 8.6173303 * 1e-5 / 27.211385;  // kBoltz_Eh_K (constants.f90:22)
+  hipLaunchKernelGGL(extsum_kernel, dim3(nblocks), dim3(XS_THREADS), 0, s, b, kB * b.T_kelvin, X, nk,
+                     E, V, A, Om, w, part);
+}

@@ -9,12 +9,13 @@ module core
   use mep_capi
   use mep_gpu, only: gpu_model_ctx, make_run_params
   use matrix_math, only: get_linspace
-  use k_space, only: get_mp_grid, normalize_k_int
+  use k_space, only: get_mp_grid, normalize_k_int, k_norm
   use input_paras, only: seed_name, valence_bands, do_gauge_trafo, do_mep, do_ahc, do_kubo, &
                          do_opt, do_gyro, do_write_mep_bands, wf_centers, N_hw, hw_min, hw_max, &
                          N_eF, eF_min, eF_max, T_kelvin, i_eta_smr, out_dir
   use file_io, only: read_tb_basis, write_hw_list, write_mep_bands, write_mep_tensors, &
-                     write_kubo_mep_tensors, write_ahc_tensor
+                     write_kubo_mep_tensors, write_ahc_tensor, write_opt_tensors, &
+                     write_gyro_tensors
   implicit none
   private
   public :: core_worker, fermi_selector
@@ -26,6 +27,8 @@ contains
     real(dp), allocatable, target :: Ne(:), ahc(:, :, :), kic(:, :, :), klc(:, :, :), &
                                      kcs(:, :, :), nic(:, :, :), nlc(:, :, :), ncs(:, :, :)
     complex(dp), allocatable, target :: velo(:, :, :, :), ohc(:, :, :, :)
+    complex(dp), allocatable, target :: opt_s(:, :, :, :), opt_a(:, :, :, :), photo(:, :, :, :, :), &
+                                        gyro_C(:, :, :), gyro_D(:, :, :), gyro_Dw(:, :, :, :)
     type(c_ptr) :: ctx
     type(mep_run_params) :: p
     type(mep_sums) :: s
@@ -37,8 +40,6 @@ contains
 
     call get_linspace(eF_min, eF_max, N_eF, ef_lst)
     call get_linspace(hw_min, hw_max, N_hw, hw_lst)
-    if (do_opt) write(*, '(a)') "[core_worker]: WARNING do_opt is not part of this build; skipped"
-    if (do_gyro) write(*, '(a)') "[core_worker]: WARNING do_gyro is not part of this build; skipped"
     call read_tb_basis(seed_name, R_vect, H_tb, r_tb)
     use_pos = allocated(r_tb)
     call gpu_model_ctx(do_gauge_trafo, H_tb, r_tb, use_pos, R_vect, wf_centers, ctx)
@@ -47,6 +48,8 @@ contains
     if (do_ahc) jobs = ior(jobs, int(MEP_JOB_AHC))
     if (do_kubo) jobs = ior(jobs, int(MEP_JOB_KUBO))
     if (do_mep) jobs = ior(jobs, int(MEP_JOB_MEP))
+    if (do_opt) jobs = ior(jobs, int(MEP_JOB_OPT))
+    if (do_gyro) jobs = ior(jobs, int(MEP_JOB_GYRO))
     nval = max(1, valence_bands)
     p = make_run_params(jobs, N_eF, eF_min, eF_max, N_hw, hw_min, hw_max, T_kelvin, &
                         aimag(i_eta_smr), nval)
@@ -63,6 +66,14 @@ contains
     s%niu_ic = c_loc(nic)
     s%niu_lc = c_loc(nlc)
     s%niu_cs = c_loc(ncs)
+    allocate(opt_s(3, 3, N_hw, N_eF), opt_a(3, 3, N_hw, N_eF), photo(3, 3, 3, N_hw, N_eF), &
+             gyro_C(3, 3, N_eF), gyro_D(3, 3, N_eF), gyro_Dw(3, 3, N_hw, N_eF))
+    s%opt_s = c_loc(opt_s)
+    s%opt_a = c_loc(opt_a)
+    s%photo = c_loc(photo)
+    s%gyro_C = c_loc(gyro_C)
+    s%gyro_D = c_loc(gyro_D)
+    s%gyro_Dw = c_loc(gyro_Dw)
     mp = int(get_mp_grid(), c_int32_t)
     call system_clock(t0, rate)
     rc = mep_run_mesh(ctx, mp, p, s)
@@ -75,6 +86,13 @@ contains
 
     eF_idx = fermi_selector(n_ki_glob, Ne, ef_lst, valence_bands)
     call integrate_over_k_space(eF_idx, nic, nlc, ncs, kic, klc, kcs, ahc, velo, ohc)
+    ! optical tensors at eF_idx, photogalvanic and gyrotropic tensors at every
+    ! E_F, all normalised (core.f90:512-531)
+    if (do_opt) then
+      call write_opt_tensors(opt_s(:, :, :, eF_idx) / k_norm(), opt_a(:, :, :, eF_idx) / k_norm(), &
+                             photo / k_norm())
+    end if
+    if (do_gyro) call write_gyro_tensors(gyro_C / k_norm(), gyro_D / k_norm(), gyro_Dw / k_norm())
   end subroutine core_worker
 
   integer function fermi_selector(n_ki_glob, Ne_loc_sum, eF_lst, Ne_sys) result(opt_idx)

@@ -11,10 +11,11 @@ module file_io
   implicit none
   private
   public :: read_hr_file, read_r_file, read_tb_basis, save_npy, write_hw_list, write_mep_bands, &
-            write_mep_tensors, write_kubo_mep_tensors, write_ahc_tensor
+            write_mep_tensors, write_kubo_mep_tensors, write_ahc_tensor, write_opt_tensors, &
+            write_gyro_tensors
 
   interface save_npy
-    module procedure save_d1, save_d2, save_d3, save_z3, save_z4
+    module procedure save_d1, save_d2, save_d3, save_z3, save_z4, save_z5
   end interface save_npy
 contains
 
@@ -177,6 +178,30 @@ contains
     write(u) a
     close(u)
   end subroutine save_z4
+
+  subroutine save_z5(fname, a)
+    character(len=*), intent(in) :: fname
+    complex(dp), intent(in) :: a(:, :, :, :, :)
+    integer :: u
+    call npy_open(fname, "<c16", shape(a), u)
+    write(u) a
+    close(u)
+  end subroutine save_z5
+
+  ! write_opt_tensors / write_gyro_tensors (file_io.f90:362-398)
+  subroutine write_opt_tensors(s_symm, a_symm, photo_2nd)
+    complex(dp), intent(in) :: s_symm(:, :, :), a_symm(:, :, :), photo_2nd(:, :, :, :, :)
+    call save_npy(out_dir // "opt_Ssymm.npy", s_symm)
+    call save_npy(out_dir // "opt_Asymm.npy", a_symm)
+    call save_npy(out_dir // "photoC_2nd.npy", photo_2nd)
+  end subroutine write_opt_tensors
+
+  subroutine write_gyro_tensors(C_tens, D_tens, Dw_tens)
+    complex(dp), intent(in) :: C_tens(:, :, :), D_tens(:, :, :), Dw_tens(:, :, :, :)
+    call save_npy(out_dir // "gyro_C.npy", C_tens)
+    call save_npy(out_dir // "gyro_D.npy", D_tens)
+    call save_npy(out_dir // "gyro_Dw.npy", Dw_tens)
+  end subroutine write_gyro_tensors
 
   ! ---------------------------------------------------------------- writers
   subroutine write_hw_list(n_hw, hw_min, hw_max)

@@ -6,7 +6,7 @@ module k_space
   implicit none
   private
   public :: set_mp_grid, get_mp_grid, set_recip_latt, get_recip_latt, get_kpt_idx, get_rel_kpt, &
-            get_cart_kpt, get_bz_vol, get_unit_vol, normalize_k_int, kspace_allocator
+            get_cart_kpt, get_bz_vol, get_unit_vol, normalize_k_int, kspace_allocator, k_norm
 
   interface normalize_k_int
     module procedure d2_normalize, d3_normalize, z3_normalize, z4_normalize

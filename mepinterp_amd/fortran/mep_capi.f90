@@ -6,8 +6,9 @@ module mep_capi
   integer(c_int), parameter :: MEP_OK = 0
   integer(c_int32_t), parameter :: MEP_F_GAUGE_TRAFO = 1, MEP_F_CART_VELO = 2, &
                                    MEP_F_TEXTBOOK = 4, MEP_F_APPLY_DEG = 8
-  integer(c_int32_t), parameter :: MEP_JOB_AHC = 1, MEP_JOB_KUBO = 2, MEP_JOB_MEP = 4
-  integer(c_int32_t), parameter :: MEP_PREC_FP64 = 0
+  integer(c_int32_t), parameter :: MEP_JOB_AHC = 1, MEP_JOB_KUBO = 2, MEP_JOB_MEP = 4, &
+                                   MEP_JOB_OPT = 8, MEP_JOB_GYRO = 16
+  integer(c_int32_t), parameter :: MEP_PREC_FP64 = 0, MEP_PREC_MIXED = 1
 
   type, bind(C) :: mep_run_params
     integer(c_int32_t) :: jobs
@@ -26,6 +27,8 @@ module mep_capi
     type(c_ptr) :: kubo_ic = c_null_ptr, kubo_lc = c_null_ptr, kubo_cs = c_null_ptr
     type(c_ptr) :: niu_ic = c_null_ptr, niu_lc = c_null_ptr, niu_cs = c_null_ptr
     integer(c_int64_t) :: n_k = 0
+    type(c_ptr) :: opt_s = c_null_ptr, opt_a = c_null_ptr, photo = c_null_ptr
+    type(c_ptr) :: gyro_C = c_null_ptr, gyro_D = c_null_ptr, gyro_Dw = c_null_ptr
   end type mep_sums
 
   interface

@@ -80,6 +80,13 @@ def finalize(sums, p: dict, mp_grid, a_latt, n_k=None):
         out.update(ahc_tens=sums["ahc"][:, :, e] / norm,
                    ahcVELO=sums["ahc_velo"][:, :, :, e] / norm,
                    ohcVELO=sums["ohc"][:, :, :, e] / norm)
+    if p.get("do_opt"):  # core.f90:512-520: opt tensors at eF_idx, photo at every E_F
+        out.update(opt_Ssymm=sums["opt_s"][:, :, :, e] / norm,
+                   opt_Asymm=sums["opt_a"][:, :, :, e] / norm,
+                   photoC_2nd=sums["photo"] / norm)
+    if p.get("do_gyro"):  # core.f90:523-531: every E_F
+        out.update(gyro_C=sums["gyro_C"] / norm, gyro_D=sums["gyro_D"] / norm,
+                   gyro_Dw=sums["gyro_Dw"] / norm)
     return out
 
 
@@ -104,6 +111,10 @@ def write_outputs(out: dict, out_dir: str, p: dict):
     if "ahcVELO" in out:
         write_npy(os.path.join(out_dir, "ahcVELO.npy"), out["ahcVELO"])
         write_npy(os.path.join(out_dir, "ohcVELO.npy"), out["ohcVELO"])
+    # write_opt_tensors / write_gyro_tensors (file_io.f90:362-398)
+    for key in ("opt_Ssymm", "opt_Asymm", "photoC_2nd", "gyro_C", "gyro_D", "gyro_Dw"):
+        if key in out:
+            write_npy(os.path.join(out_dir, key + ".npy"), out[key])
     with open(os.path.join(out_dir, "occ.dat"), "w") as f:
         for e, n in zip(out["ef_lst"], out["Ne"]):
             f.write(f"{e * AU_TO_EV:16.8f}\t{n:16.8f}\n")
@@ -111,6 +122,7 @@ def write_outputs(out: dict, out_dir: str, p: dict):
 
 def params_from_cfg(cfg: dict) -> dict:
     return dict(do_ahc=cfg["do_ahc"], do_kubo=cfg["do_kubo"], do_mep=cfg["do_mep"],
+                do_opt=cfg.get("do_opt", False), do_gyro=cfg.get("do_gyro", False),
                 n_eF=cfg["N_eF"], eF_min=cfg["eF_min"], eF_max=cfg["eF_max"],
                 n_hw=cfg["N_hw"], hw_min=cfg["hw_min"], hw_max=cfg["hw_max"],
                 T_kelvin=cfg["T_kelvin"], eta=cfg["eta"], valence_bands=cfg["valence_bands"])
@@ -118,7 +130,8 @@ def params_from_cfg(cfg: dict) -> dict:
 
 def run_params(p: dict) -> capi.RunParams:
     jobs = (capi.MEP_JOB_AHC if p.get("do_ahc") else 0) | \
-        (capi.MEP_JOB_KUBO if p.get("do_kubo") else 0) | (capi.MEP_JOB_MEP if p.get("do_mep") else 0)
+        (capi.MEP_JOB_KUBO if p.get("do_kubo") else 0) | (capi.MEP_JOB_MEP if p.get("do_mep") else 0) | \
+        (capi.MEP_JOB_OPT if p.get("do_opt") else 0) | (capi.MEP_JOB_GYRO if p.get("do_gyro") else 0)
     return capi.Engine.params(jobs=jobs, n_eF=p.get("n_eF", 1), eF_min=p.get("eF_min", 0.0),
                               eF_max=p.get("eF_max", 0.0), n_hw=p.get("n_hw", 1),
                               hw_min=p.get("hw_min", 0.0), hw_max=p.get("hw_max", 0.0),

@@ -748,6 +748,159 @@ void orc_mep_niu_ic(int64_t N, const orc_z *V, const double *e, int n_val,
 /* ------------------------------------------------------------------------ */
 /* core.f90:165-270 -- the k-loop body, over an explicit k list             */
 /* ------------------------------------------------------------------------ */
+/* optical, photogalvanic and gyrotropic sums                               */
+/* ------------------------------------------------------------------------ */
+
+void orc_fd_stat_deriv(const double *fd, int n_ef, int64_t n, double *out) {
+  /* statistics.f90:180-190: cosh of the occupation value */
+  for (int64_t b = 0; b < n; ++b)
+    for (int i = 0; i < n_ef; ++i) out[i + n_ef * b] = 1.0 / (2.0 + 2.0 * cosh(fd[i + n_ef * b]));
+}
+
+void orc_kubo_opt_tens(int64_t N, const double *e, const orc_z *A, int n_hw,
+                       const double *hw, int n_ef, const double *fd, double eta,
+                       orc_z *os, orc_z *oa) {
+  const int64_t tot = 9 * (int64_t)n_hw * n_ef;
+  memset(os, 0, sizeof(orc_z) * tot);
+  memset(oa, 0, sizeof(orc_z) * tot);
+  if (!A) return; /* kubo.f90:226: zero unless A_ka is allocated */
+  orc_z *herm = calloc(tot, sizeof(orc_z)), *aherm = calloc(tot, sizeof(orc_z));
+  const double pi = 3.14159265358979323846;
+  for (int64_t m = 0; m < N; ++m)
+    for (int64_t n = 0; n < N; ++n) {
+      if (n == m) continue;
+      const double dE = e[m] - e[n]; /* dE_mn = en_k(m) - en_k(n) */
+      orc_z a_nm_mn[9];
+      for (int b = 0; b < 3; ++b)
+        for (int a = 0; a < 3; ++a)
+          a_nm_mn[a + 3 * b] = dE * A[a + 3 * (n + N * m)] * A[b + 3 * (m + N * n)];
+      for (int h = 0; h < n_hw; ++h) {
+        /* delta_broad (kubo.f90:279-297) and the denominator of (12.11) */
+        const orc_z dsm = pi * (dE - I * eta);
+        const double dbr = cimag(1.0 / (dsm - pi * hw[h]));
+        const double den = creal(dE - hw[h] - I * eta);
+        for (int f = 0; f < n_ef; ++f) {
+          const double df = fd[f + n_ef * m] - fd[f + n_ef * n];
+          for (int q = 0; q < 9; ++q) {
+            herm[q + 9 * (h + (int64_t)n_hw * f)] -= pi * a_nm_mn[q] * dbr * df;
+            aherm[q + 9 * (h + (int64_t)n_hw * f)] += I * (a_nm_mn[q] / den) * df;
+          }
+        }
+      }
+    }
+  for (int64_t q = 0; q < tot; ++q) {
+    os[q] = creal(herm[q]) + I * cimag(aherm[q]); /* (12.14) */
+    oa[q] = creal(aherm[q]) + I * cimag(herm[q]); /* (12.15) */
+  }
+  free(herm);
+  free(aherm);
+}
+
+void orc_photo_2nd_cond(int64_t N, const double *e, const orc_z *V, int n_hw,
+                        const double *hw, int n_ef, const double *fd, double eta,
+                        orc_z *o) {
+  memset(o, 0, sizeof(orc_z) * 27 * (int64_t)n_hw * n_ef);
+  orc_z *tmp = malloc(sizeof(orc_z) * 27 * n_hw);
+  for (int64_t n = 0; n < N; ++n)
+    for (int64_t m = 0; m < N; ++m) {
+      const orc_z dEnm = (e[n] - e[m]) - I * eta;
+      for (int64_t l = 0; l < N; ++l) {
+        if (l == n) continue;
+        const orc_z dEnl = (e[n] - e[l]) - I * eta;
+        orc_z vvv[27];
+        for (int c = 0; c < 3; ++c)
+          for (int b = 0; b < 3; ++b)
+            for (int a = 0; a < 3; ++a)
+              vvv[a + 3 * (b + 3 * c)] =
+                  V[a + 3 * (n + N * l)] * (V[b + 3 * (l + N * m)] * V[c + 3 * (m + N * n)]);
+        for (int h = 0; h < n_hw; ++h) {
+          for (int q = 0; q < 27; ++q) tmp[q + 27 * h] = 0.0;
+          for (int om = -1; om <= 1; om += 2) {
+            const orc_z dEhw = dEnl + om * hw[h] - I * eta; /* photo.f90:64 */
+            for (int q = 0; q < 27; ++q) tmp[q + 27 * h] += vvv[q] / (dEnm * dEhw);
+          }
+          for (int q = 0; q < 27; ++q) tmp[q + 27 * h] /= hw[h] * hw[h];
+        }
+        for (int f = 0; f < n_ef; ++f) {
+          const double df = fd[f + n_ef * l] - fd[f + n_ef * n];
+          for (int h = 0; h < n_hw; ++h)
+            for (int q = 0; q < 27; ++q) o[q + 27 * (h + (int64_t)n_hw * f)] += tmp[q + 27 * h] * df;
+        }
+      }
+    }
+  free(tmp);
+}
+
+void orc_gyro_C(int64_t N, const orc_z *V, int n_ef, const double *fdd, orc_z *o) {
+  memset(o, 0, sizeof(orc_z) * 9 * n_ef);
+  for (int64_t n = 0; n < N; ++n)
+    for (int f = 0; f < n_ef; ++f)
+      for (int b = 0; b < 3; ++b)
+        for (int a = 0; a < 3; ++a)
+          o[a + 3 * (b + 3 * f)] +=
+              V[a + 3 * (n + N * n)] * V[b + 3 * (n + N * n)] * fdd[f + n_ef * n];
+}
+
+void orc_gyro_D(int64_t N, const double *e, const orc_z *V, const orc_z *Om,
+                double kubo_tol, int n_ef, const double *fdd, orc_z *o) {
+  memset(o, 0, sizeof(orc_z) * 9 * n_ef);
+  for (int64_t n = 0; n < N; ++n) {
+    orc_z curv[9];
+    if (Om) {
+      for (int q = 0; q < 9; ++q) curv[q] = Om[q + 9 * (n + N * n)];
+    } else { /* get_kubo_curv (wann_interp.f90:113-146) */
+      for (int q = 0; q < 9; ++q) curv[q] = 0.0;
+      for (int64_t np = 0; np < N; ++np) {
+        if (np == n) continue;
+        const double dEE = (e[np] - e[n]) * (e[np] - e[n]);
+        if (fabs(dEE) < kubo_tol) continue;
+        for (int j = 0; j < 3; ++j)
+          for (int a = 0; a < 3; ++a)
+            curv[a + 3 * j] += cimag(V[a + 3 * (n + N * np)] * V[j + 3 * (np + N * n)]) / dEE;
+      }
+      for (int q = 0; q < 9; ++q) curv[q] *= -2.0;
+    }
+    orc_z om[3] = {0.0, 0.0, 0.0}; /* convert_tens_to_vect */
+    for (int c = 0; c < 3; ++c)
+      for (int b = 0; b < 3; ++b)
+        for (int a = 0; a < 3; ++a) om[c] += (double)levi(a, b, c) * curv[a + 3 * b];
+    for (int f = 0; f < n_ef; ++f)
+      for (int b = 0; b < 3; ++b)
+        for (int a = 0; a < 3; ++a)
+          o[a + 3 * (b + 3 * f)] -= V[a + 3 * (n + N * n)] * om[b] * fdd[f + n_ef * n];
+  }
+}
+
+void orc_gyro_Dw(int64_t N, const double *e, const orc_z *V, const orc_z *A,
+                 int n_hw, const double *hw, int n_ef, const double *fdd, orc_z *o) {
+  memset(o, 0, sizeof(orc_z) * 9 * (int64_t)n_hw * n_ef);
+  if (!A) return;
+  /* get_curvTilde_hw (gyro.f90:271-300); the reference accumulates into an
+     unzeroed allocation, zero-initialised here */
+  double *omt = calloc(3 * (int64_t)n_hw * N, sizeof(double));
+  for (int64_t n = 0; n < N; ++n)
+    for (int64_t m = 0; m < N; ++m) {
+      if (m == n) continue;
+      const double ww = (e[m] - e[n]) * (e[m] - e[n]);
+      const orc_z *a1 = &A[3 * (n + N * m)], *a2 = &A[3 * (m + N * n)];
+      double ov[3];
+      ov[0] = cimag(a1[1] * a2[2] - a1[2] * a2[1]) * ww;
+      ov[1] = cimag(a1[2] * a2[0] - a1[0] * a2[2]) * ww;
+      ov[2] = cimag(a1[0] * a2[1] - a1[1] * a2[0]) * ww;
+      for (int h = 0; h < n_hw; ++h)
+        for (int c = 0; c < 3; ++c) omt[c + 3 * (h + (int64_t)n_hw * n)] -= ov[c] / (ww - hw[h]);
+    }
+  for (int64_t n = 0; n < N; ++n)
+    for (int f = 0; f < n_ef; ++f)
+      for (int h = 0; h < n_hw; ++h)
+        for (int b = 0; b < 3; ++b)
+          for (int a = 0; a < 3; ++a)
+            o[a + 3 * (b + 3 * (h + (int64_t)n_hw * f))] -=
+                V[a + 3 * (n + N * n)] * omt[b + 3 * (h + (int64_t)n_hw * n)] * fdd[f + n_ef * n];
+  free(omt);
+}
+
+/* ------------------------------------------------------------------------ */
 
 static void sums_zero(const orc_params *p, orc_sums *s) {
   int nef = p->n_eF, nhw = p->n_hw, nv = p->valence_bands;
@@ -761,6 +914,12 @@ static void sums_zero(const orc_params *p, orc_sums *s) {
   if (s->niu_ic) memset(s->niu_ic, 0, sizeof(double) * 9 * nv);
   if (s->niu_lc) memset(s->niu_lc, 0, sizeof(double) * 9 * nv);
   if (s->niu_cs) memset(s->niu_cs, 0, sizeof(double) * 9 * nv);
+  if (s->opt_s) memset(s->opt_s, 0, sizeof(orc_z) * 9 * nhw * nef);
+  if (s->opt_a) memset(s->opt_a, 0, sizeof(orc_z) * 9 * nhw * nef);
+  if (s->photo) memset(s->photo, 0, sizeof(orc_z) * 27 * nhw * nef);
+  if (s->gyro_C) memset(s->gyro_C, 0, sizeof(orc_z) * 9 * nef);
+  if (s->gyro_D) memset(s->gyro_D, 0, sizeof(orc_z) * 9 * nef);
+  if (s->gyro_Dw) memset(s->gyro_Dw, 0, sizeof(orc_z) * 9 * nhw * nef);
 }
 
 static void addv(double *a, const double *b, int64_t n) {
@@ -797,6 +956,14 @@ int orc_accumulate(const orc_model *md, const orc_params *p, int64_t nk,
     s->niu_ic = calloc(9 * (nv > 0 ? nv : 1), sizeof(double));
     s->niu_lc = calloc(9 * (nv > 0 ? nv : 1), sizeof(double));
     s->niu_cs = calloc(9 * (nv > 0 ? nv : 1), sizeof(double));
+    s->opt_s = calloc(9 * nhw * nef, sizeof(orc_z));
+    s->opt_a = calloc(9 * nhw * nef, sizeof(orc_z));
+    s->photo = calloc(27 * nhw * nef, sizeof(orc_z));
+    s->gyro_C = calloc(9 * nef, sizeof(orc_z));
+    s->gyro_D = calloc(9 * nef, sizeof(orc_z));
+    s->gyro_Dw = calloc(9 * nhw * nef, sizeof(orc_z));
+    orc_z *xa = malloc(sizeof(orc_z) * 27 * nhw * nef), *xb = malloc(sizeof(orc_z) * 9 * nhw * nef);
+    double *fdd = malloc(sizeof(double) * nef * N);
     double *e = malloc(sizeof(double) * N), *fd = malloc(sizeof(double) * nef * N);
     double *tmp = malloc(sizeof(double) * 9 * (nef > nv ? nef : (nv > 0 ? nv : 1)));
     double *fl = malloc(sizeof(double) * nef);
@@ -839,8 +1006,22 @@ int orc_accumulate(const orc_model *md, const orc_params *p, int64_t nk,
         orc_kubo_ohc_tens(N, e, V, nhw, hw, nef, fd, p->eta, ztmp);
         addz(s->ohc, ztmp, 9 * nhw * nef);
       }
+      if (p->do_opt) { /* core.f90:235-243 */
+        orc_kubo_opt_tens(N, e, A, nhw, hw, nef, fd, p->eta, xa, xb);
+        addz(s->opt_s, xa, 9 * nhw * nef);
+        addz(s->opt_a, xb, 9 * nhw * nef);
+        orc_photo_2nd_cond(N, e, V, nhw, hw, nef, fd, p->eta, xa);
+        addz(s->photo, xa, 27 * nhw * nef);
+      }
+      if (p->do_gyro) { /* core.f90:247-253 */
+        orc_fd_stat_deriv(fd, nef, N, fdd);
+        orc_gyro_C(N, V, nef, fdd, xa); addz(s->gyro_C, xa, 9 * nef);
+        orc_gyro_D(N, e, V, Om, md->kubo_tol, nef, fdd, xa); addz(s->gyro_D, xa, 9 * nef);
+        orc_gyro_Dw(N, e, V, A, nhw, hw, nef, fdd, xa); addz(s->gyro_Dw, xa, 9 * nhw * nef);
+      }
     }
     free(e); free(fd); free(tmp); free(fl); free(ztmp); free(V); free(A); free(Om);
+    free(xa); free(xb); free(fdd);
   }
   for (int t = 0; t < nthr; ++t) {
     orc_sums *s = &part[t];
@@ -854,9 +1035,17 @@ int orc_accumulate(const orc_model *md, const orc_params *p, int64_t nk,
     addv(out->niu_ic, s->niu_ic, 9 * nv);
     addv(out->niu_lc, s->niu_lc, 9 * nv);
     addv(out->niu_cs, s->niu_cs, 9 * nv);
+    addz(out->opt_s, s->opt_s, 9 * nhw * nef);
+    addz(out->opt_a, s->opt_a, 9 * nhw * nef);
+    addz(out->photo, s->photo, 27 * nhw * nef);
+    addz(out->gyro_C, s->gyro_C, 9 * nef);
+    addz(out->gyro_D, s->gyro_D, 9 * nef);
+    addz(out->gyro_Dw, s->gyro_Dw, 9 * nhw * nef);
     free(s->Ne); free(s->ahc); free(s->ahc_velo); free(s->ohc);
     free(s->kubo_ic); free(s->kubo_lc); free(s->kubo_cs);
     free(s->niu_ic); free(s->niu_lc); free(s->niu_cs);
+    free(s->opt_s); free(s->opt_a); free(s->photo);
+    free(s->gyro_C); free(s->gyro_D); free(s->gyro_Dw);
   }
   free(part); free(ef); free(hw);
   return err;

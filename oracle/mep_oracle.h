@@ -24,9 +24,13 @@
  * connection, A^(H) before get_velo, curvature from A^(H)), the Chern-Simons
  * terms built on it (kubo_mep_CS, mep_niu_CS) and V_nn with a position
  * operator (never assigned by the reference) are restated from the source
- * only; no fixture or identity fixes their values.  Every other function is
- * checked against LAPACK, finite differences, closed forms, the FeMn
- * fixtures or a second independent formula (tests/test_oracle_*.py).
+ * only; no fixture or identity fixes their values.  The optical, photo-
+ * galvanic and gyrotropic sums (kubo_opt_tens, photo_2nd_cond, get_gyro_*)
+ * are pinned only by symmetry identities (Hermitian / anti-Hermitian parts,
+ * index symmetry) and by brute-force loops over a tiny model in the tests;
+ * their absolute values are otherwise parity unpinned.  Every other
+ * function is checked against LAPACK, finite differences, closed forms, the
+ * FeMn fixtures or a second independent formula (tests/test_oracle_*.py).
  */
 #ifndef MEP_ORACLE_H
 #define MEP_ORACLE_H
@@ -63,6 +67,7 @@ typedef struct {
   double eta; /* Hartree (i_eta_smr = i*eta) */
   int valence_bands;
   int n_threads;
+  int do_opt, do_gyro; /* jobs%do_opt, jobs%do_gyro (core.f90:235-253) */
 } orc_params;
 
 /* Raw (un-normalised) k-sums, all Fermi levels kept; sizes as commented. */
@@ -77,6 +82,12 @@ typedef struct {
   double *niu_ic;    /* (3,3,valence_bands)     mep_niu_IC              */
   double *niu_lc;    /* (3,3,valence_bands)                             */
   double *niu_cs;    /* (3,3,valence_bands)                             */
+  orc_z *opt_s;      /* (3,3,n_hw,n_eF)         kubo_opt_tens opt_symm  */
+  orc_z *opt_a;      /* (3,3,n_hw,n_eF)         kubo_opt_tens opt_asymm */
+  orc_z *photo;      /* (3,3,3,n_hw,n_eF)       photo_2nd_cond          */
+  orc_z *gyro_C;     /* (3,3,n_eF)              get_gyro_C              */
+  orc_z *gyro_D;     /* (3,3,n_eF)              get_gyro_D              */
+  orc_z *gyro_Dw;    /* (3,3,n_hw,n_eF)         get_gyro_Dw             */
 } orc_sums;
 
 /* k_space.f90 */
@@ -122,6 +133,24 @@ void orc_mep_niu_lc(int64_t n, const orc_z *V, const double *e, int n_val,
                     double kubo_tol, double *o);
 void orc_mep_niu_cs(int64_t n, const orc_z *A, const orc_z *Om, int n_val,
                     double *o);
+
+/* statistics.f90:160-196 d1_fd_stat_deriv: 1 / (2 + 2 cosh(f)) evaluated on
+   the occupations f = fd(ef, n) themselves, as the reference does */
+void orc_fd_stat_deriv(const double *fd, int n_ef, int64_t n, double *out);
+/* kubo.f90:204-405 (wann guide 12.10, 12.11, 12.14, 12.15); zero without A */
+void orc_kubo_opt_tens(int64_t n, const double *e, const orc_z *A, int n_hw,
+                       const double *hw, int n_ef, const double *fd, double eta,
+                       orc_z *opt_s, orc_z *opt_a);
+/* photo.f90:16-86 (Yang et al. PRB 97 241118 Eq. 1) */
+void orc_photo_2nd_cond(int64_t n, const double *e, const orc_z *V, int n_hw,
+                        const double *hw, int n_ef, const double *fd, double eta,
+                        orc_z *o);
+/* gyro.f90:153-268 (Souza et al. PRB 97 035158 Eqs. 2, 12, C20) */
+void orc_gyro_C(int64_t n, const orc_z *V, int n_ef, const double *fdd, orc_z *o);
+void orc_gyro_D(int64_t n, const double *e, const orc_z *V, const orc_z *Om,
+                double kubo_tol, int n_ef, const double *fdd, orc_z *o);
+void orc_gyro_Dw(int64_t n, const double *e, const orc_z *V, const orc_z *A,
+                 int n_hw, const double *hw, int n_ef, const double *fdd, orc_z *o);
 
 /* core.f90 k-loop body over an explicit list of k-points (3, nk), summed
    into s (zeroed here).  OpenMP over k, per-thread partials reduced in

@@ -36,12 +36,14 @@ class _Params(C.Structure):
                 ("n_eF", C.c_int), ("eF_min", C.c_double), ("eF_max", C.c_double),
                 ("n_hw", C.c_int), ("hw_min", C.c_double), ("hw_max", C.c_double),
                 ("T_kelvin", C.c_double), ("eta", C.c_double),
-                ("valence_bands", C.c_int), ("n_threads", C.c_int)]
+                ("valence_bands", C.c_int), ("n_threads", C.c_int),
+                ("do_opt", C.c_int), ("do_gyro", C.c_int)]
 
 
 class _Sums(C.Structure):
     _fields_ = [(k, c_dp) for k in ("Ne", "ahc", "ahc_velo", "ohc", "kubo_ic", "kubo_lc",
-                                     "kubo_cs", "niu_ic", "niu_lc", "niu_cs")]
+                                     "kubo_cs", "niu_ic", "niu_lc", "niu_cs", "opt_s",
+                                     "opt_a", "photo", "gyro_C", "gyro_D", "gyro_Dw")]
 
 
 def build():
@@ -261,7 +263,7 @@ class RunParams:
 
     def __init__(self, do_mep=False, do_kubo=False, do_ahc=True, n_eF=1, eF_min=0.0,
                  eF_max=0.0, n_hw=1, hw_min=0.0, hw_max=0.0, T_kelvin=0.0, eta=0.0,
-                 valence_bands=1, n_threads=0):
+                 valence_bands=1, n_threads=0, do_opt=False, do_gyro=False):
         self.__dict__.update(locals())
         del self.__dict__["self"]
 
@@ -284,7 +286,13 @@ def accumulate(model: Model, params: RunParams, kpts):
            "kubo_cs": np.zeros((3, 3, nef), order="F"),
            "niu_ic": np.zeros((3, 3, nv), order="F"),
            "niu_lc": np.zeros((3, 3, nv), order="F"),
-           "niu_cs": np.zeros((3, 3, nv), order="F")}
+           "niu_cs": np.zeros((3, 3, nv), order="F"),
+           "opt_s": np.zeros((3, 3, nhw, nef), np.complex128, order="F"),
+           "opt_a": np.zeros((3, 3, nhw, nef), np.complex128, order="F"),
+           "photo": np.zeros((3, 3, 3, nhw, nef), np.complex128, order="F"),
+           "gyro_C": np.zeros((3, 3, nef), np.complex128, order="F"),
+           "gyro_D": np.zeros((3, 3, nef), np.complex128, order="F"),
+           "gyro_Dw": np.zeros((3, 3, nhw, nef), np.complex128, order="F")}
     s = _Sums()
     for k, _ in _Sums._fields_:
         setattr(s, k, _p(out[k]))
@@ -332,4 +340,76 @@ def finalize(sums, params: RunParams, mp_grid, a_latt):
         out.update(ahc_tens=sums["ahc"][:, :, e] / norm,
                    ahcVELO=sums["ahc_velo"][:, :, :, e] / norm,
                    ohcVELO=sums["ohc"][:, :, :, e] / norm)
+    if getattr(params, "do_opt", False):  # core.f90:512-520: opt at eF_idx, photo all E_F
+        out.update(opt_Ssymm=sums["opt_s"][:, :, :, e] / norm,
+                   opt_Asymm=sums["opt_a"][:, :, :, e] / norm,
+                   photoC_2nd=sums["photo"] / norm)
+    if getattr(params, "do_gyro", False):  # core.f90:523-531: every E_F
+        out.update(gyro_C=sums["gyro_C"] / norm, gyro_D=sums["gyro_D"] / norm,
+                   gyro_Dw=sums["gyro_Dw"] / norm)
     return out
+
+
+def fd_stat_deriv(fd):
+    """statistics.f90:160-196 on fd (n_eF, n): 1 / (2 + 2 cosh(fd))."""
+    fd = np.asfortranarray(fd, dtype=np.float64)
+    out = np.zeros_like(fd, order="F")
+    lib().orc_fd_stat_deriv(_p(fd), C.c_int(fd.shape[0]), C.c_int64(fd.shape[1]), _p(out))
+    return out
+
+
+def kubo_opt_tens(e, A, hw, fd, eta):
+    """kubo.f90:204-405 -> (opt_symm, opt_asymm), each (3,3,n_hw,n_eF)."""
+    e = np.ascontiguousarray(e, dtype=np.float64)
+    hw = np.ascontiguousarray(hw, dtype=np.float64)
+    fd = np.asfortranarray(fd, dtype=np.float64)
+    n, nhw, nef = e.size, hw.size, fd.shape[0]
+    os_ = np.zeros((3, 3, nhw, nef), np.complex128, order="F")
+    oa = np.zeros_like(os_)
+    Ap = None if A is None else np.asfortranarray(A, dtype=np.complex128)
+    lib().orc_kubo_opt_tens(C.c_int64(n), _p(e), _p(Ap), C.c_int(nhw), _p(hw), C.c_int(nef),
+                            _p(fd), C.c_double(eta), _p(os_), _p(oa))
+    return os_, oa
+
+
+def photo_2nd_cond(e, V, hw, fd, eta):
+    """photo.f90:16-86 -> (3,3,3,n_hw,n_eF)."""
+    e = np.ascontiguousarray(e, dtype=np.float64)
+    hw = np.ascontiguousarray(hw, dtype=np.float64)
+    fd = np.asfortranarray(fd, dtype=np.float64)
+    V = np.asfortranarray(V, dtype=np.complex128)
+    o = np.zeros((3, 3, 3, hw.size, fd.shape[0]), np.complex128, order="F")
+    lib().orc_photo_2nd_cond(C.c_int64(e.size), _p(e), _p(V), C.c_int(hw.size), _p(hw),
+                             C.c_int(fd.shape[0]), _p(fd), C.c_double(eta), _p(o))
+    return o
+
+
+def gyro_C(V, fdd):
+    V = np.asfortranarray(V, dtype=np.complex128)
+    fdd = np.asfortranarray(fdd, dtype=np.float64)
+    o = np.zeros((3, 3, fdd.shape[0]), np.complex128, order="F")
+    lib().orc_gyro_C(C.c_int64(V.shape[1]), _p(V), C.c_int(fdd.shape[0]), _p(fdd), _p(o))
+    return o
+
+
+def gyro_D(e, V, Om, kubo_tol, fdd):
+    e = np.ascontiguousarray(e, dtype=np.float64)
+    V = np.asfortranarray(V, dtype=np.complex128)
+    fdd = np.asfortranarray(fdd, dtype=np.float64)
+    Omp = None if Om is None else np.asfortranarray(Om, dtype=np.complex128)
+    o = np.zeros((3, 3, fdd.shape[0]), np.complex128, order="F")
+    lib().orc_gyro_D(C.c_int64(e.size), _p(e), _p(V), _p(Omp), C.c_double(kubo_tol),
+                     C.c_int(fdd.shape[0]), _p(fdd), _p(o))
+    return o
+
+
+def gyro_Dw(e, V, A, hw, fdd):
+    e = np.ascontiguousarray(e, dtype=np.float64)
+    V = np.asfortranarray(V, dtype=np.complex128)
+    hw = np.ascontiguousarray(hw, dtype=np.float64)
+    fdd = np.asfortranarray(fdd, dtype=np.float64)
+    Ap = None if A is None else np.asfortranarray(A, dtype=np.complex128)
+    o = np.zeros((3, 3, hw.size, fdd.shape[0]), np.complex128, order="F")
+    lib().orc_gyro_Dw(C.c_int64(e.size), _p(e), _p(V), _p(Ap), C.c_int(hw.size), _p(hw),
+                      C.c_int(fdd.shape[0]), _p(fdd), _p(o))
+    return o

@@ -154,3 +154,41 @@ def test_mepinterp_band_structure_mode(tmp_path):
         assert np.allclose(ee[idx == q + 1], eo, atol=6e-9)
         V = F.read_velo(os.path.join(tmp, "out", "velo", f"velo_Vka.{q + 1:07d}"), 10)
         assert np.max(np.abs(V - Vo)) < 1e-7
+
+
+def test_mepinterp_optical_and_gyrotropic_outputs(tmp_path):
+    """jobs%do_opt / jobs%do_gyro: out/opt_Ssymm.npy, opt_Asymm.npy,
+    photoC_2nd.npy, gyro_C/D/Dw.npy (file_io.f90:362-398) against the
+    oracle on a model with a position operator."""
+    import oracle as O
+    from mepinterp_amd import models
+    from mepinterp_amd.formats import read_hr_dat, read_input_cfg, read_r_dat, write_hr_dat, \
+        write_input_cfg, write_r_dat
+    exe = _need("mepInterp")
+    tmp = str(tmp_path)
+    os.makedirs(os.path.join(tmp, "w90files"))
+    mdl = models.make_c1(seed=6)
+    write_hr_dat(os.path.join(tmp, "w90files", "m_hr.dat"), mdl["R_frac"], mdl["H"])
+    write_r_dat(os.path.join(tmp, "w90files", "m_r.dat"), mdl["R_frac"], mdl["r"])
+    write_input_cfg(os.path.join(tmp, "input.cfg"), dict(
+        do_ahc=True, do_opt=True, do_gyro=True, a_latt=mdl["a_latt"], seed_name="m",
+        wf_centers=None, mp_grid=(3, 3, 3), N_eF=2, eF_min_eV=-0.2, eF_max_eV=0.2,
+        T_kelvin=300.0, eta_eV=0.05, kubo_tol=1e-5, valence_bands=4, N_hw=3,
+        hw_min_eV=0.1, hw_max_eV=0.5))
+    r = subprocess.run([exe], cwd=tmp, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    cfg = read_input_cfg(os.path.join(tmp, "input.cfg"))
+    R, H, _ = read_hr_dat(os.path.join(tmp, "w90files", "m_hr.dat"))
+    rr = read_r_dat(os.path.join(tmp, "w90files", "m_r.dat"), R)
+    om = O.Model(R, H, r=rr, a_latt=cfg["a_latt"], kubo_tol=1e-5)
+    p = O.RunParams(do_ahc=True, do_opt=True, do_gyro=True, n_eF=2, eF_min=cfg["eF_min"],
+                    eF_max=cfg["eF_max"], n_hw=3, hw_min=cfg["hw_min"], hw_max=cfg["hw_max"],
+                    T_kelvin=300.0, eta=cfg["eta"], valence_bands=4)
+    o = O.finalize(O.accumulate(om, p, O.mp_mesh((3, 3, 3))), p, (3, 3, 3), cfg["a_latt"])
+    out = os.path.join(tmp, "out")
+    for key, shape in (("opt_Ssymm", (3, 3, 3)), ("opt_Asymm", (3, 3, 3)),
+                       ("photoC_2nd", (3, 3, 3, 3, 2)), ("gyro_C", (3, 3, 2)),
+                       ("gyro_D", (3, 3, 2)), ("gyro_Dw", (3, 3, 3, 2))):
+        a = np.load(os.path.join(out, key + ".npy"))
+        assert a.shape == shape and a.dtype == np.complex128, key
+        assert np.max(np.abs(a - o[key])) < 1e-9 * np.max(np.abs(o[key])), key

@@ -276,3 +276,28 @@ def test_mixed_precision_rejects_position_operator():
         rp.precision = capi.MEP_PREC_MIXED
         with pytest.raises(capi.MepError):
             eng.run_mesh((2, 2, 2), rp)
+
+
+
+@pytest.mark.parametrize("name", ["c1_pos", "rand12"])
+def test_optical_and_gyrotropic_sums_match_oracle(name):
+    """MEP_JOB_OPT (kubo_opt_tens, photo_2nd_cond) and MEP_JOB_GYRO
+    (get_gyro_C / D / Dw) through run_mesh against the oracle's k-loop."""
+    import oracle as O
+    from mepinterp_amd import capi, models
+    m = models.make_c1(seed=2) if name == "c1_pos" else models.make_random(seed=12, n=12, L=1)
+    mp = (3, 3, 2)
+    jobs = capi.MEP_JOB_AHC | capi.MEP_JOB_OPT | capi.MEP_JOB_GYRO
+    kw = dict(n_eF=2, eF_min=-0.01, eF_max=0.01, n_hw=3, hw_min=0.01, hw_max=0.05,
+              T_kelvin=300.0, eta=0.004, valence_bands=4)
+    with _engine(m) as eng:
+        g = eng.run_mesh(mp, _params(jobs, **kw))
+    p = O.RunParams(do_ahc=True, do_opt=True, do_gyro=True, **kw)
+    o = O.accumulate(oracle_model(m), p, O.mp_mesh(mp))
+    keys = ["photo", "gyro_C", "gyro_D"] + (["opt_s", "opt_a", "gyro_Dw"] if name == "c1_pos" else [])
+    for key in keys:
+        assert np.abs(o[key]).max() > 0, key
+        assert _rel(g[key], o[key]) < 1e-9, f"{key}: {_rel(g[key], o[key]):.2e}"
+    if name != "c1_pos":
+        for key in ("opt_s", "opt_a", "gyro_Dw"):
+            assert not np.abs(g[key]).any() and not np.abs(o[key]).any()

@@ -284,3 +284,78 @@ def test_mp_mesh_closed_form_and_order():
     assert len(set(idx)) < len(idx)
     # the mesh is symmetric (no Gamma for even N)
     assert np.all(np.abs(k).max(axis=1) < 0.5)
+
+
+
+# ------------------------------------------------- optical / gyrotropic sums
+
+def _pos_model_k(seed=5):
+    import oracle as O
+    from mepinterp_amd import models
+    m = models.make_c1(seed=seed)
+    om = oracle_model(m)
+    e, V, A, Om = O.get_wann_interp(om, np.array([0.13, -0.21, 0.37]))
+    return e, V, A, Om
+
+
+def test_kubo_opt_tens_symmetric_and_antisymmetric_parts():
+    """opt_Ssymm = Re(herm) + i Im(aherm) is symmetric and opt_Asymm =
+    Re(aherm) + i Im(herm) antisymmetric in (i, j) when A_ka is Hermitian
+    (wann guide 12.14/12.15)."""
+    import oracle as O
+    e, V, A, Om = _pos_model_k()
+    # Hermitian part of the reference-semantics connection
+    Ah = 0.5 * (A + np.conj(np.transpose(A, (0, 2, 1))))
+    hw = np.array([0.01, 0.03])
+    fd = O.fd_distrib(e, np.array([-0.01, 0.02]), 300.0)
+    s, a = O.kubo_opt_tens(e, Ah, hw, fd, 0.004)
+    assert np.abs(s).max() > 0
+    np.testing.assert_allclose(s, np.transpose(s, (1, 0, 2, 3)), atol=1e-12 * np.abs(s).max())
+    np.testing.assert_allclose(a, -np.transpose(a, (1, 0, 2, 3)), atol=1e-12 * np.abs(a).max())
+    z_s, z_a = O.kubo_opt_tens(e, None, hw, fd, 0.004)
+    assert not z_s.any() and not z_a.any()
+
+
+def test_photo_2nd_cond_against_einsum():
+    """photo_2nd_cond (photo.f90:16-86) as loops in C against a vectorised
+    restatement with independent indexing."""
+    import oracle as O
+    from mepinterp_amd import models
+    m = models.make_random(seed=4, n=5, L=1)
+    e, V, _, _ = O.get_wann_interp(oracle_model(m), np.array([0.1, 0.2, -0.3]))
+    hw = np.array([0.02, 0.05])
+    eta = 0.003
+    fd = O.fd_distrib(e, np.array([0.0]), 300.0)
+    o = O.photo_2nd_cond(e, V, hw, fd, eta)
+    n = e.size
+    dEnm = (e[:, None] - e[None, :]) - 1j * eta                    # (n, m)
+    dEnl = (e[:, None] - e[None, :]) - 1j * eta                    # (n, l)
+    ref = np.zeros((3, 3, 3, hw.size, 1), complex)
+    mask = ~np.eye(n, dtype=bool)                                  # l != n
+    df = (fd[0][None, :] - fd[0][:, None])                         # (n, l): f_l - f_n
+    vvv = np.einsum("anl,blm,cmn->abcnml", V, V, V)
+    for h, w in enumerate(hw):
+        g = sum(1.0 / (dEnm[:, :, None] * (dEnl[:, None, :] + om * w - 1j * eta)) for om in (-1, 1))
+        g = g * (mask * df)[:, None, :] / w ** 2                   # (n, m, l)
+        ref[..., h, 0] = np.einsum("abcnml,nml->abc", vvv, g)
+    np.testing.assert_allclose(o, ref, rtol=1e-10, atol=1e-12 * np.abs(ref).max())
+
+
+def test_gyro_tensors_against_direct_sums():
+    import oracle as O
+    e, V, A, Om = _pos_model_k()
+    fd = O.fd_distrib(e, np.array([0.0, 0.01]), 300.0)
+    fdd = O.fd_stat_deriv(fd)
+    # the reference's derivative is 1/(2 + 2 cosh(f)) of the occupation
+    np.testing.assert_allclose(fdd, 1.0 / (2.0 + 2.0 * np.cosh(fd)), rtol=1e-15)
+    dg = np.einsum("ann->an", V)
+    C = O.gyro_C(V, fdd)
+    np.testing.assert_allclose(C, np.einsum("an,bn,fn->abf", dg, dg, fdd), rtol=1e-12)
+    np.testing.assert_allclose(C, np.transpose(C, (1, 0, 2)), rtol=1e-12)
+    # D with the given curvature: omega_c = eps_abc Omega_ab(n,n)
+    D = O.gyro_D(e, V, Om, 1e-5, fdd)
+    eps = np.zeros((3, 3, 3))
+    for a, b, c in ((0, 1, 2), (1, 2, 0), (2, 0, 1)):
+        eps[a, b, c], eps[b, a, c] = 1, -1
+    om = np.einsum("abc,abn->cn", eps, np.einsum("abnn->abn", Om))
+    np.testing.assert_allclose(D, -np.einsum("an,bn,fn->abf", dg, om, fdd), rtol=1e-12)
