@@ -128,6 +128,129 @@ def read_r_dat(path: str, R_frac, R_float: bool = False):
     return r / AU_TO_ANGSTROM
 
 
+# --------------------------------------------------------------------------
+# _tb.dat  (Wannier90 write_tb: Hamiltonian and position operator together)
+# --------------------------------------------------------------------------
+
+def read_tb_dat(path: str, R_float: bool = False):
+    """Read ``<seed>_tb.dat`` like ``read_tb_file`` (file_io.f90:555-658).
+
+    Layout: a comment line; the unit cell in Angstrom, one lattice vector a_i
+    per line; n_wf; n_R; the degeneracy list (15 per line, skipped); per R a
+    blank line, the R line and n_wf**2 lines ``m n Re Im`` in eV; then per R a
+    blank line, the R line and n_wf**2 lines ``m n Re(x) Im(x) Re(y) Im(y)
+    Re(z) Im(z)`` in Angstrom.
+
+    Returns ``(R_frac (3,nR), H (n,n,nR) Hartree, r (3,n,n,nR) Bohr,
+    a_latt (3,3) Bohr with rows a_i)``.  Hamiltonian entries below 1e-7 eV
+    are clamped to 0 as the reference does (:612-613).  R stays fractional:
+    the reference stores matmul(unit_cell, R) (:605) and then uses it as the
+    fractional R of the Fourier phase (wann_interp.f90:213-219), which breaks
+    the interpolation of every model whose unit cell is not the identity;
+    this reader keeps the lattice-vector coefficients the file holds.
+    """
+    with open(path) as f:
+        lines = [ln for ln in f.read().splitlines()]
+    pos = 1
+
+    def nxt():
+        nonlocal pos
+        while pos < len(lines) and not lines[pos].strip():
+            pos += 1
+        if pos >= len(lines):
+            raise ValueError("[read_tb_dat]: unexpected end of file")
+        ln = lines[pos]
+        pos += 1
+        return ln.split()
+
+    cell = np.array([[float(t) for t in nxt()[:3]] for _ in range(3)], dtype=np.float64)
+    n_wf = int(nxt()[0])
+    n_R = int(nxt()[0])
+    deg = []
+    while len(deg) < n_R:
+        deg.extend(int(t) for t in nxt())
+    R_frac = np.zeros((3, n_R), dtype=np.float64, order="F")
+    H = np.zeros((n_wf, n_wf, n_R), dtype=np.complex128, order="F")
+    r = np.zeros((3, n_wf, n_wf, n_R), dtype=np.complex128, order="F")
+
+    def read_R():
+        t = nxt()
+        v = np.array([float(x) for x in t[:3]])
+        return v if R_float else np.trunc(v)
+
+    for s in range(n_R):
+        R_frac[:, s] = read_R()
+        for _ in range(n_wf * n_wf):
+            t = nxt()
+            m, n = int(t[0]) - 1, int(t[1]) - 1
+            re_, im_ = float(t[2]), float(t[3])
+            re_ = 0.0 if abs(re_) < 1e-7 else re_
+            im_ = 0.0 if abs(im_) < 1e-7 else im_
+            H[m, n, s] = re_ + 1j * im_
+    for s in range(n_R):
+        R = read_R()
+        if np.linalg.norm(R - R_frac[:, s]) > 1e-8:
+            print("[read_tb_dat]: WARNING rHopp has different sc order then tHopp")
+        for _ in range(n_wf * n_wf):
+            t = nxt()
+            m, n = int(t[0]) - 1, int(t[1]) - 1
+            v = [float(x) for x in t[2:8]]
+            for a in range(3):
+                r[a, m, n, s] = v[2 * a] + 1j * v[2 * a + 1]
+    return R_frac, H / AU_TO_EV, r / AU_TO_ANGSTROM, cell / AU_TO_ANGSTROM
+
+
+def write_tb_dat(path: str, R_frac, H_hartree, r_bohr, a_latt_bohr, deg=None,
+                 comment="written by mepinterp_amd"):
+    """Write the ``_tb.dat`` layout read_tb_dat reads (Wannier90 write_tb):
+    unit cell and positions in Angstrom, energies in eV, integer R."""
+    R_frac = np.asarray(R_frac)
+    H = np.asarray(H_hartree) * AU_TO_EV
+    r = np.asarray(r_bohr) * AU_TO_ANGSTROM
+    cell = np.asarray(a_latt_bohr) * AU_TO_ANGSTROM
+    n_wf, _, n_R = H.shape
+    if deg is None:
+        deg = np.ones(n_R, dtype=np.int64)
+    with open(path, "w") as f:
+        f.write(comment.strip() + "\n")
+        for i in range(3):
+            f.write("".join(f"{c: 24.16e}" for c in cell[i]) + "\n")
+        f.write(f"{n_wf:11d}\n{n_R:11d}\n")
+        for i in range(0, n_R, 15):
+            f.write("".join(f"{int(d):5d}" for d in deg[i:i + 15]) + "\n")
+        for s in range(n_R):
+            R = np.rint(R_frac[:, s]).astype(np.int64)
+            f.write(f"\n{R[0]:5d}{R[1]:5d}{R[2]:5d}\n")
+            for n in range(n_wf):
+                for m in range(n_wf):
+                    z = H[m, n, s]
+                    f.write(f"{m + 1:5d}{n + 1:5d}  {z.real: .17e}  {z.imag: .17e}\n")
+        for s in range(n_R):
+            R = np.rint(R_frac[:, s]).astype(np.int64)
+            f.write(f"\n{R[0]:5d}{R[1]:5d}{R[2]:5d}\n")
+            for n in range(n_wf):
+                for m in range(n_wf):
+                    v = r[:, m, n, s]
+                    f.write(f"{m + 1:5d}{n + 1:5d}" +
+                            "".join(f"  {c.real: .17e}  {c.imag: .17e}" for c in v) + "\n")
+
+
+def read_tb_basis(base: str, R_float: bool = False):
+    """read_tb_basis (file_io.f90:800-860): ``<base>_tb.dat`` when present
+    (Hamiltonian, position operator and unit cell), else ``<base>_hr.dat``
+    plus an optional ``<base>_r.dat``.  Returns ``(R_frac, H, r or None,
+    a_latt or None)``; a_latt is the _tb.dat unit cell that overrides the
+    input.cfg lattice (file_io.f90:578-579)."""
+    if os.path.exists(base + "_tb.dat"):
+        R, H, r, cell = read_tb_dat(base + "_tb.dat", R_float=R_float)
+        return R, H, r, cell
+    if not os.path.exists(base + "_hr.dat"):
+        raise FileNotFoundError(f'[read_tb_basis]: "{base}_hr.dat" does not exist')
+    R, H, _ = read_hr_dat(base + "_hr.dat", R_float=R_float)
+    r = read_r_dat(base + "_r.dat", R, R_float) if os.path.exists(base + "_r.dat") else None
+    return R, H, r, None
+
+
 def read_latt(path: str):
     """Read a ``.latt`` file (comment line + 3 rows a_i in Bohr)."""
     rows = []

@@ -2,15 +2,16 @@
 !   <seed>_hr.dat reader (file_io.f90:663-746 of the reference),
 !   <seed>_r.dat  reader (:750-795; opened as <seed>_r.dat, R fractional,
 !                  positions in Angstrom),
+!   <seed>_tb.dat reader (:555-658; R kept fractional),
 !   NPY v2.0 Fortran-order writer with the header layout of the reference's
 !   output files, and the response-tensor writers (:186-359).
 module file_io
   use iso_fortran_env, only: int8, int32
   use constants, only: dp, fp_acc, aUtoEv, aUtoAngstrm
-  use input_paras, only: w90_dir, out_dir, use_R_float
+  use input_paras, only: w90_dir, out_dir, use_R_float, a_latt
   implicit none
   private
-  public :: read_hr_file, read_r_file, read_tb_basis, save_npy, write_hw_list, write_mep_bands, &
+  public :: read_hr_file, read_r_file, read_tb_file, read_tb_basis, save_npy, write_hw_list, write_mep_bands, &
             write_mep_tensors, write_kubo_mep_tensors, write_ahc_tensor, write_opt_tensors, &
             write_gyro_tensors
 
@@ -96,11 +97,93 @@ contains
     r_mat = r_mat / aUtoAngstrm
   end subroutine read_r_file
 
-  subroutine read_tb_basis(seed_name, R_vect, H_mat, r_mat)
+  subroutine read_tb_file(seed_name, R_vect, H_mat, r_mat)
+    ! Wannier90 _tb.dat: a comment line, the unit cell in Angstrom (one
+    ! lattice vector per line, overriding a_latt as file_io.f90:578-579 does),
+    ! n_wf, n_R, the degeneracy list (skipped), then per R a blank line, the R
+    ! line and n_wf**2 lines "m n Re Im" (eV), then per R a blank line, the R
+    ! line and n_wf**2 lines "m n Re(x) Im(x) Re(y) Im(y) Re(z) Im(z)" (Angstrom).
+    ! R is kept fractional: the reference stores matmul(unit_cell, R)
+    ! (file_io.f90:605) and then uses it as the fractional R of the Fourier
+    ! phase (wann_interp.f90:213-219).
     character(len=*), intent(in) :: seed_name
     real(dp), allocatable, intent(inout) :: R_vect(:, :)
     complex(dp), allocatable, intent(inout) :: H_mat(:, :, :), r_mat(:, :, :, :)
-    logical :: hr_exist, r_exist
+    integer :: u, f_nwfs, f_nSC, sc, it, m, n, ios, int3(3), idx(2), line
+    real(dp) :: cell(3, 3), real2(2), real6(6), real3(3)
+    open(newunit=u, file=seed_name // "_tb.dat", form="formatted", action="read", status="old", iostat=ios)
+    if (ios /= 0) stop "[read_tb_file]: ERROR could not open the _tb.dat file"
+    read(u, *)
+    do m = 1, 3
+      read(u, *) cell(m, 1:3)
+    end do
+    cell = cell / aUtoAngstrm
+    if (maxval(abs(cell - a_latt)) > 1e-6_dp * maxval(abs(cell))) &
+      write(*, *) "[read_tb_file]: WARNING the _tb.dat unit cell differs from the input.cfg lattice"
+    a_latt = cell
+    read(u, *) f_nwfs
+    read(u, *) f_nSC
+    allocate(R_vect(3, f_nSC), H_mat(f_nwfs, f_nwfs, f_nSC), r_mat(3, f_nwfs, f_nwfs, f_nSC))
+    H_mat = cmplx(0.0_dp, 0.0_dp, dp)
+    r_mat = cmplx(0.0_dp, 0.0_dp, dp)
+    do line = 1, (f_nSC + 14) / 15
+      read(u, *)
+    end do
+    do sc = 1, f_nSC
+      read(u, *)
+      call read_R(real3)
+      R_vect(:, sc) = real3
+      do it = 1, f_nwfs**2
+        read(u, *) idx(1:2), real2(1:2)
+        if (abs(real2(1)) < 1e-7_dp) real2(1) = 0.0_dp
+        if (abs(real2(2)) < 1e-7_dp) real2(2) = 0.0_dp
+        H_mat(idx(1), idx(2), sc) = cmplx(real2(1), real2(2), dp)
+      end do
+    end do
+    do sc = 1, f_nSC
+      read(u, *)
+      call read_R(real3)
+      if (norm2(real3 - R_vect(:, sc)) > 1e-8_dp) &
+        write(*, *) "[read_tb_file]: WARNING rHopp has different sc order then tHopp"
+      do it = 1, f_nwfs**2
+        read(u, *) idx(1:2), real6(1:6)
+        m = idx(1)
+        n = idx(2)
+        r_mat(1, m, n, sc) = cmplx(real6(1), real6(2), dp)
+        r_mat(2, m, n, sc) = cmplx(real6(3), real6(4), dp)
+        r_mat(3, m, n, sc) = cmplx(real6(5), real6(6), dp)
+      end do
+    end do
+    close(u)
+    H_mat = H_mat / aUtoEv
+    r_mat = r_mat / aUtoAngstrm
+    write(*, '(a,i6,a,i6,a)') "[read_tb_file]: SUCCESS (nWfs=", f_nwfs, "; nrpts=", f_nSC, ")"
+  contains
+    subroutine read_R(v)
+      real(dp), intent(out) :: v(3)
+      if (use_R_float) then
+        read(u, *) v
+      else
+        read(u, *) int3
+        v = real(int3, dp)
+      end if
+    end subroutine read_R
+  end subroutine read_tb_file
+
+  subroutine read_tb_basis(seed_name, R_vect, H_mat, r_mat)
+    ! <seed>_tb.dat when present, else <seed>_hr.dat and an optional
+    ! <seed>_r.dat (file_io.f90:800-860)
+    character(len=*), intent(in) :: seed_name
+    real(dp), allocatable, intent(inout) :: R_vect(:, :)
+    complex(dp), allocatable, intent(inout) :: H_mat(:, :, :), r_mat(:, :, :, :)
+    logical :: hr_exist, r_exist, tb_exist
+    inquire(file=w90_dir // seed_name // "_tb.dat", exist=tb_exist)
+    if (tb_exist) then
+      write(*, *) '[read_tb_basis]: found data file ' // w90_dir // seed_name // '_tb.dat'
+      call read_tb_file(w90_dir // seed_name, R_vect, H_mat, r_mat)
+      if (size(H_mat, 1) /= size(H_mat, 2)) stop "[read_tb_basis]: H_mat is not square"
+      return
+    end if
     inquire(file=w90_dir // seed_name // "_hr.dat", exist=hr_exist)
     if (.not. hr_exist) then
       write(*, *) '[read_tb_basis]: "' // w90_dir // seed_name // '_hr.dat" does not exist'

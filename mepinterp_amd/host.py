@@ -8,7 +8,7 @@ same steps, same names, same outputs.
                             normalize_k_int (core.f90:419-603,
                             k_space.f90:81-202)
 * ``write_outputs``         the NPY writers of file_io.f90:186-359
-* ``core_worker``           core.f90:79-298: input.cfg -> w90files/<seed>_hr.dat
+* ``core_worker``           core.f90:79-298: input.cfg -> w90files/<seed>_tb.dat or _hr.dat (+ _r.dat)
                             -> device k-loop -> out/*.npy
 * ``band_worker``           band_calc.f90:30-121: ./kpts -> raw/enK.*,
                             out/eBands.dat (and out//velo/velo_Vka.*)
@@ -22,7 +22,7 @@ import os
 import numpy as np
 
 from . import capi
-from .formats import (AU_TO_EV, read_hr_dat, read_input_cfg, read_kpts, read_r_dat,
+from .formats import (AU_TO_EV, read_input_cfg, read_kpts, read_tb_basis,
                       write_en_binary, write_en_global, write_npy)
 
 
@@ -139,15 +139,26 @@ def run_params(p: dict) -> capi.RunParams:
                               valence_bands=max(1, p.get("valence_bands", 1)))
 
 
+def _model_latt(cfg: dict, cell):
+    """Lattice handed to the interpolation: the _tb.dat unit cell overrides
+    the input.cfg lattice (file_io.f90:578-579); the k-sum normalisation keeps
+    the input.cfg cell volume, as the reference's unit_vol is set from
+    input.cfg only (input_paras.f90:281)."""
+    if cell is None:
+        return cfg["a_latt"]
+    if np.max(np.abs(np.asarray(cell) - np.asarray(cfg["a_latt"]))) > 1e-6 * np.max(np.abs(cell)):
+        print("[read_tb_basis]: WARNING the _tb.dat unit cell differs from the input.cfg lattice")
+    return cell
+
+
 def core_worker(run_dir: str = "."):
     """core.f90:79-298 on the GPU: read ./input.cfg and w90files/<seed>_hr.dat,
     integrate over the MP mesh, write out/*.npy and out/occ.dat."""
     cfg = read_input_cfg(os.path.join(run_dir, "input.cfg"))
     base = os.path.join(run_dir, "w90files", cfg["seed_name"])
-    R, H, deg = read_hr_dat(base + "_hr.dat", R_float=cfg["R_vect_float"])
-    r = read_r_dat(base + "_r.dat", R, cfg["R_vect_float"]) if os.path.exists(base + "_r.dat") else None
+    R, H, r, cell = read_tb_basis(base, R_float=cfg["R_vect_float"])
     p = params_from_cfg(cfg)
-    with capi.Engine(R, H, r=r, wf_centers=cfg["wf_centers"], a_latt=cfg["a_latt"],
+    with capi.Engine(R, H, r=r, wf_centers=cfg["wf_centers"], a_latt=_model_latt(cfg, cell),
                      kubo_tol=cfg["kubo_tol"], gauge_trafo=cfg["do_gauge_trafo"],
                      cart_velo=cfg["use_cart_velo"]) as eng:
         sums = eng.run_mesh(cfg["mp_grid"], run_params(p))
@@ -161,10 +172,9 @@ def band_worker(run_dir: str = "."):
     one batch, write raw/enK.<k> and out/eBands.dat."""
     cfg = read_input_cfg(os.path.join(run_dir, "input.cfg"))
     base = os.path.join(run_dir, "w90files", cfg["seed_name"])
-    R, H, _ = read_hr_dat(base + "_hr.dat", R_float=cfg["R_vect_float"])
-    r = read_r_dat(base + "_r.dat", R, cfg["R_vect_float"]) if os.path.exists(base + "_r.dat") else None
+    R, H, r, cell = read_tb_basis(base, R_float=cfg["R_vect_float"])
     k = read_kpts(os.path.join(run_dir, "kpts"))
-    with capi.Engine(R, H, r=r, wf_centers=cfg["wf_centers"], a_latt=cfg["a_latt"],
+    with capi.Engine(R, H, r=r, wf_centers=cfg["wf_centers"], a_latt=_model_latt(cfg, cell),
                      kubo_tol=cfg["kubo_tol"], gauge_trafo=cfg["do_gauge_trafo"],
                      cart_velo=cfg["use_cart_velo"]) as eng:
         e, V, _, _ = eng.interp(k, velo=r is not None)

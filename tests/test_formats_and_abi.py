@@ -184,3 +184,80 @@ def test_band_structure_files_roundtrip(tmp_path):
     idx, kk, ee = F.read_en_global(str(tmp_path / "eBands.dat"))
     assert list(idx) == [1, 1, 2, 2, 3, 3]
     assert np.allclose(ee, e.T.ravel(), atol=5e-9)
+
+
+# a two-orbital model in Wannier90's write_tb layout (unit cell in Angstrom,
+# energies in eV, positions in Angstrom), typed out by hand
+TB_DAT = """ written on 26Feb2019 at 10:00:00
+    2.000000    0.000000    0.000000
+    1.000000    1.732051    0.000000
+    0.000000    0.000000    5.000000
+           2
+           2
+    1    2
+
+    0    0    0
+    1    1   -0.50000000E+00    0.00000000E+00
+    2    1    0.10000000E+00    0.20000000E+00
+    1    2    0.10000000E+00   -0.20000000E+00
+    2    2    0.50000000E+00    0.00000000E+00
+
+    1   -1    0
+    1    1    0.25000000E+00    0.00000000E+00
+    2    1    0.00000000E+00    0.00000000E+00
+    1    2    0.30000000E-07    0.00000000E+00
+    2    2   -0.25000000E+00    0.10000000E+00
+
+    0    0    0
+    1    1    0.10000000E+00    0.00000000E+00    0.20000000E+00    0.00000000E+00    0.30000000E+00    0.00000000E+00
+    2    1    0.00000000E+00    0.01000000E+00    0.00000000E+00    0.00000000E+00    0.00000000E+00    0.00000000E+00
+    1    2    0.00000000E+00   -0.01000000E+00    0.00000000E+00    0.00000000E+00    0.00000000E+00    0.00000000E+00
+    2    2    1.10000000E+00    0.00000000E+00    0.60000000E+00    0.00000000E+00    2.50000000E+00    0.00000000E+00
+
+    1   -1    0
+    1    1    0.00000000E+00    0.00000000E+00    0.00000000E+00    0.00000000E+00    0.00000000E+00    0.00000000E+00
+    2    1    0.00000000E+00    0.00000000E+00    0.40000000E+00    0.00000000E+00    0.00000000E+00    0.00000000E+00
+    1    2    0.00000000E+00    0.00000000E+00    0.00000000E+00    0.00000000E+00    0.00000000E+00    0.00000000E+00
+    2    2    0.00000000E+00    0.00000000E+00    0.00000000E+00    0.00000000E+00    0.00000000E+00    0.00000000E+00
+"""
+
+
+def test_tb_dat_reads_wannier90_layout(tmp_path):
+    """read_tb_file (file_io.f90:555-658) on a hand-typed file: unit cell
+    rows a_i in Angstrom -> Bohr, R left as lattice coefficients, eV ->
+    Hartree with the 1e-7 eV clamp, Angstrom -> Bohr positions."""
+    from mepinterp_amd.formats import AU_TO_ANGSTROM, AU_TO_EV, read_tb_dat
+    p = tmp_path / "m_tb.dat"
+    p.write_text(TB_DAT)
+    R, H, r, cell = read_tb_dat(str(p))
+    assert R.shape == (3, 2) and H.shape == (2, 2, 2) and r.shape == (3, 2, 2, 2)
+    np.testing.assert_array_equal(R, [[0, 1], [0, -1], [0, 0]])
+    np.testing.assert_allclose(cell[1], np.array([1.0, 1.732051, 0.0]) / AU_TO_ANGSTROM, rtol=1e-15)
+    assert H[1, 0, 0] == pytest.approx((0.1 + 0.2j) / AU_TO_EV, rel=1e-15)
+    assert H[0, 1, 1] == 0.0  # 3e-8 eV is clamped
+    assert H[1, 1, 1] == pytest.approx((-0.25 + 0.1j) / AU_TO_EV, rel=1e-15)
+    np.testing.assert_allclose(r[:, 1, 1, 0], np.array([1.1, 0.6, 2.5]) / AU_TO_ANGSTROM, rtol=1e-15)
+    assert r[1, 1, 0, 1] == pytest.approx(0.4 / AU_TO_ANGSTROM, rel=1e-15)
+    assert r[0, 0, 1, 0] == pytest.approx(-0.01j / AU_TO_ANGSTROM, rel=1e-15)
+
+
+def test_tb_dat_roundtrip_and_preferred_over_hr(tmp_path):
+    """write_tb_dat -> read_tb_dat is exact to the printed digits, and
+    read_tb_basis takes _tb.dat over _hr.dat/_r.dat (file_io.f90:817-822)."""
+    from mepinterp_amd import models
+    from mepinterp_amd.formats import read_tb_basis, write_hr_dat, write_tb_dat
+    m = models.make_c1(seed=3)
+    cell = np.array([[7.0, 0.0, 0.0], [3.5, 6.06, 0.0], [0.0, 0.0, 9.0]])
+    base = str(tmp_path / "x")
+    write_tb_dat(base + "_tb.dat", m["R_frac"], m["H"], m["r"], cell)
+    write_hr_dat(base + "_hr.dat", m["R_frac"], 2.0 * m["H"])
+    R, H, r, c = read_tb_basis(base)
+    np.testing.assert_array_equal(R, m["R_frac"])
+    big = np.abs(m["H"]) * 27.211385 >= 1e-7
+    np.testing.assert_allclose(H[big], m["H"][big], rtol=1e-15)
+    np.testing.assert_allclose(r, m["r"], rtol=1e-15, atol=1e-300)
+    np.testing.assert_allclose(c, cell, rtol=1e-15)
+    os.remove(base + "_tb.dat")
+    R2, H2, r2, c2 = read_tb_basis(base)
+    assert r2 is None and c2 is None
+    np.testing.assert_allclose(H2[big], 2.0 * m["H"][big], rtol=1e-15)

@@ -192,3 +192,50 @@ def test_mepinterp_optical_and_gyrotropic_outputs(tmp_path):
         a = np.load(os.path.join(out, key + ".npy"))
         assert a.shape == shape and a.dtype == np.complex128, key
         assert np.max(np.abs(a - o[key])) < 1e-9 * np.max(np.abs(o[key])), key
+
+
+def test_mepinterp_reads_tb_dat(tmp_path):
+    """w90files/<seed>_tb.dat (file_io.f90:555-658, preferred over _hr.dat by
+    read_tb_basis :817-822): Hamiltonian, position operator and a non-cubic
+    unit cell in one file, with Cartesian velocities, against the oracle on
+    the values read back from the same file (R fractional)."""
+    import oracle as O
+    from mepinterp_amd import models
+    from mepinterp_amd.formats import read_tb_dat, write_input_cfg, write_tb_dat
+    exe = _need("mepInterp")
+    tmp = str(tmp_path)
+    os.makedirs(os.path.join(tmp, "w90files"))
+    mdl = models.make_c1(seed=8)
+    cell = np.array([[6.0, 0.0, 0.0], [-3.0, 5.196152422706632, 0.0], [0.0, 0.0, 8.0]])
+    write_tb_dat(os.path.join(tmp, "w90files", "t_tb.dat"), mdl["R_frac"], mdl["H"], mdl["r"], cell)
+    write_input_cfg(os.path.join(tmp, "input.cfg"), dict(
+        do_ahc=True, do_mep=True, do_kubo=True, a_latt=cell, seed_name="t", wf_centers=None,
+        use_cart_velo=True, mp_grid=(4, 4, 4), N_eF=3, eF_min_eV=-0.2, eF_max_eV=0.2,
+        T_kelvin=300.0, eta_eV=0.05, kubo_tol=1e-5, valence_bands=4, N_hw=2,
+        hw_min_eV=0.0, hw_max_eV=0.5))
+    r = subprocess.run([exe], cwd=tmp, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "_tb.dat" in r.stdout
+    R, H, rr, c = read_tb_dat(os.path.join(tmp, "w90files", "t_tb.dat"))
+    from mepinterp_amd.formats import read_input_cfg
+    cfg = read_input_cfg(os.path.join(tmp, "input.cfg"))
+    om = O.Model(R, H, r=rr, a_latt=c, use_cart_velo=True, kubo_tol=1e-5)
+    p = O.RunParams(do_ahc=True, do_mep=True, do_kubo=True, n_eF=3, eF_min=cfg["eF_min"],
+                    eF_max=cfg["eF_max"], n_hw=2, hw_min=cfg["hw_min"], hw_max=cfg["hw_max"],
+                    T_kelvin=300.0, eta=cfg["eta"], valence_bands=4)
+    o = O.finalize(O.accumulate(om, p, O.mp_mesh((4, 4, 4))), p, (4, 4, 4), cfg["a_latt"])
+    out = os.path.join(tmp, "out")
+    for key in ("ahc_tens", "ahcVELO", "mep_tens", "mep_cs", "kubo_mep_tens", "kubo_mep_cs"):
+        a = np.load(os.path.join(out, key + ".npy"))
+        scale = np.max(np.abs(o[key]))
+        assert scale > 0, key
+        assert np.max(np.abs(a - o[key])) < 1e-9 * scale, key
+    # the Python host reads the same file the same way
+    from mepinterp_amd import host
+    tmp2 = os.path.join(tmp, "py")
+    os.makedirs(os.path.join(tmp2, "w90files"))
+    shutil.copy(os.path.join(tmp, "w90files", "t_tb.dat"), os.path.join(tmp2, "w90files"))
+    shutil.copy(os.path.join(tmp, "input.cfg"), tmp2)
+    py = host.core_worker(tmp2)
+    for key in ("ahc_tens", "mep_tens", "kubo_mep_tens"):
+        assert np.max(np.abs(py[key] - o[key])) < 1e-9 * np.max(np.abs(o[key])), key
