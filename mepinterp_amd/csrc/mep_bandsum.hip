@@ -517,6 +517,351 @@ __global__ __launch_bounds__(BS_THREADS, 2) void bandsum_kernel(BandArgs b, doub
   for (int64_t c = t; c < L.total; c += BS_THREADS) out[c] += acc[c];
 }
 
+// ===========================================================================
+// 32 < n <= 128 (no position operator).  Same sums and the same record
+// layout; V and E are read from HBM / L2 instead of LDS, the workgroup's
+// accumulator record lives in `part` (one writer per entry, fixed order).
+//   per-band sums      thread (n0, part) over n = part, part + P, ...; parts
+//                      combined through LDS
+//   hw-resolved sums   for each (hw, E_F) the 256 threads split the pairs
+//   itinerant terms    C_j column tiles on FP64 MFMA, wave item (tc, j)
+//                      accumulating over the row tiles in registers
+// LDS: E n | fd nef*n | S 2*9*256 | F2, F2m 2*9*n | red 4*36 | windows
+// ===========================================================================
+static __device__ inline double rinv(const double* Es, int m, int c) {
+  return (m == c) ? 0.0 : 1.0 / (Es[c] - Es[m]);  // 1 / (E_c - E_m)
+}
+
+__global__ __launch_bounds__(256) void bandsum_wide_kernel(BandArgs b, double Tsmear, AccLayout L,
+                                                           int64_t nk, const double* __restrict__ E,
+                                                           const zd* __restrict__ V,
+                                                           const double* __restrict__ w,
+                                                           double* __restrict__ part) {
+  extern __shared__ double smem[];
+  const int n = b.n, nef = b.n_ef, nhw = b.n_hw, nval = b.n_val;
+  double* Es = smem;
+  double* fd = Es + n;
+  double* S = fd + (int64_t)nef * n;  // [9][256]
+  double* S2 = S + 9 * 256;           // [9][256]
+  double* F2 = S2 + 9 * 256;          // [9][n]
+  double* F2m = F2 + 9 * n;           // [9][n]
+  double* red = F2m + 9 * n;          // [4][36]
+  uint8_t* win = reinterpret_cast<uint8_t*>(red + 4 * 36);  // [nval][n] lo, hi
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const int n2 = n * n;
+  const bool finiteT = b.T_kelvin > 1e-2;
+  const bool doAHC = (b.jobs & MEP_JOB_AHC) != 0;
+  const bool doK = (b.jobs & MEP_JOB_KUBO) != 0;
+  const bool doNjob = (b.jobs & MEP_JOB_MEP) != 0;
+  const double tol = b.kubo_tol;
+  double* acc = part + (int64_t)blockIdx.x * L.total;
+  const int npad = n <= 64 ? 64 : 128, P = 256 / npad;
+  const int p0 = t % npad, pp = t / npad;
+  const bool pact = p0 < n;
+
+  for (int64_t k = blockIdx.x; k < nk; k += gridDim.x) {
+    const double wk = w ? w[k] : 1.0;
+    const zd* Vk = V + k * 3 * (int64_t)n2;
+    __syncthreads();
+    if (t < n) Es[t] = E[k * n + t];
+    __syncthreads();
+    if (doNjob)
+      for (int idx = t; idx < nval * n; idx += 256) {
+        const int nn = idx % n, n0 = idx / n;
+        if (nn == n0) continue;
+        const double E0 = Es[n0], dn = E0 - Es[nn];
+        const double dn2 = dn * dn;
+        int lo = n0, hi = n0;
+        while (lo > 0 && !(fabs(dn2 * (E0 - Es[lo - 1])) > tol)) --lo;
+        while (hi < n - 1 && !(fabs(dn2 * (E0 - Es[hi + 1])) > tol)) ++hi;
+        win[2 * idx] = (uint8_t)lo;
+        win[2 * idx + 1] = (uint8_t)hi;
+      }
+    for (int idx = t; idx < nef * n; idx += 256) {
+      if (b.fd_in) {
+        fd[idx] = b.fd_in[k * (int64_t)nef * n + idx];
+      } else {
+        const int ef = (nef == 1) ? 0 : idx % nef, nb = (nef == 1) ? idx : idx / nef;
+        const double e = Es[nb], f = b.ef[ef];
+        fd[idx] = finiteT ? 1.0 / (1.0 + exp((e - f) / Tsmear)) : ((e < f) ? 1.0 : 0.0);
+      }
+    }
+    __syncthreads();
+    for (int ef = t; ef < nef; ef += 256) {
+      double sN = 0.0;
+      for (int m0 = 0; m0 < n; ++m0) sN += fd[ef + nef * m0];
+      acc[L.Ne + ef] += wk * sN;
+    }
+    // ---- Berry curvature per band -> kubo_ahc_tens --------------------------
+    if (doAHC) {
+      double om[9];
+#pragma unroll
+      for (int q = 0; q < 9; ++q) om[q] = 0.0;
+      if (pact)
+        for (int nn = pp; nn < n; nn += P) {
+          if (nn == p0) continue;
+          zd x[3], y[3];
+#pragma unroll
+          for (int a = 0; a < 3; ++a) {
+            x[a] = Vk[a * (int64_t)n2 + p0 + (int64_t)n * nn];
+            y[a] = Vk[a * (int64_t)n2 + nn + (int64_t)n * p0];
+          }
+          const double r = rinv(Es, nn, p0);  // 1 / (E_0 - E_n)
+          const double r2 = r * r;
+#pragma unroll
+          for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int i = 0; i < 3; ++i) om[i + 3 * j] -= 2.0 * zmul(x[i], y[j]).im * r2;
+        }
+#pragma unroll
+      for (int q = 0; q < 9; ++q) S[q * 256 + t] = om[q];
+      __syncthreads();
+      for (int it = t; it < 9 * nef; it += 256) {
+        const int q = it % 9, ef = it / 9;
+        double sA = 0.0;
+        for (int m0 = 0; m0 < n; ++m0) {
+          double o = 0.0;
+          for (int u = 0; u < P; ++u) o += S[q * 256 + u * npad + m0];
+          sA += fd[ef + nef * m0] * o;
+        }
+        acc[L.ahc + q + 9 * ef] -= wk * sA;
+      }
+      __syncthreads();
+    }
+    // ---- local (LC) MEP terms ----------------------------------------------
+    if (doK || doNjob) {
+      double f3[9], f3m[9];
+#pragma unroll
+      for (int q = 0; q < 9; ++q) f3[q] = f3m[q] = 0.0;
+      const bool doN = doNjob && pact && p0 < nval;
+      if (pact) {
+        const double E0 = Es[p0];
+        zd z0[3];
+#pragma unroll
+        for (int l = 0; l < 3; ++l) z0[l] = Vk[l * (int64_t)n2 + p0 + (int64_t)n * p0];
+        for (int nn = pp; nn < n; nn += P) {
+          if (nn == p0) continue;
+          zd x[3], y[3];
+#pragma unroll
+          for (int a = 0; a < 3; ++a) {
+            x[a] = Vk[a * (int64_t)n2 + p0 + (int64_t)n * nn];
+            y[a] = Vk[a * (int64_t)n2 + nn + (int64_t)n * p0];
+          }
+          const double r = rinv(Es, nn, p0);  // 1 / (E_0 - E_n)
+          const double r3 = r * r * r;
+          zd cr[3];
+          cr[0] = zsub(zmul(y[1], z0[2]), zmul(y[2], z0[1]));
+          cr[1] = zsub(zmul(y[2], z0[0]), zmul(y[0], z0[2]));
+          cr[2] = zsub(zmul(y[0], z0[1]), zmul(y[1], z0[0]));
+          const double dn = E0 - Es[nn];
+          const bool keep3 = doN && (fabs(dn * dn * dn) > tol);
+#pragma unroll
+          for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+              const double v = zmul(x[i], cr[j]).re * r3;
+              f3[i + 3 * j] += v;
+              if (keep3) f3m[i + 3 * j] += v;
+            }
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < 9; ++q) {
+        S[q * 256 + t] = f3[q];
+        S2[q * 256 + t] = f3m[q];
+      }
+      __syncthreads();
+      if (doK)
+        for (int it = t; it < 9 * nef; it += 256) {
+          const int q = it % 9, ef = it / 9;
+          double sL = 0.0;
+          for (int m0 = 0; m0 < n; ++m0) {
+            double o = 0.0;
+            for (int u = 0; u < P; ++u) o += S[q * 256 + u * npad + m0];
+            sL += fd[ef + nef * m0] * o;
+          }
+          acc[L.klc + q + 9 * ef] += wk * sL;
+        }
+      if (doNjob)
+        for (int it = t; it < 9 * nval; it += 256) {
+          const int q = it % 9, m0 = it / 9;
+          double o = 0.0;
+          for (int u = 0; u < P; ++u) o += S2[q * 256 + u * npad + m0];
+          acc[L.nlc + q + 9 * m0] += wk * o;
+        }
+      __syncthreads();
+    }
+    // ---- hw-resolved Hall and optical conductivity ---------------------------
+    if (doAHC)
+      for (int h = 0; h < nhw; ++h)
+        for (int ef = 0; ef < nef; ++ef) {
+          const zd wh = zd{b.hw[h], b.eta};
+          const zd wh2 = zmul(wh, wh);
+          double sv[18], so[18];
+#pragma unroll
+          for (int q = 0; q < 18; ++q) sv[q] = so[q] = 0.0;
+          for (int idx = t; idx < n2; idx += 256) {
+            const int x = idx % n, y = idx / n;
+            if (x == y) continue;
+            // pairs of equal occupation contribute exactly zero
+            const double dfv = fd[ef + nef * x] - fd[ef + nef * y];
+            if (dfv == 0.0) continue;
+            const double dE = Es[x] - Es[y];
+            const zd denv = zd{dE * dE - wh2.re, -wh2.im};
+            const zd deno = zmul(zd{dE, 0.0}, zd{dE - wh.re, -wh.im});
+            const double iv = dfv / zabs2(denv), io = dfv / zabs2(deno);
+            const zd rv = zd{denv.re * iv, -denv.im * iv};
+            const zd ro = zd{deno.re * io, -deno.im * io};
+            zd vxy[3], vyx[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+              vxy[a] = Vk[a * (int64_t)n2 + x + (int64_t)n * y];
+              vyx[a] = Vk[a * (int64_t)n2 + y + (int64_t)n * x];
+            }
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+#pragma unroll
+              for (int i = 0; i < 3; ++i) {
+                const int q = i + 3 * j;
+                const double vim = zmul(vxy[i], vyx[j]).im;
+                sv[2 * q] += vim * rv.re;
+                sv[2 * q + 1] += vim * rv.im;
+                const zd q2 = zmuli(zmul(zmul(vyx[i], vxy[j]), ro));
+                so[2 * q] += q2.re;
+                so[2 * q + 1] += q2.im;
+              }
+          }
+#pragma unroll
+          for (int q = 0; q < 18; ++q) {
+            sv[q] = wave_sum(sv[q]);
+            so[q] = wave_sum(so[q]);
+          }
+          if (lane == 0)
+#pragma unroll
+            for (int q = 0; q < 18; ++q) {
+              red[wv * 36 + q] = sv[q];
+              red[wv * 36 + 18 + q] = so[q];
+            }
+          __syncthreads();
+          if (t < 36) {
+            const double sR = red[t] + red[36 + t] + red[72 + t] + red[108 + t];
+            double* dst = (t < 18) ? acc + L.velo + 2 * 9 * (h + (int64_t)nhw * ef) + t
+                                   : acc + L.ohc + 2 * 9 * (h + (int64_t)nhw * ef) + (t - 18);
+            *dst += wk * sR;
+          }
+          __syncthreads();
+        }
+    // ---- itinerant (IC) terms on MFMA ----------------------------------------
+    if (doK || doNjob) {
+      const int nt = (n + 15) / 16;
+      const int lr = lane & 15, kq = lane >> 4;
+      for (int item = wv; item < 3 * nt; item += 4) {
+        const int j = item / nt, tc = item % nt;
+        const int kk_ = (j + 1) % 3, ll_ = (j + 2) % 3;
+        const zd* Vk_ = Vk + kk_ * (int64_t)n2;
+        const zd* Vl_ = Vk + ll_ * (int64_t)n2;
+        const int c0 = 16 * tc, n0 = c0 + lr;
+        const bool colok = n0 < n;
+        const bool doN = doNjob && colok && n0 < nval;
+        double f2[3] = {0.0, 0.0, 0.0}, f2m[3] = {0.0, 0.0, 0.0};
+        for (int tr = 0; tr < nt; ++tr) {
+          const int r0 = 16 * tr, ar = r0 + lr;
+          mep_v4d p1 = {0, 0, 0, 0}, p2 = {0, 0, 0, 0}, p3 = {0, 0, 0, 0};
+          for (int m0 = 0; m0 < n; m0 += 4) {
+            const int m = m0 + kq;
+            zd a0 = zd{0.0, 0.0}, a1 = zd{0.0, 0.0}, b0 = zd{0.0, 0.0}, b1 = zd{0.0, 0.0};
+            if (m < n) {
+              if (ar < n) {
+                a0 = Vk_[ar + (int64_t)n * m];
+                a1 = Vl_[ar + (int64_t)n * m];
+              }
+              if (colok) {
+                const double rr = rinv(Es, m, n0);
+                b0 = zscale(Vl_[m + (int64_t)n * n0], rr);
+                b1 = zscale(Vk_[m + (int64_t)n * n0], -rr);
+              }
+            }
+            p1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0.re, b0.re, p1, 0, 0, 0);
+            p2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0.im, b0.im, p2, 0, 0, 0);
+            p3 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0.re + a0.im, b0.re + b0.im, p3, 0, 0, 0);
+            p1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1.re, b1.re, p1, 0, 0, 0);
+            p2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1.im, b1.im, p2, 0, 0, 0);
+            p3 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1.re + a1.im, b1.re + b1.im, p3, 0, 0, 0);
+          }
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int row = r0 + kq + 4 * v;
+            if (!colok || row >= n || row == n0) continue;
+            const zd cv = zd{p1[v] - p2[v], p3[v] - p1[v] - p2[v]};
+            const double rr = rinv(Es, row, n0);
+            const double rr2 = rr * rr;
+            zd ai[3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+              ai[i] = zscale(Vk[i * (int64_t)n2 + n0 + (int64_t)n * row], rr2);
+              f2[i] -= ai[i].re * cv.re - ai[i].im * cv.im;
+            }
+            if (doN) {
+              const int lo = win[2 * (n0 * n + row)], hi = win[2 * (n0 * n + row) + 1];
+              if (lo == 0 && hi == n - 1) continue;  // every m excluded
+              zd kept = zd{0.0, 0.0};
+              auto term = [&](int mm) {
+                const double rm = rinv(Es, mm, n0);
+                const zd bl = zscale(Vl_[mm + (int64_t)n * n0], rm);
+                const zd bk = zscale(Vk_[mm + (int64_t)n * n0], rm);
+                return zsub(zmul(Vk_[row + (int64_t)n * mm], bl), zmul(Vl_[row + (int64_t)n * mm], bk));
+              };
+              if (2 * (hi - lo + 1) > n) {
+                for (int mm = 0; mm < lo; ++mm) kept = zadd(kept, term(mm));
+                for (int mm = hi + 1; mm < n; ++mm) kept = zadd(kept, term(mm));
+              } else {
+                zd ex = zd{0.0, 0.0};
+                for (int mm = lo; mm <= hi; ++mm)
+                  if (mm != n0) ex = zadd(ex, term(mm));
+                kept = zsub(cv, ex);
+              }
+#pragma unroll
+              for (int i = 0; i < 3; ++i) f2m[i] -= ai[i].re * kept.re - ai[i].im * kept.im;
+            }
+          }
+        }
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          f2[i] += __shfl_xor(f2[i], 16, 64);
+          f2[i] += __shfl_xor(f2[i], 32, 64);
+          f2m[i] += __shfl_xor(f2m[i], 16, 64);
+          f2m[i] += __shfl_xor(f2m[i], 32, 64);
+        }
+        if (kq == 0 && colok)
+#pragma unroll
+          for (int i = 0; i < 3; ++i) {
+            F2[(i + 3 * j) * n + n0] = f2[i];
+            F2m[(i + 3 * j) * n + n0] = f2m[i];
+          }
+      }
+      __syncthreads();
+      if (doK)
+        for (int it = t; it < 9 * nef; it += 256) {
+          const int q = it % 9, ef = it / 9;
+          double sI = 0.0;
+          for (int m0 = 0; m0 < n; ++m0) sI += fd[ef + nef * m0] * F2[q * n + m0];
+          acc[L.kic + q + 9 * ef] += wk * sI;
+        }
+      if (doNjob)
+        for (int it = t; it < 9 * nval; it += 256) {
+          const int q = it % 9, m0 = it / 9;
+          acc[L.nic + q + 9 * m0] += wk * F2m[q * n + m0];
+        }
+    }
+  }
+}
+
+size_t bandsum_wide_lds_bytes(const BandArgs& b) {
+  const size_t n = b.n;
+  return sizeof(double) * (n + (size_t)b.n_ef * n + 2 * 9 * 256 + 2 * 9 * n + 4 * 36) +
+         2 * (size_t)(b.n_val > 0 ? b.n_val : 1) * n;
+}
+
 __global__ void reduce_parts_kernel(int nblocks, int64_t len, const double* __restrict__ part,
                                     double* __restrict__ acc) {
   int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -531,14 +876,15 @@ static void launch_bs(hipStream_t s, const BandArgs& b, double Tsmear, const Acc
                       int64_t nk, const double* E, const zd* V, const zd* A, const zd* Om,
                       const double* w, int nblocks, double* part) {
   size_t lds = BsLds<NMAX>::bytes(b.n_ef, L.total);
-  hipFuncSetAttribute((const void*)bandsum_kernel<NMAX>,
-                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  (void)hipFuncSetAttribute((const void*)bandsum_kernel<NMAX>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   hipLaunchKernelGGL(bandsum_kernel<NMAX>, dim3(nblocks), dim3(BS_THREADS), lds, s, b, Tsmear, L,
                      nk, E, V, A, Om, w, part);
 }
 
 size_t bandsum_lds_bytes(const BandArgs& b) {
   AccLayout L = acc_layout(b);
+  if (b.n > 32) return bandsum_wide_lds_bytes(b);
   return b.n <= 16 ? BsLds<16>::bytes(b.n_ef, L.total) : BsLds<32>::bytes(b.n_ef, L.total);
 }
 
@@ -548,10 +894,17 @@ void launch_bandsum(hipStream_t s, const BandArgs& b, int64_t nk, const double* 
   // kBoltz_Eh_K * T  (constants.f90:22, statistics.f90:77)
   const double kB = 8.6173303 * 1e-5 / 27.211385;
   double Tsmear = kB * b.T_kelvin;
-  if (b.n <= 16)
+  if (b.n > 32) {
+    const size_t lds = bandsum_wide_lds_bytes(b);
+    (void)hipFuncSetAttribute((const void*)bandsum_wide_kernel,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(bandsum_wide_kernel, dim3(nblocks), dim3(256), lds, s, b, Tsmear, L, nk, E,
+                       V, w, part);
+  } else if (b.n <= 16) {
     launch_bs<16>(s, b, Tsmear, L, nk, E, V, A, Om, w, nblocks, part);
-  else
+  } else {
     launch_bs<32>(s, b, Tsmear, L, nk, E, V, A, Om, w, nblocks, part);
+  }
 }
 
 void launch_reduce_parts(hipStream_t s, int nblocks, int64_t len, const double* part, double* acc) {

@@ -196,7 +196,7 @@ extern "C" int mep_ctx_create(int64_t n_wf, int64_t n_R, const double* R_frac,
   *out = nullptr;
   if (n_wf <= 0 || n_R <= 0 || !R_frac || !H_real || !a_latt)
     return fail(MEP_ERR_ARG, "mep_ctx_create: n_wf, n_R, R_frac, H_real and a_latt are required");
-  if (n_wf > 32) return fail(MEP_ERR_ARG, "mep_ctx_create: n_wf > 32 is not supported");
+  if (n_wf > 128) return fail(MEP_ERR_ARG, "mep_ctx_create: n_wf > 128 is not supported");
   if (r_real && n_wf > 16)
     return fail(MEP_ERR_ARG, "mep_ctx_create: a position operator needs n_wf <= 16");
   if (deg && (flags & MEP_F_APPLY_DEG))
@@ -428,6 +428,7 @@ static int interp_block(mep_ctx* c, int64_t nb, bool velo, int prec = MEP_PREC_F
   va.pos_op = c->pos ? 1 : 0;
   va.textbook = (c->flags & MEP_F_TEXTBOOK) ? 1 : 0;
   va.kubo_tol = c->kubo_tol;
+  va.scratch = reinterpret_cast<zd*>(c->dscr);  // free once the eigensolve is done
   if (prec == MEP_PREC_MIXED) {
     // FP32 phase tile, FP32 GEMM, FP32 eigensolve (U in V's slot), FP64
     // refinement (U -> dU), FP64 velocities from the FP32 dH
@@ -458,8 +459,12 @@ static int interp_block(mep_ctx* c, int64_t nb, bool velo, int prec = MEP_PREC_F
                        reinterpret_cast<double*>(c->dHk));
   });
   timed(c, K_EIGH, [&] {
-    launch_eigh(c->st, (int)n, nb, c->dHk, c->ncols, c->packed, g, c->dE, c->dU, c->dinfo,
-                c->dscr);
+    if (n > 32)
+      launch_eigh_wide(c->st, (int)n, nb, c->dHk, c->ncols, c->packed, g, c->dE, c->dU, c->dinfo,
+                       c->dscr);
+    else
+      launch_eigh(c->st, (int)n, nb, c->dHk, c->ncols, c->packed, g, c->dE, c->dU, c->dinfo,
+                  c->dscr);
   });
   if (velo)
     timed(c, K_VELO, [&] {
@@ -507,6 +512,8 @@ static int band_setup(mep_ctx* c, const mep_run_params* p, int64_t kblock, BandS
     return fail(MEP_ERR_ARG, "unknown precision");
   if (p->precision == MEP_PREC_MIXED && c->pos)
     return fail(MEP_ERR_ARG, "the mixed-precision path does not take a position operator");
+  if (p->precision == MEP_PREC_MIXED && c->n > 32)
+    return fail(MEP_ERR_ARG, "the mixed-precision path supports n_wf <= 32");
   BandArgs& b = bs.b;
   b.n = (int)c->n;
   b.jobs = p->jobs;

@@ -4,7 +4,8 @@
 // get_wann_interp (wann_interp.f90:95).  Contract kept: only the upper
 // triangle of H(k) is read, eigenvalues ascending, eigenvectors in columns.
 //
-// Supported sizes: n <= 32 (the caller checks).
+// Supported sizes: n <= 32 here, 32 < n <= 128 in the wide kernels below
+// (the caller checks).
 // Three kernels per k-block:
 //   eigh_house (a team of NMAX lanes per k, lane i <-> row i, matrix in LDS):
 //      Householder reduction A = Q T Q^H with rank-2 trailing updates; the
@@ -166,6 +167,101 @@ __global__ __launch_bounds__(64) void eigh_house_kernel(int n, int64_t nk,
 // appended, in application order, to the k-point's flat stream
 // rot[k][q] = (c, s), col[k][q] = i; the count goes to the side record.
 // Ascending eigenvalues go to E, each band's rank to the side record.
+// The QL recurrence of one k-point, run by one lane; d and e live in LDS
+// columns ds[i * LANES], es[i * LANES] (already offset to the lane).
+template <int LANES, class R>
+__device__ inline void ql_one(int n, int64_t k, R* ds, R* es, R* __restrict__ side,
+                              R* __restrict__ rot, uint8_t* __restrict__ rcol,
+                              double* __restrict__ E, int* __restrict__ info) {
+  const R EPS = cplx_of<R>::eps;
+  const int64_t cap = rot_cap(n);
+  const R* sd = side + k * side_len(n);
+  for (int i = 0; i < n; ++i) {
+    ds[i * LANES] = sd[i];
+    es[i * LANES] = sd[n + i];
+  }
+  R* rk = rot + k * 2 * cap;
+  uint8_t* ck = rcol + k * cap;
+  int64_t pos = 0;
+  int fail = 0;
+  for (int l = 0; l < n && !fail; ++l) {
+    int iter = 0;
+    for (;;) {
+      int m = l;
+      for (; m < n - 1; ++m) {
+        const R dd = fabs(ds[m * LANES]) + fabs(ds[(m + 1) * LANES]);
+        if (fabs(es[m * LANES]) <= EPS * dd) break;
+      }
+      if (m == l) break;
+      if (++iter > 60 || pos + (m - l) > cap) {
+        fail = 1;
+        break;
+      }
+      const R dl = ds[l * LANES], el = es[l * LANES];
+      R g = (ds[(l + 1) * LANES] - dl) / (R(2) * el);
+      R r = sqrt(g * g + R(1));
+      g = ds[m * LANES] - dl + el / (g + copysign(r, g));
+      R s = R(1), c = R(1), p = R(0);
+      bool underflow = false;
+      R ei = es[(m - 1) * LANES], di = ds[(m - 1) * LANES];
+      R di1 = ds[m * LANES];
+      for (int i = m - 1; i >= l; --i) {
+        // operands of the next step are not touched by this one
+        const R ein = (i > l) ? es[(i - 1) * LANES] : R(0);
+        const R din = (i > l) ? ds[(i - 1) * LANES] : R(0);
+        const R f = s * ei, b = c * ei;
+        r = sqrt(f * f + g * g);
+        es[(i + 1) * LANES] = r;
+        if (r == R(0)) {
+          ds[(i + 1) * LANES] = di1 - p;
+          es[m * LANES] = R(0);
+          underflow = true;
+          break;
+        }
+        const R rr = R(1) / r;
+        s = f * rr;
+        c = g * rr;
+        g = di1 - p;
+        r = (di - g) * s + R(2) * c * b;
+        p = s * r;
+        ds[(i + 1) * LANES] = g + p;
+        g = c * r - b;
+        rk[2 * pos] = c;
+        rk[2 * pos + 1] = s;
+        ck[pos] = (uint8_t)i;
+        pos += 1;
+        ei = ein;
+        di1 = di;
+        di = din;
+      }
+      if (underflow) continue;
+      ds[l * LANES] -= p;
+      es[l * LANES] = g;
+      es[m * LANES] = R(0);
+    }
+  }
+  if (fail) atomicOr(info, 1);
+  // ascending order: rank of each eigenvalue (ties by index)
+  for (int j = 0; j < n; ++j) {
+    const R dj = ds[j * LANES];
+    int rnk = 0;
+    for (int i = 0; i < n; ++i) {
+      const R di = ds[i * LANES];
+      rnk += (di < dj) || (di == dj && i < j);
+    }
+    E[k * n + rnk] = dj;
+    es[j * LANES] = (R)rnk;  // e is no longer needed: keep the rank
+  }
+  R* sdw = side + k * side_len(n);
+  for (int j = 0; j < n; ++j) sdw[n + j] = es[j * LANES];  // rank over e's slot
+  sdw[5 * n] = (R)pos;
+}
+
+// One lane per k-point: implicit QL on the real tridiagonal (d, e) held in
+// LDS columns.  Every Givens rotation G(i, i+1; c, s) of every sweep is
+// appended, in application order, to the k-point's flat stream
+// rot[k][q] = (c, s), col[k][q] = i; the count goes to the side record.
+// Ascending eigenvalues go to E, each band's rank to the side record.
 template <int NMAX, class R>
 __global__ __launch_bounds__(64) void eigh_ql_kernel(int n, int64_t nk,
                                                      R* __restrict__ side,
@@ -173,95 +269,30 @@ __global__ __launch_bounds__(64) void eigh_ql_kernel(int n, int64_t nk,
                                                      uint8_t* __restrict__ rcol,
                                                      double* __restrict__ E,
                                                      int* __restrict__ info) {
-  using Z = typename cplx_of<R>::type;
   __shared__ R ds[NMAX * 64], es[NMAX * 64];
   const int lane = threadIdx.x;
-  const R EPS = cplx_of<R>::eps;
-  const int64_t cap = rot_cap(n);
   for (int64_t kb = (int64_t)blockIdx.x * 64; kb < nk; kb += (int64_t)gridDim.x * 64) {
     const int64_t k = kb + lane;
-    if (k < nk) {
-      const R* sd = side + k * side_len(n);
-      for (int i = 0; i < n; ++i) {
-        ds[i * 64 + lane] = sd[i];
-        es[i * 64 + lane] = sd[n + i];
-      }
-      R* rk = rot + k * 2 * cap;
-      uint8_t* ck = rcol + k * cap;
-      int64_t pos = 0;
-      int fail = 0;
-      for (int l = 0; l < n && !fail; ++l) {
-        int iter = 0;
-        for (;;) {
-          int m = l;
-          for (; m < n - 1; ++m) {
-            const R dd = fabs(ds[m * 64 + lane]) + fabs(ds[(m + 1) * 64 + lane]);
-            if (fabs(es[m * 64 + lane]) <= EPS * dd) break;
-          }
-          if (m == l) break;
-          if (++iter > 60 || pos + (m - l) > cap) {
-            fail = 1;
-            break;
-          }
-          const R dl = ds[l * 64 + lane], el = es[l * 64 + lane];
-          R g = (ds[(l + 1) * 64 + lane] - dl) / (R(2) * el);
-          R r = sqrt(g * g + R(1));
-          g = ds[m * 64 + lane] - dl + el / (g + copysign(r, g));
-          R s = R(1), c = R(1), p = R(0);
-          bool underflow = false;
-          R ei = es[(m - 1) * 64 + lane], di = ds[(m - 1) * 64 + lane];
-          R di1 = ds[m * 64 + lane];
-          for (int i = m - 1; i >= l; --i) {
-            // operands of the next step are not touched by this one
-            const R ein = (i > l) ? es[(i - 1) * 64 + lane] : R(0);
-            const R din = (i > l) ? ds[(i - 1) * 64 + lane] : R(0);
-            const R f = s * ei, b = c * ei;
-            r = sqrt(f * f + g * g);
-            es[(i + 1) * 64 + lane] = r;
-            if (r == R(0)) {
-              ds[(i + 1) * 64 + lane] = di1 - p;
-              es[m * 64 + lane] = R(0);
-              underflow = true;
-              break;
-            }
-            const R rr = R(1) / r;
-            s = f * rr;
-            c = g * rr;
-            g = di1 - p;
-            r = (di - g) * s + R(2) * c * b;
-            p = s * r;
-            ds[(i + 1) * 64 + lane] = g + p;
-            g = c * r - b;
-            rk[2 * pos] = c;
-            rk[2 * pos + 1] = s;
-            ck[pos] = (uint8_t)i;
-            pos += 1;
-            ei = ein;
-            di1 = di;
-            di = din;
-          }
-          if (underflow) continue;
-          ds[l * 64 + lane] -= p;
-          es[l * 64 + lane] = g;
-          es[m * 64 + lane] = R(0);
-        }
-      }
-      if (fail) atomicOr(info, 1);
-      // ascending order: rank of each eigenvalue (ties by index)
-      for (int j = 0; j < n; ++j) {
-        const R dj = ds[j * 64 + lane];
-        int rnk = 0;
-        for (int i = 0; i < n; ++i) {
-          const R di = ds[i * 64 + lane];
-          rnk += (di < dj) || (di == dj && i < j);
-        }
-        E[k * n + rnk] = dj;
-        es[j * 64 + lane] = (R)rnk;  // e is no longer needed: keep the rank
-      }
-      R* sdw = side + k * side_len(n);
-      for (int j = 0; j < n; ++j) sdw[n + j] = es[j * 64 + lane];  // rank over e's slot
-      sdw[5 * n] = (R)pos;
-    }
+    if (k < nk) ql_one<64, R>(n, k, ds + lane, es + lane, side, rot, rcol, E, info);
+  }
+}
+
+// the same for 32 < n <= 128: 32 lanes per workgroup, d and e in dynamic LDS
+// (n x 32 each)
+template <class R>
+__global__ __launch_bounds__(32) void eigh_ql_wide_kernel(int n, int64_t nk,
+                                                          R* __restrict__ side,
+                                                          R* __restrict__ rot,
+                                                          uint8_t* __restrict__ rcol,
+                                                          double* __restrict__ E,
+                                                          int* __restrict__ info) {
+  extern __shared__ double ql_smem[];
+  R* ds = reinterpret_cast<R*>(ql_smem);
+  R* es = ds + (int64_t)n * 32;
+  const int lane = threadIdx.x;
+  for (int64_t kb = (int64_t)blockIdx.x * 32; kb < nk; kb += (int64_t)gridDim.x * 32) {
+    const int64_t k = kb + lane;
+    if (k < nk) ql_one<32, R>(n, k, ds + lane, es + lane, side, rot, rcol, E, info);
   }
 }
 
@@ -401,6 +432,295 @@ __global__ __launch_bounds__(64) void eigh_vec_kernel(int n, int64_t nk,
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
+}
+
+// ===========================================================================
+// 32 < n <= 128: one 256-thread workgroup per k-point for the reduction and
+// the back-transformation; the QL stage is shared (32 lanes per workgroup).
+// ===========================================================================
+
+// packed lower triangle (column-major): element (i >= j) at lo_off(j) + i
+__device__ inline int lo_off(int j, int n) { return j * n - (j * (j + 1)) / 2; }
+
+// block-wide sum of one double over 256 threads (every thread gets it);
+// red holds 4 doubles; the call ends with a barrier
+__device__ inline double block_sum256(double v, double* red) {
+  v = wave_sum(v);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  return red[0] + red[1] + red[2] + red[3];
+}
+
+// Householder reduction with the Hermitian matrix as a packed lower
+// triangle in LDS (n(n+1)/2 complex, 132 KB at n = 128).  Output as the
+// narrow kernel: reflectors in the strict lower triangle of U's slot, d, the
+// subdiagonal phases and tau in the side record.
+__global__ __launch_bounds__(256) void eigh_house_wide_kernel(int n, int64_t nk,
+                                                              const zd* __restrict__ Hk,
+                                                              int64_t strideHk, int packed,
+                                                              const double* __restrict__ kpts,
+                                                              const double* __restrict__ tau,
+                                                              zd* __restrict__ U,
+                                                              double* __restrict__ side) {
+  extern __shared__ double hw_smem[];
+  const int np = n * (n + 1) / 2;
+  zd* Lw = reinterpret_cast<zd*>(hw_smem);
+  zd* vv = Lw + np;
+  zd* ww = vv + n;
+  zd* pp = ww + n;    // [4][n] partial products
+  zd* sub = pp + 4 * n;
+  double* tauv = reinterpret_cast<double*>(sub + n);
+  double* red = tauv + n;  // 4
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const double two_pi = 6.283185307179586476925;
+  for (int64_t k = blockIdx.x; k < nk; k += gridDim.x) {
+    __syncthreads();
+    // lower triangle: L(i, j) = conj(H(j, i)) with the tight-binding phase
+    for (int idx = t; idx < n * n; idx += 256) {
+      const int i = idx % n, j = idx / n;  // H(j, i), j <= i
+      if (j > i) continue;
+      const int64_t o = packed ? j + ((int64_t)i * (i + 1)) / 2 : j + (int64_t)n * i;
+      zd h = Hk[k * strideHk + o];
+      if (tau && i != j) {
+        const double kx = kpts[3 * k], ky = kpts[3 * k + 1], kz = kpts[3 * k + 2];
+        const double ang = two_pi * (kx * (tau[3 * i] - tau[3 * j]) +
+                                     ky * (tau[3 * i + 1] - tau[3 * j + 1]) +
+                                     kz * (tau[3 * i + 2] - tau[3 * j + 2]));
+        double sn, cs;
+        sincos(ang, &sn, &cs);
+        h = zmul(zd{cs, sn}, h);
+      }
+      Lw[lo_off(j, n) + i] = (i == j) ? zd{h.re, 0.0} : zconj(h);
+    }
+    __syncthreads();
+    for (int kc = 0; kc + 2 < n; ++kc) {
+      const int m = n - kc - 1;
+      zd* col = Lw + lo_off(kc, n) + kc + 1;  // x_r = col[r], r < m
+      const zd x = (t < m) ? col[t] : zd{0.0, 0.0};
+      const double tail2 = block_sum256((t >= 1 && t < m) ? zabs2(x) : 0.0, red);
+      const zd x0 = col[0];
+      const bool refl = tail2 > 0.0;
+      const double ax02 = zabs2(x0);
+      const double xn = sqrt(tail2 + ax02);
+      const double ax0 = sqrt(ax02);
+      const zd alpha = !refl ? x0 : ((ax0 > 0.0) ? zscale(x0, -xn / ax0) : zd{-xn, 0.0});
+      const zd v0 = zsub(x0, alpha);
+      const double vn2 = tail2 + zabs2(v0);
+      const double tk = refl ? 2.0 / vn2 : 0.0;
+      if (t < m) vv[t] = refl ? ((t == 0) ? v0 : x) : zd{0.0, 0.0};
+      __syncthreads();
+      // p = A v over the trailing block; wave wv takes columns c = wv mod 4
+      const int base = kc + 1;
+      for (int r = lane; r < m; r += 64) {
+        const int row = base + r;
+        zd acc = zd{0.0, 0.0};
+        for (int c = wv; c < m; c += 4) {
+          const int cl = base + c;
+          const zd a = (row >= cl) ? Lw[lo_off(cl, n) + row] : zconj(Lw[lo_off(row, n) + cl]);
+          acc = zadd(acc, zmul(a, vv[c]));
+        }
+        pp[wv * n + r] = acc;
+      }
+      __syncthreads();
+      zd p = zd{0.0, 0.0};
+      if (t < m) {
+        p = zadd(zadd(pp[t], pp[n + t]), zadd(pp[2 * n + t], pp[3 * n + t]));
+        p = zscale(p, tk);
+      }
+      const zd vt = (t < m) ? vv[t] : zd{0.0, 0.0};
+      const double K = 0.5 * tk * block_sum256((t < m) ? (vt.re * p.re + vt.im * p.im) : 0.0, red);
+      if (t < m) ww[t] = zsub(p, zscale(vt, K));
+      __syncthreads();
+      if (refl)
+        for (int c = wv; c < m; c += 4) {
+          const zd vc = vv[c], wc = ww[c];
+          zd* colc = Lw + lo_off(base + c, n) + base;
+          for (int r = c + lane; r < m; r += 64) {
+            const zd upd = zadd(zmul(vv[r], zconj(wc)), zmul(ww[r], zconj(vc)));
+            colc[r] = zsub(colc[r], upd);
+          }
+        }
+      __syncthreads();
+      if (t < m) col[t] = vv[t];
+      if (t == 0) {
+        sub[kc] = alpha;
+        tauv[kc] = tk;
+      }
+      __syncthreads();
+    }
+    if (t == 0 && n >= 2) {
+      sub[n - 2] = Lw[lo_off(n - 2, n) + n - 1];
+      tauv[n - 2] = 0.0;
+    }
+    __syncthreads();
+    double* sd = side + k * side_len(n);
+    zd* Uk = U + k * (int64_t)n * n;
+    for (int idx = t; idx < n * n; idx += 256) {
+      const int i = idx % n, j = idx / n;
+      if (i > j) Uk[idx] = Lw[lo_off(j, n) + i];
+    }
+    if (t < n) {
+      sd[t] = Lw[lo_off(t, n) + t].re;
+      sd[2 * n + t] = (t < n - 1) ? tauv[t] : 0.0;
+    }
+    if (t == 0) {
+      zd ph = zd{1.0, 0.0};
+      sd[3 * n] = 1.0;
+      sd[3 * n + 1] = 0.0;
+      for (int i = 0; i + 1 < n; ++i) {
+        const zd s_ = sub[i];
+        const double as = sqrt(zabs2(s_));
+        sd[n + i] = as;
+        if (as > 0.0) ph = zmul(ph, zscale(s_, 1.0 / as));
+        sd[3 * n + 2 * (i + 1)] = ph.re;
+        sd[3 * n + 2 * (i + 1) + 1] = ph.im;
+      }
+      sd[n + n - 1] = 0.0;
+    }
+  }
+}
+
+size_t eigh_house_wide_lds(int n) {
+  return sizeof(zd) * ((size_t)n * (n + 1) / 2 + 8 * (size_t)n) + sizeof(double) * ((size_t)n + 4);
+}
+
+// Rotation replay and back-transformation for 32 < n <= 128: thread r <-> row
+// r of Z (LDS, column-major with odd pitch), each wave staging the rotation
+// stream through its own LDS slots; then thread j <-> column j of W = P Z,
+// kept in the k-point's (consumed) rotation record in HBM, row-major so the
+// reflector updates are coalesced.  The reflectors are read from U's slot
+// (broadcast loads) and U is overwritten only after every thread is done.
+__global__ __launch_bounds__(128) void eigh_vec_wide_kernel(int n, int64_t nk,
+                                                            const double* __restrict__ side,
+                                                            double* __restrict__ rot,
+                                                            const uint8_t* __restrict__ rcol,
+                                                            zd* __restrict__ U) {
+  extern __shared__ double vw_smem[];
+  const int LDZ = n | 1;
+  double* Zt = vw_smem;                       // [n][LDZ]
+  double* csb_all = Zt + (int64_t)n * LDZ;    // [waves][2*64]
+  int* ixb_all = reinterpret_cast<int*>(csb_all + 2 * 64 * (blockDim.x / 64));
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  double* csb = csb_all + 2 * 64 * wv;
+  int* ixb = ixb_all + 64 * wv;
+  const int64_t cap = rot_cap(n);
+  for (int64_t k = blockIdx.x; k < nk; k += gridDim.x) {
+    __syncthreads();
+    const double* sd = side + k * side_len(n);
+    if (t < n)
+      for (int c = 0; c < n; ++c) Zt[c * LDZ + t] = (c == t) ? 1.0 : 0.0;
+    const int64_t Q = (int64_t)sd[5 * n];
+    const double* rk = rot + k * 2 * cap;
+    const uint8_t* ck = rcol + k * cap;
+    double nc = 0.0, ns = 0.0;
+    int ni = 0;
+    if (lane < Q) {
+      nc = rk[2 * lane];
+      ns = rk[2 * lane + 1];
+      ni = ck[lane];
+    }
+    int prev = -2;
+    double carry = 0.0;
+    for (int64_t q0 = 0; q0 < Q; q0 += 64) {
+      __builtin_amdgcn_wave_barrier();
+      csb[2 * lane] = nc;
+      csb[2 * lane + 1] = ns;
+      ixb[lane] = ni;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      const int64_t qn = q0 + 64 + lane;
+      if (qn < Q) {
+        nc = rk[2 * qn];
+        ns = rk[2 * qn + 1];
+        ni = ck[qn];
+      }
+      const int cnt = (Q - q0 < 64) ? (int)(Q - q0) : 64;
+      if (t < n)
+        for (int r = 0; r < cnt; ++r) {
+          const double c = csb[2 * r], s = csb[2 * r + 1];
+          const int i = ixb[r];
+          const double zi = Zt[i * LDZ + t];
+          const double zi1 = (prev == i + 1) ? carry : Zt[(i + 1) * LDZ + t];
+          Zt[(i + 1) * LDZ + t] = s * zi + c * zi1;
+          carry = c * zi - s * zi1;
+          Zt[i * LDZ + t] = carry;
+          prev = i;
+        }
+    }
+    __syncthreads();
+    // W = P Z, row-major in the consumed rotation record
+    zd* W = reinterpret_cast<zd*>(rot + k * 2 * cap);
+    const zd* Uk = U + k * (int64_t)n * n;
+    const int j = t;
+    if (j < n)
+      for (int i = 0; i < n; ++i)
+        W[(int64_t)i * n + j] = zscale(zd{sd[3 * n + 2 * i], sd[3 * n + 2 * i + 1]}, Zt[j * LDZ + i]);
+    // reflectors in reverse order; every thread owns its column
+    if (j < n)
+      for (int kc = n - 3; kc >= 0; --kc) {
+        const double tk = sd[2 * n + kc];
+        if (tk == 0.0) continue;
+        const zd* vcol = Uk + (int64_t)n * kc;
+        zd sacc = zd{0.0, 0.0};
+        for (int i = kc + 1; i < n; ++i) sacc = zadd(sacc, zcmul(vcol[i], W[(int64_t)i * n + j]));
+        sacc = zscale(sacc, tk);
+        for (int i = kc + 1; i < n; ++i)
+          W[(int64_t)i * n + j] = zsub(W[(int64_t)i * n + j], zmul(vcol[i], sacc));
+      }
+    // gauge: largest-modulus component (first on ties) real > 0
+    zd phs = zd{1.0, 0.0};
+    if (j < n) {
+      double best = -1.0;
+      zd u = zd{1.0, 0.0};
+      for (int i = 0; i < n; ++i) {
+        const zd wi = W[(int64_t)i * n + j];
+        const double m2 = zabs2(wi);
+        if (m2 > best) {
+          best = m2;
+          u = wi;
+        }
+      }
+      const double au = sqrt(zabs2(u));
+      if (au > 0.0) phs = zd{u.re / au, -u.im / au};
+    }
+    __syncthreads();  // every reflector read before U is overwritten
+    if (j < n) {
+      const int rnk = (int)sd[n + j];
+      zd* Uo = U + k * (int64_t)n * n + (int64_t)n * rnk;
+      for (int i = 0; i < n; ++i) Uo[i] = zmul(W[(int64_t)i * n + j], phs);
+    }
+  }
+}
+
+size_t eigh_vec_wide_lds(int n) {
+  return sizeof(double) * ((size_t)n * (n | 1) + 2 * 64 * 2) + sizeof(int) * 64 * 2;
+}
+
+void launch_eigh_wide(hipStream_t s, int n, int64_t nk, const zd* Hk, int64_t strideHk, int packed,
+                      const KGeom& g, double* E, zd* U, int* info, double* scratch) {
+  double* side = scratch;
+  double* rot = side + nk * side_len(n);
+  uint8_t* rcol = reinterpret_cast<uint8_t*>(rot + nk * 2 * rot_cap(n));
+  const size_t lh = eigh_house_wide_lds(n), lv = eigh_vec_wide_lds(n);
+  const size_t lq = sizeof(double) * 2 * (size_t)n * 32;
+  (void)hipFuncSetAttribute((const void*)eigh_house_wide_kernel,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lh);
+  (void)hipFuncSetAttribute((const void*)eigh_vec_wide_kernel,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lv);
+  (void)hipFuncSetAttribute((const void*)eigh_ql_wide_kernel<double>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lq);
+  unsigned grid = (unsigned)(nk < 4096 ? nk : 4096);
+  hipLaunchKernelGGL(eigh_house_wide_kernel, dim3(grid), dim3(256), lh, s, n, nk, Hk, strideHk,
+                     packed, g.kpts, g.tau, U, side);
+  int64_t qb = (nk + 31) / 32;
+  unsigned qgrid = (unsigned)(qb < 65536 ? qb : 65536);
+  hipLaunchKernelGGL(eigh_ql_wide_kernel<double>, dim3(qgrid), dim3(32), lq, s, n, nk, side, rot,
+                     rcol, E, info);
+  const int vthreads = n <= 64 ? 64 : 128;
+  hipLaunchKernelGGL(eigh_vec_wide_kernel, dim3(grid), dim3(vthreads), lv, s, n, nk, side, rot, rcol,
+                     U);
 }
 
 size_t eigh_scratch_bytes(int n, int64_t nk) {

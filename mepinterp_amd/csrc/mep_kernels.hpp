@@ -22,13 +22,16 @@ void launch_phase_f32(hipStream_t s, int64_t nk, int Kh, int has_zero, const dou
 void launch_ft_gemm_f32(hipStream_t s, int M, int N, int K, const float* Pt, const float* X,
                         float* C);
 
-// Batched Hermitian eigensolve of H(k) (upper triangle), n <= 32.
+// Batched Hermitian eigensolve of H(k) (upper triangle), n <= 32
+// (launch_eigh_wide: 32 < n <= 128, FP64).
 // E (nk, n) ascending; U (nk, n, n) column-major eigenvectors.  scratch
 // holds eigh_scratch_bytes(n, nk) bytes (side records + rotation records).
 // packed: H(k) holds the upper triangle only (element i <= j at i + j(j+1)/2).
 size_t eigh_scratch_bytes(int n, int64_t nk);
 void launch_eigh(hipStream_t s, int n, int64_t nk, const zd* Hk, int64_t strideHk, int packed,
                  const KGeom& g, double* E, zd* U, int* info, double* scratch);
+void launch_eigh_wide(hipStream_t s, int n, int64_t nk, const zd* Hk, int64_t strideHk, int packed,
+                      const KGeom& g, double* E, zd* U, int* info, double* scratch);
 // the same in FP32 arithmetic (FP32 H(k) and U; eigenvalues written as FP64)
 void launch_eigh_f32(hipStream_t s, int n, int64_t nk, const zf* Hk, int64_t strideHk, int packed,
                      const KGeom& g, double* E, zf* U, int* info, double* scratch);
@@ -42,6 +45,7 @@ struct VeloArgs {
   int pos_op;
   int textbook;
   double kubo_tol;
+  zd* scratch;  // n > 32: n x n per workgroup (up to 2048 workgroups)
 };
 void launch_velo(hipStream_t s, const VeloArgs& va, int64_t nk, const zd* Hk, int64_t strideHk,
                  const KGeom& g, const zd* U, const double* E, zd* V, zd* A, zd* Om);

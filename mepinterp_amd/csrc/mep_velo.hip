@@ -119,6 +119,65 @@ __global__ __launch_bounds__(256) void velo_mfma_kernel(int n, int64_t nk, const
   }
 }
 
+// --------------------------------------------------------------------------
+// 32 < n <= 128, no position operator: V_a = U^H dH_a U on FP64 MFMA with
+// the operands read from HBM / L2 (16 x 16 output tiles spread over the 4
+// waves, three real products per complex one).  T = dH_a U goes to a
+// per-workgroup slot Tscr[blockIdx] of n x n.
+// --------------------------------------------------------------------------
+typedef double velo_v4d __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void velo_wide_kernel(int n, int64_t nk, const zd* __restrict__ Hk,
+                                                        int64_t strideHk, int packed, TBPhase tb0,
+                                                        int gauge_trafo, const zd* __restrict__ U,
+                                                        zd* __restrict__ V, zd* __restrict__ Tscr) {
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const int lr = lane & 15, kq = lane >> 4;
+  const int n2 = n * n, nt = (n + 15) / 16;
+  zd* T = Tscr + (int64_t)blockIdx.x * n2;
+  for (int64_t k = blockIdx.x; k < nk; k += gridDim.x) {
+    const TBPhase tb = tb0.at(k);
+    const zd* X = Hk + k * strideHk;
+    const zd* Uk = U + k * (int64_t)n2;
+    zd* Vk = V + k * 3 * (int64_t)n2;
+    for (int a = 0; a < 3; ++a) {
+      if (!gauge_trafo) {
+        for (int idx = t; idx < n2; idx += 256)
+          Vk[a * (int64_t)n2 + idx] = load_tb(X, packed, n, idx % n, idx / n, 1 + a, tb);
+        continue;
+      }
+      for (int pass = 0; pass < 2; ++pass) {
+        // pass 0: T = dH_a U; pass 1: V_a = U^H T
+        for (int tile = wv; tile < nt * nt; tile += 4) {
+          const int r0 = 16 * (tile / nt), c0 = 16 * (tile % nt);
+          const int ar = r0 + lr, bc = c0 + lr;
+          velo_v4d p1 = {0, 0, 0, 0}, p2 = {0, 0, 0, 0}, p3 = {0, 0, 0, 0};
+          for (int m0 = 0; m0 < n; m0 += 4) {
+            const int m = m0 + kq;
+            zd av = zd{0.0, 0.0}, bv = zd{0.0, 0.0};
+            if (m < n) {
+              if (ar < n)
+                av = pass == 0 ? load_tb(X, packed, n, ar, m, 1 + a, tb) : zconj(Uk[m + (int64_t)n * ar]);
+              if (bc < n) bv = pass == 0 ? Uk[m + (int64_t)n * bc] : T[m + (int64_t)n * bc];
+            }
+            p1 = __builtin_amdgcn_mfma_f64_16x16x4f64(av.re, bv.re, p1, 0, 0, 0);
+            p2 = __builtin_amdgcn_mfma_f64_16x16x4f64(av.im, bv.im, p2, 0, 0, 0);
+            p3 = __builtin_amdgcn_mfma_f64_16x16x4f64(av.re + av.im, bv.re + bv.im, p3, 0, 0, 0);
+          }
+          zd* out = pass == 0 ? T : Vk + a * (int64_t)n2;
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int row = r0 + kq + 4 * v;
+            if (row < n && bc < n)
+              out[row + (int64_t)n * bc] = zd{p1[v] - p2[v], p3[v] - p1[v] - p2[v]};
+          }
+        }
+        __syncthreads();
+      }
+    }
+  }
+}
+
 // V (3,n,n) per k: planar [a][i + n j]  <->  reference (a fastest)
 __global__ void v_planar_to_ref(int n, int64_t nk, const zd* __restrict__ Vp, zd* __restrict__ Vr) {
   int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -318,6 +377,12 @@ void launch_velo(hipStream_t s, const VeloArgs& va, int64_t nk, const zd* Hk, in
   if (va.pos_op) {
     hipLaunchKernelGGL(velo_pos_kernel, dim3(grid), dim3(64), 0, s, va.n, nk, Hk, strideHk, tb,
                        va.gauge_trafo, va.textbook, va.kubo_tol, U, E, V, A, Om);
+    return;
+  }
+  if (va.n > 32) {
+    unsigned wgrid = (unsigned)(nk < 2048 ? nk : 2048);
+    hipLaunchKernelGGL(velo_wide_kernel, dim3(wgrid), dim3(256), 0, s, va.n, nk, Hk, strideHk,
+                       va.packed, tb, va.gauge_trafo, U, V, va.scratch);
     return;
   }
   hipLaunchKernelGGL(velo_mfma_kernel<zd>, dim3(grid), dim3(256), 0, s, va.n, nk, Hk, strideHk,

@@ -36,6 +36,11 @@ CASES = [
     (16, 2, True, False),
     (24, 1, True, True),
     (32, 2, False, False),
+    # 32 < n <= 128: the wide eigensolver and velocity kernels
+    (40, 1, True, False),
+    (64, 1, False, True),
+    (96, 1, False, False),
+    (128, 1, True, True),
 ]
 
 
@@ -190,14 +195,17 @@ def test_run_mesh_matches_oracle(name):
     _compare_sums(g, o, rel=1e-8, keys=keys, atol=atol)
 
 
-@pytest.mark.parametrize("n,nef,nhw", [(20, 1, 1), (32, 1, 1), (29, 17, 3)])
-def test_run_mesh_large_bands_matches_oracle(n, nef, nhw):
-    """n_wf > 16 (32-band kernel variants), a single (hw, E_F) pair-sum path
-    and more E_F points than one pass of the workgroup covers."""
+@pytest.mark.parametrize("n,nef,nhw,mp", [(20, 1, 1, (4, 3, 3)), (32, 1, 1, (4, 3, 3)),
+                                           (29, 17, 3, (4, 3, 3)), (40, 3, 4, (3, 3, 2)),
+                                           (72, 1, 1, (2, 2, 2)), (96, 2, 3, (2, 2, 1)),
+                                           (128, 1, 2, (2, 1, 1))])
+def test_run_mesh_large_bands_matches_oracle(n, nef, nhw, mp):
+    """n_wf > 16 (32-band kernel variants), a single (hw, E_F) pair-sum path,
+    more E_F points than one pass of the workgroup covers, and the wide
+    kernels for 32 < n_wf <= 128."""
     import oracle as O
     from mepinterp_amd import capi, models
     m = models.make_random(seed=40 + n, n=n, L=1)
-    mp = (4, 3, 3)
     jobs = capi.MEP_JOB_AHC | capi.MEP_JOB_KUBO | capi.MEP_JOB_MEP
     kw = dict(n_eF=nef, eF_min=-0.02, eF_max=0.02, n_hw=nhw, hw_min=0.0, hw_max=0.1,
               T_kelvin=300.0, eta=0.004, valence_bands=n // 2)
