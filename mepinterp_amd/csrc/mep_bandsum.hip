@@ -523,7 +523,7 @@ __global__ __launch_bounds__(BS_THREADS, 2) void bandsum_kernel(BandArgs b, doub
 // accumulator record lives in `part` (one writer per entry, fixed order).
 //   per-band sums      thread (n0, part) over n = part, part + P, ...; parts
 //                      combined through LDS
-//   hw-resolved sums   for each (hw, E_F) the 256 threads split the pairs
+//   hw-resolved sums   a kernel of their own (hwsum_kernel below)
 //   itinerant terms    C_j column tiles on FP64 MFMA, wave item (tc, j)
 //                      accumulating over the row tiles in registers
 // LDS: E n | fd nef*n | S 2*9*256 | F2, F2m 2*9*n | red 4*36 | windows
@@ -691,66 +691,6 @@ __global__ __launch_bounds__(256) void bandsum_wide_kernel(BandArgs b, double Ts
         }
       __syncthreads();
     }
-    // ---- hw-resolved Hall and optical conductivity ---------------------------
-    if (doAHC)
-      for (int h = 0; h < nhw; ++h)
-        for (int ef = 0; ef < nef; ++ef) {
-          const zd wh = zd{b.hw[h], b.eta};
-          const zd wh2 = zmul(wh, wh);
-          double sv[18], so[18];
-#pragma unroll
-          for (int q = 0; q < 18; ++q) sv[q] = so[q] = 0.0;
-          for (int idx = t; idx < n2; idx += 256) {
-            const int x = idx % n, y = idx / n;
-            if (x == y) continue;
-            // pairs of equal occupation contribute exactly zero
-            const double dfv = fd[ef + nef * x] - fd[ef + nef * y];
-            if (dfv == 0.0) continue;
-            const double dE = Es[x] - Es[y];
-            const zd denv = zd{dE * dE - wh2.re, -wh2.im};
-            const zd deno = zmul(zd{dE, 0.0}, zd{dE - wh.re, -wh.im});
-            const double iv = dfv / zabs2(denv), io = dfv / zabs2(deno);
-            const zd rv = zd{denv.re * iv, -denv.im * iv};
-            const zd ro = zd{deno.re * io, -deno.im * io};
-            zd vxy[3], vyx[3];
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-              vxy[a] = Vk[a * (int64_t)n2 + x + (int64_t)n * y];
-              vyx[a] = Vk[a * (int64_t)n2 + y + (int64_t)n * x];
-            }
-#pragma unroll
-            for (int j = 0; j < 3; ++j)
-#pragma unroll
-              for (int i = 0; i < 3; ++i) {
-                const int q = i + 3 * j;
-                const double vim = zmul(vxy[i], vyx[j]).im;
-                sv[2 * q] += vim * rv.re;
-                sv[2 * q + 1] += vim * rv.im;
-                const zd q2 = zmuli(zmul(zmul(vyx[i], vxy[j]), ro));
-                so[2 * q] += q2.re;
-                so[2 * q + 1] += q2.im;
-              }
-          }
-#pragma unroll
-          for (int q = 0; q < 18; ++q) {
-            sv[q] = wave_sum(sv[q]);
-            so[q] = wave_sum(so[q]);
-          }
-          if (lane == 0)
-#pragma unroll
-            for (int q = 0; q < 18; ++q) {
-              red[wv * 36 + q] = sv[q];
-              red[wv * 36 + 18 + q] = so[q];
-            }
-          __syncthreads();
-          if (t < 36) {
-            const double sR = red[t] + red[36 + t] + red[72 + t] + red[108 + t];
-            double* dst = (t < 18) ? acc + L.velo + 2 * 9 * (h + (int64_t)nhw * ef) + t
-                                   : acc + L.ohc + 2 * 9 * (h + (int64_t)nhw * ef) + (t - 18);
-            *dst += wk * sR;
-          }
-          __syncthreads();
-        }
     // ---- itinerant (IC) terms on MFMA ----------------------------------------
     if (doK || doNjob) {
       const int nt = (n + 15) / 16;
@@ -862,6 +802,148 @@ size_t bandsum_wide_lds_bytes(const BandArgs& b) {
          2 * (size_t)(b.n_val > 0 ? b.n_val : 1) * n;
 }
 
+// ===========================================================================
+// velo_ahc_tens / kubo_ohc_tens for 32 < n <= 128: per k, the pairs (x, y),
+// x != y, are staged in LDS in chunks of 256 (dE, x, y and the 27 products
+// Im(V_i(x,y) V_j(y,x)), V_i(y,x) V_j(x,y)); thread (item, split) walks every
+// split-th staged pair for its item = (hw, E_F), accumulating in registers
+// over all k; the splits are combined once at the end.  Items beyond 256 are
+// taken in further passes over the k-points.
+// ===========================================================================
+constexpr int HW_CH = 256;   // pairs per staged chunk
+constexpr int HW_PS = 30;    // doubles per staged pair: dE, x, y, 9 vim, 18 q2o
+
+__global__ __launch_bounds__(256) void hwsum_kernel(BandArgs b, double Tsmear, AccLayout L,
+                                                    int64_t nk, const double* __restrict__ E,
+                                                    const zd* __restrict__ V,
+                                                    const double* __restrict__ w,
+                                                    double* __restrict__ part) {
+  extern __shared__ double smem[];
+  const int n = b.n, nef = b.n_ef, nhw = b.n_hw;
+  double* Es = smem;
+  double* fd = Es + n;
+  double* st = fd + (int64_t)nef * n;  // [HW_CH][HW_PS]; reused for the split reduction
+  const int t = threadIdx.x;
+  const int n2 = n * n;
+  const bool finiteT = b.T_kelvin > 1e-2;
+  const int items = nhw * nef;
+  double* accp = part + (int64_t)blockIdx.x * L.total;
+  for (int g0 = 0; g0 < items; g0 += 256) {
+    const int IW = (items - g0 < 256) ? items - g0 : 256;
+    const int S = 256 / IW;
+    const int it = g0 + t % IW, split = t / IW;
+    const bool act = split < S;
+    const int h = it % nhw, ef = it / nhw;
+    const zd wh = zd{b.hw[act ? h : 0], b.eta};
+    const zd wh2 = zmul(wh, wh);
+    double sv[18], so[18];
+#pragma unroll
+    for (int q = 0; q < 18; ++q) sv[q] = so[q] = 0.0;
+    for (int64_t k = blockIdx.x; k < nk; k += gridDim.x) {
+      const double wk = w ? w[k] : 1.0;
+      const zd* Vk = V + k * 3 * (int64_t)n2;
+      __syncthreads();
+      if (t < n) Es[t] = E[k * n + t];
+      __syncthreads();
+      for (int idx = t; idx < nef * n; idx += 256) {
+        if (b.fd_in) {
+          fd[idx] = b.fd_in[k * (int64_t)nef * n + idx];
+        } else {
+          const int e_ = (nef == 1) ? 0 : idx % nef, nb = (nef == 1) ? idx : idx / nef;
+          const double e = Es[nb], f = b.ef[e_];
+          fd[idx] = finiteT ? 1.0 / (1.0 + exp((e - f) / Tsmear)) : ((e < f) ? 1.0 : 0.0);
+        }
+      }
+      for (int c0 = 0; c0 < n2; c0 += HW_CH) {
+        __syncthreads();
+        {
+          const int pidx = c0 + t;
+          double* sp = st + t * HW_PS;
+          const int x = pidx % n, y = pidx / n;
+          if (pidx < n2 && x != y) {
+            zd vxy[3], vyx[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+              vxy[a] = Vk[a * (int64_t)n2 + x + (int64_t)n * y];
+              vyx[a] = Vk[a * (int64_t)n2 + y + (int64_t)n * x];
+            }
+            sp[0] = Es[x] - Es[y];
+            sp[1] = (double)x;
+            sp[2] = (double)y;
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+#pragma unroll
+              for (int i = 0; i < 3; ++i) {
+                const int q = i + 3 * j;
+                sp[3 + q] = zmul(vxy[i], vyx[j]).im;
+                const zd q2 = zmul(vyx[i], vxy[j]);
+                sp[12 + 2 * q] = q2.re;
+                sp[12 + 2 * q + 1] = q2.im;
+              }
+          } else {
+            sp[1] = -1.0;  // no pair
+          }
+        }
+        __syncthreads();
+        if (act) {
+          const int cnt = (n2 - c0 < HW_CH) ? n2 - c0 : HW_CH;
+          for (int j = split; j < cnt; j += S) {
+            const double* sp = st + j * HW_PS;
+            const double xf = sp[1];
+            if (xf < 0.0) continue;
+            const int x = (int)xf, y = (int)sp[2];
+            // pairs of equal occupation contribute exactly zero
+            const double dfv = fd[ef + nef * x] - fd[ef + nef * y];
+            if (dfv == 0.0) continue;
+            const double dE = sp[0];
+            const zd denv = zd{dE * dE - wh2.re, -wh2.im};
+            const zd deno = zmul(zd{dE, 0.0}, zd{dE - wh.re, -wh.im});
+            const double f = wk * dfv;
+            const double av = zabs2(denv), ao = zabs2(deno);
+            const double fi = f / (av * ao);  // one division for both denominators
+            const double iv = fi * ao, io = fi * av;
+            const zd rv = zd{denv.re * iv, -denv.im * iv};
+            const zd ro = zd{deno.re * io, -deno.im * io};
+#pragma unroll
+            for (int q = 0; q < 9; ++q) {
+              const double vim = sp[3 + q];
+              sv[2 * q] += vim * rv.re;
+              sv[2 * q + 1] += vim * rv.im;
+              // i * (q2o * ro)
+              const double qr = sp[12 + 2 * q], qi = sp[12 + 2 * q + 1];
+              so[2 * q] -= qr * ro.im + qi * ro.re;
+              so[2 * q + 1] += qr * ro.re - qi * ro.im;
+            }
+          }
+        }
+      }
+    }
+    // combine the splits of each item: red[split][item][36] over the staging area
+    __syncthreads();
+    if (act)
+#pragma unroll
+      for (int q = 0; q < 18; ++q) {
+        st[(split * IW + t % IW) * 36 + q] = sv[q];
+        st[(split * IW + t % IW) * 36 + 18 + q] = so[q];
+      }
+    __syncthreads();
+    for (int o = t; o < IW * 36; o += 256) {
+      const int i2 = o / 36, q = o % 36;
+      double sR = 0.0;
+      for (int u = 0; u < S; ++u) sR += st[(u * IW + i2) * 36 + q];
+      const int item = g0 + i2, hh = item % nhw, e2 = item / nhw;
+      double* dst = (q < 18) ? accp + L.velo + 2 * 9 * (hh + (int64_t)nhw * e2) + q
+                             : accp + L.ohc + 2 * 9 * (hh + (int64_t)nhw * e2) + (q - 18);
+      *dst += sR;
+    }
+  }
+}
+
+size_t hwsum_lds_bytes(const BandArgs& b) {
+  const size_t stage = (size_t)HW_CH * HW_PS > 256 * 36 ? (size_t)HW_CH * HW_PS : 256 * 36;
+  return sizeof(double) * ((size_t)b.n + (size_t)b.n_ef * b.n + stage);
+}
+
 __global__ void reduce_parts_kernel(int nblocks, int64_t len, const double* __restrict__ part,
                                     double* __restrict__ acc) {
   int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -884,7 +966,10 @@ static void launch_bs(hipStream_t s, const BandArgs& b, double Tsmear, const Acc
 
 size_t bandsum_lds_bytes(const BandArgs& b) {
   AccLayout L = acc_layout(b);
-  if (b.n > 32) return bandsum_wide_lds_bytes(b);
+  if (b.n > 32) {
+    const size_t a = bandsum_wide_lds_bytes(b), h = hwsum_lds_bytes(b);
+    return a > h ? a : h;
+  }
   return b.n <= 16 ? BsLds<16>::bytes(b.n_ef, L.total) : BsLds<32>::bytes(b.n_ef, L.total);
 }
 
@@ -900,6 +985,13 @@ void launch_bandsum(hipStream_t s, const BandArgs& b, int64_t nk, const double* 
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     hipLaunchKernelGGL(bandsum_wide_kernel, dim3(nblocks), dim3(256), lds, s, b, Tsmear, L, nk, E,
                        V, w, part);
+    if (b.jobs & MEP_JOB_AHC) {
+      const size_t lh = hwsum_lds_bytes(b);
+      (void)hipFuncSetAttribute((const void*)hwsum_kernel,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lh);
+      hipLaunchKernelGGL(hwsum_kernel, dim3(nblocks), dim3(256), lh, s, b, Tsmear, L, nk, E, V, w,
+                         part);
+    }
   } else if (b.n <= 16) {
     launch_bs<16>(s, b, Tsmear, L, nk, E, V, A, Om, w, nblocks, part);
   } else {

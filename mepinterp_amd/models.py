@@ -122,3 +122,29 @@ def make_random(seed: int, n: int, L: int = 1, decay: float = 1.0, with_r: bool 
     return dict(R_frac=R, H=np.asfortranarray(H), r=None if r is None else np.asfortranarray(r),
                 wf_centers=None if tau is None else np.asfortranarray(tau),
                 a_latt=a * np.eye(3), name=f"rand_n{n}_L{L}")
+
+
+def make_c5(seed: int = 5, n: int = 96, L: int = 5, decay: float = 2.0, a: float = 6.0):
+    """Large model: n_wf = 96, R in {-5..5}^3 (NR = 1331), hoppings
+    N(0,1) exp(-|R|/2) eV (complex, H(-R) = H(R)^dagger), Hermitian onsite."""
+    m = make_random(seed=seed, n=n, L=L, decay=decay, a=a)
+    m["name"] = f"c5_n{n}_nr{m['R_frac'].shape[1]}"
+    return m
+
+
+C3_N = tuple(range(8, 69, 4))        # 8, 12, ..., 68 (16 models)
+C3_L = (1, 2, 3, 4)                  # NR = 27, 125, 343, 729
+
+
+def make_c3(seed: int = 3):
+    """The 16 models of the batched workload: n_wf = 8, 12, ..., 68 with NR
+    cycling through 27, 125, 343, 729 and meshes from 24^3 to 48^3.
+    Returns a list of (model, mp_grid)."""
+    out = []
+    for i, n in enumerate(C3_N):
+        L = C3_L[i % 4]
+        m = make_random(seed=seed * 1000 + i, n=n, L=L, decay=1.0 + 0.25 * L)
+        m["name"] = f"c3_{i:02d}_n{n}_nr{(2 * L + 1) ** 3}"
+        q = 24 + (24 * i) // 15
+        out.append((m, (q, q, q)))
+    return out
