@@ -1,0 +1,101 @@
+"""Run one of the named workloads on the GPU and print one JSON line.
+
+    python -m mepinterp_amd.configs c1 [--mesh Q]
+    python -m mepinterp_amd.configs c3 [--scale S]
+    python -m mepinterp_amd.configs c5 [--mesh Q]
+
+* c1  n_wf = 8, NR = 27 with a position operator, 32^3 mesh, sigma_xy at
+      E_F = 0, T = 300 K, eta = 1e-3 eV
+* c3  16 models, n_wf = 8 ... 68, NR = 27 ... 729, meshes 24^3 ... 48^3
+      (each divided by --scale), one mep_run_mesh per model; aggregate k-pts/s
+* c5  n_wf = 96, NR = 1331, Q^3 mesh (128 by default), alpha_ij (Kubo and
+      Niu) plus sigma_ab(omega) on 256 frequencies in 0-5 eV, eta = 0.05 eV
+
+Times are wall-clock per mep_run_mesh call (model upload excluded) and the
+per-stage HIP-event split of the same call.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import time
+
+from . import capi, host, models
+
+AU_TO_EV = 27.211385
+STAGES = ("kgen", "phase", "ft_gemm", "eigh", "velo", "bandsum", "reduce")
+
+
+def _run(m, mp, params, warm=True):
+    with capi.Engine(m["R_frac"], m["H"], r=m.get("r"), wf_centers=m.get("wf_centers"),
+                     a_latt=m["a_latt"]) as eng:
+        if warm:
+            eng.run_mesh((4, 4, 4), params)
+        eng.set_profiling(True)
+        eng.reset_times()
+        t0 = time.perf_counter()
+        sums = eng.run_mesh(mp, params)
+        dt = time.perf_counter() - t0
+        split = {s: eng.kernel_time(s)[0] for s in STAGES}
+    return sums, dt, split
+
+
+def run_c1(q: int = 32):
+    m = models.make_c1(seed=1)
+    p = dict(do_ahc=True, do_kubo=False, do_mep=False, n_eF=1, eF_min=0.0, eF_max=0.0, n_hw=1,
+             hw_min=0.0, hw_max=0.0, T_kelvin=300.0, eta=1e-3 / AU_TO_EV, valence_bands=4)
+    params = host.run_params(p)
+    sums, dt, split = _run(m, (q, q, q), params)
+    out = host.finalize(sums, p, (q, q, q), m["a_latt"])
+    return dict(config="c1", workload=f"n_wf=8 NR=27 with r(R), {q}^3 MP mesh, sigma_xy",
+                n_k=q ** 3, seconds=dt, k_pts_s=q ** 3 / dt, kernel_ms=split,
+                sigma_xy=float(out["ahc_tens"][0, 1]))
+
+
+def run_c3(scale: int = 1):
+    jobs = dict(do_ahc=True, do_kubo=True, do_mep=True, n_eF=1, eF_min=0.0, eF_max=0.0, n_hw=1,
+                hw_min=0.0, hw_max=0.0, T_kelvin=300.0, eta=1e-3 / AU_TO_EV)
+    per = []
+    tot_k, tot_t = 0, 0.0
+    for m, mp in models.make_c3():
+        q = max(4, mp[0] // scale)
+        p = dict(jobs, valence_bands=m["H"].shape[0] // 2)
+        _, dt, split = _run(m, (q, q, q), host.run_params(p))
+        per.append(dict(model=m["name"], n_k=q ** 3, seconds=dt, k_pts_s=q ** 3 / dt))
+        tot_k += q ** 3
+        tot_t += dt
+    return dict(config="c3", workload="16 models n_wf=8..68, NR=27..729, meshes 24^3..48^3"
+                + (f" / {scale}" if scale > 1 else "") + ", one mep_run_mesh per model",
+                n_k=tot_k, seconds=tot_t, k_pts_s=tot_k / tot_t, models=per)
+
+
+def run_c5(q: int = 128):
+    m = models.make_c5()
+    p = dict(do_ahc=True, do_kubo=True, do_mep=True, n_eF=1, eF_min=0.0, eF_max=0.0, n_hw=256,
+             hw_min=0.0, hw_max=5.0 / AU_TO_EV, T_kelvin=300.0, eta=0.05 / AU_TO_EV,
+             valence_bands=48)
+    sums, dt, split = _run(m, (q, q, q), host.run_params(p))
+    out = host.finalize(sums, p, (q, q, q), m["a_latt"])
+    return dict(config="c5", workload=f"n_wf=96 NR=1331, {q}^3 MP mesh, alpha_ij (Kubo, Niu), "
+                "sigma_ab(omega) on 256 frequencies in 0-5 eV, eta=0.05 eV, FP64",
+                n_k=q ** 3, seconds=dt, k_pts_s=q ** 3 / dt, kernel_ms=split,
+                alpha_niu_xx=float(out["mep_tens"][0, 0]))
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("config", choices=("c1", "c3", "c5"))
+    ap.add_argument("--mesh", type=int, default=None, help="Q for a Q^3 mesh (c1, c5)")
+    ap.add_argument("--scale", type=int, default=1, help="divide the c3 meshes by this")
+    a = ap.parse_args(argv)
+    if a.config == "c1":
+        r = run_c1(a.mesh or 32)
+    elif a.config == "c3":
+        r = run_c3(a.scale)
+    else:
+        r = run_c5(a.mesh or 128)
+    print(json.dumps(r), flush=True)
+
+
+if __name__ == "__main__":
+    main()
