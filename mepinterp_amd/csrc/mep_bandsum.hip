@@ -804,26 +804,36 @@ size_t bandsum_wide_lds_bytes(const BandArgs& b) {
 
 // ===========================================================================
 // velo_ahc_tens / kubo_ohc_tens for 32 < n <= 128: per k, the pairs (x, y),
-// x != y, are staged in LDS in chunks of 256 (dE, x, y and the 27 products
-// Im(V_i(x,y) V_j(y,x)), V_i(y,x) V_j(x,y)); thread (item, split) walks every
-// split-th staged pair for its item = (hw, E_F), accumulating in registers
-// over all k; the splits are combined once at the end.  Items beyond 256 are
-// taken in further passes over the k-points.
+// x != y, whose occupations differ at some E_F are compacted, in pair order,
+// into LDS chunks of up to 256 records (dE, f_x - f_y per E_F, the 9
+// Im(V_i(x,y) V_j(y,x)) and the 9 complex V_i(y,x) V_j(x,y)); thread
+// (item, split) walks every split-th record for its item = (hw, E_F),
+// accumulating in registers over all k; the splits are combined once at the
+// end.  Items beyond 256 are taken in further passes over the k-points.
 // ===========================================================================
-constexpr int HW_CH = 256;   // pairs per staged chunk
-constexpr int HW_PS = 30;    // doubles per staged pair: dE, x, y, 9 vim, 18 q2o
+constexpr int HW_CH = 256;  // pair candidates per chunk (one per thread)
 
-__global__ __launch_bounds__(256) void hwsum_kernel(BandArgs b, double Tsmear, AccLayout L,
-                                                    int64_t nk, const double* __restrict__ E,
-                                                    const zd* __restrict__ V,
-                                                    const double* __restrict__ w,
-                                                    double* __restrict__ part) {
+__device__ inline double rcp_nr(double x) {
+  // reciprocal: hardware estimate refined by two Newton steps
+  double r = __builtin_amdgcn_rcp(x);
+  r = fma(r, fma(-x, r, 1.0), r);
+  r = fma(r, fma(-x, r, 1.0), r);
+  return r;
+}
+
+__global__ __launch_bounds__(256, 3) void hwsum_kernel(BandArgs b, double Tsmear, AccLayout L,
+                                                       int64_t nk, const double* __restrict__ E,
+                                                       const zd* __restrict__ V,
+                                                       const double* __restrict__ w,
+                                                       double* __restrict__ part) {
   extern __shared__ double smem[];
   const int n = b.n, nef = b.n_ef, nhw = b.n_hw;
+  const int PS = 28 + nef;  // doubles per record: dE, dfv[nef], 9 vim, 18 q2o
   double* Es = smem;
   double* fd = Es + n;
-  double* st = fd + (int64_t)nef * n;  // [HW_CH][HW_PS]; reused for the split reduction
-  const int t = threadIdx.x;
+  int* cnt_w = reinterpret_cast<int*>(fd + (int64_t)nef * n);  // [4] per-wave counts
+  double* st = fd + (int64_t)nef * n + 4;  // [HW_CH][PS]; reused for the split reduction
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   const int n2 = n * n;
   const bool finiteT = b.T_kelvin > 1e-2;
   const int items = nhw * nef;
@@ -855,93 +865,96 @@ __global__ __launch_bounds__(256) void hwsum_kernel(BandArgs b, double Tsmear, A
         }
       }
       for (int c0 = 0; c0 < n2; c0 += HW_CH) {
+        __syncthreads();  // fd ready; the previous chunk consumed
+        const int pidx = c0 + t;
+        const int x = pidx % n, y = pidx / n;
+        bool keep = pidx < n2 && x != y;
+        if (keep) {
+          bool any = false;
+          for (int e_ = 0; e_ < nef; ++e_) any |= fd[e_ + nef * x] != fd[e_ + nef * y];
+          keep = any;  // pairs of equal occupation contribute exactly zero
+        }
+        const uint64_t ball = __ballot(keep);
+        const int below = __popcll(ball & ((1ull << lane) - 1ull));
+        if (lane == 0) cnt_w[wv] = __popcll(ball);
         __syncthreads();
-        {
-          const int pidx = c0 + t;
-          double* sp = st + t * HW_PS;
-          const int x = pidx % n, y = pidx / n;
-          if (pidx < n2 && x != y) {
-            zd vxy[3], vyx[3];
+        int base = 0;
+        for (int u = 0; u < wv; ++u) base += cnt_w[u];
+        const int cnt = cnt_w[0] + cnt_w[1] + cnt_w[2] + cnt_w[3];
+        if (keep) {
+          double* sp = st + (int64_t)(base + below) * PS;
+          zd vxy[3], vyx[3];
 #pragma unroll
-            for (int a = 0; a < 3; ++a) {
-              vxy[a] = Vk[a * (int64_t)n2 + x + (int64_t)n * y];
-              vyx[a] = Vk[a * (int64_t)n2 + y + (int64_t)n * x];
-            }
-            sp[0] = Es[x] - Es[y];
-            sp[1] = (double)x;
-            sp[2] = (double)y;
-#pragma unroll
-            for (int j = 0; j < 3; ++j)
-#pragma unroll
-              for (int i = 0; i < 3; ++i) {
-                const int q = i + 3 * j;
-                sp[3 + q] = zmul(vxy[i], vyx[j]).im;
-                const zd q2 = zmul(vyx[i], vxy[j]);
-                sp[12 + 2 * q] = q2.re;
-                sp[12 + 2 * q + 1] = q2.im;
-              }
-          } else {
-            sp[1] = -1.0;  // no pair
+          for (int a = 0; a < 3; ++a) {
+            vxy[a] = Vk[a * (int64_t)n2 + x + (int64_t)n * y];
+            vyx[a] = Vk[a * (int64_t)n2 + y + (int64_t)n * x];
           }
+          sp[0] = Es[x] - Es[y];
+          for (int e_ = 0; e_ < nef; ++e_) sp[1 + e_] = wk * (fd[e_ + nef * x] - fd[e_ + nef * y]);
+          double* pq = sp + 1 + nef;
+#pragma unroll
+          for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+              const int q = i + 3 * j;
+              pq[q] = zmul(vxy[i], vyx[j]).im;
+              const zd q2 = zmul(vyx[i], vxy[j]);
+              pq[9 + 2 * q] = q2.re;
+              pq[9 + 2 * q + 1] = q2.im;
+            }
         }
         __syncthreads();
-        if (act) {
-          const int cnt = (n2 - c0 < HW_CH) ? n2 - c0 : HW_CH;
+        if (act)
           for (int j = split; j < cnt; j += S) {
-            const double* sp = st + j * HW_PS;
-            const double xf = sp[1];
-            if (xf < 0.0) continue;
-            const int x = (int)xf, y = (int)sp[2];
-            // pairs of equal occupation contribute exactly zero
-            const double dfv = fd[ef + nef * x] - fd[ef + nef * y];
-            if (dfv == 0.0) continue;
+            const double* sp = st + (int64_t)j * PS;
             const double dE = sp[0];
+            const double f = sp[1 + ef];
+            const double* pq = sp + 1 + nef;
             const zd denv = zd{dE * dE - wh2.re, -wh2.im};
             const zd deno = zmul(zd{dE, 0.0}, zd{dE - wh.re, -wh.im});
-            const double f = wk * dfv;
             const double av = zabs2(denv), ao = zabs2(deno);
-            const double fi = f / (av * ao);  // one division for both denominators
+            // one reciprocal for both denominators; f = 0 records (another
+            // E_F differs) contribute exactly zero
+            const double fi = f * rcp_nr(av * ao);
             const double iv = fi * ao, io = fi * av;
             const zd rv = zd{denv.re * iv, -denv.im * iv};
             const zd ro = zd{deno.re * io, -deno.im * io};
 #pragma unroll
             for (int q = 0; q < 9; ++q) {
-              const double vim = sp[3 + q];
+              const double vim = pq[q];
               sv[2 * q] += vim * rv.re;
               sv[2 * q + 1] += vim * rv.im;
               // i * (q2o * ro)
-              const double qr = sp[12 + 2 * q], qi = sp[12 + 2 * q + 1];
+              const double qr = pq[9 + 2 * q], qi = pq[9 + 2 * q + 1];
               so[2 * q] -= qr * ro.im + qi * ro.re;
               so[2 * q + 1] += qr * ro.re - qi * ro.im;
             }
           }
-        }
       }
     }
-    // combine the splits of each item: red[split][item][36] over the staging area
-    __syncthreads();
-    if (act)
+    // combine the splits of each item, velo then ohc: red[split][item][18]
+    // over the staging area
+    for (int half = 0; half < 2; ++half) {
+      __syncthreads();
+      if (act)
 #pragma unroll
-      for (int q = 0; q < 18; ++q) {
-        st[(split * IW + t % IW) * 36 + q] = sv[q];
-        st[(split * IW + t % IW) * 36 + 18 + q] = so[q];
+        for (int q = 0; q < 18; ++q) st[(split * IW + t % IW) * 18 + q] = half ? so[q] : sv[q];
+      __syncthreads();
+      for (int o = t; o < IW * 18; o += 256) {
+        const int i2 = o / 18, q = o % 18;
+        double sR = 0.0;
+        for (int u = 0; u < S; ++u) sR += st[(u * IW + i2) * 18 + q];
+        const int item = g0 + i2, hh = item % nhw, e2 = item / nhw;
+        accp[(half ? L.ohc : L.velo) + 2 * 9 * (hh + (int64_t)nhw * e2) + q] += sR;
       }
-    __syncthreads();
-    for (int o = t; o < IW * 36; o += 256) {
-      const int i2 = o / 36, q = o % 36;
-      double sR = 0.0;
-      for (int u = 0; u < S; ++u) sR += st[(u * IW + i2) * 36 + q];
-      const int item = g0 + i2, hh = item % nhw, e2 = item / nhw;
-      double* dst = (q < 18) ? accp + L.velo + 2 * 9 * (hh + (int64_t)nhw * e2) + q
-                             : accp + L.ohc + 2 * 9 * (hh + (int64_t)nhw * e2) + (q - 18);
-      *dst += sR;
     }
   }
 }
 
 size_t hwsum_lds_bytes(const BandArgs& b) {
-  const size_t stage = (size_t)HW_CH * HW_PS > 256 * 36 ? (size_t)HW_CH * HW_PS : 256 * 36;
-  return sizeof(double) * ((size_t)b.n + (size_t)b.n_ef * b.n + stage);
+  const size_t ps = 28 + (size_t)b.n_ef;
+  const size_t stage = (size_t)HW_CH * ps > 256 * 18 ? (size_t)HW_CH * ps : 256 * 18;
+  return sizeof(double) * ((size_t)b.n + (size_t)b.n_ef * b.n + 4 + stage);
 }
 
 __global__ void reduce_parts_kernel(int nblocks, int64_t len, const double* __restrict__ part,

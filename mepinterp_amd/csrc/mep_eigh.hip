@@ -585,22 +585,19 @@ size_t eigh_house_wide_lds(int n) {
   return sizeof(zd) * ((size_t)n * (n + 1) / 2 + 8 * (size_t)n) + sizeof(double) * ((size_t)n + 4);
 }
 
-// Rotation replay and back-transformation for 32 < n <= 128: thread r <-> row
-// r of Z (LDS, column-major with odd pitch), each wave staging the rotation
-// stream through its own LDS slots; then thread j <-> column j of W = P Z,
-// kept in the k-point's (consumed) rotation record in HBM, row-major so the
-// reflector updates are coalesced.  The reflectors are read from U's slot
-// (broadcast loads) and U is overwritten only after every thread is done.
-__global__ __launch_bounds__(128) void eigh_vec_wide_kernel(int n, int64_t nk,
-                                                            const double* __restrict__ side,
-                                                            double* __restrict__ rot,
-                                                            const uint8_t* __restrict__ rcol,
-                                                            zd* __restrict__ U) {
+// Rotation replay for 32 < n <= 128: thread r <-> row r of Z (LDS,
+// column-major with odd pitch), each wave staging the rotation stream
+// through its own LDS slots; then W = P Z is written row-major into the
+// k-point's (consumed) rotation record.
+__global__ __launch_bounds__(128) void eigh_replay_wide_kernel(int n, int64_t nk,
+                                                               const double* __restrict__ side,
+                                                               double* __restrict__ rot,
+                                                               const uint8_t* __restrict__ rcol) {
   extern __shared__ double vw_smem[];
   const int LDZ = n | 1;
   double* Zt = vw_smem;                       // [n][LDZ]
   double* csb_all = Zt + (int64_t)n * LDZ;    // [waves][2*64]
-  int* ixb_all = reinterpret_cast<int*>(csb_all + 2 * 64 * (blockDim.x / 64));
+  int* ixb_all = reinterpret_cast<int*>(csb_all + 2 * 64 * 2);
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   double* csb = csb_all + 2 * 64 * wv;
   int* ixb = ixb_all + 64 * wv;
@@ -652,50 +649,97 @@ __global__ __launch_bounds__(128) void eigh_vec_wide_kernel(int n, int64_t nk,
     __syncthreads();
     // W = P Z, row-major in the consumed rotation record
     zd* W = reinterpret_cast<zd*>(rot + k * 2 * cap);
-    const zd* Uk = U + k * (int64_t)n * n;
     const int j = t;
     if (j < n)
       for (int i = 0; i < n; ++i)
         W[(int64_t)i * n + j] = zscale(zd{sd[3 * n + 2 * i], sd[3 * n + 2 * i + 1]}, Zt[j * LDZ + i]);
-    // reflectors in reverse order; every thread owns its column
-    if (j < n)
-      for (int kc = n - 3; kc >= 0; --kc) {
-        const double tk = sd[2 * n + kc];
-        if (tk == 0.0) continue;
-        const zd* vcol = Uk + (int64_t)n * kc;
-        zd sacc = zd{0.0, 0.0};
-        for (int i = kc + 1; i < n; ++i) sacc = zadd(sacc, zcmul(vcol[i], W[(int64_t)i * n + j]));
-        sacc = zscale(sacc, tk);
-        for (int i = kc + 1; i < n; ++i)
-          W[(int64_t)i * n + j] = zsub(W[(int64_t)i * n + j], zmul(vcol[i], sacc));
-      }
-    // gauge: largest-modulus component (first on ties) real > 0
-    zd phs = zd{1.0, 0.0};
-    if (j < n) {
-      double best = -1.0;
-      zd u = zd{1.0, 0.0};
-      for (int i = 0; i < n; ++i) {
-        const zd wi = W[(int64_t)i * n + j];
-        const double m2 = zabs2(wi);
-        if (m2 > best) {
-          best = m2;
-          u = wi;
-        }
-      }
-      const double au = sqrt(zabs2(u));
-      if (au > 0.0) phs = zd{u.re / au, -u.im / au};
-    }
-    __syncthreads();  // every reflector read before U is overwritten
-    if (j < n) {
-      const int rnk = (int)sd[n + j];
-      zd* Uo = U + k * (int64_t)n * n + (int64_t)n * rnk;
-      for (int i = 0; i < n; ++i) Uo[i] = zmul(W[(int64_t)i * n + j], phs);
-    }
   }
 }
 
-size_t eigh_vec_wide_lds(int n) {
+size_t eigh_replay_wide_lds(int n) {
   return sizeof(double) * ((size_t)n * (n | 1) + 2 * 64 * 2) + sizeof(int) * 64 * 2;
+}
+
+// columns per LDS block of the back-transformation (about 76 KB of LDS)
+static inline int back_cb(int n) {
+  int cb = (76 * 1024) / (16 * (n + 1));
+  return cb < n ? cb : n;
+}
+
+// Back-transformation U = Q W for 32 < n <= 128: a block of CB columns of W
+// in LDS, two threads per column (even / odd rows, combined by a lane
+// shuffle), the reflectors read from U's slot (the same address across a
+// wave).  Each block goes back to the rotation record with the column's
+// gauge phase applied; U is written once every block is done.
+__global__ __launch_bounds__(256) void eigh_back_wide_kernel(int n, int64_t nk, int CB,
+                                                             const double* __restrict__ side,
+                                                             double* __restrict__ rot,
+                                                             zd* __restrict__ U) {
+  extern __shared__ double bw_smem[];
+  const int LDW = CB + 1;
+  zd* Ws = reinterpret_cast<zd*>(bw_smem);  // [n][LDW]
+  const int t = threadIdx.x, jj = t >> 1, hf = t & 1;
+  const int64_t cap = rot_cap(n);
+  for (int64_t k = blockIdx.x; k < nk; k += gridDim.x) {
+    const double* sd = side + k * side_len(n);
+    zd* W = reinterpret_cast<zd*>(rot + k * 2 * cap);
+    const zd* Uk = U + k * (int64_t)n * n;
+    for (int c0 = 0; c0 < n; c0 += CB) {
+      const int cbn = (n - c0 < CB) ? n - c0 : CB;
+      __syncthreads();
+      for (int idx = t; idx < n * cbn; idx += blockDim.x) {
+        const int i = idx / cbn, c = idx % cbn;
+        Ws[i * LDW + c] = W[(int64_t)i * n + c0 + c];
+      }
+      __syncthreads();
+      const bool act = jj < cbn;
+      if (act)
+        for (int kc = n - 3; kc >= 0; --kc) {
+          const double tk = sd[2 * n + kc];
+          if (tk == 0.0) continue;
+          const zd* vcol = Uk + (int64_t)n * kc;
+          const int i0 = kc + 1 + hf;
+          zd sacc = zd{0.0, 0.0};
+          for (int i = i0; i < n; i += 2) sacc = zadd(sacc, zcmul(vcol[i], Ws[i * LDW + jj]));
+          sacc.re += __shfl_xor(sacc.re, 1, 64);
+          sacc.im += __shfl_xor(sacc.im, 1, 64);
+          sacc = zscale(sacc, tk);
+          for (int i = i0; i < n; i += 2) Ws[i * LDW + jj] = zsub(Ws[i * LDW + jj], zmul(vcol[i], sacc));
+        }
+      // gauge: largest-modulus component (first on ties) real > 0
+      double best = -1.0;
+      int bi = n;
+      if (act)
+        for (int i = hf; i < n; i += 2) {
+          const double m2 = zabs2(Ws[i * LDW + jj]);
+          if (m2 > best) {
+            best = m2;
+            bi = i;
+          }
+        }
+      const double ob = __shfl_xor(best, 1, 64);
+      const int oi = __shfl_xor(bi, 1, 64);
+      if (ob > best || (ob == best && oi < bi)) {
+        best = ob;
+        bi = oi;
+      }
+      zd phs = zd{1.0, 0.0};
+      if (act && bi < n) {
+        const zd u = Ws[bi * LDW + jj];
+        const double au = sqrt(zabs2(u));
+        if (au > 0.0) phs = zd{u.re / au, -u.im / au};
+      }
+      if (act)
+        for (int i = hf; i < n; i += 2) W[(int64_t)i * n + c0 + jj] = zmul(Ws[i * LDW + jj], phs);
+    }
+    __syncthreads();  // every reflector read before U is overwritten
+    for (int idx = t; idx < n * n; idx += blockDim.x) {
+      const int i = idx % n, j = idx / n;
+      const int rnk = (int)sd[n + j];
+      U[k * (int64_t)n * n + (int64_t)n * rnk + i] = W[(int64_t)i * n + j];
+    }
+    __syncthreads();
+  }
 }
 
 void launch_eigh_wide(hipStream_t s, int n, int64_t nk, const zd* Hk, int64_t strideHk, int packed,
@@ -703,12 +747,16 @@ void launch_eigh_wide(hipStream_t s, int n, int64_t nk, const zd* Hk, int64_t st
   double* side = scratch;
   double* rot = side + nk * side_len(n);
   uint8_t* rcol = reinterpret_cast<uint8_t*>(rot + nk * 2 * rot_cap(n));
-  const size_t lh = eigh_house_wide_lds(n), lv = eigh_vec_wide_lds(n);
+  const int CB = back_cb(n);
+  const size_t lh = eigh_house_wide_lds(n), lr = eigh_replay_wide_lds(n);
+  const size_t lb = sizeof(zd) * (size_t)n * (CB + 1);
   const size_t lq = sizeof(double) * 2 * (size_t)n * 32;
   (void)hipFuncSetAttribute((const void*)eigh_house_wide_kernel,
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lh);
-  (void)hipFuncSetAttribute((const void*)eigh_vec_wide_kernel,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lv);
+  (void)hipFuncSetAttribute((const void*)eigh_replay_wide_kernel,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lr);
+  (void)hipFuncSetAttribute((const void*)eigh_back_wide_kernel,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lb);
   (void)hipFuncSetAttribute((const void*)eigh_ql_wide_kernel<double>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lq);
   unsigned grid = (unsigned)(nk < 4096 ? nk : 4096);
@@ -718,8 +766,11 @@ void launch_eigh_wide(hipStream_t s, int n, int64_t nk, const zd* Hk, int64_t st
   unsigned qgrid = (unsigned)(qb < 65536 ? qb : 65536);
   hipLaunchKernelGGL(eigh_ql_wide_kernel<double>, dim3(qgrid), dim3(32), lq, s, n, nk, side, rot,
                      rcol, E, info);
-  const int vthreads = n <= 64 ? 64 : 128;
-  hipLaunchKernelGGL(eigh_vec_wide_kernel, dim3(grid), dim3(vthreads), lv, s, n, nk, side, rot, rcol,
+  const int rthreads = n <= 64 ? 64 : 128;
+  hipLaunchKernelGGL(eigh_replay_wide_kernel, dim3(grid), dim3(rthreads), lr, s, n, nk, side, rot,
+                     rcol);
+  const int bthreads = ((2 * CB + 63) / 64) * 64;
+  hipLaunchKernelGGL(eigh_back_wide_kernel, dim3(grid), dim3(bthreads), lb, s, n, nk, CB, side, rot,
                      U);
 }
 
