@@ -140,6 +140,36 @@ int mep_run_kpts(mep_ctx *ctx, int64_t nk, const double *kpts, const double *wei
 /* Kernel timing with HIP events on the context's stream (off by default).
    name: "phase", "ft_gemm", "eigh", "refine", "velo", "bandsum", "reduce",
    "kgen".                                                                   */
+/* Per-k Berry curvature at one Fermi level: omega(9,nk), element i + 3j =
+   sum_n f(n) sum_{m != n} 2 Im(V_i,nm V_j,mn) / (E_m - E_n)^2, i.e. the
+   k-point's summand of kubo_ahc_tens (kubo.f90:37-78) before the k-sum, with
+   the occupations of d1_fd_stat (statistics.f90:65-98).  It feeds the
+   learned refinement below; the reference has no per-k output of it.       */
+int mep_curv_batch(mep_ctx *ctx, int64_t nk, const double *kpts, double eF, double T_kelvin,
+                   double *omega);
+
+/* Learned adaptive refinement.  No counterpart in the reference, which
+   integrates on uniform meshes only: a small MLP (sin/cos Fourier features
+   of k, three tanh layers of `hidden` units, FP32 weights) learns
+   log|Omega(k)| from a coarse mesh and ranks the cells to refine.  Training
+   (mean squared error, Adam) and inference run in HIP kernels.
+   Parameters: W1[6 n_freq padded to 16][hidden] b1 W2[hidden][hidden] b2
+   W3[hidden][hidden] b3 W4[hidden][16] b4[16] (column 0 is the output).    */
+typedef struct mep_mlp mep_mlp;
+int mep_mlp_create(int32_t n_freq, int32_t hidden, uint64_t seed, mep_mlp **mlp);
+int mep_mlp_destroy(mep_mlp *mlp);
+int mep_mlp_num_params(const mep_mlp *mlp, int64_t *n_params);
+int mep_mlp_get_params(mep_mlp *mlp, float *params);
+int mep_mlp_set_params(mep_mlp *mlp, const float *params);
+/* epochs over (kpts(3,n), target(n)) in shuffled batches (seeded); the mean
+   loss of the last epoch in *final_loss */
+int mep_mlp_train(mep_mlp *mlp, int64_t n, const double *kpts, const float *target,
+                  int32_t epochs, int32_t batch, float lr, uint64_t seed, float *final_loss);
+int mep_mlp_predict(mep_mlp *mlp, int64_t n, const double *kpts, float *out);
+/* loss and gradient of one batch of all n points, parameters unchanged */
+int mep_mlp_grad(mep_mlp *mlp, int64_t n, const double *kpts, const float *target, float *grad,
+                 float *loss);
+
 int mep_set_profiling(mep_ctx *ctx, int32_t on);
 int mep_get_kernel_time(mep_ctx *ctx, const char *name, double *total_ms, int64_t *launches);
 int mep_reset_kernel_times(mep_ctx *ctx);

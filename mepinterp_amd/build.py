@@ -13,6 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmepinterp_amd.so")
 SOURCES = ["mep_ft.hip", "mep_eigh.hip", "mep_refine.hip", "mep_velo.hip", "mep_bandsum.hip",
+           "mep_mlp.hip",
            "mep_extsum.hip", "mep_capi.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

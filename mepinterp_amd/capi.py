@@ -30,7 +30,9 @@ MEP_PREC_MIXED = 1
 # every symbol include/mepinterp.h declares
 EXPORTS = ("mep_ctx_create", "mep_ctx_destroy", "mep_last_error", "mep_interp_batch",
            "mep_bandsum_batch", "mep_run_mesh", "mep_run_kpts", "mep_set_profiling",
-           "mep_get_kernel_time", "mep_reset_kernel_times", "mep_set_kblock")
+           "mep_get_kernel_time", "mep_reset_kernel_times", "mep_set_kblock", "mep_curv_batch",
+           "mep_mlp_create", "mep_mlp_destroy", "mep_mlp_num_params", "mep_mlp_get_params",
+           "mep_mlp_set_params", "mep_mlp_train", "mep_mlp_predict", "mep_mlp_grad")
 
 c_dp = C.POINTER(C.c_double)
 
@@ -84,6 +86,17 @@ def load(path: str = LIB_PATH):
     lib.mep_get_kernel_time.argtypes = [C.c_void_p, C.c_char_p, c_dp, C.POINTER(C.c_int64)]
     lib.mep_reset_kernel_times.argtypes = [C.c_void_p]
     lib.mep_set_kblock.argtypes = [C.c_void_p, C.c_int64]
+    lib.mep_curv_batch.argtypes = [C.c_void_p, C.c_int64, c_dp, C.c_double, C.c_double, c_dp]
+    c_fp = C.POINTER(C.c_float)
+    lib.mep_mlp_create.argtypes = [C.c_int32, C.c_int32, C.c_uint64, C.POINTER(C.c_void_p)]
+    lib.mep_mlp_destroy.argtypes = [C.c_void_p]
+    lib.mep_mlp_num_params.argtypes = [C.c_void_p, C.POINTER(C.c_int64)]
+    lib.mep_mlp_get_params.argtypes = [C.c_void_p, c_fp]
+    lib.mep_mlp_set_params.argtypes = [C.c_void_p, c_fp]
+    lib.mep_mlp_train.argtypes = [C.c_void_p, C.c_int64, c_dp, c_fp, C.c_int32, C.c_int32,
+                                  C.c_float, C.c_uint64, c_fp]
+    lib.mep_mlp_predict.argtypes = [C.c_void_p, C.c_int64, c_dp, c_fp]
+    lib.mep_mlp_grad.argtypes = [C.c_void_p, C.c_int64, c_dp, c_fp, c_fp, c_fp]
     _lib = lib
     return lib
 
@@ -92,6 +105,10 @@ def _p(a):
     if a is None:
         return None
     return a.ctypes.data_as(c_dp)
+
+
+def _pf(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_float))
 
 
 def _check(rc, what):
@@ -236,3 +253,76 @@ class Engine:
 
     def set_kblock(self, kb):
         _check(load().mep_set_kblock(self.ctx, int(kb)), "mep_set_kblock")
+
+    def curv(self, kpts, eF=0.0, T_kelvin=0.0):
+        """Per-k Berry curvature summed over occupied bands, (3, 3, nk)."""
+        k = np.asfortranarray(kpts, dtype=np.float64).reshape(3, -1, order="F")
+        out = np.zeros((3, 3, k.shape[1]), order="F")
+        _check(load().mep_curv_batch(self.ctx, k.shape[1], _p(k), float(eF), float(T_kelvin),
+                                     _p(out)), "mep_curv_batch")
+        return out
+
+
+class MLP:
+    """The learned-refinement network on the GPU (one mep_mlp)."""
+
+    def __init__(self, n_freq=4, hidden=128, seed=0):
+        self.h = C.c_void_p()
+        _check(load().mep_mlp_create(int(n_freq), int(hidden), int(seed), C.byref(self.h)),
+               "mep_mlp_create")
+        n = C.c_int64()
+        _check(load().mep_mlp_num_params(self.h, C.byref(n)), "mep_mlp_num_params")
+        self.n_params = n.value
+        self.n_freq, self.hidden = int(n_freq), int(hidden)
+
+    def close(self):
+        if self.h:
+            load().mep_mlp_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def params(self):
+        p = np.zeros(self.n_params, np.float32)
+        _check(load().mep_mlp_get_params(self.h, _pf(p)), "mep_mlp_get_params")
+        return p
+
+    @params.setter
+    def params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.n_params
+        _check(load().mep_mlp_set_params(self.h, _pf(p)), "mep_mlp_set_params")
+
+    def train(self, kpts, target, epochs=20, batch=4096, lr=1e-3, seed=0):
+        k = np.asfortranarray(kpts, dtype=np.float64).reshape(3, -1, order="F")
+        t = np.ascontiguousarray(target, dtype=np.float32)
+        loss = np.zeros(1, np.float32)
+        _check(load().mep_mlp_train(self.h, k.shape[1], _p(k), _pf(t), int(epochs), int(batch),
+                                    float(lr), int(seed), _pf(loss)), "mep_mlp_train")
+        return float(loss[0])
+
+    def predict(self, kpts):
+        k = np.asfortranarray(kpts, dtype=np.float64).reshape(3, -1, order="F")
+        out = np.zeros(k.shape[1], np.float32)
+        _check(load().mep_mlp_predict(self.h, k.shape[1], _p(k), _pf(out)), "mep_mlp_predict")
+        return out
+
+    def grad(self, kpts, target):
+        k = np.asfortranarray(kpts, dtype=np.float64).reshape(3, -1, order="F")
+        t = np.ascontiguousarray(target, dtype=np.float32)
+        g = np.zeros(self.n_params, np.float32)
+        loss = np.zeros(1, np.float32)
+        _check(load().mep_mlp_grad(self.h, k.shape[1], _p(k), _pf(t), _pf(g), _pf(loss)),
+               "mep_mlp_grad")
+        return g, float(loss[0])

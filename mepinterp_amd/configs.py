@@ -2,12 +2,17 @@
 
     python -m mepinterp_amd.configs c1 [--mesh Q]
     python -m mepinterp_amd.configs c3 [--scale S]
+    python -m mepinterp_amd.configs c4 [--mesh Q]
     python -m mepinterp_amd.configs c5 [--mesh Q]
 
 * c1  n_wf = 8, NR = 27 with a position operator, 32^3 mesh, sigma_xy at
       E_F = 0, T = 300 K, eta = 1e-3 eV
 * c3  16 models, n_wf = 8 ... 68, NR = 27 ... 729, meshes 24^3 ... 48^3
       (each divided by --scale), one mep_run_mesh per model; aggregate k-pts/s
+* c4  n_wf = 24, NR = 343 with eight planted nodes (gap 1e-3 eV): sigma_xy
+      on a Q^3 base mesh (48 by default) with the top 5 % of cells refined by
+      a 4^3 sub-mesh, cells chosen by the trained network, by the coarse
+      |Omega| and at random, against the uniform base and 256^3 meshes
 * c5  n_wf = 96, NR = 1331, Q^3 mesh (128 by default), alpha_ij (Kubo and
       Niu) plus sigma_ab(omega) on 256 frequencies in 0-5 eV, eta = 0.05 eV
 
@@ -69,6 +74,31 @@ def run_c3(scale: int = 1):
                 n_k=tot_k, seconds=tot_t, k_pts_s=tot_k / tot_t, models=per)
 
 
+def run_c4(q: int = 48):
+    from . import refine
+    m = models.make_c4()
+    mp = (q, q, q)
+    res = {}
+    with capi.Engine(m["R_frac"], m["H"], a_latt=m["a_latt"]) as eng:
+        t0 = time.perf_counter()
+        ref = refine.sigma_uniform(eng, (256, 256, 256), m["a_latt"])
+        t_ref = time.perf_counter() - t0
+        base = refine.sigma_uniform(eng, mp, m["a_latt"])
+        for sel in ("mlp", "coarse", "random"):
+            t0 = time.perf_counter()
+            s, info = refine.refine_sigma(eng, mp, m["a_latt"], select=sel, epochs=40, batch=2048,
+                                          seed=3)
+            res[sel] = dict(sigma_xy=s, abs_err=abs(s - ref), n_k=info["n_k"],
+                            seconds=time.perf_counter() - t0)
+            if sel == "mlp":
+                res[sel].update(train_loss=info["loss"], target_var=info["target_var"])
+    return dict(config="c4", workload=f"n_wf=24 NR=343, 8 planted nodes (gap 1e-3 eV), {q}^3 base "
+                "mesh, top 5 % of cells refined 4^3, sigma_xy at E_F=0, T=0",
+                sigma_xy_256=ref, n_k_256=256 ** 3, seconds_256=t_ref,
+                uniform_base=dict(sigma_xy=base, abs_err=abs(base - ref), n_k=q ** 3),
+                refined=res, k_ratio=256 ** 3 / res["mlp"]["n_k"])
+
+
 def run_c5(q: int = 128):
     m = models.make_c5()
     p = dict(do_ahc=True, do_kubo=True, do_mep=True, n_eF=1, eF_min=0.0, eF_max=0.0, n_hw=256,
@@ -84,14 +114,16 @@ def run_c5(q: int = 128):
 
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
-    ap.add_argument("config", choices=("c1", "c3", "c5"))
-    ap.add_argument("--mesh", type=int, default=None, help="Q for a Q^3 mesh (c1, c5)")
+    ap.add_argument("config", choices=("c1", "c3", "c4", "c5"))
+    ap.add_argument("--mesh", type=int, default=None, help="Q for a Q^3 mesh (c1, c4, c5)")
     ap.add_argument("--scale", type=int, default=1, help="divide the c3 meshes by this")
     a = ap.parse_args(argv)
     if a.config == "c1":
         r = run_c1(a.mesh or 32)
     elif a.config == "c3":
         r = run_c3(a.scale)
+    elif a.config == "c4":
+        r = run_c4(a.mesh or 48)
     else:
         r = run_c5(a.mesh or 128)
     print(json.dumps(r), flush=True)

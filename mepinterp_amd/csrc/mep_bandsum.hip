@@ -957,6 +957,59 @@ size_t hwsum_lds_bytes(const BandArgs& b) {
   return sizeof(double) * ((size_t)b.n + (size_t)b.n_ef * b.n + 4 + stage);
 }
 
+// ===========================================================================
+// Per-k Berry curvature at one Fermi level: out[k][i + 3j] = the k-point's
+// term of kubo_ahc_tens (kubo.f90:37-78),
+//   sum_n0 f(n0) sum_{n != n0} 2 Im(V_i(n0,n) V_j(n,n0)) / (E_n - E_n0)^2,
+// one k-point per wavefront, the pairs spread over the lanes (any n).
+// ===========================================================================
+__global__ __launch_bounds__(64) void curv_kernel(int n, int64_t nk, const double* __restrict__ E,
+                                                  const zd* __restrict__ V, double ef,
+                                                  double Tsmear, int finiteT,
+                                                  double* __restrict__ out) {
+  const int lane = threadIdx.x;
+  const int n2 = n * n;
+  for (int64_t k = blockIdx.x; k < nk; k += gridDim.x) {
+    const double* Ek = E + k * n;
+    const zd* Vk = V + k * 3 * (int64_t)n2;
+    double om[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) om[q] = 0.0;
+    for (int idx = lane; idx < n2; idx += 64) {
+      const int n0 = idx % n, nn = idx / n;
+      if (n0 == nn) continue;
+      const double e0 = Ek[n0];
+      const double f = finiteT ? 1.0 / (1.0 + exp((e0 - ef) / Tsmear)) : ((e0 < ef) ? 1.0 : 0.0);
+      if (f == 0.0) continue;
+      const double r = 1.0 / (e0 - Ek[nn]);
+      const double w2 = 2.0 * f * r * r;
+      zd x[3], y[3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        x[a] = Vk[a * (int64_t)n2 + n0 + (int64_t)n * nn];
+        y[a] = Vk[a * (int64_t)n2 + nn + (int64_t)n * n0];
+      }
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) om[i + 3 * j] += w2 * zmul(x[i], y[j]).im;
+    }
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+      const double v = wave_sum(om[q]);
+      if (lane == 0) out[k * 9 + q] = v;
+    }
+  }
+}
+
+void launch_curv(hipStream_t s, int n, int64_t nk, const double* E, const zd* V, double ef,
+                 double T_kelvin, double* out) {
+  const double kB = 8.6173303 * 1e-5 / 27.211385;  // constants.f90:22
+  unsigned grid = (unsigned)(nk < 65536 ? nk : 65536);
+  hipLaunchKernelGGL(curv_kernel, dim3(grid), dim3(64), 0, s, n, nk, E, V, ef, kB * T_kelvin,
+                     T_kelvin > 1e-2 ? 1 : 0, out);
+}
+
 __global__ void reduce_parts_kernel(int nblocks, int64_t len, const double* __restrict__ part,
                                     double* __restrict__ acc) {
   int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;

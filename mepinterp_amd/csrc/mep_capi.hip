@@ -13,6 +13,9 @@
 
 static thread_local std::string g_err;
 
+// error text for the other translation units of the library
+void mep_set_error(const std::string& msg) { g_err = msg; }
+
 static int fail(int code, const std::string& msg) {
   g_err = msg;
   return code;
@@ -74,6 +77,8 @@ struct mep_ctx {
   zd* dOm = nullptr;
   int* dinfo = nullptr;
   double* dscr = nullptr;  // eigensolver records
+  double* dcurv = nullptr;  // per-k curvature (mep_curv_batch)
+  int64_t curv_cap = 0;
   double* dpart = nullptr;
   int64_t part_cap = 0;
   double* dacc = nullptr;
@@ -360,6 +365,7 @@ extern "C" int mep_ctx_destroy(mep_ctx* c) {
   free_ws(c);
   hipFree(c->dR); hipFree(c->dtau); hipFree(c->dX); hipFree(c->dX32); hipFree(c->dinfo);
   hipFree(c->dpart); hipFree(c->dacc); hipFree(c->def); hipFree(c->dhw); hipFree(c->dfd);
+  hipFree(c->dcurv);
   for (auto e : c->pool) hipEventDestroy(e);
   if (c->st) hipStreamDestroy(c->st);
   delete c;
@@ -660,6 +666,36 @@ extern "C" int mep_interp_batch(mep_ctx* c, int64_t nk, const double* kpts, doub
       HIPCHK(hipMemcpyAsync(Om_kab + 2 * 9 * n2 * k0, c->dOm, sizeof(zd) * 9 * n2 * nb,
                             hipMemcpyDeviceToHost, c->st));
     }
+    CHK(check_info(c));
+  }
+  HIPCHK(hipStreamSynchronize(c->st));
+  resolve_times(c);
+  return MEP_OK;
+}
+
+extern "C" int mep_curv_batch(mep_ctx* c, int64_t nk, const double* kpts, double ef,
+                              double T_kelvin, double* omega) {
+  if (!c || nk < 0 || (nk > 0 && (!kpts || !omega)))
+    return fail(MEP_ERR_ARG, "mep_curv_batch: bad argument");
+  if (nk == 0) return MEP_OK;
+  CHK(ensure_ws(c, nk));
+  if (9 * c->kcap > c->curv_cap) {
+    hipFree(c->dcurv);
+    c->dcurv = nullptr;
+    HIPCHK(hipMalloc(&c->dcurv, sizeof(double) * 9 * c->kcap));
+    c->curv_cap = 9 * c->kcap;
+  }
+  for (int64_t k0 = 0; k0 < nk; k0 += c->kcap) {
+    int64_t nb = (nk - k0 < c->kcap) ? nk - k0 : c->kcap;
+    HIPCHK(hipMemcpyAsync(c->dk, kpts + 3 * k0, sizeof(double) * 3 * nb, hipMemcpyHostToDevice,
+                          c->st));
+    CHK(interp_block(c, nb, true));
+    timed(c, K_BANDSUM, [&] {
+      launch_curv(c->st, (int)c->n, nb, c->dE, c->dV, ef, T_kelvin, c->dcurv);
+    });
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(omega + 9 * k0, c->dcurv, sizeof(double) * 9 * nb,
+                          hipMemcpyDeviceToHost, c->st));
     CHK(check_info(c));
   }
   HIPCHK(hipStreamSynchronize(c->st));

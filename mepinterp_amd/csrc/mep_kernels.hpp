@@ -92,3 +92,8 @@ struct ExtLayout {
 ExtLayout ext_layout(const BandArgs& b);
 void launch_extsum(hipStream_t s, const BandArgs& b, int64_t nk, const double* E, const zd* V,
                    const zd* A, const zd* Om, const double* w, int nblocks, double* part);
+
+// per-k Berry curvature summed over occupied bands at one E_F (V planar):
+// out (nk, 9), element i + 3j
+void launch_curv(hipStream_t s, int n, int64_t nk, const double* E, const zd* V, double ef,
+                 double T_kelvin, double* out);

@@ -148,3 +148,65 @@ def make_c3(seed: int = 3):
         q = 24 + (24 * i) // 15
         out.append((m, (q, q, q)))
     return out
+
+
+def make_c4(seed: int = 4, n: int = 24, L: int = 3, n_nodes: int = 8, gap_eV: float = 1e-3,
+            v_eV: float = 0.005, m_eV: float = 0.02, couple_eV: float = 0.002, bg_eV: float = 3.0,
+            bg_hop_eV: float = 0.05, decay: float = 1.5, a: float = 6.0):
+    """Learned-refinement workload: n_wf = 24, R in {-3..3}^3 (NR = 343).
+
+    Orbitals (2w, 2w+1), w < n_nodes, carry a planted 2x2 node at a seeded
+    k_w: h_w(k) = v (sin p_x s_x + sin p_y s_y) + (m (3 - sum_a cos p_a) +
+    gap/2) s_z with p = 2 pi (k - k_w), gapped by `gap` at k_w only (a
+    Wilson term removes the other zeros of the lattice sines).  The other
+    orbitals are background bands centred at -bg and +bg (half each) with
+    hoppings N(0, bg_hop) exp(-|R|/decay) eV, and every node orbital couples
+    to them with N(0, couple) exp(-|R|/decay) eV.  E_F = 0 lies in the gap of
+    every node, so at T = 0 the model is an insulator whose Berry curvature
+    is smooth except near the eight nodes."""
+    rng = np.random.default_rng(seed)
+    R = cube_R(L)
+    nR = R.shape[1]
+    idx = {tuple(R[:, s].astype(int)): s for s in range(nR)}
+    nw = 2 * n_nodes
+    nb = n - nw
+    H = np.zeros((n, n, nR), dtype=np.complex128)
+
+    def rand_block():
+        X = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+        X[:nw, :nw] = 0.0
+        X[:nw, nw:] *= couple_eV
+        X[nw:, :nw] *= couple_eV
+        X[nw:, nw:] *= bg_hop_eV
+        return X
+
+    for s in range(nR):
+        Rv = tuple(R[:, s].astype(int))
+        if Rv > (0, 0, 0):
+            X = rand_block() * np.exp(-np.linalg.norm(R[:, s]) / decay)
+            H[:, :, s] = X
+            H[:, :, idx[tuple(-np.array(Rv))]] = X.conj().T
+    X0 = rand_block()
+    X0 = 0.5 * (X0 + X0.conj().T)
+    X0[nw:, nw:] += np.diag(np.where(np.arange(nb) < nb // 2, -bg_eV, bg_eV))
+    H[:, :, idx[(0, 0, 0)]] += X0
+    # planted nodes
+    sx, sy, sz = _PAULI
+    kw = rng.uniform(-0.5, 0.5, size=(3, n_nodes))
+    e = np.eye(3, dtype=int)
+    for w in range(n_nodes):
+        o = slice(2 * w, 2 * w + 2)
+        H[o, o, idx[(0, 0, 0)]] += (3.0 * m_eV + 0.5 * gap_eV) * sz
+        for a_ in range(3):
+            ph = np.exp(-2j * np.pi * kw[a_, w])  # e^{-2 pi i k_w.a}
+            plus, minus = idx[tuple(e[a_])], idx[tuple(-e[a_])]
+            # -m cos p_a s_z
+            H[o, o, plus] += -0.5 * m_eV * ph * sz
+            H[o, o, minus] += -0.5 * m_eV * np.conj(ph) * sz
+            if a_ < 2:
+                s_ = sx if a_ == 0 else sy
+                # v sin p_a s_a = v (e^{i p} - e^{-i p}) / (2i) s_a
+                H[o, o, plus] += v_eV * ph / 2j * s_
+                H[o, o, minus] += -v_eV * np.conj(ph) / 2j * s_
+    return dict(R_frac=R, H=np.asfortranarray(H / AU_TO_EV), r=None, wf_centers=None,
+                a_latt=a * np.eye(3), name=f"c4_n{n}_nr{nR}", nodes=kw)
