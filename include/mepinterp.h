@@ -140,6 +140,15 @@ int mep_run_kpts(mep_ctx *ctx, int64_t nk, const double *kpts, const double *wei
 /* Kernel timing with HIP events on the context's stream (off by default).
    name: "phase", "ft_gemm", "eigh", "refine", "velo", "bandsum", "reduce",
    "kgen".                                                                   */
+/* Several models in one call: mep_run_mesh (the k-loop of core.f90:165-270)
+   for each context i on the mesh mp_grids(3,i) with params[i]; the k-blocks
+   of all models are enqueued on their contexts' own streams, interleaved
+   block by block so that the GPU runs them concurrently, and waited for
+   once.  No counterpart in the reference, which runs one model per
+   process.  outs[i] as for mep_run_mesh (outs may be NULL).                */
+int mep_run_batch(int32_t n_models, mep_ctx *const *ctxs, const int32_t *mp_grids,
+                  const mep_run_params *params, mep_sums *outs);
+
 /* Per-k Berry curvature at one Fermi level: omega(9,nk), element i + 3j =
    sum_n f(n) sum_{m != n} 2 Im(V_i,nm V_j,mn) / (E_m - E_n)^2, i.e. the
    k-point's summand of kubo_ahc_tens (kubo.f90:37-78) before the k-sum, with

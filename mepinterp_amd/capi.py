@@ -31,6 +31,7 @@ MEP_PREC_MIXED = 1
 EXPORTS = ("mep_ctx_create", "mep_ctx_destroy", "mep_last_error", "mep_interp_batch",
            "mep_bandsum_batch", "mep_run_mesh", "mep_run_kpts", "mep_set_profiling",
            "mep_get_kernel_time", "mep_reset_kernel_times", "mep_set_kblock", "mep_curv_batch",
+           "mep_run_batch",
            "mep_mlp_create", "mep_mlp_destroy", "mep_mlp_num_params", "mep_mlp_get_params",
            "mep_mlp_set_params", "mep_mlp_train", "mep_mlp_predict", "mep_mlp_grad")
 
@@ -87,6 +88,8 @@ def load(path: str = LIB_PATH):
     lib.mep_reset_kernel_times.argtypes = [C.c_void_p]
     lib.mep_set_kblock.argtypes = [C.c_void_p, C.c_int64]
     lib.mep_curv_batch.argtypes = [C.c_void_p, C.c_int64, c_dp, C.c_double, C.c_double, c_dp]
+    lib.mep_run_batch.argtypes = [C.c_int32, C.POINTER(C.c_void_p), C.POINTER(C.c_int32),
+                                  C.POINTER(RunParams), C.POINTER(Sums)]
     c_fp = C.POINTER(C.c_float)
     lib.mep_mlp_create.argtypes = [C.c_int32, C.c_int32, C.c_uint64, C.POINTER(C.c_void_p)]
     lib.mep_mlp_destroy.argtypes = [C.c_void_p]
@@ -261,6 +264,24 @@ class Engine:
         _check(load().mep_curv_batch(self.ctx, k.shape[1], _p(k), float(eF), float(T_kelvin),
                                      _p(out)), "mep_curv_batch")
         return out
+
+
+def run_batch(engines, mp_grids, params):
+    """mep_run_batch: several models' meshes in one call, run concurrently
+    (one stream per engine).  Returns one sums dict per engine."""
+    n = len(engines)
+    ctxs = (C.c_void_p * n)(*[e.ctx for e in engines])
+    mp = (C.c_int32 * (3 * n))(*[int(x) for g in mp_grids for x in g])
+    ps = (RunParams * n)(*params)
+    outs, sums = [], (Sums * n)()
+    for i, p in enumerate(params):
+        o, s_ = Engine._sums_buffers(p)
+        outs.append(o)
+        sums[i] = s_
+    _check(load().mep_run_batch(n, ctxs, mp, ps, sums), "mep_run_batch")
+    for i in range(n):
+        outs[i]["n_k"] = sums[i].n_k
+    return outs
 
 
 class MLP:

@@ -8,7 +8,9 @@
 * c1  n_wf = 8, NR = 27 with a position operator, 32^3 mesh, sigma_xy at
       E_F = 0, T = 300 K, eta = 1e-3 eV
 * c3  16 models, n_wf = 8 ... 68, NR = 27 ... 729, meshes 24^3 ... 48^3
-      (each divided by --scale), one mep_run_mesh per model; aggregate k-pts/s
+      (each divided by --scale): aggregate k-pts/s of one mep_run_mesh per
+      model, and of one mep_run_batch over all of them with autotuned
+      k-blocks
 * c4  n_wf = 24, NR = 343 with eight planted nodes (gap 1e-3 eV): sigma_xy
       on a Q^3 base mesh (48 by default) with the top 5 % of cells refined by
       a 4^3 sub-mesh, cells chosen by the trained network, by the coarse
@@ -58,20 +60,44 @@ def run_c1(q: int = 32):
 
 
 def run_c3(scale: int = 1):
+    from . import tune
     jobs = dict(do_ahc=True, do_kubo=True, do_mep=True, n_eF=1, eF_min=0.0, eF_max=0.0, n_hw=1,
                 hw_min=0.0, hw_max=0.0, T_kelvin=300.0, eta=1e-3 / AU_TO_EV)
-    per = []
-    tot_k, tot_t = 0, 0.0
+    work = []
     for m, mp in models.make_c3():
         q = max(4, mp[0] // scale)
         p = dict(jobs, valence_bands=m["H"].shape[0] // 2)
-        _, dt, split = _run(m, (q, q, q), host.run_params(p))
-        per.append(dict(model=m["name"], n_k=q ** 3, seconds=dt, k_pts_s=q ** 3 / dt))
-        tot_k += q ** 3
-        tot_t += dt
+        work.append((m, (q, q, q), host.run_params(p)))
+    engines = [capi.Engine(m["R_frac"], m["H"], a_latt=m["a_latt"]) for m, _, _ in work]
+    try:
+        # per-model launches with the library's default k-blocks
+        per, t_seq = [], 0.0
+        seq_sums = []
+        for eng, (m, mp, rp) in zip(engines, work):
+            eng.run_mesh((4, 4, 4), rp)
+            t0 = time.perf_counter()
+            seq_sums.append(eng.run_mesh(mp, rp))
+            dt = time.perf_counter() - t0
+            t_seq += dt
+            per.append(dict(model=m["name"], n_k=mp[0] ** 3, seconds=dt, k_pts_s=mp[0] ** 3 / dt))
+        # autotuned k-blocks (cached), then one batched call over all models
+        kb = [tune.tune(eng, mp, rp, m["R_frac"].shape[1]) for eng, (m, mp, rp) in zip(engines, work)]
+        capi.run_batch(engines, [w[1] for w in work], [w[2] for w in work])
+        t0 = time.perf_counter()
+        bat = capi.run_batch(engines, [w[1] for w in work], [w[2] for w in work])
+        t_bat = time.perf_counter() - t0
+        dev = max(float(abs(b["ahc"][0, 1, 0] - s["ahc"][0, 1, 0]) /
+                        max(abs(s["ahc"][0, 1, 0]), 1e-300)) for b, s in zip(bat, seq_sums))
+    finally:
+        for e in engines:
+            e.close()
+    tot_k = sum(w[1][0] ** 3 for w in work)
     return dict(config="c3", workload="16 models n_wf=8..68, NR=27..729, meshes 24^3..48^3"
-                + (f" / {scale}" if scale > 1 else "") + ", one mep_run_mesh per model",
-                n_k=tot_k, seconds=tot_t, k_pts_s=tot_k / tot_t, models=per)
+                + (f" / {scale}" if scale > 1 else ""),
+                n_k=tot_k, per_model=dict(seconds=t_seq, k_pts_s=tot_k / t_seq),
+                batched=dict(seconds=t_bat, k_pts_s=tot_k / t_bat, kblocks=kb,
+                             max_rel_dev_sigma_xy_vs_per_model=dev),
+                models=per)
 
 
 def run_c4(q: int = 48):

@@ -743,6 +743,24 @@ extern "C" int mep_bandsum_batch(mep_ctx* c, int64_t nk, const double* e_k, cons
   return unpack(c, bs, p, nk, out);
 }
 
+// one k-block of a run: k-points (generated or uploaded), interpolation and
+// band sums, all on the context's stream
+static int run_block(mep_ctx* c, BandSetup& bs, int64_t k0, int64_t nb, const int32_t* mp,
+                     const double* kpts, const double* weights, const mep_run_params* p) {
+  if (mp) {
+    int m3[3] = {mp[0], mp[1], mp[2]};
+    timed(c, K_KGEN, [&] { launch_mp_kpts(c->st, k0, nb, m3, c->dk); });
+  } else {
+    HIPCHK(hipMemcpyAsync(c->dk, kpts + 3 * k0, sizeof(double) * 3 * nb, hipMemcpyHostToDevice,
+                          c->st));
+  }
+  if (weights)
+    HIPCHK(hipMemcpyAsync(c->dw, weights + k0, sizeof(double) * nb, hipMemcpyHostToDevice, c->st));
+  CHK(interp_block(c, nb, true, p->precision));
+  CHK(band_block(c, bs, nb, weights ? c->dw : nullptr));
+  return MEP_OK;
+}
+
 static int run_common(mep_ctx* c, int64_t nk, const int32_t* mp, const double* kpts,
                       const double* weights, const mep_run_params* p, mep_sums* out) {
   CHK(ensure_ws(c, nk));
@@ -750,20 +768,47 @@ static int run_common(mep_ctx* c, int64_t nk, const int32_t* mp, const double* k
   CHK(band_setup(c, p, c->kcap, bs));
   for (int64_t k0 = 0; k0 < nk; k0 += c->kcap) {
     int64_t nb = (nk - k0 < c->kcap) ? nk - k0 : c->kcap;
-    if (mp) {
-      int m3[3] = {mp[0], mp[1], mp[2]};
-      timed(c, K_KGEN, [&] { launch_mp_kpts(c->st, k0, nb, m3, c->dk); });
-    } else {
-      HIPCHK(hipMemcpyAsync(c->dk, kpts + 3 * k0, sizeof(double) * 3 * nb, hipMemcpyHostToDevice,
-                            c->st));
-    }
-    if (weights)
-      HIPCHK(hipMemcpyAsync(c->dw, weights + k0, sizeof(double) * nb, hipMemcpyHostToDevice, c->st));
-    CHK(interp_block(c, nb, true, p->precision));
-    CHK(band_block(c, bs, nb, weights ? c->dw : nullptr));
+    CHK(run_block(c, bs, k0, nb, mp, kpts, weights, p));
   }
   CHK(check_info(c));
   return unpack(c, bs, p, nk, out);
+}
+
+extern "C" int mep_run_batch(int32_t n_models, mep_ctx* const* ctxs, const int32_t* mp_grids,
+                             const mep_run_params* params, mep_sums* outs) {
+  if (n_models < 1 || !ctxs || !mp_grids || !params)
+    return fail(MEP_ERR_ARG, "mep_run_batch: bad argument");
+  std::vector<BandSetup> bs((size_t)n_models);
+  std::vector<int64_t> nk((size_t)n_models);
+  int64_t rounds = 0;
+  for (int32_t i = 0; i < n_models; ++i) {
+    mep_ctx* c = ctxs[i];
+    const int32_t* mp = mp_grids + 3 * i;
+    if (!c) return fail(MEP_ERR_ARG, "mep_run_batch: a context is NULL");
+    for (int32_t j = 0; j < i; ++j)
+      if (ctxs[j] == c) return fail(MEP_ERR_ARG, "mep_run_batch: a context appears twice");
+    if (mp[0] < 1 || mp[1] < 1 || mp[2] < 1)
+      return fail(MEP_ERR_ARG, "mep_run_batch: mp_grid entries must be >= 1");
+    nk[i] = (int64_t)mp[0] * mp[1] * mp[2];
+    CHK(ensure_ws(c, nk[i]));
+    CHK(band_setup(c, &params[i], c->kcap, bs[i]));
+    const int64_t nblk = (nk[i] + c->kcap - 1) / c->kcap;
+    if (nblk > rounds) rounds = nblk;
+  }
+  // block r of every model, model after model, each on its own stream
+  for (int64_t r = 0; r < rounds; ++r)
+    for (int32_t i = 0; i < n_models; ++i) {
+      mep_ctx* c = ctxs[i];
+      const int64_t k0 = r * c->kcap;
+      if (k0 >= nk[i]) continue;
+      const int64_t nb = (nk[i] - k0 < c->kcap) ? nk[i] - k0 : c->kcap;
+      CHK(run_block(c, bs[i], k0, nb, mp_grids + 3 * i, nullptr, nullptr, &params[i]));
+    }
+  for (int32_t i = 0; i < n_models; ++i) {
+    CHK(check_info(ctxs[i]));
+    CHK(unpack(ctxs[i], bs[i], &params[i], nk[i], outs ? &outs[i] : nullptr));
+  }
+  return MEP_OK;
 }
 
 extern "C" int mep_run_mesh(mep_ctx* c, const int32_t mp_grid[3], const mep_run_params* p,

@@ -309,3 +309,26 @@ def test_optical_and_gyrotropic_sums_match_oracle(name):
     if name != "c1_pos":
         for key in ("opt_s", "opt_a", "gyro_Dw"):
             assert not np.abs(g[key]).any() and not np.abs(o[key]).any()
+
+
+def test_run_batch_equals_per_model_runs():
+    """mep_run_batch over models of uneven n_wf and NR (both kernel
+    families) gives each model exactly its own mep_run_mesh result."""
+    from mepinterp_amd import capi, models
+    ms = [models.make_random(seed=60 + n, n=n, L=L) for n, L in ((8, 1), (20, 2), (40, 1))]
+    meshes = [(6, 5, 4), (4, 4, 4), (3, 3, 2)]
+    ps = [_params(capi.MEP_JOB_AHC | capi.MEP_JOB_KUBO | capi.MEP_JOB_MEP, n_eF=2, eF_min=-0.01,
+                  eF_max=0.01, n_hw=2, hw_max=0.05, T_kelvin=300.0, eta=0.004,
+                  valence_bands=m["H"].shape[0] // 2) for m in ms]
+    engines = [_engine(m) for m in ms]
+    try:
+        engines[0].set_kblock(64)  # several blocks for one model
+        single = [e.run_mesh(mp, p) for e, mp, p in zip(engines, meshes, ps)]
+        batch = capi.run_batch(engines, meshes, ps)
+    finally:
+        for e in engines:
+            e.close()
+    for s, b in zip(single, batch):
+        assert b["n_k"] == s["n_k"]
+        for key in KEYS:
+            np.testing.assert_array_equal(b[key], s[key], err_msg=key)
