@@ -31,7 +31,7 @@ MEP_PREC_MIXED = 1
 EXPORTS = ("mep_ctx_create", "mep_ctx_destroy", "mep_last_error", "mep_interp_batch",
            "mep_bandsum_batch", "mep_run_mesh", "mep_run_kpts", "mep_set_profiling",
            "mep_get_kernel_time", "mep_reset_kernel_times", "mep_set_kblock", "mep_curv_batch",
-           "mep_run_batch",
+           "mep_run_batch", "mep_set_workspace_limit",
            "mep_mlp_create", "mep_mlp_destroy", "mep_mlp_num_params", "mep_mlp_get_params",
            "mep_mlp_set_params", "mep_mlp_train", "mep_mlp_predict", "mep_mlp_grad")
 
@@ -87,6 +87,7 @@ def load(path: str = LIB_PATH):
     lib.mep_get_kernel_time.argtypes = [C.c_void_p, C.c_char_p, c_dp, C.POINTER(C.c_int64)]
     lib.mep_reset_kernel_times.argtypes = [C.c_void_p]
     lib.mep_set_kblock.argtypes = [C.c_void_p, C.c_int64]
+    lib.mep_set_workspace_limit.argtypes = [C.c_void_p, C.c_double]
     lib.mep_curv_batch.argtypes = [C.c_void_p, C.c_int64, c_dp, C.c_double, C.c_double, c_dp]
     lib.mep_run_batch.argtypes = [C.c_int32, C.POINTER(C.c_void_p), C.POINTER(C.c_int32),
                                   C.POINTER(RunParams), C.POINTER(Sums)]
@@ -256,6 +257,9 @@ class Engine:
 
     def set_kblock(self, kb):
         _check(load().mep_set_kblock(self.ctx, int(kb)), "mep_set_kblock")
+
+    def set_workspace_limit(self, nbytes):
+        _check(load().mep_set_workspace_limit(self.ctx, float(nbytes)), "mep_set_workspace_limit")
 
     def curv(self, kpts, eF=0.0, T_kelvin=0.0):
         """Per-k Berry curvature summed over occupied bands, (3, 3, nk)."""

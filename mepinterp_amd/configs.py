@@ -70,6 +70,8 @@ def run_c3(scale: int = 1):
         work.append((m, (q, q, q), host.run_params(p)))
     engines = [capi.Engine(m["R_frac"], m["H"], a_latt=m["a_latt"]) for m, _, _ in work]
     try:
+        for e in engines:  # all sixteen stay resident: share about 160 GB of HBM
+            e.set_workspace_limit(160e9 / len(engines))
         # per-model launches with the library's default k-blocks
         per, t_seq = [], 0.0
         seq_sums = []

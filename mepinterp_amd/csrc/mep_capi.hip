@@ -78,6 +78,7 @@ struct mep_ctx {
   int* dinfo = nullptr;
   double* dscr = nullptr;  // eigensolver records
   double* dcurv = nullptr;  // per-k curvature (mep_curv_batch)
+  double ws_limit = 16.0e9;  // bytes of per-block work buffers
   int64_t curv_cap = 0;
   double* dpart = nullptr;
   int64_t part_cap = 0;
@@ -158,9 +159,11 @@ static int64_t bytes_per_k(const mep_ctx* c) {
 }
 
 static int ensure_ws(mep_ctx* c, int64_t nk_total) {
-  // default: about 16 GB of work buffers per k-block (HBM holds 288 GB),
-  // split evenly over the blocks of this run
-  int64_t want = c->kblock_req > 0 ? c->kblock_req : (int64_t)(16.0e9 / (double)bytes_per_k(c));
+  // default: about 16 GB of work buffers per k-block (HBM holds 288 GB; see
+  // mep_set_workspace_limit), split evenly over the blocks of this run
+  const int64_t fit = (int64_t)(c->ws_limit / (double)bytes_per_k(c));
+  int64_t want = c->kblock_req > 0 ? c->kblock_req : fit;
+  if (want > fit) want = fit;
   if (want < 1) want = 1;
   if (want > nk_total) want = nk_total;
   if (c->kblock_req <= 0 && want < nk_total) {
@@ -376,6 +379,13 @@ extern "C" int mep_set_kblock(mep_ctx* c, int64_t kb) {
   if (!c || kb < 0) return fail(MEP_ERR_ARG, "mep_set_kblock: bad argument");
   c->kblock_req = kb;
   if (kb > 0 && kb < c->kcap) free_ws(c);
+  return MEP_OK;
+}
+
+extern "C" int mep_set_workspace_limit(mep_ctx* c, double bytes) {
+  if (!c || !(bytes > 0.0)) return fail(MEP_ERR_ARG, "mep_set_workspace_limit: bad argument");
+  c->ws_limit = bytes;
+  if ((double)c->kcap * bytes_per_k(c) > bytes) free_ws(c);
   return MEP_OK;
 }
 
