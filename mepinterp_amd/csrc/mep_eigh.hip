@@ -7,7 +7,8 @@
 // Supported sizes: n <= 32 here, 32 < n <= 128 in the wide kernels below
 // (the caller checks).
 // Three kernels per k-block:
-//   eigh_house (a team of NMAX lanes per k, lane i <-> row i, matrix in LDS):
+//   eigh_house (a team of NMAX lanes per k, lane i <-> row i, matrix in LDS;
+//      in FP64 at NMAX = 32 the rows stay in registers, eigh_house_reg):
 //      Householder reduction A = Q T Q^H with rank-2 trailing updates; the
 //      reflectors are stored in U's slot of the k-point, the diagonal phases
 //      P that make the subdiagonal real (T = P T_r P^H), d, e and tau in a
@@ -167,6 +168,160 @@ __global__ __launch_bounds__(64) void eigh_house_kernel(int n, int64_t nk,
 // appended, in application order, to the k-point's flat stream
 // rot[k][q] = (c, s), col[k][q] = i; the count goes to the side record.
 // Ascending eigenvalues go to E, each band's rank to the side record.
+// The same reduction with each lane's row of the Hermitian matrix in
+// registers (lane t <-> row t; rows and columns >= n are zero).  Every step
+// only reads and updates the lane's own row: the Householder vector and the
+// update vector are broadcast through a small LDS slot per team, and all
+// column indices are compile-time (the step loop is unrolled), so the
+// matrix never touches LDS.  Output identical in layout to
+// eigh_house_kernel.
+template <int NMAX, class R>
+__global__ __launch_bounds__(64) void eigh_house_reg_kernel(int n, int64_t nk,
+                                                            const typename cplx_of<R>::type* __restrict__ Hk,
+                                                            int64_t strideHk, int packed,
+                                                            const double* __restrict__ kpts,
+                                                            const double* __restrict__ tau,
+                                                            typename cplx_of<R>::type* __restrict__ U,
+                                                            R* __restrict__ side) {
+  using Z = typename cplx_of<R>::type;
+  constexpr int TEAMS = 64 / NMAX;
+  constexpr int LP = NMAX + 1;  // per-team pitch: the teams' broadcasts hit different banks
+  __shared__ Z vv_s[TEAMS * LP], ww_s[TEAMS * LP], sub_s[TEAMS * LP];
+  __shared__ R tauv_s[TEAMS * LP];
+  const int lane = threadIdx.x;
+  const int tm = lane / NMAX, t = lane % NMAX;
+  Z* vv = vv_s + tm * LP;
+  Z* ww = ww_s + tm * LP;
+  Z* sub = sub_s + tm * LP;
+  R* tauv = tauv_s + tm * LP;
+  const R two_pi = R(6.283185307179586476925);
+  const Z zero = Z{R(0), R(0)};
+
+  for (int64_t kb = (int64_t)blockIdx.x * TEAMS; kb < nk; kb += (int64_t)gridDim.x * TEAMS) {
+    const int64_t k = kb + tm;
+    const bool valid = k < nk;
+    // ---- row t: upper triangle from H(k), lower as conjugates -----------
+    Z row[NMAX];
+#pragma unroll
+    for (int j = 0; j < NMAX; ++j) {
+      Z h = zero;
+      if (valid && t < n && j < n) {
+        const int i0 = j >= t ? t : j, j0 = j >= t ? j : t;  // (i0 <= j0) element
+        const int64_t o = packed ? i0 + ((int64_t)j0 * (j0 + 1)) / 2 : i0 + (int64_t)n * j0;
+        h = Hk[k * strideHk + o];
+        if (tau && i0 != j0) {
+          const R kx = kpts[3 * k], ky = kpts[3 * k + 1], kz = kpts[3 * k + 2];
+          R ang = two_pi * (kx * (tau[3 * j0] - tau[3 * i0]) + ky * (tau[3 * j0 + 1] - tau[3 * i0 + 1]) +
+                            kz * (tau[3 * j0 + 2] - tau[3 * i0 + 2]));
+          R sn, cs;
+          sin_cos(ang, &sn, &cs);
+          h = zmul(Z{cs, sn}, h);
+        }
+        if (j < t) h = zconj(h);
+        if (j == t) h.im = R(0);
+      }
+      row[j] = h;
+    }
+    // ---- Householder steps (compile-time column index kc) ----------------
+#pragma unroll
+    for (int kc = 0; kc < NMAX - 2; ++kc) {
+      if (kc + 2 >= n) continue;  // fixed trip count: every index stays compile-time
+      const bool below = t > kc;  // rows kc+1 .. n-1 take part
+      const Z x = below ? row[kc] : zero;
+      const R tail2 = team_sum<NMAX, R>((t > kc + 1) ? zabs2(x) : R(0));
+      const int src = tm * NMAX + kc + 1;
+      const Z x0 = Z{__shfl(x.re, src, 64), __shfl(x.im, src, 64)};
+      const bool refl = tail2 > R(0);
+      const R ax02 = zabs2(x0);
+      const R xn = sqrt(tail2 + ax02);
+      const R ax0 = sqrt(ax02);
+      const Z alpha = !refl ? x0 : ((ax0 > R(0)) ? zscale(x0, -xn / ax0) : Z{-xn, R(0)});
+      Z v = (refl && below) ? x : zero;
+      if (t == kc + 1 && refl) v = zsub(x, alpha);
+      const R vn2 = team_sum<NMAX, R>(zabs2(v));
+      const R tk = refl ? R(2) / vn2 : R(0);
+      vv[t] = v;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      Z p = zero;
+#pragma unroll
+      for (int j = kc + 1; j < NMAX; ++j) p = zadd(p, zmul(row[j], vv[j]));
+      p = below ? zscale(p, tk) : zero;
+      const R K = R(0.5) * tk * team_sum<NMAX, R>(v.re * p.re + v.im * p.im);
+      const Z w = zsub(p, zscale(v, K));
+      ww[t] = w;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      if (refl && below) {
+#pragma unroll
+        for (int j = kc + 1; j < NMAX; ++j) {
+          const Z vj = vv[j], wj = ww[j];
+          row[j] = zsub(row[j], zadd(zmul(v, zconj(wj)), zmul(w, zconj(vj))));
+        }
+      }
+      if (below) row[kc] = v;
+      if (t == 0) {
+        sub[kc] = alpha;
+        tauv[kc] = tk;
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    // ---- outputs ----------------------------------------------------------
+    // diagonal and A(n-1, n-2) by compile-time picks
+    R dgl = R(0);
+    Z last = zero;
+#pragma unroll
+    for (int j = 0; j < NMAX; ++j) {
+      if (j == t) dgl = row[j].re;
+      if (j == n - 2) last = row[j];
+    }
+    if (n >= 2) {
+      const int src = tm * NMAX + n - 1;
+      const Z l2 = Z{__shfl(last.re, src, 64), __shfl(last.im, src, 64)};
+      if (t == 0) {
+        sub[n - 2] = l2;
+        tauv[n - 2] = R(0);
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (valid) {
+      R* sd = side + k * side_len(n);
+      Z* Uk = U + k * (int64_t)n * n;
+      // reflectors: row t keeps v_t of every column kc < t in row[kc]
+#pragma unroll
+      for (int kc = 0; kc < NMAX; ++kc)
+        if (kc < t && t < n) Uk[t + (int64_t)n * kc] = row[kc];
+      if (t < n) {
+        sd[t] = dgl;
+        sd[2 * n + t] = (t < n - 1) ? tauv[t] : R(0);
+      }
+      if (t == 0) {
+        Z ph = Z{R(1), R(0)};
+        sd[3 * n] = R(1);
+        sd[3 * n + 1] = R(0);
+        for (int i = 0; i + 1 < n; ++i) {
+          const Z s_ = sub[i];
+          const R as = sqrt(zabs2(s_));
+          sd[n + i] = as;
+          if (as > R(0)) ph = zmul(ph, zscale(s_, R(1) / as));
+          sd[3 * n + 2 * (i + 1)] = ph.re;
+          sd[3 * n + 2 * (i + 1) + 1] = ph.im;
+        }
+        sd[n + n - 1] = R(0);
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+}
+
 // The QL recurrence of one k-point, run by one lane; d and e live in LDS
 // columns ds[i * LANES], es[i * LANES] (already offset to the lane).
 template <int LANES, class R>
@@ -791,8 +946,12 @@ static void launch_eigh_t(hipStream_t s, int n, int64_t nk, const typename cplx_
   int64_t qb = (nk + 63) / 64;
   unsigned qgrid = (unsigned)(qb < 65536 ? qb : 65536);
 #define MEP_EIGH_LAUNCH(NM)                                                                      \
-  hipLaunchKernelGGL((eigh_house_kernel<NM, R>), dim3(grid), dim3(64), 0, s, n, nk, Hk, strideHk, \
-                     packed, g.kpts, g.tau, U, side);                                            \
+  if constexpr (NM == 32 && sizeof(R) == 8)                                                              \
+    hipLaunchKernelGGL((eigh_house_reg_kernel<NM, R>), dim3(grid), dim3(64), 0, s, n, nk, Hk,     \
+                       strideHk, packed, g.kpts, g.tau, U, side);                                \
+  else                                                                                           \
+    hipLaunchKernelGGL((eigh_house_kernel<NM, R>), dim3(grid), dim3(64), 0, s, n, nk, Hk, strideHk, \
+                       packed, g.kpts, g.tau, U, side);                                          \
   hipLaunchKernelGGL((eigh_ql_kernel<NM, R>), dim3(qgrid), dim3(64), 0, s, n, nk, side, rot,     \
                      rcol, E, info);                                                             \
   hipLaunchKernelGGL((eigh_vec_kernel<NM, R>), dim3(grid), dim3(64), 0, s, n, nk, side, rot,     \
