@@ -324,10 +324,15 @@ __global__ __launch_bounds__(64) void eigh_house_reg_kernel(int n, int64_t nk,
 
 // The QL recurrence of one k-point, run by one lane; d and e live in LDS
 // columns ds[i * LANES], es[i * LANES] (already offset to the lane).
+// rec = false: eigenvalues only, no rotation stored; the side record keeps
+// d and e and its count slot receives -1 when every gap between eigenvalues
+// is at least 1e-5 |T| (1e-3 |T| in FP32; the vectors are then computed
+// directly from T) or -2 otherwise (the caller reruns the k-point with
+// rec = true).
 template <int LANES, class R>
 __device__ inline void ql_one(int n, int64_t k, R* ds, R* es, R* __restrict__ side,
                               R* __restrict__ rot, uint8_t* __restrict__ rcol,
-                              double* __restrict__ E, int* __restrict__ info) {
+                              double* __restrict__ E, int* __restrict__ info, bool rec = true) {
   const R EPS = cplx_of<R>::eps;
   const int64_t cap = rot_cap(n);
   const R* sd = side + k * side_len(n);
@@ -348,7 +353,7 @@ __device__ inline void ql_one(int n, int64_t k, R* ds, R* es, R* __restrict__ si
         if (fabs(es[m * LANES]) <= EPS * dd) break;
       }
       if (m == l) break;
-      if (++iter > 60 || pos + (m - l) > cap) {
+      if (++iter > 60 || (rec && pos + (m - l) > cap)) {
         fail = 1;
         break;
       }
@@ -381,9 +386,11 @@ __device__ inline void ql_one(int n, int64_t k, R* ds, R* es, R* __restrict__ si
         p = s * r;
         ds[(i + 1) * LANES] = g + p;
         g = c * r - b;
-        rk[2 * pos] = c;
-        rk[2 * pos + 1] = s;
-        ck[pos] = (uint8_t)i;
+        if (rec) {
+          rk[2 * pos] = c;
+          rk[2 * pos + 1] = s;
+          ck[pos] = (uint8_t)i;
+        }
         pos += 1;
         ei = ein;
         di1 = di;
@@ -396,18 +403,27 @@ __device__ inline void ql_one(int n, int64_t k, R* ds, R* es, R* __restrict__ si
     }
   }
   if (fail) atomicOr(info, 1);
-  // ascending order: rank of each eigenvalue (ties by index)
+  // ascending order: rank of each eigenvalue (ties by index); smallest gap
+  R gap = R(1e30), big = R(0);
   for (int j = 0; j < n; ++j) {
     const R dj = ds[j * LANES];
     int rnk = 0;
     for (int i = 0; i < n; ++i) {
       const R di = ds[i * LANES];
       rnk += (di < dj) || (di == dj && i < j);
+      if (i != j) gap = fmin(gap, fabs(di - dj));
     }
+    big = fmax(big, fabs(dj));
     E[k * n + rnk] = dj;
     es[j * LANES] = (R)rnk;  // e is no longer needed: keep the rank
   }
   R* sdw = side + k * side_len(n);
+  if (!rec) {
+    // FP32 vectors are refined in FP64 afterwards: a looser separation suffices
+    const R sep = (sizeof(R) == 8) ? R(1e-5) : R(1e-3);
+    sdw[5 * n] = (gap >= sep * big) ? R(-1) : R(-2);
+    return;
+  }
   for (int j = 0; j < n; ++j) sdw[n + j] = es[j * LANES];  // rank over e's slot
   sdw[5 * n] = (R)pos;
 }
@@ -417,18 +433,22 @@ __device__ inline void ql_one(int n, int64_t k, R* ds, R* es, R* __restrict__ si
 // appended, in application order, to the k-point's flat stream
 // rot[k][q] = (c, s), col[k][q] = i; the count goes to the side record.
 // Ascending eigenvalues go to E, each band's rank to the side record.
+// mode 0: every k-point records its rotations; mode 1: eigenvalues and the
+// separation verdict only; mode 2: the k-points with verdict -2 record.
 template <int NMAX, class R>
 __global__ __launch_bounds__(64) void eigh_ql_kernel(int n, int64_t nk,
                                                      R* __restrict__ side,
                                                      R* __restrict__ rot,
                                                      uint8_t* __restrict__ rcol,
                                                      double* __restrict__ E,
-                                                     int* __restrict__ info) {
+                                                     int* __restrict__ info, int mode) {
   __shared__ R ds[NMAX * 64], es[NMAX * 64];
   const int lane = threadIdx.x;
   for (int64_t kb = (int64_t)blockIdx.x * 64; kb < nk; kb += (int64_t)gridDim.x * 64) {
     const int64_t k = kb + lane;
-    if (k < nk) ql_one<64, R>(n, k, ds + lane, es + lane, side, rot, rcol, E, info);
+    if (k >= nk) continue;
+    if (mode == 2 && side[k * side_len(n) + 5 * n] != R(-2)) continue;
+    ql_one<64, R>(n, k, ds + lane, es + lane, side, rot, rcol, E, info, mode != 1);
   }
 }
 
@@ -454,18 +474,22 @@ __global__ __launch_bounds__(32) void eigh_ql_wide_kernel(int n, int64_t nk,
 // A team per k-point.  Z starts as the identity in LDS (lane r <-> row r);
 // the rotation stream is staged through LDS one team-width chunk at a time,
 // the next chunk's loads in flight while the current one is applied.  Then
-// U = Q P Z: lane j takes column j of P Z into registers, the packed
-// reflectors are staged into the Z space, and applied in reverse order.
+// U = Q P Z: lane j takes column j of P Z into registers, the reflectors are
+// staged into the Z space as full zero-padded columns, and applied in
+// reverse order without branches.
 template <int NMAX, class R>
 __global__ __launch_bounds__(64) void eigh_vec_kernel(int n, int64_t nk,
                                                       const R* __restrict__ side,
                                                       const R* __restrict__ rot,
                                                       const uint8_t* __restrict__ rcol,
+                                                      const double* __restrict__ E,
                                                       typename cplx_of<R>::type* __restrict__ U) {
   using Z = typename cplx_of<R>::type;
   constexpr int TEAMS = 64 / NMAX;
   constexpr int LDZ = NMAX + 1;
-  __shared__ R Zt_s[TEAMS][NMAX * LDZ];  // Zt[col*LDZ + row]; later the reflectors
+  // Zt[col*LDZ + row]; later the reflectors as full columns (NMAX-2) x NMAX
+  constexpr int ZS = (NMAX * LDZ > 2 * (NMAX - 2) * NMAX) ? NMAX * LDZ : 2 * (NMAX - 2) * NMAX;
+  __shared__ R Zt_s[TEAMS][ZS];
   __shared__ R cs_s[TEAMS][2 * NMAX];
   __shared__ int ix_s[TEAMS][NMAX];
   const int lane = threadIdx.x;
@@ -478,9 +502,14 @@ __global__ __launch_bounds__(64) void eigh_vec_kernel(int n, int64_t nk,
     const int64_t k = kb + tm;
     const bool valid = k < nk;
     const R* sd = side + (valid ? k : 0) * side_len(n);
+    // count slot -1: eigenvalues separated, no rotations recorded; the
+    // eigenvectors of the real tridiagonal come straight from T below
+    const bool direct = valid && sd[5 * n] < R(0);
+    if (!direct) {
 #pragma unroll
-    for (int c = 0; c < NMAX; ++c) Zt[c * LDZ + t] = (c == t) ? R(1) : R(0);
-    const int64_t Q = valid ? (int64_t)sd[5 * n] : 0;
+      for (int c = 0; c < NMAX; ++c) Zt[c * LDZ + t] = (c == t) ? R(1) : R(0);
+    }
+    const int64_t Q = (valid && !direct) ? (int64_t)sd[5 * n] : 0;
     const R* rk = rot + (valid ? k : 0) * 2 * cap;
     const uint8_t* ck = rcol + (valid ? k : 0) * cap;
     R nc = R(0), ns = R(0);
@@ -518,6 +547,82 @@ __global__ __launch_bounds__(64) void eigh_vec_kernel(int n, int64_t nk,
         prev = i;
       }
     }
+    if (direct) {
+      // lane t: eigenvector of T = tridiag(e, d, e) for lambda_t (ascending),
+      // from the twisted factorization at the index r of smallest |gamma_r|:
+      //   dp_i = d_i - l - e_{i-1}^2 / dp_{i-1}   (top down)
+      //   um_i = d_i - l - e_i^2 / um_{i+1}       (bottom up)
+      //   gamma_i = dp_i + um_i - (d_i - l)
+      //   z_r = 1, z_i = -e_i z_{i+1} / dp_i (i < r), z_i = -e_{i-1} z_{i-1} / um_i (i > r)
+      // (Dhillon and Parlett's twisted factorizations); pivots below
+      // eps |T| are clamped to +-eps |T|.
+      const R lam = (t < n) ? (R)E[k * n + t] : R(0);
+      R tn = R(0);
+#pragma unroll
+      for (int i = 0; i < NMAX; ++i)
+        if (i < n) tn = fmax(tn, fabs(sd[i]) + R(2) * fabs(sd[n + i]));
+      const R pivmin = cplx_of<R>::eps * (tn > R(0) ? tn : R(1));
+      R um[NMAX], dp[NMAX];
+      R u = R(1);
+#pragma unroll
+      for (int i = NMAX - 1; i >= 0; --i) {
+        R v = R(1);
+        if (i < n) {
+          const R ei = (i < n - 1) ? sd[n + i] : R(0);
+          v = sd[i] - lam - ((i < n - 1) ? ei * ei / u : R(0));
+          if (fabs(v) < pivmin) v = (v < R(0)) ? -pivmin : pivmin;
+          u = v;
+        }
+        um[i] = v;
+      }
+      R dd = R(1);
+#pragma unroll
+      for (int i = 0; i < NMAX; ++i) {
+        R v = R(1);
+        if (i < n) {
+          const R em = (i > 0) ? sd[n + i - 1] : R(0);
+          v = sd[i] - lam - ((i > 0) ? em * em / dd : R(0));
+          if (fabs(v) < pivmin) v = (v < R(0)) ? -pivmin : pivmin;
+          dd = v;
+        }
+        dp[i] = v;
+      }
+      int r = 0;
+      R gmin = R(0);
+#pragma unroll
+      for (int i = 0; i < NMAX; ++i)
+        if (i < n) {
+          const R g = fabs(dp[i] + um[i] - (sd[i] - lam));
+          if (i == 0 || g < gmin) {
+            gmin = g;
+            r = i;
+          }
+        }
+      R zp = R(1);
+#pragma unroll
+      for (int i = NMAX - 1; i >= 0; --i)
+        if (i < r) {
+          zp = -(sd[n + i] / dp[i]) * zp;
+          dp[i] = zp;
+        }
+      zp = R(1);
+#pragma unroll
+      for (int i = 0; i < NMAX; ++i)
+        if (i > r && i < n) {
+          zp = -(sd[n + i - 1] / um[i]) * zp;
+          um[i] = zp;
+        }
+      R nz = R(0);
+#pragma unroll
+      for (int i = 0; i < NMAX; ++i) {
+        const R zi = (i >= n) ? R(0) : (i < r ? dp[i] : (i == r ? R(1) : um[i]));
+        dp[i] = zi;
+        nz += zi * zi;
+      }
+      const R inv = R(1) / sqrt(nz);
+#pragma unroll
+      for (int i = 0; i < NMAX; ++i) Zt[t * LDZ + i] = dp[i] * inv;
+    }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -531,30 +636,27 @@ __global__ __launch_bounds__(64) void eigh_vec_kernel(int n, int64_t nk,
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    // ---- packed reflectors (strict lower triangle) into the Z space ---------
-    Z* Vp = reinterpret_cast<Z*>(Zt);
+    // ---- reflectors as full columns (zero at and above the diagonal and
+    // beyond n) into the Z space, so the updates below run branch-free ----
+    Z* Vf = reinterpret_cast<Z*>(Zt);
     const Z* Uk = U + (valid ? k : 0) * (int64_t)n * n;
     if (valid)
-      for (int kc = 0; kc + 2 < n; ++kc) {
-        const int off = kc * (n - 1) - (kc * (kc - 1)) / 2;
-        const int r = kc + 1 + t;
-        if (r < n) Vp[off + t] = Uk[r + (int64_t)n * kc];
-      }
+      for (int kc = 0; kc + 2 < n; ++kc)
+        Vf[kc * NMAX + t] = (t > kc && t < n) ? Uk[t + (int64_t)n * kc] : Z{R(0), R(0)};
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     if (valid) {
       for (int kc = n - 3; kc >= 0; --kc) {
-        const int off = kc * (n - 1) - (kc * (kc - 1)) / 2 - (kc + 1);  // Vp[off + i], i > kc
+        const Z* v = Vf + kc * NMAX;
         const R tk = sd[2 * n + kc];
-        Z sacc = Z{R(0), R(0)};
+        // four interleaved partial sums of v^H W
+        Z s4[4] = {Z{R(0), R(0)}, Z{R(0), R(0)}, Z{R(0), R(0)}, Z{R(0), R(0)}};
 #pragma unroll
-        for (int i = 0; i < NMAX; ++i)
-          if (i > kc && i < n) sacc = zadd(sacc, zcmul(Vp[off + i], W[i]));
-        sacc = zscale(sacc, tk);
+        for (int i = 0; i < NMAX; ++i) s4[i & 3] = zadd(s4[i & 3], zcmul(v[i], W[i]));
+        const Z sacc = zscale(zadd(zadd(s4[0], s4[1]), zadd(s4[2], s4[3])), tk);
 #pragma unroll
-        for (int i = 0; i < NMAX; ++i)
-          if (i > kc && i < n) W[i] = zsub(W[i], zmul(Vp[off + i], sacc));
+        for (int i = 0; i < NMAX; ++i) W[i] = zsub(W[i], zmul(v[i], sacc));
       }
     }
     // gauge convention: largest-modulus component (first on ties) real > 0
@@ -576,7 +678,7 @@ __global__ __launch_bounds__(64) void eigh_vec_kernel(int n, int64_t nk,
         for (int i = 0; i < NMAX; ++i) W[i] = zmul(W[i], phs);
       }
     }
-    const int rnk = (valid && t < n) ? (int)sd[n + t] : 0;
+    const int rnk = (valid && t < n) ? (direct ? t : (int)sd[n + t]) : 0;
     if (valid && t < n) {
       Z* Uo = U + k * (int64_t)n * n + (int64_t)n * rnk;
 #pragma unroll
@@ -953,9 +1055,11 @@ static void launch_eigh_t(hipStream_t s, int n, int64_t nk, const typename cplx_
     hipLaunchKernelGGL((eigh_house_kernel<NM, R>), dim3(grid), dim3(64), 0, s, n, nk, Hk, strideHk, \
                        packed, g.kpts, g.tau, U, side);                                          \
   hipLaunchKernelGGL((eigh_ql_kernel<NM, R>), dim3(qgrid), dim3(64), 0, s, n, nk, side, rot,     \
-                     rcol, E, info);                                                             \
+                     rcol, E, info, 1);                                                          \
+  hipLaunchKernelGGL((eigh_ql_kernel<NM, R>), dim3(qgrid), dim3(64), 0, s, n, nk, side, rot,     \
+                     rcol, E, info, 2);                                                          \
   hipLaunchKernelGGL((eigh_vec_kernel<NM, R>), dim3(grid), dim3(64), 0, s, n, nk, side, rot,     \
-                     rcol, U);
+                     rcol, E, U);
   if (n <= 8) {
     MEP_EIGH_LAUNCH(8)
   } else if (n <= 16) {
