@@ -103,8 +103,8 @@ __device__ inline void pair_sums(int n, int p0, int part, bool act, const zd* Vs
   for (int q = 0; q < 4 * QN; ++q) pv[q] = 0.0;
   if (act) {
     const double E0 = Es[p0];
+
     for (int nn = part; nn < n; nn += PARTS) {
-      if (nn == p0) continue;
       zd x[3], y[3];
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
@@ -116,8 +116,10 @@ __device__ inline void pair_sums(int n, int p0, int part, bool act, const zd* Vs
       const zd denv = zd{dE * dE - w2.re, -w2.im};
       const zd deno = zmul(zd{dE, 0.0}, zd{dE - wq.re, -wq.im});
       const double iv = 1.0 / zabs2(denv), io = 1.0 / zabs2(deno);
-      const zd rv = zd{denv.re * iv * dfv, -denv.im * iv * dfv};  // dfv / denv
-      const zd ro = zd{deno.re * io * dfv, -deno.im * io * dfv};  // dfv / deno
+      // selects, not a branch, drop n == n0 (where 1/deno is not finite)
+      const bool off = nn != p0;
+      const zd rv = off ? zd{denv.re * iv * dfv, -denv.im * iv * dfv} : zd{0.0, 0.0};  // dfv / denv
+      const zd ro = off ? zd{deno.re * io * dfv, -deno.im * io * dfv} : zd{0.0, 0.0};  // dfv / deno
 #pragma unroll
       for (int u = 0; u < QN; ++u) {
         const int q = QB + u, i = q % 3, j = q / 3;
@@ -175,6 +177,9 @@ __global__ __launch_bounds__(BS_THREADS, 2) void bandsum_kernel(BandArgs b, doub
   const bool pmode = doAHC && (nhw * nef == 1);
   const double tol = b.kubo_tol;
   for (int64_t c = t; c < L.total; c += BS_THREADS) acc[c] = 0.0;
+  // V planes, r, E and fd are staged for indices < n only: their padding
+  // (up to NMAX) stays zero for every k, so loads need no bounds branch
+  for (int c = t; c < 7 * PL + (1 + nef) * NMAX; c += BS_THREADS) smem[c] = 0.0;
 
   const zd wq = zd{b.hw[0], b.eta};
   const zd w2 = zmul(wq, wq);
@@ -187,13 +192,27 @@ __global__ __launch_bounds__(BS_THREADS, 2) void bandsum_kernel(BandArgs b, doub
     // ---- stage V (planar [a][i + n j]) and E ------------------------------
     const zd* Vk = V + k * 3 * (int64_t)n2;
     {
-      const int i = t % NMAX;
+      // every load of the k-point is issued before the first LDS store
+      constexpr int JS = BS_THREADS / NMAX, NJ = NMAX / JS;
+      const int i = t % NMAX, jb = t / NMAX;
+      zd vt[3][NJ];
 #pragma unroll
       for (int a = 0; a < 3; ++a)
-        for (int j = t / NMAX; j < n; j += BS_THREADS / NMAX)
-          if (i < n) Vs[a * PL + i + LD * j] = Vk[a * n2 + i + n * j];
+#pragma unroll
+        for (int jj = 0; jj < NJ; ++jj) {
+          const int j = jb + JS * jj;
+          vt[a][jj] = (i < n && j < n) ? Vk[a * n2 + i + n * j] : zd{0.0, 0.0};
+        }
+      const double et = (t < n) ? E[k * n + t] : 0.0;
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int jj = 0; jj < NJ; ++jj) {
+          const int j = jb + JS * jj;
+          if (i < n && j < n) Vs[a * PL + i + LD * j] = vt[a][jj];
+        }
+      if (t < n) Es[t] = et;
     }
-    if (t < n) Es[t] = E[k * n + t];
     __syncthreads();
     // ---- r(m, c) = 1 / (E_c - E_m); occupations ---------------------------
     {
@@ -239,9 +258,10 @@ __global__ __launch_bounds__(BS_THREADS, 2) void bandsum_kernel(BandArgs b, doub
       double om[9];
 #pragma unroll
       for (int q = 0; q < 9; ++q) om[q] = 0.0;
+      // r = 0 on the diagonal and in the padding: no bounds or n == n0 branch
       if (pact && doAHC)
+
         for (int nn = pp; nn < n; nn += PARTS) {
-          if (nn == p0) continue;
           zd x[3], y[3];
 #pragma unroll
           for (int a = 0; a < 3; ++a) {
@@ -280,15 +300,15 @@ __global__ __launch_bounds__(BS_THREADS, 2) void bandsum_kernel(BandArgs b, doub
           zd z0[3];
 #pragma unroll
           for (int l = 0; l < 3; ++l) z0[l] = Vs[l * PL + p0 + LD * p0];
+
           for (int nn = pp; nn < n; nn += PARTS) {
-            if (nn == p0) continue;
             zd x[3], y[3];
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
               x[a] = Vs[a * PL + p0 + LD * nn];
               y[a] = Vs[a * PL + nn + LD * p0];
             }
-            const double r = Rs[nn + LD * p0];
+            const double r = Rs[nn + LD * p0];  // 0 for nn == n0: adds nothing
             const double r3 = r * r * r;
             // F3_ij = eps_jkl Re(V_i(n0,n) V_k(n,n0) V_l(n0,n0)) / (E0-En)^3
             zd cr[3];
@@ -370,101 +390,161 @@ __global__ __launch_bounds__(BS_THREADS, 2) void bandsum_kernel(BandArgs b, doub
     }
 
     // ==== itinerant (IC) terms on MFMA ======================================
+    // Wave item: one 16 x 16 block (tr, tc) of all three C_j when the
+    // matrices hold several blocks (NT = 2), one C_j when they hold one.
+    // The three products share their operands, V_c(row, m) and
+    // B_c(m, col) = V_c(m, col) r(m, col), and give nine independent MFMA
+    // chains.  Loads are unconditional: LDS rows and columns >= n are zero
+    // and r is zero on the diagonal, so padded entries add nothing.
     if (doK || doNjob) {
-      const int nt = (n + 15) / 16;
-      const int ntile = 3 * nt * nt;
-      for (int tile = wv; tile < ntile; tile += 4) {
-        const int j = tile / (nt * nt), tr = (tile / nt) % nt, tc = tile % nt;
-        const int kk_ = (j + 1) % 3, ll_ = (j + 2) % 3;
-        const zd* Vk_ = Vs + kk_ * PL;
-        const zd* Vl_ = Vs + ll_ * PL;
+      constexpr int NB = NT * NT;               // 16 x 16 blocks per matrix
+      constexpr int JN = (NB >= 4) ? 3 : 1;     // components per wave item
+      constexpr int NITEM = NB * (3 / JN);
+      constexpr int NP = (JN == 3) ? 3 : 2;      // velocity planes an item reads
+      for (int item = wv; item < NITEM; item += 4) {
+        const int blk = item % NB, jsel = item / NB;  // jsel: the component when JN == 1
+        const int tr = blk / NT, tc = blk % NT;
         const int r0 = 16 * tr, c0 = 16 * tc;
         const int ar = r0 + (lane & 15), bc = c0 + (lane & 15), kq = lane >> 4;
+        // plane of operand slot c; component jj pairs slot ks(jj) (its k)
+        // with slot ls(jj) (its l)
+        int pl[NP];
+#pragma unroll
+        for (int c = 0; c < NP; ++c) pl[c] = (JN == 3) ? c : (jsel + 1 + c) % 3;
+        auto ks = [](int jj) { return (JN == 3) ? (jj + 1) % 3 : 0; };
+        auto ls = [](int jj) { return (JN == 3) ? (jj + 2) % 3 : 1; };
         // C = sum_m A(., m) B(m, .) with three real products per complex
         // one: P1 = Ar Br, P2 = Ai Bi, P3 = (Ar + Ai)(Br + Bi);
         // Re C = P1 - P2, Im C = P3 - P1 - P2
-        mep_v4d p1 = {0, 0, 0, 0}, p2 = {0, 0, 0, 0}, p3 = {0, 0, 0, 0};
+        mep_v4d p1[JN], p2[JN], p3[JN];
+#pragma unroll
+        for (int jj = 0; jj < JN; ++jj) p1[jj] = p2[jj] = p3[jj] = mep_v4d{0, 0, 0, 0};
 #pragma unroll
         for (int k0 = 0; k0 < NMAX; k0 += 4) {
           const int m = k0 + kq;
-          zd a0 = zd{0.0, 0.0}, a1 = zd{0.0, 0.0}, b0 = zd{0.0, 0.0}, b1 = zd{0.0, 0.0};
-          if (m < n) {
-            if (ar < n) {
-              a0 = Vk_[ar + LD * m];
-              a1 = Vl_[ar + LD * m];
-            }
-            if (bc < n) {
-              const double rr = Rs[m + LD * bc];
-              b0 = zscale(Vl_[m + LD * bc], rr);
-              b1 = zscale(Vk_[m + LD * bc], -rr);
-            }
+          const double rr = Rs[m + LD * bc];
+          zd a[NP], s[NP];
+          double as[NP], ss[NP];
+#pragma unroll
+          for (int c = 0; c < NP; ++c) {
+            a[c] = Vs[pl[c] * PL + ar + LD * m];
+            s[c] = zscale(Vs[pl[c] * PL + m + LD * bc], rr);
+            as[c] = a[c].re + a[c].im;
+            ss[c] = s[c].re + s[c].im;
           }
-          p1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0.re, b0.re, p1, 0, 0, 0);
-          p2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0.im, b0.im, p2, 0, 0, 0);
-          p3 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0.re + a0.im, b0.re + b0.im, p3, 0, 0, 0);
-          p1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1.re, b1.re, p1, 0, 0, 0);
-          p2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1.im, b1.im, p2, 0, 0, 0);
-          p3 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1.re + a1.im, b1.re + b1.im, p3, 0, 0, 0);
+          // C_j += V_k B_l, then C_j -= V_l B_k   ((j, k, l) cyclic)
+#pragma unroll
+          for (int jj = 0; jj < JN; ++jj) {
+            const int kk = ks(jj), ll = ls(jj);
+            p1[jj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[kk].re, s[ll].re, p1[jj], 0, 0, 0);
+            p2[jj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[kk].im, s[ll].im, p2[jj], 0, 0, 0);
+            p3[jj] = __builtin_amdgcn_mfma_f64_16x16x4f64(as[kk], ss[ll], p3[jj], 0, 0, 0);
+          }
+#pragma unroll
+          for (int jj = 0; jj < JN; ++jj) {
+            const int kk = ks(jj), ll = ls(jj);
+            p1[jj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ll].re, -s[kk].re, p1[jj], 0, 0, 0);
+            p2[jj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ll].im, -s[kk].im, p2[jj], 0, 0, 0);
+            p3[jj] = __builtin_amdgcn_mfma_f64_16x16x4f64(as[ll], -ss[kk], p3[jj], 0, 0, 0);
+          }
         }
-        // contraction with A_i(n0, n) = V_i(n0, n) r(n, n0)^2, n0 = column
+        // contraction with A_i(n0, n) = V_i(n0, n) r(n, n0)^2, n0 = column;
+        // r(n, n0) = 0 for n = n0 and outside n, so those rows add nothing.
+        // f2[i + 3 jj]: component (i, j) with j = jj (JN == 3) or jsel
         const int n0 = bc;
         const bool colok = n0 < n;
         const bool doN = doNjob && colok && n0 < nval;
-        double f2[3] = {0.0, 0.0, 0.0}, f2m[3] = {0.0, 0.0, 0.0};
+        double f2[3 * JN], f2m[3 * JN];
+#pragma unroll
+        for (int q = 0; q < 3 * JN; ++q) f2[q] = f2m[q] = 0.0;
+        // mep_niu keeps, for (n0, n), the m outside one window [lo, hi]:
+        // narrow windows take -Re[A C] here and add back the excluded terms
+        // below; wide windows sum the kept terms directly below
+        auto wide = [&](int lo, int hi) { return 2 * (hi - lo + 1) > n; };
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           const int row = r0 + kq + 4 * v;
-          if (!colok || row >= n || row == n0) continue;
-          const zd cv = zd{p1[v] - p2[v], p3[v] - p1[v] - p2[v]};
           const double rr = Rs[row + LD * n0];
           const double rr2 = rr * rr;
+          bool narrow = false;
+          if (doN && row < n && row != n0) {
+            const int lo = win[2 * (n0 * NMAX + row)], hi = win[2 * (n0 * NMAX + row) + 1];
+            narrow = !(lo == 0 && hi == n - 1) && !wide(lo, hi);
+          }
           zd ai[3];
 #pragma unroll
-          for (int i = 0; i < 3; ++i) {
-            ai[i] = zscale(Vs[i * PL + n0 + LD * row], rr2);
-            f2[i] -= ai[i].re * cv.re - ai[i].im * cv.im;
+          for (int i = 0; i < 3; ++i) ai[i] = zscale(Vs[i * PL + n0 + LD * row], rr2);
+#pragma unroll
+          for (int jj = 0; jj < JN; ++jj) {
+            const zd cv = zd{p1[jj][v] - p2[jj][v], p3[jj][v] - p1[jj][v] - p2[jj][v]};
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+              const double c = ai[i].re * cv.re - ai[i].im * cv.im;
+              f2[i + 3 * jj] -= c;
+              f2m[i + 3 * jj] -= narrow ? c : 0.0;
+            }
           }
-          if (doN) {
+        }
+        // window terms, after the accumulators are dead
+        if (doN)
+#pragma unroll 1
+          for (int v = 0; v < 4; ++v) {
+            const int row = r0 + kq + 4 * v;
+            if (row >= n || row == n0) continue;
             const int lo = win[2 * (n0 * NMAX + row)], hi = win[2 * (n0 * NMAX + row) + 1];
             if (lo == 0 && hi == n - 1) continue;  // every m excluded
-            zd kept = zd{0.0, 0.0};
-            auto term = [&](int mm) {
-              const double rm = Rs[mm + LD * n0];
-              const zd bl = zscale(Vl_[mm + LD * n0], rm), bk = zscale(Vk_[mm + LD * n0], rm);
-              return zsub(zmul(Vk_[row + LD * mm], bl), zmul(Vl_[row + LD * mm], bk));
-            };
-            if (2 * (hi - lo + 1) > n) {
-              for (int mm = 0; mm < lo; ++mm) kept = zadd(kept, term(mm));
-              for (int mm = hi + 1; mm < n; ++mm) kept = zadd(kept, term(mm));
-            } else {
-              zd ex = zd{0.0, 0.0};
-              for (int mm = lo; mm <= hi; ++mm)
-                if (mm != n0) ex = zadd(ex, term(mm));
-              kept = zsub(cv, ex);
+            const bool wd = wide(lo, hi);
+            const double rr = Rs[row + LD * n0];
+            const double rr2 = rr * rr;
+            zd ai[3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) ai[i] = zscale(Vs[i * PL + n0 + LD * row], rr2);
+#pragma unroll
+            for (int jj = 0; jj < JN; ++jj) {
+              const zd* Vk_ = Vs + pl[ks(jj)] * PL;
+              const zd* Vl_ = Vs + pl[ls(jj)] * PL;
+              auto term = [&](int mm) {
+                const double rm = Rs[mm + LD * n0];
+                const zd bl = zscale(Vl_[mm + LD * n0], rm), bk = zscale(Vk_[mm + LD * n0], rm);
+                return zsub(zmul(Vk_[row + LD * mm], bl), zmul(Vl_[row + LD * mm], bk));
+              };
+              zd x = zd{0.0, 0.0};
+              if (wd) {  // kept terms: f2m -= Re(A x)
+#pragma unroll 1
+                for (int mm = 0; mm < lo; ++mm) x = zadd(x, term(mm));
+#pragma unroll 1
+                for (int mm = hi + 1; mm < n; ++mm) x = zadd(x, term(mm));
+                x = zd{-x.re, -x.im};
+              } else {   // excluded terms: f2m += Re(A x)
+#pragma unroll 1
+                for (int mm = lo; mm <= hi; ++mm)
+                  if (mm != n0) x = zadd(x, term(mm));
+              }
+#pragma unroll
+              for (int i = 0; i < 3; ++i) f2m[i + 3 * jj] += ai[i].re * x.re - ai[i].im * x.im;
             }
-#pragma unroll
-            for (int i = 0; i < 3; ++i) f2m[i] -= ai[i].re * kept.re - ai[i].im * kept.im;
           }
-        }
-        // sum the 4 row groups of the tile (lanes l, l^16, l^32, l^48)
+        // sum the 4 row groups of the block (lanes l, l^16, l^32, l^48)
 #pragma unroll
-        for (int i = 0; i < 3; ++i) {
-          f2[i] += __shfl_xor(f2[i], 16, 64);
-          f2[i] += __shfl_xor(f2[i], 32, 64);
-          f2m[i] += __shfl_xor(f2m[i], 16, 64);
-          f2m[i] += __shfl_xor(f2m[i], 32, 64);
+        for (int q = 0; q < 3 * JN; ++q) {
+          f2[q] += __shfl_xor(f2[q], 16, 64);
+          f2[q] += __shfl_xor(f2[q], 32, 64);
+          f2m[q] += __shfl_xor(f2m[q], 16, 64);
+          f2m[q] += __shfl_xor(f2m[q], 32, 64);
         }
+        const int jb = (JN == 3) ? 0 : jsel;  // component of f2[3 * 0 ..]
         if (kq == 0 && colok)
 #pragma unroll
-          for (int i = 0; i < 3; ++i) QF[(tr * 9 + i + 3 * j) * NMAX + n0] = f2m[i];
-        // kubo_mep_IC: f-weighted sum over the tile's 16 bands
+          for (int q = 0; q < 3 * JN; ++q) QF[(tr * 9 + q + 3 * jb) * NMAX + n0] = f2m[q];
+        // kubo_mep_IC: f-weighted sum over the block's 16 bands
         if (doK)
           for (int ef = 0; ef < nef; ++ef) {
             const double f = (colok && kq == 0) ? fd[ef + nef * n0] : 0.0;
 #pragma unroll
-            for (int i = 0; i < 3; ++i) {
-              const double s = band_sum<16>(f * f2[i]);
-              if (lane == 0) TS[(tile * 3 + i) * nef + ef] = s;
+            for (int q = 0; q < 3 * JN; ++q) {
+              const int i = q % 3, j = q / 3 + jb;
+              const double s = band_sum<16>(f * f2[q]);
+              if (lane == 0) TS[((j * NB + blk) * 3 + i) * nef + ef] = s;
             }
           }
       }
@@ -473,14 +553,14 @@ __global__ __launch_bounds__(BS_THREADS, 2) void bandsum_kernel(BandArgs b, doub
         for (int it = t; it < 9 * nef; it += BS_THREADS) {
           const int q = it % 9, ef = it / 9, i = q % 3, j = q / 3;
           double s = 0.0;
-          for (int tt = 0; tt < nt * nt; ++tt) s += TS[((j * nt * nt + tt) * 3 + i) * nef + ef];
+          for (int tt = 0; tt < NT * NT; ++tt) s += TS[((j * NT * NT + tt) * 3 + i) * nef + ef];
           acc[L.kic + q + 9 * ef] += wk * s;
         }
       if (doNjob)
         for (int it = t; it < 9 * nval; it += BS_THREADS) {
           const int q = it % 9, n0 = it / 9;
           double s = 0.0;
-          for (int tr = 0; tr < nt; ++tr) s += QF[(tr * 9 + q) * NMAX + n0];
+          for (int tr = 0; tr < NT; ++tr) s += QF[(tr * 9 + q) * NMAX + n0];
           acc[L.nic + q + 9 * n0] += wk * s;
         }
     }
