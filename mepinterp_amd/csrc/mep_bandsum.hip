@@ -56,7 +56,8 @@ AccLayout acc_layout(const BandArgs& b) {
 constexpr int BS_THREADS = 256;
 
 // LDS plan: V planes (zd) 3*PL | r table PL | E NMAX | fd nef*NMAX |
-// F2 guarded NT*9*NMAX | IC tile sums 9*NT*NT*3*nef | red NMAX | acc total |
+// F2 guarded NT*9*NMAX | IC tile sums 9*NT*NT*nef | red NMAX |
+// per-band Berry and LC tensors 18*NMAX | pair sums 96 | acc total |
 // Niu windows 2*NMAX*NMAX bytes              (doubles after the V planes)
 template <int NMAX>
 struct BsLds {
@@ -65,7 +66,7 @@ struct BsLds {
   static constexpr int NT = NMAX / 16;  // 16-row tiles per matrix
   static size_t doubles(int nef, int64_t total) {
     return (size_t)PL + NMAX + (size_t)nef * NMAX + (size_t)NT * 9 * NMAX +
-           (size_t)9 * NT * NT * nef + NMAX + (size_t)total;
+           (size_t)9 * NT * NT * nef + NMAX + 18 * NMAX + 96 + (size_t)total;
   }
   static size_t bytes(int nef, int64_t total) {
     return sizeof(zd) * 3 * (size_t)PL + sizeof(double) * doubles(nef, total) +
@@ -73,76 +74,75 @@ struct BsLds {
   }
 };
 
-// sum over the NMAX lanes of each lane group (every lane receives it)
-template <int NMAX>
-__device__ inline double band_sum(double v) {
-#pragma unroll
-  for (int o = 1; o < NMAX; o <<= 1) v += __shfl_xor(v, o, 64);
+// ---- cross-lane sums on the VALU: DPP row patterns and the gfx950
+// permlane swaps (no LDS round trip) ---------------------------------------
+__device__ inline double join_d(unsigned lo, unsigned hi) {
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+// v of the lane that DPP pattern CTRL selects (every source lane active)
+template <int CTRL>
+__device__ inline double dpp_d(double v) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)u, CTRL, 0xf, 0xf, false);
+  const unsigned hi =
+      (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), CTRL, 0xf, 0xf, false);
+  return join_d(lo, hi);
+}
+// lanes with bit 5 clear: a(l) + a(l ^ 32); lanes with bit 5 set: b(l) + b(l ^ 32)
+__device__ inline double swap32_sum(double a, double b) {
+  const unsigned long long ua = (unsigned long long)__double_as_longlong(a);
+  const unsigned long long ub = (unsigned long long)__double_as_longlong(b);
+  const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)ua, (unsigned)ub, false, false);
+  const auto hi =
+      __builtin_amdgcn_permlane32_swap((unsigned)(ua >> 32), (unsigned)(ub >> 32), false, false);
+  return join_d(lo[0], hi[0]) + join_d(lo[1], hi[1]);
+}
+// the same over lane bit 4 (rows 0, 2 keep a, rows 1, 3 keep b)
+__device__ inline double swap16_sum(double a, double b) {
+  const unsigned long long ua = (unsigned long long)__double_as_longlong(a);
+  const unsigned long long ub = (unsigned long long)__double_as_longlong(b);
+  const auto lo = __builtin_amdgcn_permlane16_swap((unsigned)ua, (unsigned)ub, false, false);
+  const auto hi =
+      __builtin_amdgcn_permlane16_swap((unsigned)(ua >> 32), (unsigned)(ub >> 32), false, false);
+  return join_d(lo[0], hi[0]) + join_d(lo[1], hi[1]);
+}
+// the same over lane bit 3 (hi = bit 3 set)
+__device__ inline double ror8_sum(double a, double b, bool hi) {
+  const double send = hi ? a : b;
+  return (hi ? b : a) + dpp_d<0x128>(send);  // row_ror:8
+}
+// sum over the lanes of each aligned group of 8 (every lane receives it)
+__device__ inline double group8_sum(double v) {
+  v += dpp_d<0x141>(v);  // row_half_mirror
+  v += dpp_d<0x4E>(v);   // quad_perm [2, 3, 0, 1]
+  v += dpp_d<0xB1>(v);   // quad_perm [1, 0, 3, 2]
   return v;
 }
-// sum over the lane groups (offsets NMAX, 2 NMAX, ... < 64)
-template <int NMAX>
-__device__ inline double part_sum(double v) {
-#pragma unroll
-  for (int o = NMAX; o < 64; o <<= 1) v += __shfl_xor(v, o, 64);
+// sum over the 16 lanes of each row (every lane receives it)
+__device__ inline double row16_sum(double v) {
+  v += dpp_d<0x128>(v);  // row_ror:8
+  v += dpp_d<0x124>(v);  // row_ror:4
+  v += dpp_d<0x122>(v);  // row_ror:2
+  v += dpp_d<0x121>(v);  // row_ror:1
   return v;
 }
-
-// velo_ahc_tens / kubo_ohc_tens pair sums at a single (hw, E_F) for the
-// components q = QB .. QB+QN-1 (q = i + 3j), accumulated over every pair by
-// one wave and added by lane 0:
-//   velo_q += (f0 - fn) Im(V_i(n0,n) V_j(n,n0)) / ((E0-En)^2 - (hw + i eta)^2)
-//   ohc_q  += i (f0 - fn) V_i(n,n0) V_j(n0,n) / ((E0-En)(E0-En - hw - i eta))
-template <int NMAX, int QB, int QN>
-__device__ inline void pair_sums(int n, int p0, int part, bool act, const zd* Vs, const double* Es,
-                                 const double* fd, zd wq, zd w2, double wk, double* av,
-                                 double* ao) {
-  constexpr int LD = NMAX + 1, PL = NMAX * LD, PARTS = 64 / NMAX;
-  double pv[4 * QN];
+// v(l) + v(l ^ 16) + v(l ^ 32) + v(l ^ 48) (every lane receives it)
+__device__ inline double rows4_sum(double v) {
+  v = swap16_sum(v, v);
+  return swap32_sum(v, v);
+}
+// Sums of 24 values over lane bits 3-5, scattered: lane l ends with the
+// slots 12 b5 + 6 b4 + 3 b3 + s (s < 3) of l's bits b5 b4 b3, each summed
+// over the 8 lanes that share l's bits 0-2.
+__device__ inline void scatter_sum24(const double* v, double* o, int lane) {
+  double h[12], q[6];
 #pragma unroll
-  for (int q = 0; q < 4 * QN; ++q) pv[q] = 0.0;
-  if (act) {
-    const double E0 = Es[p0];
-
-    for (int nn = part; nn < n; nn += PARTS) {
-      zd x[3], y[3];
+  for (int j = 0; j < 12; ++j) h[j] = swap32_sum(v[j], v[j + 12]);
 #pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        x[a] = Vs[a * PL + p0 + LD * nn];
-        y[a] = Vs[a * PL + nn + LD * p0];
-      }
-      const double dE = E0 - Es[nn];
-      const double dfv = fd[p0] - fd[nn];
-      const zd denv = zd{dE * dE - w2.re, -w2.im};
-      const zd deno = zmul(zd{dE, 0.0}, zd{dE - wq.re, -wq.im});
-      const double iv = 1.0 / zabs2(denv), io = 1.0 / zabs2(deno);
-      // selects, not a branch, drop n == n0 (where 1/deno is not finite)
-      const bool off = nn != p0;
-      const zd rv = off ? zd{denv.re * iv * dfv, -denv.im * iv * dfv} : zd{0.0, 0.0};  // dfv / denv
-      const zd ro = off ? zd{deno.re * io * dfv, -deno.im * io * dfv} : zd{0.0, 0.0};  // dfv / deno
+  for (int j = 0; j < 6; ++j) q[j] = swap16_sum(h[j], h[j + 6]);
+  const bool b3 = (lane & 8) != 0;
 #pragma unroll
-      for (int u = 0; u < QN; ++u) {
-        const int q = QB + u, i = q % 3, j = q / 3;
-        const double vim = zmul(x[i], y[j]).im;
-        pv[2 * u] += vim * rv.re;
-        pv[2 * u + 1] += vim * rv.im;
-        const zd o = zmuli(zmul(zmul(y[i], x[j]), ro));
-        pv[2 * QN + 2 * u] += o.re;
-        pv[2 * QN + 2 * u + 1] += o.im;
-      }
-    }
-  }
-#pragma unroll
-  for (int q = 0; q < 4 * QN; ++q) pv[q] = band_sum<64>(pv[q]);
-  if (threadIdx.x % 64 == 0)
-#pragma unroll
-    for (int u = 0; u < QN; ++u) {
-      const int q = QB + u;
-      av[2 * q] += wk * pv[2 * u];
-      av[2 * q + 1] += wk * pv[2 * u + 1];
-      ao[2 * q] += wk * pv[2 * QN + 2 * u];
-      ao[2 * q + 1] += wk * pv[2 * QN + 2 * u + 1];
-    }
+  for (int j = 0; j < 3; ++j) o[j] = ror8_sum(q[j], q[j + 3], b3);
 }
 
 template <int NMAX>
@@ -156,7 +156,6 @@ __global__ __launch_bounds__(BS_THREADS, 2) void bandsum_kernel(BandArgs b, doub
                                                                 double* __restrict__ part) {
   using S_ = BsLds<NMAX>;
   constexpr int LD = S_::LD, PL = S_::PL, NT = S_::NT;
-  constexpr int PARTS = 64 / NMAX;
   extern __shared__ double smem[];
   const int n = b.n, nef = b.n_ef, nhw = b.n_hw, nval = b.n_val;
   zd* Vs = reinterpret_cast<zd*>(smem);
@@ -166,7 +165,11 @@ __global__ __launch_bounds__(BS_THREADS, 2) void bandsum_kernel(BandArgs b, doub
   double* QF = fd + (int64_t)nef * NMAX;        // [NT][9][NMAX] guarded IC per band
   double* TS = QF + NT * 9 * NMAX;              // [3*NT*NT tiles][3][nef] Kubo IC tile sums
   double* red = TS + 9 * NT * NT * nef;
-  double* acc = red + NMAX;
+  double* OMA = red + NMAX;                     // [9][NMAX] Berry curvature per band
+  double* F3A = OMA + 9 * NMAX;                 // [9][NMAX] Kubo LC tensor per band
+  double* PW = F3A + 9 * NMAX;                  // velo / ohc pair sums: [4 waves][24]
+                                                // (Hermitian V) or [36] (general V)
+  double* acc = PW + 96;
   uint8_t* win = reinterpret_cast<uint8_t*>(acc + L.total);  // [n0][n] lo, hi
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   const int n2 = n * n;
@@ -175,6 +178,10 @@ __global__ __launch_bounds__(BS_THREADS, 2) void bandsum_kernel(BandArgs b, doub
   const bool doK = (b.jobs & MEP_JOB_KUBO) != 0;
   const bool doNjob = (b.jobs & MEP_JOB_MEP) != 0;
   const bool pmode = doAHC && (nhw * nef == 1);
+  // V(n, m) = conj(V(m, n)) holds for V built here without a position
+  // operator (with one, the reference's connection is not Hermitian; V from
+  // a caller is taken as it is)
+  const bool herm = b.v_herm != 0;
   const double tol = b.kubo_tol;
   for (int64_t c = t; c < L.total; c += BS_THREADS) acc[c] = 0.0;
   // V planes, r, E and fd are staged for indices < n only: their padding
@@ -183,8 +190,6 @@ __global__ __launch_bounds__(BS_THREADS, 2) void bandsum_kernel(BandArgs b, doub
 
   const zd wq = zd{b.hw[0], b.eta};
   const zd w2 = zmul(wq, wq);
-  const int p0 = lane % NMAX, pp = lane / NMAX;  // pair phase: band p0, part pp
-  const bool pact = p0 < n;
 
   for (int64_t k = blockIdx.x; k < nk; k += gridDim.x) {
     const double wk = w ? w[k] : 1.0;
@@ -245,62 +250,49 @@ __global__ __launch_bounds__(BS_THREADS, 2) void bandsum_kernel(BandArgs b, doub
     }
     __syncthreads();
 
-    // ==== pair phase: lane (p0, pp) of every wave walks n = pp, pp + PARTS,
-    // ...; wave 0: electron count and Berry curvature; wave 1: local MEP
-    // terms; waves 2-3: velo/ohc pair sums.  Each wave then goes straight on
-    // to its itinerant tiles.
-    if (wv == 0) {
-      if (t < 64 && pp == 0)
-        for (int ef = 0; ef < nef; ++ef) {
-          const double s = band_sum<NMAX>(pact ? fd[ef + nef * p0] : 0.0);
-          if (lane == 0) acc[L.Ne + ef] += wk * s;
-        }
-      double om[9];
+    if (!herm) {
+      // ==== pair phase, general V (position operator: the reference's
+      // connection makes V non-Hermitian, so V(n, n0) is read as stored).
+      // Lane (p0, pp) of each wave walks n = pp, pp + P, ...: wave 0 the
+      // Berry curvature of p0, wave 1 its local (LC) terms, waves 2-3 the
+      // velo / ohc pair sums at the single (hw, E_F).
+      constexpr int PARTS = 64 / NMAX;
+      const int p0 = lane % NMAX, pp = lane / NMAX;
+      if (wv == 0) {
+        double om[9];
 #pragma unroll
-      for (int q = 0; q < 9; ++q) om[q] = 0.0;
-      // r = 0 on the diagonal and in the padding: no bounds or n == n0 branch
-      if (pact && doAHC)
-
-        for (int nn = pp; nn < n; nn += PARTS) {
-          zd x[3], y[3];
+        for (int q = 0; q < 9; ++q) om[q] = 0.0;
+        if (doAHC)
+          for (int nn = pp; nn < n; nn += PARTS) {
+            zd x[3], y[3];
 #pragma unroll
-          for (int a = 0; a < 3; ++a) {
-            x[a] = Vs[a * PL + p0 + LD * nn];  // V_a(n0, n)
-            y[a] = Vs[a * PL + nn + LD * p0];  // V_a(n, n0)
+            for (int a = 0; a < 3; ++a) {
+              x[a] = Vs[a * PL + p0 + LD * nn];  // V_a(n0, n)
+              y[a] = Vs[a * PL + nn + LD * p0];  // V_a(n, n0)
+            }
+            const double r = Rs[nn + LD * p0];  // 1 / (E0 - En)
+            const double r2 = r * r;
+            // Omega_n0,ij = -2 Im(V_i(n0,n) V_j(n,n0)) / (En - E0)^2
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+#pragma unroll
+              for (int i = 0; i < 3; ++i) om[i + 3 * j] -= 2.0 * zmul(x[i], y[j]).im * r2;
           }
-          const double r = Rs[nn + LD * p0];  // 1 / (E0 - En)
-          const double r2 = r * r;
-          // Omega_n0,ij = -2 Im(V_i(n0,n) V_j(n,n0)) / (En - E0)^2
 #pragma unroll
-          for (int j = 0; j < 3; ++j)
+        for (int q = 0; q < 9; ++q) om[q] = (NMAX == 32) ? swap32_sum(om[q], om[q]) : rows4_sum(om[q]);
+        if (pp == 0)
 #pragma unroll
-            for (int i = 0; i < 3; ++i) om[i + 3 * j] -= 2.0 * zmul(x[i], y[j]).im * r2;
-        }
-      if (doAHC) {
-        // kubo_ahc_tens: -sum_n0 f(E_F, n0) Omega_n0
-#pragma unroll
-        for (int q = 0; q < 9; ++q) om[q] = part_sum<NMAX>(om[q]);
-        for (int ef = 0; ef < nef; ++ef) {
-          const double f = (pact && pp == 0) ? fd[ef + nef * p0] : 0.0;
-#pragma unroll
-          for (int q = 0; q < 9; ++q) {
-            const double s = band_sum<64>(f * om[q]);
-            if (lane == 0) acc[L.ahc + q + 9 * ef] -= wk * s;
-          }
-        }
-      }
-    } else if (wv == 1) {
-      if (doK || doNjob) {
+          for (int q = 0; q < 9; ++q) OMA[q * NMAX + p0] = om[q];
+      } else if (wv == 1) {
         double f3[9], f3m[9];
 #pragma unroll
         for (int q = 0; q < 9; ++q) f3[q] = f3m[q] = 0.0;
-        const bool doN = doNjob && pact && p0 < nval;
-        if (pact) {
+        const bool doN = doNjob && p0 < n && p0 < nval;
+        if (doK || doNjob) {
           const double E0 = Es[p0];
           zd z0[3];
 #pragma unroll
           for (int l = 0; l < 3; ++l) z0[l] = Vs[l * PL + p0 + LD * p0];
-
           for (int nn = pp; nn < n; nn += PARTS) {
             zd x[3], y[3];
 #pragma unroll
@@ -329,28 +321,195 @@ __global__ __launch_bounds__(BS_THREADS, 2) void bandsum_kernel(BandArgs b, doub
         }
 #pragma unroll
         for (int q = 0; q < 9; ++q) {
-          f3[q] = part_sum<NMAX>(f3[q]);
-          f3m[q] = part_sum<NMAX>(f3m[q]);
+          f3[q] = (NMAX == 32) ? swap32_sum(f3[q], f3[q]) : rows4_sum(f3[q]);
+          f3m[q] = (NMAX == 32) ? swap32_sum(f3m[q], f3m[q]) : rows4_sum(f3m[q]);
         }
-        if (doN && pp == 0)
+        if (pp == 0) {
 #pragma unroll
-          for (int q = 0; q < 9; ++q) acc[L.nlc + q + 9 * p0] += wk * f3m[q];
-        if (doK)
-          for (int ef = 0; ef < nef; ++ef) {
-            const double f = (pact && pp == 0) ? fd[ef + nef * p0] : 0.0;
+          for (int q = 0; q < 9; ++q) F3A[q * NMAX + p0] = f3[q];
+          if (doN)
 #pragma unroll
-            for (int q = 0; q < 9; ++q) {
-              const double s = band_sum<64>(f * f3[q]);
-              if (lane == 0) acc[L.klc + q + 9 * ef] += wk * s;
-            }
+            for (int q = 0; q < 9; ++q) acc[L.nlc + q + 9 * p0] += wk * f3m[q];
+        }
+      } else if (pmode && wv == 2) {
+        // components q = i + 3j:
+        //   velo_q += (f0 - fn) Im(V_i(n0,n) V_j(n,n0)) / ((E0-En)^2 - (hw + i eta)^2)
+        //   ohc_q  += i (f0 - fn) V_i(n,n0) V_j(n0,n) / ((E0-En)(E0-En - hw - i eta))
+        double pv[36];
+#pragma unroll
+        for (int u = 0; u < 36; ++u) pv[u] = 0.0;
+        const double E0 = Es[p0];
+        for (int nn = pp; nn < n; nn += PARTS) {
+          zd x[3], y[3];
+#pragma unroll
+          for (int a = 0; a < 3; ++a) {
+            x[a] = Vs[a * PL + p0 + LD * nn];
+            y[a] = Vs[a * PL + nn + LD * p0];
           }
+          const double dE = E0 - Es[nn];
+          const double dfv = fd[p0] - fd[nn];
+          const zd denv = zd{dE * dE - w2.re, -w2.im};
+          const zd deno = zmul(zd{dE, 0.0}, zd{dE - wq.re, -wq.im});
+          const double iv = 1.0 / zabs2(denv), io = 1.0 / zabs2(deno);
+          const bool off = nn != p0;
+          const zd rv = off ? zd{denv.re * iv * dfv, -denv.im * iv * dfv} : zd{0.0, 0.0};
+          const zd ro = off ? zd{deno.re * io * dfv, -deno.im * io * dfv} : zd{0.0, 0.0};
+#pragma unroll
+          for (int q = 0; q < 9; ++q) {
+            const int i = q % 3, j = q / 3;
+            const double vim = zmul(x[i], y[j]).im;
+            pv[2 * q] += vim * rv.re;
+            pv[2 * q + 1] += vim * rv.im;
+            const zd o = zmuli(zmul(zmul(y[i], x[j]), ro));
+            pv[18 + 2 * q] += o.re;
+            pv[19 + 2 * q] += o.im;
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 36; ++u) pv[u] = rows4_sum(row16_sum(pv[u]));
+        if (lane == 0)
+#pragma unroll
+          for (int u = 0; u < 36; ++u) PW[u] = pv[u];
       }
-    } else if (pmode) {
-      // wave 2: components q = 0..4, wave 3: q = 5..8 (q = i + 3j)
-      if (wv == 2)
-        pair_sums<NMAX, 0, 5>(n, p0, pp, pact, Vs, Es, fd, wq, w2, wk, acc + L.velo, acc + L.ohc);
-      else
-        pair_sums<NMAX, 5, 4>(n, p0, pp, pact, Vs, Es, fd, wq, w2, wk, acc + L.velo, acc + L.ohc);
+    } else {
+    // ==== pair phase: wave w owns the bands n0 = 8w .. 8w + 7; lane
+    // (n0 = 8w + lane % 8, part = lane / 8) walks n = part, part + 8, ...
+    // V is Hermitian, so V_a(n, n0) = conj(x_a) with x_a = V_a(n0, n), and
+    // every pair sum is built from P_ij = x_i conj(x_j) (i < j) and |x_i|^2.
+    // r = 1 / (E0 - En) is 0 for n = n0 and in the padding.
+    if (wv * 8 < n) {
+      const int n0 = wv * 8 + (lane & 7), part = lane >> 3;
+      const bool doN = doNjob && n0 < nval && n0 < n;
+      const double E0 = Es[n0];
+      double om[3] = {0.0, 0.0, 0.0};        // sum r^2 Im P_ij
+      double gd[3] = {0.0, 0.0, 0.0};        // sum r^3 |x_i|^2
+      double gdm[3] = {0.0, 0.0, 0.0};       // the same over Niu-kept n
+      zd go[3], gom[3];                      // sum r^3 P_ij (i < j), kept
+      double ps[24];                         // velo / ohc pair sums
+#pragma unroll
+      for (int c = 0; c < 3; ++c) go[c] = gom[c] = zd{0.0, 0.0};
+#pragma unroll
+      for (int c = 0; c < 24; ++c) ps[c] = 0.0;
+      const double f0 = pmode ? fd[n0] : 0.0;
+      for (int nn = part; nn < n; nn += 8) {
+        zd x[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) x[a] = Vs[a * PL + n0 + LD * nn];
+        const double r = Rs[nn + LD * n0];
+        const double r2 = r * r, r3 = r2 * r;
+        zd P[3];
+        double d[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) d[a] = x[a].re * x[a].re + x[a].im * x[a].im;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const zd xi = x[c == 2 ? 1 : 0], xj = x[c == 0 ? 1 : 2];
+          P[c] = zd{xi.re * xj.re + xi.im * xj.im, xi.im * xj.re - xi.re * xj.im};
+        }
+        if (doAHC)
+#pragma unroll
+          for (int c = 0; c < 3; ++c) om[c] += r2 * P[c].im;
+        if (doK || doNjob) {
+          const double dn = E0 - Es[nn];
+          const double r3m = (doN && fabs(dn * dn * dn) > tol) ? r3 : 0.0;
+#pragma unroll
+          for (int a = 0; a < 3; ++a) {
+            gd[a] += r3 * d[a];
+            gdm[a] += r3m * d[a];
+          }
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            go[c] = zd{go[c].re + r3 * P[c].re, go[c].im + r3 * P[c].im};
+            gom[c] = zd{gom[c].re + r3m * P[c].re, gom[c].im + r3m * P[c].im};
+          }
+        }
+        if (pmode) {
+          const double dE = E0 - Es[nn];
+          const double dfv = f0 - fd[nn];
+          const zd denv = zd{dE * dE - w2.re, -w2.im};
+          const zd deno = zmul(zd{dE, 0.0}, zd{dE - wq.re, -wq.im});
+          const double iv = 1.0 / zabs2(denv), io = 1.0 / zabs2(deno);
+          // selects, not a branch, drop n == n0 (where 1/deno is not finite)
+          const bool off = nn != n0;
+          const zd rv = off ? zd{denv.re * iv * dfv, -denv.im * iv * dfv} : zd{0.0, 0.0};
+          const zd ro = off ? zd{deno.re * io * dfv, -deno.im * io * dfv} : zd{0.0, 0.0};
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            ps[2 * c] += rv.re * P[c].im;
+            ps[2 * c + 1] += rv.im * P[c].im;
+            ps[6 + 2 * c] += ro.re * P[c].re;
+            ps[7 + 2 * c] += ro.re * P[c].im;
+            ps[12 + 2 * c] += ro.im * P[c].re;
+            ps[13 + 2 * c] += ro.im * P[c].im;
+            ps[18 + c] += ro.re * d[c];
+            ps[21 + c] += ro.im * d[c];
+          }
+        }
+      }
+      // local terms of n0: F3_ij = Re(z_l G_ik - z_k G_il), (j, k, l) cyclic,
+      // z = V(n0, n0), G_ik = sum_n r^3 x_i conj(x_k) (Hermitian in i, k)
+      double v24[24];
+      {
+        zd z[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) z[a] = Vs[a * PL + n0 + LD * n0];
+        auto G = [&](const double* gd_, const zd* go_, int i, int k) {
+          if (i == k) return zd{gd_[i], 0.0};
+          const zd g = go_[i + k - 1];
+          return i < k ? g : zd{g.re, -g.im};
+        };
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          const int k = (j + 1) % 3, l = (j + 2) % 3;
+#pragma unroll
+          for (int i = 0; i < 3; ++i) {
+            const zd a = G(gd, go, i, k), bb = G(gd, go, i, l);
+            const zd am = G(gdm, gom, i, k), bm = G(gdm, gom, i, l);
+            v24[i + 3 * j] = (z[l].re * a.re - z[l].im * a.im) - (z[k].re * bb.re - z[k].im * bb.im);
+            v24[9 + i + 3 * j] =
+                (z[l].re * am.re - z[l].im * am.im) - (z[k].re * bm.re - z[k].im * bm.im);
+          }
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          v24[18 + c] = om[c];
+          v24[21 + c] = 0.0;
+        }
+      }
+      // sums over the 8 parts (lane bits 3-5), scattered: lane (n0, g) ends
+      // with the slots 12 g5 + 6 g4 + 3 g3 + s, s < 3, of its n0
+      double v3[3];
+      scatter_sum24(v24, v3, lane);
+      {
+        const int base = 12 * ((lane >> 5) & 1) + 6 * ((lane >> 4) & 1) + 3 * ((lane >> 3) & 1);
+#pragma unroll
+        for (int s = 0; s < 3; ++s) {
+          const int slot = base + s;
+          if (slot < 9) {
+            F3A[slot * NMAX + n0] = v3[s];
+          } else if (slot < 18) {
+            if (doN) acc[L.nlc + (slot - 9) + 9 * n0] += wk * v3[s];
+          } else if (slot < 21) {  // Omega_ij = -2 v, Omega_ji = 2 v for pair (i, j)
+            const int c = slot - 18, i = (c == 2) ? 1 : 0, j = (c == 0) ? 1 : 2;
+            OMA[(i + 3 * j) * NMAX + n0] = -2.0 * v3[s];
+            OMA[(j + 3 * i) * NMAX + n0] = 2.0 * v3[s];
+          } else {                 // Omega_ii = 0
+            OMA[4 * (slot - 21) * NMAX + n0] = 0.0;
+          }
+        }
+      }
+      if (pmode) {
+        scatter_sum24(ps, v3, lane);
+        const int base = 12 * ((lane >> 5) & 1) + 6 * ((lane >> 4) & 1) + 3 * ((lane >> 3) & 1);
+#pragma unroll
+        for (int s = 0; s < 3; ++s) v3[s] = group8_sum(v3[s]);
+        if ((lane & 7) == 0)
+#pragma unroll
+          for (int s = 0; s < 3; ++s) PW[wv * 24 + base + s] = v3[s];
+      }
+    } else if (pmode && lane < 24) {
+      PW[wv * 24 + lane] = 0.0;
+    }
     }
     // ---- hw-resolved Hall and optical conductivity for several (hw, E_F) ---
     if (doAHC && !pmode) {
@@ -461,15 +620,21 @@ __global__ __launch_bounds__(BS_THREADS, 2) void bandsum_kernel(BandArgs b, doub
         // narrow windows take -Re[A C] here and add back the excluded terms
         // below; wide windows sum the kept terms directly below
         auto wide = [&](int lo, int hi) { return 2 * (hi - lo + 1) > n; };
+        // rows whose window holds more than n0 itself (the others need no
+        // second pass) and is not the whole band range
+        bool wterm[4];
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           const int row = r0 + kq + 4 * v;
           const double rr = Rs[row + LD * n0];
           const double rr2 = rr * rr;
           bool narrow = false;
+          wterm[v] = false;
           if (doN && row < n && row != n0) {
             const int lo = win[2 * (n0 * NMAX + row)], hi = win[2 * (n0 * NMAX + row) + 1];
-            narrow = !(lo == 0 && hi == n - 1) && !wide(lo, hi);
+            const bool all = lo == 0 && hi == n - 1;
+            narrow = !all && !wide(lo, hi);
+            wterm[v] = !all && !(lo == n0 && hi == n0);
           }
           zd ai[3];
 #pragma unroll
@@ -489,10 +654,9 @@ __global__ __launch_bounds__(BS_THREADS, 2) void bandsum_kernel(BandArgs b, doub
         if (doN)
 #pragma unroll 1
           for (int v = 0; v < 4; ++v) {
+            if (!wterm[v]) continue;
             const int row = r0 + kq + 4 * v;
-            if (row >= n || row == n0) continue;
             const int lo = win[2 * (n0 * NMAX + row)], hi = win[2 * (n0 * NMAX + row) + 1];
-            if (lo == 0 && hi == n - 1) continue;  // every m excluded
             const bool wd = wide(lo, hi);
             const double rr = Rs[row + LD * n0];
             const double rr2 = rr * rr;
@@ -527,10 +691,8 @@ __global__ __launch_bounds__(BS_THREADS, 2) void bandsum_kernel(BandArgs b, doub
         // sum the 4 row groups of the block (lanes l, l^16, l^32, l^48)
 #pragma unroll
         for (int q = 0; q < 3 * JN; ++q) {
-          f2[q] += __shfl_xor(f2[q], 16, 64);
-          f2[q] += __shfl_xor(f2[q], 32, 64);
-          f2m[q] += __shfl_xor(f2m[q], 16, 64);
-          f2m[q] += __shfl_xor(f2m[q], 32, 64);
+          f2[q] = rows4_sum(f2[q]);
+          f2m[q] = rows4_sum(f2m[q]);
         }
         const int jb = (JN == 3) ? 0 : jsel;  // component of f2[3 * 0 ..]
         if (kq == 0 && colok)
@@ -543,27 +705,74 @@ __global__ __launch_bounds__(BS_THREADS, 2) void bandsum_kernel(BandArgs b, doub
 #pragma unroll
             for (int q = 0; q < 3 * JN; ++q) {
               const int i = q % 3, j = q / 3 + jb;
-              const double s = band_sum<16>(f * f2[q]);
+              const double s = row16_sum(f * f2[q]);
               if (lane == 0) TS[((j * NB + blk) * 3 + i) * nef + ef] = s;
             }
           }
       }
-      __syncthreads();
-      if (doK)
-        for (int it = t; it < 9 * nef; it += BS_THREADS) {
-          const int q = it % 9, ef = it / 9, i = q % 3, j = q / 3;
-          double s = 0.0;
-          for (int tt = 0; tt < NT * NT; ++tt) s += TS[((j * NT * NT + tt) * 3 + i) * nef + ef];
-          acc[L.kic + q + 9 * ef] += wk * s;
-        }
-      if (doNjob)
-        for (int it = t; it < 9 * nval; it += BS_THREADS) {
-          const int q = it % 9, n0 = it / 9;
-          double s = 0.0;
-          for (int tr = 0; tr < NT; ++tr) s += QF[(tr * 9 + q) * NMAX + n0];
-          acc[L.nic + q + 9 * n0] += wk * s;
-        }
     }
+    __syncthreads();
+    // ---- sums over the bands n0 and over the waves' partial records --------
+    // Ne, kubo_ahc_tens (-sum f Omega) and kubo_mep_LC for every E_F
+    // for every E_F; item (c, ef) is summed by 8 lanes, lane g taking the
+    // bands m0 = g, g + 8, ...
+    for (int base = 0; base < 8 * 19 * nef; base += BS_THREADS) {
+      const int it = base + t, item = it >> 3, g = it & 7;
+      const int c = item % 19, ef = item / 19;
+      const bool valid = item < 19 * nef &&
+                         (c == 0 || (c < 10 && doAHC) || (c >= 10 && doK));
+      double s = 0.0;
+      if (valid) {
+        const double* src = (c == 0) ? nullptr : (c < 10) ? OMA + (c - 1) * NMAX : F3A + (c - 10) * NMAX;
+        for (int m0 = g; m0 < n; m0 += 8) s += fd[ef + nef * m0] * (src ? src[m0] : 1.0);
+      }
+      s = group8_sum(s);  // the loop bounds are uniform over the wave
+      if (valid && g == 0) {
+        if (c == 0)
+          acc[L.Ne + ef] += wk * s;
+        else if (c < 10)
+          acc[L.ahc + (c - 1) + 9 * ef] -= wk * s;
+        else
+          acc[L.klc + (c - 10) + 9 * ef] += wk * s;
+      }
+    }
+    // velo_ahc_tens / kubo_ohc_tens at the single (hw, E_F): the last 18
+    // threads, o = q (velo) and 9 + q (ohc)
+    if (pmode && t >= BS_THREADS - 18) {
+      const int o = t - (BS_THREADS - 18);  // 0..8 velo, 9..17 ohc
+      const int q = o % 9, i = q % 3, j = q / 3, c = i + j - 1;
+      auto u = [&](int e) { return ((PW[e] + PW[24 + e]) + PW[48 + e]) + PW[72 + e]; };
+      zd val = zd{0.0, 0.0};
+      if (!herm) {
+        val = zd{PW[2 * o], PW[2 * o + 1]};
+      } else if (o < 9) {  // velo: sum rv Im P_ij, antisymmetric in (i, j)
+        if (i != j) val = (i < j) ? zd{u(2 * c), u(2 * c + 1)} : zd{-u(2 * c), -u(2 * c + 1)};
+      } else {      // ohc: i sum ro conj(P_ij) = i conj(X - iY) for X = sum Re(ro) P, Y = sum Im(ro) P
+        if (i == j) {
+          val = zd{-u(21 + i), u(18 + i)};
+        } else {
+          const double xr = u(6 + 2 * c), xi = u(7 + 2 * c), yr = u(12 + 2 * c), yi = u(13 + 2 * c);
+          val = (i < j) ? zd{xi - yr, xr + yi} : zd{-(xi + yr), xr - yi};
+        }
+      }
+      double* dst = acc + (o < 9 ? L.velo : L.ohc) + 2 * q;
+      dst[0] += wk * val.re;
+      dst[1] += wk * val.im;
+    }
+    if (doK)
+      for (int it = t; it < 9 * nef; it += BS_THREADS) {
+        const int q = it % 9, ef = it / 9, i = q % 3, j = q / 3;
+        double s = 0.0;
+        for (int tt = 0; tt < NT * NT; ++tt) s += TS[((j * NT * NT + tt) * 3 + i) * nef + ef];
+        acc[L.kic + q + 9 * ef] += wk * s;
+      }
+    if (doNjob)
+      for (int it = t; it < 9 * nval; it += BS_THREADS) {
+        const int q = it % 9, n0 = it / 9;
+        double s = 0.0;
+        for (int tr = 0; tr < NT; ++tr) s += QF[(tr * 9 + q) * NMAX + n0];
+        acc[L.nic + q + 9 * n0] += wk * s;
+      }
     // ---- Chern-Simons term (needs A and Omega) ------------------------------
     if ((doK || doNjob) && b.pos_op && A && Om) {
       __syncthreads();

@@ -540,6 +540,7 @@ static int band_setup(mep_ctx* c, const mep_run_params* p, int64_t kblock, BandS
   b.eta = p->eta;
   b.kubo_tol = c->kubo_tol;
   b.pos_op = c->pos ? 1 : 0;
+  b.v_herm = c->pos ? 0 : 1;  // the library builds V itself
   std::vector<double> ef, hw;
   linspace(p->eF_min, p->eF_max, p->n_eF, ef);
   linspace(p->hw_min, p->hw_max, p->n_hw, hw);
@@ -724,6 +725,7 @@ extern "C" int mep_bandsum_batch(mep_ctx* c, int64_t nk, const double* e_k, cons
   CHK(ensure_ws(c, nk > 0 ? nk : 1));
   BandSetup bs;
   CHK(band_setup(c, p, c->kcap, bs));
+  bs.b.v_herm = 0;  // V comes from the caller: no symmetry assumed
   const int64_t n = c->n, n2 = n * n, nef = p->n_eF;
   if (fd_distrib && nef * n * c->kcap > c->fd_cap) {
     hipFree(c->dfd);

@@ -73,6 +73,7 @@ struct BandArgs {
   const double* hw;  // (n_hw) device
   const double* fd_in;  // (n_ef, n, nk) device occupations, or null (fd_stat)
   int pos_op;
+  int v_herm;  // V(n, m) = conj(V(m, n)) may be assumed (V built here, no position operator)
 };
 // layout of one accumulator record (doubles)
 struct AccLayout {
