@@ -82,39 +82,167 @@ __device__ inline zd load_tb(const HT* X, int packed, int N, int row, int col, i
 }
 
 // --------------------------------------------------------------------------
-// no position operator: V_a = U^H dH_a U (6 complex n x n products per k on
-// FP64 MFMA, 4 waves), or V_a = dH_a without the gauge transformation.
-// V is written planar: V[k][a][i + n*j].
+// no position operator: V_a = U^H dH_a U for the three directions, or
+// V_a = dH_a without the gauge transformation.  V is written planar:
+// V[k][a][i + n*j].
+//
+// One k-point per 256-thread workgroup; the waves own the 2 x 2 output
+// tiles of 16 x 16.  U and the three dH_a are staged in LDS (dH_a is
+// Hermitian: its packed upper triangle is read once, coalesced, and both
+// halves are written).  T_a = dH_a U is formed in registers for all three a
+// (nine MFMA chains: three real products per complex one), written over
+// dH_a, and V_a = U^H T_a goes from the accumulators straight to HBM.
+// LDS rows and columns >= n stay zero, so no operand load is guarded.
 // --------------------------------------------------------------------------
 template <class HT>
-__global__ __launch_bounds__(256) void velo_mfma_kernel(int n, int64_t nk, const HT* __restrict__ Hk,
-                                                        int64_t strideHk, int packed, TBPhase tb0,
-                                                        int gauge_trafo, const zd* __restrict__ U,
-                                                        zd* __restrict__ V) {
-  constexpr int LD = 33;
-  __shared__ zd Us[32 * LD], Ds[32 * LD], Ts[32 * LD];
-  const int t = threadIdx.x;
+__global__ __launch_bounds__(256, 2) void velo_mfma_kernel(int n, int64_t nk,
+                                                           const HT* __restrict__ Hk,
+                                                           int64_t strideHk, int packed,
+                                                           TBPhase tb0, int gauge_trafo,
+                                                           const zd* __restrict__ U,
+                                                           zd* __restrict__ V) {
+  constexpr int NM = 32, LD = NM + 1, PL = NM * LD;
+  __shared__ zd Us[PL], Ds[3 * PL];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int n2 = n * n;
+  const int np = packed ? n * (n + 1) / 2 : n2;
+  for (int c = t; c < PL; c += 256) Us[c] = zd{0.0, 0.0};
+  for (int c = t; c < 3 * PL; c += 256) Ds[c] = zd{0.0, 0.0};
+  const int r0 = 16 * (w & 1), c0 = 16 * (w >> 1);
+  const int ar = r0 + (lane & 15), bc = c0 + (lane & 15), kq = lane >> 4;
+  const bool tile = r0 < n && c0 < n;  // wave-uniform
   for (int64_t k = blockIdx.x; k < nk; k += gridDim.x) {
     const TBPhase tb = tb0.at(k);
     const HT* X = Hk + k * strideHk;
     zd* Vk = V + k * 3 * (int64_t)n2;
-    if (gauge_trafo)
-      for (int idx = t; idx < n2; idx += 256) Us[idx % n + LD * (idx / n)] = U[k * (int64_t)n2 + idx];
-    for (int a = 0; a < 3; ++a) {
-      for (int idx = t; idx < n2; idx += 256) {
-        int i = idx % n, j = idx / n;
-        Ds[i + LD * j] = load_tb(X, packed, n, i, j, 1 + a, tb);
+    __syncthreads();  // the previous k-point's readers are done
+    // ---- stage: every global load of the k-point before the LDS stores
+    {
+      constexpr int NU = (NM * NM + 255) / 256, ND = (NM * (NM + 1) / 2 + 255) / 256;
+      zd ut[NU], dt[3][ND], ht[ND];
+#pragma unroll
+      for (int u = 0; u < NU; ++u) {
+        const int idx = t + 256 * u;
+        ut[u] = (gauge_trafo && idx < n2) ? U[k * (int64_t)n2 + idx] : zd{0.0, 0.0};
       }
-      __syncthreads();
-      if (gauge_trafo) {
-        zgemm_tiles<false>(n, Ds, LD, BPlain{Us, LD}, Ts, LD);  // T = dH U
-        __syncthreads();
-        zgemm_tiles<true>(n, Us, LD, BPlain{Ts, LD}, Ds, LD);   // V_a = U^H T
-        __syncthreads();
+      // element p of the packed upper triangle: (i, j), i <= j, p = i + j(j+1)/2
+      int pi[ND], pj[ND];
+#pragma unroll
+      for (int u = 0; u < ND; ++u) {
+        const int p = t + 256 * u;
+        int j = (int)((sqrtf(8.0f * (float)p + 1.0f) - 1.0f) * 0.5f);
+        if ((j + 1) * (j + 2) / 2 <= p) ++j;
+        if (j * (j + 1) / 2 > p) --j;
+        pj[u] = j;
+        pi[u] = p - j * (j + 1) / 2;
       }
-      for (int idx = t; idx < n2; idx += 256) Vk[a * (int64_t)n2 + idx] = Ds[idx % n + LD * (idx / n)];
-      __syncthreads();
+      if (packed) {
+        const int64_t NPk = np;
+#pragma unroll
+        for (int u = 0; u < ND; ++u) {
+          const int p = t + 256 * u;
+          const bool ok = p < np;
+#pragma unroll
+          for (int a = 0; a < 3; ++a) dt[a][u] = ok ? up64(X[(1 + a) * NPk + p]) : zd{0.0, 0.0};
+          ht[u] = (ok && tb.tau) ? up64(X[p]) : zd{0.0, 0.0};
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < NU; ++u) {
+        const int idx = t + 256 * u;
+        if (idx < n2) Us[idx % n + LD * (idx / n)] = ut[u];
+      }
+      if (packed) {
+#pragma unroll
+        for (int u = 0; u < ND; ++u) {
+          const int p = t + 256 * u;
+          if (p >= np) continue;
+          const int i = pi[u], j = pj[u];
+          zd ph = zd{1.0, 0.0};
+          double dtau[3] = {0.0, 0.0, 0.0};
+          if (tb.tau) ph = tb.phase(i, j, dtau);
+#pragma unroll
+          for (int a = 0; a < 3; ++a) {
+            // dH'_a(i, j) = e^{2 pi i k.(tau_j - tau_i)} (dH_a + i c_a(tau_j - tau_i) H)
+            zd x = dt[a][u];
+            if (tb.tau) x = zmul(ph, zadd(x, zmuli(zscale(ht[u], tb.c(a, dtau)))));
+            Ds[a * PL + j + LD * i] = zconj(x);
+            Ds[a * PL + i + LD * j] = x;  // last: the diagonal keeps x as stored
+          }
+        }
+      } else {
+        for (int a = 0; a < 3; ++a)
+          for (int idx = t; idx < n2; idx += 256) {
+            const int i = idx % n, j = idx / n;
+            Ds[a * PL + i + LD * j] = load_tb(X, 0, n, i, j, 1 + a, tb);
+          }
+      }
+    }
+    __syncthreads();
+    if (!gauge_trafo) {
+      for (int a = 0; a < 3; ++a)
+        for (int idx = t; idx < n2; idx += 256)
+          Vk[a * (int64_t)n2 + idx] = Ds[a * PL + idx % n + LD * (idx / n)];
+      continue;
+    }
+    // ---- T_a = dH_a U (C = sum_m A(., m) B(m, .): P1 = Ar Br, P2 = Ai Bi,
+    // P3 = (Ar + Ai)(Br + Bi); Re C = P1 - P2, Im C = P3 - P1 - P2)
+    mep_v4d p1[3], p2[3], p3[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) p1[a] = p2[a] = p3[a] = mep_v4d{0, 0, 0, 0};
+    if (tile)
+#pragma unroll
+      for (int k0 = 0; k0 < NM; k0 += 4) {
+        if (k0 >= n) break;
+        const int m = k0 + kq;
+        const zd b = Us[m + LD * bc];
+        const double bs = b.re + b.im;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          const zd x = Ds[a * PL + ar + LD * m];
+          p1[a] = __builtin_amdgcn_mfma_f64_16x16x4f64(x.re, b.re, p1[a], 0, 0, 0);
+          p2[a] = __builtin_amdgcn_mfma_f64_16x16x4f64(x.im, b.im, p2[a], 0, 0, 0);
+          p3[a] = __builtin_amdgcn_mfma_f64_16x16x4f64(x.re + x.im, bs, p3[a], 0, 0, 0);
+        }
+      }
+    __syncthreads();  // every wave has read dH
+    if (tile)
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int row = r0 + kq + 4 * v;
+          Ds[a * PL + row + LD * bc] = zd{p1[a][v] - p2[a][v], p3[a][v] - p1[a][v] - p2[a][v]};
+        }
+    __syncthreads();
+    // ---- V_a = U^H T_a, A(row, m) = conj(U(m, row))
+#pragma unroll
+    for (int a = 0; a < 3; ++a) p1[a] = p2[a] = p3[a] = mep_v4d{0, 0, 0, 0};
+    if (tile) {
+#pragma unroll
+      for (int k0 = 0; k0 < NM; k0 += 4) {
+        if (k0 >= n) break;
+        const int m = k0 + kq;
+        const zd u = Us[m + LD * ar];
+        const double us = u.re - u.im;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          const zd y = Ds[a * PL + m + LD * bc];
+          p1[a] = __builtin_amdgcn_mfma_f64_16x16x4f64(u.re, y.re, p1[a], 0, 0, 0);
+          p2[a] = __builtin_amdgcn_mfma_f64_16x16x4f64(-u.im, y.im, p2[a], 0, 0, 0);
+          p3[a] = __builtin_amdgcn_mfma_f64_16x16x4f64(us, y.re + y.im, p3[a], 0, 0, 0);
+        }
+      }
+      if (bc < n)
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int row = r0 + kq + 4 * v;
+            if (row < n)
+              Vk[a * (int64_t)n2 + row + (int64_t)n * bc] =
+                  zd{p1[a][v] - p2[a][v], p3[a][v] - p1[a][v] - p2[a][v]};
+          }
     }
   }
 }
