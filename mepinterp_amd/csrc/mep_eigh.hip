@@ -557,10 +557,18 @@ __global__ __launch_bounds__(64) void eigh_vec_kernel(int n, int64_t nk,
       // (Dhillon and Parlett's twisted factorizations); pivots below
       // eps |T| are clamped to +-eps |T|.
       const R lam = (t < n) ? (R)E[k * n + t] : R(0);
+      // d and e once, every load issued up front (indices clamped to n - 1)
+      R dl[NMAX], el[NMAX];
+#pragma unroll
+      for (int i = 0; i < NMAX; ++i) {
+        const int ic = i < n ? i : n - 1;
+        dl[i] = sd[ic];
+        el[i] = sd[n + ic];
+      }
       R tn = R(0);
 #pragma unroll
       for (int i = 0; i < NMAX; ++i)
-        if (i < n) tn = fmax(tn, fabs(sd[i]) + R(2) * fabs(sd[n + i]));
+        if (i < n) tn = fmax(tn, fabs(dl[i]) + R(2) * fabs(el[i]));
       const R pivmin = cplx_of<R>::eps * (tn > R(0) ? tn : R(1));
       R um[NMAX], dp[NMAX];
       R u = R(1);
@@ -568,8 +576,8 @@ __global__ __launch_bounds__(64) void eigh_vec_kernel(int n, int64_t nk,
       for (int i = NMAX - 1; i >= 0; --i) {
         R v = R(1);
         if (i < n) {
-          const R ei = (i < n - 1) ? sd[n + i] : R(0);
-          v = sd[i] - lam - ((i < n - 1) ? ei * ei / u : R(0));
+          const R ei = (i < n - 1) ? el[i] : R(0);
+          v = dl[i] - lam - ((i < n - 1) ? ei * ei / u : R(0));
           if (fabs(v) < pivmin) v = (v < R(0)) ? -pivmin : pivmin;
           u = v;
         }
@@ -580,8 +588,8 @@ __global__ __launch_bounds__(64) void eigh_vec_kernel(int n, int64_t nk,
       for (int i = 0; i < NMAX; ++i) {
         R v = R(1);
         if (i < n) {
-          const R em = (i > 0) ? sd[n + i - 1] : R(0);
-          v = sd[i] - lam - ((i > 0) ? em * em / dd : R(0));
+          const R em = (i > 0) ? el[i - 1] : R(0);
+          v = dl[i] - lam - ((i > 0) ? em * em / dd : R(0));
           if (fabs(v) < pivmin) v = (v < R(0)) ? -pivmin : pivmin;
           dd = v;
         }
@@ -592,7 +600,7 @@ __global__ __launch_bounds__(64) void eigh_vec_kernel(int n, int64_t nk,
 #pragma unroll
       for (int i = 0; i < NMAX; ++i)
         if (i < n) {
-          const R g = fabs(dp[i] + um[i] - (sd[i] - lam));
+          const R g = fabs(dp[i] + um[i] - (dl[i] - lam));
           if (i == 0 || g < gmin) {
             gmin = g;
             r = i;
@@ -602,14 +610,14 @@ __global__ __launch_bounds__(64) void eigh_vec_kernel(int n, int64_t nk,
 #pragma unroll
       for (int i = NMAX - 1; i >= 0; --i)
         if (i < r) {
-          zp = -(sd[n + i] / dp[i]) * zp;
+          zp = -(el[i] / dp[i]) * zp;
           dp[i] = zp;
         }
       zp = R(1);
 #pragma unroll
       for (int i = 0; i < NMAX; ++i)
         if (i > r && i < n) {
-          zp = -(sd[n + i - 1] / um[i]) * zp;
+          zp = -(el[i - 1] / um[i]) * zp;
           um[i] = zp;
         }
       R nz = R(0);
@@ -629,10 +637,14 @@ __global__ __launch_bounds__(64) void eigh_vec_kernel(int n, int64_t nk,
     // ---- lane j <-> column j of P Z ----------------------------------------
     Z W[NMAX];
 #pragma unroll
-    for (int i = 0; i < NMAX; ++i)
-      W[i] = (valid && i < n && t < n)
-                 ? zscale(Z{sd[3 * n + 2 * i], sd[3 * n + 2 * i + 1]}, Zt[t * LDZ + i])
-                 : Z{R(0), R(0)};
+    for (int i = 0; i < NMAX; ++i) {
+      const int ic = i < n ? i : n - 1;
+      const Z ph = Z{sd[3 * n + 2 * ic], sd[3 * n + 2 * ic + 1]};
+      W[i] = (valid && i < n && t < n) ? zscale(ph, Zt[t * LDZ + i]) : Z{R(0), R(0)};
+    }
+    // tau of the reflectors into the (now free) rotation staging area
+    R* taus = csb;
+    if (t < n) taus[t] = sd[2 * n + t];
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -640,16 +652,29 @@ __global__ __launch_bounds__(64) void eigh_vec_kernel(int n, int64_t nk,
     // beyond n) into the Z space, so the updates below run branch-free ----
     Z* Vf = reinterpret_cast<Z*>(Zt);
     const Z* Uk = U + (valid ? k : 0) * (int64_t)n * n;
-    if (valid)
-      for (int kc = 0; kc + 2 < n; ++kc)
-        Vf[kc * NMAX + t] = (t > kc && t < n) ? Uk[t + (int64_t)n * kc] : Z{R(0), R(0)};
+    if (valid) {
+      const int tc = t < n ? t : n - 1;
+      for (int k0 = 0; k0 + 2 < n; k0 += 8) {
+        Z vb[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int kc = k0 + u;
+          vb[u] = Uk[tc + (int64_t)n * (kc < n ? kc : n - 1)];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int kc = k0 + u;
+          if (kc + 2 < n) Vf[kc * NMAX + t] = (t > kc && t < n) ? vb[u] : Z{R(0), R(0)};
+        }
+      }
+    }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     if (valid) {
       for (int kc = n - 3; kc >= 0; --kc) {
         const Z* v = Vf + kc * NMAX;
-        const R tk = sd[2 * n + kc];
+        const R tk = taus[kc];
         // four interleaved partial sums of v^H W
         Z s4[4] = {Z{R(0), R(0)}, Z{R(0), R(0)}, Z{R(0), R(0)}, Z{R(0), R(0)}};
 #pragma unroll
