@@ -24,15 +24,11 @@
 //   largest-modulus component is made real and positive.
 #include "mep_common.hpp"
 #include "mep_kernels.hpp"
+#include "mep_lane.hpp"
 
 // NMAX lanes form a team that owns one k-point; a wavefront runs 64/NMAX
-// teams side by side.  Team reductions use width-NMAX shuffles.
-template <int W, class T>
-__device__ inline T team_sum(T v) {
-#pragma unroll
-  for (int o = W / 2; o >= 1; o >>= 1) v += __shfl_xor(v, o, W);
-  return v;
-}
+// teams side by side.  Team reductions and broadcasts run on the VALU
+// (team_sum, team_bcast of mep_lane.hpp).
 
 // side record per k-point (doubles): d[n] e[n] tau[n] ph[2n] count
 __host__ __device__ inline int64_t side_len(int n) { return 5 * (int64_t)n + 1; }
@@ -92,7 +88,7 @@ __global__ __launch_bounds__(64) void eigh_house_kernel(int n, int64_t nk,
       const int m = n - kc - 1;
       const Z x = (t < m) ? a[(kc + 1 + t) + LDA * kc] : Z{R(0), R(0)};
       const R tail2 = team_sum<NMAX, R>((t >= 1 && t < m) ? zabs2(x) : R(0));
-      const Z x0 = Z{__shfl(x.re, 0, NMAX), __shfl(x.im, 0, NMAX)};
+      const Z x0 = Z{team_bcast<NMAX>(x.re, 0), team_bcast<NMAX>(x.im, 0)};
       const bool refl = tail2 > R(0);
       const R ax02 = zabs2(x0);
       const R xn = sqrt(tail2 + ax02);
@@ -229,8 +225,7 @@ __global__ __launch_bounds__(64) void eigh_house_reg_kernel(int n, int64_t nk,
       const bool below = t > kc;  // rows kc+1 .. n-1 take part
       const Z x = below ? row[kc] : zero;
       const R tail2 = team_sum<NMAX, R>((t > kc + 1) ? zabs2(x) : R(0));
-      const int src = tm * NMAX + kc + 1;
-      const Z x0 = Z{__shfl(x.re, src, 64), __shfl(x.im, src, 64)};
+      const Z x0 = Z{team_bcast<NMAX>(x.re, kc + 1), team_bcast<NMAX>(x.im, kc + 1)};
       const bool refl = tail2 > R(0);
       const R ax02 = zabs2(x0);
       const R xn = sqrt(tail2 + ax02);
@@ -280,8 +275,7 @@ __global__ __launch_bounds__(64) void eigh_house_reg_kernel(int n, int64_t nk,
       if (j == n - 2) last = row[j];
     }
     if (n >= 2) {
-      const int src = tm * NMAX + n - 1;
-      const Z l2 = Z{__shfl(last.re, src, 64), __shfl(last.im, src, 64)};
+      const Z l2 = Z{team_bcast<NMAX>(last.re, n - 1), team_bcast<NMAX>(last.im, n - 1)};
       if (t == 0) {
         sub[n - 2] = l2;
         tauv[n - 2] = R(0);
