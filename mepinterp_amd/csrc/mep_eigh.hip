@@ -96,7 +96,8 @@ __global__ __launch_bounds__(64) void eigh_house_kernel(int n, int64_t nk,
       const Z alpha = !refl ? x0 : ((ax0 > R(0)) ? zscale(x0, -xn / ax0) : Z{-xn, R(0)});
       Z v = refl ? x : Z{R(0), R(0)};
       if (t == 0 && refl) v = zsub(x, alpha);
-      const R vn2 = team_sum<NMAX, R>((t < m) ? zabs2(v) : R(0));
+      // |v|^2 = |x|^2 - 2 Re(conj(alpha) x0) + |alpha|^2 = 2 xn (xn + |x0|)
+      const R vn2 = R(2) * xn * (xn + ax0);
       const R tk = refl ? R(2) / vn2 : R(0);
       vv[t] = (t < m) ? v : Z{R(0), R(0)};
       __syncthreads();
@@ -233,7 +234,8 @@ __global__ __launch_bounds__(64) void eigh_house_reg_kernel(int n, int64_t nk,
       const Z alpha = !refl ? x0 : ((ax0 > R(0)) ? zscale(x0, -xn / ax0) : Z{-xn, R(0)});
       Z v = (refl && below) ? x : zero;
       if (t == kc + 1 && refl) v = zsub(x, alpha);
-      const R vn2 = team_sum<NMAX, R>(zabs2(v));
+      // |v|^2 = |x|^2 - 2 Re(conj(alpha) x0) + |alpha|^2 = 2 xn (xn + |x0|)
+      const R vn2 = R(2) * xn * (xn + ax0);
       const R tk = refl ? R(2) / vn2 : R(0);
       vv[t] = v;
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -241,7 +243,11 @@ __global__ __launch_bounds__(64) void eigh_house_reg_kernel(int n, int64_t nk,
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       Z p = zero;
 #pragma unroll
-      for (int j = kc + 1; j < NMAX; ++j) p = zadd(p, zmul(row[j], vv[j]));
+      for (int j = kc + 1; j < NMAX; ++j) {
+        const Z a_ = row[j], b_ = vv[j];
+        p.re = fma(a_.re, b_.re, fma(-a_.im, b_.im, p.re));
+        p.im = fma(a_.re, b_.im, fma(a_.im, b_.re, p.im));
+      }
       p = below ? zscale(p, tk) : zero;
       const R K = R(0.5) * tk * team_sum<NMAX, R>(v.re * p.re + v.im * p.im);
       const Z w = zsub(p, zscale(v, K));
@@ -252,8 +258,12 @@ __global__ __launch_bounds__(64) void eigh_house_reg_kernel(int n, int64_t nk,
       if (refl && below) {
 #pragma unroll
         for (int j = kc + 1; j < NMAX; ++j) {
+          // row_j -= v conj(w_j) + w conj(v_j)
           const Z vj = vv[j], wj = ww[j];
-          row[j] = zsub(row[j], zadd(zmul(v, zconj(wj)), zmul(w, zconj(vj))));
+          Z r_ = row[j];
+          r_.re = fma(-v.re, wj.re, fma(-v.im, wj.im, fma(-w.re, vj.re, fma(-w.im, vj.im, r_.re))));
+          r_.im = fma(-v.im, wj.re, fma(v.re, wj.im, fma(-w.im, vj.re, fma(w.re, vj.im, r_.im))));
+          row[j] = r_;
         }
       }
       if (below) row[kc] = v;
