@@ -83,6 +83,8 @@ __global__ __launch_bounds__(256) void refine_kernel(int n, int64_t nk, const zf
   const int t = threadIdx.x;
   const int n2 = n * n;
   const double two_pi = 2.0 * 3.14159265358979323846;
+  // rows and columns >= n of every matrix stay zero (the products read them)
+  for (int c = t; c < NMAX * LD; c += 256) Hs[c] = Us[c] = Ts[c] = Ys[c] = zd{0.0, 0.0};
   for (int64_t k = blockIdx.x; k < nk; k += gridDim.x) {
     const zf* Hb = Hk + k * strideHk;
     // H(k) in FP64 (Hermitian, tight-binding phase) into Hs
@@ -118,10 +120,10 @@ __global__ __launch_bounds__(256) void refine_kernel(int n, int64_t nk, const zf
     //   F_ij = (S_ij - lambda_j G_ij) / (lambda_j - lambda_i) for separated
     //   pairs, F_ij = -G_ij / 2 inside a group and on the diagonal
     //   (F_ii = (1 - G_ii) / 2);  X <- X (I + F)
-    zgemm_tiles<false>(n, Hs, LD, BPlain{Us, LD}, Ts, LD);  // T = H X
-    zgemm_tiles<true>(n, Us, LD, BPlain{Us, LD}, Ys, LD);   // G = X^H X
+    zgemm_tiles<NMAX, false>(n, Hs, LD, BPlain{Us, LD}, Ts, LD);  // T = H X
+    zgemm_tiles<NMAX, true>(n, Us, LD, BPlain{Us, LD}, Ys, LD);   // G = X^H X
     __syncthreads();
-    zgemm_tiles<true>(n, Us, LD, BPlain{Ts, LD}, Hs, LD);   // S = X^H T (H is reloaded later)
+    zgemm_tiles<NMAX, true>(n, Us, LD, BPlain{Ts, LD}, Hs, LD);   // S = X^H T (H is reloaded later)
     __syncthreads();
     if (t < n) Er[t] = Hs[t + LD * t].re / Ys[t + LD * t].re;
     __syncthreads();
@@ -144,7 +146,7 @@ __global__ __launch_bounds__(256) void refine_kernel(int n, int64_t nk, const zf
       Ts[i + LD * j] = f;
     }
     __syncthreads();
-    zgemm_tiles<false>(n, Us, LD, BPlain{Ts, LD}, Ys, LD);  // X F
+    zgemm_tiles<NMAX, false>(n, Us, LD, BPlain{Ts, LD}, Ys, LD);  // X F
     __syncthreads();
     for (int idx = t; idx < n2; idx += 256) {
       const int o = idx % n + LD * (idx / n);
@@ -153,7 +155,7 @@ __global__ __launch_bounds__(256) void refine_kernel(int n, int64_t nk, const zf
     load_h();
     __syncthreads();
     // ---- Rayleigh quotients of the refined vectors -------------------------
-    zgemm_tiles<false>(n, Hs, LD, BPlain{Us, LD}, Ts, LD);  // T = H X
+    zgemm_tiles<NMAX, false>(n, Hs, LD, BPlain{Us, LD}, Ts, LD);  // T = H X
     __syncthreads();
     if (t < n) {
       double num = 0.0, den = 0.0;
