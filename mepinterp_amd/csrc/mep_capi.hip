@@ -64,6 +64,17 @@ struct mep_ctx {
   double* dtau = nullptr;
   double* dX = nullptr;  // (K, 2 ncols) real coefficients
   float* dX32 = nullptr;  // the same in FP32 (mixed-precision path)
+  // separable sums on Monkhorst-Pack meshes: the coefficients in a box of
+  // integer R (origin sox, soy, soz; extent sbx x sby x sbz), the phase
+  // tables of the current mesh and the z-stage workspace
+  bool sep = false;
+  int sox = 0, soy = 0, soz = 0, sbx = 0, sby = 0, sbz = 0;
+  zd* dXb = nullptr;
+  zd* dPh = nullptr;
+  int64_t ph_cap = 0;
+  int sep_mesh[3] = {0, 0, 0};
+  zd* dY1 = nullptr;
+  int64_t y1_cap = 0;
   // k-block workspace
   int64_t kcap = 0, kblock_req = 0;
   double* dk = nullptr;
@@ -357,6 +368,47 @@ extern "C" int mep_ctx_create(int64_t n_wf, int64_t n_R, const double* R_frac,
   }
   if ((e = hipMalloc(&c->dinfo, sizeof(int))) != hipSuccess) return bad(e, "hipMalloc info");
   if ((e = hipMemset(c->dinfo, 0, sizeof(int))) != hipSuccess) return bad(e, "memset info");
+  // coefficients in a box of integer R for the separable mesh sums, when
+  // the box is compact (no more than 4 NR + 64 points, at most 16 along x)
+  {
+    bool integral = n_R > 0;
+    double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    for (int64_t s2 = 0; s2 < n_R && integral; ++s2)
+      for (int a = 0; a < 3; ++a) {
+        const double v = R_frac[3 * s2 + a];
+        if (v != std::floor(v) || std::fabs(v) > 1.0e6) integral = false;
+        if (s2 == 0 || v < lo[a]) lo[a] = v;
+        if (s2 == 0 || v > hi[a]) hi[a] = v;
+      }
+    if (integral) {
+      const int bx = (int)(hi[0] - lo[0]) + 1, by = (int)(hi[1] - lo[1]) + 1,
+                bz = (int)(hi[2] - lo[2]) + 1;
+      const int64_t box = (int64_t)bx * by * bz;
+      if (bx <= 16 && box <= 4 * n_R + 64) {
+        std::vector<zd> Xb((size_t)(box * NC), zd{0.0, 0.0});
+        for (int64_t s2 = 0; s2 < n_R; ++s2) {
+          std::fill(xs.begin(), xs.end(), zd{0.0, 0.0});
+          build_x(s2, xs);
+          const int ix = (int)(R_frac[3 * s2] - lo[0]), iy = (int)(R_frac[3 * s2 + 1] - lo[1]),
+                    iz = (int)(R_frac[3 * s2 + 2] - lo[2]);
+          zd* dst = &Xb[(size_t)((((int64_t)ix * by + iy) * bz + iz) * NC)];
+          for (int64_t q = 0; q < NC; ++q) dst[q] = zadd(dst[q], xs[q]);
+        }
+        if ((e = hipMalloc(&c->dXb, sizeof(zd) * Xb.size())) != hipSuccess)
+          return bad(e, "hipMalloc box coefficients");
+        if ((e = hipMemcpy(c->dXb, Xb.data(), sizeof(zd) * Xb.size(), hipMemcpyHostToDevice)) !=
+            hipSuccess)
+          return bad(e, "upload box coefficients");
+        c->sep = true;
+        c->sox = (int)lo[0];
+        c->soy = (int)lo[1];
+        c->soz = (int)lo[2];
+        c->sbx = bx;
+        c->sby = by;
+        c->sbz = bz;
+      }
+    }
+  }
   *out = c;
   return MEP_OK;
 }
@@ -367,6 +419,7 @@ extern "C" int mep_ctx_destroy(mep_ctx* c) {
   resolve_times(c);
   free_ws(c);
   hipFree(c->dR); hipFree(c->dtau); hipFree(c->dX); hipFree(c->dX32); hipFree(c->dinfo);
+  hipFree(c->dXb); hipFree(c->dPh); hipFree(c->dY1);
   hipFree(c->dpart); hipFree(c->dacc); hipFree(c->def); hipFree(c->dhw); hipFree(c->dfd);
   hipFree(c->dcurv);
   for (auto e : c->pool) hipEventDestroy(e);
@@ -433,7 +486,78 @@ static KGeom geom(const mep_ctx* c) {
 // near-degenerate threshold of the mixed-precision refinement: 1e-3 eV
 static const double kMixedGroupHartree = 1e-3 / 27.211385;
 
-static int interp_block(mep_ctx* c, int64_t nb, bool velo, int prec = MEP_PREC_FP64) {
+// phase tables of the mesh mp and the z-stage workspace for the context's
+// k-blocks (separable sums); no-op without a box
+static int sep_prepare(mep_ctx* c, const int32_t* mp) {
+  if (!c->sep) return MEP_OK;
+  const int BX = c->sbx <= 8 ? 8 : 16;
+  if (mp[0] != c->sep_mesh[0] || mp[1] != c->sep_mesh[1] || mp[2] != c->sep_mesh[2]) {
+    const int64_t len = (int64_t)mp[0] * BX + (int64_t)mp[1] * c->sby + (int64_t)mp[2] * c->sbz;
+    std::vector<zd> ph((size_t)len, zd{0.0, 0.0});
+    const double two_pi = 2.0 * 3.14159265358979323846;
+    // k_i = (2 (q + 1) - N - 1) / (2 N), as mp_kpts_kernel
+    auto fill = [&](zd* dst, int nq, int stride, int nb, int origin) {
+      for (int q = 0; q < nq; ++q) {
+        const double kq = (2.0 * (double)(q + 1) - (double)nq - 1.0) / (2.0 * (double)nq);
+        for (int i = 0; i < nb; ++i) {
+          const double ang = two_pi * (kq * (double)(origin + i));
+          dst[(size_t)q * stride + i] = zd{std::cos(ang), std::sin(ang)};
+        }
+      }
+    };
+    fill(ph.data(), mp[0], BX, c->sbx, c->sox);
+    fill(ph.data() + (size_t)mp[0] * BX, mp[1], c->sby, c->sby, c->soy);
+    fill(ph.data() + (size_t)mp[0] * BX + (size_t)mp[1] * c->sby, mp[2], c->sbz, c->sbz, c->soz);
+    HIPCHK(hipStreamSynchronize(c->st));  // the previous mesh's tables may still be read
+    if (len > c->ph_cap) {
+      hipFree(c->dPh);
+      c->dPh = nullptr;
+      c->ph_cap = 0;
+      HIPCHK(hipMalloc(&c->dPh, sizeof(zd) * len));
+      c->ph_cap = len;
+    }
+    HIPCHK(hipMemcpy(c->dPh, ph.data(), sizeof(zd) * len, hipMemcpyHostToDevice));
+    for (int a = 0; a < 3; ++a) c->sep_mesh[a] = mp[a];
+  }
+  const int64_t plane = (int64_t)mp[0] * mp[1];
+  int64_t nz = (c->kcap + plane - 1) / plane + 1;
+  if (nz > mp[2]) nz = mp[2];
+  const int64_t need = nz * c->sbx * c->sby * c->ncols;
+  if (need > c->y1_cap) {
+    HIPCHK(hipStreamSynchronize(c->st));
+    hipFree(c->dY1);
+    c->dY1 = nullptr;
+    c->y1_cap = 0;
+    HIPCHK(hipMalloc(&c->dY1, sizeof(zd) * need));
+    c->y1_cap = need;
+  }
+  return MEP_OK;
+}
+
+static SepFT sep_args(mep_ctx* c) {
+  const int BX = c->sbx <= 8 ? 8 : 16;
+  SepFT f;
+  f.NC = c->ncols;
+  f.bx = c->sbx;
+  f.by = c->sby;
+  f.bz = c->sbz;
+  f.n1 = c->sep_mesh[0];
+  f.n2 = c->sep_mesh[1];
+  f.n3 = c->sep_mesh[2];
+  f.Xb = c->dXb;
+  f.ex = c->dPh;
+  f.ey = c->dPh + (int64_t)f.n1 * BX;
+  f.ez = f.ey + (int64_t)f.n2 * f.by;
+  f.Y1 = c->dY1;
+  return f;
+}
+
+// mp (optional): the k-block is k0 .. k0+nb-1 of that Monkhorst-Pack mesh,
+// prepared by sep_prepare -> separable Fourier sums instead of the GEMM
+static int interp_block(mep_ctx* c, int64_t nb, bool velo, int prec = MEP_PREC_FP64,
+                        const int32_t* mp = nullptr, int64_t k0 = 0) {
+  const bool sep = mp && c->sep && mp[0] == c->sep_mesh[0] && mp[1] == c->sep_mesh[1] &&
+                   mp[2] == c->sep_mesh[2];
   const int64_t n = c->n;
   KGeom g = geom(c);
   VeloArgs va;
@@ -451,11 +575,16 @@ static int interp_block(mep_ctx* c, int64_t nb, bool velo, int prec = MEP_PREC_F
     float* P32 = reinterpret_cast<float*>(c->dP);
     zf* H32 = reinterpret_cast<zf*>(c->dHk);
     zf* U32 = reinterpret_cast<zf*>(c->dV);
-    timed(c, K_PHASE, [&] { launch_phase_f32(c->st, nb, c->Kh, c->has_zero, c->dk, c->dR, P32); });
-    timed(c, K_GEMM, [&] {
-      launch_ft_gemm_f32(c->st, (int)nb, (int)(2 * c->ncols), (int)c->K, P32, c->dX32,
-                         reinterpret_cast<float*>(H32));
-    });
+    if (sep) {
+      timed(c, K_GEMM, [&] { launch_sep_ft(c->st, sep_args(c), k0, nb, nullptr, H32); });
+    } else {
+      timed(c, K_PHASE,
+            [&] { launch_phase_f32(c->st, nb, c->Kh, c->has_zero, c->dk, c->dR, P32); });
+      timed(c, K_GEMM, [&] {
+        launch_ft_gemm_f32(c->st, (int)nb, (int)(2 * c->ncols), (int)c->K, P32, c->dX32,
+                           reinterpret_cast<float*>(H32));
+      });
+    }
     timed(c, K_EIGH, [&] {
       launch_eigh_f32(c->st, (int)n, nb, H32, c->ncols, c->packed, g, c->dE, U32, c->dinfo,
                       c->dscr);
@@ -469,11 +598,15 @@ static int interp_block(mep_ctx* c, int64_t nb, bool velo, int prec = MEP_PREC_F
     HIPCHK(hipGetLastError());
     return MEP_OK;
   }
-  timed(c, K_PHASE, [&] { launch_phase(c->st, nb, c->Kh, c->has_zero, c->dk, c->dR, c->dP); });
-  timed(c, K_GEMM, [&] {
-    launch_ft_gemm_f64(c->st, (int)nb, (int)(2 * c->ncols), (int)c->K, c->dP, c->dX,
-                       reinterpret_cast<double*>(c->dHk));
-  });
+  if (sep) {
+    timed(c, K_GEMM, [&] { launch_sep_ft(c->st, sep_args(c), k0, nb, c->dHk, nullptr); });
+  } else {
+    timed(c, K_PHASE, [&] { launch_phase(c->st, nb, c->Kh, c->has_zero, c->dk, c->dR, c->dP); });
+    timed(c, K_GEMM, [&] {
+      launch_ft_gemm_f64(c->st, (int)nb, (int)(2 * c->ncols), (int)c->K, c->dP, c->dX,
+                         reinterpret_cast<double*>(c->dHk));
+    });
+  }
   timed(c, K_EIGH, [&] {
     if (n > 32)
       launch_eigh_wide(c->st, (int)n, nb, c->dHk, c->ncols, c->packed, g, c->dE, c->dU, c->dinfo,
@@ -768,7 +901,7 @@ static int run_block(mep_ctx* c, BandSetup& bs, int64_t k0, int64_t nb, const in
   }
   if (weights)
     HIPCHK(hipMemcpyAsync(c->dw, weights + k0, sizeof(double) * nb, hipMemcpyHostToDevice, c->st));
-  CHK(interp_block(c, nb, true, p->precision));
+  CHK(interp_block(c, nb, true, p->precision, mp, k0));
   CHK(band_block(c, bs, nb, weights ? c->dw : nullptr));
   return MEP_OK;
 }
@@ -776,6 +909,7 @@ static int run_block(mep_ctx* c, BandSetup& bs, int64_t k0, int64_t nb, const in
 static int run_common(mep_ctx* c, int64_t nk, const int32_t* mp, const double* kpts,
                       const double* weights, const mep_run_params* p, mep_sums* out) {
   CHK(ensure_ws(c, nk));
+  if (mp) CHK(sep_prepare(c, mp));
   BandSetup bs;
   CHK(band_setup(c, p, c->kcap, bs));
   for (int64_t k0 = 0; k0 < nk; k0 += c->kcap) {
@@ -803,6 +937,7 @@ extern "C" int mep_run_batch(int32_t n_models, mep_ctx* const* ctxs, const int32
       return fail(MEP_ERR_ARG, "mep_run_batch: mp_grid entries must be >= 1");
     nk[i] = (int64_t)mp[0] * mp[1] * mp[2];
     CHK(ensure_ws(c, nk[i]));
+    CHK(sep_prepare(c, mp));
     CHK(band_setup(c, &params[i], c->kcap, bs[i]));
     const int64_t nblk = (nk[i] + c->kcap - 1) / c->kcap;
     if (nblk > rounds) rounds = nblk;

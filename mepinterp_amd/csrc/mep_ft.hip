@@ -304,3 +304,114 @@ void launch_ft_gemm_f32(hipStream_t s, int M, int N, int K, const float* Pt, con
   hipLaunchKernelGGL(ft_gemm_f32_kernel, dim3((unsigned)tiles), dim3(256), 0, s, M, N, K, Pt, X, C,
                      tiles_n, tiles);
 }
+
+// --------------------------------------------------------------------------
+// Separable Fourier sums on a Monkhorst-Pack mesh.
+//
+// On the mesh the phase factorises, e^{2 pi i k.R} = ex(x, Rx) ey(y, Ry)
+// ez(z, Rz), so with the coefficients X(R) scattered into a box
+// Xb[ix][iy][iz][q] (R = origin + (ix, iy, iz); zero where no R)
+//   Y1(z, ix, iy, q) = sum_iz Xb(ix, iy, iz, q) ez(z, iz)
+//   Y2(y, z, ix, q)  = sum_iy Y1(z, ix, iy, q) ey(y, iy)
+//   H(x, y, z, q)    = sum_ix Y2(y, z, ix, q) ex(x, ix)
+// i.e. about bx + bx by / n1 complex products per k and column instead of
+// NR: the stage is bound by writing H(k).  The first sum runs over the
+// z-planes a k-block touches (sep_z_kernel); the other two are fused per
+// mesh line (y, z) (sep_xy_kernel: Y2 in registers, one thread per column,
+// the x-loop writes the line's k-points).  Phase tables, per mesh index:
+// e*[index * BX + i] (ex padded with zeros to BX entries).
+// --------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void sep_z_kernel(int64_t NC, int bx, int by, int bz, int z0,
+                                                    const zd* __restrict__ Xb,
+                                                    const zd* __restrict__ ez,
+                                                    zd* __restrict__ Y1) {
+  const int64_t q = blockIdx.x * (int64_t)256 + threadIdx.x;
+  const int c = blockIdx.y;  // (zi * bx + ix) * by + iy
+  if (q >= NC) return;
+  const int iy = c % by, ix = (c / by) % bx, zi = c / (bx * by);
+  const zd* xs = Xb + (int64_t)(ix * by + iy) * bz * NC + q;
+  const zd* ph = ez + (int64_t)(z0 + zi) * bz;
+  double sr = 0.0, si = 0.0;
+  for (int iz = 0; iz < bz; ++iz) {
+    const zd x = xs[(int64_t)iz * NC], p = ph[iz];
+    sr = fma(x.re, p.re, fma(-x.im, p.im, sr));
+    si = fma(x.re, p.im, fma(x.im, p.re, si));
+  }
+  Y1[(int64_t)c * NC + q] = zd{sr, si};
+}
+
+__device__ inline void store_h(zd* p, double re, double im) { *p = zd{re, im}; }
+__device__ inline void store_h(zf* p, double re, double im) { *p = zf{(float)re, (float)im}; }
+
+template <int BX, class OT>
+__global__ __launch_bounds__(256) void sep_xy_kernel(int64_t NC, int bx, int by, int n1, int n2,
+                                                     int64_t k0, int64_t nk, int64_t line0,
+                                                     int64_t nlines, int z0,
+                                                     const zd* __restrict__ Y1,
+                                                     const zd* __restrict__ ex,
+                                                     const zd* __restrict__ ey,
+                                                     OT* __restrict__ H) {
+  const int64_t q = blockIdx.x * (int64_t)256 + threadIdx.x;
+  if (q >= NC) return;  // no barrier below
+  for (int64_t li = blockIdx.y; li < nlines; li += gridDim.y) {
+    const int64_t line = line0 + li;
+    const int y = (int)(line % n2), zi = (int)(line / n2) - z0;
+    // Y2(ix) = sum_iy Y1(zi, ix, iy, q) ey(y, iy)
+    double yr[BX], yi[BX];
+#pragma unroll
+    for (int ix = 0; ix < BX; ++ix) {
+      yr[ix] = 0.0;
+      yi[ix] = 0.0;
+      if (ix < bx) {
+        const zd* src = Y1 + (int64_t)(zi * bx + ix) * by * NC + q;
+        const zd* ph = ey + (int64_t)y * by;
+        for (int iy = 0; iy < by; ++iy) {
+          const zd v = src[(int64_t)iy * NC], p = ph[iy];
+          yr[ix] = fma(v.re, p.re, fma(-v.im, p.im, yr[ix]));
+          yi[ix] = fma(v.re, p.im, fma(v.im, p.re, yi[ix]));
+        }
+      }
+    }
+    // the line's k-points inside [k0, k0 + nk)
+    const int64_t kb = line * n1;
+    const int xa = (int)(k0 > kb ? k0 - kb : 0);
+    const int xb = (int)(k0 + nk - kb < n1 ? k0 + nk - kb : n1);
+    OT* out = H + (kb - k0) * NC + q;
+    for (int x = xa; x < xb; ++x) {
+      const zd* ph = ex + (int64_t)x * BX;
+      double hr = 0.0, hi = 0.0;
+#pragma unroll
+      for (int ix = 0; ix < BX; ++ix) {
+        const zd p = ph[ix];
+        hr = fma(yr[ix], p.re, fma(-yi[ix], p.im, hr));
+        hi = fma(yr[ix], p.im, fma(yi[ix], p.re, hi));
+      }
+      store_h(out + (int64_t)x * NC, hr, hi);
+    }
+  }
+}
+
+void launch_sep_ft(hipStream_t s, const SepFT& f, int64_t k0, int64_t nk, zd* H64, zf* H32) {
+  const int64_t plane = (int64_t)f.n1 * f.n2;
+  const int z0 = (int)(k0 / plane), z1 = (int)((k0 + nk - 1) / plane);
+  const int nz = z1 - z0 + 1;
+  const unsigned qb = (unsigned)((f.NC + 255) / 256);
+  hipLaunchKernelGGL(sep_z_kernel, dim3(qb, (unsigned)(nz * f.bx * f.by)), dim3(256), 0, s, f.NC,
+                     f.bx, f.by, f.bz, z0, f.Xb, f.ez, f.Y1);
+  const int64_t line0 = k0 / f.n1, line1 = (k0 + nk - 1) / f.n1;
+  const int64_t nlines = line1 - line0 + 1;
+  const unsigned ly = (unsigned)(nlines < 65535 ? nlines : 65535);
+#define MEP_SEP_XY(BX)                                                                          \
+  if (H32)                                                                                      \
+    hipLaunchKernelGGL((sep_xy_kernel<BX, zf>), dim3(qb, ly), dim3(256), 0, s, f.NC, f.bx, f.by, \
+                       f.n1, f.n2, k0, nk, line0, nlines, z0, f.Y1, f.ex, f.ey, H32);           \
+  else                                                                                          \
+    hipLaunchKernelGGL((sep_xy_kernel<BX, zd>), dim3(qb, ly), dim3(256), 0, s, f.NC, f.bx, f.by, \
+                       f.n1, f.n2, k0, nk, line0, nlines, z0, f.Y1, f.ex, f.ey, H64);
+  if (f.bx <= 8) {
+    MEP_SEP_XY(8)
+  } else {
+    MEP_SEP_XY(16)
+  }
+#undef MEP_SEP_XY
+}

@@ -17,6 +17,23 @@ void launch_phase(hipStream_t s, int64_t nk, int Kh, int has_zero, const double*
 void launch_ft_gemm_f64(hipStream_t s, int M, int N, int K, const double* Pt, const double* X,
                         double* C);
 // mixed precision: FP32 phase tile and FP32 GEMM (FP32 MFMA accumulation)
+// Separable Fourier sums on a Monkhorst-Pack mesh (mep_ft.hip): box
+// coefficients Xb [bx][by][bz][NC], per-mesh-index phase tables
+// ex [n1][BX] (zero-padded to BX = 8 or 16), ey [n2][by], ez [n3][bz], and
+// the z-stage workspace Y1 [planes][bx][by][NC] for the planes of a k-block.
+struct SepFT {
+  int64_t NC;
+  int bx, by, bz;
+  int n1, n2, n3;
+  const zd* Xb;
+  const zd* ex;
+  const zd* ey;
+  const zd* ez;
+  zd* Y1;
+};
+// H(k) of the mesh k-points k0 .. k0+nk-1 into H64 (FP64) or, when H32 is
+// not null, H32 (rounded to FP32), in the Fourier GEMM's output layout
+void launch_sep_ft(hipStream_t s, const SepFT& f, int64_t k0, int64_t nk, zd* H64, zf* H32);
 void launch_phase_f32(hipStream_t s, int64_t nk, int Kh, int has_zero, const double* kpts,
                       const double* Rh, float* Pt);
 void launch_ft_gemm_f32(hipStream_t s, int M, int N, int K, const float* Pt, const float* X,
