@@ -440,21 +440,45 @@ __device__ inline void ql_one(int n, int64_t k, R* ds, R* es, R* __restrict__ si
 // mode 0: every k-point records its rotations; mode 1: eigenvalues and the
 // separation verdict only; mode 2: the k-points with verdict -2 record.
 template <int NMAX, class R>
+// mode 1: every k-point, eigenvalues and the separation verdict; the
+// k-points that need rotation records (count slot -2) are appended to
+// list[] (order immaterial: each is solved on its own).  mode 2: the listed
+// k-points again, recording the rotations.
 __global__ __launch_bounds__(64) void eigh_ql_kernel(int n, int64_t nk,
                                                      R* __restrict__ side,
                                                      R* __restrict__ rot,
                                                      uint8_t* __restrict__ rcol,
                                                      double* __restrict__ E,
-                                                     int* __restrict__ info, int mode) {
+                                                     int* __restrict__ info, int mode,
+                                                     int* __restrict__ list,
+                                                     int* __restrict__ count) {
   __shared__ R ds[NMAX * 64], es[NMAX * 64];
   const int lane = threadIdx.x;
+  if (mode == 2) {
+    const int64_t cnt = *count;
+    for (int64_t i = (int64_t)blockIdx.x * 64 + lane; i < cnt; i += (int64_t)gridDim.x * 64)
+      ql_one<64, R>(n, list[i], ds + lane, es + lane, side, rot, rcol, E, info, true);
+    return;
+  }
   for (int64_t kb = (int64_t)blockIdx.x * 64; kb < nk; kb += (int64_t)gridDim.x * 64) {
     const int64_t k = kb + lane;
-    if (k >= nk) continue;
-    if (mode == 2 && side[k * side_len(n) + 5 * n] != R(-2)) continue;
-    ql_one<64, R>(n, k, ds + lane, es + lane, side, rot, rcol, E, info, mode != 1);
+    bool again = false;
+    if (k < nk) {
+      ql_one<64, R>(n, k, ds + lane, es + lane, side, rot, rcol, E, info, false);
+      again = side[k * side_len(n) + 5 * n] == R(-2);
+    }
+    const unsigned long long m = __ballot(again);
+    if (m) {
+      const int leader = __ffsll((long long)m) - 1;
+      int base = 0;
+      if (lane == leader) base = atomicAdd(count, __popcll(m));
+      base = __shfl(base, leader, 64);
+      if (again) list[base + __popcll(m & ((1ull << lane) - 1))] = (int)k;
+    }
   }
 }
+
+__global__ void zero_count_kernel(int* count) { *count = 0; }
 
 // the same for 32 < n <= 128: 32 lanes per workgroup, d and e in dynamic LDS
 // (n x 32 each)
@@ -1060,8 +1084,15 @@ void launch_eigh_wide(hipStream_t s, int n, int64_t nk, const zd* Hk, int64_t st
                      U);
 }
 
+// side records, rotation records, rotation columns, then (256-byte
+// aligned) the list of k-points rerun with records and its count
+static size_t eigh_list_offset(int n, int64_t nk) {
+  const size_t b =
+      (size_t)nk * (sizeof(double) * (size_t)(side_len(n) + 2 * rot_cap(n)) + (size_t)rot_cap(n));
+  return (b + 255) & ~(size_t)255;
+}
 size_t eigh_scratch_bytes(int n, int64_t nk) {
-  return (size_t)nk * (sizeof(double) * (size_t)(side_len(n) + 2 * rot_cap(n)) + (size_t)rot_cap(n));
+  return eigh_list_offset(n, nk) + sizeof(int) * (size_t)nk + 256;
 }
 
 template <class R>
@@ -1071,6 +1102,8 @@ static void launch_eigh_t(hipStream_t s, int n, int64_t nk, const typename cplx_
   R* side = reinterpret_cast<R*>(scratch);
   R* rot = side + nk * side_len(n);
   uint8_t* rcol = reinterpret_cast<uint8_t*>(rot + nk * 2 * rot_cap(n));
+  int* list = reinterpret_cast<int*>(reinterpret_cast<uint8_t*>(scratch) + eigh_list_offset(n, nk));
+  int* count = list + nk;
   const int teams = n <= 8 ? 8 : n <= 16 ? 4 : 2;
   int64_t blocks = (nk + teams - 1) / teams;
   unsigned grid = (unsigned)(blocks < 65536 ? blocks : 65536);
@@ -1083,10 +1116,11 @@ static void launch_eigh_t(hipStream_t s, int n, int64_t nk, const typename cplx_
   else                                                                                           \
     hipLaunchKernelGGL((eigh_house_kernel<NM, R>), dim3(grid), dim3(64), 0, s, n, nk, Hk, strideHk, \
                        packed, g.kpts, g.tau, U, side);                                          \
+  hipLaunchKernelGGL(zero_count_kernel, dim3(1), dim3(1), 0, s, count);                        \
   hipLaunchKernelGGL((eigh_ql_kernel<NM, R>), dim3(qgrid), dim3(64), 0, s, n, nk, side, rot,     \
-                     rcol, E, info, 1);                                                          \
+                     rcol, E, info, 1, list, count);                                             \
   hipLaunchKernelGGL((eigh_ql_kernel<NM, R>), dim3(qgrid), dim3(64), 0, s, n, nk, side, rot,     \
-                     rcol, E, info, 2);                                                          \
+                     rcol, E, info, 2, list, count);                                             \
   hipLaunchKernelGGL((eigh_vec_kernel<NM, R>), dim3(grid), dim3(64), 0, s, n, nk, side, rot,     \
                      rcol, E, U);
   if (n <= 8) {
