@@ -49,8 +49,11 @@ extern "C" {
 #define MEP_JOB_GYRO 16          /* jobs%do_gyro: get_gyro_C / D / Dw         */
 
 /* arithmetic of the k-space pipeline */
-#define MEP_PREC_FP64 0          /* FP64 MFMA Fourier GEMM, FP64 eigensolve  */
-#define MEP_PREC_MIXED 1         /* FP32 MFMA Fourier GEMM and FP32 eigensolve;
+#define MEP_PREC_FP64 0          /* FP64 Fourier sums (separable on meshes,
+                                    MFMA GEMM for k lists), FP64 eigensolve */
+#define MEP_PREC_MIXED 1         /* H(k) in FP32 (FP64 separable sums rounded
+                                    on meshes, FP32 MFMA GEMM for k lists) and
+                                    FP32 eigensolve;
                                     FP64 Rayleigh-quotient refinement of every
                                     eigenvalue and Rayleigh-Ritz of groups
                                     closer than 1e-3 eV; FP64 velocities, band
@@ -138,8 +141,8 @@ int mep_run_kpts(mep_ctx *ctx, int64_t nk, const double *kpts, const double *wei
                  const mep_run_params *p, mep_sums *out);
 
 /* Kernel timing with HIP events on the context's stream (off by default).
-   name: "phase", "ft_gemm", "eigh", "refine", "velo", "bandsum", "reduce",
-   "kgen".                                                                   */
+   name: "phase", "ft_gemm" (the Fourier sums: separable on meshes, GEMM for
+   k lists), "eigh", "refine", "velo", "bandsum", "reduce", "kgen".          */
 /* Several models in one call: mep_run_mesh (the k-loop of core.f90:165-270)
    for each context i on the mesh mp_grids(3,i) with params[i]; the k-blocks
    of all models are enqueued on their contexts' own streams, interleaved

@@ -332,3 +332,29 @@ def test_run_batch_equals_per_model_runs():
         assert b["n_k"] == s["n_k"]
         for key in KEYS:
             np.testing.assert_array_equal(b[key], s[key], err_msg=key)
+
+
+@pytest.mark.parametrize("tb,kblock", [(False, 0), (True, 0), (False, 17), (True, 50)])
+def test_separable_mesh_sums_equal_the_kpoint_gemm(tb, kblock):
+    """mep_run_mesh sums H(k) separably on the mesh; mep_run_kpts over the
+    same k-points takes the Fourier GEMM.  Octahedral R set (not a box),
+    non-cubic mesh, k-blocks that split mesh lines and z-planes."""
+    from mepinterp_amd import capi, models
+    from mepinterp_amd.refine import mp_mesh
+    m = models.make_random(seed=31, n=12, L=2, tb=tb)
+    keep = np.abs(m["R_frac"]).sum(axis=0) <= 3
+    m["R_frac"] = np.asfortranarray(m["R_frac"][:, keep])
+    m["H"] = np.asfortranarray(m["H"][:, :, keep])
+    mp = (5, 7, 3)
+    p = _params(capi.MEP_JOB_AHC | capi.MEP_JOB_KUBO | capi.MEP_JOB_MEP, n_eF=2, eF_min=-0.01,
+                eF_max=0.01, n_hw=3, hw_min=0.0, hw_max=0.2, T_kelvin=300.0,
+                eta=0.1 / AU_TO_EV, valence_bands=5)
+    with _engine(m) as eng:
+        if kblock:
+            eng.set_kblock(kblock)
+        sep = eng.run_mesh(mp, p)
+        gem = eng.run_kpts(mp_mesh(mp), p)
+    for key in KEYS:
+        a, b = sep[key], gem[key]
+        scale = max(np.max(np.abs(b)), 1e-300)
+        assert np.max(np.abs(a - b)) <= 1e-11 * scale, (key, np.max(np.abs(a - b)) / scale)
