@@ -188,7 +188,7 @@ int mep_reset_kernel_times(mep_ctx *ctx);
 
 /* Largest k-block the context processes per pass (device workspace bound). */
 int mep_set_kblock(mep_ctx *ctx, int64_t kblock);
-/* Upper bound on the bytes of per-k-block work buffers (default 16e9); k-blocks
+/* Upper bound on the bytes of per-k-block work buffers (default 64e9); k-blocks
    shrink to fit.  For several resident contexts (mep_run_batch) whose
    buffers must share the 288 GB of HBM.  No counterpart in the reference.  */
 int mep_set_workspace_limit(mep_ctx *ctx, double bytes);

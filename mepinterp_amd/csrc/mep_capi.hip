@@ -89,7 +89,7 @@ struct mep_ctx {
   int* dinfo = nullptr;
   double* dscr = nullptr;  // eigensolver records
   double* dcurv = nullptr;  // per-k curvature (mep_curv_batch)
-  double ws_limit = 16.0e9;  // bytes of per-block work buffers
+  double ws_limit = 64.0e9;  // bytes of per-block work buffers (of 288 GB HBM)
   int64_t curv_cap = 0;
   double* dpart = nullptr;
   int64_t part_cap = 0;

@@ -241,13 +241,16 @@ __global__ __launch_bounds__(64) void eigh_house_reg_kernel(int n, int64_t nk,
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      Z p = zero;
+      // four interleaved partial sums keep the FMA chains short
+      Z p4[4] = {zero, zero, zero, zero};
 #pragma unroll
       for (int j = kc + 1; j < NMAX; ++j) {
         const Z a_ = row[j], b_ = vv[j];
-        p.re = fma(a_.re, b_.re, fma(-a_.im, b_.im, p.re));
-        p.im = fma(a_.re, b_.im, fma(a_.im, b_.re, p.im));
+        Z& q = p4[j & 3];
+        q.re = fma(a_.re, b_.re, fma(-a_.im, b_.im, q.re));
+        q.im = fma(a_.re, b_.im, fma(a_.im, b_.re, q.im));
       }
+      Z p = zadd(zadd(p4[0], p4[1]), zadd(p4[2], p4[3]));
       p = below ? zscale(p, tk) : zero;
       const R K = R(0.5) * tk * team_sum<NMAX, R>(v.re * p.re + v.im * p.im);
       const Z w = zsub(p, zscale(v, K));

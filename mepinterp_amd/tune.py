@@ -1,8 +1,8 @@
 """k-block autotuner for mep_run_mesh / mep_run_batch.
 
 For a model of (n_wf, NR) at a given precision, the tuner times the full run
-of the model's own mesh for a few k-block sizes (0 = the library's default of
-about 16 GB of work buffers per block) and keeps the fastest.  Choices are
+of the model's own mesh for a few k-block sizes (0 = the library's default:
+the largest block within the context's workspace limit) and keeps the fastest.  Choices are
 cached in a JSON file keyed by "n_wf/NR/precision/mesh size", so a workload is
 measured once: $MEPINTERP_TUNE_CACHE, else ~/.cache/mepinterp_amd/tune.json.
 """
