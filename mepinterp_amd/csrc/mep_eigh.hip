@@ -8,7 +8,8 @@
 // (the caller checks).
 // Three kernels per k-block:
 //   eigh_house (a team of NMAX lanes per k, lane i <-> row i, matrix in LDS;
-//      in FP64 at NMAX = 32 the rows stay in registers, eigh_house_reg):
+//      in FP64 for 16 < n <= 32 one k per wavefront with half a row per lane
+//      in registers, eigh_house_half):
 //      Householder reduction A = Q T Q^H with rank-2 trailing updates; the
 //      reflectors are stored in U's slot of the k-point, the diagonal phases
 //      P that make the subdiagonal real (T = P T_r P^H), d, e and tau in a
@@ -165,112 +166,119 @@ __global__ __launch_bounds__(64) void eigh_house_kernel(int n, int64_t nk,
 // appended, in application order, to the k-point's flat stream
 // rot[k][q] = (c, s), col[k][q] = i; the count goes to the side record.
 // Ascending eigenvalues go to E, each band's rank to the side record.
-// The same reduction with each lane's row of the Hermitian matrix in
-// registers (lane t <-> row t; rows and columns >= n are zero).  Every step
-// only reads and updates the lane's own row: the Householder vector and the
-// update vector are broadcast through a small LDS slot per team, and all
-// column indices are compile-time (the step loop is unrolled), so the
-// matrix never touches LDS.  Output identical in layout to
-// eigh_house_kernel.
-template <int NMAX, class R>
-__global__ __launch_bounds__(64) void eigh_house_reg_kernel(int n, int64_t nk,
-                                                            const typename cplx_of<R>::type* __restrict__ Hk,
-                                                            int64_t strideHk, int packed,
-                                                            const double* __restrict__ kpts,
-                                                            const double* __restrict__ tau,
-                                                            typename cplx_of<R>::type* __restrict__ U,
-                                                            R* __restrict__ side) {
-  using Z = typename cplx_of<R>::type;
-  constexpr int TEAMS = 64 / NMAX;
-  constexpr int LP = NMAX + 1;  // per-team pitch: the teams' broadcasts hit different banks
-  __shared__ Z vv_s[TEAMS * LP], ww_s[TEAMS * LP], sub_s[TEAMS * LP];
-  __shared__ R tauv_s[TEAMS * LP];
-  const int lane = threadIdx.x;
-  const int tm = lane / NMAX, t = lane % NMAX;
-  Z* vv = vv_s + tm * LP;
-  Z* ww = ww_s + tm * LP;
-  Z* sub = sub_s + tm * LP;
-  R* tauv = tauv_s + tm * LP;
-  const R two_pi = R(6.283185307179586476925);
-  const Z zero = Z{R(0), R(0)};
-
-  for (int64_t kb = (int64_t)blockIdx.x * TEAMS; kb < nk; kb += (int64_t)gridDim.x * TEAMS) {
-    const int64_t k = kb + tm;
-    const bool valid = k < nk;
-    // ---- row t: upper triangle from H(k), lower as conjugates -----------
-    Z row[NMAX];
+// Householder tridiagonalisation in FP64 for 16 < n <= 32 with one k-point
+// per wavefront: lane (t, h) = (lane % 32, lane / 32) keeps columns
+// 16h .. 16h+15 of row t in registers (64 VGPRs, so four wavefronts share a
+// SIMD).  Column kc lives in half kc / 16; its entries reach the other half
+// through a permlane swap.  v_j = w_j = 0 for j <= kc, so the product and the
+// rank-2 update run over all of a lane's columns without a branch.  Same
+// outputs as eigh_house_kernel.
+__global__ __launch_bounds__(64, 4) void eigh_house_half_kernel(int n, int64_t nk,
+                                                             const zd* __restrict__ Hk,
+                                                             int64_t strideHk, int packed,
+                                                             const double* __restrict__ kpts,
+                                                             const double* __restrict__ tau,
+                                                             zd* __restrict__ U,
+                                                             double* __restrict__ side) {
+  constexpr int NMAX = 32, HC = 16;
+  __shared__ zd vv[NMAX], ww[NMAX], sub[NMAX];
+  __shared__ double tauv[NMAX];
+  const int lane = threadIdx.x & 63;
+  const int t = lane & 31, h = lane >> 5;
+  const double two_pi = 6.283185307179586476925;
+  const zd zero = zd{0.0, 0.0};
+  // the value of lane ^ 32 (the other half of the same row)
+  auto other = [&](double v) {
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)u, (unsigned)u, false, false);
+    const auto hi =
+        __builtin_amdgcn_permlane32_swap((unsigned)(u >> 32), (unsigned)(u >> 32), false, false);
+    return h == 0 ? join_d(lo[1], hi[1]) : join_d(lo[0], hi[0]);
+  };
+  for (int64_t k = blockIdx.x; k < nk; k += gridDim.x) {
+    // ---- row t, columns 16h + jj: upper triangle from H(k), lower as conjugates
+    zd row[HC];
 #pragma unroll
-    for (int j = 0; j < NMAX; ++j) {
-      Z h = zero;
-      if (valid && t < n && j < n) {
+    for (int jj = 0; jj < HC; ++jj) {
+      const int j = HC * h + jj;
+      zd x = zero;
+      if (t < n && j < n) {
         const int i0 = j >= t ? t : j, j0 = j >= t ? j : t;  // (i0 <= j0) element
         const int64_t o = packed ? i0 + ((int64_t)j0 * (j0 + 1)) / 2 : i0 + (int64_t)n * j0;
-        h = Hk[k * strideHk + o];
+        x = Hk[k * strideHk + o];
         if (tau && i0 != j0) {
-          const R kx = kpts[3 * k], ky = kpts[3 * k + 1], kz = kpts[3 * k + 2];
-          R ang = two_pi * (kx * (tau[3 * j0] - tau[3 * i0]) + ky * (tau[3 * j0 + 1] - tau[3 * i0 + 1]) +
-                            kz * (tau[3 * j0 + 2] - tau[3 * i0 + 2]));
-          R sn, cs;
+          const double ang = two_pi * (kpts[3 * k] * (tau[3 * j0] - tau[3 * i0]) +
+                                       kpts[3 * k + 1] * (tau[3 * j0 + 1] - tau[3 * i0 + 1]) +
+                                       kpts[3 * k + 2] * (tau[3 * j0 + 2] - tau[3 * i0 + 2]));
+          double sn, cs;
           sin_cos(ang, &sn, &cs);
-          h = zmul(Z{cs, sn}, h);
+          x = zmul(zd{cs, sn}, x);
         }
-        if (j < t) h = zconj(h);
-        if (j == t) h.im = R(0);
+        if (j < t) x = zconj(x);
+        if (j == t) x.im = 0.0;
       }
-      row[j] = h;
+      row[jj] = x;
     }
-    // ---- Householder steps (compile-time column index kc) ----------------
-#pragma unroll
+#pragma clang loop unroll(full)
     for (int kc = 0; kc < NMAX - 2; ++kc) {
-      if (kc + 2 >= n) continue;  // fixed trip count: every index stays compile-time
-      const bool below = t > kc;  // rows kc+1 .. n-1 take part
-      const Z x = below ? row[kc] : zero;
-      const R tail2 = team_sum<NMAX, R>((t > kc + 1) ? zabs2(x) : R(0));
-      const Z x0 = Z{team_bcast<NMAX>(x.re, kc + 1), team_bcast<NMAX>(x.im, kc + 1)};
-      const bool refl = tail2 > R(0);
-      const R ax02 = zabs2(x0);
-      const R xn = sqrt(tail2 + ax02);
-      const R ax0 = sqrt(ax02);
-      const Z alpha = !refl ? x0 : ((ax0 > R(0)) ? zscale(x0, -xn / ax0) : Z{-xn, R(0)});
-      Z v = (refl && below) ? x : zero;
-      if (t == kc + 1 && refl) v = zsub(x, alpha);
+      if (kc + 2 >= n) continue;  // uniform; every index below stays compile-time
+      const int hk = kc / HC, jc = kc % HC;
+      const bool mine = h == hk;   // this lane holds column kc
+      const bool below = t > kc;   // rows kc+1 .. n-1 take part
+      const zd xo = mine ? row[jc] : zero;
+      const double tail2 = team_sum<32, double>((mine && t > kc + 1) ? zabs2(xo) : 0.0);
+      // cross-lane reads stay outside conditionals: every lane takes part
+      const double tail2o = other(tail2);
+      const double tail = h == hk ? tail2 : tail2o;  // the owning half's sum
+      const zd x0 = zd{readlane_d(xo.re, 32 * hk + kc + 1), readlane_d(xo.im, 32 * hk + kc + 1)};
+      const zd xoo = zd{other(xo.re), other(xo.im)};
+      const zd xt = mine ? xo : xoo;  // x_t in both halves
+      const bool refl = tail > 0.0;
+      const double ax02 = zabs2(x0);
+      const double xn = sqrt(tail + ax02);
+      const double ax0 = sqrt(ax02);
+      const zd alpha = !refl ? x0 : ((ax0 > 0.0) ? zscale(x0, -xn / ax0) : zd{-xn, 0.0});
+      zd v = (refl && below) ? xt : zero;
+      if (t == kc + 1 && refl) v = zsub(xt, alpha);
       // |v|^2 = |x|^2 - 2 Re(conj(alpha) x0) + |alpha|^2 = 2 xn (xn + |x0|)
-      const R vn2 = R(2) * xn * (xn + ax0);
-      const R tk = refl ? R(2) / vn2 : R(0);
-      vv[t] = v;
+      const double vn2 = 2.0 * xn * (xn + ax0);
+      const double tk = refl ? 2.0 / vn2 : 0.0;
+      if (h == 0) vv[t] = v;
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      // four interleaved partial sums keep the FMA chains short
-      Z p4[4] = {zero, zero, zero, zero};
+      // p_t = tk sum_j A(t, j) v_j: this half's columns, then the other's
+      zd p4[4] = {zero, zero, zero, zero};
 #pragma unroll
-      for (int j = kc + 1; j < NMAX; ++j) {
-        const Z a_ = row[j], b_ = vv[j];
-        Z& q = p4[j & 3];
+      for (int jj = 0; jj < HC; ++jj) {
+        const zd a_ = row[jj], b_ = vv[HC * h + jj];
+        zd& q = p4[jj & 3];
         q.re = fma(a_.re, b_.re, fma(-a_.im, b_.im, q.re));
         q.im = fma(a_.re, b_.im, fma(a_.im, b_.re, q.im));
       }
-      Z p = zadd(zadd(p4[0], p4[1]), zadd(p4[2], p4[3]));
+      zd p = zadd(zadd(p4[0], p4[1]), zadd(p4[2], p4[3]));
+      const zd po = zd{other(p.re), other(p.im)};
+      p = (h == 0) ? zadd(p, po) : zadd(po, p);  // the same order in both halves
       p = below ? zscale(p, tk) : zero;
-      const R K = R(0.5) * tk * team_sum<NMAX, R>(v.re * p.re + v.im * p.im);
-      const Z w = zsub(p, zscale(v, K));
-      ww[t] = w;
+      const double K = 0.5 * tk * team_sum<32, double>(v.re * p.re + v.im * p.im);
+      const zd w = zsub(p, zscale(v, K));
+      if (h == 0) ww[t] = w;
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       if (refl && below) {
 #pragma unroll
-        for (int j = kc + 1; j < NMAX; ++j) {
-          // row_j -= v conj(w_j) + w conj(v_j)
-          const Z vj = vv[j], wj = ww[j];
-          Z r_ = row[j];
+        for (int jj = 0; jj < HC; ++jj) {
+          // row_j -= v conj(w_j) + w conj(v_j)  (zero for j <= kc)
+          const zd vj = vv[HC * h + jj], wj = ww[HC * h + jj];
+          zd r_ = row[jj];
           r_.re = fma(-v.re, wj.re, fma(-v.im, wj.im, fma(-w.re, vj.re, fma(-w.im, vj.im, r_.re))));
           r_.im = fma(-v.im, wj.re, fma(v.re, wj.im, fma(-w.im, vj.re, fma(w.re, vj.im, r_.im))));
-          row[j] = r_;
+          row[jj] = r_;
         }
       }
-      if (below) row[kc] = v;
-      if (t == 0) {
+      if (mine && below) row[jc] = v;
+      if (lane == 0) {
         sub[kc] = alpha;
         tauv[kc] = tk;
       }
@@ -278,50 +286,53 @@ __global__ __launch_bounds__(64) void eigh_house_reg_kernel(int n, int64_t nk,
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
-    // ---- outputs ----------------------------------------------------------
-    // diagonal and A(n-1, n-2) by compile-time picks
-    R dgl = R(0);
-    Z last = zero;
+    // ---- outputs -----------------------------------------------------------
+    double dgl = 0.0;  // diagonal of row t, held by half t / 16
+    zd last = zero;    // A(n-1, n-2)
 #pragma unroll
-    for (int j = 0; j < NMAX; ++j) {
-      if (j == t) dgl = row[j].re;
-      if (j == n - 2) last = row[j];
+    for (int jj = 0; jj < HC; ++jj) {
+      if (HC * h + jj == t) dgl = row[jj].re;
+      if (HC * h + jj == n - 2) last = row[jj];
     }
-    if (n >= 2) {
-      const Z l2 = Z{team_bcast<NMAX>(last.re, n - 1), team_bcast<NMAX>(last.im, n - 1)};
-      if (t == 0) {
+    const int hd = t / HC;
+    const double dglo = other(dgl);
+    dgl = (h == hd) ? dgl : dglo;
+    {
+      const int src = 32 * ((n - 2) / HC) + n - 1;
+      const zd l2 = zd{readlane_d(last.re, src), readlane_d(last.im, src)};
+      if (lane == 0) {
         sub[n - 2] = l2;
-        tauv[n - 2] = R(0);
+        tauv[n - 2] = 0.0;
       }
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    if (valid) {
-      R* sd = side + k * side_len(n);
-      Z* Uk = U + k * (int64_t)n * n;
-      // reflectors: row t keeps v_t of every column kc < t in row[kc]
+    double* sd = side + k * side_len(n);
+    zd* Uk = U + k * (int64_t)n * n;
+    // reflectors: row t keeps v_t of every column kc < t
 #pragma unroll
-      for (int kc = 0; kc < NMAX; ++kc)
-        if (kc < t && t < n) Uk[t + (int64_t)n * kc] = row[kc];
-      if (t < n) {
-        sd[t] = dgl;
-        sd[2 * n + t] = (t < n - 1) ? tauv[t] : R(0);
+    for (int jj = 0; jj < HC; ++jj) {
+      const int kc = HC * h + jj;
+      if (kc < t && t < n) Uk[t + (int64_t)n * kc] = row[jj];
+    }
+    if (h == 0 && t < n) {
+      sd[t] = dgl;
+      sd[2 * n + t] = (t < n - 1) ? tauv[t] : 0.0;
+    }
+    if (lane == 0) {
+      zd ph = zd{1.0, 0.0};
+      sd[3 * n] = 1.0;
+      sd[3 * n + 1] = 0.0;
+      for (int i = 0; i + 1 < n; ++i) {
+        const zd s_ = sub[i];
+        const double as = sqrt(zabs2(s_));
+        sd[n + i] = as;
+        if (as > 0.0) ph = zmul(ph, zscale(s_, 1.0 / as));
+        sd[3 * n + 2 * (i + 1)] = ph.re;
+        sd[3 * n + 2 * (i + 1) + 1] = ph.im;
       }
-      if (t == 0) {
-        Z ph = Z{R(1), R(0)};
-        sd[3 * n] = R(1);
-        sd[3 * n + 1] = R(0);
-        for (int i = 0; i + 1 < n; ++i) {
-          const Z s_ = sub[i];
-          const R as = sqrt(zabs2(s_));
-          sd[n + i] = as;
-          if (as > R(0)) ph = zmul(ph, zscale(s_, R(1) / as));
-          sd[3 * n + 2 * (i + 1)] = ph.re;
-          sd[3 * n + 2 * (i + 1) + 1] = ph.im;
-        }
-        sd[n + n - 1] = R(0);
-      }
+      sd[n + n - 1] = 0.0;
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -1112,10 +1123,11 @@ static void launch_eigh_t(hipStream_t s, int n, int64_t nk, const typename cplx_
   unsigned grid = (unsigned)(blocks < 65536 ? blocks : 65536);
   int64_t qb = (nk + 63) / 64;
   unsigned qgrid = (unsigned)(qb < 65536 ? qb : 65536);
+  const unsigned hgrid = (unsigned)(nk < 65536 ? nk : 65536);  // one k per wavefront
 #define MEP_EIGH_LAUNCH(NM)                                                                      \
   if constexpr (NM == 32 && sizeof(R) == 8)                                                              \
-    hipLaunchKernelGGL((eigh_house_reg_kernel<NM, R>), dim3(grid), dim3(64), 0, s, n, nk, Hk,     \
-                       strideHk, packed, g.kpts, g.tau, U, side);                                \
+    hipLaunchKernelGGL(eigh_house_half_kernel, dim3(hgrid), dim3(64), 0, s, n, nk, Hk, strideHk,  \
+                       packed, g.kpts, g.tau, U, side);                                          \
   else                                                                                           \
     hipLaunchKernelGGL((eigh_house_kernel<NM, R>), dim3(grid), dim3(64), 0, s, n, nk, Hk, strideHk, \
                        packed, g.kpts, g.tau, U, side);                                          \
