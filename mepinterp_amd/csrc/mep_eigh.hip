@@ -347,8 +347,8 @@ __global__ __launch_bounds__(64, 4) void eigh_house_half_kernel(int n, int64_t n
 // is at least 1e-5 |T| (1e-3 |T| in FP32; the vectors are then computed
 // directly from T) or -2 otherwise (the caller reruns the k-point with
 // rec = true).
-template <int LANES, class R>
-__device__ inline void ql_one(int n, int64_t k, R* ds, R* es, R* __restrict__ side,
+template <int LANES, class R, int NMAX = 0>
+__device__ __attribute__((always_inline)) inline void ql_one(int n, int64_t k, R* ds, R* es, R* __restrict__ side,
                               R* __restrict__ rot, uint8_t* __restrict__ rcol,
                               double* __restrict__ E, int* __restrict__ info, bool rec = true) {
   const R EPS = cplx_of<R>::eps;
@@ -421,8 +421,32 @@ __device__ inline void ql_one(int n, int64_t k, R* ds, R* es, R* __restrict__ si
     }
   }
   if (fail) atomicOr(info, 1);
-  // ascending order: rank of each eigenvalue (ties by index); smallest gap
   R gap = R(1e30), big = R(0);
+  if constexpr (NMAX > 0) {
+    // n <= NMAX: the ranks from registers (compile-time indices), then the
+    // eigenvalues in ascending order through the LDS column, and the
+    // smallest gap between neighbours
+    R dv[NMAX];
+#pragma unroll
+    for (int i = 0; i < NMAX; ++i) dv[i] = (i < n) ? ds[i * LANES] : R(0);
+#pragma unroll
+    for (int j = 0; j < NMAX; ++j) {
+      if (j >= n) continue;  // fixed trip count: every index stays compile-time
+      int rnk = 0;
+#pragma unroll
+      for (int i = 0; i < NMAX; ++i)
+        if (i < n && i != j) rnk += (dv[i] < dv[j]) || (dv[i] == dv[j] && i < j);
+      big = fmax(big, fabs(dv[j]));
+      es[j * LANES] = (R)rnk;  // e is no longer needed: keep the rank
+      ds[rnk * LANES] = dv[j];
+    }
+    for (int i = 0; i < n; ++i) {
+      const R v = ds[i * LANES];
+      E[k * n + i] = v;
+      if (i > 0) gap = fmin(gap, v - ds[(i - 1) * LANES]);
+    }
+  } else {
+  // ascending order: rank of each eigenvalue (ties by index); smallest gap
   for (int j = 0; j < n; ++j) {
     const R dj = ds[j * LANES];
     int rnk = 0;
@@ -434,6 +458,7 @@ __device__ inline void ql_one(int n, int64_t k, R* ds, R* es, R* __restrict__ si
     big = fmax(big, fabs(dj));
     E[k * n + rnk] = dj;
     es[j * LANES] = (R)rnk;  // e is no longer needed: keep the rank
+  }
   }
   R* sdw = side + k * side_len(n);
   if (!rec) {
@@ -471,14 +496,14 @@ __global__ __launch_bounds__(64) void eigh_ql_kernel(int n, int64_t nk,
   if (mode == 2) {
     const int64_t cnt = *count;
     for (int64_t i = (int64_t)blockIdx.x * 64 + lane; i < cnt; i += (int64_t)gridDim.x * 64)
-      ql_one<64, R>(n, list[i], ds + lane, es + lane, side, rot, rcol, E, info, true);
+      ql_one<64, R, NMAX>(n, list[i], ds + lane, es + lane, side, rot, rcol, E, info, true);
     return;
   }
   for (int64_t kb = (int64_t)blockIdx.x * 64; kb < nk; kb += (int64_t)gridDim.x * 64) {
     const int64_t k = kb + lane;
     bool again = false;
     if (k < nk) {
-      ql_one<64, R>(n, k, ds + lane, es + lane, side, rot, rcol, E, info, false);
+      ql_one<64, R, NMAX>(n, k, ds + lane, es + lane, side, rot, rcol, E, info, false);
       again = side[k * side_len(n) + 5 * n] == R(-2);
     }
     const unsigned long long m = __ballot(again);
