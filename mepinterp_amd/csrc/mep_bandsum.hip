@@ -1033,13 +1033,6 @@ size_t bandsum_wide_lds_bytes(const BandArgs& b) {
 // ===========================================================================
 constexpr int HW_CH = 256;  // pair candidates per chunk (one per thread)
 
-__device__ inline double rcp_nr(double x) {
-  // reciprocal: hardware estimate refined by two Newton steps
-  double r = __builtin_amdgcn_rcp(x);
-  r = fma(r, fma(-x, r, 1.0), r);
-  r = fma(r, fma(-x, r, 1.0), r);
-  return r;
-}
 
 __global__ __launch_bounds__(256, 3) void hwsum_kernel(BandArgs b, double Tsmear, AccLayout L,
                                                        int64_t nk, const double* __restrict__ E,

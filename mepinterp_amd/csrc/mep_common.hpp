@@ -62,6 +62,19 @@ struct cplx_of<float> {
   using type = zf;
   static constexpr float eps = 1.1920929e-07f;
 };
+// reciprocal of a normal, nonzero x: hardware estimate refined by Newton
+// steps (a shorter dependency chain than the IEEE division sequence)
+__device__ inline double rcp_nr(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = fma(r, fma(-x, r, 1.0), r);
+  r = fma(r, fma(-x, r, 1.0), r);
+  return r;
+}
+__device__ inline float rcp_nr(float x) {
+  float r = __builtin_amdgcn_rcpf(x);
+  r = fmaf(r, fmaf(-x, r, 1.0f), r);
+  return r;
+}
 __device__ inline void sin_cos(double a, double* s, double* c) { sincos(a, s, c); }
 __device__ inline void sin_cos(float a, float* s, float* c) { sincosf(a, s, c); }
 
