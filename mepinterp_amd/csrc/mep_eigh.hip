@@ -624,14 +624,15 @@ __global__ __launch_bounds__(64) void eigh_vec_kernel(int n, int64_t nk,
       // (Dhillon and Parlett's twisted factorizations); pivots below
       // eps |T| are clamped to +-eps |T|.
       const R lam = (t < n) ? (R)E[k * n + t] : R(0);
-      // d and e once, every load issued up front (indices clamped to n - 1)
-      R dl[NMAX], el[NMAX];
-#pragma unroll
-      for (int i = 0; i < NMAX; ++i) {
-        const int ic = i < n ? i : n - 1;
-        dl[i] = sd[ic];
-        el[i] = sd[n + ic];
-      }
+      // d and e of the k-point in the team's rotation-staging slot (unused
+      // on this path): one coalesced load, then LDS broadcasts
+      csb[t] = (t < n) ? sd[t] : R(0);
+      csb[NMAX + t] = (t < n) ? sd[n + t] : R(0);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      const R* dl = csb;
+      const R* el = csb + NMAX;
       R tn = R(0);
 #pragma unroll
       for (int i = 0; i < NMAX; ++i)
@@ -742,13 +743,22 @@ __global__ __launch_bounds__(64) void eigh_vec_kernel(int n, int64_t nk,
       for (int kc = n - 3; kc >= 0; --kc) {
         const Z* v = Vf + kc * NMAX;
         const R tk = taus[kc];
-        // four interleaved partial sums of v^H W
+        // four interleaved partial sums of v^H W, fused multiply-adds
         Z s4[4] = {Z{R(0), R(0)}, Z{R(0), R(0)}, Z{R(0), R(0)}, Z{R(0), R(0)}};
 #pragma unroll
-        for (int i = 0; i < NMAX; ++i) s4[i & 3] = zadd(s4[i & 3], zcmul(v[i], W[i]));
+        for (int i = 0; i < NMAX; ++i) {
+          const Z vi = v[i], wi = W[i];
+          Z& q = s4[i & 3];
+          q.re = fma(vi.re, wi.re, fma(vi.im, wi.im, q.re));
+          q.im = fma(vi.re, wi.im, fma(-vi.im, wi.re, q.im));
+        }
         const Z sacc = zscale(zadd(zadd(s4[0], s4[1]), zadd(s4[2], s4[3])), tk);
 #pragma unroll
-        for (int i = 0; i < NMAX; ++i) W[i] = zsub(W[i], zmul(v[i], sacc));
+        for (int i = 0; i < NMAX; ++i) {  // W -= v sacc
+          const Z vi = v[i];
+          W[i].re = fma(-vi.re, sacc.re, fma(vi.im, sacc.im, W[i].re));
+          W[i].im = fma(-vi.re, sacc.im, fma(-vi.im, sacc.re, W[i].im));
+        }
       }
     }
     // gauge convention: largest-modulus component (first on ties) real > 0
