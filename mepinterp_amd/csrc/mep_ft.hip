@@ -340,8 +340,16 @@ __global__ __launch_bounds__(256) void sep_z_kernel(int64_t NC, int bx, int by, 
   Y1[(int64_t)c * NC + q] = zd{sr, si};
 }
 
-__device__ inline void store_h(zd* p, double re, double im) { *p = zd{re, im}; }
-__device__ inline void store_h(zf* p, double re, double im) { *p = zf{(float)re, (float)im}; }
+// H(k) is written once and read by the next kernel long after the caches
+// have turned over: non-temporal stores
+__device__ inline void store_h(zd* p, double re, double im) {
+  __builtin_nontemporal_store(re, &p->re);
+  __builtin_nontemporal_store(im, &p->im);
+}
+__device__ inline void store_h(zf* p, double re, double im) {
+  __builtin_nontemporal_store((float)re, &p->re);
+  __builtin_nontemporal_store((float)im, &p->im);
+}
 
 template <int BX, class OT>
 __global__ __launch_bounds__(256) void sep_xy_kernel(int64_t NC, int bx, int by, int n1, int n2,
