@@ -1034,14 +1034,27 @@ size_t bandsum_wide_lds_bytes(const BandArgs& b) {
 constexpr int HW_CH = 256;  // pair candidates per chunk (one per thread)
 
 
+// Records per pair (x, y) of differing occupation: dE, w_k (f_x - f_y) for
+// every E_F, then the V products.  General V: 9 Im(V_i(x,y) V_j(y,x)) and 9
+// complex V_i(y,x) V_j(x,y) (27 values).  V built here (Hermitian): with
+// P_ij = x_i conj(x_j), x_a = V_a(x, y), only Im P_ij and P_ij (i < j) and
+// |x_i|^2 (12 values), and the items accumulate sum rv Im P_ij,
+// X = sum Re(ro) P_ij, Y = sum Im(ro) P_ij and the same of |x_i|^2.
+template <bool HERM>
+__host__ __device__ constexpr int hw_vals() { return HERM ? 12 : 27; }
+template <bool HERM>
+__host__ __device__ constexpr int hw_accs() { return HERM ? 24 : 36; }
+
+template <bool HERM>
 __global__ __launch_bounds__(256, 3) void hwsum_kernel(BandArgs b, double Tsmear, AccLayout L,
                                                        int64_t nk, const double* __restrict__ E,
                                                        const zd* __restrict__ V,
                                                        const double* __restrict__ w,
                                                        double* __restrict__ part) {
+  constexpr int NV = hw_vals<HERM>(), NA = hw_accs<HERM>(), HA = NA / 2;
   extern __shared__ double smem[];
   const int n = b.n, nef = b.n_ef, nhw = b.n_hw;
-  const int PS = 28 + nef;  // doubles per record: dE, dfv[nef], 9 vim, 18 q2o
+  const int PS = 1 + nef + NV;  // doubles per record: dE, dfv[nef], the products
   double* Es = smem;
   double* fd = Es + n;
   int* cnt_w = reinterpret_cast<int*>(fd + (int64_t)nef * n);  // [4] per-wave counts
@@ -1059,9 +1072,9 @@ __global__ __launch_bounds__(256, 3) void hwsum_kernel(BandArgs b, double Tsmear
     const int h = it % nhw, ef = it / nhw;
     const zd wh = zd{b.hw[act ? h : 0], b.eta};
     const zd wh2 = zmul(wh, wh);
-    double sv[18], so[18];
+    double acc[NA];
 #pragma unroll
-    for (int q = 0; q < 18; ++q) sv[q] = so[q] = 0.0;
+    for (int q = 0; q < NA; ++q) acc[q] = 0.0;
     for (int64_t k = blockIdx.x; k < nk; k += gridDim.x) {
       const double wk = w ? w[k] : 1.0;
       const zd* Vk = V + k * 3 * (int64_t)n2;
@@ -1096,25 +1109,38 @@ __global__ __launch_bounds__(256, 3) void hwsum_kernel(BandArgs b, double Tsmear
         const int cnt = cnt_w[0] + cnt_w[1] + cnt_w[2] + cnt_w[3];
         if (keep) {
           double* sp = st + (int64_t)(base + below) * PS;
-          zd vxy[3], vyx[3];
-#pragma unroll
-          for (int a = 0; a < 3; ++a) {
-            vxy[a] = Vk[a * (int64_t)n2 + x + (int64_t)n * y];
-            vyx[a] = Vk[a * (int64_t)n2 + y + (int64_t)n * x];
-          }
           sp[0] = Es[x] - Es[y];
           for (int e_ = 0; e_ < nef; ++e_) sp[1 + e_] = wk * (fd[e_ + nef * x] - fd[e_ + nef * y]);
           double* pq = sp + 1 + nef;
+          zd vxy[3];
 #pragma unroll
-          for (int j = 0; j < 3; ++j)
+          for (int a = 0; a < 3; ++a) vxy[a] = Vk[a * (int64_t)n2 + x + (int64_t)n * y];
+          if constexpr (HERM) {
+            // pq: Im P_c (3), P_c (3 complex), |x_a|^2 (3); c = (0,1), (0,2), (1,2)
 #pragma unroll
-            for (int i = 0; i < 3; ++i) {
-              const int q = i + 3 * j;
-              pq[q] = zmul(vxy[i], vyx[j]).im;
-              const zd q2 = zmul(vyx[i], vxy[j]);
-              pq[9 + 2 * q] = q2.re;
-              pq[9 + 2 * q + 1] = q2.im;
+            for (int c = 0; c < 3; ++c) {
+              const zd xi = vxy[c == 2 ? 1 : 0], xj = vxy[c == 0 ? 1 : 2];
+              const zd P = zd{xi.re * xj.re + xi.im * xj.im, xi.im * xj.re - xi.re * xj.im};
+              pq[c] = P.im;
+              pq[3 + 2 * c] = P.re;
+              pq[4 + 2 * c] = P.im;
+              pq[9 + c] = zabs2(vxy[c]);
             }
+          } else {
+            zd vyx[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) vyx[a] = Vk[a * (int64_t)n2 + y + (int64_t)n * x];
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+#pragma unroll
+              for (int i = 0; i < 3; ++i) {
+                const int q = i + 3 * j;
+                pq[q] = zmul(vxy[i], vyx[j]).im;
+                const zd q2 = zmul(vyx[i], vxy[j]);
+                pq[9 + 2 * q] = q2.re;
+                pq[9 + 2 * q + 1] = q2.im;
+              }
+          }
         }
         __syncthreads();
         if (act)
@@ -1132,41 +1158,100 @@ __global__ __launch_bounds__(256, 3) void hwsum_kernel(BandArgs b, double Tsmear
             const double iv = fi * ao, io = fi * av;
             const zd rv = zd{denv.re * iv, -denv.im * iv};
             const zd ro = zd{deno.re * io, -deno.im * io};
+            if constexpr (HERM) {
+              // acc: [0..5] rv Im P_c, [6..11] Re(ro) |x_a|^2 then Im(ro) |x_a|^2,
+              // [12..17] Re(ro) P_c, [18..23] Im(ro) P_c
 #pragma unroll
-            for (int q = 0; q < 9; ++q) {
-              const double vim = pq[q];
-              sv[2 * q] += vim * rv.re;
-              sv[2 * q + 1] += vim * rv.im;
-              // i * (q2o * ro)
-              const double qr = pq[9 + 2 * q], qi = pq[9 + 2 * q + 1];
-              so[2 * q] -= qr * ro.im + qi * ro.re;
-              so[2 * q + 1] += qr * ro.re - qi * ro.im;
+              for (int c = 0; c < 3; ++c) {
+                acc[2 * c] += pq[c] * rv.re;
+                acc[2 * c + 1] += pq[c] * rv.im;
+                acc[6 + c] += ro.re * pq[9 + c];
+                acc[9 + c] += ro.im * pq[9 + c];
+                acc[12 + 2 * c] += ro.re * pq[3 + 2 * c];
+                acc[13 + 2 * c] += ro.re * pq[4 + 2 * c];
+                acc[18 + 2 * c] += ro.im * pq[3 + 2 * c];
+                acc[19 + 2 * c] += ro.im * pq[4 + 2 * c];
+              }
+            } else {
+              // acc: [0..17] velo_q (re, im), [18..35] ohc_q (re, im)
+#pragma unroll
+              for (int q = 0; q < 9; ++q) {
+                const double vim = pq[q];
+                acc[2 * q] += vim * rv.re;
+                acc[2 * q + 1] += vim * rv.im;
+                // i * (q2o * ro)
+                const double qr = pq[9 + 2 * q], qi = pq[9 + 2 * q + 1];
+                acc[18 + 2 * q] -= qr * ro.im + qi * ro.re;
+                acc[18 + 2 * q + 1] += qr * ro.re - qi * ro.im;
+              }
             }
           }
       }
     }
-    // combine the splits of each item, velo then ohc: red[split][item][18]
-    // over the staging area
+    // combine the splits of each item in two halves of the accumulators
+    // (red[split][item][HA] over the staging area); each half holds
+    // everything its outputs need
     for (int half = 0; half < 2; ++half) {
       __syncthreads();
       if (act)
 #pragma unroll
-        for (int q = 0; q < 18; ++q) st[(split * IW + t % IW) * 18 + q] = half ? so[q] : sv[q];
+        for (int q = 0; q < HA; ++q) st[(split * IW + t % IW) * HA + q] = acc[half * HA + q];
       __syncthreads();
-      for (int o = t; o < IW * 18; o += 256) {
-        const int i2 = o / 18, q = o % 18;
+      auto red = [&](int i2, int q) {
         double sR = 0.0;
-        for (int u = 0; u < S; ++u) sR += st[(u * IW + i2) * 18 + q];
-        const int item = g0 + i2, hh = item % nhw, e2 = item / nhw;
-        accp[(half ? L.ohc : L.velo) + 2 * 9 * (hh + (int64_t)nhw * e2) + q] += sR;
+        for (int u = 0; u < S; ++u) sR += st[(u * IW + i2) * HA + q];
+        return sR;
+      };
+      if constexpr (!HERM) {
+        for (int o = t; o < IW * 18; o += 256) {
+          const int i2 = o / 18, q = o % 18;
+          const int item = g0 + i2, hh = item % nhw, e2 = item / nhw;
+          accp[(half ? L.ohc : L.velo) + 2 * 9 * (hh + (int64_t)nhw * e2) + q] += red(i2, q);
+        }
+      } else {
+        // half 0: velo (antisymmetric, zero diagonal) and the ohc diagonal;
+        // half 1: the ohc off-diagonal entries
+        const int per = half ? 12 : 24;
+        for (int o = t; o < IW * per; o += 256) {
+          const int i2 = o / per, r = o % per;
+          const int item = g0 + i2, hh = item % nhw, e2 = item / nhw;
+          const int64_t rec = 2 * 9 * (hh + (int64_t)nhw * e2);
+          if (half == 0) {
+            if (r < 18) {  // velo_q, q = i + 3j, component r % 2
+              const int q = r / 2, cpx = r % 2, i = q % 3, j = q / 3;
+              if (i != j) {
+                const double v = red(i2, 2 * (i + j - 1) + cpx);
+                accp[L.velo + rec + r] += (i < j) ? v : -v;
+              }
+            } else {       // ohc_ii = i sum ro |x_i|^2 = (-Y, X)
+              const int i = (r - 18) / 2, cpx = (r - 18) % 2;
+              const double v = cpx ? red(i2, 6 + i) : -red(i2, 9 + i);
+              accp[L.ohc + rec + 2 * (4 * i) + cpx] += v;
+            }
+          } else {         // ohc_ij (i < j) = (Xi - Yr, Xr + Yi); ohc_ji = (-(Xi + Yr), Xr - Yi)
+            const int c = r / 4, sel = r % 4;  // sel: (ij re, ij im, ji re, ji im)
+            const int i = (c == 2) ? 1 : 0, j = (c == 0) ? 1 : 2;
+            const double xr = red(i2, 2 * c), xi = red(i2, 2 * c + 1);
+            const double yr = red(i2, 6 + 2 * c), yi = red(i2, 6 + 2 * c + 1);
+            double v;
+            int q;
+            if (sel == 0) { v = xi - yr; q = i + 3 * j; }
+            else if (sel == 1) { v = xr + yi; q = i + 3 * j; }
+            else if (sel == 2) { v = -(xi + yr); q = j + 3 * i; }
+            else { v = xr - yi; q = j + 3 * i; }
+            accp[L.ohc + rec + 2 * q + (sel & 1)] += v;
+          }
+        }
       }
     }
   }
 }
 
 size_t hwsum_lds_bytes(const BandArgs& b) {
-  const size_t ps = 28 + (size_t)b.n_ef;
-  const size_t stage = (size_t)HW_CH * ps > 256 * 18 ? (size_t)HW_CH * ps : 256 * 18;
+  const size_t nv = b.v_herm ? hw_vals<true>() : hw_vals<false>();
+  const size_t ha = (b.v_herm ? hw_accs<true>() : hw_accs<false>()) / 2;
+  const size_t ps = 1 + (size_t)b.n_ef + nv;
+  const size_t stage = (size_t)HW_CH * ps > 256 * ha ? (size_t)HW_CH * ps : 256 * ha;
   return sizeof(double) * ((size_t)b.n + (size_t)b.n_ef * b.n + 4 + stage);
 }
 
@@ -1266,10 +1351,17 @@ void launch_bandsum(hipStream_t s, const BandArgs& b, int64_t nk, const double* 
                        V, w, part);
     if (b.jobs & MEP_JOB_AHC) {
       const size_t lh = hwsum_lds_bytes(b);
-      (void)hipFuncSetAttribute((const void*)hwsum_kernel,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lh);
-      hipLaunchKernelGGL(hwsum_kernel, dim3(nblocks), dim3(256), lh, s, b, Tsmear, L, nk, E, V, w,
-                         part);
+      if (b.v_herm) {
+        (void)hipFuncSetAttribute((const void*)hwsum_kernel<true>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lh);
+        hipLaunchKernelGGL(hwsum_kernel<true>, dim3(nblocks), dim3(256), lh, s, b, Tsmear, L, nk,
+                           E, V, w, part);
+      } else {
+        (void)hipFuncSetAttribute((const void*)hwsum_kernel<false>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lh);
+        hipLaunchKernelGGL(hwsum_kernel<false>, dim3(nblocks), dim3(256), lh, s, b, Tsmear, L, nk,
+                           E, V, w, part);
+      }
     }
   } else if (b.n <= 16) {
     launch_bs<16>(s, b, Tsmear, L, nk, E, V, A, Om, w, nblocks, part);

@@ -122,11 +122,13 @@ def _compare_sums(g, o, rel=REL, keys=KEYS, atol=None):
         assert _rel(a, b) < rel, f"{key}: rel err {_rel(a, b):.3e}"
 
 
-@pytest.mark.parametrize("T,nhw,nef", [(300.0, 1, 1), (0.0, 5, 3), (300.0, 70, 2)])
-def test_bandsum_batch_matches_oracle(T, nhw, nef):
-    """Same (e, V) on both sides: isolates the band-sum kernel."""
+@pytest.mark.parametrize("n,T,nhw,nef", [(12, 300.0, 1, 1), (12, 0.0, 5, 3), (12, 300.0, 70, 2),
+                                          (40, 300.0, 9, 2)])
+def test_bandsum_batch_matches_oracle(n, T, nhw, nef):
+    """Same (e, V) on both sides: isolates the band-sum kernel.  V from the
+    caller takes the general (no symmetry assumed) paths, n = 40 the wide ones."""
     from mepinterp_amd import capi, models
-    m = models.make_random(seed=21, n=12, L=1)
+    m = models.make_random(seed=21, n=n, L=1)
     k = rand_kpts(40, seed=2)
     om = oracle_model(m)
     import oracle as O
