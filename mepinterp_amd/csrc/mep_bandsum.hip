@@ -152,9 +152,18 @@ __global__ __launch_bounds__(BS_THREADS, 2) void bandsum_kernel(BandArgs b, doub
     __syncthreads();
     // ---- r(m, c) = 1 / (E_c - E_m); occupations ---------------------------
     {
-      const int m = t % NMAX;
-      for (int c = t / NMAX; c < n; c += BS_THREADS / NMAX)
-        if (m < n) Rs[m + LD * c] = (m == c) ? 0.0 : 1.0 / (Es[c] - Es[m]);
+      // one division per pair m < c: r(c, m) = -r(m, c) exactly in IEEE
+      // arithmetic; the diagonal stays 0 (zeroed once with the padding)
+      const int npair = n * (n - 1) / 2;
+      for (int pi = t; pi < npair; pi += BS_THREADS) {
+        int c = (int)((1.0f + sqrtf(1.0f + 8.0f * (float)pi)) * 0.5f);  // p = c(c-1)/2 + m
+        if (c * (c - 1) / 2 > pi) --c;
+        if ((c + 1) * c / 2 <= pi) ++c;
+        const int m = pi - c * (c - 1) / 2;
+        const double v = 1.0 / (Es[c] - Es[m]);
+        Rs[m + LD * c] = v;
+        Rs[c + LD * m] = -v;
+      }
     }
     if (doNjob)
       for (int idx = t; idx < nval * NMAX; idx += BS_THREADS) {
