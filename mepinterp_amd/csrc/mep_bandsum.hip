@@ -123,6 +123,12 @@ __global__ __launch_bounds__(BS_THREADS, 2) void bandsum_kernel(BandArgs b, doub
   const zd w2 = zmul(wq, wq);
 
   for (int64_t k = blockIdx.x; k < nk; k += gridDim.x) {
+    // the thread index is made opaque per k-point: otherwise the compiler
+    // hoists the lane-dependent indexes and masks of every unrolled loop out
+    // of this one and spills them (the wave index stays scalar)
+    int t = threadIdx.x;
+    asm volatile("" : "+v"(t));
+    const int lane = t & 63, wv = __builtin_amdgcn_readfirstlane(t >> 6);
     const double wk = w ? w[k] : 1.0;
     __syncthreads();
     // ---- stage V (planar [a][i + n j]) and E ------------------------------

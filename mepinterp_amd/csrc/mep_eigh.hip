@@ -183,19 +183,22 @@ __global__ __launch_bounds__(64, 4) void eigh_house_half_kernel(int n, int64_t n
   constexpr int NMAX = 32, HC = 16;
   __shared__ zd vv[NMAX], ww[NMAX], sub[NMAX];
   __shared__ double tauv[NMAX];
-  const int lane = threadIdx.x & 63;
-  const int t = lane & 31, h = lane >> 5;
   const double two_pi = 6.283185307179586476925;
   const zd zero = zd{0.0, 0.0};
-  // the value of lane ^ 32 (the other half of the same row)
-  auto other = [&](double v) {
-    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
-    const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)u, (unsigned)u, false, false);
-    const auto hi =
-        __builtin_amdgcn_permlane32_swap((unsigned)(u >> 32), (unsigned)(u >> 32), false, false);
-    return h == 0 ? join_d(lo[1], hi[1]) : join_d(lo[0], hi[0]);
-  };
   for (int64_t k = blockIdx.x; k < nk; k += gridDim.x) {
+    // the lane index is made opaque per k-point: otherwise the compiler
+    // hoists every step's lane masks out of this loop and spills them
+    int lane = threadIdx.x & 63;
+    asm volatile("" : "+v"(lane));
+    const int t = lane & 31, h = lane >> 5;
+    // the value of lane ^ 32 (the other half of the same row)
+    auto other = [&](double v) {
+      const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+      const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)u, (unsigned)u, false, false);
+      const auto hi =
+          __builtin_amdgcn_permlane32_swap((unsigned)(u >> 32), (unsigned)(u >> 32), false, false);
+      return h == 0 ? join_d(lo[1], hi[1]) : join_d(lo[0], hi[0]);
+    };
     // ---- row t, columns 16h + jj: upper triangle from H(k), lower as conjugates
     zd row[HC];
 #pragma unroll
