@@ -545,8 +545,8 @@ __global__ __launch_bounds__(32) void eigh_ql_wide_kernel(int n, int64_t nk,
 // the rotation stream is staged through LDS one team-width chunk at a time,
 // the next chunk's loads in flight while the current one is applied.  Then
 // U = Q P Z: lane j takes column j of P Z into registers, the reflectors are
-// staged into the Z space as full zero-padded columns, and applied in
-// reverse order without branches.
+// staged into the Z space eight at a time as full zero-padded columns (the
+// next eight loads in flight), and applied in reverse order without branches.
 template <int NMAX, class R>
 __global__ __launch_bounds__(64) void eigh_vec_kernel(int n, int64_t nk,
                                                       const R* __restrict__ side,
@@ -557,18 +557,24 @@ __global__ __launch_bounds__(64) void eigh_vec_kernel(int n, int64_t nk,
   using Z = typename cplx_of<R>::type;
   constexpr int TEAMS = 64 / NMAX;
   constexpr int LDZ = NMAX + 1;
-  // Zt[col*LDZ + row]; later the reflectors as full columns (NMAX-2) x NMAX
-  constexpr int ZS = (NMAX * LDZ > 2 * (NMAX - 2) * NMAX) ? NMAX * LDZ : 2 * (NMAX - 2) * NMAX;
+  // Zt[col*LDZ + row]; later RCH reflectors at a time as full columns,
+  // RCH x NMAX (18 KB per workgroup for n <= 32 in FP64: two wavefronts per
+  // SIMD fit in the LDS)
+  constexpr int RCH = 8;  // reflectors staged per chunk
+  constexpr int ZS = (NMAX * LDZ > 2 * RCH * NMAX) ? NMAX * LDZ : 2 * RCH * NMAX;
   __shared__ R Zt_s[TEAMS][ZS];
   __shared__ R cs_s[TEAMS][2 * NMAX];
   __shared__ int ix_s[TEAMS][NMAX];
-  const int lane = threadIdx.x;
-  const int tm = lane / NMAX, t = lane % NMAX;
-  R* Zt = Zt_s[tm];
-  R* csb = cs_s[tm];
-  int* ixb = ix_s[tm];
   const int64_t cap = rot_cap(n);
   for (int64_t kb = (int64_t)blockIdx.x * TEAMS; kb < nk; kb += (int64_t)gridDim.x * TEAMS) {
+    // the lane index is made opaque per k-point: otherwise the compiler
+    // hoists lane-dependent values out of this loop into spill registers
+    int lane = threadIdx.x;
+    asm volatile("" : "+v"(lane));
+    const int tm = lane / NMAX, t = lane % NMAX;
+    R* Zt = Zt_s[tm];
+    R* csb = cs_s[tm];
+    int* ixb = ix_s[tm];
     const int64_t k = kb + tm;
     const bool valid = k < nk;
     const R* sd = side + (valid ? k : 0) * side_len(n);
@@ -719,48 +725,61 @@ __global__ __launch_bounds__(64) void eigh_vec_kernel(int n, int64_t nk,
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    // ---- reflectors as full columns (zero at and above the diagonal and
-    // beyond n) into the Z space, so the updates below run branch-free ----
+    // ---- reflectors, eight at a time, as full columns (zero at and above
+    // the diagonal and beyond n) in the Z space, so the updates run
+    // branch-free; applied last to first, the next eight loads in flight
     Z* Vf = reinterpret_cast<Z*>(Zt);
     const Z* Uk = U + (valid ? k : 0) * (int64_t)n * n;
-    if (valid) {
-      const int tc = t < n ? t : n - 1;
-      for (int k0 = 0; k0 + 2 < n; k0 += 8) {
-        Z vb[8];
+    const int tc = t < n ? t : n - 1;
+    const int nchunk = n > 2 ? (n - 2 + RCH - 1) / RCH : 0;
+    Z vb[RCH];
+    auto load_chunk = [&](int c) {
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          const int kc = k0 + u;
-          vb[u] = Uk[tc + (int64_t)n * (kc < n ? kc : n - 1)];
-        }
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          const int kc = k0 + u;
-          if (kc + 2 < n) Vf[kc * NMAX + t] = (t > kc && t < n) ? vb[u] : Z{R(0), R(0)};
-        }
+      for (int u = 0; u < RCH; ++u) {
+        const int kc = RCH * c + u;
+        vb[u] = Uk[tc + (int64_t)n * (kc < n ? kc : n - 1)];
       }
-    }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    if (valid) {
-      for (int kc = n - 3; kc >= 0; --kc) {
-        const Z* v = Vf + kc * NMAX;
-        const R tk = taus[kc];
-        // four interleaved partial sums of v^H W, fused multiply-adds
-        Z s4[4] = {Z{R(0), R(0)}, Z{R(0), R(0)}, Z{R(0), R(0)}, Z{R(0), R(0)}};
+    };
+    if (nchunk > 0) load_chunk(nchunk - 1);
+    for (int c = nchunk - 1; c >= 0; --c) {
+      // the previous chunk's (or the W set-up's) reads of the Z space are done
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
-        for (int i = 0; i < NMAX; ++i) {
-          const Z vi = v[i], wi = W[i];
-          Z& q = s4[i & 3];
-          q.re = fma(vi.re, wi.re, fma(vi.im, wi.im, q.re));
-          q.im = fma(vi.re, wi.im, fma(-vi.im, wi.re, q.im));
-        }
-        const Z sacc = zscale(zadd(zadd(s4[0], s4[1]), zadd(s4[2], s4[3])), tk);
+      for (int u = 0; u < RCH; ++u) {
+        const int kc = RCH * c + u;
+        Vf[u * NMAX + t] = (kc + 2 < n && t > kc && t < n) ? vb[u] : Z{R(0), R(0)};
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      if (c > 0) load_chunk(c - 1);
+      if (valid) {
+        const int ulast = (RCH * c + RCH - 1 < n - 3) ? RCH - 1 : n - 3 - RCH * c;
+        for (int u = ulast; u >= 0; --u) {
+          const int kc = RCH * c + u;
+          const Z* v = Vf + u * NMAX;
+          const R tk = taus[kc];
+          // four interleaved partial sums of v^H W, fused multiply-adds
+          Z s4[4] = {Z{R(0), R(0)}, Z{R(0), R(0)}, Z{R(0), R(0)}, Z{R(0), R(0)}};
 #pragma unroll
-        for (int i = 0; i < NMAX; ++i) {  // W -= v sacc
-          const Z vi = v[i];
-          W[i].re = fma(-vi.re, sacc.re, fma(vi.im, sacc.im, W[i].re));
-          W[i].im = fma(-vi.re, sacc.im, fma(-vi.im, sacc.re, W[i].im));
+          for (int i = 0; i < NMAX; ++i) {
+            const Z vi = v[i], wi = W[i];
+            Z& q = s4[i & 3];
+            q.re = fma(vi.re, wi.re, fma(vi.im, wi.im, q.re));
+            q.im = fma(vi.re, wi.im, fma(-vi.im, wi.re, q.im));
+          }
+          const Z sacc = zscale(zadd(zadd(s4[0], s4[1]), zadd(s4[2], s4[3])), tk);
+          // v is read again from LDS (a broadcast) rather than held beside W:
+          // holding both would double the registers and halve the waves
+          asm volatile("" ::: "memory");
+#pragma unroll
+          for (int i = 0; i < NMAX; ++i) {  // W -= v sacc
+            const Z vi = v[i];
+            W[i].re = fma(-vi.re, sacc.re, fma(vi.im, sacc.im, W[i].re));
+            W[i].im = fma(-vi.re, sacc.im, fma(-vi.im, sacc.re, W[i].im));
+          }
         }
       }
     }
