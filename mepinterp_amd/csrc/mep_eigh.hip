@@ -172,7 +172,10 @@ __global__ __launch_bounds__(64) void eigh_house_kernel(int n, int64_t nk,
 // SIMD).  Column kc lives in half kc / 16; its entries reach the other half
 // through a permlane swap.  v_j = w_j = 0 for j <= kc, so the product and the
 // rank-2 update run over all of a lane's columns without a branch.  Same
-// outputs as eigh_house_kernel.
+// outputs as eigh_house_kernel.  TAU: Wannier centres given (tight-binding
+// convention); without them the phase code is compiled out, which keeps the
+// load of the row within the register budget.
+template <bool TAU>
 __global__ __launch_bounds__(64, 4) void eigh_house_half_kernel(int n, int64_t nk,
                                                              const zd* __restrict__ Hk,
                                                              int64_t strideHk, int packed,
@@ -181,7 +184,7 @@ __global__ __launch_bounds__(64, 4) void eigh_house_half_kernel(int n, int64_t n
                                                              zd* __restrict__ U,
                                                              double* __restrict__ side) {
   constexpr int NMAX = 32, HC = 16;
-  __shared__ zd vv[NMAX], ww[NMAX], sub[NMAX];
+  __shared__ zd vv[NMAX], ww[NMAX], sub[NMAX], fph[NMAX];
   __shared__ double tauv[NMAX];
   const double two_pi = 6.283185307179586476925;
   const zd zero = zd{0.0, 0.0};
@@ -200,6 +203,24 @@ __global__ __launch_bounds__(64, 4) void eigh_house_half_kernel(int n, int64_t n
       return h == 0 ? join_d(lo[1], hi[1]) : join_d(lo[0], hi[0]);
     };
     // ---- row t, columns 16h + jj: upper triangle from H(k), lower as conjugates
+    // tight-binding convention: element (t, j) takes e^{2 pi i k.(tau_j - tau_t)}
+    // = f_j conj(f_t), f_j = e^{2 pi i k.tau_j}: one sincos per band, through LDS
+    zd ft = zd{1.0, 0.0};
+    if (TAU) {
+      if (h == 0) {
+        zd f = zd{1.0, 0.0};
+        if (t < n) {
+          const double ang = two_pi * (kpts[3 * k] * tau[3 * t] + kpts[3 * k + 1] * tau[3 * t + 1] +
+                                       kpts[3 * k + 2] * tau[3 * t + 2]);
+          sin_cos(ang, &f.im, &f.re);
+        }
+        fph[t] = f;
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      ft = zconj(fph[t]);
+    }
     zd row[HC];
 #pragma unroll
     for (int jj = 0; jj < HC; ++jj) {
@@ -209,15 +230,8 @@ __global__ __launch_bounds__(64, 4) void eigh_house_half_kernel(int n, int64_t n
         const int i0 = j >= t ? t : j, j0 = j >= t ? j : t;  // (i0 <= j0) element
         const int64_t o = packed ? i0 + ((int64_t)j0 * (j0 + 1)) / 2 : i0 + (int64_t)n * j0;
         x = Hk[k * strideHk + o];
-        if (tau && i0 != j0) {
-          const double ang = two_pi * (kpts[3 * k] * (tau[3 * j0] - tau[3 * i0]) +
-                                       kpts[3 * k + 1] * (tau[3 * j0 + 1] - tau[3 * i0 + 1]) +
-                                       kpts[3 * k + 2] * (tau[3 * j0 + 2] - tau[3 * i0 + 2]));
-          double sn, cs;
-          sin_cos(ang, &sn, &cs);
-          x = zmul(zd{cs, sn}, x);
-        }
         if (j < t) x = zconj(x);
+        if (TAU && j != t) x = zmul(x, zmul(fph[j], ft));
         if (j == t) x.im = 0.0;
       }
       row[jj] = x;
@@ -1182,10 +1196,14 @@ static void launch_eigh_t(hipStream_t s, int n, int64_t nk, const typename cplx_
   unsigned qgrid = (unsigned)(qb < 65536 ? qb : 65536);
   const unsigned hgrid = (unsigned)(nk < 65536 ? nk : 65536);  // one k per wavefront
 #define MEP_EIGH_LAUNCH(NM)                                                                      \
-  if constexpr (NM == 32 && sizeof(R) == 8)                                                              \
-    hipLaunchKernelGGL(eigh_house_half_kernel, dim3(hgrid), dim3(64), 0, s, n, nk, Hk, strideHk,  \
-                       packed, g.kpts, g.tau, U, side);                                          \
-  else                                                                                           \
+  if constexpr (NM == 32 && sizeof(R) == 8) {                                                    \
+    if (g.tau)                                                                                   \
+      hipLaunchKernelGGL(eigh_house_half_kernel<true>, dim3(hgrid), dim3(64), 0, s, n, nk, Hk,    \
+                         strideHk, packed, g.kpts, g.tau, U, side);                              \
+    else                                                                                         \
+      hipLaunchKernelGGL(eigh_house_half_kernel<false>, dim3(hgrid), dim3(64), 0, s, n, nk, Hk,   \
+                         strideHk, packed, g.kpts, g.tau, U, side);                              \
+  } else                                                                                         \
     hipLaunchKernelGGL((eigh_house_kernel<NM, R>), dim3(grid), dim3(64), 0, s, n, nk, Hk, strideHk, \
                        packed, g.kpts, g.tau, U, side);                                          \
   hipLaunchKernelGGL(zero_count_kernel, dim3(1), dim3(1), 0, s, count);                        \

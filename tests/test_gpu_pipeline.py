@@ -36,6 +36,7 @@ CASES = [
     (16, 2, True, False),
     (24, 1, True, True),
     (32, 2, False, False),
+    (32, 1, True, False),
     # 32 < n <= 128: the wide eigensolver and velocity kernels
     (40, 1, True, False),
     (64, 1, False, True),
