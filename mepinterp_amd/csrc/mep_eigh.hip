@@ -167,12 +167,14 @@ __global__ __launch_bounds__(64) void eigh_house_kernel(int n, int64_t nk,
 // rot[k][q] = (c, s), col[k][q] = i; the count goes to the side record.
 // Ascending eigenvalues go to E, each band's rank to the side record.
 // Householder tridiagonalisation in FP64 for 16 < n <= 32 with one k-point
-// per wavefront: lane (t, h) = (lane % 32, lane / 32) keeps columns
-// 16h .. 16h+15 of row t in registers (64 VGPRs, so four wavefronts share a
-// SIMD).  Column kc lives in half kc / 16; its entries reach the other half
-// through a permlane swap.  v_j = w_j = 0 for j <= kc, so the product and the
-// rank-2 update run over all of a lane's columns without a branch.  Same
-// outputs as eigh_house_kernel.  TAU: Wannier centres given (tight-binding
+// per wavefront: lane (t, h) = (lane % 32, lane / 32) keeps the columns
+// j = 2 jj + h (jj < 16) of row t in registers (64 VGPRs, so four wavefronts
+// share a SIMD).  Column kc lives in half kc % 2 at jj = kc / 2; its entries
+// reach the other half through a permlane swap.  v_j = w_j = 0 for j <= kc,
+// so at step kc the product and the rank-2 update skip every jj whose two
+// columns are both <= kc (known at compile time in the unrolled loop: about
+// half of the work) and run over the rest without a branch.  Same outputs as
+// eigh_house_kernel.  TAU: Wannier centres given (tight-binding
 // convention); without them the phase code is compiled out, which keeps the
 // load of the row within the register budget.
 template <bool TAU>
@@ -224,7 +226,7 @@ __global__ __launch_bounds__(64, 4) void eigh_house_half_kernel(int n, int64_t n
     zd row[HC];
 #pragma unroll
     for (int jj = 0; jj < HC; ++jj) {
-      const int j = HC * h + jj;
+      const int j = 2 * jj + h;
       zd x = zero;
       if (t < n && j < n) {
         const int i0 = j >= t ? t : j, j0 = j >= t ? j : t;  // (i0 <= j0) element
@@ -239,7 +241,8 @@ __global__ __launch_bounds__(64, 4) void eigh_house_half_kernel(int n, int64_t n
 #pragma clang loop unroll(full)
     for (int kc = 0; kc < NMAX - 2; ++kc) {
       if (kc + 2 >= n) continue;  // uniform; every index below stays compile-time
-      const int hk = kc / HC, jc = kc % HC;
+      const int hk = kc % 2, jc = kc / 2;
+      const int jj0 = (kc + 1) / 2;  // jj < jj0: columns 2 jj, 2 jj + 1 <= kc
       const bool mine = h == hk;   // this lane holds column kc
       const bool below = t > kc;   // rows kc+1 .. n-1 take part
       const zd xo = mine ? row[jc] : zero;
@@ -267,8 +270,8 @@ __global__ __launch_bounds__(64, 4) void eigh_house_half_kernel(int n, int64_t n
       // p_t = tk sum_j A(t, j) v_j: this half's columns, then the other's
       zd p4[4] = {zero, zero, zero, zero};
 #pragma unroll
-      for (int jj = 0; jj < HC; ++jj) {
-        const zd a_ = row[jj], b_ = vv[HC * h + jj];
+      for (int jj = jj0; jj < HC; ++jj) {
+        const zd a_ = row[jj], b_ = vv[2 * jj + h];
         zd& q = p4[jj & 3];
         q.re = fma(a_.re, b_.re, fma(-a_.im, b_.im, q.re));
         q.im = fma(a_.re, b_.im, fma(a_.im, b_.re, q.im));
@@ -285,9 +288,9 @@ __global__ __launch_bounds__(64, 4) void eigh_house_half_kernel(int n, int64_t n
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       if (refl && below) {
 #pragma unroll
-        for (int jj = 0; jj < HC; ++jj) {
+        for (int jj = jj0; jj < HC; ++jj) {
           // row_j -= v conj(w_j) + w conj(v_j)  (zero for j <= kc)
-          const zd vj = vv[HC * h + jj], wj = ww[HC * h + jj];
+          const zd vj = vv[2 * jj + h], wj = ww[2 * jj + h];
           zd r_ = row[jj];
           r_.re = fma(-v.re, wj.re, fma(-v.im, wj.im, fma(-w.re, vj.re, fma(-w.im, vj.im, r_.re))));
           r_.im = fma(-v.im, wj.re, fma(v.re, wj.im, fma(-w.im, vj.re, fma(w.re, vj.im, r_.im))));
@@ -304,18 +307,18 @@ __global__ __launch_bounds__(64, 4) void eigh_house_half_kernel(int n, int64_t n
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
     // ---- outputs -----------------------------------------------------------
-    double dgl = 0.0;  // diagonal of row t, held by half t / 16
+    double dgl = 0.0;  // diagonal of row t, held by half t % 2
     zd last = zero;    // A(n-1, n-2)
 #pragma unroll
     for (int jj = 0; jj < HC; ++jj) {
-      if (HC * h + jj == t) dgl = row[jj].re;
-      if (HC * h + jj == n - 2) last = row[jj];
+      if (2 * jj + h == t) dgl = row[jj].re;
+      if (2 * jj + h == n - 2) last = row[jj];
     }
-    const int hd = t / HC;
+    const int hd = t % 2;
     const double dglo = other(dgl);
     dgl = (h == hd) ? dgl : dglo;
     {
-      const int src = 32 * ((n - 2) / HC) + n - 1;
+      const int src = 32 * ((n - 2) % 2) + n - 1;
       const zd l2 = zd{readlane_d(last.re, src), readlane_d(last.im, src)};
       if (lane == 0) {
         sub[n - 2] = l2;
@@ -330,7 +333,7 @@ __global__ __launch_bounds__(64, 4) void eigh_house_half_kernel(int n, int64_t n
     // reflectors: row t keeps v_t of every column kc < t
 #pragma unroll
     for (int jj = 0; jj < HC; ++jj) {
-      const int kc = HC * h + jj;
+      const int kc = 2 * jj + h;
       if (kc < t && t < n) Uk[t + (int64_t)n * kc] = row[jj];
     }
     if (h == 0 && t < n) {
