@@ -758,7 +758,11 @@ __global__ __launch_bounds__(64) void eigh_vec_kernel(int n, int64_t nk,
       }
     };
     if (nchunk > 0) load_chunk(nchunk - 1);
-    for (int c = nchunk - 1; c >= 0; --c) {
+    // chunk and reflector indexes are compile-time (full unroll), so each
+    // reflector's products start below its diagonal: v_i = 0 for i <= kc
+#pragma clang loop unroll(full)
+    for (int c = (NMAX - 2 + RCH - 1) / RCH - 1; c >= 0; --c) {
+      if (c >= nchunk) continue;  // uniform
       // the previous chunk's (or the W set-up's) reads of the Z space are done
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
@@ -773,15 +777,17 @@ __global__ __launch_bounds__(64) void eigh_vec_kernel(int n, int64_t nk,
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       if (c > 0) load_chunk(c - 1);
       if (valid) {
-        const int ulast = (RCH * c + RCH - 1 < n - 3) ? RCH - 1 : n - 3 - RCH * c;
-        for (int u = ulast; u >= 0; --u) {
+#pragma clang loop unroll(full)
+        for (int u = RCH - 1; u >= 0; --u) {
           const int kc = RCH * c + u;
+          if (kc + 2 >= n || kc + 2 >= NMAX) continue;  // uniform
           const Z* v = Vf + u * NMAX;
           const R tk = taus[kc];
           // four interleaved partial sums of v^H W, fused multiply-adds
           Z s4[4] = {Z{R(0), R(0)}, Z{R(0), R(0)}, Z{R(0), R(0)}, Z{R(0), R(0)}};
 #pragma unroll
           for (int i = 0; i < NMAX; ++i) {
+            if (i <= kc) continue;
             const Z vi = v[i], wi = W[i];
             Z& q = s4[i & 3];
             q.re = fma(vi.re, wi.re, fma(vi.im, wi.im, q.re));
@@ -793,10 +799,12 @@ __global__ __launch_bounds__(64) void eigh_vec_kernel(int n, int64_t nk,
           asm volatile("" ::: "memory");
 #pragma unroll
           for (int i = 0; i < NMAX; ++i) {  // W -= v sacc
+            if (i <= kc) continue;
             const Z vi = v[i];
             W[i].re = fma(-vi.re, sacc.re, fma(vi.im, sacc.im, W[i].re));
             W[i].im = fma(-vi.re, sacc.im, fma(-vi.im, sacc.re, W[i].im));
           }
+          asm volatile("" ::: "memory");  // the next reflector's loads wait
         }
       }
     }
