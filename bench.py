@@ -34,7 +34,7 @@ PEAK_HBM_GBS = 8000.0
 # counters' readings for a plain 1 GiB copy (x2.0 / x1.0)
 # (profiles/kspace_n32_64cubed_fp64.md).
 SEP_XY_TRAFFIC_BYTES_PER_K = (0.881e9 + 8.862e9) / 262144.0
-BANDSUM_TRAFFIC_BYTES_PER_K = (14.299e9 + 0.047e9) / 262144.0
+BANDSUM_TRAFFIC_BYTES_PER_K = (12.957e9 + 0.004e9) / 262144.0
 
 
 def log(*a):
