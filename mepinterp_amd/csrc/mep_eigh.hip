@@ -247,22 +247,22 @@ __global__ __launch_bounds__(64, 4) void eigh_house_half_kernel(int n, int64_t n
       const bool below = t > kc;   // rows kc+1 .. n-1 take part
       const zd xo = mine ? row[jc] : zero;
       const double tail2 = team_sum<32, double>((mine && t > kc + 1) ? zabs2(xo) : 0.0);
-      // cross-lane reads stay outside conditionals: every lane takes part
-      const double tail2o = other(tail2);
-      const double tail = h == hk ? tail2 : tail2o;  // the owning half's sum
+      // cross-lane reads stay outside conditionals: every lane takes part.
+      // tail2 and xo are exactly zero in the half that does not own column
+      // kc, so the sums over both halves are the owning half's values
+      const double tail = swap32_sum(tail2, tail2);
       const zd x0 = zd{readlane_d(xo.re, 32 * hk + kc + 1), readlane_d(xo.im, 32 * hk + kc + 1)};
-      const zd xoo = zd{other(xo.re), other(xo.im)};
-      const zd xt = mine ? xo : xoo;  // x_t in both halves
+      const zd xt = zd{swap32_sum(xo.re, xo.re), swap32_sum(xo.im, xo.im)};  // x_t in both halves
       const bool refl = tail > 0.0;
       const double ax02 = zabs2(x0);
       const double xn = sqrt(tail + ax02);
       const double ax0 = sqrt(ax02);
-      const zd alpha = !refl ? x0 : ((ax0 > 0.0) ? zscale(x0, -xn / ax0) : zd{-xn, 0.0});
+      const zd alpha = !refl ? x0 : ((ax0 > 0.0) ? zscale(x0, -xn * rcp_nr(ax0)) : zd{-xn, 0.0});
       zd v = (refl && below) ? xt : zero;
       if (t == kc + 1 && refl) v = zsub(xt, alpha);
       // |v|^2 = |x|^2 - 2 Re(conj(alpha) x0) + |alpha|^2 = 2 xn (xn + |x0|)
       const double vn2 = 2.0 * xn * (xn + ax0);
-      const double tk = refl ? 2.0 / vn2 : 0.0;
+      const double tk = refl ? 2.0 * rcp_nr(vn2) : 0.0;
       if (h == 0) vv[t] = v;
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
@@ -272,13 +272,17 @@ __global__ __launch_bounds__(64, 4) void eigh_house_half_kernel(int n, int64_t n
 #pragma unroll
       for (int jj = jj0; jj < HC; ++jj) {
         const zd a_ = row[jj], b_ = vv[2 * jj + h];
-        zd& q = p4[jj & 3];
-        q.re = fma(a_.re, b_.re, fma(-a_.im, b_.im, q.re));
-        q.im = fma(a_.re, b_.im, fma(a_.im, b_.re, q.im));
+        zd& q = p4[(jj - jj0) & 3];
+        if (jj - jj0 < 4) {  // the first product of each chain sets it
+          q.re = fma(a_.re, b_.re, -a_.im * b_.im);
+          q.im = fma(a_.re, b_.im, a_.im * b_.re);
+        } else {
+          q.re = fma(a_.re, b_.re, fma(-a_.im, b_.im, q.re));
+          q.im = fma(a_.re, b_.im, fma(a_.im, b_.re, q.im));
+        }
       }
       zd p = zadd(zadd(p4[0], p4[1]), zadd(p4[2], p4[3]));
-      const zd po = zd{other(p.re), other(p.im)};
-      p = (h == 0) ? zadd(p, po) : zadd(po, p);  // the same order in both halves
+      p = zd{swap32_sum(p.re, p.re), swap32_sum(p.im, p.im)};  // the same order in both halves
       p = below ? zscale(p, tk) : zero;
       const double K = 0.5 * tk * team_sum<32, double>(v.re * p.re + v.im * p.im);
       const zd w = zsub(p, zscale(v, K));
