@@ -70,6 +70,14 @@ __device__ inline double rcp_nr(double x) {
   r = fma(r, fma(-x, r, 1.0), r);
   return r;
 }
+// 1/sqrt(x) for normal x > 0: the hardware estimate and two Newton steps
+__device__ inline double rsq_nr(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  double e = fma(-x * y, y, 1.0);
+  y = fma(0.5 * y, e, y);
+  e = fma(-x * y, y, 1.0);
+  return fma(0.5 * y, e, y);
+}
 __device__ inline float rcp_nr(float x) {
   float r = __builtin_amdgcn_rcpf(x);
   r = fmaf(r, fmaf(-x, r, 1.0f), r);

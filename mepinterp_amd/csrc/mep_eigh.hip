@@ -412,7 +412,15 @@ __device__ __attribute__((always_inline)) inline void ql_one(int n, int64_t k, R
         const R ein = (i > l) ? es[(i - 1) * LANES] : R(0);
         const R din = (i > l) ? ds[(i - 1) * LANES] : R(0);
         const R f = s * ei, b = c * ei;
-        r = sqrt(f * f + g * g);
+        const R r2 = f * f + g * g;
+        R rr;
+        if constexpr (sizeof(R) == 8) {
+          // r and 1/r from one reciprocal square root: a shorter serial chain
+          rr = rsq_nr(r2);
+          r = (r2 > R(0)) ? r2 * rr : R(0);
+        } else {
+          r = sqrt(r2);
+        }
         es[(i + 1) * LANES] = r;
         if (r == R(0)) {
           ds[(i + 1) * LANES] = di1 - p;
@@ -420,7 +428,7 @@ __device__ __attribute__((always_inline)) inline void ql_one(int n, int64_t k, R
           underflow = true;
           break;
         }
-        const R rr = R(1) / r;
+        if constexpr (sizeof(R) != 8) rr = R(1) / r;
         s = f * rr;
         c = g * rr;
         g = di1 - p;
