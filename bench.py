@@ -127,9 +127,18 @@ def main():
     bs_flop_k = 48.0 * n ** 3                   # 3 products C_j = V_k B_l - V_l B_k (2 each)
     bs_bytes_k = 48.0 * n * n + 8.0 * n         # V (c128) + E (f64) read once
     velo_flop_k = 48.0 * n ** 3                 # U^H (dH_a U) for a = x, y, z
+    # 17 <= n <= 32: the band-sum kernel forms V itself (no velocity launch);
+    # it then reads dH_a (packed) and U instead of V
+    fused = split["velo"][1] == 0
+    if fused:
+        bs_flop_k += velo_flop_k
+        bs_bytes_k = 16.0 * 3 * n_up + 16.0 * n * n + 8.0 * n
     stages = {}
     for name, fl in (("velo", velo_flop_k), ("bandsum", bs_flop_k)):
         ms, cnt = split[name]
+        if cnt == 0:
+            stages[name] = {"fused_into": "bandsum_kernel<32, true>"}
+            continue
         ach = fl * nk / (ms * 1e-3) / 1e12
         stages[name] = {"bound": "mfma", "flop_per_k": fl, "achieved_tflops": ach,
                         "frac": ach / PEAK_FP64_TFLOPS, "launch_ms": ms / max(cnt, 1)}
@@ -154,7 +163,8 @@ def main():
     else:
         roofline = {"bound": "mfma", "achieved": d["achieved_tflops"], "peak": PEAK_FP64_TFLOPS,
                     "unit": "TFLOP/s", "frac": d["frac"], "traffic": d.get("traffic_per_launch"),
-                    "kernel": {"velo": "velo_mfma_kernel<zd>", "bandsum": "bandsum_kernel<32>"}[dom],
+                    "kernel": {"velo": "velo_mfma_kernel<zd>",
+                               "bandsum": "bandsum_kernel<32, %s>" % ("true" if fused else "false")}[dom],
                     "launch_ms": d["launch_ms"], "stages": stages}
     out = host.finalize(sums, p, mp, m["a_latt"])
     # the mixed-precision path on the same mesh: throughput and error per

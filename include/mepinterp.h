@@ -39,6 +39,13 @@ extern "C" {
                                     differs with a position operator)       */
 #define MEP_F_APPLY_DEG 8        /* divide X(R) by deg(R); the reference reads
                                     deg but never applies it                */
+#define MEP_F_SEPARATE_VELO 16   /* runs (mep_run_mesh / _kpts / _batch): keep
+                                    the velocities in their own kernel, V
+                                    through HBM.  By default, for FP64,
+                                    17 <= n_wf <= 32, no position operator,
+                                    the Wannier convention and no optical /
+                                    gyrotropic jobs, the band-sum kernel forms
+                                    V itself (same arithmetic, same results) */
 
 /* band-sum jobs (mep_run_params.jobs) */
 #define MEP_JOB_AHC 1            /* jobs%do_ahc : kubo_ahc_tens, velo_ahc_tens, kubo_ohc_tens */

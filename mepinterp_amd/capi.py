@@ -19,6 +19,7 @@ MEP_F_GAUGE_TRAFO = 1
 MEP_F_CART_VELO = 2
 MEP_F_TEXTBOOK = 4
 MEP_F_APPLY_DEG = 8
+MEP_F_SEPARATE_VELO = 16
 MEP_JOB_AHC = 1
 MEP_JOB_KUBO = 2
 MEP_JOB_MEP = 4
@@ -134,7 +135,8 @@ class Engine:
     """
 
     def __init__(self, R_frac, H, r=None, wf_centers=None, a_latt=None, kubo_tol=1e-5,
-                 gauge_trafo=True, cart_velo=False, textbook=False, deg=None, apply_deg=False):
+                 gauge_trafo=True, cart_velo=False, textbook=False, deg=None, apply_deg=False,
+                 separate_velo=False):
         lib = load()
         self.R = _f(R_frac, np.float64)
         self.H = _f(H, np.complex128)
@@ -145,7 +147,8 @@ class Engine:
         self.nR = self.H.shape[2]
         self.pos = self.r is not None
         flags = (MEP_F_GAUGE_TRAFO if gauge_trafo else 0) | (MEP_F_CART_VELO if cart_velo else 0) \
-            | (MEP_F_TEXTBOOK if textbook else 0) | (MEP_F_APPLY_DEG if apply_deg else 0)
+            | (MEP_F_TEXTBOOK if textbook else 0) | (MEP_F_APPLY_DEG if apply_deg else 0) \
+            | (MEP_F_SEPARATE_VELO if separate_velo else 0)
         d = None if deg is None else np.ascontiguousarray(deg, dtype=np.int64)
         ctx = C.c_void_p()
         rc = lib.mep_ctx_create(self.n, self.nR, _p(self.R), _p(self.tau), _p(self.H), _p(self.r),

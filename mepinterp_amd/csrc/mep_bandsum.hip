@@ -76,7 +76,13 @@ struct BsLds {
 };
 
 
-template <int NMAX>
+// FUSED: V is not read from HBM but formed in the V planes from the packed
+// H(k) / dH_a(k) of the Fourier stage (Hk, per-k stride strideHk) and the
+// eigenvectors U: V_a = U^H dH_a U on FP64 MFMA, with the same products and
+// the same order as velo_mfma_kernel (so V, and every sum, is bit-identical
+// to the two-kernel path).  n in 17..32, no position operator, Wannier
+// convention.
+template <int NMAX, bool FUSED>
 __global__ __launch_bounds__(BS_THREADS, 2) void bandsum_kernel(BandArgs b, double Tsmear,
                                                                 AccLayout L, int64_t nk,
                                                                 const double* __restrict__ E,
@@ -84,9 +90,16 @@ __global__ __launch_bounds__(BS_THREADS, 2) void bandsum_kernel(BandArgs b, doub
                                                                 const zd* __restrict__ A,
                                                                 const zd* __restrict__ Om,
                                                                 const double* __restrict__ w,
-                                                                double* __restrict__ part) {
+                                                                double* __restrict__ part,
+                                                                const zd* __restrict__ Hk,
+                                                                int64_t strideHk,
+                                                                const zd* __restrict__ U) {
   using S_ = BsLds<NMAX>;
   constexpr int LD = S_::LD, PL = S_::PL, NT = S_::NT;
+  // U's plane (PL complex) lives over the per-k tables between the r table
+  // and the accumulators while V is formed
+  static_assert(!FUSED || (NMAX == 32 && PL + NMAX + NT * 9 * NMAX + NMAX + 18 * NMAX + 96 >= 2 * PL),
+                "fused velocities: U's plane must fit below the accumulators");
   extern __shared__ double smem[];
   const int n = b.n, nef = b.n_ef, nhw = b.n_hw, nval = b.n_val;
   zd* Vs = reinterpret_cast<zd*>(smem);
@@ -132,6 +145,117 @@ __global__ __launch_bounds__(BS_THREADS, 2) void bandsum_kernel(BandArgs b, doub
     const double wk = w ? w[k] : 1.0;
     __syncthreads();
     // ---- stage V (planar [a][i + n j]) and E ------------------------------
+    if constexpr (FUSED) {
+      // V_a = U^H (dH_a U): U over the tables' region, dH_a (both halves of
+      // the packed upper triangle) in the V planes, T_a = dH_a U written over
+      // dH_a, then V_a from the accumulators into the V planes.  Rows and
+      // columns >= n stay zero in every plane, so no operand load is guarded.
+      zd* Us = reinterpret_cast<zd*>(Rs);
+      for (int c = t; c < PL; c += BS_THREADS) Us[c] = zd{0.0, 0.0};
+      constexpr int NU = (NMAX * NMAX + BS_THREADS - 1) / BS_THREADS;
+      constexpr int ND = (NMAX * (NMAX + 1) / 2 + BS_THREADS - 1) / BS_THREADS;
+      const int np = n * (n + 1) / 2;
+      const zd* Uk = U + k * (int64_t)n2;
+      const zd* X = Hk + k * strideHk;
+      zd ut[NU], dt[3][ND];
+      int pi[ND], pj[ND];
+#pragma unroll
+      for (int u = 0; u < NU; ++u) {
+        const int idx = t + BS_THREADS * u;
+        ut[u] = (idx < n2) ? Uk[idx] : zd{0.0, 0.0};
+      }
+#pragma unroll
+      for (int u = 0; u < ND; ++u) {
+        // element p of the packed upper triangle: (i, j), i <= j, p = i + j(j+1)/2
+        const int p = t + BS_THREADS * u;
+        int j = (int)((sqrtf(8.0f * (float)p + 1.0f) - 1.0f) * 0.5f);
+        if ((j + 1) * (j + 2) / 2 <= p) ++j;
+        if (j * (j + 1) / 2 > p) --j;
+        pj[u] = j;
+        pi[u] = p - j * (j + 1) / 2;
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+          dt[a][u] = (p < np) ? X[(int64_t)(1 + a) * np + p] : zd{0.0, 0.0};
+      }
+      __syncthreads();  // U's plane is zero
+#pragma unroll
+      for (int u = 0; u < NU; ++u) {
+        const int idx = t + BS_THREADS * u;
+        if (idx < n2) Us[idx % n + LD * (idx / n)] = ut[u];
+      }
+#pragma unroll
+      for (int u = 0; u < ND; ++u) {
+        const int p = t + BS_THREADS * u;
+        if (p >= np) continue;
+        const int i = pi[u], j = pj[u];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          Vs[a * PL + j + LD * i] = zconj(dt[a][u]);
+          Vs[a * PL + i + LD * j] = dt[a][u];  // last: the diagonal keeps the value as stored
+        }
+      }
+      __syncthreads();
+      // the 4 waves own the 2 x 2 output tiles of 16 x 16 (n > 16: all busy)
+      const int r0 = 16 * (wv & 1), c0 = 16 * (wv >> 1);
+      const int ar = r0 + (lane & 15), bc = c0 + (lane & 15), kq = lane >> 4;
+      mep_v4d q1[3], q2[3], q3[3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) q1[a] = q2[a] = q3[a] = mep_v4d{0, 0, 0, 0};
+#pragma unroll
+      for (int k0 = 0; k0 < NMAX; k0 += 4) {
+        if (k0 >= n) break;
+        const int m = k0 + kq;
+        const zd bu = Us[m + LD * bc];
+        const double bsum = bu.re + bu.im;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          const zd x = Vs[a * PL + ar + LD * m];
+          q1[a] = __builtin_amdgcn_mfma_f64_16x16x4f64(x.re, bu.re, q1[a], 0, 0, 0);
+          q2[a] = __builtin_amdgcn_mfma_f64_16x16x4f64(x.im, bu.im, q2[a], 0, 0, 0);
+          q3[a] = __builtin_amdgcn_mfma_f64_16x16x4f64(x.re + x.im, bsum, q3[a], 0, 0, 0);
+        }
+      }
+      __syncthreads();  // every wave has read dH
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int row = r0 + kq + 4 * v;
+          Vs[a * PL + row + LD * bc] = zd{q1[a][v] - q2[a][v], q3[a][v] - q1[a][v] - q2[a][v]};
+        }
+      __syncthreads();
+#pragma unroll
+      for (int a = 0; a < 3; ++a) q1[a] = q2[a] = q3[a] = mep_v4d{0, 0, 0, 0};
+#pragma unroll
+      for (int k0 = 0; k0 < NMAX; k0 += 4) {
+        if (k0 >= n) break;
+        const int m = k0 + kq;
+        const zd uu = Us[m + LD * ar];  // A(row, m) = conj(U(m, row))
+        const double us = uu.re - uu.im;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          const zd y = Vs[a * PL + m + LD * bc];
+          q1[a] = __builtin_amdgcn_mfma_f64_16x16x4f64(uu.re, y.re, q1[a], 0, 0, 0);
+          q2[a] = __builtin_amdgcn_mfma_f64_16x16x4f64(-uu.im, y.im, q2[a], 0, 0, 0);
+          q3[a] = __builtin_amdgcn_mfma_f64_16x16x4f64(us, y.re + y.im, q3[a], 0, 0, 0);
+        }
+      }
+      __syncthreads();  // every wave has read T and U
+      // V (zero in rows and columns >= n, as T and U are) into the planes
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int row = r0 + kq + 4 * v;
+          Vs[a * PL + row + LD * bc] = zd{q1[a][v] - q2[a][v], q3[a][v] - q1[a][v] - q2[a][v]};
+        }
+      // the r table, E and the occupations are zero again, E in its slot
+      const int zn = PL + NMAX + nef * NMAX;
+      for (int c = t; c < zn; c += BS_THREADS) {
+        const int e = c - PL;
+        Rs[c] = (e >= 0 && e < n) ? E[k * n + e] : 0.0;
+      }
+    } else {
     const zd* Vk = V + k * 3 * (int64_t)n2;
     {
       // every load of the k-point is issued before the first LDS store
@@ -154,6 +278,7 @@ __global__ __launch_bounds__(BS_THREADS, 2) void bandsum_kernel(BandArgs b, doub
           if (i < n && j < n) Vs[a * PL + i + LD * j] = vt[a][jj];
         }
       if (t < n) Es[t] = et;
+    }
     }
     __syncthreads();
     // ---- r(m, c) = 1 / (E_c - E_m); occupations ---------------------------
@@ -1332,15 +1457,16 @@ __global__ void reduce_parts_kernel(int nblocks, int64_t len, const double* __re
   acc[c] += s;
 }
 
-template <int NMAX>
+template <int NMAX, bool FUSED>
 static void launch_bs(hipStream_t s, const BandArgs& b, double Tsmear, const AccLayout& L,
                       int64_t nk, const double* E, const zd* V, const zd* A, const zd* Om,
-                      const double* w, int nblocks, double* part) {
+                      const double* w, int nblocks, double* part, const VeloFuse* f) {
   size_t lds = BsLds<NMAX>::bytes(b.n_ef, L.total);
-  (void)hipFuncSetAttribute((const void*)bandsum_kernel<NMAX>,
+  (void)hipFuncSetAttribute((const void*)bandsum_kernel<NMAX, FUSED>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  hipLaunchKernelGGL(bandsum_kernel<NMAX>, dim3(nblocks), dim3(BS_THREADS), lds, s, b, Tsmear, L,
-                     nk, E, V, A, Om, w, part);
+  hipLaunchKernelGGL((bandsum_kernel<NMAX, FUSED>), dim3(nblocks), dim3(BS_THREADS), lds, s, b,
+                     Tsmear, L, nk, E, V, A, Om, w, part, f ? f->Hk : nullptr,
+                     f ? f->strideHk : (int64_t)0, f ? f->U : nullptr);
 }
 
 size_t bandsum_lds_bytes(const BandArgs& b) {
@@ -1352,8 +1478,11 @@ size_t bandsum_lds_bytes(const BandArgs& b) {
   return b.n <= 16 ? BsLds<16>::bytes(b.n_ef, L.total) : BsLds<32>::bytes(b.n_ef, L.total);
 }
 
+bool bandsum_can_fuse(const BandArgs& b) { return b.n > 16 && b.n <= 32 && b.v_herm && !b.pos_op; }
+
 void launch_bandsum(hipStream_t s, const BandArgs& b, int64_t nk, const double* E, const zd* V,
-                    const zd* A, const zd* Om, const double* w, int nblocks, double* part) {
+                    const zd* A, const zd* Om, const double* w, int nblocks, double* part,
+                    const VeloFuse* fuse) {
   AccLayout L = acc_layout(b);
   // kBoltz_Eh_K * T  (constants.f90:22, statistics.f90:77)
   const double kB = 8.6173303 * 1e-5 / 27.211385;
@@ -1379,9 +1508,11 @@ void launch_bandsum(hipStream_t s, const BandArgs& b, int64_t nk, const double* 
       }
     }
   } else if (b.n <= 16) {
-    launch_bs<16>(s, b, Tsmear, L, nk, E, V, A, Om, w, nblocks, part);
+    launch_bs<16, false>(s, b, Tsmear, L, nk, E, V, A, Om, w, nblocks, part, nullptr);
+  } else if (fuse) {
+    launch_bs<32, true>(s, b, Tsmear, L, nk, E, V, A, Om, w, nblocks, part, fuse);
   } else {
-    launch_bs<32>(s, b, Tsmear, L, nk, E, V, A, Om, w, nblocks, part);
+    launch_bs<32, false>(s, b, Tsmear, L, nk, E, V, A, Om, w, nblocks, part, nullptr);
   }
 }
 

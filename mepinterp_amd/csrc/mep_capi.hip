@@ -719,10 +719,17 @@ static int band_setup(mep_ctx* c, const mep_run_params* p, int64_t kblock, BandS
   return MEP_OK;
 }
 
-static int band_block(mep_ctx* c, BandSetup& bs, int64_t nb, const double* w) {
+// fuse: the band-sum kernel forms V itself from H(k)/dH(k) and U (the
+// velocity kernel is not launched and V never reaches HBM)
+static int band_block(mep_ctx* c, BandSetup& bs, int64_t nb, const double* w, bool fuse = false) {
   int nblk = (int)(nb < bs.nblocks ? nb : bs.nblocks);
+  VeloFuse vf;
+  vf.Hk = c->dHk;
+  vf.strideHk = c->ncols;
+  vf.U = c->dU;
   timed(c, K_BANDSUM, [&] {
-    launch_bandsum(c->st, bs.b, nb, c->dE, c->dV, c->dA, c->dOm, w, nblk, c->dpart);
+    launch_bandsum(c->st, bs.b, nb, c->dE, c->dV, c->dA, c->dOm, w, nblk, c->dpart,
+                   fuse ? &vf : nullptr);
     if (bs.ext)
       launch_extsum(c->st, bs.b, nb, c->dE, c->dV, c->dA, c->dOm, w, nblk,
                     c->dpart + (int64_t)bs.nblocks * bs.L.total);
@@ -901,8 +908,15 @@ static int run_block(mep_ctx* c, BandSetup& bs, int64_t k0, int64_t nb, const in
   }
   if (weights)
     HIPCHK(hipMemcpyAsync(c->dw, weights + k0, sizeof(double) * nb, hipMemcpyHostToDevice, c->st));
-  CHK(interp_block(c, nb, true, p->precision, mp, k0));
-  CHK(band_block(c, bs, nb, weights ? c->dw : nullptr));
+  // velocities inside the band-sum kernel where its conditions hold: FP64,
+  // 17 <= n <= 32, V = U^H dH U (gauge transformation, no position
+  // operator), Wannier convention, and no optical / gyrotropic sums (those
+  // read V from HBM)
+  const bool fuse = p->precision == MEP_PREC_FP64 && !bs.ext && c->packed && !c->tb &&
+                    (c->flags & MEP_F_GAUGE_TRAFO) && !(c->flags & MEP_F_SEPARATE_VELO) &&
+                    bandsum_can_fuse(bs.b);
+  CHK(interp_block(c, nb, !fuse, p->precision, mp, k0));
+  CHK(band_block(c, bs, nb, weights ? c->dw : nullptr, fuse));
   return MEP_OK;
 }
 

@@ -97,8 +97,19 @@ struct AccLayout {
   int64_t Ne, ahc, velo, ohc, kic, klc, kcs, nic, nlc, ncs, total;
 };
 AccLayout acc_layout(const BandArgs& b);
+// V formed inside the band-sum kernel instead of read from HBM: the packed
+// H(k) / dH_a(k) of the Fourier stage (per-k stride strideHk) and U.  Only
+// where bandsum_can_fuse(b) (17 <= n <= 32, V built here, no position
+// operator) and for the Wannier convention.
+struct VeloFuse {
+  const zd* Hk;
+  int64_t strideHk;
+  const zd* U;
+};
+bool bandsum_can_fuse(const BandArgs& b);
 void launch_bandsum(hipStream_t s, const BandArgs& b, int64_t nk, const double* E, const zd* V,
-                    const zd* A, const zd* Om, const double* w, int nblocks, double* part);
+                    const zd* A, const zd* Om, const double* w, int nblocks, double* part,
+                    const VeloFuse* fuse = nullptr);
 void launch_reduce_parts(hipStream_t s, int nblocks, int64_t len, const double* part, double* acc);
 
 // optical / photogalvanic / gyrotropic sums (MEP_JOB_OPT, MEP_JOB_GYRO):

@@ -5,7 +5,8 @@ module mep_capi
 
   integer(c_int), parameter :: MEP_OK = 0
   integer(c_int32_t), parameter :: MEP_F_GAUGE_TRAFO = 1, MEP_F_CART_VELO = 2, &
-                                   MEP_F_TEXTBOOK = 4, MEP_F_APPLY_DEG = 8
+                                   MEP_F_TEXTBOOK = 4, MEP_F_APPLY_DEG = 8, &
+                                   MEP_F_SEPARATE_VELO = 16
   integer(c_int32_t), parameter :: MEP_JOB_AHC = 1, MEP_JOB_KUBO = 2, MEP_JOB_MEP = 4, &
                                    MEP_JOB_OPT = 8, MEP_JOB_GYRO = 16
   integer(c_int32_t), parameter :: MEP_PREC_FP64 = 0, MEP_PREC_MIXED = 1

@@ -165,13 +165,18 @@ def test_bandsum_batch_matches_oracle(n, T, nhw, nef):
     _compare_sums(g, o)
 
 
-@pytest.mark.parametrize("name", ["rand16_tb", "c1_pos", "femn"])
+@pytest.mark.parametrize("name", ["rand16_tb", "rand24", "c1_pos", "femn"])
 def test_run_mesh_matches_oracle(name):
     import oracle as O
     from mepinterp_amd import capi, models
     if name == "rand16_tb":
         m = models.make_random(seed=3, n=16, L=1, tb=True)
         mp, T, nhw, nval = (6, 5, 4), 300.0, 3, 8
+    elif name == "rand24":
+        # 17 <= n <= 32 without centres: the velocities are formed inside
+        # the band-sum kernel
+        m = models.make_random(seed=5, n=24, L=1, tb=False)
+        mp, T, nhw, nval = (5, 4, 4), 300.0, 3, 12
     elif name == "c1_pos":
         m = models.make_c1(seed=1)
         mp, T, nhw, nval = (6, 6, 6), 300.0, 2, 4
@@ -361,3 +366,25 @@ def test_separable_mesh_sums_equal_the_kpoint_gemm(tb, kblock):
         a, b = sep[key], gem[key]
         scale = max(np.max(np.abs(b)), 1e-300)
         assert np.max(np.abs(a - b)) <= 1e-11 * scale, (key, np.max(np.abs(a - b)) / scale)
+
+
+@pytest.mark.parametrize("n,kblock", [(17, 0), (24, 37), (32, 0), (32, 100)])
+def test_fused_velocity_band_sums_equal_the_two_kernel_path(n, kblock):
+    """For 17 <= n <= 32 (no position operator, Wannier convention) the
+    band-sum kernel forms V = U^H dH U itself; MEP_F_SEPARATE_VELO keeps the
+    velocity kernel and V in HBM.  Same products in the same order: the sums
+    agree exactly, also with k-blocks that do not divide the mesh."""
+    from mepinterp_amd import capi, models
+    m = models.make_random(seed=40 + n, n=n, L=1, tb=False)
+    mp = (6, 5, 4)
+    p = _params(capi.MEP_JOB_AHC | capi.MEP_JOB_KUBO | capi.MEP_JOB_MEP, n_eF=3, eF_min=-0.02,
+                eF_max=0.02, n_hw=4, hw_min=0.0, hw_max=0.3, T_kelvin=300.0,
+                eta=0.1 / AU_TO_EV, valence_bands=n // 2)
+    out = []
+    for separate in (False, True):
+        with _engine(m, separate_velo=separate) as eng:
+            if kblock:
+                eng.set_kblock(kblock)
+            out.append(eng.run_mesh(mp, p))
+    for key in KEYS:
+        assert np.array_equal(out[0][key], out[1][key]), (key, _rel(out[0][key], out[1][key]))
