@@ -1448,13 +1448,29 @@ void launch_curv(hipStream_t s, int n, int64_t nk, const double* E, const zd* V,
                      T_kelvin > 1e-2 ? 1 : 0, out);
 }
 
-__global__ void reduce_parts_kernel(int nblocks, int64_t len, const double* __restrict__ part,
-                                    double* __restrict__ acc) {
-  int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (c >= len) return;
+// acc[c] += sum over the nblocks partial records of part[g][c].  A
+// workgroup takes 64 consecutive c (one per lane, coalesced rows); its 16
+// wavefronts take the records g = w, w + 16, ... in order, and the 16 wave
+// sums are added in wave order: a fixed summation order, so the result is
+// deterministic.
+constexpr int RP_WAVES = 16;
+__global__ __launch_bounds__(64 * RP_WAVES) void reduce_parts_kernel(int nblocks, int64_t len,
+                                                                    const double* __restrict__ part,
+                                                                    double* __restrict__ acc) {
+  __shared__ double ws[RP_WAVES][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int64_t c = blockIdx.x * (int64_t)64 + lane;
   double s = 0.0;
-  for (int g = 0; g < nblocks; ++g) s += part[(int64_t)g * len + c];
-  acc[c] += s;
+  if (c < len)
+    for (int g = w; g < nblocks; g += RP_WAVES) s += part[(int64_t)g * len + c];
+  ws[w][lane] = s;
+  __syncthreads();
+  if (w == 0 && c < len) {
+    double t = 0.0;
+#pragma unroll
+    for (int v = 0; v < RP_WAVES; ++v) t += ws[v][lane];
+    acc[c] += t;
+  }
 }
 
 template <int NMAX, bool FUSED>
@@ -1517,9 +1533,9 @@ void launch_bandsum(hipStream_t s, const BandArgs& b, int64_t nk, const double* 
 }
 
 void launch_reduce_parts(hipStream_t s, int nblocks, int64_t len, const double* part, double* acc) {
-  int64_t nb = (len + 255) / 256;
-  hipLaunchKernelGGL(reduce_parts_kernel, dim3((unsigned)nb), dim3(256), 0, s, nblocks, len, part,
-                     acc);
+  int64_t nb = (len + 63) / 64;
+  hipLaunchKernelGGL(reduce_parts_kernel, dim3((unsigned)nb), dim3(64 * RP_WAVES), 0, s, nblocks,
+                     len, part, acc);
 }
 
 __global__ void fill_zero_kernel(double* __restrict__ p, int64_t n) {
