@@ -34,7 +34,8 @@ PEAK_HBM_GBS = 8000.0
 # counters' readings for a plain 1 GiB copy (x2.0 / x1.0)
 # (profiles/kspace_n32_64cubed_fp64.md).
 SEP_XY_TRAFFIC_BYTES_PER_K = (0.881e9 + 8.862e9) / 262144.0
-BANDSUM_TRAFFIC_BYTES_PER_K = (12.957e9 + 0.004e9) / 262144.0
+BANDSUM_TRAFFIC_BYTES_PER_K = (12.957e9 + 0.004e9) / 262144.0        # V read from HBM
+BANDSUM_FUSED_TRAFFIC_BYTES_PER_K = (11.010e9 + 0.004e9) / 262144.0  # dH and U read
 
 
 def log(*a):
@@ -150,7 +151,8 @@ def main():
                          "traffic_per_launch": SEP_XY_TRAFFIC_BYTES_PER_K * k_per_launch}
     stages["bandsum"]["hbm_bytes_per_k"] = bs_bytes_k
     stages["bandsum"]["achieved_gbs"] = bs_bytes_k * nk / (split["bandsum"][0] * 1e-3) / 1e9
-    stages["bandsum"]["traffic_per_launch"] = BANDSUM_TRAFFIC_BYTES_PER_K * k_per_launch
+    stages["bandsum"]["traffic_per_launch"] = (
+        BANDSUM_FUSED_TRAFFIC_BYTES_PER_K if fused else BANDSUM_TRAFFIC_BYTES_PER_K) * k_per_launch
     stages["eigh"] = {"bound": "latency", "ms": split["eigh"][0]}
     # the dominant single kernel (eigh is three kernels)
     dom = max(("ft_gemm", "velo", "bandsum"), key=lambda nm: split[nm][0])
