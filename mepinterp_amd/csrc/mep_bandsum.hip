@@ -135,6 +135,32 @@ __global__ __launch_bounds__(BS_THREADS, 2) void bandsum_kernel(BandArgs b, doub
   const zd wq = zd{b.hw[0], b.eta};
   const zd w2 = zmul(wq, wq);
 
+  // FUSED: U and the packed dH_a of a k-point, loaded into registers during
+  // the last (light) phase of the previous k-point
+  constexpr int NU = (NMAX * NMAX + BS_THREADS - 1) / BS_THREADS;
+  constexpr int ND = (NMAX * (NMAX + 1) / 2 + BS_THREADS - 1) / BS_THREADS;
+  const int np = n * (n + 1) / 2;
+  zd ut[FUSED ? NU : 1], dt[3][FUSED ? ND : 1];
+  auto fetch = [&](int64_t kk) {
+    const int tt = threadIdx.x;
+    const bool ok = kk < nk;
+    const zd* Uk = U + (ok ? kk : 0) * (int64_t)n2;
+    const zd* X = Hk + (ok ? kk : 0) * strideHk;
+#pragma unroll
+    for (int u = 0; u < NU; ++u) {
+      const int idx = tt + BS_THREADS * u;
+      ut[u] = (ok && idx < n2) ? Uk[idx] : zd{0.0, 0.0};
+    }
+#pragma unroll
+    for (int u = 0; u < ND; ++u) {
+      const int p = tt + BS_THREADS * u;
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+        dt[a][u] = (ok && p < np) ? X[(int64_t)(1 + a) * np + p] : zd{0.0, 0.0};
+    }
+  };
+  if constexpr (FUSED) fetch(blockIdx.x);
+
   for (int64_t k = blockIdx.x; k < nk; k += gridDim.x) {
     // the thread index is made opaque per k-point: otherwise the compiler
     // hoists the lane-dependent indexes and masks of every unrolled loop out
@@ -152,31 +178,6 @@ __global__ __launch_bounds__(BS_THREADS, 2) void bandsum_kernel(BandArgs b, doub
       // columns >= n stay zero in every plane, so no operand load is guarded.
       zd* Us = reinterpret_cast<zd*>(Rs);
       for (int c = t; c < PL; c += BS_THREADS) Us[c] = zd{0.0, 0.0};
-      constexpr int NU = (NMAX * NMAX + BS_THREADS - 1) / BS_THREADS;
-      constexpr int ND = (NMAX * (NMAX + 1) / 2 + BS_THREADS - 1) / BS_THREADS;
-      const int np = n * (n + 1) / 2;
-      const zd* Uk = U + k * (int64_t)n2;
-      const zd* X = Hk + k * strideHk;
-      zd ut[NU], dt[3][ND];
-      int pi[ND], pj[ND];
-#pragma unroll
-      for (int u = 0; u < NU; ++u) {
-        const int idx = t + BS_THREADS * u;
-        ut[u] = (idx < n2) ? Uk[idx] : zd{0.0, 0.0};
-      }
-#pragma unroll
-      for (int u = 0; u < ND; ++u) {
-        // element p of the packed upper triangle: (i, j), i <= j, p = i + j(j+1)/2
-        const int p = t + BS_THREADS * u;
-        int j = (int)((sqrtf(8.0f * (float)p + 1.0f) - 1.0f) * 0.5f);
-        if ((j + 1) * (j + 2) / 2 <= p) ++j;
-        if (j * (j + 1) / 2 > p) --j;
-        pj[u] = j;
-        pi[u] = p - j * (j + 1) / 2;
-#pragma unroll
-        for (int a = 0; a < 3; ++a)
-          dt[a][u] = (p < np) ? X[(int64_t)(1 + a) * np + p] : zd{0.0, 0.0};
-      }
       __syncthreads();  // U's plane is zero
 #pragma unroll
       for (int u = 0; u < NU; ++u) {
@@ -187,7 +188,11 @@ __global__ __launch_bounds__(BS_THREADS, 2) void bandsum_kernel(BandArgs b, doub
       for (int u = 0; u < ND; ++u) {
         const int p = t + BS_THREADS * u;
         if (p >= np) continue;
-        const int i = pi[u], j = pj[u];
+        // element p of the packed upper triangle: (i, j), i <= j, p = i + j(j+1)/2
+        int j = (int)((sqrtf(8.0f * (float)p + 1.0f) - 1.0f) * 0.5f);
+        if ((j + 1) * (j + 2) / 2 <= p) ++j;
+        if (j * (j + 1) / 2 > p) --j;
+        const int i = p - j * (j + 1) / 2;
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
           Vs[a * PL + j + LD * i] = zconj(dt[a][u]);
@@ -783,6 +788,7 @@ __global__ __launch_bounds__(BS_THREADS, 2) void bandsum_kernel(BandArgs b, doub
       }
     }
     __syncthreads();
+    if constexpr (FUSED) fetch(k + gridDim.x);  // in flight during the sums below
     // ---- sums over the bands n0 and over the waves' partial records --------
     // Ne, kubo_ahc_tens (-sum f Omega) and kubo_mep_LC for every E_F
     // for every E_F; item (c, ef) is summed by 8 lanes, lane g taking the
