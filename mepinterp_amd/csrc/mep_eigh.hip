@@ -552,21 +552,43 @@ __global__ __launch_bounds__(64) void eigh_ql_kernel(int n, int64_t nk,
 __global__ void zero_count_kernel(int* count) { *count = 0; }
 
 // the same for 32 < n <= 128: 32 lanes per workgroup, d and e in dynamic LDS
-// (n x 32 each)
+// (n x 32 each); modes as eigh_ql_kernel (1: eigenvalues and the separation
+// verdict, listing the k-points that need rotation records; 2: the listed
+// k-points, recording)
 template <class R>
 __global__ __launch_bounds__(32) void eigh_ql_wide_kernel(int n, int64_t nk,
                                                           R* __restrict__ side,
                                                           R* __restrict__ rot,
                                                           uint8_t* __restrict__ rcol,
                                                           double* __restrict__ E,
-                                                          int* __restrict__ info) {
+                                                          int* __restrict__ info, int mode,
+                                                          int* __restrict__ list,
+                                                          int* __restrict__ count) {
   extern __shared__ double ql_smem[];
   R* ds = reinterpret_cast<R*>(ql_smem);
   R* es = ds + (int64_t)n * 32;
   const int lane = threadIdx.x;
+  if (mode == 2) {
+    const int64_t cnt = *count;
+    for (int64_t i = (int64_t)blockIdx.x * 32 + lane; i < cnt; i += (int64_t)gridDim.x * 32)
+      ql_one<32, R>(n, list[i], ds + lane, es + lane, side, rot, rcol, E, info, true);
+    return;
+  }
   for (int64_t kb = (int64_t)blockIdx.x * 32; kb < nk; kb += (int64_t)gridDim.x * 32) {
     const int64_t k = kb + lane;
-    if (k < nk) ql_one<32, R>(n, k, ds + lane, es + lane, side, rot, rcol, E, info);
+    bool again = false;
+    if (k < nk) {
+      ql_one<32, R>(n, k, ds + lane, es + lane, side, rot, rcol, E, info, false);
+      again = side[k * side_len(n) + 5 * n] == R(-2);
+    }
+    const unsigned long long m = __ballot(again);
+    if (m) {
+      const int leader = __ffsll((long long)m) - 1;
+      int base = 0;
+      if (lane == leader) base = atomicAdd(count, __popcll(m));
+      base = __shfl(base, leader, 64);
+      if (again) list[base + __popcll(m & ((1ull << lane) - 1))] = (int)k;
+    }
   }
 }
 
@@ -1003,14 +1025,19 @@ size_t eigh_house_wide_lds(int n) {
   return sizeof(zd) * ((size_t)n * (n + 1) / 2 + 8 * (size_t)n) + sizeof(double) * ((size_t)n + 4);
 }
 
-// Rotation replay for 32 < n <= 128: thread r <-> row r of Z (LDS,
-// column-major with odd pitch), each wave staging the rotation stream
-// through its own LDS slots; then W = P Z is written row-major into the
-// k-point's (consumed) rotation record.
+// Eigenvectors Z of the real tridiagonal for 32 < n <= 128, Z in LDS
+// (column-major with odd pitch).  Separated k-points (count slot -1): thread
+// j computes column j, the vector of lambda_j, from the twisted factorization
+// of T - lambda_j (as eigh_vec_kernel; its O(n) work arrays live in the
+// thread's own column of Z), and the ranks become the identity.  Otherwise
+// thread r <-> row r replays the recorded rotations, each wave staging the
+// stream through its own LDS slots.  Then W = P Z is written row-major into
+// the k-point's rotation record (consumed, or unused).
 __global__ __launch_bounds__(128) void eigh_replay_wide_kernel(int n, int64_t nk,
-                                                               const double* __restrict__ side,
+                                                               double* __restrict__ side,
                                                                double* __restrict__ rot,
-                                                               const uint8_t* __restrict__ rcol) {
+                                                               const uint8_t* __restrict__ rcol,
+                                                               const double* __restrict__ E) {
   extern __shared__ double vw_smem[];
   const int LDZ = n | 1;
   double* Zt = vw_smem;                       // [n][LDZ]
@@ -1022,7 +1049,72 @@ __global__ __launch_bounds__(128) void eigh_replay_wide_kernel(int n, int64_t nk
   const int64_t cap = rot_cap(n);
   for (int64_t k = blockIdx.x; k < nk; k += gridDim.x) {
     __syncthreads();
-    const double* sd = side + k * side_len(n);
+    double* sd = side + k * side_len(n);
+    if (sd[5 * n] < 0.0) {  // uniform over the workgroup
+      // d and e of the k-point in the staging slots (2n <= 256 doubles)
+      double* dl = csb_all;
+      double* el = csb_all + n;
+      for (int i = t; i < n; i += blockDim.x) {
+        dl[i] = sd[i];
+        el[i] = sd[n + i];
+      }
+      __syncthreads();
+      if (t < n) {
+        double* z = Zt + (int64_t)t * LDZ;  // column t: work array, then the vector
+        const double lam = E[k * n + t];
+        double tn = 0.0;
+        for (int i = 0; i < n; ++i) tn = fmax(tn, fabs(dl[i]) + 2.0 * fabs(el[i]));
+        const double pivmin = cplx_of<double>::eps * (tn > 0.0 ? tn : 1.0);
+        // dp_i = d_i - l - e_{i-1}^2 / dp_{i-1}  (top down), into z
+        double dd = 1.0;
+        for (int i = 0; i < n; ++i) {
+          double v = dl[i] - lam - ((i > 0) ? el[i - 1] * el[i - 1] / dd : 0.0);
+          if (fabs(v) < pivmin) v = (v < 0.0) ? -pivmin : pivmin;
+          dd = v;
+          z[i] = v;
+        }
+        // um_i = d_i - l - e_i^2 / um_{i+1} (bottom up); r = the first index
+        // of smallest |gamma_i| = |dp_i + um_i - (d_i - l)|
+        double u = 1.0, gmin = 0.0;
+        int r = n - 1;
+        for (int i = n - 1; i >= 0; --i) {
+          double v = dl[i] - lam - ((i < n - 1) ? el[i] * el[i] / u : 0.0);
+          if (fabs(v) < pivmin) v = (v < 0.0) ? -pivmin : pivmin;
+          u = v;
+          const double g = fabs(z[i] + v - (dl[i] - lam));
+          if (i == n - 1 || g <= gmin) {
+            gmin = g;
+            r = i;
+          }
+        }
+        // z_i = -e_i z_{i+1} / dp_i below r (dp_i still in z)
+        double zp = 1.0, nz = 1.0;
+        for (int i = r - 1; i >= 0; --i) {
+          zp = -(el[i] / z[i]) * zp;
+          z[i] = zp;
+          nz += zp * zp;
+        }
+        // um_i again above r (into z), then z_i = -e_{i-1} z_{i-1} / um_i
+        u = 1.0;
+        for (int i = n - 1; i > r; --i) {
+          double v = dl[i] - lam - ((i < n - 1) ? el[i] * el[i] / u : 0.0);
+          if (fabs(v) < pivmin) v = (v < 0.0) ? -pivmin : pivmin;
+          u = v;
+          z[i] = v;
+        }
+        zp = 1.0;
+        for (int i = r + 1; i < n; ++i) {
+          zp = -(el[i - 1] / z[i]) * zp;
+          z[i] = zp;
+          nz += zp * zp;
+        }
+        z[r] = 1.0;
+        const double inv = 1.0 / sqrt(nz);
+        for (int i = 0; i < n; ++i) z[i] *= inv;
+      }
+      __syncthreads();  // every thread has read e: its slot takes the ranks
+      if (t < n) sd[n + t] = (double)t;
+    } else {
     if (t < n)
       for (int c = 0; c < n; ++c) Zt[c * LDZ + t] = (c == t) ? 1.0 : 0.0;
     const int64_t Q = (int64_t)sd[5 * n];
@@ -1064,8 +1156,9 @@ __global__ __launch_bounds__(128) void eigh_replay_wide_kernel(int n, int64_t nk
           prev = i;
         }
     }
+    }
     __syncthreads();
-    // W = P Z, row-major in the consumed rotation record
+    // W = P Z, row-major in the rotation record
     zd* W = reinterpret_cast<zd*>(rot + k * 2 * cap);
     const int j = t;
     if (j < n)
@@ -1160,6 +1253,8 @@ __global__ __launch_bounds__(256) void eigh_back_wide_kernel(int n, int64_t nk, 
   }
 }
 
+static size_t eigh_list_offset(int n, int64_t nk);
+
 void launch_eigh_wide(hipStream_t s, int n, int64_t nk, const zd* Hk, int64_t strideHk, int packed,
                       const KGeom& g, double* E, zd* U, int* info, double* scratch) {
   double* side = scratch;
@@ -1180,13 +1275,18 @@ void launch_eigh_wide(hipStream_t s, int n, int64_t nk, const zd* Hk, int64_t st
   unsigned grid = (unsigned)(nk < 4096 ? nk : 4096);
   hipLaunchKernelGGL(eigh_house_wide_kernel, dim3(grid), dim3(256), lh, s, n, nk, Hk, strideHk,
                      packed, g.kpts, g.tau, U, side);
+  int* list = reinterpret_cast<int*>(reinterpret_cast<uint8_t*>(scratch) + eigh_list_offset(n, nk));
+  int* count = list + nk;
   int64_t qb = (nk + 31) / 32;
   unsigned qgrid = (unsigned)(qb < 65536 ? qb : 65536);
+  hipLaunchKernelGGL(zero_count_kernel, dim3(1), dim3(1), 0, s, count);
   hipLaunchKernelGGL(eigh_ql_wide_kernel<double>, dim3(qgrid), dim3(32), lq, s, n, nk, side, rot,
-                     rcol, E, info);
+                     rcol, E, info, 1, list, count);
+  hipLaunchKernelGGL(eigh_ql_wide_kernel<double>, dim3(qgrid), dim3(32), lq, s, n, nk, side, rot,
+                     rcol, E, info, 2, list, count);
   const int rthreads = n <= 64 ? 64 : 128;
   hipLaunchKernelGGL(eigh_replay_wide_kernel, dim3(grid), dim3(rthreads), lr, s, n, nk, side, rot,
-                     rcol);
+                     rcol, E);
   const int bthreads = ((2 * CB + 63) / 64) * 64;
   hipLaunchKernelGGL(eigh_back_wide_kernel, dim3(grid), dim3(bthreads), lb, s, n, nk, CB, side, rot,
                      U);
