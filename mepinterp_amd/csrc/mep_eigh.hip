@@ -1171,9 +1171,10 @@ size_t eigh_replay_wide_lds(int n) {
   return sizeof(double) * ((size_t)n * (n | 1) + 2 * 64 * 2) + sizeof(int) * 64 * 2;
 }
 
-// columns per LDS block of the back-transformation (about 76 KB of LDS)
+// columns per LDS block of the back-transformation (about 73 KB of LDS, so
+// that two workgroups fit a CU with the reflector buffers and tau)
 static inline int back_cb(int n) {
-  int cb = (76 * 1024) / (16 * (n + 1));
+  int cb = (73 * 1024) / (16 * (n + 1));
   return cb < n ? cb : n;
 }
 
@@ -1189,7 +1190,9 @@ __global__ __launch_bounds__(256) void eigh_back_wide_kernel(int n, int64_t nk, 
   extern __shared__ double bw_smem[];
   const int LDW = CB + 1;
   zd* Ws = reinterpret_cast<zd*>(bw_smem);  // [n][LDW]
-  const int t = threadIdx.x, jj = t >> 1, hf = t & 1;
+  zd* Vb = Ws + (int64_t)n * LDW;           // [2][n] reflector columns (double buffer)
+  double* taus = reinterpret_cast<double*>(Vb + 2 * n);  // [n]
+  const int t = threadIdx.x, jj = t >> 1, hf = t & 1;  // blockDim.x >= n
   const int64_t cap = rot_cap(n);
   for (int64_t k = blockIdx.x; k < nk; k += gridDim.x) {
     const double* sd = side + k * side_len(n);
@@ -1202,21 +1205,39 @@ __global__ __launch_bounds__(256) void eigh_back_wide_kernel(int n, int64_t nk, 
         const int i = idx / cbn, c = idx % cbn;
         Ws[i * LDW + c] = W[(int64_t)i * n + c0 + c];
       }
+      // tau in LDS; the reflector columns pass through LDS one at a time,
+      // the next one's load in flight while the current one is applied
+      if (t < n) {
+        taus[t] = sd[2 * n + t];
+        Vb[t] = (n >= 3) ? Uk[(int64_t)n * (n - 3) + t] : zd{0.0, 0.0};
+      }
       __syncthreads();
       const bool act = jj < cbn;
-      if (act)
-        for (int kc = n - 3; kc >= 0; --kc) {
-          const double tk = sd[2 * n + kc];
-          if (tk == 0.0) continue;
-          const zd* vcol = Uk + (int64_t)n * kc;
+      int cur = 0;
+      for (int kc = n - 3; kc >= 0; --kc) {
+        const zd nxt = (kc > 0 && t < n) ? Uk[(int64_t)n * (kc - 1) + t] : zd{0.0, 0.0};
+        const double tk = taus[kc];
+        if (act && tk != 0.0) {
+          const zd* vcol = Vb + cur * n;
           const int i0 = kc + 1 + hf;
-          zd sacc = zd{0.0, 0.0};
-          for (int i = i0; i < n; i += 2) sacc = zadd(sacc, zcmul(vcol[i], Ws[i * LDW + jj]));
+          zd s0 = zd{0.0, 0.0}, s1 = zd{0.0, 0.0};
+          int i = i0;
+          for (; i + 2 < n; i += 4) {
+            s0 = zadd(s0, zcmul(vcol[i], Ws[i * LDW + jj]));
+            s1 = zadd(s1, zcmul(vcol[i + 2], Ws[(i + 2) * LDW + jj]));
+          }
+          if (i < n) s0 = zadd(s0, zcmul(vcol[i], Ws[i * LDW + jj]));
+          zd sacc = zadd(s0, s1);
           sacc.re += __shfl_xor(sacc.re, 1, 64);
           sacc.im += __shfl_xor(sacc.im, 1, 64);
           sacc = zscale(sacc, tk);
-          for (int i = i0; i < n; i += 2) Ws[i * LDW + jj] = zsub(Ws[i * LDW + jj], zmul(vcol[i], sacc));
+          for (int i2 = i0; i2 < n; i2 += 2)
+            Ws[i2 * LDW + jj] = zsub(Ws[i2 * LDW + jj], zmul(vcol[i2], sacc));
         }
+        if (t < n) Vb[(cur ^ 1) * n + t] = nxt;
+        __syncthreads();
+        cur ^= 1;
+      }
       // gauge: largest-modulus component (first on ties) real > 0
       double best = -1.0;
       int bi = n;
@@ -1262,7 +1283,7 @@ void launch_eigh_wide(hipStream_t s, int n, int64_t nk, const zd* Hk, int64_t st
   uint8_t* rcol = reinterpret_cast<uint8_t*>(rot + nk * 2 * rot_cap(n));
   const int CB = back_cb(n);
   const size_t lh = eigh_house_wide_lds(n), lr = eigh_replay_wide_lds(n);
-  const size_t lb = sizeof(zd) * (size_t)n * (CB + 1);
+  const size_t lb = sizeof(zd) * ((size_t)n * (CB + 1) + 2 * (size_t)n) + sizeof(double) * (size_t)n;
   const size_t lq = sizeof(double) * 2 * (size_t)n * 32;
   (void)hipFuncSetAttribute((const void*)eigh_house_wide_kernel,
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lh);
@@ -1287,7 +1308,8 @@ void launch_eigh_wide(hipStream_t s, int n, int64_t nk, const zd* Hk, int64_t st
   const int rthreads = n <= 64 ? 64 : 128;
   hipLaunchKernelGGL(eigh_replay_wide_kernel, dim3(grid), dim3(rthreads), lr, s, n, nk, side, rot,
                      rcol, E);
-  const int bthreads = ((2 * CB + 63) / 64) * 64;
+  int bthreads = ((2 * CB + 63) / 64) * 64;
+  if (bthreads < n) bthreads = ((n + 63) / 64) * 64;  // one thread per reflector row
   hipLaunchKernelGGL(eigh_back_wide_kernel, dim3(grid), dim3(bthreads), lb, s, n, nk, CB, side, rot,
                      U);
 }
