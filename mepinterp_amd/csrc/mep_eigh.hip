@@ -885,7 +885,7 @@ __device__ inline int lo_off(int j, int n) { return j * n - (j * (j + 1)) / 2; }
 // block-wide sum of one double over 256 threads (every thread gets it);
 // red holds 4 doubles; the call ends with a barrier
 __device__ inline double block_sum256(double v, double* red) {
-  v = wave_sum(v);
+  v = rows4_sum(row16_sum(v));  // wave sum on DPP and permlane swaps
   __syncthreads();
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
   __syncthreads();
@@ -954,13 +954,33 @@ __global__ __launch_bounds__(256) void eigh_house_wide_kernel(int n, int64_t nk,
       const int base = kc + 1;
       for (int r = lane; r < m; r += 64) {
         const int row = base + r;
-        zd acc = zd{0.0, 0.0};
-        for (int c = wv; c < m; c += 4) {
-          const int cl = base + c;
-          const zd a = (row >= cl) ? Lw[lo_off(cl, n) + row] : zconj(Lw[lo_off(row, n) + cl]);
-          acc = zadd(acc, zmul(a, vv[c]));
+        const int lrow = lo_off(row, n);
+        // A(row, cl): L(row, cl) below the diagonal, conj L(cl, row) above;
+        // one load from a selected address (no branch), two chains
+        auto elem = [&](int cl) {
+          const bool lower = row >= cl;
+          zd a = Lw[lower ? lo_off(cl, n) + row : lrow + cl];
+          a.im = lower ? a.im : -a.im;
+          return a;
+        };
+        zd acc0 = zd{0.0, 0.0}, acc1 = zd{0.0, 0.0};  // c = wv + 8j, wv + 4 + 8j
+        zd acc2 = zd{0.0, 0.0}, acc3 = zd{0.0, 0.0};  // c = wv + 8 + 16j, wv + 12 + 16j
+        int c = wv;
+        for (; c + 12 < m; c += 16) {
+          const zd a = elem(base + c), a1 = elem(base + c + 4);
+          const zd a2 = elem(base + c + 8), a3 = elem(base + c + 12);
+          acc0 = zadd(acc0, zmul(a, vv[c]));
+          acc1 = zadd(acc1, zmul(a1, vv[c + 4]));
+          acc2 = zadd(acc2, zmul(a2, vv[c + 8]));
+          acc3 = zadd(acc3, zmul(a3, vv[c + 12]));
         }
-        pp[wv * n + r] = acc;
+        for (; c + 4 < m; c += 8) {
+          const zd a = elem(base + c), a1 = elem(base + c + 4);
+          acc0 = zadd(acc0, zmul(a, vv[c]));
+          acc1 = zadd(acc1, zmul(a1, vv[c + 4]));
+        }
+        if (c < m) acc0 = zadd(acc0, zmul(elem(base + c), vv[c]));
+        pp[wv * n + r] = zadd(zadd(acc0, acc1), zadd(acc2, acc3));
       }
       __syncthreads();
       zd p = zd{0.0, 0.0};
@@ -982,12 +1002,13 @@ __global__ __launch_bounds__(256) void eigh_house_wide_kernel(int n, int64_t nk,
           }
         }
       __syncthreads();
+      // no barrier after this: each thread rewrites only its own vv[t] next,
+      // and the next step reads columns > kc only
       if (t < m) col[t] = vv[t];
       if (t == 0) {
         sub[kc] = alpha;
         tauv[kc] = tk;
       }
-      __syncthreads();
     }
     if (t == 0 && n >= 2) {
       sub[n - 2] = Lw[lo_off(n - 2, n) + n - 1];
