@@ -70,7 +70,7 @@ def test_learned_refinement_beats_refinement_without_the_network():
     """Planted nodes (gap 1e-3 eV) in an insulator on a coarse 16^3 mesh: the
     top 5 % of cells, refined with a 4^3 sub-mesh, chosen by the trained
     network, against the same number of cells chosen by the coarse |Omega|
-    itself and at random, with a uniform 192^3 mesh as the judge."""
+    itself and at random, with a uniform 192^3 mesh as the reference value."""
     from mepinterp_amd import capi, models, refine
     m = models.make_c4(seed=4)
     mp = (16, 16, 16)
