@@ -388,3 +388,32 @@ def test_fused_velocity_band_sums_equal_the_two_kernel_path(n, kblock):
             out.append(eng.run_mesh(mp, p))
     for key in KEYS:
         assert np.array_equal(out[0][key], out[1][key]), (key, _rel(out[0][key], out[1][key]))
+
+
+@pytest.mark.parametrize("n", [40, 64])
+def test_wide_near_degenerate_k_points_take_the_rotation_path(n):
+    """Two uncoupled copies of one model, the second shifted by 1e-9 Hartree:
+    every eigenvalue has a partner 1e-9 away, below the separation threshold,
+    so every k-point of the wide eigensolver is listed for the second QL pass
+    (rotation records) and the rotation replay.  V has no element between the
+    copies, so the eigenvalues and |V_nm|^2 are well defined."""
+    import oracle as O
+    from mepinterp_amd import models
+    h = n // 2
+    m1 = models.make_random(seed=61, n=h, L=1)
+    R = m1["R_frac"]
+    H = np.zeros((n, n, R.shape[1]), dtype=np.complex128)
+    H[:h, :h] = m1["H"]
+    H[h:, h:] = m1["H"]
+    r0 = int(np.where(np.all(R == 0, axis=0))[0][0])
+    H[h:, h:, r0] += 1e-9 * np.eye(h)
+    m = dict(R_frac=R, H=np.asfortranarray(H), a_latt=m1["a_latt"])
+    k = rand_kpts(12, seed=9)
+    with _engine(m) as eng:
+        e, V, _, _ = eng.interp(k)
+    om = oracle_model(m)
+    for ik in range(k.shape[1]):
+        eo, Vo, _, _ = O.get_wann_interp(om, k[:, ik])
+        assert np.min(np.diff(eo)) < 2e-9  # the pairs are below the separation threshold
+        assert np.max(np.abs(e[:, ik] - eo)) < E_TOL, f"eigenvalues differ at k #{ik}"
+        assert _rel(np.abs(V[..., ik]) ** 2, np.abs(Vo) ** 2) < REL
