@@ -390,13 +390,14 @@ def test_fused_velocity_band_sums_equal_the_two_kernel_path(n, kblock):
         assert np.array_equal(out[0][key], out[1][key]), (key, _rel(out[0][key], out[1][key]))
 
 
-@pytest.mark.parametrize("n", [40, 64])
-def test_wide_near_degenerate_k_points_take_the_rotation_path(n):
+@pytest.mark.parametrize("n", [24, 32, 40, 64])
+def test_near_degenerate_k_points_take_the_rotation_path(n):
     """Two uncoupled copies of one model, the second shifted by 1e-9 Hartree:
     every eigenvalue has a partner 1e-9 away, below the separation threshold,
-    so every k-point of the wide eigensolver is listed for the second QL pass
-    (rotation records) and the rotation replay.  V has no element between the
-    copies, so the eigenvalues and |V_nm|^2 are well defined."""
+    so every k-point is listed for the second QL pass (rotation records) and
+    its vectors come from the rotation replay (n <= 32: eigh_vec_kernel; n >
+    32: the wide kernels).  V has no element between the copies, so the
+    eigenvalues and |V_nm|^2 are well defined."""
     import oracle as O
     from mepinterp_amd import models
     h = n // 2
