@@ -1290,7 +1290,8 @@ __global__ __launch_bounds__(256, 3) void hwsum_kernel(BandArgs b, double Tsmear
         }
         __syncthreads();
         if (act)
-          for (int j = split; j < cnt; j += S) {
+#pragma unroll 4
+          for (int j = split; j < cnt; j += S) {  // several records' LDS loads in flight
             const double* sp = st + (int64_t)j * PS;
             const double dE = sp[0];
             const double f = sp[1 + ef];
