@@ -705,7 +705,7 @@ __global__ __launch_bounds__(64) void eigh_vec_kernel(int n, int64_t nk,
         R v = R(1);
         if (i < n) {
           const R ei = (i < n - 1) ? el[i] : R(0);
-          v = dl[i] - lam - ((i < n - 1) ? ei * ei / u : R(0));
+          v = dl[i] - lam - ((i < n - 1) ? ei * ei * rcp_nr(u) : R(0));
           if (fabs(v) < pivmin) v = (v < R(0)) ? -pivmin : pivmin;
           u = v;
         }
@@ -717,7 +717,7 @@ __global__ __launch_bounds__(64) void eigh_vec_kernel(int n, int64_t nk,
         R v = R(1);
         if (i < n) {
           const R em = (i > 0) ? el[i - 1] : R(0);
-          v = dl[i] - lam - ((i > 0) ? em * em / dd : R(0));
+          v = dl[i] - lam - ((i > 0) ? em * em * rcp_nr(dd) : R(0));
           if (fabs(v) < pivmin) v = (v < R(0)) ? -pivmin : pivmin;
           dd = v;
         }
@@ -738,14 +738,14 @@ __global__ __launch_bounds__(64) void eigh_vec_kernel(int n, int64_t nk,
 #pragma unroll
       for (int i = NMAX - 1; i >= 0; --i)
         if (i < r) {
-          zp = -(el[i] / dp[i]) * zp;
+          zp = -(el[i] * rcp_nr(dp[i])) * zp;
           dp[i] = zp;
         }
       zp = R(1);
 #pragma unroll
       for (int i = 0; i < NMAX; ++i)
         if (i > r && i < n) {
-          zp = -(el[i - 1] / um[i]) * zp;
+          zp = -(el[i - 1] * rcp_nr(um[i])) * zp;
           um[i] = zp;
         }
       R nz = R(0);
