@@ -50,16 +50,53 @@ def workload():
     return m, p, (64, 64, 64)
 
 
+def _host_cpus():
+    """CPUs this process may run on (the box's share), the model name and the
+    physical cores the machine reports."""
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except AttributeError:
+        allowed = os.cpu_count() or 1
+    model, phys = "unknown", set()
+    try:
+        with open("/proc/cpuinfo") as f:
+            cur = {}
+            for line in f:
+                if ":" not in line:
+                    if cur:
+                        phys.add((cur.get("physical id"), cur.get("core id")))
+                    cur = {}
+                    continue
+                k, v = (x.strip() for x in line.split(":", 1))
+                cur[k] = v
+                if k == "model name":
+                    model = v
+            if cur:
+                phys.add((cur.get("physical id"), cur.get("core id")))
+    except OSError:
+        pass
+    return allowed, model, len(phys)
+
+
+def _oracle_params(O, p, threads):
+    return O.RunParams(do_ahc=p["do_ahc"], do_kubo=p["do_kubo"], do_mep=p["do_mep"],
+                       n_eF=p["n_eF"], eF_min=p["eF_min"], eF_max=p["eF_max"], n_hw=p["n_hw"],
+                       hw_min=p.get("hw_min", 0.0), hw_max=p.get("hw_max", 0.0),
+                       T_kelvin=p["T_kelvin"], eta=p["eta"], valence_bands=p["valence_bands"],
+                       n_threads=threads)
+
+
 def cpu_baseline(m, p, mp, seconds=12.0):
-    """The oracle (FP64, reference loop structure) on the host cores, over a
-    fixed seeded sample of the same mesh's k-points."""
+    """The oracle (FP64, reference loop structure, -O3) on the host cores,
+    over a fixed seeded sample of the same mesh's k-points: once on every
+    CPU of this process's share (at most 16, the GPU box's share per GPU)
+    and once on one core.  Returns (record, sample k-points, oracle sums)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
-    threads = max(1, min(16, os.cpu_count() or 1))
+    allowed, model, phys = _host_cpus()
+    threads = max(1, min(16, allowed))
     om = O.Model(m["R_frac"], m["H"], a_latt=m["a_latt"], kubo_tol=1e-5)
-    rp = O.RunParams(do_ahc=p["do_ahc"], do_kubo=p["do_kubo"], do_mep=p["do_mep"], n_eF=p["n_eF"],
-                     eF_min=p["eF_min"], eF_max=p["eF_max"], n_hw=p["n_hw"], T_kelvin=p["T_kelvin"],
-                     eta=p["eta"], valence_bands=p["valence_bands"], n_threads=threads)
+    rp = _oracle_params(O, p, threads)
     mesh = O.mp_mesh(mp)
     rng = np.random.default_rng(1234)
     probe = mesh[:, rng.choice(mesh.shape[1], size=threads * 2, replace=False)]
@@ -68,13 +105,38 @@ def cpu_baseline(m, p, mp, seconds=12.0):
     per_k = (time.perf_counter() - t0) / probe.shape[1]
     nk = int(max(threads * 4, min(mesh.shape[1], seconds / max(per_k, 1e-9))))
     nk = (nk // threads) * threads
-    sample = mesh[:, rng.choice(mesh.shape[1], size=nk, replace=False)]
+    sample = np.asfortranarray(mesh[:, rng.choice(mesh.shape[1], size=nk, replace=False)])
     t0 = time.perf_counter()
-    O.accumulate(om, rp, sample)
+    sums = O.accumulate(om, rp, sample)
     dt = time.perf_counter() - t0
-    return {"value": nk / dt, "unit": "k-pts/s", "cores": threads, "kind": "port",
-            "sample": f"{nk} seeded random k-points of the 64^3 mesh, same jobs, "
-                      f"{dt:.1f} s on {threads} host threads (OpenMP over k)"}
+    # one core, on the first points of the same sample (about a tenth of the time)
+    n1 = max(4, int(0.1 * seconds / max(per_k * threads, 1e-9)))
+    t0 = time.perf_counter()
+    O.accumulate(om, _oracle_params(O, p, 1), np.asfortranarray(sample[:, :n1]))
+    dt1 = time.perf_counter() - t0
+    rec = {"value": nk / dt, "unit": "k-pts/s", "cores": threads, "kind": "port",
+           "one_core_value": n1 / dt1, "cpu_model": model, "physical_cores_seen": phys,
+           "cpus_allowed": allowed,
+           "sample": f"{nk} seeded random k-points of the 64^3 mesh, same jobs, {dt:.1f} s on "
+                     f"{threads} host threads (OpenMP over k); one core: the first {n1} of them "
+                     f"in {dt1:.1f} s"}
+    return rec, sample, sums
+
+
+def oracle_error(eng, rp_gpu, p, sample, o):
+    """GPU run_kpts on the oracle's sample against the oracle's sums: relative
+    error of sigma_xy, the AHC tensor, the Kubo alpha tensor (IC + LC + CS at
+    the first E_F) and the Niu alpha tensor (summed over the valence bands)."""
+    g = eng.run_kpts(sample, rp_gpu)
+    nv = p["valence_bands"]
+
+    def tens(s):
+        return {"sigma_xy": np.array([s["ahc"][0, 1, 0]]), "ahc_tens": s["ahc"][:, :, 0],
+                "kubo_alpha": (s["kubo_ic"] + s["kubo_lc"] + s["kubo_cs"])[:, :, 0],
+                "niu_alpha": (s["niu_ic"] + s["niu_lc"] + s["niu_cs"])[:, :, :nv].sum(axis=2)}
+    tg, to = tens(g), tens(o)
+    return {k: float(np.max(np.abs(tg[k] - to[k])) / max(np.max(np.abs(to[k])), 1e-300))
+            for k in tg}
 
 
 def main():
@@ -165,6 +227,12 @@ def main():
     else:
         roofline = {"bound": "mfma", "achieved": d["achieved_tflops"], "peak": PEAK_FP64_TFLOPS,
                     "unit": "TFLOP/s", "frac": d["frac"], "traffic": d.get("traffic_per_launch"),
+                    "frac_basis": "algorithmic complex-product flops (8 real flops per complex "
+                                  "multiply-add, 48 n^3 velocity + 48 n^3 band sums per k); the "
+                                  "kernel issues 3 real MFMA products per complex product and "
+                                  "runs its pair sums on the VALU, so the MFMA pipe is busy for "
+                                  "at most 3/4 of frac",
+                    "mfma_issued_frac_bound": 0.75 * d["frac"],
                     "kernel": {"velo": "velo_mfma_kernel<zd>",
                                "bandsum": "bandsum_kernel<32, %s>" % ("true" if fused else "false")}[dom],
                     "launch_ms": d["launch_ms"], "stages": stages}
@@ -196,12 +264,26 @@ def main():
     log(f"[bench] sigma_xy = {out['ahc_tens'][0, 1]:.10e}  alpha_Niu_xx = {out['mep_tens'][0, 0]:.6e}")
 
     cpu = None
+    sample = osums = None
     if not args.no_cpu:
         try:
-            cpu = cpu_baseline(m, p, mp, seconds=args.cpu_seconds)
+            cpu, sample, osums = cpu_baseline(m, p, mp, seconds=args.cpu_seconds)
             log(f"[bench] cpu baseline {cpu}")
         except Exception as e:  # the baseline is reported, never the product
             log(f"[bench] cpu baseline failed: {e}")
+    if sample is None:  # no CPU leg: a small sample for the error check only
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        rng = np.random.default_rng(1234)
+        mesh = O.mp_mesh(mp)
+        sample = np.asfortranarray(mesh[:, rng.choice(mesh.shape[1], size=2048, replace=False)])
+        osums = O.accumulate(O.Model(m["R_frac"], m["H"], a_latt=m["a_latt"], kubo_tol=1e-5),
+                             _oracle_params(O, p, max(1, min(16, _host_cpus()[0]))), sample)
+    # the metric's error half: GPU sums against the FP64 oracle on the same k-points
+    err = {"n_k": int(sample.shape[1]), "fp64": oracle_error(eng, rp, p, sample, osums),
+           "mixed": oracle_error(eng, rpm, p, sample, osums)}
+    log(f"[bench] error vs the FP64 oracle on {sample.shape[1]} k-points: {err}")
+    mixed["rel_err_vs_oracle"] = err["mixed"]
     res = {
         "metric": "k-pts/s (interp+eigh+Berry) at n_wf=32",
         "value": value,
@@ -222,6 +304,7 @@ def main():
                    "kernel_ms": {k: v[0] for k, v in split.items()}},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "oracle_rel_err": err,
         "mixed_precision": mixed,
     }
     print(json.dumps(res), flush=True)

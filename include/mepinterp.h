@@ -78,6 +78,11 @@ typedef struct {
   double eta;                    /* Fermi%eta_smearing (Hartree)             */
   int32_t valence_bands;         /* MEP%valence_bands                        */
   int32_t precision;             /* MEP_PREC_*                               */
+  const double *hw_lst;          /* (n_hw) frequencies [Hartree] as the caller's
+                                    hw_lst argument of velo_ahc_tens /
+                                    kubo_ohc_tens (kubo.f90:86-104, 145-164),
+                                    any spacing; NULL = get_linspace(hw_min,
+                                    hw_max, n_hw) (matrix_math.f90:74-100)   */
 } mep_run_params;
 
 /* Raw k-sums (not normalised; all Fermi levels kept).  Host buffers owned
@@ -141,6 +146,12 @@ int mep_bandsum_batch(mep_ctx *ctx, int64_t nk, const double *e_k, const double 
    raw sums come back to the host.  fermi_selector and normalize_k_int
    (core.f90:317-603) stay with the caller.                                  */
 int mep_run_mesh(mep_ctx *ctx, const int32_t mp_grid[3], const mep_run_params *p, mep_sums *out);
+
+/* The k-points mep_run_mesh generates on the device, k0 .. k0+nk-1 in the
+   visiting order of core.f90:165-170, read back into kpts(3,nk): get_rel_kpt
+   of get_kpt_idx(kix,kiy,kiz) (k_space.f90:341-375), x fastest.  Bit-exact
+   with the reference's (2 q - N - 1) / (2 N).                              */
+int mep_mesh_kpts(mep_ctx *ctx, const int32_t mp_grid[3], int64_t k0, int64_t nk, double *kpts);
 
 /* Same over an explicit k list (3,nk) with optional weights(nk) (NULL = 1):
    sub-mesh refinement and sampled baselines.                                */

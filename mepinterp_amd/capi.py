@@ -32,7 +32,7 @@ MEP_PREC_MIXED = 1
 EXPORTS = ("mep_ctx_create", "mep_ctx_destroy", "mep_last_error", "mep_interp_batch",
            "mep_bandsum_batch", "mep_run_mesh", "mep_run_kpts", "mep_set_profiling",
            "mep_get_kernel_time", "mep_reset_kernel_times", "mep_set_kblock", "mep_curv_batch",
-           "mep_run_batch", "mep_set_workspace_limit",
+           "mep_run_batch", "mep_set_workspace_limit", "mep_mesh_kpts",
            "mep_mlp_create", "mep_mlp_destroy", "mep_mlp_num_params", "mep_mlp_get_params",
            "mep_mlp_set_params", "mep_mlp_train", "mep_mlp_predict", "mep_mlp_grad")
 
@@ -43,7 +43,7 @@ class RunParams(C.Structure):
     _fields_ = [("jobs", C.c_int32), ("n_eF", C.c_int32), ("eF_min", C.c_double),
                 ("eF_max", C.c_double), ("n_hw", C.c_int32), ("hw_min", C.c_double),
                 ("hw_max", C.c_double), ("T_kelvin", C.c_double), ("eta", C.c_double),
-                ("valence_bands", C.c_int32), ("precision", C.c_int32)]
+                ("valence_bands", C.c_int32), ("precision", C.c_int32), ("hw_lst", c_dp)]
 
 
 class Sums(C.Structure):
@@ -84,6 +84,7 @@ def load(path: str = LIB_PATH):
                                  C.POINTER(Sums)]
     lib.mep_run_kpts.argtypes = [C.c_void_p, C.c_int64, c_dp, c_dp, C.POINTER(RunParams),
                                  C.POINTER(Sums)]
+    lib.mep_mesh_kpts.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.c_int64, C.c_int64, c_dp]
     lib.mep_set_profiling.argtypes = [C.c_void_p, C.c_int32]
     lib.mep_get_kernel_time.argtypes = [C.c_void_p, C.c_char_p, c_dp, C.POINTER(C.c_int64)]
     lib.mep_reset_kernel_times.argtypes = [C.c_void_p]
@@ -188,10 +189,18 @@ class Engine:
 
     @staticmethod
     def params(jobs=MEP_JOB_AHC, n_eF=1, eF_min=0.0, eF_max=0.0, n_hw=1, hw_min=0.0, hw_max=0.0,
-               T_kelvin=0.0, eta=0.0, valence_bands=1, precision=MEP_PREC_FP64):
-        return RunParams(int(jobs), int(n_eF), float(eF_min), float(eF_max), int(max(1, n_hw)),
-                         float(hw_min), float(hw_max), float(T_kelvin), float(eta),
-                         int(valence_bands), int(precision))
+               T_kelvin=0.0, eta=0.0, valence_bands=1, precision=MEP_PREC_FP64, hw_lst=None):
+        """Run parameters; ``hw_lst`` (Hartree) replaces the linspace of
+        (hw_min, hw_max, n_hw) with the caller's frequencies."""
+        hw = None
+        if hw_lst is not None:
+            hw = np.ascontiguousarray(hw_lst, dtype=np.float64).ravel()
+            n_hw = hw.size
+        p = RunParams(int(jobs), int(n_eF), float(eF_min), float(eF_max), int(max(1, n_hw)),
+                      float(hw_min), float(hw_max), float(T_kelvin), float(eta),
+                      int(valence_bands), int(precision), _p(hw))
+        p.hw_keep = hw  # the list lives as long as the parameters
+        return p
 
     @staticmethod
     def _sums_buffers(p: RunParams):
@@ -234,6 +243,14 @@ class Engine:
         _check(rc, "mep_run_mesh")
         out["n_k"] = s.n_k
         return out
+
+    def mesh_kpts(self, mp_grid, k0=0, nk=None):
+        """The device-generated k-points k0 .. k0+nk-1 of the MP mesh (3, nk)."""
+        mp = (C.c_int32 * 3)(*[int(x) for x in mp_grid])
+        nk = int(np.prod(mp_grid)) - int(k0) if nk is None else int(nk)
+        k = np.zeros((3, nk), order="F")
+        _check(load().mep_mesh_kpts(self.ctx, mp, int(k0), nk, _p(k)), "mep_mesh_kpts")
+        return k
 
     def run_kpts(self, kpts, p: RunParams, weights=None):
         k = np.asfortranarray(kpts, dtype=np.float64).reshape(3, -1, order="F")

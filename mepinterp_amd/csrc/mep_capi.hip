@@ -676,7 +676,15 @@ static int band_setup(mep_ctx* c, const mep_run_params* p, int64_t kblock, BandS
   b.v_herm = c->pos ? 0 : 1;  // the library builds V itself
   std::vector<double> ef, hw;
   linspace(p->eF_min, p->eF_max, p->n_eF, ef);
-  linspace(p->hw_min, p->hw_max, p->n_hw, hw);
+  if (p->hw_lst) {
+    // the caller's frequency list, any spacing (velo_ahc_tens / kubo_ohc_tens
+    // take hw_lst as an argument, kubo.f90:86-104, 145-164)
+    hw.assign(p->hw_lst, p->hw_lst + p->n_hw);
+    for (double v : hw)
+      if (!std::isfinite(v)) return fail(MEP_ERR_ARG, "hw_lst holds a non-finite frequency");
+  } else {
+    linspace(p->hw_min, p->hw_max, p->n_hw, hw);
+  }
   if ((int64_t)ef.size() > c->ef_cap) {
     hipFree(c->def);
     HIPCHK(hipMalloc(&c->def, sizeof(double) * ef.size()));
@@ -979,6 +987,36 @@ extern "C" int mep_run_mesh(mep_ctx* c, const int32_t mp_grid[3], const mep_run_
     return fail(MEP_ERR_ARG, "mep_run_mesh: mp_grid entries must be >= 1");
   int64_t nk = (int64_t)mp_grid[0] * mp_grid[1] * mp_grid[2];
   return run_common(c, nk, mp_grid, nullptr, nullptr, p, out);
+}
+
+extern "C" int mep_mesh_kpts(mep_ctx* c, const int32_t mp_grid[3], int64_t k0, int64_t nk,
+                             double* kpts) {
+  if (!c || !mp_grid || !kpts || nk < 0 || k0 < 0)
+    return fail(MEP_ERR_ARG, "mep_mesh_kpts: bad argument");
+  if (mp_grid[0] < 1 || mp_grid[1] < 1 || mp_grid[2] < 1)
+    return fail(MEP_ERR_ARG, "mep_mesh_kpts: mp_grid entries must be >= 1");
+  const int64_t ntot = (int64_t)mp_grid[0] * mp_grid[1] * mp_grid[2];
+  if (k0 + nk > ntot) return fail(MEP_ERR_ARG, "mep_mesh_kpts: k0 + nk exceeds the mesh");
+  if (nk == 0) return MEP_OK;
+  // the same kernel as the runs' k-point generation, into a buffer of its
+  // own (the run workspace may not exist yet)
+  const int64_t chunk = nk < (1 << 20) ? nk : (1 << 20);
+  double* dk = nullptr;
+  HIPCHK(hipMalloc(&dk, sizeof(double) * 3 * chunk));
+  const int m3[3] = {mp_grid[0], mp_grid[1], mp_grid[2]};
+  int rc = MEP_OK;
+  for (int64_t q = 0; q < nk && rc == MEP_OK; q += chunk) {
+    const int64_t nb = (nk - q < chunk) ? nk - q : chunk;
+    timed(c, K_KGEN, [&] { launch_mp_kpts(c->st, k0 + q, nb, m3, dk); });
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(kpts + 3 * q, dk, sizeof(double) * 3 * nb, hipMemcpyDeviceToHost, c->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+    if (e != hipSuccess) rc = fail(MEP_ERR_HIP, std::string("mep_mesh_kpts: ") + hipGetErrorString(e));
+  }
+  hipFree(dk);
+  resolve_times(c);
+  return rc;
 }
 
 extern "C" int mep_run_kpts(mep_ctx* c, int64_t nk, const double* kpts, const double* weights,

@@ -9,10 +9,12 @@ module core
   use mep_capi
   use mep_gpu, only: gpu_model_ctx, make_run_params
   use matrix_math, only: get_linspace
-  use k_space, only: get_mp_grid, normalize_k_int, k_norm
+  use k_space, only: get_mp_grid, normalize_k_int, k_norm, get_kpt_idx, get_rel_kpt
   use input_paras, only: seed_name, valence_bands, do_gauge_trafo, do_mep, do_ahc, do_kubo, &
                          do_opt, do_gyro, do_write_mep_bands, wf_centers, N_hw, hw_min, hw_max, &
-                         N_eF, eF_min, eF_max, T_kelvin, i_eta_smr, out_dir
+                         N_eF, eF_min, eF_max, T_kelvin, i_eta_smr, out_dir, do_write_velo, &
+                         my_mkdir
+  use band_calc, only: write_velo
   use file_io, only: read_tb_basis, write_hw_list, write_mep_bands, write_mep_tensors, &
                      write_kubo_mep_tensors, write_ahc_tensor, write_opt_tensors, &
                      write_gyro_tensors
@@ -84,6 +86,10 @@ contains
       real(t1 - t0, dp) / real(rate, dp), " s (", real(n_ki_glob, dp) * real(rate, dp) / &
       max(real(t1 - t0, dp), 1.0_dp), " k-pts/s)"
 
+    ! jobs%do_write_velo: out/velo/velo_Vka.<ki> for every k-point of the
+    ! mesh, as the reference's k-loop writes them (core.f90:258)
+    if (do_write_velo .and. jobs /= 0) call write_mesh_velo(ctx, size(H_tb, 1), use_pos)
+
     eF_idx = fermi_selector(n_ki_glob, Ne, ef_lst, valence_bands)
     call integrate_over_k_space(eF_idx, nic, nlc, ncs, kic, klc, kcs, ahc, velo, ohc)
     ! optical tensors at eF_idx, photogalvanic and gyrotropic tensors at every
@@ -94,6 +100,50 @@ contains
     end if
     if (do_gyro) call write_gyro_tensors(gyro_C / k_norm(), gyro_D / k_norm(), gyro_Dw / k_norm())
   end subroutine core_worker
+
+  ! the velocities of the mesh's k-points, in the reference's visiting order
+  ! (kiz slowest, kix fastest; core.f90:165-178), interpolated in batches of
+  ! up to 4096 k-points (about 50 MB of V per batch) and written one file per
+  ! k-point (file_io.f90:146-183)
+  subroutine write_mesh_velo(ctx, n, use_pos)
+    type(c_ptr), intent(in) :: ctx
+    integer, intent(in) :: n
+    logical, intent(in) :: use_pos
+    real(dp), allocatable, target :: kp(:, :), en(:, :)
+    complex(dp), allocatable, target :: V(:, :, :, :), A(:, :, :, :), Om(:, :, :, :, :)
+    integer, allocatable :: idx(:)
+    type(c_ptr) :: ap, op
+    integer :: g(3), kix, kiy, kiz, q, nb, chunk
+    chunk = max(64, min(4096, 50000000 / (48 * n * n)))
+    call my_mkdir(out_dir // "/velo/")
+    g = get_mp_grid()
+    allocate(kp(3, chunk), en(n, chunk), V(3, n, n, chunk), idx(chunk))
+    ap = c_null_ptr
+    op = c_null_ptr
+    if (use_pos) then
+      allocate(A(3, n, n, chunk), Om(3, 3, n, n, chunk))
+      ap = c_loc(A)
+      op = c_loc(Om)
+    end if
+    nb = 0
+    do kiz = 1, g(3)
+      do kiy = 1, g(2)
+        do kix = 1, g(1)
+          nb = nb + 1
+          idx(nb) = get_kpt_idx(kix, kiy, kiz)
+          kp(:, nb) = get_rel_kpt(idx(nb), kix, kiy, kiz)
+          if (nb == chunk .or. (kix == g(1) .and. kiy == g(2) .and. kiz == g(3))) then
+            call mep_check(mep_interp_batch(ctx, int(nb, c_int64_t), c_loc(kp), c_loc(en), &
+                                            c_loc(V), ap, op), "write_mesh_velo")
+            do q = 1, nb
+              call write_velo(idx(q), V(:, :, :, q))
+            end do
+            nb = 0
+          end if
+        end do
+      end do
+    end do
+  end subroutine write_mesh_velo
 
   integer function fermi_selector(n_ki_glob, Ne_loc_sum, eF_lst, Ne_sys) result(opt_idx)
     ! core.f90:317-415: E_F minimising |N_el(E_F) - N_el,sys|, first on ties;

@@ -21,6 +21,8 @@ program dropin_check
   real(dp), allocatable :: e_all(:, :), ahc(:, :, :, :), kic(:, :, :, :), klc(:, :, :, :), &
                            kcs(:, :, :, :), nic(:, :, :, :), nlc(:, :, :, :), ncs(:, :, :, :)
   complex(dp), allocatable :: V_all(:, :, :, :), velo(:, :, :, :), ohc(:, :, :, :)
+  complex(dp), allocatable :: H_conj(:, :, :)
+  real(dp), allocatable :: hw_nu(:), e_conj(:, :)
   integer :: u, nk, ik, n, i
   if (.not. init_parameters()) stop "no input.cfg"
   call get_linspace(eF_min, eF_max, N_eF, ef_lst)
@@ -57,6 +59,31 @@ program dropin_check
     nlc(:, :, :, ik) = mep_niu_LC(V_ka, en_k)
     ncs(:, :, :, ik) = mep_niu_CS(A_ka, Om_kab)
   end do
+  ! a second model with the same |H(R)| element by element (complex
+  ! conjugate): the resident model must not be reused for it
+  H_conj = conjg(H_tb)
+  allocate(e_conj(n, nk))
+  do ik = 1, nk
+    call get_wann_interp(do_gauge_trafo, H_conj, r_tb, R_vect, wf_centers, ik, kpts(:, ik), en_k, &
+                         V_ka, A_ka, Om_kab)
+    e_conj(:, ik) = en_k
+  end do
+  call save_npy(out_dir // "dropin_e_conj.npy", e_conj)
+  ! a non-uniform frequency list, kept as given (kubo.f90:86-104, 145-164)
+  allocate(hw_nu(N_hw + 2))
+  do i = 1, N_hw + 2
+    hw_nu(i) = (hw_max + 0.1_dp) * (real(i - 1, dp) / real(N_hw + 1, dp))**2
+  end do
+  call get_wann_interp(do_gauge_trafo, H_tb, r_tb, R_vect, wf_centers, 1, kpts(:, 1), en_k, &
+                       V_ka, A_ka, Om_kab)
+  do i = 1, n
+    fd(:, i) = fd_stat(en_k(i), ef_lst, T_kelvin)
+  end do
+  call save_npy(out_dir // "dropin_hw_nu.npy", hw_nu)
+  call save_npy(out_dir // "dropin_velo_nu.npy", &
+                reshape(velo_ahc_tens(en_k, V_ka, hw_nu, fd, i_eta_smr), [3, 3, (N_hw + 2) * N_eF]))
+  call save_npy(out_dir // "dropin_ohc_nu.npy", &
+                reshape(kubo_ohc_tens(en_k, V_ka, hw_nu, fd, i_eta_smr), [3, 3, (N_hw + 2) * N_eF]))
   call save_npy(out_dir // "dropin_e.npy", e_all)
   call save_npy(out_dir // "dropin_V.npy", V_all)
   call save_npy(out_dir // "dropin_ahc.npy", reshape(ahc, [3, 3, N_eF * nk]))

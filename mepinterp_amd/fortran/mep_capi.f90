@@ -21,6 +21,7 @@ module mep_capi
     real(c_double) :: eta
     integer(c_int32_t) :: valence_bands
     integer(c_int32_t) :: precision
+    type(c_ptr) :: hw_lst = c_null_ptr
   end type mep_run_params
 
   type, bind(C) :: mep_sums

@@ -13,7 +13,10 @@ module mep_gpu
   type(c_ptr), save :: model_ctx = c_null_ptr
   integer, save :: model_n = -1, model_nR = -1, model_flags = -1
   logical, save :: model_tb = .false., model_pos = .false.
-  real(dp), save :: model_sig = -1.0_dp
+  ! the inputs of the resident model, kept for an exact comparison
+  complex(dp), allocatable, save :: model_H(:, :, :), model_rpos(:, :, :, :)
+  real(dp), allocatable, save :: model_Rvec(:, :), model_wc(:, :)
+  real(dp), save :: model_al(3, 3) = 0.0_dp, model_tol = -1.0_dp
   type(c_ptr), save :: band_ctx = c_null_ptr
   integer, save :: band_n = -1
   logical, save :: band_pos = .false.
@@ -28,10 +31,15 @@ contains
     band_ctx = c_null_ptr
     model_n = -1
     band_n = -1
+    if (allocated(model_H)) deallocate(model_H)
+    if (allocated(model_rpos)) deallocate(model_rpos)
+    if (allocated(model_Rvec)) deallocate(model_Rvec)
+    if (allocated(model_wc)) deallocate(model_wc)
   end subroutine gpu_release
 
   ! context for the model (H, optional r, R, optional centres); reused while
-  ! sizes, flags and a checksum of the inputs are unchanged
+  ! the sizes, the flags and every input value are unchanged (compared
+  ! element by element against the copies of the resident model)
   subroutine gpu_model_ctx(gauge_trafo, H_real, r_real, use_pos, R_frac, wf_centers, ctx)
     logical, intent(in) :: gauge_trafo, use_pos
     complex(dp), intent(in) :: H_real(:, :, :)
@@ -43,20 +51,23 @@ contains
     real(dp), allocatable, target :: Rvec_c(:, :), wc(:, :)
     real(dp), target :: al(3, 3)
     type(c_ptr) :: rp, wp
-    real(dp) :: sig
     integer :: n, nR, flags
     integer(c_int) :: rc
+    logical :: same
     n = size(H_real, 1)
     nR = size(H_real, 3)
     flags = 0
     if (gauge_trafo) flags = ior(flags, int(MEP_F_GAUGE_TRAFO))
     if (use_cart_velo) flags = ior(flags, int(MEP_F_CART_VELO))
-    sig = sum(abs(H_real)) + sum(abs(R_frac)) + kubo_tol + sum(abs(a_latt))
-    if (use_pos) sig = sig + sum(abs(r_real))
-    if (allocated(wf_centers)) sig = sig + sum(abs(wf_centers)) + 1.0_dp
-    if (c_associated(model_ctx) .and. n == model_n .and. nR == model_nR .and. &
-        flags == model_flags .and. (use_pos .eqv. model_pos) .and. &
-        (allocated(wf_centers) .eqv. model_tb) .and. sig == model_sig) then
+    same = c_associated(model_ctx) .and. n == model_n .and. nR == model_nR .and. &
+           flags == model_flags .and. (use_pos .eqv. model_pos) .and. &
+           (allocated(wf_centers) .eqv. model_tb)
+    if (same) same = all(a_latt == model_al) .and. kubo_tol == model_tol
+    if (same) same = all(R_frac == model_Rvec)
+    if (same) same = all(H_real == model_H)
+    if (same .and. use_pos) same = all(r_real == model_rpos)
+    if (same .and. allocated(wf_centers)) same = all(wf_centers(:, 1:n) == model_wc)
+    if (same) then
       ctx = model_ctx
       return
     end if
@@ -87,7 +98,12 @@ contains
     model_flags = flags
     model_pos = use_pos
     model_tb = allocated(wf_centers)
-    model_sig = sig
+    model_H = H_real
+    model_Rvec = R_frac
+    model_al = a_latt
+    model_tol = kubo_tol
+    if (use_pos) model_rpos = r_real
+    if (allocated(wf_centers)) model_wc = wf_centers(:, 1:n)
     ctx = model_ctx
   end subroutine gpu_model_ctx
 
@@ -141,6 +157,7 @@ contains
     p%eta = eta
     p%valence_bands = int(max(1, nval), c_int32_t)
     p%precision = MEP_PREC_FP64
+    p%hw_lst = c_null_ptr
   end function make_run_params
 
   ! band sums of ONE k-point with caller-given occupations fd(n_eF, n)
@@ -153,7 +170,7 @@ contains
                                                   kcs(:, :, :), nic(:, :, :), nlc(:, :, :), ncs(:, :, :)
     complex(dp), allocatable, target, intent(out) :: velo(:, :, :, :), ohc(:, :, :, :)
     complex(dp), optional, intent(in) :: A_ka(:, :, :), Om_kab(:, :, :, :)
-    real(dp), allocatable, target :: e(:), fd(:, :), Ne(:)
+    real(dp), allocatable, target :: e(:), fd(:, :), Ne(:), hw(:)
     complex(dp), allocatable, target :: V(:, :, :), A(:, :, :), Om(:, :, :, :)
     type(c_ptr) :: ctx, ap, op
     type(mep_run_params) :: p
@@ -183,8 +200,12 @@ contains
       op = c_loc(Om)
     end if
     if (size(hw_lst) >= 1) then
+      ! the caller's frequencies as given (any spacing), kubo.f90:86-104, 145-164
       p = make_run_params(jobs, nef, 0.0_dp, 0.0_dp, nhw, hw_lst(1), hw_lst(size(hw_lst)), &
                           0.0_dp, eta, nval)
+      allocate(hw(nhw))
+      hw = hw_lst
+      p%hw_lst = c_loc(hw)
     else
       p = make_run_params(jobs, nef, 0.0_dp, 0.0_dp, 1, 0.0_dp, 0.0_dp, 0.0_dp, eta, nval)
     end if

@@ -124,6 +124,24 @@ def test_fortran_per_k_interface(tmp_path, kind):
                 else got[key][..., ik]
             scale = max(np.abs(w).max(), 1e-300)
             assert np.max(np.abs(g - w.reshape(g.shape, order="F"))) <= 1e-8 * scale, (key, ik)
+    # the complex-conjugate model (same |H(R)|) is a different model: its
+    # eigenvalues are those of H at -k, not a reuse of the resident context
+    om_c = O.Model(R, np.conj(H), r=rr, wf_centers=cfg["wf_centers"], a_latt=cfg["a_latt"],
+                   kubo_tol=1e-5)
+    e_c = np.load(os.path.join(out, "dropin_e_conj.npy"))
+    for ik in range(k.shape[1]):
+        e, _, _, _ = O.get_wann_interp(om_c, k[:, ik])
+        assert np.max(np.abs(e_c[:, ik] - e)) < 1e-10 / AU_TO_EV, ik
+    assert np.max(np.abs(e_c - e_f)) > 1e-6
+    # a non-uniform hw list reaches the band sums unchanged
+    hw_nu = np.load(os.path.join(out, "dropin_hw_nu.npy"))
+    assert np.max(np.abs(np.diff(hw_nu, 2))) > 1e-6  # not a linspace
+    e, V, _, _ = O.get_wann_interp(om, k[:, 0])
+    fd = O.fd_distrib(e, ef, T)
+    for key, fn in (("velo_nu", O.velo_ahc_tens), ("ohc_nu", O.kubo_ohc_tens)):
+        g = np.load(os.path.join(out, f"dropin_{key}.npy"))
+        w = fn(e, V, hw_nu, fd, cfg["eta"]).reshape(g.shape, order="F")
+        assert np.max(np.abs(g - w)) <= 1e-8 * np.abs(w).max(), key
 
 
 def test_mepinterp_band_structure_mode(tmp_path):
@@ -154,6 +172,45 @@ def test_mepinterp_band_structure_mode(tmp_path):
         assert np.allclose(ee[idx == q + 1], eo, atol=6e-9)
         V = F.read_velo(os.path.join(tmp, "out", "velo", f"velo_Vka.{q + 1:07d}"), 10)
         assert np.max(np.abs(V - Vo)) < 1e-7
+
+
+def test_mepinterp_mesh_mode_writes_velocities(tmp_path):
+    """jobs%do_write_velo in the k-space integration: out/velo/velo_Vka.<ki>
+    for every k-point of the mesh, ki = get_kpt_idx(kix, kiy, kiz)
+    (core.f90:258, file_io.f90:146-183), against the oracle's V at
+    get_rel_kpt of the same index."""
+    import oracle as O
+    from mepinterp_amd import formats as F
+    from mepinterp_amd import models
+    exe = _need("mepInterp")
+    tmp = str(tmp_path)
+    os.makedirs(os.path.join(tmp, "w90files"))
+    m = models.make_random(seed=32, n=8, L=1, tb=True)
+    F.write_hr_dat(os.path.join(tmp, "w90files", "mv_hr.dat"), m["R_frac"], m["H"])
+    mp = (3, 3, 2)
+    F.write_input_cfg(os.path.join(tmp, "input.cfg"), dict(
+        do_ahc=True, do_write_velo=True, a_latt=m["a_latt"], seed_name="mv",
+        wf_centers=m["wf_centers"], mp_grid=mp, T_kelvin=0.0, valence_bands=4))
+    r = subprocess.run([exe], cwd=tmp, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    R, H, _ = F.read_hr_dat(os.path.join(tmp, "w90files", "mv_hr.dat"))
+    om = O.Model(R, H, wf_centers=m["wf_centers"], a_latt=m["a_latt"])
+    # get_kpt_idx strides by N2 and N3 (k_space.f90:345): on this non-cubic
+    # mesh some indices repeat, and the k-point visited last owns the file,
+    # as in the reference's loop
+    mesh = O.mp_mesh(mp)
+    want = {}
+    for kiz in range(1, mp[2] + 1):
+        for kiy in range(1, mp[1] + 1):
+            for kix in range(1, mp[0] + 1):
+                ki = O.kpt_idx(mp, kix, kiy, kiz)
+                want[ki] = mesh[:, (kix - 1) + mp[0] * ((kiy - 1) + mp[1] * (kiz - 1))]
+    files = sorted(os.listdir(os.path.join(tmp, "out", "velo")))
+    assert files == [f"velo_Vka.{ki:07d}" for ki in sorted(want)]
+    for ki, kpt in want.items():
+        _, Vo, _, _ = O.get_wann_interp(om, kpt)
+        V = F.read_velo(os.path.join(tmp, "out", "velo", f"velo_Vka.{ki:07d}"), 8)
+        assert np.max(np.abs(V - Vo)) < 1e-7, ki
 
 
 def test_mepinterp_optical_and_gyrotropic_outputs(tmp_path):
