@@ -11,7 +11,8 @@
       (each divided by --scale): aggregate k-pts/s of one mep_run_mesh per
       model, and of one mep_run_batch over all of them with autotuned
       k-blocks
-* c4  n_wf = 24, NR = 343 with eight planted nodes (gap 1e-3 eV): sigma_xy
+* c4  n_wf = 24, NR = 343, eight Chern layers with a planted node each
+      (gap 1e-3 eV, net Chern number 2): sigma_xy
       on a Q^3 base mesh (48 by default) with the top 5 % of cells refined by
       a 4^3 sub-mesh, cells chosen by the trained network, by the coarse
       |Omega| and at random, against the uniform base and 256^3 meshes
@@ -102,7 +103,7 @@ def run_c3(scale: int = 1):
                 models=per)
 
 
-def run_c4(q: int = 48):
+def run_c4(q: int = 48, box: int = 9, flat: float = 1.0):
     from . import refine
     m = models.make_c4()
     mp = (q, q, q)
@@ -112,19 +113,26 @@ def run_c4(q: int = 48):
         ref = refine.sigma_uniform(eng, (256, 256, 256), m["a_latt"])
         t_ref = time.perf_counter() - t0
         base = refine.sigma_uniform(eng, mp, m["a_latt"])
-        for sel in ("mlp", "coarse", "random"):
-            t0 = time.perf_counter()
-            s, info = refine.refine_sigma(eng, mp, m["a_latt"], select=sel, epochs=40, batch=2048,
-                                          seed=3)
-            res[sel] = dict(sigma_xy=s, abs_err=abs(s - ref), n_k=info["n_k"],
-                            seconds=time.perf_counter() - t0)
-            if sel == "mlp":
-                res[sel].update(train_loss=info["loss"], target_var=info["target_var"])
-    return dict(config="c4", workload=f"n_wf=24 NR=343, 8 planted nodes (gap 1e-3 eV), {q}^3 base "
-                "mesh, top 5 % of cells refined 4^3, sigma_xy at E_F=0, T=0",
+        for est in ("window", "replace"):
+            for sel in ("mlp", "coarse", "random"):
+                t0 = time.perf_counter()
+                s, info = refine.refine_sigma(eng, mp, m["a_latt"], select=sel, epochs=40,
+                                              batch=2048, seed=3, estimator=est, box=box,
+                                              flat=flat)
+                r = dict(sigma_xy=s, abs_err=abs(s - ref), rel_err=abs(s - ref) / abs(ref),
+                         n_k=info["n_k"], seconds=time.perf_counter() - t0)
+                if sel == "mlp":
+                    r.update(train_loss=info["loss"], target_var=info["target_var"])
+                if est == "window":
+                    r.update(boxes=int(info["boxes"].shape[1]))
+                res[f"{est}_{sel}"] = r
+    return dict(config="c4", workload=f"n_wf=24 NR=343, 8 Chern layers with planted nodes (gap 1e-3 "
+                f"eV, net Chern number 2), {q}^3 base mesh, 5 % of cells refined 4^3 (windowed: "
+                f"{box}^3-cell boxes, flat {flat}), sigma_xy at E_F=0, T=0",
                 sigma_xy_256=ref, n_k_256=256 ** 3, seconds_256=t_ref,
-                uniform_base=dict(sigma_xy=base, abs_err=abs(base - ref), n_k=q ** 3),
-                refined=res, k_ratio=256 ** 3 / res["mlp"]["n_k"])
+                uniform_base=dict(sigma_xy=base, abs_err=abs(base - ref),
+                                  rel_err=abs(base - ref) / abs(ref), n_k=q ** 3),
+                refined=res, k_ratio=256 ** 3 / res["window_mlp"]["n_k"])
 
 
 def run_c5(q: int = 128):
@@ -145,13 +153,15 @@ def main(argv=None):
     ap.add_argument("config", choices=("c1", "c3", "c4", "c5"))
     ap.add_argument("--mesh", type=int, default=None, help="Q for a Q^3 mesh (c1, c4, c5)")
     ap.add_argument("--scale", type=int, default=1, help="divide the c3 meshes by this")
+    ap.add_argument("--box", type=int, default=9, help="c4: cells per box edge (odd)")
+    ap.add_argument("--flat", type=float, default=1.0, help="c4: half-width of the window's flat top")
     a = ap.parse_args(argv)
     if a.config == "c1":
         r = run_c1(a.mesh or 32)
     elif a.config == "c3":
         r = run_c3(a.scale)
     elif a.config == "c4":
-        r = run_c4(a.mesh or 48)
+        r = run_c4(a.mesh or 48, a.box, a.flat)
     else:
         r = run_c5(a.mesh or 128)
     print(json.dumps(r), flush=True)

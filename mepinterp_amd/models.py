@@ -151,19 +151,24 @@ def make_c3(seed: int = 3):
 
 
 def make_c4(seed: int = 4, n: int = 24, L: int = 3, n_nodes: int = 8, gap_eV: float = 1e-3,
-            v_eV: float = 0.005, m_eV: float = 0.02, couple_eV: float = 0.002, bg_eV: float = 3.0,
-            bg_hop_eV: float = 0.05, decay: float = 1.5, a: float = 6.0):
+            v_eV: float = 0.005, m_eV: float = 0.02, mz_eV: float = 0.008,
+            couple_eV: float = 0.002, bg_eV: float = 3.0, bg_hop_eV: float = 0.05,
+            decay: float = 1.5, a: float = 6.0):
     """Learned-refinement workload: n_wf = 24, R in {-3..3}^3 (NR = 343).
 
-    Orbitals (2w, 2w+1), w < n_nodes, carry a planted 2x2 node at a seeded
-    k_w: h_w(k) = v (sin p_x s_x + sin p_y s_y) + (m (3 - sum_a cos p_a) +
-    gap/2) s_z with p = 2 pi (k - k_w), gapped by `gap` at k_w only (a
-    Wilson term removes the other zeros of the lattice sines).  The other
+    Orbitals (2w, 2w+1), w < n_nodes, form a gapped Chern layer with one
+    planted node at a seeded k_w: with p = 2 pi (k - k_w),
+        h_w(k) = v (sin p_x s_x + sin p_y s_y) + s_w M(p) s_z,
+        M(p) = gap/2 + m_z (1 - cos p_z) - m (2 - cos p_x - cos p_y).
+    M is gap/2 at p = 0 and negative at the other three (p_x, p_y) zeros of
+    the sines, so every k_z slice of block w has Chern number -s_w, and the
+    gap closes to `gap` only at k_w (a massive Dirac point whose Berry
+    curvature peaks within ~gap / v of it).  The signs s_w are seeded with
+    sum_w s_w = +-2, so sigma_xy is of order 2 / (2 pi a) a.u.  The other
     orbitals are background bands centred at -bg and +bg (half each) with
     hoppings N(0, bg_hop) exp(-|R|/decay) eV, and every node orbital couples
     to them with N(0, couple) exp(-|R|/decay) eV.  E_F = 0 lies in the gap of
-    every node, so at T = 0 the model is an insulator whose Berry curvature
-    is smooth except near the eight nodes."""
+    every node."""
     rng = np.random.default_rng(seed)
     R = cube_R(L)
     nR = R.shape[1]
@@ -190,23 +195,28 @@ def make_c4(seed: int = 4, n: int = 24, L: int = 3, n_nodes: int = 8, gap_eV: fl
     X0 = 0.5 * (X0 + X0.conj().T)
     X0[nw:, nw:] += np.diag(np.where(np.arange(nb) < nb // 2, -bg_eV, bg_eV))
     H[:, :, idx[(0, 0, 0)]] += X0
-    # planted nodes
+    # planted Chern layers: (n_nodes + 2) / 2 blocks of one sign, the rest
+    # of the other, in seeded order and overall sign
+    sw = np.array([1.0] * ((n_nodes + 2) // 2) + [-1.0] * ((n_nodes - 2) // 2))
+    sw = rng.permutation(sw) * rng.choice([-1.0, 1.0])
     sx, sy, sz = _PAULI
     kw = rng.uniform(-0.5, 0.5, size=(3, n_nodes))
     e = np.eye(3, dtype=int)
+    # M = (gap/2 + m_z - 2 m) + m cos p_x + m cos p_y - m_z cos p_z
+    cos_coef = (m_eV, m_eV, -mz_eV)
     for w in range(n_nodes):
         o = slice(2 * w, 2 * w + 2)
-        H[o, o, idx[(0, 0, 0)]] += (3.0 * m_eV + 0.5 * gap_eV) * sz
+        H[o, o, idx[(0, 0, 0)]] += sw[w] * (0.5 * gap_eV + mz_eV - 2.0 * m_eV) * sz
         for a_ in range(3):
             ph = np.exp(-2j * np.pi * kw[a_, w])  # e^{-2 pi i k_w.a}
             plus, minus = idx[tuple(e[a_])], idx[tuple(-e[a_])]
-            # -m cos p_a s_z
-            H[o, o, plus] += -0.5 * m_eV * ph * sz
-            H[o, o, minus] += -0.5 * m_eV * np.conj(ph) * sz
+            # c cos p_a s_z = c (e^{i p} + e^{-i p}) / 2 s_z
+            H[o, o, plus] += 0.5 * sw[w] * cos_coef[a_] * ph * sz
+            H[o, o, minus] += 0.5 * sw[w] * cos_coef[a_] * np.conj(ph) * sz
             if a_ < 2:
                 s_ = sx if a_ == 0 else sy
                 # v sin p_a s_a = v (e^{i p} - e^{-i p}) / (2i) s_a
                 H[o, o, plus] += v_eV * ph / 2j * s_
                 H[o, o, minus] += -v_eV * np.conj(ph) / 2j * s_
     return dict(R_frac=R, H=np.asfortranarray(H / AU_TO_EV), r=None, wf_centers=None,
-                a_latt=a * np.eye(3), name=f"c4_n{n}_nr{nR}", nodes=kw)
+                a_latt=a * np.eye(3), name=f"c4_n{n}_nr{nR}", nodes=kw, signs=sw)

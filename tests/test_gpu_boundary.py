@@ -168,3 +168,32 @@ def test_direct_vectors_just_above_the_separation_threshold(n):
         assert np.max(np.abs(e[:, ik] - eo)) < E_TOL
         assert _rel(np.abs(V[..., ik]) ** 2, np.abs(Vo) ** 2) < 1e-8
         assert _rel(np.einsum("ann->an", V[..., ik]), np.einsum("ann->an", Vo)) < 1e-9
+
+
+def test_textbook_switch_matches_oracle():
+    """MEP_F_TEXTBOOK (PRB 74 195118 as printed) with a position operator:
+    A, Om and V per k, and the run sums including the Chern-Simons terms,
+    against the oracle's textbook path (pinned by mathematics on the CPU)."""
+    import oracle as O
+    from mepinterp_amd import capi, models
+    m = models.make_c1(seed=12)
+    k = rand_kpts(12, seed=13)
+    om = oracle_model(m, textbook=True)
+    kw = dict(n_eF=2, eF_min=-0.01, eF_max=0.01, n_hw=2, hw_min=0.0, hw_max=0.1,
+              T_kelvin=300.0, eta=0.004, valence_bands=4)
+    with _engine(m, textbook=True) as eng:
+        e, V, A, Om = eng.interp(k)
+        g = eng.run_mesh((4, 4, 3), capi.Engine.params(
+            jobs=capi.MEP_JOB_AHC | capi.MEP_JOB_KUBO | capi.MEP_JOB_MEP, **kw))
+    for ik in range(k.shape[1]):
+        eo, Vo, Ao, Omo = O.get_wann_interp(om, k[:, ik])
+        assert np.max(np.abs(e[:, ik] - eo)) < E_TOL
+        assert _rel(V[..., ik], Vo) < 1e-8
+        assert _rel(A[..., ik], Ao) < 1e-8
+        assert _rel(Om[..., ik], Omo) < 1e-8
+    o = O.accumulate(om, O.RunParams(do_ahc=True, do_kubo=True, do_mep=True, **kw),
+                     O.mp_mesh((4, 4, 3)))
+    for key in ("Ne", "ahc", "ahc_velo", "ohc", "kubo_ic", "kubo_lc", "kubo_cs", "niu_ic",
+                "niu_lc", "niu_cs"):
+        assert np.max(np.abs(o[key])) > 0, key
+        assert _rel(g[key], o[key]) < 1e-8, key
