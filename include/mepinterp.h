@@ -158,6 +158,23 @@ int mep_mesh_kpts(mep_ctx *ctx, const int32_t mp_grid[3], int64_t k0, int64_t nk
 int mep_run_kpts(mep_ctx *ctx, int64_t nk, const double *kpts, const double *weights,
                  const mep_run_params *p, mep_sums *out);
 
+/* mep_run_kpts without device allocation.  No counterpart in the reference;
+   for callers that cannot allocate device memory: code under HIP graph
+   capture, tight serving loops, harnesses that forbid hipMalloc.  Every work
+   buffer of the call (k-points, phase tile, H(k), eigenvectors, velocities,
+   eigensolver records, band-sum records) comes from one file-scope device
+   array of MEP_STATIC_SCRATCH_BYTES on each GPU, held until the process
+   ends; the model stays where mep_ctx_create put it.  Same kernels, same k
+   -> workgroup map and same sums as mep_run_kpts on the same list.
+   Limit: the buffers of the nk k-points must fit the array (about 190 KB per
+   k-point at n_wf = 32, NR = 343, so about 350 k-points); a larger nk returns
+   MEP_ERR_ARG and mep_last_error() names the largest nk that fits.  Every
+   call on one GPU uses that GPU's array: calls that overlap on one GPU
+   (other threads, other contexts) race.                                    */
+#define MEP_STATIC_SCRATCH_BYTES (64ull << 20)
+int mep_run_kpts_static(mep_ctx *ctx, int64_t nk, const double *kpts, const double *weights,
+                        const mep_run_params *p, mep_sums *out);
+
 /* Kernel timing with HIP events on the context's stream (off by default).
    name: "phase", "ft_gemm" (the Fourier sums: separable on meshes, GEMM for
    k lists), "eigh", "refine", "velo", "bandsum", "reduce", "kgen".          */

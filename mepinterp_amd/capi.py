@@ -32,7 +32,7 @@ MEP_PREC_MIXED = 1
 EXPORTS = ("mep_ctx_create", "mep_ctx_destroy", "mep_last_error", "mep_interp_batch",
            "mep_bandsum_batch", "mep_run_mesh", "mep_run_kpts", "mep_set_profiling",
            "mep_get_kernel_time", "mep_reset_kernel_times", "mep_set_kblock", "mep_curv_batch",
-           "mep_run_batch", "mep_set_workspace_limit", "mep_mesh_kpts",
+           "mep_run_batch", "mep_set_workspace_limit", "mep_mesh_kpts", "mep_run_kpts_static",
            "mep_mlp_create", "mep_mlp_destroy", "mep_mlp_num_params", "mep_mlp_get_params",
            "mep_mlp_set_params", "mep_mlp_train", "mep_mlp_predict", "mep_mlp_grad")
 
@@ -85,6 +85,8 @@ def load(path: str = LIB_PATH):
     lib.mep_run_kpts.argtypes = [C.c_void_p, C.c_int64, c_dp, c_dp, C.POINTER(RunParams),
                                  C.POINTER(Sums)]
     lib.mep_mesh_kpts.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.c_int64, C.c_int64, c_dp]
+    lib.mep_run_kpts_static.argtypes = [C.c_void_p, C.c_int64, c_dp, c_dp, C.POINTER(RunParams),
+                                        C.POINTER(Sums)]
     lib.mep_set_profiling.argtypes = [C.c_void_p, C.c_int32]
     lib.mep_get_kernel_time.argtypes = [C.c_void_p, C.c_char_p, c_dp, C.POINTER(C.c_int64)]
     lib.mep_reset_kernel_times.argtypes = [C.c_void_p]
@@ -258,6 +260,17 @@ class Engine:
         out, s = self._sums_buffers(p)
         rc = load().mep_run_kpts(self.ctx, k.shape[1], _p(k), _p(w), C.byref(p), C.byref(s))
         _check(rc, "mep_run_kpts")
+        out["n_k"] = s.n_k
+        return out
+
+    def run_kpts_static(self, kpts, p: RunParams, weights=None):
+        """mep_run_kpts_static: run_kpts with the work buffers in the library's
+        static device array (no device allocation in the call)."""
+        k = np.asfortranarray(kpts, dtype=np.float64).reshape(3, -1, order="F")
+        w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+        out, s = self._sums_buffers(p)
+        rc = load().mep_run_kpts_static(self.ctx, k.shape[1], _p(k), _p(w), C.byref(p), C.byref(s))
+        _check(rc, "mep_run_kpts_static")
         out["n_k"] = s.n_k
         return out
 

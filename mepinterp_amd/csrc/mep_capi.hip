@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -47,6 +48,7 @@ size_t bandsum_lds_bytes(const BandArgs& b);
 void launch_fill_zero(hipStream_t s, double* p, int64_t n);
 
 struct mep_ctx {
+  int device = 0;  // the HIP device current at mep_ctx_create
   int64_t n = 0, nR = 0;
   int32_t flags = 0;
   bool pos = false, tb = false;
@@ -226,6 +228,10 @@ extern "C" int mep_ctx_create(int64_t n_wf, int64_t n_R, const double* R_frac,
     return fail(MEP_ERR_HIP, "mep_ctx_create: no HIP device");
 
   mep_ctx* c = new mep_ctx();
+  if (hipGetDevice(&c->device) != hipSuccess) {
+    delete c;
+    return fail(MEP_ERR_HIP, "mep_ctx_create: hipGetDevice failed");
+  }
   c->n = n_wf;
   c->nR = n_R;
   c->flags = flags;
@@ -652,7 +658,8 @@ struct BandSetup {
   int nblocks;
 };
 
-static int band_setup(mep_ctx* c, const mep_run_params* p, int64_t kblock, BandSetup& bs) {
+// validation and the record layouts of one run (no device work)
+static int band_plan(mep_ctx* c, const mep_run_params* p, int64_t kblock, BandSetup& bs) {
   if (!p) return fail(MEP_ERR_ARG, "run params are NULL");
   if (p->n_eF < 1 || p->n_hw < 1) return fail(MEP_ERR_ARG, "n_eF and n_hw must be >= 1");
   if ((p->jobs & MEP_JOB_MEP) && (p->valence_bands < 1 || p->valence_bands > c->n))
@@ -663,6 +670,10 @@ static int band_setup(mep_ctx* c, const mep_run_params* p, int64_t kblock, BandS
     return fail(MEP_ERR_ARG, "the mixed-precision path does not take a position operator");
   if (p->precision == MEP_PREC_MIXED && c->n > 32)
     return fail(MEP_ERR_ARG, "the mixed-precision path supports n_wf <= 32");
+  if (p->hw_lst)
+    for (int32_t i = 0; i < p->n_hw; ++i)
+      if (!std::isfinite(p->hw_lst[i]))
+        return fail(MEP_ERR_ARG, "hw_lst holds a non-finite frequency");
   BandArgs& b = bs.b;
   b.n = (int)c->n;
   b.jobs = p->jobs;
@@ -674,14 +685,32 @@ static int band_setup(mep_ctx* c, const mep_run_params* p, int64_t kblock, BandS
   b.kubo_tol = c->kubo_tol;
   b.pos_op = c->pos ? 1 : 0;
   b.v_herm = c->pos ? 0 : 1;  // the library builds V itself
+  b.ef = nullptr;
+  b.hw = nullptr;
+  b.fd_in = nullptr;
+  if (bandsum_lds_bytes(b) > 160 * 1024)
+    return fail(MEP_ERR_ARG, "band-sum accumulators exceed the 160 KiB LDS of one workgroup "
+                             "(reduce N_hw * N_eF)");
+  bs.L = acc_layout(b);
+  bs.ext = (p->jobs & (MEP_JOB_OPT | MEP_JOB_GYRO)) != 0;
+  bs.X = ext_layout(b);
+  if (bs.ext && (c->n > 32 || p->n_eF > 16))
+    return fail(MEP_ERR_ARG, "optical / gyrotropic sums: n_wf <= 32 and n_eF <= 16");
+  bs.nblocks = (int)(kblock < 1024 ? kblock : 1024);
+  return MEP_OK;
+}
+
+static int64_t band_record_len(const BandSetup& bs) { return bs.L.total + (bs.ext ? bs.X.total : 0); }
+
+static int band_setup(mep_ctx* c, const mep_run_params* p, int64_t kblock, BandSetup& bs) {
+  CHK(band_plan(c, p, kblock, bs));
+  BandArgs& b = bs.b;
   std::vector<double> ef, hw;
   linspace(p->eF_min, p->eF_max, p->n_eF, ef);
   if (p->hw_lst) {
     // the caller's frequency list, any spacing (velo_ahc_tens / kubo_ohc_tens
     // take hw_lst as an argument, kubo.f90:86-104, 145-164)
     hw.assign(p->hw_lst, p->hw_lst + p->n_hw);
-    for (double v : hw)
-      if (!std::isfinite(v)) return fail(MEP_ERR_ARG, "hw_lst holds a non-finite frequency");
   } else {
     linspace(p->hw_min, p->hw_max, p->n_hw, hw);
   }
@@ -699,29 +728,19 @@ static int band_setup(mep_ctx* c, const mep_run_params* p, int64_t kblock, BandS
   HIPCHK(hipMemcpy(c->dhw, hw.data(), sizeof(double) * hw.size(), hipMemcpyHostToDevice));
   b.ef = c->def;
   b.hw = c->dhw;
-  b.fd_in = nullptr;
-  if (bandsum_lds_bytes(b) > 160 * 1024)
-    return fail(MEP_ERR_ARG, "band-sum accumulators exceed the 160 KiB LDS of one workgroup "
-                             "(reduce N_hw * N_eF)");
-  bs.L = acc_layout(b);
-  bs.ext = (p->jobs & (MEP_JOB_OPT | MEP_JOB_GYRO)) != 0;
-  bs.X = ext_layout(b);
-  if (bs.ext && (c->n > 32 || p->n_eF > 16))
-    return fail(MEP_ERR_ARG, "optical / gyrotropic sums: n_wf <= 32 and n_eF <= 16");
-  const int64_t xtot = bs.ext ? bs.X.total : 0;
-  bs.nblocks = (int)(kblock < 1024 ? kblock : 1024);
-  int64_t need = (int64_t)bs.nblocks * (bs.L.total + xtot);
+  const int64_t rec = band_record_len(bs);
+  int64_t need = (int64_t)bs.nblocks * rec;
   if (need > c->part_cap) {
     hipFree(c->dpart);
     HIPCHK(hipMalloc(&c->dpart, sizeof(double) * need));
     c->part_cap = need;
   }
-  if (bs.L.total + xtot > c->acc_cap) {
+  if (rec > c->acc_cap) {
     hipFree(c->dacc);
-    HIPCHK(hipMalloc(&c->dacc, sizeof(double) * (bs.L.total + xtot)));
-    c->acc_cap = bs.L.total + xtot;
+    HIPCHK(hipMalloc(&c->dacc, sizeof(double) * rec));
+    c->acc_cap = rec;
   }
-  launch_fill_zero(c->st, c->dacc, bs.L.total + xtot);
+  launch_fill_zero(c->st, c->dacc, rec);
   launch_fill_zero(c->st, c->dpart, need);
   HIPCHK(hipGetLastError());
   return MEP_OK;
@@ -746,8 +765,36 @@ static int band_block(mep_ctx* c, BandSetup& bs, int64_t nb, const double* w, bo
   return MEP_OK;
 }
 
-static int unpack(mep_ctx* c, const BandSetup& bs, const mep_run_params* p, int64_t nk,
-                  mep_sums* out) {
+// every output member of mep_sums with its offset and length in the
+// reduced record: f(dst, offset, doubles)
+template <class F>
+static void for_each_output(const BandSetup& bs, const mep_run_params* p, mep_sums* out, F&& f) {
+  const int64_t nef = p->n_eF, nhw = p->n_hw, nv = bs.b.n_val;
+  f(out->Ne, bs.L.Ne, nef);
+  f(out->ahc, bs.L.ahc, 9 * nef);
+  f(out->ahc_velo, bs.L.velo, 18 * nhw * nef);
+  f(out->ohc, bs.L.ohc, 18 * nhw * nef);
+  f(out->kubo_ic, bs.L.kic, 9 * nef);
+  f(out->kubo_lc, bs.L.klc, 9 * nef);
+  f(out->kubo_cs, bs.L.kcs, 9 * nef);
+  if (p->jobs & MEP_JOB_MEP) {
+    f(out->niu_ic, bs.L.nic, 9 * nv);
+    f(out->niu_lc, bs.L.nlc, 9 * nv);
+    f(out->niu_cs, bs.L.ncs, 9 * nv);
+  }
+  if (p->jobs & MEP_JOB_OPT) {
+    f(out->opt_s, bs.L.total + bs.X.opt_s, 18 * nhw * nef);
+    f(out->opt_a, bs.L.total + bs.X.opt_a, 18 * nhw * nef);
+    f(out->photo, bs.L.total + bs.X.photo, 54 * nhw * nef);
+  }
+  if (p->jobs & MEP_JOB_GYRO) {
+    f(out->gyro_C, bs.L.total + bs.X.gyro_C, 18 * nef);
+    f(out->gyro_D, bs.L.total + bs.X.gyro_D, 18 * nef);
+    f(out->gyro_Dw, bs.L.total + bs.X.gyro_Dw, 18 * nhw * nef);
+  }
+}
+
+static void reduce_records(mep_ctx* c, const BandSetup& bs) {
   // the partial records accumulate over every k-block of the run
   timed(c, K_REDUCE, [&] {
     launch_reduce_parts(c->st, bs.nblocks, bs.L.total, c->dpart, c->dacc);
@@ -755,40 +802,21 @@ static int unpack(mep_ctx* c, const BandSetup& bs, const mep_run_params* p, int6
       launch_reduce_parts(c->st, bs.nblocks, bs.X.total, c->dpart + (int64_t)bs.nblocks * bs.L.total,
                           c->dacc + bs.L.total);
   });
+}
+
+static int unpack(mep_ctx* c, const BandSetup& bs, const mep_run_params* p, int64_t nk,
+                  mep_sums* out) {
+  reduce_records(c, bs);
   HIPCHK(hipGetLastError());
-  const int64_t xtot = bs.ext ? bs.X.total : 0;
-  std::vector<double> h(bs.L.total + xtot);
-  HIPCHK(hipMemcpyAsync(h.data(), c->dacc, sizeof(double) * (bs.L.total + xtot), hipMemcpyDeviceToHost,
-                        c->st));
+  const int64_t rec = band_record_len(bs);
+  std::vector<double> h(rec);
+  HIPCHK(hipMemcpyAsync(h.data(), c->dacc, sizeof(double) * rec, hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
   resolve_times(c);
   if (!out) return MEP_OK;
-  const int64_t nef = p->n_eF, nhw = p->n_hw, nv = bs.b.n_val;
-  auto cp = [&](double* dst, int64_t off, int64_t len) {
+  for_each_output(bs, p, out, [&](double* dst, int64_t off, int64_t len) {
     if (dst) memcpy(dst, h.data() + off, sizeof(double) * len);
-  };
-  cp(out->Ne, bs.L.Ne, nef);
-  cp(out->ahc, bs.L.ahc, 9 * nef);
-  cp(out->ahc_velo, bs.L.velo, 18 * nhw * nef);
-  cp(out->ohc, bs.L.ohc, 18 * nhw * nef);
-  cp(out->kubo_ic, bs.L.kic, 9 * nef);
-  cp(out->kubo_lc, bs.L.klc, 9 * nef);
-  cp(out->kubo_cs, bs.L.kcs, 9 * nef);
-  if (p->jobs & MEP_JOB_MEP) {
-    cp(out->niu_ic, bs.L.nic, 9 * nv);
-    cp(out->niu_lc, bs.L.nlc, 9 * nv);
-    cp(out->niu_cs, bs.L.ncs, 9 * nv);
-  }
-  if (p->jobs & MEP_JOB_OPT) {
-    cp(out->opt_s, bs.L.total + bs.X.opt_s, 18 * nhw * nef);
-    cp(out->opt_a, bs.L.total + bs.X.opt_a, 18 * nhw * nef);
-    cp(out->photo, bs.L.total + bs.X.photo, 54 * nhw * nef);
-  }
-  if (p->jobs & MEP_JOB_GYRO) {
-    cp(out->gyro_C, bs.L.total + bs.X.gyro_C, 18 * nef);
-    cp(out->gyro_D, bs.L.total + bs.X.gyro_D, 18 * nef);
-    cp(out->gyro_Dw, bs.L.total + bs.X.gyro_Dw, 18 * nhw * nef);
-  }
+  });
   out->n_k = nk;
   return MEP_OK;
 }
@@ -1023,4 +1051,199 @@ extern "C" int mep_run_kpts(mep_ctx* c, int64_t nk, const double* kpts, const do
                             const mep_run_params* p, mep_sums* out) {
   if (!c || nk < 1 || !kpts) return fail(MEP_ERR_ARG, "mep_run_kpts: bad argument");
   return run_common(c, nk, nullptr, kpts, weights, p, out);
+}
+
+// ---------------------------------------------------------------------------
+// mep_run_kpts_static: mep_run_kpts with every work buffer in a file-scope
+// device array, for callers that cannot allocate device memory
+// ---------------------------------------------------------------------------
+// One copy per GPU, held until the process ends.  Host code never hands the
+// array's name to a kernel or a copy: its device address comes from
+// hipGetSymbolAddress, once per device, with that device current.
+static __device__ __align__(256) unsigned char scratch[MEP_STATIC_SCRATCH_BYTES];
+
+static const int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static unsigned char* g_scratch_dev[kMaxDevices] = {};
+
+static int static_scratch(int dev, unsigned char** out) {
+  if (dev < 0 || dev >= kMaxDevices)
+    return fail(MEP_ERR_ARG, "mep_run_kpts_static: device index beyond 63");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(MEP_ERR_HIP, std::string("mep_run_kpts_static: hipGetSymbolAddress: ") +
+                                   hipGetErrorString(e));
+    g_scratch_dev[dev] = static_cast<unsigned char*>(ptr);
+  }
+  *out = g_scratch_dev[dev];
+  return MEP_OK;
+}
+
+// get_linspace (matrix_math.f90:74-100) on the device, rounded as the host's
+// lo + i * ((hi - lo) / (n - 1)) is (no fused multiply-add)
+__global__ void linspace_kernel(double lo, double hi, int n, double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (n == 1) {
+    out[0] = lo;
+    return;
+  }
+  const double d = (hi - lo) / (double)(n - 1);
+  out[i] = __dadd_rn(lo, __dmul_rn((double)i, d));
+}
+
+static void launch_linspace(hipStream_t s, double lo, double hi, int n, double* out) {
+  hipLaunchKernelGGL(linspace_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, lo, hi, n,
+                     out);
+}
+
+// the work buffers of a context, as ensure_ws and band_setup size them
+struct WsPtrs {
+  double *dk = nullptr, *dw = nullptr, *dP = nullptr, *dE = nullptr, *dscr = nullptr;
+  zd *dHk = nullptr, *dU = nullptr, *dV = nullptr, *dA = nullptr, *dOm = nullptr;
+  double *def = nullptr, *dhw = nullptr, *dpart = nullptr, *dacc = nullptr;
+};
+
+// Carves the buffers of kc k-points and one band-sum record set out of
+// base, each 256-byte aligned; returns the bytes used (with base == nullptr
+// only the size is computed).
+static size_t static_layout(const mep_ctx* c, int64_t kc, const BandSetup& bs,
+                            const mep_run_params* p, unsigned char* base, WsPtrs* w) {
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    off = (off + 255) & ~(size_t)255;
+    unsigned char* q = base ? base + off : nullptr;
+    off += bytes;
+    return q;
+  };
+  const int64_t n = c->n, n2 = n * n, rec = band_record_len(bs);
+  WsPtrs t;
+  t.dk = reinterpret_cast<double*>(take(sizeof(double) * 3 * kc));
+  t.dw = reinterpret_cast<double*>(take(sizeof(double) * kc));
+  t.dP = reinterpret_cast<double*>(take(sizeof(double) * c->K * kc));
+  t.dHk = reinterpret_cast<zd*>(take(sizeof(zd) * c->hk_per_k * kc));
+  t.dE = reinterpret_cast<double*>(take(sizeof(double) * n * kc));
+  t.dU = reinterpret_cast<zd*>(take(sizeof(zd) * n2 * kc));
+  t.dV = reinterpret_cast<zd*>(take(sizeof(zd) * 3 * n2 * kc));
+  t.dscr = reinterpret_cast<double*>(take(eigh_scratch_bytes((int)n, kc)));
+  if (c->pos) {
+    t.dA = reinterpret_cast<zd*>(take(sizeof(zd) * 3 * n2 * kc));
+    t.dOm = reinterpret_cast<zd*>(take(sizeof(zd) * 9 * n2 * kc));
+  }
+  t.def = reinterpret_cast<double*>(take(sizeof(double) * p->n_eF));
+  t.dhw = reinterpret_cast<double*>(take(sizeof(double) * p->n_hw));
+  t.dpart = reinterpret_cast<double*>(take(sizeof(double) * bs.nblocks * rec));
+  t.dacc = reinterpret_cast<double*>(take(sizeof(double) * rec));
+  if (w) *w = t;
+  return off;
+}
+
+// Points the context's work buffers at the carved ones for the lifetime of
+// the object; its own buffers (if any) are left untouched and restored.
+struct WsSwap {
+  mep_ctx* c;
+  WsPtrs saved;
+  int64_t kcap, ef_cap, hw_cap, part_cap, acc_cap;
+  WsSwap(mep_ctx* c_, const WsPtrs& w, int64_t kc, const BandSetup& bs, const mep_run_params* p)
+      : c(c_) {
+    saved.dk = c->dk; saved.dw = c->dw; saved.dP = c->dP; saved.dHk = c->dHk; saved.dE = c->dE;
+    saved.dU = c->dU; saved.dV = c->dV; saved.dA = c->dA; saved.dOm = c->dOm;
+    saved.dscr = c->dscr; saved.def = c->def; saved.dhw = c->dhw; saved.dpart = c->dpart;
+    saved.dacc = c->dacc;
+    kcap = c->kcap; ef_cap = c->ef_cap; hw_cap = c->hw_cap; part_cap = c->part_cap;
+    acc_cap = c->acc_cap;
+    c->dk = w.dk; c->dw = w.dw; c->dP = w.dP; c->dHk = w.dHk; c->dE = w.dE; c->dU = w.dU;
+    c->dV = w.dV; c->dA = w.dA; c->dOm = w.dOm; c->dscr = w.dscr; c->def = w.def;
+    c->dhw = w.dhw; c->dpart = w.dpart; c->dacc = w.dacc;
+    c->kcap = kc;
+    c->ef_cap = p->n_eF;
+    c->hw_cap = p->n_hw;
+    c->part_cap = (int64_t)bs.nblocks * band_record_len(bs);
+    c->acc_cap = band_record_len(bs);
+  }
+  ~WsSwap() {
+    c->dk = saved.dk; c->dw = saved.dw; c->dP = saved.dP; c->dHk = saved.dHk; c->dE = saved.dE;
+    c->dU = saved.dU; c->dV = saved.dV; c->dA = saved.dA; c->dOm = saved.dOm;
+    c->dscr = saved.dscr; c->def = saved.def; c->dhw = saved.dhw; c->dpart = saved.dpart;
+    c->dacc = saved.dacc;
+    c->kcap = kcap; c->ef_cap = ef_cap; c->hw_cap = hw_cap; c->part_cap = part_cap;
+    c->acc_cap = acc_cap;
+  }
+};
+
+struct DeviceRestore {
+  int dev;
+  bool on;
+  ~DeviceRestore() {
+    if (on) hipSetDevice(dev);
+  }
+};
+
+extern "C" int mep_run_kpts_static(mep_ctx* c, int64_t nk, const double* kpts,
+                                   const double* weights, const mep_run_params* p, mep_sums* out) {
+  if (!c || nk < 1 || !kpts || !p) return fail(MEP_ERR_ARG, "mep_run_kpts_static: bad argument");
+  // one block of nk k-points, as mep_run_kpts takes a list that fits its
+  // workspace: the same k -> workgroup map and record count, the same sums
+  BandSetup bs;
+  CHK(band_plan(c, p, nk, bs));
+  const size_t need = static_layout(c, nk, bs, p, nullptr, nullptr);
+  if (need > MEP_STATIC_SCRATCH_BYTES) {
+    int64_t lo = 0, hi = nk - 1;  // the largest k-count that fits, for the message
+    while (lo < hi) {
+      const int64_t mid = (lo + hi + 1) / 2;
+      BandSetup b2;
+      if (band_plan(c, p, mid, b2) == MEP_OK &&
+          static_layout(c, mid, b2, p, nullptr, nullptr) <= MEP_STATIC_SCRATCH_BYTES)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    return fail(MEP_ERR_ARG, "mep_run_kpts_static: " + std::to_string(nk) + " k-points need " +
+                                 std::to_string(need) + " bytes of work buffers; the static "
+                                 "scratch holds " + std::to_string(MEP_STATIC_SCRATCH_BYTES) +
+                                 " bytes (at most " + std::to_string(lo) +
+                                 " k-points for this model and these parameters)");
+  }
+  int cur = 0;
+  HIPCHK(hipGetDevice(&cur));
+  DeviceRestore restore{cur, cur != c->device};
+  if (cur != c->device) HIPCHK(hipSetDevice(c->device));
+  unsigned char* base = nullptr;
+  CHK(static_scratch(c->device, &base));
+  WsPtrs w;
+  static_layout(c, nk, bs, p, base, &w);
+  WsSwap swap(c, w, nk, bs, p);
+  // Fermi levels and frequencies (get_linspace, or the caller's hw_lst);
+  // the records start from zero: the array keeps the previous call's data
+  bs.b.ef = c->def;
+  bs.b.hw = c->dhw;
+  launch_linspace(c->st, p->eF_min, p->eF_max, p->n_eF, c->def);
+  if (p->hw_lst)
+    HIPCHK(hipMemcpyAsync(c->dhw, p->hw_lst, sizeof(double) * p->n_hw, hipMemcpyHostToDevice,
+                          c->st));
+  else
+    launch_linspace(c->st, p->hw_min, p->hw_max, p->n_hw, c->dhw);
+  const int64_t rec = band_record_len(bs);
+  launch_fill_zero(c->st, c->dacc, rec);
+  launch_fill_zero(c->st, c->dpart, (int64_t)bs.nblocks * rec);
+  HIPCHK(hipGetLastError());
+  CHK(run_block(c, bs, 0, nk, nullptr, kpts, weights, p));
+  CHK(check_info(c));
+  reduce_records(c, bs);
+  HIPCHK(hipGetLastError());
+  // each sum straight from the device record into the caller's array
+  hipError_t e = hipSuccess;
+  if (out)
+    for_each_output(bs, p, out, [&](double* dst, int64_t off, int64_t len) {
+      if (dst && e == hipSuccess)
+        e = hipMemcpyAsync(dst, c->dacc + off, sizeof(double) * len, hipMemcpyDeviceToHost, c->st);
+    });
+  HIPCHK(e);
+  HIPCHK(hipStreamSynchronize(c->st));
+  resolve_times(c);
+  if (out) out->n_k = nk;
+  return MEP_OK;
 }
