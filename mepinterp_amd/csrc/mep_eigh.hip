@@ -549,6 +549,151 @@ __global__ __launch_bounds__(64) void eigh_ql_kernel(int n, int64_t nk,
   }
 }
 
+// Pass 1 of the QL stage (eigenvalues and the separation verdict) with d and
+// e in registers: one lane per k-point, NMAX <= 32, every array index a
+// compile-time constant, no LDS.  The active block always starts at index
+// 0: when e[0] is negligible, d[0] is an eigenvalue and both arrays rotate
+// down by one (d[0] moves to the end), so after NMAX deflations d[j] holds
+// the j-th eigenvalue found, where ql_one's LDS column holds it.  Entries
+// n .. NMAX-1 are decoupled padding (e = 0) that deflates as soon as it
+// reaches index 0.  Each trip of the wave loop first runs the pending
+// deflations of every lane and then one QL sweep, unrolled over the NMAX-1
+// rotation steps: the steps at or beyond a lane's m are masked off (and
+// skipped by the whole wave once no lane needs them).  Same arithmetic in
+// the same order as ql_one: identical eigenvalues and verdicts.
+template <int NMAX, class R>
+__global__ __launch_bounds__(64) void eigh_ql_reg_kernel(int n, int64_t nk,
+                                                         R* __restrict__ side,
+                                                         double* __restrict__ E,
+                                                         int* __restrict__ info,
+                                                         int* __restrict__ list,
+                                                         int* __restrict__ count) {
+  const R EPS = cplx_of<R>::eps;
+  const int lane = threadIdx.x;
+  for (int64_t kb = (int64_t)blockIdx.x * 64; kb < nk; kb += (int64_t)gridDim.x * 64) {
+    const int64_t k = kb + lane;
+    const bool valid = k < nk;
+    R d[NMAX], e[NMAX];
+    {
+      const R* sd = side + (valid ? k : (int64_t)0) * side_len(n);
+#pragma unroll
+      for (int i = 0; i < NMAX; ++i) {
+        d[i] = (valid && i < n) ? sd[i] : R(0);
+        e[i] = (valid && i < n) ? sd[n + i] : R(0);  // e[n-1] = 0 in the record
+      }
+    }
+    int ndef = valid ? 0 : NMAX;  // eigenvalues found (rotations done)
+    int iter = 0;
+    bool fail = false;
+    while (ndef < NMAX) {
+      // deflations: d[0] has converged when e[0] is negligible
+      for (;;) {
+        const bool dfl = ndef < NMAX && fabs(e[0]) <= EPS * (fabs(d[0]) + fabs(d[1]));
+        if (!dfl) break;
+        const R d0 = d[0];
+#pragma unroll
+        for (int i = 0; i + 1 < NMAX; ++i) {
+          d[i] = d[i + 1];
+          e[i] = e[i + 1];
+        }
+        d[NMAX - 1] = d0;
+        e[NMAX - 1] = R(0);
+        ++ndef;
+        iter = 0;
+      }
+      if (ndef >= NMAX) break;
+      // m: the first negligible subdiagonal (the end of the active block,
+      // whose e is zero, at the latest)
+      int m = NMAX - 1;
+#pragma unroll
+      for (int i = NMAX - 2; i >= 1; --i)
+        if (fabs(e[i]) <= EPS * (fabs(d[i]) + fabs(d[i + 1]))) m = i;
+      if (++iter > 60) {
+        fail = true;
+        break;
+      }
+      R dm = d[NMAX - 1];
+#pragma unroll
+      for (int j = NMAX - 2; j >= 1; --j) dm = (j == m) ? d[j] : dm;
+      const R dl = d[0], el = e[0];
+      R g = (d[1] - dl) / (R(2) * el);
+      R r = sqrt(g * g + R(1));
+      g = dm - dl + el / (g + copysign(r, g));
+      R s = R(1), c = R(1), p = R(0);
+      bool run = true, underflow = false;
+#pragma unroll
+      for (int i = NMAX - 2; i >= 0; --i) {
+        if (run && i < m) {
+          const R f = s * e[i], b = c * e[i];
+          const R r2 = f * f + g * g;
+          R rr;
+          if constexpr (sizeof(R) == 8) {
+            rr = rsq_nr(r2);
+            r = (r2 > R(0)) ? r2 * rr : R(0);
+          } else {
+            r = sqrt(r2);
+          }
+          e[i + 1] = r;
+          if (r == R(0)) {
+            d[i + 1] = d[i + 1] - p;
+            underflow = true;
+            run = false;
+          } else {
+            if constexpr (sizeof(R) != 8) rr = R(1) / r;
+            s = f * rr;
+            c = g * rr;
+            g = d[i + 1] - p;
+            r = (d[i] - g) * s + R(2) * c * b;
+            p = s * r;
+            d[i + 1] = g + p;
+            g = c * r - b;
+          }
+        }
+      }
+      if (!underflow) {
+        d[0] -= p;
+        e[0] = g;
+      }
+#pragma unroll
+      for (int j = 1; j < NMAX; ++j)
+        if (j == m) e[j] = R(0);
+    }
+    if (fail) atomicOr(info, 1);
+    bool again = false;
+    if (valid) {
+      // ascending order by rank (ties by position), the largest modulus and
+      // the smallest gap between eigenvalues
+      R gap = R(1e30), big = R(0);
+#pragma unroll
+      for (int j = 0; j < NMAX; ++j) {
+        if (j >= n) continue;
+        int rnk = 0;
+#pragma unroll
+        for (int i = 0; i < NMAX; ++i)
+          if (i < n && i != j) {
+            rnk += (d[i] < d[j]) || (d[i] == d[j] && i < j);
+            if (i > j) gap = fmin(gap, fabs(d[i] - d[j]));
+          }
+        big = fmax(big, fabs(d[j]));
+        E[k * n + rnk] = d[j];
+      }
+      // FP32 vectors are refined in FP64 afterwards: a looser separation suffices
+      const R sep = (sizeof(R) == 8) ? R(1e-5) : R(1e-3);
+      const R verdict = (gap >= sep * big) ? R(-1) : R(-2);
+      side[k * side_len(n) + 5 * n] = verdict;
+      again = verdict == R(-2);
+    }
+    const unsigned long long msk = __ballot(again);
+    if (msk) {
+      const int leader = __ffsll((long long)msk) - 1;
+      int base = 0;
+      if (lane == leader) base = atomicAdd(count, __popcll(msk));
+      base = __shfl(base, leader, 64);
+      if (again) list[base + __popcll(msk & ((1ull << lane) - 1))] = (int)k;
+    }
+  }
+}
+
 __global__ void zero_count_kernel(int* count) { *count = 0; }
 
 // the same for 32 < n <= 128: 32 lanes per workgroup, d and e in dynamic LDS
@@ -1373,8 +1518,8 @@ static void launch_eigh_t(hipStream_t s, int n, int64_t nk, const typename cplx_
     hipLaunchKernelGGL((eigh_house_kernel<NM, R>), dim3(grid), dim3(64), 0, s, n, nk, Hk, strideHk, \
                        packed, g.kpts, g.tau, U, side);                                          \
   hipLaunchKernelGGL(zero_count_kernel, dim3(1), dim3(1), 0, s, count);                        \
-  hipLaunchKernelGGL((eigh_ql_kernel<NM, R>), dim3(qgrid), dim3(64), 0, s, n, nk, side, rot,     \
-                     rcol, E, info, 1, list, count);                                             \
+  hipLaunchKernelGGL((eigh_ql_reg_kernel<NM, R>), dim3(qgrid), dim3(64), 0, s, n, nk, side, E,  \
+                     info, list, count);                                                         \
   hipLaunchKernelGGL((eigh_ql_kernel<NM, R>), dim3(qgrid), dim3(64), 0, s, n, nk, side, rot,     \
                      rcol, E, info, 2, list, count);                                             \
   hipLaunchKernelGGL((eigh_vec_kernel<NM, R>), dim3(grid), dim3(64), 0, s, n, nk, side, rot,     \
