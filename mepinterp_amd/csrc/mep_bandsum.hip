@@ -100,7 +100,7 @@ __global__ __launch_bounds__(BS_THREADS, 2) void bandsum_kernel(BandArgs b, doub
   // and the accumulators while V is formed
   static_assert(!FUSED || (NMAX == 32 && PL + NMAX + NT * 9 * NMAX + NMAX + 18 * NMAX + 96 >= 2 * PL),
                 "fused velocities: U's plane must fit below the accumulators");
-  extern __shared__ double smem[];
+  extern __shared__ __attribute__((aligned(16))) double smem[];
   const int n = b.n, nef = b.n_ef, nhw = b.n_hw, nval = b.n_val;
   zd* Vs = reinterpret_cast<zd*>(smem);
   double* Rs = reinterpret_cast<double*>(Vs + 3 * PL);
@@ -177,12 +177,12 @@ __global__ __launch_bounds__(BS_THREADS, 2) void bandsum_kernel(BandArgs b, doub
       // dH_a, then V_a from the accumulators into the V planes.  Rows and
       // columns >= n stay zero in every plane, so no operand load is guarded.
       zd* Us = reinterpret_cast<zd*>(Rs);
-      for (int c = t; c < PL; c += BS_THREADS) Us[c] = zd{0.0, 0.0};
+      for (int c = t; c < PL; c += BS_THREADS) stz(Us + c, zd{0.0, 0.0});
       __syncthreads();  // U's plane is zero
 #pragma unroll
       for (int u = 0; u < NU; ++u) {
         const int idx = t + BS_THREADS * u;
-        if (idx < n2) Us[idx % n + LD * (idx / n)] = ut[u];
+        if (idx < n2) stz(Us + idx % n + LD * (idx / n), ut[u]);
       }
 #pragma unroll
       for (int u = 0; u < ND; ++u) {
@@ -195,8 +195,8 @@ __global__ __launch_bounds__(BS_THREADS, 2) void bandsum_kernel(BandArgs b, doub
         const int i = p - j * (j + 1) / 2;
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-          Vs[a * PL + j + LD * i] = zconj(dt[a][u]);
-          Vs[a * PL + i + LD * j] = dt[a][u];  // last: the diagonal keeps the value as stored
+          stz(Vs + a * PL + j + LD * i, zconj(dt[a][u]));
+          stz(Vs + a * PL + i + LD * j, dt[a][u]);  // last: the diagonal keeps the value as stored
         }
       }
       __syncthreads();
@@ -210,11 +210,11 @@ __global__ __launch_bounds__(BS_THREADS, 2) void bandsum_kernel(BandArgs b, doub
       for (int k0 = 0; k0 < NMAX; k0 += 4) {
         if (k0 >= n) break;
         const int m = k0 + kq;
-        const zd bu = Us[m + LD * bc];
+        const zd bu = ldz(Us + m + LD * bc);
         const double bsum = bu.re + bu.im;
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-          const zd x = Vs[a * PL + ar + LD * m];
+          const zd x = ldz(Vs + a * PL + ar + LD * m);
           q1[a] = __builtin_amdgcn_mfma_f64_16x16x4f64(x.re, bu.re, q1[a], 0, 0, 0);
           q2[a] = __builtin_amdgcn_mfma_f64_16x16x4f64(x.im, bu.im, q2[a], 0, 0, 0);
           q3[a] = __builtin_amdgcn_mfma_f64_16x16x4f64(x.re + x.im, bsum, q3[a], 0, 0, 0);
@@ -226,7 +226,7 @@ __global__ __launch_bounds__(BS_THREADS, 2) void bandsum_kernel(BandArgs b, doub
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           const int row = r0 + kq + 4 * v;
-          Vs[a * PL + row + LD * bc] = zd{q1[a][v] - q2[a][v], q3[a][v] - q1[a][v] - q2[a][v]};
+          stz(Vs + a * PL + row + LD * bc, zd{q1[a][v] - q2[a][v], q3[a][v] - q1[a][v] - q2[a][v]});
         }
       __syncthreads();
 #pragma unroll
@@ -235,11 +235,11 @@ __global__ __launch_bounds__(BS_THREADS, 2) void bandsum_kernel(BandArgs b, doub
       for (int k0 = 0; k0 < NMAX; k0 += 4) {
         if (k0 >= n) break;
         const int m = k0 + kq;
-        const zd uu = Us[m + LD * ar];  // A(row, m) = conj(U(m, row))
+        const zd uu = ldz(Us + m + LD * ar);  // A(row, m) = conj(U(m, row))
         const double us = uu.re - uu.im;
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-          const zd y = Vs[a * PL + m + LD * bc];
+          const zd y = ldz(Vs + a * PL + m + LD * bc);
           q1[a] = __builtin_amdgcn_mfma_f64_16x16x4f64(uu.re, y.re, q1[a], 0, 0, 0);
           q2[a] = __builtin_amdgcn_mfma_f64_16x16x4f64(-uu.im, y.im, q2[a], 0, 0, 0);
           q3[a] = __builtin_amdgcn_mfma_f64_16x16x4f64(us, y.re + y.im, q3[a], 0, 0, 0);
@@ -252,7 +252,7 @@ __global__ __launch_bounds__(BS_THREADS, 2) void bandsum_kernel(BandArgs b, doub
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           const int row = r0 + kq + 4 * v;
-          Vs[a * PL + row + LD * bc] = zd{q1[a][v] - q2[a][v], q3[a][v] - q1[a][v] - q2[a][v]};
+          stz(Vs + a * PL + row + LD * bc, zd{q1[a][v] - q2[a][v], q3[a][v] - q1[a][v] - q2[a][v]});
         }
       // the r table, E and the occupations are zero again, E in its slot
       const int zn = PL + NMAX + nef * NMAX;
@@ -280,7 +280,7 @@ __global__ __launch_bounds__(BS_THREADS, 2) void bandsum_kernel(BandArgs b, doub
 #pragma unroll
         for (int jj = 0; jj < NJ; ++jj) {
           const int j = jb + JS * jj;
-          if (i < n && j < n) Vs[a * PL + i + LD * j] = vt[a][jj];
+          if (i < n && j < n) stz(Vs + a * PL + i + LD * j, vt[a][jj]);
         }
       if (t < n) Es[t] = et;
     }
@@ -343,8 +343,8 @@ __global__ __launch_bounds__(BS_THREADS, 2) void bandsum_kernel(BandArgs b, doub
             zd x[3], y[3];
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
-              x[a] = Vs[a * PL + p0 + LD * nn];  // V_a(n0, n)
-              y[a] = Vs[a * PL + nn + LD * p0];  // V_a(n, n0)
+              x[a] = ldz(Vs + a * PL + p0 + LD * nn);  // V_a(n0, n)
+              y[a] = ldz(Vs + a * PL + nn + LD * p0);  // V_a(n, n0)
             }
             const double r = Rs[nn + LD * p0];  // 1 / (E0 - En)
             const double r2 = r * r;
@@ -368,13 +368,13 @@ __global__ __launch_bounds__(BS_THREADS, 2) void bandsum_kernel(BandArgs b, doub
           const double E0 = Es[p0];
           zd z0[3];
 #pragma unroll
-          for (int l = 0; l < 3; ++l) z0[l] = Vs[l * PL + p0 + LD * p0];
+          for (int l = 0; l < 3; ++l) z0[l] = ldz(Vs + l * PL + p0 + LD * p0);
           for (int nn = pp; nn < n; nn += PARTS) {
             zd x[3], y[3];
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
-              x[a] = Vs[a * PL + p0 + LD * nn];
-              y[a] = Vs[a * PL + nn + LD * p0];
+              x[a] = ldz(Vs + a * PL + p0 + LD * nn);
+              y[a] = ldz(Vs + a * PL + nn + LD * p0);
             }
             const double r = Rs[nn + LD * p0];  // 0 for nn == n0: adds nothing
             const double r3 = r * r * r;
@@ -419,8 +419,8 @@ __global__ __launch_bounds__(BS_THREADS, 2) void bandsum_kernel(BandArgs b, doub
           zd x[3], y[3];
 #pragma unroll
           for (int a = 0; a < 3; ++a) {
-            x[a] = Vs[a * PL + p0 + LD * nn];
-            y[a] = Vs[a * PL + nn + LD * p0];
+            x[a] = ldz(Vs + a * PL + p0 + LD * nn);
+            y[a] = ldz(Vs + a * PL + nn + LD * p0);
           }
           const double dE = E0 - Es[nn];
           const double dfv = fd[p0] - fd[nn];
@@ -470,7 +470,7 @@ __global__ __launch_bounds__(BS_THREADS, 2) void bandsum_kernel(BandArgs b, doub
       for (int nn = part; nn < n; nn += 8) {
         zd x[3];
 #pragma unroll
-        for (int a = 0; a < 3; ++a) x[a] = Vs[a * PL + n0 + LD * nn];
+        for (int a = 0; a < 3; ++a) x[a] = ldz(Vs + a * PL + n0 + LD * nn);
         const double r = Rs[nn + LD * n0];
         const double r2 = r * r, r3 = r2 * r;
         zd P[3];
@@ -528,7 +528,7 @@ __global__ __launch_bounds__(BS_THREADS, 2) void bandsum_kernel(BandArgs b, doub
       {
         zd z[3];
 #pragma unroll
-        for (int a = 0; a < 3; ++a) z[a] = Vs[a * PL + n0 + LD * n0];
+        for (int a = 0; a < 3; ++a) z[a] = ldz(Vs + a * PL + n0 + LD * n0);
         auto G = [&](const double* gd_, const zd* go_, int i, int k) {
           if (i == k) return zd{gd_[i], 0.0};
           const zd g = go_[i + k - 1];
@@ -607,8 +607,8 @@ __global__ __launch_bounds__(BS_THREADS, 2) void bandsum_kernel(BandArgs b, doub
 #pragma unroll
               for (int i = 0; i < 3; ++i) {
                 const int q = i + 3 * j;
-                const double vim = zmul(Vs[i * PL + x + LD * y], Vs[j * PL + y + LD * x]).im;
-                const zd q2o = zmul(zmul(Vs[i * PL + y + LD * x], Vs[j * PL + x + LD * y]), ro);
+                const double vim = zmul(ldz(Vs + i * PL + x + LD * y), ldz(Vs + j * PL + y + LD * x)).im;
+                const zd q2o = zmul(zmul(ldz(Vs + i * PL + y + LD * x), ldz(Vs + j * PL + x + LD * y)), ro);
                 for (int ef = 0; ef < nef; ++ef) {
                   const double dfv = wk * (fd[ef + nef * x] - fd[ef + nef * y]);
                   double* av = acc + L.velo + 2 * 9 * (h + (int64_t)nhw * ef);
@@ -662,8 +662,8 @@ __global__ __launch_bounds__(BS_THREADS, 2) void bandsum_kernel(BandArgs b, doub
           double as[NP], ss[NP];
 #pragma unroll
           for (int c = 0; c < NP; ++c) {
-            a[c] = Vs[pl[c] * PL + ar + LD * m];
-            s[c] = zscale(Vs[pl[c] * PL + m + LD * bc], rr);
+            a[c] = ldz(Vs + pl[c] * PL + ar + LD * m);
+            s[c] = zscale(ldz(Vs + pl[c] * PL + m + LD * bc), rr);
             as[c] = a[c].re + a[c].im;
             ss[c] = s[c].re + s[c].im;
           }
@@ -714,7 +714,7 @@ __global__ __launch_bounds__(BS_THREADS, 2) void bandsum_kernel(BandArgs b, doub
           }
           zd ai[3];
 #pragma unroll
-          for (int i = 0; i < 3; ++i) ai[i] = zscale(Vs[i * PL + n0 + LD * row], rr2);
+          for (int i = 0; i < 3; ++i) ai[i] = zscale(ldz(Vs + i * PL + n0 + LD * row), rr2);
 #pragma unroll
           for (int jj = 0; jj < JN; ++jj) {
             const zd cv = zd{p1[jj][v] - p2[jj][v], p3[jj][v] - p1[jj][v] - p2[jj][v]};
@@ -738,7 +738,7 @@ __global__ __launch_bounds__(BS_THREADS, 2) void bandsum_kernel(BandArgs b, doub
             const double rr2 = rr * rr;
             zd ai[3];
 #pragma unroll
-            for (int i = 0; i < 3; ++i) ai[i] = zscale(Vs[i * PL + n0 + LD * row], rr2);
+            for (int i = 0; i < 3; ++i) ai[i] = zscale(ldz(Vs + i * PL + n0 + LD * row), rr2);
 #pragma unroll
             for (int jj = 0; jj < JN; ++jj) {
               const zd* Vk_ = Vs + pl[ks(jj)] * PL;

@@ -26,6 +26,22 @@ __host__ __device__ inline zd zconj(zd a) { return zd{a.re, -a.im}; }
 __host__ __device__ inline zd zscale(zd a, double s) { return zd{a.re * s, a.im * s}; }
 __host__ __device__ inline zd zmuli(zd a) { return zd{-a.im, a.re}; }  // i*a
 __host__ __device__ inline double zabs2(zd a) { return a.re * a.re + a.im * a.im; }
+// One 16-byte access of a complex value at a 16-byte aligned address.  zd
+// is only 8-byte aligned, so a plain zd load becomes two 8-byte accesses;
+// over LDS at a 16-byte lane stride each of those is a 2-way bank conflict,
+// where one 16-byte access per lane is conflict-free.
+typedef double mep_d2 __attribute__((ext_vector_type(2)));
+__device__ inline zd ldz(const zd* p) {
+  const mep_d2 v = *reinterpret_cast<const mep_d2*>(p);
+  return zd{v.x, v.y};
+}
+__device__ inline void stz(zd* p, zd a) {
+  mep_d2 v;
+  v.x = a.re;
+  v.y = a.im;
+  *reinterpret_cast<mep_d2*>(p) = v;
+}
+
 // a / b  (textbook formula; denominators here are well scaled)
 __host__ __device__ inline zd zdiv(zd a, zd b) {
   double d = b.re * b.re + b.im * b.im;
