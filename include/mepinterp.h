@@ -227,6 +227,15 @@ int mep_set_kblock(mep_ctx *ctx, int64_t kblock);
    shrink to fit.  For several resident contexts (mep_run_batch) whose
    buffers must share the 288 GB of HBM.  No counterpart in the reference.  */
 int mep_set_workspace_limit(mep_ctx *ctx, double bytes);
+/* Kernel variant of one stage (value 0 restores the library's default).  A
+   tuning knob with no counterpart in the reference; mepinterp_amd/tune.py
+   times the variants and caches its choice.  Every variant gives the same
+   sums to rounding.
+     MEP_VAR_EIGH  FP64 Householder reduction for 16 < n <= 32:
+                   1 = one k-point per wavefront, half a row per lane (default);
+                   2 = two k-points per wavefront, one row per lane. */
+#define MEP_VAR_EIGH 1
+int mep_set_variant(mep_ctx *ctx, int32_t what, int32_t value);
 
 #ifdef __cplusplus
 }

@@ -27,12 +27,14 @@ MEP_JOB_OPT = 8
 MEP_JOB_GYRO = 16
 MEP_PREC_FP64 = 0
 MEP_PREC_MIXED = 1
+MEP_VAR_EIGH = 1
 
 # every symbol include/mepinterp.h declares
 EXPORTS = ("mep_ctx_create", "mep_ctx_destroy", "mep_last_error", "mep_interp_batch",
            "mep_bandsum_batch", "mep_run_mesh", "mep_run_kpts", "mep_set_profiling",
            "mep_get_kernel_time", "mep_reset_kernel_times", "mep_set_kblock", "mep_curv_batch",
            "mep_run_batch", "mep_set_workspace_limit", "mep_mesh_kpts", "mep_run_kpts_static",
+           "mep_set_variant",
            "mep_mlp_create", "mep_mlp_destroy", "mep_mlp_num_params", "mep_mlp_get_params",
            "mep_mlp_set_params", "mep_mlp_train", "mep_mlp_predict", "mep_mlp_grad")
 
@@ -92,6 +94,7 @@ def load(path: str = LIB_PATH):
     lib.mep_reset_kernel_times.argtypes = [C.c_void_p]
     lib.mep_set_kblock.argtypes = [C.c_void_p, C.c_int64]
     lib.mep_set_workspace_limit.argtypes = [C.c_void_p, C.c_double]
+    lib.mep_set_variant.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
     lib.mep_curv_batch.argtypes = [C.c_void_p, C.c_int64, c_dp, C.c_double, C.c_double, c_dp]
     lib.mep_run_batch.argtypes = [C.c_int32, C.POINTER(C.c_void_p), C.POINTER(C.c_int32),
                                   C.POINTER(RunParams), C.POINTER(Sums)]
@@ -293,6 +296,10 @@ class Engine:
 
     def set_workspace_limit(self, nbytes):
         _check(load().mep_set_workspace_limit(self.ctx, float(nbytes)), "mep_set_workspace_limit")
+
+    def set_variant(self, what, value):
+        """Kernel variant of one stage (VAR_EIGH: 1 or 2; 0 = the default)."""
+        _check(load().mep_set_variant(self.ctx, int(what), int(value)), "mep_set_variant")
 
     def curv(self, kpts, eF=0.0, T_kelvin=0.0):
         """Per-k Berry curvature summed over occupied bands, (3, 3, nk)."""

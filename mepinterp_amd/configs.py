@@ -84,7 +84,7 @@ def run_c3(scale: int = 1):
             t_seq += dt
             per.append(dict(model=m["name"], n_k=mp[0] ** 3, seconds=dt, k_pts_s=mp[0] ** 3 / dt))
         # autotuned k-blocks (cached), then one batched call over all models
-        kb = [tune.tune(eng, mp, rp, m["R_frac"].shape[1]) for eng, (m, mp, rp) in zip(engines, work)]
+        tuned = [tune.tune(eng, mp, rp, m["R_frac"].shape[1]) for eng, (m, mp, rp) in zip(engines, work)]
         capi.run_batch(engines, [w[1] for w in work], [w[2] for w in work])
         t0 = time.perf_counter()
         bat = capi.run_batch(engines, [w[1] for w in work], [w[2] for w in work])
@@ -98,7 +98,7 @@ def run_c3(scale: int = 1):
     return dict(config="c3", workload="16 models n_wf=8..68, NR=27..729, meshes 24^3..48^3"
                 + (f" / {scale}" if scale > 1 else ""),
                 n_k=tot_k, per_model=dict(seconds=t_seq, k_pts_s=tot_k / t_seq),
-                batched=dict(seconds=t_bat, k_pts_s=tot_k / t_bat, kblocks=kb,
+                batched=dict(seconds=t_bat, k_pts_s=tot_k / t_bat, tuned=tuned,
                              max_rel_dev_sigma_xy_vs_per_model=dev),
                 models=per)
 

@@ -79,6 +79,7 @@ struct mep_ctx {
   int64_t y1_cap = 0;
   // k-block workspace
   int64_t kcap = 0, kblock_req = 0;
+  int var_eigh = 0;  // MEP_VAR_EIGH
   double* dk = nullptr;
   double* dw = nullptr;
   double* dP = nullptr;  // (K, kblock) real phase tile
@@ -441,6 +442,16 @@ extern "C" int mep_set_kblock(mep_ctx* c, int64_t kb) {
   return MEP_OK;
 }
 
+extern "C" int mep_set_variant(mep_ctx* c, int32_t what, int32_t value) {
+  if (!c) return fail(MEP_ERR_ARG, "mep_set_variant: null context");
+  if (what == MEP_VAR_EIGH) {
+    if (value < 0 || value > 2) return fail(MEP_ERR_ARG, "mep_set_variant: MEP_VAR_EIGH takes 0, 1 or 2");
+    c->var_eigh = value;
+    return MEP_OK;
+  }
+  return fail(MEP_ERR_ARG, "mep_set_variant: unknown stage");
+}
+
 extern "C" int mep_set_workspace_limit(mep_ctx* c, double bytes) {
   if (!c || !(bytes > 0.0)) return fail(MEP_ERR_ARG, "mep_set_workspace_limit: bad argument");
   c->ws_limit = bytes;
@@ -619,7 +630,7 @@ static int interp_block(mep_ctx* c, int64_t nb, bool velo, int prec = MEP_PREC_F
                        c->dscr);
     else
       launch_eigh(c->st, (int)n, nb, c->dHk, c->ncols, c->packed, g, c->dE, c->dU, c->dinfo,
-                  c->dscr);
+                  c->dscr, c->var_eigh);
   });
   if (velo)
     timed(c, K_VELO, [&] {

@@ -161,11 +161,6 @@ __global__ __launch_bounds__(64) void eigh_house_kernel(int n, int64_t nk,
   }
 }
 
-// One lane per k-point: implicit QL on the real tridiagonal (d, e) held in
-// LDS columns.  Every Givens rotation G(i, i+1; c, s) of every sweep is
-// appended, in application order, to the k-point's flat stream
-// rot[k][q] = (c, s), col[k][q] = i; the count goes to the side record.
-// Ascending eigenvalues go to E, each band's rank to the side record.
 // Householder tridiagonalisation in FP64 for 16 < n <= 32 with one k-point
 // per wavefront: lane (t, h) = (lane % 32, lane / 32) keeps the columns
 // j = 2 jj + h (jj < 16) of row t in registers (64 VGPRs, so four wavefronts
@@ -357,6 +352,182 @@ __global__ __launch_bounds__(64, 4) void eigh_house_half_kernel(int n, int64_t n
         sd[3 * n + 2 * (i + 1) + 1] = ph.im;
       }
       sd[n + n - 1] = 0.0;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+}
+
+// Householder tridiagonalisation in FP64 for 16 < n <= 32, two k-points per
+// wavefront: lane (t, g) = (lane % 32, lane / 32) keeps the whole row t of
+// k-point 2 b + g in registers.  The cross-lane work of a column (its norm,
+// |x0|^2, v^H p) runs once for both k-points, and p = A v needs no lane swap
+// (the row is whole).  Columns j <= kc are skipped at compile time (full
+// unroll); rows t <= kc carry v = w = 0 through the update, which leaves them
+// unchanged.  Same outputs as eigh_house_kernel.
+template <bool TAU>
+__global__ __launch_bounds__(64, 2) void eigh_house_row2_kernel(int n, int64_t nk,
+                                                              const zd* __restrict__ Hk,
+                                                              int64_t strideHk, int packed,
+                                                              const double* __restrict__ kpts,
+                                                              const double* __restrict__ tau,
+                                                              zd* __restrict__ U,
+                                                              double* __restrict__ side) {
+  constexpr int NMAX = 32, LDV = NMAX + 1;  // LDV: the two k-points' vectors in other banks
+  __shared__ __attribute__((aligned(16))) zd vv[2 * LDV], ww[2 * LDV];
+  __shared__ zd sub[2 * NMAX], fph[2 * NMAX];
+  __shared__ double tauv[2 * NMAX];
+  const double two_pi = 6.283185307179586476925;
+  const zd zero = zd{0.0, 0.0};
+  for (int64_t kb = 2 * (int64_t)blockIdx.x; kb < nk; kb += 2 * (int64_t)gridDim.x) {
+    // the lane index is made opaque per pair: otherwise the compiler hoists
+    // every step's lane masks out of this loop and spills them
+    int lane = threadIdx.x & 63;
+    asm volatile("" : "+v"(lane));
+    const int t = lane & 31, g = lane >> 5;
+    const int64_t k = kb + g;
+    const bool valid = k < nk;  // the last pair may hold one k-point
+    zd* vg = vv + g * LDV;
+    zd* wg = ww + g * LDV;
+    // ---- row t: upper triangle from H(k), lower as conjugates; with
+    // centres, element (t, j) takes f_j conj(f_t), f_j = e^{2 pi i k.tau_j}
+    zd ft = zd{1.0, 0.0};
+    if (TAU) {
+      zd f = zd{1.0, 0.0};
+      if (valid && t < n) {
+        const double ang = two_pi * (kpts[3 * k] * tau[3 * t] + kpts[3 * k + 1] * tau[3 * t + 1] +
+                                     kpts[3 * k + 2] * tau[3 * t + 2]);
+        sin_cos(ang, &f.im, &f.re);
+      }
+      fph[g * NMAX + t] = f;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      ft = zconj(fph[g * NMAX + t]);
+    }
+    zd row[NMAX];
+#pragma unroll
+    for (int j = 0; j < NMAX; ++j) {
+      zd x = zero;
+      if (valid && t < n && j < n) {
+        const int i0 = j >= t ? t : j, j0 = j >= t ? j : t;  // (i0 <= j0) element
+        const int64_t o = packed ? i0 + ((int64_t)j0 * (j0 + 1)) / 2 : i0 + (int64_t)n * j0;
+        x = Hk[k * strideHk + o];
+        if (j < t) x = zconj(x);
+        if (TAU && j != t) x = zmul(x, zmul(fph[g * NMAX + j], ft));
+        if (j == t) x.im = 0.0;
+      }
+      row[j] = x;
+    }
+#pragma clang loop unroll(full)
+    for (int kc = 0; kc < NMAX - 2; ++kc) {
+      if (kc + 2 >= n) continue;  // uniform; every index below stays compile-time
+      const bool below = t > kc;  // rows kc+1 .. n-1 take part
+      const zd xo = row[kc];      // A(t, kc)
+      const double a2 = zabs2(xo);
+      const double tail = team_sum<32, double>(t > kc + 1 ? a2 : 0.0);
+      const double ax02 = team_bcast<32>(a2, kc + 1);  // |A(kc+1, kc)|^2 of this k-point
+      const bool refl = tail > 0.0;
+      const double xn = sqrt(tail + ax02);
+      const double ax0 = sqrt(ax02);
+      // alpha is formed in every lane from its own A(t, kc); lane kc + 1's
+      // (x0 = A(kc+1, kc)) is the one kept
+      const zd alpha = !refl ? xo : ((ax0 > 0.0) ? zscale(xo, -xn * rcp_nr(ax0)) : zd{-xn, 0.0});
+      zd v = (refl && below) ? xo : zero;
+      if (t == kc + 1 && refl) v = zsub(xo, alpha);
+      // |v|^2 = |x|^2 - 2 Re(conj(alpha) x0) + |alpha|^2 = 2 xn (xn + |x0|)
+      const double vn2 = 2.0 * xn * (xn + ax0);
+      const double tk = refl ? 2.0 * rcp_nr(vn2) : 0.0;
+      stz(vg + t, v);
+      if (t == kc + 1) {
+        sub[g * NMAX + kc] = alpha;
+        tauv[g * NMAX + kc] = tk;
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      // p_t = tk sum_{j > kc} A(t, j) v_j, four interleaved chains
+      zd p4[4] = {zero, zero, zero, zero};
+#pragma unroll
+      for (int j = 0; j < NMAX; ++j) {
+        if (j <= kc) continue;
+        const zd a_ = row[j], b_ = ldz(vg + j);
+        zd& q = p4[(j - kc - 1) & 3];
+        if (j - kc - 1 < 4) {  // the first product of each chain sets it
+          q.re = fma(a_.re, b_.re, -a_.im * b_.im);
+          q.im = fma(a_.re, b_.im, a_.im * b_.re);
+        } else {
+          q.re = fma(a_.re, b_.re, fma(-a_.im, b_.im, q.re));
+          q.im = fma(a_.re, b_.im, fma(a_.im, b_.re, q.im));
+        }
+      }
+      zd p = zadd(zadd(p4[0], p4[1]), zadd(p4[2], p4[3]));
+      p = below ? zscale(p, tk) : zero;
+      const double K = 0.5 * tk * team_sum<32, double>(v.re * p.re + v.im * p.im);
+      const zd w = zsub(p, zscale(v, K));
+      stz(wg + t, w);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+      for (int j = 0; j < NMAX; ++j) {
+        if (j <= kc) continue;
+        // row_j -= v conj(w_j) + w conj(v_j)
+        const zd vj = ldz(vg + j), wj = ldz(wg + j);
+        zd r_ = row[j];
+        r_.re = fma(-v.re, wj.re, fma(-v.im, wj.im, fma(-w.re, vj.re, fma(-w.im, vj.im, r_.re))));
+        r_.im = fma(-v.im, wj.re, fma(v.re, wj.im, fma(-w.im, vj.re, fma(w.re, vj.im, r_.im))));
+        row[j] = r_;
+      }
+      if (below) row[kc] = v;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    // ---- outputs -----------------------------------------------------------
+    double dgl = 0.0;  // A(t, t)
+#pragma unroll
+    for (int j = 0; j < NMAX; ++j)
+      if (j == t) dgl = row[j].re;
+    {
+      zd last = zero;  // A(n-1, n-2), held by lane n - 1
+#pragma unroll
+      for (int j = 0; j < NMAX; ++j)
+        if (j == n - 2) last = row[j];
+      if (t == n - 1) {
+        sub[g * NMAX + n - 2] = last;
+        tauv[g * NMAX + n - 2] = 0.0;
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (valid) {
+      double* sd = side + k * side_len(n);
+      zd* Uk = U + k * (int64_t)n * n;
+      // reflectors: row t keeps v_t of every column kc < t
+#pragma unroll
+      for (int j = 0; j < NMAX; ++j)
+        if (j < t && t < n) Uk[t + (int64_t)n * j] = row[j];
+      if (t < n) {
+        sd[t] = dgl;
+        sd[2 * n + t] = (t < n - 1) ? tauv[g * NMAX + t] : 0.0;
+      }
+      if (t == 0) {
+        zd ph = zd{1.0, 0.0};
+        sd[3 * n] = 1.0;
+        sd[3 * n + 1] = 0.0;
+        for (int i = 0; i + 1 < n; ++i) {
+          const zd s_ = sub[g * NMAX + i];
+          const double as = sqrt(zabs2(s_));
+          sd[n + i] = as;
+          if (as > 0.0) ph = zmul(ph, zscale(s_, 1.0 / as));
+          sd[3 * n + 2 * (i + 1)] = ph.re;
+          sd[3 * n + 2 * (i + 1) + 1] = ph.im;
+        }
+        sd[n + n - 1] = 0.0;
+      }
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -1494,7 +1665,7 @@ size_t eigh_scratch_bytes(int n, int64_t nk) {
 template <class R>
 static void launch_eigh_t(hipStream_t s, int n, int64_t nk, const typename cplx_of<R>::type* Hk,
                           int64_t strideHk, int packed, const KGeom& g, double* E,
-                          typename cplx_of<R>::type* U, int* info, void* scratch) {
+                          typename cplx_of<R>::type* U, int* info, void* scratch, int house) {
   R* side = reinterpret_cast<R*>(scratch);
   R* rot = side + nk * side_len(n);
   uint8_t* rcol = reinterpret_cast<uint8_t*>(rot + nk * 2 * rot_cap(n));
@@ -1506,9 +1677,18 @@ static void launch_eigh_t(hipStream_t s, int n, int64_t nk, const typename cplx_
   int64_t qb = (nk + 63) / 64;
   unsigned qgrid = (unsigned)(qb < 65536 ? qb : 65536);
   const unsigned hgrid = (unsigned)(nk < 65536 ? nk : 65536);  // one k per wavefront
+  const int64_t h2 = (nk + 1) / 2;
+  const unsigned h2grid = (unsigned)(h2 < 65536 ? h2 : 65536);  // two k per wavefront
 #define MEP_EIGH_LAUNCH(NM)                                                                      \
   if constexpr (NM == 32 && sizeof(R) == 8) {                                                    \
-    if (g.tau)                                                                                   \
+    if (house == 2) {                                                                            \
+      if (g.tau)                                                                                 \
+        hipLaunchKernelGGL(eigh_house_row2_kernel<true>, dim3(h2grid), dim3(64), 0, s, n, nk,    \
+                           Hk, strideHk, packed, g.kpts, g.tau, U, side);                        \
+      else                                                                                       \
+        hipLaunchKernelGGL(eigh_house_row2_kernel<false>, dim3(h2grid), dim3(64), 0, s, n, nk,   \
+                           Hk, strideHk, packed, g.kpts, g.tau, U, side);                        \
+    } else if (g.tau)                                                                            \
       hipLaunchKernelGGL(eigh_house_half_kernel<true>, dim3(hgrid), dim3(64), 0, s, n, nk, Hk,    \
                          strideHk, packed, g.kpts, g.tau, U, side);                              \
     else                                                                                         \
@@ -1535,11 +1715,11 @@ static void launch_eigh_t(hipStream_t s, int n, int64_t nk, const typename cplx_
 }
 
 void launch_eigh(hipStream_t s, int n, int64_t nk, const zd* Hk, int64_t strideHk, int packed,
-                 const KGeom& g, double* E, zd* U, int* info, double* scratch) {
-  launch_eigh_t<double>(s, n, nk, Hk, strideHk, packed, g, E, U, info, scratch);
+                 const KGeom& g, double* E, zd* U, int* info, double* scratch, int house) {
+  launch_eigh_t<double>(s, n, nk, Hk, strideHk, packed, g, E, U, info, scratch, house);
 }
 
 void launch_eigh_f32(hipStream_t s, int n, int64_t nk, const zf* Hk, int64_t strideHk, int packed,
                      const KGeom& g, double* E, zf* U, int* info, double* scratch) {
-  launch_eigh_t<float>(s, n, nk, Hk, strideHk, packed, g, E, U, info, scratch);
+  launch_eigh_t<float>(s, n, nk, Hk, strideHk, packed, g, E, U, info, scratch, 0);
 }

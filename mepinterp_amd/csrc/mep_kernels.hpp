@@ -45,8 +45,10 @@ void launch_ft_gemm_f32(hipStream_t s, int M, int N, int K, const float* Pt, con
 // holds eigh_scratch_bytes(n, nk) bytes (side records + rotation records).
 // packed: H(k) holds the upper triangle only (element i <= j at i + j(j+1)/2).
 size_t eigh_scratch_bytes(int n, int64_t nk);
+// house: FP64 Householder variant for 16 < n <= 32 (1 = one k-point per
+// wavefront, 2 = two; 0 = the default, 1)
 void launch_eigh(hipStream_t s, int n, int64_t nk, const zd* Hk, int64_t strideHk, int packed,
-                 const KGeom& g, double* E, zd* U, int* info, double* scratch);
+                 const KGeom& g, double* E, zd* U, int* info, double* scratch, int house = 0);
 void launch_eigh_wide(hipStream_t s, int n, int64_t nk, const zd* Hk, int64_t strideHk, int packed,
                       const KGeom& g, double* E, zd* U, int* info, double* scratch);
 // the same in FP32 arithmetic (FP32 H(k) and U; eigenvalues written as FP64)

@@ -151,3 +151,56 @@ def test_c5_small_mesh_matches_oracle():
     os_ = host.finalize(o, p, mp, m["a_latt"])
     for key in ("mep_tens", "kubo_mep_tens", "ahcVELO", "ohcVELO"):
         assert _rel(gs[key], os_[key]) < REL, key
+
+
+@pytest.mark.parametrize("centres", [False, True])
+def test_c2_householder_variants_match_oracle(centres):
+    """Both FP64 Householder kernels for 16 < n <= 32 (one k-point per
+    wavefront with half rows per lane; two k-points per wavefront with whole
+    rows) against the oracle, with and without tight-binding centres, on an
+    odd k count (the two-k kernel's last pair holds one k-point)."""
+    import oracle as O
+    from mepinterp_amd import capi, host, models
+    m = models.make_c2(seed=2)
+    if centres:
+        rng = np.random.default_rng(7)
+        m = dict(m, wf_centers=np.asfortranarray(rng.uniform(0.0, 1.0, (3, m["H"].shape[0]))))
+    rng = np.random.default_rng(11)
+    k = np.asfortranarray(rng.uniform(-0.5, 0.5, (3, 301)))
+    p = dict(ALL_JOBS, n_eF=1, eF_min=0.0, eF_max=0.0, n_hw=1, hw_min=0.0, hw_max=0.0,
+             T_kelvin=300.0, eta=1e-3 / AU_TO_EV, valence_bands=16)
+    pk = {kk: v for kk, v in p.items() if kk in O.RunParams.__init__.__code__.co_varnames}
+    o = O.accumulate(oracle_model(m), O.RunParams(**pk), k)
+    with _engine(m) as eng:
+        for var in (1, 2):
+            eng.set_variant(capi.MEP_VAR_EIGH, var)
+            g = eng.run_kpts(k, host.run_params(p))
+            assert g["n_k"] == 301
+            _compare(g, o, what=f"C2 eigh variant {var} centres={centres}")
+        with pytest.raises(RuntimeError):
+            eng.set_variant(capi.MEP_VAR_EIGH, 3)
+        with pytest.raises(RuntimeError):
+            eng.set_variant(99, 1)
+
+
+def test_tuner_picks_and_caches(tmp_path):
+    """tune.tune times the eigensolver variants and k-blocks once, applies the
+    choice, and reads it back from its cache on the next call."""
+    import json
+    from mepinterp_amd import host, models, tune
+    m = models.make_c2(seed=2)
+    p = dict(ALL_JOBS, n_eF=1, eF_min=0.0, eF_max=0.0, n_hw=1, hw_min=0.0, hw_max=0.0,
+             T_kelvin=300.0, eta=1e-3 / AU_TO_EV, valence_bands=16)
+    rp = host.run_params(p)
+    cache = str(tmp_path / "tune.json")
+    mp = (16, 16, 16)
+    with _engine(m) as eng:
+        c1 = tune.tune(eng, mp, rp, m["R_frac"].shape[1], reps=1, path=cache)
+        assert c1["eigh"] in (1, 2) and c1["kblock"] in tune.KBLOCKS
+        table = json.load(open(cache))
+        entry = table[tune.key(32, m["R_frac"].shape[1], 0, 16 ** 3)]
+        assert set(entry["seconds_eigh"]) == {"1", "2"}
+        c2 = tune.tune(eng, mp, rp, m["R_frac"].shape[1], reps=1, path=cache)
+        assert c2 == c1
+        s = eng.run_mesh(mp, rp)
+        assert s["n_k"] == 16 ** 3
