@@ -64,7 +64,7 @@ struct VeloArgs {
   int pos_op;
   int textbook;
   double kubo_tol;
-  zd* scratch;  // n > 32: n x n per workgroup (up to 2048 workgroups)
+  zd* scratch;  // n > 32: 3 n x n per workgroup (up to 2048 workgroups)
 };
 void launch_velo(hipStream_t s, const VeloArgs& va, int64_t nk, const zd* Hk, int64_t strideHk,
                  const KGeom& g, const zd* U, const double* E, zd* V, zd* A, zd* Om);

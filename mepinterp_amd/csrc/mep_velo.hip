@@ -248,61 +248,79 @@ __global__ __launch_bounds__(256, 2) void velo_mfma_kernel(int n, int64_t nk,
 }
 
 // --------------------------------------------------------------------------
-// 32 < n <= 128, no position operator: V_a = U^H dH_a U on FP64 MFMA with
-// the operands read from HBM / L2 (16 x 16 output tiles spread over the 4
-// waves, three real products per complex one).  T = dH_a U goes to a
-// per-workgroup slot Tscr[blockIdx] of n x n.
+// 32 < n <= 128, no position operator: V_a = U^H dH_a U on FP64 MFMA, three
+// real products per complex one.  Pass 0 forms T_a = dH_a U for a = x, y, z
+// in the workgroup's slot Tscr[blockIdx] (3 n^2 complex), pass 1 V_a =
+// U^H T_a; the 3 nt^2 (component, 16 x 16 tile) items of a pass are spread
+// over the four waves, with one barrier per pass.  Operands come from L2 /
+// HBM in chunks of VW_CH four-deep steps: the chunk's loads are all issued
+// before its MFMAs, so their latencies overlap.
 // --------------------------------------------------------------------------
 typedef double velo_v4d __attribute__((ext_vector_type(4)));
+constexpr int VW_CH = 8;
 
-__global__ __launch_bounds__(256) void velo_wide_kernel(int n, int64_t nk, const zd* __restrict__ Hk,
-                                                        int64_t strideHk, int packed, TBPhase tb0,
-                                                        int gauge_trafo, const zd* __restrict__ U,
-                                                        zd* __restrict__ V, zd* __restrict__ Tscr) {
-  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+template <int PASS>
+__device__ inline void velo_wide_pass(int n, int wv, int lane, const zd* __restrict__ X, int packed,
+                                      const TBPhase& tb, const zd* __restrict__ Uk,
+                                      zd* __restrict__ T, zd* __restrict__ Vk) {
   const int lr = lane & 15, kq = lane >> 4;
-  const int n2 = n * n, nt = (n + 15) / 16;
-  zd* T = Tscr + (int64_t)blockIdx.x * n2;
+  const int n2 = n * n, nt = (n + 15) / 16, ntile = nt * nt;
+  for (int item = wv; item < 3 * ntile; item += 4) {
+    const int a = item / ntile, tile = item % ntile;
+    const int r0 = 16 * (tile / nt), c0 = 16 * (tile % nt);
+    const int ar = r0 + lr, bc = c0 + lr;
+    const zd* Ta = T + (int64_t)a * n2;
+    velo_v4d p1 = {0, 0, 0, 0}, p2 = {0, 0, 0, 0}, p3 = {0, 0, 0, 0};
+    for (int m0 = 0; m0 < n; m0 += 4 * VW_CH) {
+      zd av[VW_CH], bv[VW_CH];
+#pragma unroll
+      for (int i = 0; i < VW_CH; ++i) {
+        const int m = m0 + 4 * i + kq;
+        av[i] = bv[i] = zd{0.0, 0.0};
+        if (m < n) {
+          if (ar < n)
+            av[i] = PASS == 0 ? load_tb(X, packed, n, ar, m, 1 + a, tb) : zconj(Uk[m + (int64_t)n * ar]);
+          if (bc < n) bv[i] = PASS == 0 ? Uk[m + (int64_t)n * bc] : Ta[m + (int64_t)n * bc];
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < VW_CH; ++i) {
+        if (m0 + 4 * i >= n) break;  // uniform
+        p1 = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i].re, bv[i].re, p1, 0, 0, 0);
+        p2 = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i].im, bv[i].im, p2, 0, 0, 0);
+        p3 = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i].re + av[i].im, bv[i].re + bv[i].im, p3, 0, 0, 0);
+      }
+    }
+    zd* out = PASS == 0 ? T + (int64_t)a * n2 : Vk + (int64_t)a * n2;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int row = r0 + kq + 4 * v;
+      if (row < n && bc < n) out[row + (int64_t)n * bc] = zd{p1[v] - p2[v], p3[v] - p1[v] - p2[v]};
+    }
+  }
+}
+
+__global__ __launch_bounds__(256, 2) void velo_wide_kernel(int n, int64_t nk, const zd* __restrict__ Hk,
+                                                           int64_t strideHk, int packed, TBPhase tb0,
+                                                           int gauge_trafo, const zd* __restrict__ U,
+                                                           zd* __restrict__ V, zd* __restrict__ Tscr) {
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const int n2 = n * n;
+  zd* T = Tscr + (int64_t)blockIdx.x * 3 * n2;
   for (int64_t k = blockIdx.x; k < nk; k += gridDim.x) {
     const TBPhase tb = tb0.at(k);
     const zd* X = Hk + k * strideHk;
     const zd* Uk = U + k * (int64_t)n2;
     zd* Vk = V + k * 3 * (int64_t)n2;
-    for (int a = 0; a < 3; ++a) {
-      if (!gauge_trafo) {
-        for (int idx = t; idx < n2; idx += 256)
-          Vk[a * (int64_t)n2 + idx] = load_tb(X, packed, n, idx % n, idx / n, 1 + a, tb);
-        continue;
-      }
-      for (int pass = 0; pass < 2; ++pass) {
-        // pass 0: T = dH_a U; pass 1: V_a = U^H T
-        for (int tile = wv; tile < nt * nt; tile += 4) {
-          const int r0 = 16 * (tile / nt), c0 = 16 * (tile % nt);
-          const int ar = r0 + lr, bc = c0 + lr;
-          velo_v4d p1 = {0, 0, 0, 0}, p2 = {0, 0, 0, 0}, p3 = {0, 0, 0, 0};
-          for (int m0 = 0; m0 < n; m0 += 4) {
-            const int m = m0 + kq;
-            zd av = zd{0.0, 0.0}, bv = zd{0.0, 0.0};
-            if (m < n) {
-              if (ar < n)
-                av = pass == 0 ? load_tb(X, packed, n, ar, m, 1 + a, tb) : zconj(Uk[m + (int64_t)n * ar]);
-              if (bc < n) bv = pass == 0 ? Uk[m + (int64_t)n * bc] : T[m + (int64_t)n * bc];
-            }
-            p1 = __builtin_amdgcn_mfma_f64_16x16x4f64(av.re, bv.re, p1, 0, 0, 0);
-            p2 = __builtin_amdgcn_mfma_f64_16x16x4f64(av.im, bv.im, p2, 0, 0, 0);
-            p3 = __builtin_amdgcn_mfma_f64_16x16x4f64(av.re + av.im, bv.re + bv.im, p3, 0, 0, 0);
-          }
-          zd* out = pass == 0 ? T : Vk + a * (int64_t)n2;
-#pragma unroll
-          for (int v = 0; v < 4; ++v) {
-            const int row = r0 + kq + 4 * v;
-            if (row < n && bc < n)
-              out[row + (int64_t)n * bc] = zd{p1[v] - p2[v], p3[v] - p1[v] - p2[v]};
-          }
-        }
-        __syncthreads();
-      }
+    if (!gauge_trafo) {
+      for (int idx = t; idx < 3 * n2; idx += 256)
+        Vk[idx] = load_tb(X, packed, n, (idx % n2) % n, (idx % n2) / n, 1 + idx / n2, tb);
+      continue;
     }
+    velo_wide_pass<0>(n, wv, lane, X, packed, tb, Uk, T, Vk);
+    __syncthreads();  // T complete
+    velo_wide_pass<1>(n, wv, lane, X, packed, tb, Uk, T, Vk);
+    __syncthreads();  // T read before the next k-point overwrites it
   }
 }
 
@@ -508,6 +526,9 @@ void launch_velo(hipStream_t s, const VeloArgs& va, int64_t nk, const zd* Hk, in
     return;
   }
   if (va.n > 32) {
+    // T slots: 3 n^2 complex per workgroup (48 n^2 bytes), within the
+    // eigensolver's rotation records that va.scratch reuses (64 n^2 bytes or
+    // more per k-point of the block, and there are at least wgrid of them)
     unsigned wgrid = (unsigned)(nk < 2048 ? nk : 2048);
     hipLaunchKernelGGL(velo_wide_kernel, dim3(wgrid), dim3(256), 0, s, va.n, nk, Hk, strideHk,
                        va.packed, tb, va.gauge_trafo, U, V, va.scratch);
