@@ -898,11 +898,13 @@ static __device__ inline double rinv(const double* Es, int m, int c) {
   return (m == c) ? 0.0 : 1.0 / (Es[c] - Es[m]);  // 1 / (E_c - E_m)
 }
 
+constexpr int BW_CH = 4;  // four-deep MFMA steps per batch of operand loads
 __global__ __launch_bounds__(256) void bandsum_wide_kernel(BandArgs b, double Tsmear, AccLayout L,
                                                            int64_t nk, const double* __restrict__ E,
                                                            const zd* __restrict__ V,
                                                            const double* __restrict__ w,
-                                                           double* __restrict__ part) {
+                                                           double* __restrict__ part,
+                                                           zd* __restrict__ Bscr) {
   extern __shared__ double smem[];
   const int n = b.n, nef = b.n_ef, nhw = b.n_hw, nval = b.n_val;
   double* Es = smem;
@@ -1058,14 +1060,27 @@ __global__ __launch_bounds__(256) void bandsum_wide_kernel(BandArgs b, double Ts
       __syncthreads();
     }
     // ---- itinerant (IC) terms on MFMA ----------------------------------------
+    // B_a(m, n0) = V_a(m, n0) r(m, n0), r(m, n0) = 1 / (E_n0 - E_m) (0 on the
+    // diagonal): one division per pair, into the workgroup's slot of Bscr,
+    // so the MFMA loop below only loads (in chunks of BW_CH steps whose
+    // loads are all issued before the chunk's products)
     if (doK || doNjob) {
       const int nt = (n + 15) / 16;
       const int lr = lane & 15, kq = lane >> 4;
+      zd* Bw = Bscr + (int64_t)blockIdx.x * 3 * n2;
+      for (int idx = t; idx < n2; idx += 256) {
+        const double rr = rinv(Es, idx % n, idx / n);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) Bw[a * (int64_t)n2 + idx] = zscale(Vk[a * (int64_t)n2 + idx], rr);
+      }
+      __syncthreads();
       for (int item = wv; item < 3 * nt; item += 4) {
         const int j = item / nt, tc = item % nt;
         const int kk_ = (j + 1) % 3, ll_ = (j + 2) % 3;
         const zd* Vk_ = Vk + kk_ * (int64_t)n2;
         const zd* Vl_ = Vk + ll_ * (int64_t)n2;
+        const zd* Bk_ = Bw + kk_ * (int64_t)n2;
+        const zd* Bl_ = Bw + ll_ * (int64_t)n2;
         const int c0 = 16 * tc, n0 = c0 + lr;
         const bool colok = n0 < n;
         const bool doN = doNjob && colok && n0 < nval;
@@ -1073,26 +1088,34 @@ __global__ __launch_bounds__(256) void bandsum_wide_kernel(BandArgs b, double Ts
         for (int tr = 0; tr < nt; ++tr) {
           const int r0 = 16 * tr, ar = r0 + lr;
           mep_v4d p1 = {0, 0, 0, 0}, p2 = {0, 0, 0, 0}, p3 = {0, 0, 0, 0};
-          for (int m0 = 0; m0 < n; m0 += 4) {
-            const int m = m0 + kq;
-            zd a0 = zd{0.0, 0.0}, a1 = zd{0.0, 0.0}, b0 = zd{0.0, 0.0}, b1 = zd{0.0, 0.0};
-            if (m < n) {
-              if (ar < n) {
-                a0 = Vk_[ar + (int64_t)n * m];
-                a1 = Vl_[ar + (int64_t)n * m];
-              }
-              if (colok) {
-                const double rr = rinv(Es, m, n0);
-                b0 = zscale(Vl_[m + (int64_t)n * n0], rr);
-                b1 = zscale(Vk_[m + (int64_t)n * n0], -rr);
+          for (int m0 = 0; m0 < n; m0 += 4 * BW_CH) {
+            zd a0[BW_CH], a1[BW_CH], b0[BW_CH], b1[BW_CH];
+#pragma unroll
+            for (int i = 0; i < BW_CH; ++i) {
+              const int m = m0 + 4 * i + kq;
+              a0[i] = a1[i] = b0[i] = b1[i] = zd{0.0, 0.0};
+              if (m < n) {
+                if (ar < n) {
+                  a0[i] = Vk_[ar + (int64_t)n * m];
+                  a1[i] = Vl_[ar + (int64_t)n * m];
+                }
+                if (colok) {
+                  b0[i] = Bl_[m + (int64_t)n * n0];
+                  const zd bk = Bk_[m + (int64_t)n * n0];
+                  b1[i] = zd{-bk.re, -bk.im};
+                }
               }
             }
-            p1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0.re, b0.re, p1, 0, 0, 0);
-            p2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0.im, b0.im, p2, 0, 0, 0);
-            p3 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0.re + a0.im, b0.re + b0.im, p3, 0, 0, 0);
-            p1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1.re, b1.re, p1, 0, 0, 0);
-            p2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1.im, b1.im, p2, 0, 0, 0);
-            p3 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1.re + a1.im, b1.re + b1.im, p3, 0, 0, 0);
+#pragma unroll
+            for (int i = 0; i < BW_CH; ++i) {
+              if (m0 + 4 * i >= n) break;  // uniform
+              p1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[i].re, b0[i].re, p1, 0, 0, 0);
+              p2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[i].im, b0[i].im, p2, 0, 0, 0);
+              p3 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[i].re + a0[i].im, b0[i].re + b0[i].im, p3, 0, 0, 0);
+              p1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[i].re, b1[i].re, p1, 0, 0, 0);
+              p2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[i].im, b1[i].im, p2, 0, 0, 0);
+              p3 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[i].re + a1[i].im, b1[i].re + b1[i].im, p3, 0, 0, 0);
+            }
           }
 #pragma unroll
           for (int v = 0; v < 4; ++v) {
@@ -1505,7 +1528,7 @@ bool bandsum_can_fuse(const BandArgs& b) { return b.n > 16 && b.n <= 32 && b.v_h
 
 void launch_bandsum(hipStream_t s, const BandArgs& b, int64_t nk, const double* E, const zd* V,
                     const zd* A, const zd* Om, const double* w, int nblocks, double* part,
-                    const VeloFuse* fuse) {
+                    const VeloFuse* fuse, zd* wide_scr) {
   AccLayout L = acc_layout(b);
   // kBoltz_Eh_K * T  (constants.f90:22, statistics.f90:77)
   const double kB = 8.6173303 * 1e-5 / 27.211385;
@@ -1515,7 +1538,7 @@ void launch_bandsum(hipStream_t s, const BandArgs& b, int64_t nk, const double* 
     (void)hipFuncSetAttribute((const void*)bandsum_wide_kernel,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     hipLaunchKernelGGL(bandsum_wide_kernel, dim3(nblocks), dim3(256), lds, s, b, Tsmear, L, nk, E,
-                       V, w, part);
+                       V, w, part, wide_scr);
     if (b.jobs & MEP_JOB_AHC) {
       const size_t lh = hwsum_lds_bytes(b);
       if (b.v_herm) {

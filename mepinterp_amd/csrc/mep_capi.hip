@@ -766,8 +766,11 @@ static int band_block(mep_ctx* c, BandSetup& bs, int64_t nb, const double* w, bo
   vf.strideHk = c->ncols;
   vf.U = c->dU;
   timed(c, K_BANDSUM, [&] {
+    // n > 32: the eigensolver records (64 n^2 bytes or more per k-point of
+    // the block, consumed by now) hold the 48 n^2 bytes per workgroup of
+    // the wide kernel's scaled columns; nblk <= nb workgroups
     launch_bandsum(c->st, bs.b, nb, c->dE, c->dV, c->dA, c->dOm, w, nblk, c->dpart,
-                   fuse ? &vf : nullptr);
+                   fuse ? &vf : nullptr, reinterpret_cast<zd*>(c->dscr));
     if (bs.ext)
       launch_extsum(c->st, bs.b, nb, c->dE, c->dV, c->dA, c->dOm, w, nblk,
                     c->dpart + (int64_t)bs.nblocks * bs.L.total);

@@ -1358,6 +1358,156 @@ __global__ __launch_bounds__(256) void eigh_house_wide_kernel(int n, int64_t nk,
   }
 }
 
+// Householder reduction for 32 < n <= 64 with one k-point per wavefront:
+// lane t <-> row t, the full Hermitian matrix in LDS column-major (pitch n,
+// so a column is one contiguous 16-byte-per-lane access), no workgroup
+// barrier.  Per column kc: the tail norm and v^H p are wave sums, p = A v
+// runs over columns j > kc in four chains, and the rank-2 update writes the
+// rows t > kc of those columns.  Same outputs as eigh_house_wide_kernel.
+__global__ __launch_bounds__(64) void eigh_house_lds_kernel(int n, int64_t nk,
+                                                            const zd* __restrict__ Hk,
+                                                            int64_t strideHk, int packed,
+                                                            const double* __restrict__ kpts,
+                                                            const double* __restrict__ tau,
+                                                            zd* __restrict__ U,
+                                                            double* __restrict__ side) {
+  extern __shared__ __attribute__((aligned(16))) double hl_smem[];
+  zd* A = reinterpret_cast<zd*>(hl_smem);  // A(r, c) at A[r + n c]
+  zd* vv = A + (int64_t)n * n;
+  zd* ww = vv + 64;
+  zd* sub = ww + 64;
+  zd* fph = sub + 64;
+  double* tauv = reinterpret_cast<double*>(fph + 64);
+  const int t = threadIdx.x;
+  const double two_pi = 6.283185307179586476925;
+  const zd zero = zd{0.0, 0.0};
+  auto wave_sum = [](double x) { return rows4_sum(row16_sum(x)); };
+  auto sync = [] {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  };
+  for (int64_t k = blockIdx.x; k < nk; k += gridDim.x) {
+    // with centres, element (i, j) takes f_j conj(f_i), f_j = e^{2 pi i k.tau_j}
+    if (tau) {
+      zd f = zd{1.0, 0.0};
+      if (t < n) {
+        const double ang = two_pi * (kpts[3 * k] * tau[3 * t] + kpts[3 * k + 1] * tau[3 * t + 1] +
+                                     kpts[3 * k + 2] * tau[3 * t + 2]);
+        sin_cos(ang, &f.im, &f.re);
+      }
+      fph[t] = f;
+      sync();
+    }
+    for (int idx = t; idx < n * n; idx += 64) {
+      const int i = idx % n, j = idx / n;
+      const int i0 = i <= j ? i : j, j0 = i <= j ? j : i;  // (i0 <= j0) element
+      const int64_t o = packed ? i0 + ((int64_t)j0 * (j0 + 1)) / 2 : i0 + (int64_t)n * j0;
+      zd x = Hk[k * strideHk + o];
+      if (j < i) x = zconj(x);
+      if (tau && i != j) x = zmul(x, zmul(fph[j], zconj(fph[i])));
+      if (i == j) x.im = 0.0;
+      stz(A + idx, x);
+    }
+    sync();
+    for (int kc = 0; kc + 2 < n; ++kc) {
+      const bool below = t > kc && t < n;  // rows kc+1 .. n-1 take part
+      const zd xo = (t < n) ? ldz(A + t + (int64_t)n * kc) : zero;
+      const double a2 = zabs2(xo);
+      const double tail = wave_sum(t > kc + 1 ? a2 : 0.0);
+      const double ax02 = readlane_d(a2, kc + 1);
+      const bool refl = tail > 0.0;
+      const double xn = sqrt(tail + ax02);
+      const double ax0 = sqrt(ax02);
+      // lane kc + 1 (x0 = A(kc+1, kc)) keeps its alpha
+      const zd alpha = !refl ? xo : ((ax0 > 0.0) ? zscale(xo, -xn * rcp_nr(ax0)) : zd{-xn, 0.0});
+      zd v = (refl && below) ? xo : zero;
+      if (t == kc + 1 && refl) v = zsub(xo, alpha);
+      // |v|^2 = |x|^2 - 2 Re(conj(alpha) x0) + |alpha|^2 = 2 xn (xn + |x0|)
+      const double vn2 = 2.0 * xn * (xn + ax0);
+      const double tk = refl ? 2.0 * rcp_nr(vn2) : 0.0;
+      stz(vv + t, v);
+      if (t == kc + 1) {
+        sub[kc] = alpha;
+        tauv[kc] = tk;
+      }
+      sync();
+      zd p4[4] = {zero, zero, zero, zero};
+      if (below) {
+        const zd* col = A + t;
+        int j = kc + 1;
+        for (; j + 3 < n; j += 4) {
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const zd a_ = ldz(col + (int64_t)n * (j + u)), b_ = ldz(vv + j + u);
+            p4[u].re = fma(a_.re, b_.re, fma(-a_.im, b_.im, p4[u].re));
+            p4[u].im = fma(a_.re, b_.im, fma(a_.im, b_.re, p4[u].im));
+          }
+        }
+        for (; j < n; ++j) {
+          const zd a_ = ldz(col + (int64_t)n * j), b_ = ldz(vv + j);
+          p4[0].re = fma(a_.re, b_.re, fma(-a_.im, b_.im, p4[0].re));
+          p4[0].im = fma(a_.re, b_.im, fma(a_.im, b_.re, p4[0].im));
+        }
+      }
+      zd p = zadd(zadd(p4[0], p4[1]), zadd(p4[2], p4[3]));
+      p = below ? zscale(p, tk) : zero;
+      const double K = 0.5 * tk * wave_sum(v.re * p.re + v.im * p.im);
+      const zd w = zsub(p, zscale(v, K));
+      stz(ww + t, w);
+      sync();
+      if (refl && below) {
+        zd* col = A + t;
+        for (int j = kc + 1; j < n; ++j) {
+          // A(t, j) -= v conj(w_j) + w conj(v_j)
+          const zd vj = ldz(vv + j), wj = ldz(ww + j);
+          zd r_ = ldz(col + (int64_t)n * j);
+          r_.re = fma(-v.re, wj.re, fma(-v.im, wj.im, fma(-w.re, vj.re, fma(-w.im, vj.im, r_.re))));
+          r_.im = fma(-v.im, wj.re, fma(v.re, wj.im, fma(-w.im, vj.re, fma(w.re, vj.im, r_.im))));
+          stz(col + (int64_t)n * j, r_);
+        }
+      }
+      if (below) stz(A + t + (int64_t)n * kc, v);
+      sync();
+    }
+    if (t == 0) {
+      sub[n - 2] = ldz(A + (n - 1) + (int64_t)n * (n - 2));
+      tauv[n - 2] = 0.0;
+    }
+    sync();
+    double* sd = side + k * side_len(n);
+    zd* Uk = U + k * (int64_t)n * n;
+    // reflectors: the strict lower triangle
+    for (int idx = t; idx < n * n; idx += 64) {
+      const int i = idx % n, j = idx / n;
+      if (i > j) Uk[idx] = ldz(A + idx);
+    }
+    if (t < n) {
+      sd[t] = A[t + (int64_t)n * t].re;
+      sd[2 * n + t] = (t < n - 1) ? tauv[t] : 0.0;
+    }
+    if (t == 0) {
+      zd ph = zd{1.0, 0.0};
+      sd[3 * n] = 1.0;
+      sd[3 * n + 1] = 0.0;
+      for (int i = 0; i + 1 < n; ++i) {
+        const zd s_ = sub[i];
+        const double as = sqrt(zabs2(s_));
+        sd[n + i] = as;
+        if (as > 0.0) ph = zmul(ph, zscale(s_, 1.0 / as));
+        sd[3 * n + 2 * (i + 1)] = ph.re;
+        sd[3 * n + 2 * (i + 1) + 1] = ph.im;
+      }
+      sd[n + n - 1] = 0.0;
+    }
+    sync();
+  }
+}
+
+size_t eigh_house_lds_bytes(int n) {
+  return sizeof(zd) * ((size_t)n * n + 4 * 64) + sizeof(double) * 64;
+}
+
 size_t eigh_house_wide_lds(int n) {
   return sizeof(zd) * ((size_t)n * (n + 1) / 2 + 8 * (size_t)n) + sizeof(double) * ((size_t)n + 4);
 }
@@ -1631,8 +1781,17 @@ void launch_eigh_wide(hipStream_t s, int n, int64_t nk, const zd* Hk, int64_t st
   (void)hipFuncSetAttribute((const void*)eigh_ql_wide_kernel<double>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lq);
   unsigned grid = (unsigned)(nk < 4096 ? nk : 4096);
-  hipLaunchKernelGGL(eigh_house_wide_kernel, dim3(grid), dim3(256), lh, s, n, nk, Hk, strideHk,
-                     packed, g.kpts, g.tau, U, side);
+  if (n <= 64) {
+    const size_t ll = eigh_house_lds_bytes(n);
+    (void)hipFuncSetAttribute((const void*)eigh_house_lds_kernel,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)ll);
+    const unsigned lgrid = (unsigned)(nk < 65536 ? nk : 65536);
+    hipLaunchKernelGGL(eigh_house_lds_kernel, dim3(lgrid), dim3(64), ll, s, n, nk, Hk, strideHk,
+                       packed, g.kpts, g.tau, U, side);
+  } else {
+    hipLaunchKernelGGL(eigh_house_wide_kernel, dim3(grid), dim3(256), lh, s, n, nk, Hk, strideHk,
+                       packed, g.kpts, g.tau, U, side);
+  }
   int* list = reinterpret_cast<int*>(reinterpret_cast<uint8_t*>(scratch) + eigh_list_offset(n, nk));
   int* count = list + nk;
   int64_t qb = (nk + 31) / 32;

@@ -109,9 +109,11 @@ struct VeloFuse {
   const zd* U;
 };
 bool bandsum_can_fuse(const BandArgs& b);
+// wide_scr (n > 32): 3 n^2 complex per workgroup (nblocks of them), the
+// columns of V scaled by 1 / (E_n0 - E_m) for the itinerant products
 void launch_bandsum(hipStream_t s, const BandArgs& b, int64_t nk, const double* E, const zd* V,
                     const zd* A, const zd* Om, const double* w, int nblocks, double* part,
-                    const VeloFuse* fuse = nullptr);
+                    const VeloFuse* fuse = nullptr, zd* wide_scr = nullptr);
 void launch_reduce_parts(hipStream_t s, int nblocks, int64_t len, const double* part, double* acc);
 
 // optical / photogalvanic / gyrotropic sums (MEP_JOB_OPT, MEP_JOB_GYRO):
