@@ -186,6 +186,15 @@ int mep_run_kpts_static(mep_ctx *ctx, int64_t nk, const double *kpts, const doub
    process.  outs[i] as for mep_run_mesh (outs may be NULL).                */
 int mep_run_batch(int32_t n_models, mep_ctx *const *ctxs, const int32_t *mp_grids,
                   const mep_run_params *params, mep_sums *outs);
+/* mep_run_batch with the batch's kernels (zeroing, every k-block of every
+   model on the models' streams, the reductions) captured once into a HIP
+   graph: the first call for a set of (contexts, meshes, parameters)
+   captures, later calls with the same set replay the graph after the host
+   set-up.  A reallocation, a variant change or mep_ctx_destroy of one of
+   the contexts drops its graphs (the next call recaptures).  The same sums
+   as mep_run_batch, bit for bit.  No counterpart in the reference.          */
+int mep_run_batch_graph(int32_t n_models, mep_ctx *const *ctxs, const int32_t *mp_grids,
+                        const mep_run_params *params, mep_sums *outs);
 
 /* Per-k Berry curvature at one Fermi level: omega(9,nk), element i + 3j =
    sum_n f(n) sum_{m != n} 2 Im(V_i,nm V_j,mn) / (E_m - E_n)^2, i.e. the

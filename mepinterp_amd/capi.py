@@ -34,7 +34,7 @@ EXPORTS = ("mep_ctx_create", "mep_ctx_destroy", "mep_last_error", "mep_interp_ba
            "mep_bandsum_batch", "mep_run_mesh", "mep_run_kpts", "mep_set_profiling",
            "mep_get_kernel_time", "mep_reset_kernel_times", "mep_set_kblock", "mep_curv_batch",
            "mep_run_batch", "mep_set_workspace_limit", "mep_mesh_kpts", "mep_run_kpts_static",
-           "mep_set_variant",
+           "mep_set_variant", "mep_run_batch_graph",
            "mep_mlp_create", "mep_mlp_destroy", "mep_mlp_num_params", "mep_mlp_get_params",
            "mep_mlp_set_params", "mep_mlp_train", "mep_mlp_predict", "mep_mlp_grad")
 
@@ -98,6 +98,7 @@ def load(path: str = LIB_PATH):
     lib.mep_curv_batch.argtypes = [C.c_void_p, C.c_int64, c_dp, C.c_double, C.c_double, c_dp]
     lib.mep_run_batch.argtypes = [C.c_int32, C.POINTER(C.c_void_p), C.POINTER(C.c_int32),
                                   C.POINTER(RunParams), C.POINTER(Sums)]
+    lib.mep_run_batch_graph.argtypes = lib.mep_run_batch.argtypes
     c_fp = C.POINTER(C.c_float)
     lib.mep_mlp_create.argtypes = [C.c_int32, C.c_int32, C.c_uint64, C.POINTER(C.c_void_p)]
     lib.mep_mlp_destroy.argtypes = [C.c_void_p]
@@ -310,9 +311,10 @@ class Engine:
         return out
 
 
-def run_batch(engines, mp_grids, params):
+def run_batch(engines, mp_grids, params, graph=False):
     """mep_run_batch: several models' meshes in one call, run concurrently
-    (one stream per engine).  Returns one sums dict per engine."""
+    (one stream per engine).  Returns one sums dict per engine.  graph=True:
+    mep_run_batch_graph (captured once into a HIP graph, replayed after)."""
     n = len(engines)
     ctxs = (C.c_void_p * n)(*[e.ctx for e in engines])
     mp = (C.c_int32 * (3 * n))(*[int(x) for g in mp_grids for x in g])
@@ -322,7 +324,8 @@ def run_batch(engines, mp_grids, params):
         o, s_ = Engine._sums_buffers(p)
         outs.append(o)
         sums[i] = s_
-    _check(load().mep_run_batch(n, ctxs, mp, ps, sums), "mep_run_batch")
+    name = "mep_run_batch_graph" if graph else "mep_run_batch"
+    _check(getattr(load(), name)(n, ctxs, mp, ps, sums), name)
     for i in range(n):
         outs[i]["n_k"] = sums[i].n_k
     return outs

@@ -9,8 +9,9 @@
       E_F = 0, T = 300 K, eta = 1e-3 eV
 * c3  16 models, n_wf = 8 ... 68, NR = 27 ... 729, meshes 24^3 ... 48^3
       (each divided by --scale): aggregate k-pts/s of one mep_run_mesh per
-      model, and of one mep_run_batch over all of them with autotuned
-      k-blocks
+      model, of one mep_run_batch over all of them with autotuned k-blocks
+      and eigensolver variants, and of the same batch as one replayed HIP
+      graph (mep_run_batch_graph)
 * c4  n_wf = 24, NR = 343, eight Chern layers with a planted node each
       (gap 1e-3 eV, net Chern number 2): sigma_xy
       on a Q^3 base mesh (48 by default) with the top 5 % of cells refined by
@@ -27,6 +28,8 @@ from __future__ import annotations
 import argparse
 import json
 import time
+
+import numpy as np
 
 from . import capi, host, models
 
@@ -91,6 +94,17 @@ def run_c3(scale: int = 1):
         t_bat = time.perf_counter() - t0
         dev = max(float(abs(b["ahc"][0, 1, 0] - s["ahc"][0, 1, 0]) /
                         max(abs(s["ahc"][0, 1, 0]), 1e-300)) for b, s in zip(bat, seq_sums))
+        # the same batch captured into one HIP graph: capture, then replays
+        t0 = time.perf_counter()
+        capi.run_batch(engines, [w[1] for w in work], [w[2] for w in work], graph=True)
+        t_cap = time.perf_counter() - t0
+        t_rep = float("inf")
+        for _ in range(2):
+            t0 = time.perf_counter()
+            gr = capi.run_batch(engines, [w[1] for w in work], [w[2] for w in work], graph=True)
+            t_rep = min(t_rep, time.perf_counter() - t0)
+        graph_equal = all(np.array_equal(g_["ahc"], b["ahc"]) and np.array_equal(g_["kubo_ic"], b["kubo_ic"])
+                          for g_, b in zip(gr, bat))
     finally:
         for e in engines:
             e.close()
@@ -100,6 +114,8 @@ def run_c3(scale: int = 1):
                 n_k=tot_k, per_model=dict(seconds=t_seq, k_pts_s=tot_k / t_seq),
                 batched=dict(seconds=t_bat, k_pts_s=tot_k / t_bat, tuned=tuned,
                              max_rel_dev_sigma_xy_vs_per_model=dev),
+                graph=dict(capture_seconds=t_cap, seconds=t_rep, k_pts_s=tot_k / t_rep,
+                           bit_equal_to_stream_batch=bool(graph_equal)),
                 models=per)
 
 

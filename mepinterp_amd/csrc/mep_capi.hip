@@ -80,6 +80,10 @@ struct mep_ctx {
   // k-block workspace
   int64_t kcap = 0, kblock_req = 0;
   int var_eigh = 0;  // MEP_VAR_EIGH
+  // bumped whenever a device buffer is (re)allocated or a kernel variant
+  // changes: a captured batch graph holds the pointers and kernels of one
+  // generation
+  uint64_t gen = 0;
   double* dk = nullptr;
   double* dw = nullptr;
   double* dP = nullptr;  // (K, kblock) real phase tile
@@ -156,6 +160,7 @@ static void resolve_times(mep_ctx* c) {
 // workspace
 // ---------------------------------------------------------------------------
 static void free_ws(mep_ctx* c) {
+  ++c->gen;
   hipFree(c->dk); hipFree(c->dw); hipFree(c->dP); hipFree(c->dHk); hipFree(c->dE);
   hipFree(c->dU); hipFree(c->dV); hipFree(c->dA); hipFree(c->dOm); hipFree(c->dscr);
   c->dscr = nullptr;
@@ -187,7 +192,7 @@ static int ensure_ws(mep_ctx* c, int64_t nk_total) {
   want = ((want + 63) / 64) * 64;
   if (want < 64) want = 64;
   if (want <= c->kcap) return MEP_OK;
-  free_ws(c);
+  free_ws(c);  // bumps the generation
   int64_t n2 = c->n * c->n;
   HIPCHK(hipMalloc(&c->dk, sizeof(double) * 3 * want));
   HIPCHK(hipMalloc(&c->dw, sizeof(double) * want));
@@ -420,8 +425,11 @@ extern "C" int mep_ctx_create(int64_t n_wf, int64_t n_R, const double* R_frac,
   return MEP_OK;
 }
 
+static void bg_drop_ctx_locked(mep_ctx* c);
+
 extern "C" int mep_ctx_destroy(mep_ctx* c) {
   if (!c) return MEP_OK;
+  bg_drop_ctx_locked(c);
   if (c->st) hipStreamSynchronize(c->st);
   resolve_times(c);
   free_ws(c);
@@ -446,6 +454,7 @@ extern "C" int mep_set_variant(mep_ctx* c, int32_t what, int32_t value) {
   if (!c) return fail(MEP_ERR_ARG, "mep_set_variant: null context");
   if (what == MEP_VAR_EIGH) {
     if (value < 0 || value > 2) return fail(MEP_ERR_ARG, "mep_set_variant: MEP_VAR_EIGH takes 0, 1 or 2");
+    if (value != c->var_eigh) ++c->gen;
     c->var_eigh = value;
     return MEP_OK;
   }
@@ -527,6 +536,7 @@ static int sep_prepare(mep_ctx* c, const int32_t* mp) {
     fill(ph.data() + (size_t)mp[0] * BX + (size_t)mp[1] * c->sby, mp[2], c->sbz, c->sbz, c->soz);
     HIPCHK(hipStreamSynchronize(c->st));  // the previous mesh's tables may still be read
     if (len > c->ph_cap) {
+      ++c->gen;
       hipFree(c->dPh);
       c->dPh = nullptr;
       c->ph_cap = 0;
@@ -542,6 +552,7 @@ static int sep_prepare(mep_ctx* c, const int32_t* mp) {
   const int64_t need = nz * c->sbx * c->sby * c->ncols;
   if (need > c->y1_cap) {
     HIPCHK(hipStreamSynchronize(c->st));
+    ++c->gen;
     hipFree(c->dY1);
     c->dY1 = nullptr;
     c->y1_cap = 0;
@@ -726,11 +737,13 @@ static int band_setup(mep_ctx* c, const mep_run_params* p, int64_t kblock, BandS
     linspace(p->hw_min, p->hw_max, p->n_hw, hw);
   }
   if ((int64_t)ef.size() > c->ef_cap) {
+    ++c->gen;
     hipFree(c->def);
     HIPCHK(hipMalloc(&c->def, sizeof(double) * ef.size()));
     c->ef_cap = ef.size();
   }
   if ((int64_t)hw.size() > c->hw_cap) {
+    ++c->gen;
     hipFree(c->dhw);
     HIPCHK(hipMalloc(&c->dhw, sizeof(double) * hw.size()));
     c->hw_cap = hw.size();
@@ -742,11 +755,13 @@ static int band_setup(mep_ctx* c, const mep_run_params* p, int64_t kblock, BandS
   const int64_t rec = band_record_len(bs);
   int64_t need = (int64_t)bs.nblocks * rec;
   if (need > c->part_cap) {
+    ++c->gen;
     hipFree(c->dpart);
     HIPCHK(hipMalloc(&c->dpart, sizeof(double) * need));
     c->part_cap = need;
   }
   if (rec > c->acc_cap) {
+    ++c->gen;
     hipFree(c->dacc);
     HIPCHK(hipMalloc(&c->dacc, sizeof(double) * rec));
     c->acc_cap = rec;
@@ -1018,6 +1033,212 @@ extern "C" int mep_run_batch(int32_t n_models, mep_ctx* const* ctxs, const int32
   for (int32_t i = 0; i < n_models; ++i) {
     CHK(check_info(ctxs[i]));
     CHK(unpack(ctxs[i], bs[i], &params[i], nk[i], outs ? &outs[i] : nullptr));
+  }
+  return MEP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// mep_run_batch_graph: the kernels of a batch (accumulator zeroing, every
+// k-block of every model on the models' own streams, the record reductions)
+// captured once into a HIP graph and replayed by later calls with the same
+// contexts, meshes and parameters.  The host set-up runs on every call (it
+// re-uploads the E_F / frequency tables and the mesh phase tables); a
+// reallocation or a variant change since the capture (the context's
+// generation) or any difference in the parameters recaptures.
+// ---------------------------------------------------------------------------
+struct BatchGraph {
+  std::vector<mep_ctx*> ctxs;
+  std::vector<int32_t> grids;
+  std::vector<mep_run_params> params;  // hw_lst compared through `hw`
+  std::vector<std::vector<double>> hw;
+  std::vector<uint64_t> gens;
+  int device = 0;
+  hipStream_t st = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+static std::mutex g_bg_mu;
+static std::vector<BatchGraph*> g_bg;
+
+static void bg_free(BatchGraph* g) {
+  if (g->exec) hipGraphExecDestroy(g->exec);
+  if (g->st) hipStreamDestroy(g->st);
+  delete g;
+}
+
+// forget every graph that holds the context (called under g_bg_mu)
+static void bg_drop_ctx(mep_ctx* c) {
+  for (size_t i = 0; i < g_bg.size();) {
+    BatchGraph* g = g_bg[i];
+    if (std::find(g->ctxs.begin(), g->ctxs.end(), c) != g->ctxs.end()) {
+      if (g->st) hipStreamSynchronize(g->st);
+      bg_free(g);
+      g_bg.erase(g_bg.begin() + (long)i);
+    } else {
+      ++i;
+    }
+  }
+}
+
+static void bg_drop_ctx_locked(mep_ctx* c) {
+  std::lock_guard<std::mutex> lk(g_bg_mu);
+  bg_drop_ctx(c);
+}
+
+static bool same_params(const mep_run_params& a, const mep_run_params& b) {
+  return a.jobs == b.jobs && a.n_eF == b.n_eF && a.eF_min == b.eF_min && a.eF_max == b.eF_max &&
+         a.n_hw == b.n_hw && a.hw_min == b.hw_min && a.hw_max == b.hw_max &&
+         a.T_kelvin == b.T_kelvin && a.eta == b.eta && a.valence_bands == b.valence_bands &&
+         a.precision == b.precision && (a.hw_lst == nullptr) == (b.hw_lst == nullptr);
+}
+
+static bool bg_matches(const BatchGraph* g, int32_t n_models, mep_ctx* const* ctxs,
+                       const int32_t* mp_grids, const mep_run_params* params) {
+  if ((int32_t)g->ctxs.size() != n_models) return false;
+  for (int32_t i = 0; i < n_models; ++i) {
+    if (g->ctxs[i] != ctxs[i] || g->gens[i] != ctxs[i]->gen) return false;
+    for (int a = 0; a < 3; ++a)
+      if (g->grids[3 * i + a] != mp_grids[3 * i + a]) return false;
+    if (!same_params(g->params[i], params[i])) return false;
+    if (params[i].hw_lst &&
+        memcmp(g->hw[i].data(), params[i].hw_lst, sizeof(double) * (size_t)params[i].n_hw) != 0)
+      return false;
+  }
+  return true;
+}
+
+// capture the batch into g->exec (streams of the contexts joined to g->st
+// through events; profiling events are kept out of the capture)
+static int bg_capture(BatchGraph* g, int32_t n_models, mep_ctx* const* ctxs,
+                      const int32_t* mp_grids, const mep_run_params* params,
+                      std::vector<BandSetup>& bs, const std::vector<int64_t>& nk, int64_t rounds) {
+  std::vector<int> prof((size_t)n_models);
+  for (int32_t i = 0; i < n_models; ++i) {
+    prof[i] = ctxs[i]->prof;
+    ctxs[i]->prof = 0;
+    HIPCHK(hipStreamSynchronize(ctxs[i]->st));
+  }
+  std::vector<hipEvent_t> ev((size_t)n_models + 1, nullptr);
+  for (auto& e : ev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  int rc = MEP_OK;
+  HIPCHK(hipStreamBeginCapture(g->st, hipStreamCaptureModeThreadLocal));
+  {
+    hipError_t e = hipEventRecord(ev[0], g->st);
+    for (int32_t i = 0; i < n_models && e == hipSuccess; ++i) e = hipStreamWaitEvent(ctxs[i]->st, ev[0], 0);
+    if (e != hipSuccess) rc = fail(MEP_ERR_HIP, std::string("mep_run_batch_graph: ") + hipGetErrorString(e));
+    for (int32_t i = 0; i < n_models && rc == MEP_OK; ++i) {
+      mep_ctx* c = ctxs[i];
+      const int64_t rec = band_record_len(bs[i]);
+      launch_fill_zero(c->st, c->dacc, rec);
+      launch_fill_zero(c->st, c->dpart, (int64_t)bs[i].nblocks * rec);
+    }
+    for (int64_t r = 0; r < rounds && rc == MEP_OK; ++r)
+      for (int32_t i = 0; i < n_models && rc == MEP_OK; ++i) {
+        mep_ctx* c = ctxs[i];
+        const int64_t k0 = r * c->kcap;
+        if (k0 >= nk[i]) continue;
+        const int64_t nb = (nk[i] - k0 < c->kcap) ? nk[i] - k0 : c->kcap;
+        rc = run_block(c, bs[i], k0, nb, mp_grids + 3 * i, nullptr, nullptr, &params[i]);
+      }
+    for (int32_t i = 0; i < n_models && rc == MEP_OK; ++i) {
+      reduce_records(ctxs[i], bs[i]);
+      e = hipEventRecord(ev[1 + i], ctxs[i]->st);
+      if (e == hipSuccess) e = hipStreamWaitEvent(g->st, ev[1 + i], 0);
+      if (e != hipSuccess) rc = fail(MEP_ERR_HIP, std::string("mep_run_batch_graph: ") + hipGetErrorString(e));
+    }
+  }
+  hipGraph_t graph = nullptr;
+  const hipError_t ee = hipStreamEndCapture(g->st, &graph);
+  if (rc == MEP_OK && ee != hipSuccess)
+    rc = fail(MEP_ERR_HIP, std::string("mep_run_batch_graph: capture: ") + hipGetErrorString(ee));
+  if (rc == MEP_OK) {
+    const hipError_t ei = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
+    if (ei != hipSuccess)
+      rc = fail(MEP_ERR_HIP, std::string("mep_run_batch_graph: instantiate: ") + hipGetErrorString(ei));
+  }
+  if (graph) hipGraphDestroy(graph);
+  for (auto& e : ev) hipEventDestroy(e);
+  for (int32_t i = 0; i < n_models; ++i) ctxs[i]->prof = prof[i];
+  (void)hipGetLastError();
+  return rc;
+}
+
+extern "C" int mep_run_batch_graph(int32_t n_models, mep_ctx* const* ctxs,
+                                   const int32_t* mp_grids, const mep_run_params* params,
+                                   mep_sums* outs) {
+  if (n_models < 1 || !ctxs || !mp_grids || !params)
+    return fail(MEP_ERR_ARG, "mep_run_batch_graph: bad argument");
+  std::vector<BandSetup> bs((size_t)n_models);
+  std::vector<int64_t> nk((size_t)n_models);
+  int64_t rounds = 0;
+  for (int32_t i = 0; i < n_models; ++i) {
+    mep_ctx* c = ctxs[i];
+    const int32_t* mp = mp_grids + 3 * i;
+    if (!c) return fail(MEP_ERR_ARG, "mep_run_batch_graph: a context is NULL");
+    for (int32_t j = 0; j < i; ++j)
+      if (ctxs[j] == c) return fail(MEP_ERR_ARG, "mep_run_batch_graph: a context appears twice");
+    if (c->device != ctxs[0]->device)
+      return fail(MEP_ERR_ARG, "mep_run_batch_graph: the contexts are on different devices");
+    if (mp[0] < 1 || mp[1] < 1 || mp[2] < 1)
+      return fail(MEP_ERR_ARG, "mep_run_batch_graph: mp_grid entries must be >= 1");
+    if (params[i].hw_lst && params[i].n_hw < 1)
+      return fail(MEP_ERR_ARG, "mep_run_batch_graph: n_hw must be >= 1");
+    nk[i] = (int64_t)mp[0] * mp[1] * mp[2];
+    CHK(ensure_ws(c, nk[i]));
+    CHK(sep_prepare(c, mp));
+    CHK(band_setup(c, &params[i], c->kcap, bs[i]));
+    const int64_t nblk = (nk[i] + c->kcap - 1) / c->kcap;
+    if (nblk > rounds) rounds = nblk;
+  }
+  // the uploads of the set-up precede the graph on every context's stream
+  for (int32_t i = 0; i < n_models; ++i) HIPCHK(hipStreamSynchronize(ctxs[i]->st));
+  BatchGraph* g = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_bg_mu);
+    for (size_t i = 0; i < g_bg.size(); ++i)
+      if (bg_matches(g_bg[i], n_models, ctxs, mp_grids, params)) g = g_bg[i];
+    if (!g) {
+      // a stale graph over the same contexts is replaced
+      for (int32_t i = 0; i < n_models; ++i) bg_drop_ctx(ctxs[i]);
+      g = new BatchGraph();
+      g->device = ctxs[0]->device;
+      g->ctxs.assign(ctxs, ctxs + n_models);
+      g->grids.assign(mp_grids, mp_grids + 3 * n_models);
+      g->params.assign(params, params + n_models);
+      for (int32_t i = 0; i < n_models; ++i) {
+        g->hw.emplace_back(params[i].hw_lst ? std::vector<double>(params[i].hw_lst,
+                                                                  params[i].hw_lst + params[i].n_hw)
+                                            : std::vector<double>());
+        g->params[i].hw_lst = nullptr;
+        if (params[i].hw_lst) g->params[i].hw_lst = g->hw[i].data();
+      }
+      hipError_t e = hipStreamCreateWithFlags(&g->st, hipStreamNonBlocking);
+      if (e != hipSuccess) {
+        delete g;
+        return fail(MEP_ERR_HIP, std::string("mep_run_batch_graph: ") + hipGetErrorString(e));
+      }
+      const int rc = bg_capture(g, n_models, ctxs, mp_grids, params, bs, nk, rounds);
+      if (rc != MEP_OK) {
+        bg_free(g);
+        return rc;
+      }
+      for (int32_t i = 0; i < n_models; ++i) g->gens.push_back(ctxs[i]->gen);
+      g_bg.push_back(g);
+    }
+    HIPCHK(hipGraphLaunch(g->exec, g->st));
+    HIPCHK(hipStreamSynchronize(g->st));
+  }
+  for (int32_t i = 0; i < n_models; ++i) {
+    mep_ctx* c = ctxs[i];
+    CHK(check_info(c));
+    const int64_t rec = band_record_len(bs[i]);
+    std::vector<double> h(rec);
+    HIPCHK(hipMemcpy(h.data(), c->dacc, sizeof(double) * rec, hipMemcpyDeviceToHost));
+    if (outs) {
+      for_each_output(bs[i], &params[i], &outs[i], [&](double* dst, int64_t off, int64_t len) {
+        if (dst) memcpy(dst, h.data() + off, sizeof(double) * len);
+      });
+      outs[i].n_k = nk[i];
+    }
   }
   return MEP_OK;
 }
