@@ -17,8 +17,14 @@ SOURCES = ["mep_ft.hip", "mep_eigh.hip", "mep_refine.hip", "mep_velo.hip", "mep_
            "mep_extsum.hip", "mep_capi.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+# -amdgpu-mfma-vgpr-form: MFMA accumulators in VGPRs.  Otherwise the
+# compiler may place them in AGPRs and copy them to and from VGPRs around
+# every product of a loop (24 + 24 v_accvgpr moves per three FP64 MFMAs in
+# the wide velocity and band-sum kernels), and VALU issue serialises with
+# FP64 MFMA on gfx950.
 FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+         "-mllvm", "-amdgpu-mfma-vgpr-form",
          "-Wno-unused-result"]
 
 

@@ -324,6 +324,115 @@ __global__ __launch_bounds__(256, 2) void velo_wide_kernel(int n, int64_t nk, co
   }
 }
 
+// --------------------------------------------------------------------------
+// 32 < n <= 64, no position operator: the same products with the operands
+// in LDS.  Plane U holds U (pitch LV = n | 1, odd, so the 16-byte reads of
+// a 16-lane column tile fall in distinct banks); plane D holds dH_a (both
+// triangles, unpacked once per component), then T_a = dH_a U over it.  Per
+// component: stage dH_a | T = D U into registers | T over D | V_a = U^H T
+// to HBM.  The MFMA loop then issues two LDS reads and two adds per three
+// products (the L2 version above spends about fourteen VALU instructions per
+// product on addresses and guards, and VALU and FP64 MFMA issue serialise).
+// --------------------------------------------------------------------------
+constexpr int VL_MAXT = 4;  // 16 x 16 tiles per matrix edge (n <= 64)
+
+__host__ __device__ inline int velo_lds_pitch(int n) { return n | 1; }
+
+__global__ __launch_bounds__(256) void velo_lds_kernel(int n, int64_t nk, const zd* __restrict__ Hk,
+                                                       int64_t strideHk, int packed, TBPhase tb0,
+                                                       const zd* __restrict__ U,
+                                                       zd* __restrict__ V) {
+  extern __shared__ __attribute__((aligned(16))) double vl_smem[];
+  const int LV = velo_lds_pitch(n), PLV = LV * n;
+  zd* Us = reinterpret_cast<zd*>(vl_smem);
+  zd* Ds = Us + PLV;
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const int lr = lane & 15, kq = lane >> 4;
+  const int n2 = n * n, nt = (n + 15) / 16, ntile = nt * nt;
+  // tile rows / columns n .. 16 nt - 1 lie outside the planes: their
+  // operand loads read a clamped index and select zero
+  for (int64_t k = blockIdx.x; k < nk; k += gridDim.x) {
+    const TBPhase tb = tb0.at(k);
+    const zd* X = Hk + k * strideHk;
+    const zd* Uk = U + k * (int64_t)n2;
+    zd* Vk = V + k * 3 * (int64_t)n2;
+    __syncthreads();  // the previous k-point's last reads of U and T are done
+    for (int idx = t; idx < n2; idx += 256) stz(Us + idx % n + LV * (idx / n), Uk[idx]);
+    for (int a = 0; a < 3; ++a) {
+      for (int idx = t; idx < n2; idx += 256) {
+        const int i = idx % n, j = idx / n;
+        stz(Ds + i + LV * j, load_tb(X, packed, n, i, j, 1 + a, tb));
+      }
+      __syncthreads();  // U and dH_a staged
+      // T = D U: up to VL_MAXT^2 / 4 tiles per wave, kept in registers
+      velo_v4d q1[4], q2[4], q3[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) q1[u] = q2[u] = q3[u] = velo_v4d{0, 0, 0, 0};
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int tile = wv + 4 * u;
+        if (tile >= ntile) break;  // uniform
+        const int ar = 16 * (tile / nt) + lr, bc = 16 * (tile % nt) + lr;
+        const bool aok = ar < n, bok = bc < n;
+        const int arc = aok ? ar : 0, bcc = bok ? bc : 0;
+        for (int m0 = 0; m0 < n; m0 += 4) {
+          const int m = m0 + kq, mc = m < n ? m : 0;
+          zd x = ldz(Ds + arc + LV * mc), y = ldz(Us + mc + LV * bcc);
+          if (!(aok && m < n)) x = zd{0.0, 0.0};
+          if (!(bok && m < n)) y = zd{0.0, 0.0};
+          q1[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(x.re, y.re, q1[u], 0, 0, 0);
+          q2[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(x.im, y.im, q2[u], 0, 0, 0);
+          q3[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(x.re + x.im, y.re + y.im, q3[u], 0, 0, 0);
+        }
+      }
+      __syncthreads();  // every wave has read dH_a
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int tile = wv + 4 * u;
+        if (tile >= ntile) break;
+        const int r0 = 16 * (tile / nt), bc = 16 * (tile % nt) + lr;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int row = r0 + kq + 4 * v;
+          if (row < n && bc < n)
+            stz(Ds + row + LV * bc, zd{q1[u][v] - q2[u][v], q3[u][v] - q1[u][v] - q2[u][v]});
+        }
+      }
+      __syncthreads();  // T complete
+      // V_a = U^H T, straight to HBM
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int tile = wv + 4 * u;
+        if (tile >= ntile) break;
+        const int r0 = 16 * (tile / nt);
+        const int ar = r0 + lr, bc = 16 * (tile % nt) + lr;
+        const bool aok = ar < n, bok = bc < n;
+        const int arc = aok ? ar : 0, bcc = bok ? bc : 0;
+        velo_v4d p1 = {0, 0, 0, 0}, p2 = {0, 0, 0, 0}, p3 = {0, 0, 0, 0};
+        for (int m0 = 0; m0 < n; m0 += 4) {
+          const int m = m0 + kq, mc = m < n ? m : 0;
+          zd x = ldz(Us + mc + LV * arc), y = ldz(Ds + mc + LV * bcc);
+          x.im = -x.im;  // A(row, m) = conj(U(m, row))
+          if (!(aok && m < n)) x = zd{0.0, 0.0};
+          if (!(bok && m < n)) y = zd{0.0, 0.0};
+          p1 = __builtin_amdgcn_mfma_f64_16x16x4f64(x.re, y.re, p1, 0, 0, 0);
+          p2 = __builtin_amdgcn_mfma_f64_16x16x4f64(x.im, y.im, p2, 0, 0, 0);
+          p3 = __builtin_amdgcn_mfma_f64_16x16x4f64(x.re + x.im, y.re + y.im, p3, 0, 0, 0);
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int row = r0 + kq + 4 * v;
+          if (row < n && bc < n)
+            Vk[a * (int64_t)n2 + row + (int64_t)n * bc] = zd{p1[v] - p2[v], p3[v] - p1[v] - p2[v]};
+        }
+      }
+      __syncthreads();  // T read before the next component's dH overwrites it
+    }
+  }
+}
+
+size_t velo_lds_bytes(int n) { return sizeof(zd) * 2 * (size_t)velo_lds_pitch(n) * n; }
+
 // V (3,n,n) per k: planar [a][i + n j]  <->  reference (a fastest)
 __global__ void v_planar_to_ref(int n, int64_t nk, const zd* __restrict__ Vp, zd* __restrict__ Vr) {
   int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -523,6 +632,15 @@ void launch_velo(hipStream_t s, const VeloArgs& va, int64_t nk, const zd* Hk, in
   if (va.pos_op) {
     hipLaunchKernelGGL(velo_pos_kernel, dim3(grid), dim3(64), 0, s, va.n, nk, Hk, strideHk, tb,
                        va.gauge_trafo, va.textbook, va.kubo_tol, U, E, V, A, Om);
+    return;
+  }
+  if (va.n > 32 && va.n <= 16 * VL_MAXT && va.gauge_trafo) {
+    const size_t lds = velo_lds_bytes(va.n);
+    (void)hipFuncSetAttribute((const void*)velo_lds_kernel,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    unsigned lgrid = (unsigned)(nk < 8192 ? nk : 8192);
+    hipLaunchKernelGGL(velo_lds_kernel, dim3(lgrid), dim3(256), lds, s, va.n, nk, Hk, strideHk,
+                       va.packed, tb, U, V);
     return;
   }
   if (va.n > 32) {
