@@ -29,13 +29,17 @@ AU_TO_EV = 27.211385
 PEAK_FP64_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
 PEAK_HBM_GBS = 8000.0
 
-# HBM bytes per k-point moved by the kernels on this workload, from
-# rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of this script scaled by the
-# counters' readings for a plain 1 GiB copy (x2.0 / x1.0)
-# (profiles/kspace_n32_64cubed_fp64.md).
-SEP_XY_TRAFFIC_BYTES_PER_K = (0.881e9 + 8.862e9) / 262144.0
+# HBM bytes per k-point moved by the kernels on this workload, from the
+# rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes of this script (steps 1,
+# warmup 0, no CPU leg) on the current kernels, scaled by the counters'
+# readings for a plain 1 GiB copy (x2.0 / x1.0), full-mesh dispatches
+# (profiles/kspace_n32_64cubed_fp64.md).  Counters cannot be read from
+# inside the timed run, so the figures are per-k constants of that profile;
+# they match the algorithmic bytes (33,792 B/k written by the Fourier sums,
+# 41,984 B/k read by the fused band sums) to 0.05 %.
+SEP_XY_TRAFFIC_BYTES_PER_K = (0.873e9 + 8.862e9) / 262144.0
 BANDSUM_TRAFFIC_BYTES_PER_K = (12.957e9 + 0.004e9) / 262144.0        # V read from HBM
-BANDSUM_FUSED_TRAFFIC_BYTES_PER_K = (11.010e9 + 0.004e9) / 262144.0  # dH and U read
+BANDSUM_FUSED_TRAFFIC_BYTES_PER_K = (11.011e9 + 0.004e9) / 262144.0  # dH and U read
 
 
 def log(*a):

@@ -1,12 +1,15 @@
 """Run one of the named workloads on the GPU and print one JSON line.
 
     python -m mepinterp_amd.configs c1 [--mesh Q]
+    python -m mepinterp_amd.configs c2k [--kpts N]
     python -m mepinterp_amd.configs c3 [--scale S]
     python -m mepinterp_amd.configs c4 [--mesh Q]
     python -m mepinterp_amd.configs c5 [--mesh Q]
 
 * c1  n_wf = 8, NR = 27 with a position operator, 32^3 mesh, sigma_xy at
       E_F = 0, T = 300 K, eta = 1e-3 eV
+* c2k n_wf = 32, NR = 343 (the bench model) on N seeded points of its 64^3
+      mesh passed as a k list: the Fourier GEMM route
 * c3  16 models, n_wf = 8 ... 68, NR = 27 ... 729, meshes 24^3 ... 48^3
       (each divided by --scale): aggregate k-pts/s of one mep_run_mesh per
       model, of one mep_run_batch over all of them with autotuned k-blocks
@@ -63,6 +66,31 @@ def run_c1(q: int = 32):
                 sigma_xy=float(out["ahc_tens"][0, 1]))
 
 
+def run_c2k(nk: int = 262144):
+    """C2's model (n_wf = 32, NR = 343) on nk seeded points of its 64^3 mesh
+    as an explicit k list: the Fourier GEMM route (phase table + FP64 MFMA
+    GEMM) instead of the separable mesh sums."""
+    import numpy as np_
+    from .refine import mp_mesh
+    m = models.make_c2(seed=2)
+    mesh = mp_mesh((64, 64, 64))
+    rng = np_.random.default_rng(2048)
+    k = np_.asfortranarray(mesh[:, rng.choice(mesh.shape[1], size=nk, replace=False)])
+    p = dict(do_ahc=True, do_kubo=True, do_mep=True, n_eF=1, eF_min=0.0, eF_max=0.0, n_hw=1,
+             hw_min=0.0, hw_max=0.0, T_kelvin=300.0, eta=1e-3 / AU_TO_EV, valence_bands=16)
+    rp = host.run_params(p)
+    with capi.Engine(m["R_frac"], m["H"], a_latt=m["a_latt"]) as eng:
+        eng.run_kpts(k[:, :64], rp)
+        eng.set_profiling(True)
+        eng.reset_times()
+        t0 = time.perf_counter()
+        eng.run_kpts(k, rp)
+        dt = time.perf_counter() - t0
+        split = {s_: eng.kernel_time(s_)[0] for s_ in STAGES}
+    return dict(config="c2k", workload=f"n_wf=32 NR=343, {nk} seeded points of the 64^3 mesh as a "
+                "k list (Fourier GEMM)", n_k=nk, seconds=dt, k_pts_s=nk / dt, kernel_ms=split)
+
+
 def run_c3(scale: int = 1):
     from . import tune
     jobs = dict(do_ahc=True, do_kubo=True, do_mep=True, n_eF=1, eF_min=0.0, eF_max=0.0, n_hw=1,
@@ -85,7 +113,14 @@ def run_c3(scale: int = 1):
             seq_sums.append(eng.run_mesh(mp, rp))
             dt = time.perf_counter() - t0
             t_seq += dt
-            per.append(dict(model=m["name"], n_k=mp[0] ** 3, seconds=dt, k_pts_s=mp[0] ** 3 / dt))
+            # the stage split from a second, event-timed run
+            eng.set_profiling(True)
+            eng.reset_times()
+            eng.run_mesh(mp, rp)
+            eng.set_profiling(False)
+            split = {s_: round(eng.kernel_time(s_)[0], 3) for s_ in STAGES}
+            per.append(dict(model=m["name"], n_k=mp[0] ** 3, seconds=dt, k_pts_s=mp[0] ** 3 / dt,
+                            kernel_ms=split))
         # autotuned k-blocks (cached), then one batched call over all models
         tuned = [tune.tune(eng, mp, rp, m["R_frac"].shape[1]) for eng, (m, mp, rp) in zip(engines, work)]
         capi.run_batch(engines, [w[1] for w in work], [w[2] for w in work])
@@ -166,7 +201,8 @@ def run_c5(q: int = 128):
 
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
-    ap.add_argument("config", choices=("c1", "c3", "c4", "c5"))
+    ap.add_argument("config", choices=("c1", "c2k", "c3", "c4", "c5"))
+    ap.add_argument("--kpts", type=int, default=262144, help="c2k: number of k-points")
     ap.add_argument("--mesh", type=int, default=None, help="Q for a Q^3 mesh (c1, c4, c5)")
     ap.add_argument("--scale", type=int, default=1, help="divide the c3 meshes by this")
     ap.add_argument("--box", type=int, default=9, help="c4: cells per box edge (odd)")
@@ -174,6 +210,8 @@ def main(argv=None):
     a = ap.parse_args(argv)
     if a.config == "c1":
         r = run_c1(a.mesh or 32)
+    elif a.config == "c2k":
+        r = run_c2k(a.kpts)
     elif a.config == "c3":
         r = run_c3(a.scale)
     elif a.config == "c4":
