@@ -240,9 +240,12 @@ int mep_set_workspace_limit(mep_ctx *ctx, double bytes);
    tuning knob with no counterpart in the reference; mepinterp_amd/tune.py
    times the variants and caches its choice.  Every variant gives the same
    sums to rounding.
-     MEP_VAR_EIGH  FP64 Householder reduction for 16 < n <= 32:
+     MEP_VAR_EIGH  FP64 Householder reduction.  16 < n <= 32:
                    1 = one k-point per wavefront, half a row per lane (default);
-                   2 = two k-points per wavefront, one row per lane. */
+                   2 = two k-points per wavefront, one row per lane.
+                   32 < n: 1 = one 256-thread workgroup per k-point (default
+                   for n > 52); 2 = one wavefront per k-point with the matrix
+                   in LDS (n <= 64; default for n <= 52). */
 #define MEP_VAR_EIGH 1
 int mep_set_variant(mep_ctx *ctx, int32_t what, int32_t value);
 

@@ -638,7 +638,7 @@ static int interp_block(mep_ctx* c, int64_t nb, bool velo, int prec = MEP_PREC_F
   timed(c, K_EIGH, [&] {
     if (n > 32)
       launch_eigh_wide(c->st, (int)n, nb, c->dHk, c->ncols, c->packed, g, c->dE, c->dU, c->dinfo,
-                       c->dscr);
+                       c->dscr, c->var_eigh);
     else
       launch_eigh(c->st, (int)n, nb, c->dHk, c->ncols, c->packed, g, c->dE, c->dU, c->dinfo,
                   c->dscr, c->var_eigh);

@@ -1764,7 +1764,7 @@ __global__ __launch_bounds__(256) void eigh_back_wide_kernel(int n, int64_t nk, 
 static size_t eigh_list_offset(int n, int64_t nk);
 
 void launch_eigh_wide(hipStream_t s, int n, int64_t nk, const zd* Hk, int64_t strideHk, int packed,
-                      const KGeom& g, double* E, zd* U, int* info, double* scratch) {
+                      const KGeom& g, double* E, zd* U, int* info, double* scratch, int house) {
   double* side = scratch;
   double* rot = side + nk * side_len(n);
   uint8_t* rcol = reinterpret_cast<uint8_t*>(rot + nk * 2 * rot_cap(n));
@@ -1781,7 +1781,8 @@ void launch_eigh_wide(hipStream_t s, int n, int64_t nk, const zd* Hk, int64_t st
   (void)hipFuncSetAttribute((const void*)eigh_ql_wide_kernel<double>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lq);
   unsigned grid = (unsigned)(nk < 4096 ? nk : 4096);
-  if (n <= 64) {
+  if (house == 0) house = n <= 52 ? 2 : 1;  // measured crossover (C3 models n = 36 ... 64)
+  if (house == 2 && n <= 64) {
     const size_t ll = eigh_house_lds_bytes(n);
     (void)hipFuncSetAttribute((const void*)eigh_house_lds_kernel,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)ll);

@@ -49,8 +49,10 @@ size_t eigh_scratch_bytes(int n, int64_t nk);
 // wavefront, 2 = two; 0 = the default, 1)
 void launch_eigh(hipStream_t s, int n, int64_t nk, const zd* Hk, int64_t strideHk, int packed,
                  const KGeom& g, double* E, zd* U, int* info, double* scratch, int house = 0);
+// house (32 < n): 1 = one 256-thread workgroup per k-point, 2 = one wavefront
+// per k-point with the matrix in LDS (n <= 64); 0 = the default (2 for n <= 52)
 void launch_eigh_wide(hipStream_t s, int n, int64_t nk, const zd* Hk, int64_t strideHk, int packed,
-                      const KGeom& g, double* E, zd* U, int* info, double* scratch);
+                      const KGeom& g, double* E, zd* U, int* info, double* scratch, int house = 0);
 // the same in FP32 arithmetic (FP32 H(k) and U; eigenvalues written as FP64)
 void launch_eigh_f32(hipStream_t s, int n, int64_t nk, const zf* Hk, int64_t strideHk, int packed,
                      const KGeom& g, double* E, zf* U, int* info, double* scratch);

@@ -204,3 +204,24 @@ def test_tuner_picks_and_caches(tmp_path):
         assert c2 == c1
         s = eng.run_mesh(mp, rp)
         assert s["n_k"] == 16 ** 3
+
+
+@pytest.mark.parametrize("n", [40, 60])
+def test_wide_householder_variants_match_oracle(n):
+    """Both Householder kernels for 32 < n <= 64 (a 256-thread workgroup
+    per k-point; one wavefront per k-point with the matrix in LDS) against
+    the oracle on random k-points."""
+    import oracle as O
+    from mepinterp_amd import capi, host, models
+    m = models.make_random(seed=90 + n, n=n, L=1)
+    rng = np.random.default_rng(n)
+    k = np.asfortranarray(rng.uniform(-0.5, 0.5, (3, 37)))
+    p = dict(ALL_JOBS, n_eF=1, eF_min=0.0, eF_max=0.0, n_hw=1, hw_min=0.0, hw_max=0.0,
+             T_kelvin=300.0, eta=1e-3 / AU_TO_EV, valence_bands=n // 2)
+    pk = {kk: v for kk, v in p.items() if kk in O.RunParams.__init__.__code__.co_varnames}
+    o = O.accumulate(oracle_model(m), O.RunParams(**pk), k)
+    with _engine(m) as eng:
+        for var in (1, 2):
+            eng.set_variant(capi.MEP_VAR_EIGH, var)
+            g = eng.run_kpts(k, host.run_params(p))
+            _compare(g, o, what=f"n={n} eigh variant {var}")
