@@ -904,7 +904,9 @@ __global__ __launch_bounds__(256) void bandsum_wide_kernel(BandArgs b, double Ts
                                                            const zd* __restrict__ V,
                                                            const double* __restrict__ w,
                                                            double* __restrict__ part,
-                                                           zd* __restrict__ Bscr) {
+                                                           zd* __restrict__ Bscr,
+                                                           const zd* __restrict__ A,
+                                                           const zd* __restrict__ Om) {
   extern __shared__ double smem[];
   const int n = b.n, nef = b.n_ef, nhw = b.n_hw, nval = b.n_val;
   double* Es = smem;
@@ -1181,6 +1183,35 @@ __global__ __launch_bounds__(256) void bandsum_wide_kernel(BandArgs b, double Ts
           const int q = it % 9, m0 = it / 9;
           acc[L.nic + q + 9 * m0] += wk * F2m[q * n + m0];
         }
+    }
+    // ---- Chern-Simons terms (position operator: A and Omega) ---------------
+    // kubo_mep_CS / mep_niu_CS (kubo_mep.f90:16-47, mep_niu.f90:20-45):
+    // red[n0] = Re sum_a conj(A_a(n0,n0)) omega_a(n0,n0), omega_c from Omega_ab
+    if ((doK || doNjob) && b.pos_op && A && Om) {
+      __syncthreads();
+      for (int n0 = t; n0 < n; n0 += 256) {
+        const int64_t NN = (int64_t)n2;
+        const zd* Ak = A + k * 3 * NN + 3 * ((int64_t)n0 + (int64_t)n * n0);
+        const zd* Ok = Om + k * 9 * NN + 9 * ((int64_t)n0 + (int64_t)n * n0);
+        zd omc[3];
+        omc[0] = zsub(Ok[1 + 3 * 2], Ok[2 + 3 * 1]);
+        omc[1] = zsub(Ok[2 + 3 * 0], Ok[0 + 3 * 2]);
+        omc[2] = zsub(Ok[0 + 3 * 1], Ok[1 + 3 * 0]);
+        zd dp = zd{0.0, 0.0};
+#pragma unroll
+        for (int a = 0; a < 3; ++a) dp = zadd(dp, zcmul(Ak[a], omc[a]));
+        red[n0] = dp.re;
+      }
+      __syncthreads();
+      if (doK)
+        for (int ef = t; ef < nef; ef += 256) {
+          double s_ = 0.0;
+          for (int m0 = 0; m0 < n; ++m0) s_ += 0.5 * fd[ef + nef * m0] * red[m0];
+          for (int i = 0; i < 3; ++i) acc[L.kcs + i + 3 * i + 9 * ef] += wk * s_;
+        }
+      if (doNjob)
+        for (int n0 = t; n0 < nval && n0 < n; n0 += 256)
+          for (int i = 0; i < 3; ++i) acc[L.ncs + i + 3 * i + 9 * n0] += wk * (0.5 * red[n0]);
     }
   }
 }
@@ -1538,7 +1569,7 @@ void launch_bandsum(hipStream_t s, const BandArgs& b, int64_t nk, const double* 
     (void)hipFuncSetAttribute((const void*)bandsum_wide_kernel,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     hipLaunchKernelGGL(bandsum_wide_kernel, dim3(nblocks), dim3(256), lds, s, b, Tsmear, L, nk, E,
-                       V, w, part, wide_scr);
+                       V, w, part, wide_scr, A, Om);
     if (b.jobs & MEP_JOB_AHC) {
       const size_t lh = hwsum_lds_bytes(b);
       if (b.v_herm) {

@@ -224,8 +224,6 @@ extern "C" int mep_ctx_create(int64_t n_wf, int64_t n_R, const double* R_frac,
   if (n_wf <= 0 || n_R <= 0 || !R_frac || !H_real || !a_latt)
     return fail(MEP_ERR_ARG, "mep_ctx_create: n_wf, n_R, R_frac, H_real and a_latt are required");
   if (n_wf > 128) return fail(MEP_ERR_ARG, "mep_ctx_create: n_wf > 128 is not supported");
-  if (r_real && n_wf > 16)
-    return fail(MEP_ERR_ARG, "mep_ctx_create: a position operator needs n_wf <= 16");
   if (deg && (flags & MEP_F_APPLY_DEG))
     for (int64_t s = 0; s < n_R; ++s)
       if (deg[s] <= 0) return fail(MEP_ERR_ARG, "mep_ctx_create: deg(R) must be positive");
@@ -597,6 +595,9 @@ static int interp_block(mep_ctx* c, int64_t nb, bool velo, int prec = MEP_PREC_F
   va.textbook = (c->flags & MEP_F_TEXTBOOK) ? 1 : 0;
   va.kubo_tol = c->kubo_tol;
   va.scratch = reinterpret_cast<zd*>(c->dscr);  // free once the eigensolve is done
+  va.scratch_bytes = eigh_scratch_bytes((int)n, c->kcap);
+  if (velo && c->pos && !velo_pos_scratch_ok((int)n, va.scratch_bytes))
+    return fail(MEP_ERR_ARG, "position operator: the k-block is too small for the velocity slots");
   if (prec == MEP_PREC_MIXED) {
     // FP32 phase tile, FP32 GEMM, FP32 eigensolve (U in V's slot), FP64
     // refinement (U -> dU), FP64 velocities from the FP32 dH

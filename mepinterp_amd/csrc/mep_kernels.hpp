@@ -66,8 +66,12 @@ struct VeloArgs {
   int pos_op;
   int textbook;
   double kubo_tol;
-  zd* scratch;  // n > 32: 3 n x n per workgroup (up to 2048 workgroups)
+  zd* scratch;  // n > 32: 3 n x n per workgroup (up to 2048 workgroups); with a
+                // position operator and n > 16: 18 n x n per workgroup
+  size_t scratch_bytes = 0;
 };
+// the scratch holds at least one position-operator slot (or none is needed)
+bool velo_pos_scratch_ok(int n, size_t scratch_bytes);
 void launch_velo(hipStream_t s, const VeloArgs& va, int64_t nk, const zd* Hk, int64_t strideHk,
                  const KGeom& g, const zd* U, const double* E, zd* V, zd* A, zd* Om);
 

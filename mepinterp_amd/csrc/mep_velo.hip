@@ -621,6 +621,168 @@ __global__ __launch_bounds__(64) void velo_pos_kernel(int n, int64_t nk, const z
   }
 }
 
+// C = op(A) * B for n x n matrices of pitch n in global memory (the
+// workgroup's slot), 256 threads
+__device__ inline void gmm(int n, const zd* A, bool adjA, const zd* B, zd* C) {
+  for (int idx = threadIdx.x; idx < n * n; idx += 256) {
+    const int i = idx % n, j = idx / n;
+    zd s0 = zd{0.0, 0.0}, s1 = zd{0.0, 0.0};
+    int l = 0;
+    for (; l + 1 < n; l += 2) {
+      const zd a0 = adjA ? zconj(A[l + (int64_t)n * i]) : A[i + (int64_t)n * l];
+      const zd a1 = adjA ? zconj(A[l + 1 + (int64_t)n * i]) : A[i + (int64_t)n * (l + 1)];
+      s0 = zadd(s0, zmul(a0, B[l + (int64_t)n * j]));
+      s1 = zadd(s1, zmul(a1, B[l + 1 + (int64_t)n * j]));
+    }
+    if (l < n) {
+      const zd a0 = adjA ? zconj(A[l + (int64_t)n * i]) : A[i + (int64_t)n * l];
+      s0 = zadd(s0, zmul(a0, B[l + (int64_t)n * j]));
+    }
+    C[i + (int64_t)n * j] = zadd(s0, s1);
+  }
+}
+
+// The position-operator path for 16 < n <= 128: velo_pos_kernel's steps in
+// the same order (rotate_gauge :502-550, get_gauge_covar_deriv :565-617,
+// conn_gaugeTrafo :620-633, curv_gaugeTrafo :636-658, get_velo :406-444),
+// with its eighteen n x n matrices in the workgroup's slot of `slots`
+// (18 n^2 complex each; the slots lie in the consumed eigensolver records)
+// instead of LDS.
+__global__ __launch_bounds__(256) void velo_pos_global_kernel(int n, int64_t nk,
+                                                              const zd* __restrict__ Hk,
+                                                              int64_t strideHk, TBPhase tb,
+                                                              int gauge_trafo, int textbook,
+                                                              double kubo_tol,
+                                                              const zd* __restrict__ U,
+                                                              const double* __restrict__ E,
+                                                              zd* __restrict__ V, zd* __restrict__ Aout,
+                                                              zd* __restrict__ Omout,
+                                                              zd* __restrict__ slots) {
+  __shared__ double Es[128];
+  const int t = threadIdx.x;
+  const int64_t NN = (int64_t)n * n;
+  zd* base = slots + (int64_t)blockIdx.x * 18 * NN;
+  zd *Us = base, *T1 = base + NN, *T2 = base + 2 * NN;
+  zd *Hb[3], *Ab[3], *Dm[3], *Om[3], *Abar[3];
+  for (int a = 0; a < 3; ++a) {
+    Hb[a] = base + (3 + a) * NN;
+    Ab[a] = base + (6 + a) * NN;
+    Dm[a] = base + (9 + a) * NN;
+    Om[a] = base + (12 + a) * NN;
+    Abar[a] = base + (15 + a) * NN;
+  }
+  for (int64_t k = blockIdx.x; k < nk; k += gridDim.x) {
+    const TBPhase tk = tb.at(k);
+    const zd* X = Hk + k * strideHk;
+    __syncthreads();  // the previous k-point's reads of the slot are done
+    for (int idx = t; idx < n * n; idx += 256) {
+      const int i = idx % n, j = idx / n;
+      Us[idx] = U[k * NN + idx];
+      for (int a = 0; a < 3; ++a) {
+        Hb[a][idx] = load_tb(X, 0, n, i, j, 1 + a, tk);
+        Ab[a][idx] = load_tb(X, 0, n, i, j, 4 + a, tk);
+        Om[a][idx] = load_tb(X, 0, n, i, j, 7 + a, tk);
+      }
+    }
+    if (t < n) Es[t] = E[k * n + t];
+    __syncthreads();
+    if (gauge_trafo) {
+      for (int a = 0; a < 3; ++a) {
+        gmm(n, Hb[a], false, Us, T1);  // dH U
+        __syncthreads();
+        gmm(n, Us, true, T1, Hb[a]);    // Hbar = U^H dH U
+        __syncthreads();
+        if (textbook) {
+          gmm(n, Ab[a], false, Us, T2);
+          __syncthreads();
+          gmm(n, Us, true, T2, Ab[a]);
+        } else {
+          // stale temporary: A = (U^H (dH U)) U
+          gmm(n, Us, true, T1, T2);
+          __syncthreads();
+          gmm(n, T2, false, Us, Ab[a]);
+        }
+        __syncthreads();
+        gmm(n, Us, true, Om[a], T2);  // Omega_bar = U^H Omega U
+        __syncthreads();
+        gmm(n, T2, false, Us, Om[a]);
+        __syncthreads();
+      }
+      // D_a,ij = Hbar_a,ij / (E_j - E_i) for i != j and |E_j - E_i| > kubo_tol
+      for (int idx = t; idx < n * n; idx += 256) {
+        const int i = idx % n, j = idx / n;
+        const double de = Es[j] - Es[i];
+        for (int a = 0; a < 3; ++a) {
+          zd dv = zd{0.0, 0.0};
+          if (i != j && fabs(de) > kubo_tol) dv = zd{Hb[a][idx].re / de, Hb[a][idx].im / de};
+          Dm[a][idx] = dv;
+          Abar[a][idx] = Ab[a][idx];
+        }
+      }
+      __syncthreads();
+      // A^(H) = A + i D
+      for (int idx = t; idx < n * n; idx += 256)
+        for (int a = 0; a < 3; ++a) Ab[a][idx] = zadd(Ab[a][idx], zmuli(Dm[a][idx]));
+      __syncthreads();
+      // Omega^(H)_ab = Om_ab - [D_a, A_b] + [D_b, A_a] - i [D_a, D_b]
+      // for (a,b) in {(1,2), (2,0), (0,1)}; A is A^(H) (reference) or Abar
+      for (int c = 0; c < 3; ++c) {
+        const int a = (c == 0) ? 1 : (c == 1) ? 2 : 0;
+        const int b = (c == 0) ? 2 : (c == 1) ? 0 : 1;
+        zd* const* Asrc = textbook ? Abar : Ab;
+        for (int idx = t; idx < n * n; idx += 256) {
+          const int i = idx % n, j = idx / n;
+          zd s_ = Om[c][idx];
+          zd c1 = zd{0, 0}, c2 = zd{0, 0}, c3 = zd{0, 0};
+          for (int l = 0; l < n; ++l) {
+            const int64_t il = i + (int64_t)n * l, lj = l + (int64_t)n * j;
+            c1 = zadd(c1, zsub(zmul(Dm[a][il], Asrc[b][lj]), zmul(Asrc[b][il], Dm[a][lj])));
+            c2 = zadd(c2, zsub(zmul(Dm[b][il], Asrc[a][lj]), zmul(Asrc[a][il], Dm[b][lj])));
+            c3 = zadd(c3, zsub(zmul(Dm[a][il], Dm[b][lj]), zmul(Dm[b][il], Dm[a][lj])));
+          }
+          s_ = zsub(s_, c1);
+          s_ = zadd(s_, c2);
+          s_ = zsub(s_, zmuli(c3));
+          T1[idx] = s_;
+        }
+        __syncthreads();
+        for (int idx = t; idx < n * n; idx += 256) Om[c][idx] = T1[idx];
+        __syncthreads();
+      }
+    }
+    // velocity, connection and curvature in the reference's layouts
+    for (int idx = t; idx < n * n; idx += 256) {
+      const int i = idx % n, j = idx / n;
+      for (int a = 0; a < 3; ++a) {
+        zd v = Hb[a][idx];
+        if (i != j) {
+          // V_nm = H_nm - i (E_m - E_n) A_nm
+          const zd* Asel = (gauge_trafo && textbook) ? Abar[a] : Ab[a];
+          const zd ce = zd{0.0, Es[j] - Es[i]};
+          v = zsub(v, zmul(ce, Asel[idx]));
+        }
+        V[k * 3 * NN + a * NN + idx] = v;
+        Aout[k * 3 * NN + a + 3 * (int64_t)idx] = Ab[a][idx];
+      }
+      zd* O = Omout + k * 9 * NN + 9 * (int64_t)idx;
+      const zd oyz = Om[0][idx], ozx = Om[1][idx], oxy = Om[2][idx];
+      O[0] = zd{0, 0};
+      O[1 + 3 * 1] = zd{0, 0};
+      O[2 + 3 * 2] = zd{0, 0};
+      O[1 + 3 * 2] = oyz;
+      O[2 + 3 * 1] = zsub(zd{0, 0}, oyz);
+      O[2 + 3 * 0] = ozx;
+      O[0 + 3 * 2] = zsub(zd{0, 0}, ozx);
+      O[0 + 3 * 1] = oxy;
+      O[1 + 3 * 0] = zsub(zd{0, 0}, oxy);
+    }
+  }
+}
+
+bool velo_pos_scratch_ok(int n, size_t scratch_bytes) {
+  return n <= PM || scratch_bytes >= 18 * (size_t)n * n * sizeof(zd);
+}
+
 void launch_velo(hipStream_t s, const VeloArgs& va, int64_t nk, const zd* Hk, int64_t strideHk,
                  const KGeom& g, const zd* U, const double* E, zd* V, zd* A, zd* Om) {
   unsigned grid = (unsigned)(nk < 65536 ? nk : 65536);
@@ -629,9 +791,21 @@ void launch_velo(hipStream_t s, const VeloArgs& va, int64_t nk, const zd* Hk, in
   tb.kpts = g.kpts;
   tb.tau = g.tau;
   for (int i = 0; i < 9; ++i) tb.cm[i] = g.cmat[i];
-  if (va.pos_op) {
+  if (va.pos_op && va.n <= PM) {
     hipLaunchKernelGGL(velo_pos_kernel, dim3(grid), dim3(64), 0, s, va.n, nk, Hk, strideHk, tb,
                        va.gauge_trafo, va.textbook, va.kubo_tol, U, E, V, A, Om);
+    return;
+  }
+  if (va.pos_op) {
+    // one workgroup per slot of 18 n^2 complex in the scratch
+    const int64_t slot = 18 * (int64_t)va.n * va.n * (int64_t)sizeof(zd);
+    int64_t pg = (int64_t)(va.scratch_bytes / (size_t)slot);
+    if (pg > nk) pg = nk;
+    if (pg > 2048) pg = 2048;
+    if (pg < 1) pg = 1;  // the caller guarantees one slot (velo_pos_scratch_ok)
+    hipLaunchKernelGGL(velo_pos_global_kernel, dim3((unsigned)pg), dim3(256), 0, s, va.n, nk, Hk,
+                       strideHk, tb, va.gauge_trafo, va.textbook, va.kubo_tol, U, E, V, A, Om,
+                       va.scratch);
     return;
   }
   if (va.n > 32 && va.n <= 16 * VL_MAXT && va.gauge_trafo) {

@@ -418,3 +418,34 @@ def test_near_degenerate_k_points_take_the_rotation_path(n):
         assert np.min(np.diff(eo)) < 2e-9  # the pairs are below the separation threshold
         assert np.max(np.abs(e[:, ik] - eo)) < E_TOL, f"eigenvalues differ at k #{ik}"
         assert _rel(np.abs(V[..., ik]) ** 2, np.abs(Vo) ** 2) < REL
+
+
+@pytest.mark.parametrize("n,textbook", [(24, False), (24, True), (40, False)])
+def test_position_operator_beyond_16_bands_matches_oracle(n, textbook):
+    """The position-operator path above 16 bands (velocity, connection and
+    curvature in global-memory workgroup slots; for n = 40 the wide band sums
+    with their Chern-Simons terms): V, A and Omega per k and every band sum
+    against the oracle, in the reference's gauge-transform order and with
+    MEP_F_TEXTBOOK."""
+    import oracle as O
+    from mepinterp_amd import capi, models
+    m = models.make_random(seed=200 + n, n=n, L=1, with_r=True)
+    om = oracle_model(m, textbook=textbook)
+    k = rand_kpts(6, seed=n)
+    with _engine(m, textbook=textbook) as eng:
+        e, V, A, Om = eng.interp(k)
+        for ik in range(k.shape[1]):
+            eo, Vo, Ao, Omo = O.get_wann_interp(om, k[:, ik])
+            assert np.max(np.abs(e[:, ik] - eo)) < E_TOL
+            assert _rel(V[..., ik], Vo) < 1e-8, ik
+            assert _rel(A[..., ik], Ao) < 1e-8, ik
+            assert _rel(Om[..., ik], Omo) < 1e-8, ik
+        jobs = capi.MEP_JOB_AHC | capi.MEP_JOB_KUBO | capi.MEP_JOB_MEP
+        kw = dict(n_eF=2, eF_min=-0.01, eF_max=0.01, n_hw=2, hw_min=0.0, hw_max=0.05,
+                  T_kelvin=300.0, eta=0.004, valence_bands=n // 2)
+        kl = rand_kpts(40, seed=n + 1)
+        eng.set_kblock(16)  # several k-blocks
+        g = eng.run_kpts(kl, _params(jobs, **kw))
+    o = _oracle_sums(om, kl, jobs, **kw)
+    assert np.max(np.abs(o["kubo_cs"])) > 0 and np.max(np.abs(o["niu_cs"])) > 0
+    _compare_sums(g, o)
