@@ -115,7 +115,8 @@ typedef struct {
      tight-binding convention, wann_interp.f90:192-280); deg(n_R) or NULL;
      a_latt(3,3) with a_latt(i,:) = a_i (input_paras.f90:213-216).
    Sizes: n_wf <= 128, with or without a position operator.  The mixed-
-   precision path and the optical / gyrotropic sums take n_wf <= 32.        */
+   precision path takes n_wf <= 32; the optical / gyrotropic sums take
+   n_wf x n_eF up to about 9,600.                                            */
 int mep_ctx_create(int64_t n_wf, int64_t n_R, const double *R_frac, const double *wf_centers,
                    const double *H_real, const double *r_real, const int64_t *deg,
                    const double *a_latt, double kubo_tol, int32_t flags, mep_ctx **ctx);

@@ -717,8 +717,9 @@ static int band_plan(mep_ctx* c, const mep_run_params* p, int64_t kblock, BandSe
   bs.L = acc_layout(b);
   bs.ext = (p->jobs & (MEP_JOB_OPT | MEP_JOB_GYRO)) != 0;
   bs.X = ext_layout(b);
-  if (bs.ext && (c->n > 32 || p->n_eF > 16))
-    return fail(MEP_ERR_ARG, "optical / gyrotropic sums: n_wf <= 32 and n_eF <= 16");
+  if (bs.ext && ext_lds_bytes(b) > 160 * 1024)
+    return fail(MEP_ERR_ARG, "optical / gyrotropic sums: n_eF x n_wf too large for the "
+                             "occupation tables in LDS (about 9,600 above 32 bands)");
   bs.nblocks = (int)(kblock < 1024 ? kblock : 1024);
   return MEP_OK;
 }

@@ -8,8 +8,11 @@
 //                        occupation itself, as the reference evaluates it)
 // One k-point at a time per 256-thread workgroup; every output element is
 // owned by one thread, which adds its per-k value to the workgroup's record
-// (fixed order, deterministic).  V is staged in LDS (planar, padded), A and
-// Omega are read in the reference layout (position operator only).
+// (fixed order, deterministic).  For n <= 32 V is staged in LDS (planar,
+// padded); above that it is read from L2 / HBM.  A and Omega are read in the
+// reference layout (position operator only).  The occupations, their
+// derivative and the per-band curvature sit in dynamic LDS, sized by n and
+// n_eF (ext_lds_bytes).
 #include "mep_common.hpp"
 #include "mep_kernels.hpp"
 #include "../../include/mepinterp.h"
@@ -37,6 +40,13 @@ __device__ inline int levi3(int a, int b, int c) {
   return (a == b || b == c || a == c) ? 0 : (((b - a + 3) % 3 == 1) ? 1 : -1);
 }
 
+// dynamic LDS: [V planes if LDSV] | E (128) | omega (3 x 128 complex) |
+// fd (n_ef x n) | fd_stat_deriv (n_ef x n)
+__host__ __device__ inline size_t ext_lds_doubles(int n, int nef, bool ldsv) {
+  return (ldsv ? 2 * 3 * (size_t)XS_PL : 0) + 128 + 2 * 3 * 128 + 2 * (size_t)nef * n;
+}
+
+template <bool LDSV>
 __global__ __launch_bounds__(XS_THREADS) void extsum_kernel(BandArgs b, double Tsmear, ExtLayout X,
                                                            int64_t nk, const double* __restrict__ E,
                                                            const zd* __restrict__ V,
@@ -44,12 +54,13 @@ __global__ __launch_bounds__(XS_THREADS) void extsum_kernel(BandArgs b, double T
                                                            const zd* __restrict__ Om,
                                                            const double* __restrict__ w,
                                                            double* __restrict__ part) {
-  __shared__ zd Vs[3 * XS_PL];
-  __shared__ double Es[XS_NMAX];
-  __shared__ double fd[XS_NMAX * 16];   // (n_ef <= 16, band)
-  __shared__ double fdd[XS_NMAX * 16];  // fd_stat_deriv
-  __shared__ zd omg[3 * XS_NMAX];       // omega_c(n) of get_gyro_D
+  extern __shared__ __attribute__((aligned(16))) double xs_smem[];
+  zd* Vs = reinterpret_cast<zd*>(xs_smem);                         // LDSV only
+  double* Es = xs_smem + (LDSV ? 2 * 3 * XS_PL : 0);
+  zd* omg = reinterpret_cast<zd*>(Es + 128);                         // omega_c(n) of get_gyro_D
+  double* fd = reinterpret_cast<double*>(omg + 3 * 128);             // (n_ef, band)
   const int n = b.n, nef = b.n_ef, nhw = b.n_hw, n2 = n * n;
+  double* fdd = fd + (int64_t)nef * n;                               // fd_stat_deriv
   const int t = threadIdx.x;
   const bool finiteT = b.T_kelvin > 1e-2;
   const bool doOpt = (b.jobs & MEP_JOB_OPT) != 0, doGyro = (b.jobs & MEP_JOB_GYRO) != 0;
@@ -64,11 +75,16 @@ __global__ __launch_bounds__(XS_THREADS) void extsum_kernel(BandArgs b, double T
     const double wk = w ? w[k] : 1.0;
     __syncthreads();
     const zd* Vk = V + k * 3 * (int64_t)n2;
-    for (int idx = t; idx < 3 * n2; idx += XS_THREADS) {
-      const int a = idx / n2, r = idx - a * n2;
-      Vs[a * XS_PL + r % n + XS_LD * (r / n)] = Vk[idx];
-    }
-    if (t < n) Es[t] = E[k * n + t];
+    // V_a(i, j): the LDS planes or the k-point's planar V
+    auto Vat = [&](int a, int i, int j) {
+      return LDSV ? Vs[a * XS_PL + i + XS_LD * j] : Vk[a * (int64_t)n2 + i + (int64_t)n * j];
+    };
+    if (LDSV)
+      for (int idx = t; idx < 3 * n2; idx += XS_THREADS) {
+        const int a = idx / n2, r = idx - a * n2;
+        Vs[a * XS_PL + r % n + XS_LD * (r / n)] = Vk[idx];
+      }
+    for (int i = t; i < n; i += XS_THREADS) Es[i] = E[k * n + i];
     __syncthreads();
     for (int idx = t; idx < nef * n; idx += XS_THREADS) {
       double f;
@@ -84,7 +100,8 @@ __global__ __launch_bounds__(XS_THREADS) void extsum_kernel(BandArgs b, double T
     }
     // omega_c(n) = sum_ab eps_abc curv_ab(n): Omega_ab(n,n) with a position
     // operator, else get_kubo_curv (wann_interp.f90:113-146)
-    if (doGyro && t < n) {
+    for (int t0 = t; doGyro && t0 < n; t0 += XS_THREADS) {
+      const int t = t0;  // the band of this pass
       zd curv[9];
       if (pos) {
         const zd* On = Om + k * 9 * (int64_t)n2 + 9 * ((int64_t)t + (int64_t)n * t);
@@ -98,7 +115,7 @@ __global__ __launch_bounds__(XS_THREADS) void extsum_kernel(BandArgs b, double T
           for (int j = 0; j < 3; ++j)
             for (int a = 0; a < 3; ++a)
               curv[a + 3 * j].re +=
-                  zmul(Vs[a * XS_PL + t + XS_LD * np], Vs[j * XS_PL + np + XS_LD * t]).im / dEE;
+                  zmul(Vat(a, t, np), Vat(j, np, t)).im / dEE;
         }
         for (int q = 0; q < 9; ++q) curv[q] = zscale(curv[q], -2.0);
       }
@@ -151,8 +168,7 @@ __global__ __launch_bounds__(XS_THREADS) void extsum_kernel(BandArgs b, double T
             for (int l = 0; l < n; ++l) {
               if (l == nn) continue;
               const double df = fd[ef + nef * l] - fd[ef + nef * nn];
-              const zd vvv = zmul(Vs[a * XS_PL + nn + XS_LD * l],
-                                  zmul(Vs[bb * XS_PL + l + XS_LD * m], Vs[c * XS_PL + m + XS_LD * nn]));
+              const zd vvv = zmul(Vat(a, nn, l), zmul(Vat(bb, l, m), Vat(c, m, nn)));
               zd g = zd{0.0, 0.0};
               for (int om = -1; om <= 1; om += 2) {
                 const zd dEhw = zd{Es[nn] - Es[l] + om * hwv, -2.0 * eta};
@@ -171,7 +187,7 @@ __global__ __launch_bounds__(XS_THREADS) void extsum_kernel(BandArgs b, double T
         const int q = it % 9, ef = it / 9, a = q % 3, bb = q / 3;
         zd c = zd{0.0, 0.0}, d = zd{0.0, 0.0};
         for (int nn = 0; nn < n; ++nn) {
-          const zd va = Vs[a * XS_PL + nn + XS_LD * nn], vb = Vs[bb * XS_PL + nn + XS_LD * nn];
+          const zd va = Vat(a, nn, nn), vb = Vat(bb, nn, nn);
           const double f = fdd[ef + nef * nn];
           c = zadd(c, zscale(zmul(va, vb), f));
           d = zsub(d, zscale(zmul(va, omg[bb + 3 * nn]), f));
@@ -198,7 +214,7 @@ __global__ __launch_bounds__(XS_THREADS) void extsum_kernel(BandArgs b, double T
                                  zmul(Aat(k, b2, nn, m), Aat(k, b1, m, nn)));
               omt -= cr.im * ww / (ww - hwv);
             }
-            const zd va = Vs[a * XS_PL + nn + XS_LD * nn];
+            const zd va = Vat(a, nn, nn);
             dw = zsub(dw, zscale(va, omt * fdd[ef + nef * nn]));
           }
           const int64_t o = 2 * (q + 9 * (h + (int64_t)nhw * ef));
@@ -209,10 +225,24 @@ __global__ __launch_bounds__(XS_THREADS) void extsum_kernel(BandArgs b, double T
   }
 }
 
+size_t ext_lds_bytes(const BandArgs& b) {
+  return sizeof(double) * ext_lds_doubles(b.n, b.n_ef, b.n <= XS_NMAX);
+}
+
 void launch_extsum(hipStream_t s, const BandArgs& b, int64_t nk, const double* E, const zd* V,
                    const zd* A, const zd* Om, const double* w, int nblocks, double* part) {
   ExtLayout X = ext_layout(b);
   const double kB = 8.6173303 * 1e-5 / 27.211385;  // kBoltz_Eh_K (constants.f90:22)
-  hipLaunchKernelGGL(extsum_kernel, dim3(nblocks), dim3(XS_THREADS), 0, s, b, kB * b.T_kelvin, X, nk,
-                     E, V, A, Om, w, part);
+  const size_t lds = ext_lds_bytes(b);
+  if (b.n <= XS_NMAX) {
+    (void)hipFuncSetAttribute((const void*)extsum_kernel<true>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(extsum_kernel<true>, dim3(nblocks), dim3(XS_THREADS), lds, s, b,
+                       kB * b.T_kelvin, X, nk, E, V, A, Om, w, part);
+  } else {
+    (void)hipFuncSetAttribute((const void*)extsum_kernel<false>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(extsum_kernel<false>, dim3(nblocks), dim3(XS_THREADS), lds, s, b,
+                       kB * b.T_kelvin, X, nk, E, V, A, Om, w, part);
+  }
 }

@@ -123,12 +123,14 @@ void launch_bandsum(hipStream_t s, const BandArgs& b, int64_t nk, const double* 
 void launch_reduce_parts(hipStream_t s, int nblocks, int64_t len, const double* part, double* acc);
 
 // optical / photogalvanic / gyrotropic sums (MEP_JOB_OPT, MEP_JOB_GYRO):
-// record layout (complex interleaved doubles) and launcher; n_wf <= 32,
-// n_eF <= 16
+// record layout (complex interleaved doubles) and launcher; n_wf <= 128,
+// n_eF x n_wf bounded by ext_lds_bytes
 struct ExtLayout {
   int64_t opt_s, opt_a, photo, gyro_C, gyro_D, gyro_Dw, total;
 };
 ExtLayout ext_layout(const BandArgs& b);
+// dynamic LDS of the optical / gyrotropic kernel (must stay within 160 KiB)
+size_t ext_lds_bytes(const BandArgs& b);
 void launch_extsum(hipStream_t s, const BandArgs& b, int64_t nk, const double* E, const zd* V,
                    const zd* A, const zd* Om, const double* w, int nblocks, double* part);
 

@@ -449,3 +449,26 @@ def test_position_operator_beyond_16_bands_matches_oracle(n, textbook):
     o = _oracle_sums(om, kl, jobs, **kw)
     assert np.max(np.abs(o["kubo_cs"])) > 0 and np.max(np.abs(o["niu_cs"])) > 0
     _compare_sums(g, o)
+
+
+@pytest.mark.parametrize("n,n_eF,with_r", [(40, 3, True), (24, 20, False)])
+def test_optical_and_gyrotropic_sums_beyond_32_bands_and_16_levels(n, n_eF, with_r):
+    """The optical / photogalvanic / gyrotropic sums above the n_wf = 32 and
+    N_eF = 16 of the LDS-sized kernel: V read from L2 / HBM for n = 40 (with
+    a position operator, so kubo_opt_tens and get_gyro_Dw run), 20 Fermi
+    levels at n = 24, against the oracle's k-loop."""
+    import oracle as O
+    from mepinterp_amd import capi, models
+    m = models.make_random(seed=300 + n, n=n, L=1, with_r=with_r)
+    mp = (2, 2, 2)
+    jobs = capi.MEP_JOB_AHC | capi.MEP_JOB_OPT | capi.MEP_JOB_GYRO
+    kw = dict(n_eF=n_eF, eF_min=-0.01, eF_max=0.01, n_hw=3, hw_min=0.01, hw_max=0.05,
+              T_kelvin=300.0, eta=0.004, valence_bands=4)
+    with _engine(m) as eng:
+        g = eng.run_mesh(mp, _params(jobs, **kw))
+    p = O.RunParams(do_ahc=True, do_opt=True, do_gyro=True, **kw)
+    o = O.accumulate(oracle_model(m), p, O.mp_mesh(mp))
+    keys = ["photo", "gyro_C", "gyro_D"] + (["opt_s", "opt_a", "gyro_Dw"] if with_r else [])
+    for key in keys:
+        assert np.abs(o[key]).max() > 0, key
+        assert _rel(g[key], o[key]) < 1e-9, f"{key}: {_rel(g[key], o[key]):.2e}"
