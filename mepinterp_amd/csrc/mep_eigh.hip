@@ -141,21 +141,28 @@ __global__ __launch_bounds__(64) void eigh_house_kernel(int n, int64_t nk,
         sd[t] = a[t + LDA * t].re;  // d
         sd[2 * n + t] = (t < n - 1) ? tauv[t] : R(0);
       }
-      if (t == 0) {
-        // phases and the real subdiagonal (serial recurrence)
-        Z ph = Z{R(1), R(0)};
-        sd[3 * n] = R(1);
-        sd[3 * n + 1] = R(0);
-        for (int i = 0; i + 1 < n; ++i) {
-          const Z s_ = sub[i];
-          const R as = sqrt(zabs2(s_));
-          sd[n + i] = as;
-          if (as > R(0)) ph = zmul(ph, zscale(s_, R(1) / as));
-          sd[3 * n + 2 * (i + 1)] = ph.re;
-          sd[3 * n + 2 * (i + 1) + 1] = ph.im;
-        }
-        sd[n + n - 1] = R(0);
+      if (t + 1 < n) {
+        const Z s_ = sub[t];
+        const R as = sqrt(zabs2(s_));
+        sd[n + t] = as;
+        sub[t] = (as > R(0)) ? zscale(s_, R(1) / as) : Z{R(1), R(0)};
       }
+    }
+    // |s_i| and the unit phase u_i = s_i / |s_i| (1 for s_i = 0) of every i at
+    // once, then the running product ph_{i+1} = ph_i u_i in one lane: the same
+    // operations as one serial loop over i (ph u with u = 1 is exact)
+    __syncthreads();
+    if (valid && t == 0) {
+      R* sd = side + k * side_len(n);
+      Z ph = Z{R(1), R(0)};
+      sd[3 * n] = R(1);
+      sd[3 * n + 1] = R(0);
+      for (int i = 0; i + 1 < n; ++i) {
+        ph = zmul(ph, sub[i]);
+        sd[3 * n + 2 * (i + 1)] = ph.re;
+        sd[3 * n + 2 * (i + 1) + 1] = ph.im;
+      }
+      sd[n + n - 1] = R(0);
     }
     __syncthreads();
   }
@@ -339,15 +346,24 @@ __global__ __launch_bounds__(64, 4) void eigh_house_half_kernel(int n, int64_t n
       sd[t] = dgl;
       sd[2 * n + t] = (t < n - 1) ? tauv[t] : 0.0;
     }
+    // |s_i| and the unit phase u_i = s_i / |s_i| (1 for s_i = 0) of every i at
+    // once, then the running product ph_{i+1} = ph_i u_i in one lane: the same
+    // operations as one serial loop over i (ph u with u = 1 is exact)
+    if (lane + 1 < n) {
+      const zd s_ = sub[lane];
+      const double as = sqrt(zabs2(s_));
+      sd[n + lane] = as;
+      sub[lane] = (as > 0.0) ? zscale(s_, 1.0 / as) : zd{1.0, 0.0};
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     if (lane == 0) {
       zd ph = zd{1.0, 0.0};
       sd[3 * n] = 1.0;
       sd[3 * n + 1] = 0.0;
       for (int i = 0; i + 1 < n; ++i) {
-        const zd s_ = sub[i];
-        const double as = sqrt(zabs2(s_));
-        sd[n + i] = as;
-        if (as > 0.0) ph = zmul(ph, zscale(s_, 1.0 / as));
+        ph = zmul(ph, sub[i]);
         sd[3 * n + 2 * (i + 1)] = ph.re;
         sd[3 * n + 2 * (i + 1) + 1] = ph.im;
       }
@@ -514,20 +530,31 @@ __global__ __launch_bounds__(64, 2) void eigh_house_row2_kernel(int n, int64_t n
         sd[t] = dgl;
         sd[2 * n + t] = (t < n - 1) ? tauv[g * NMAX + t] : 0.0;
       }
-      if (t == 0) {
-        zd ph = zd{1.0, 0.0};
-        sd[3 * n] = 1.0;
-        sd[3 * n + 1] = 0.0;
-        for (int i = 0; i + 1 < n; ++i) {
-          const zd s_ = sub[g * NMAX + i];
-          const double as = sqrt(zabs2(s_));
-          sd[n + i] = as;
-          if (as > 0.0) ph = zmul(ph, zscale(s_, 1.0 / as));
-          sd[3 * n + 2 * (i + 1)] = ph.re;
-          sd[3 * n + 2 * (i + 1) + 1] = ph.im;
-        }
-        sd[n + n - 1] = 0.0;
+      if (t + 1 < n) {
+        const zd s_ = sub[g * NMAX + t];
+        const double as = sqrt(zabs2(s_));
+        sd[n + t] = as;
+        sub[g * NMAX + t] = (as > 0.0) ? zscale(s_, 1.0 / as) : zd{1.0, 0.0};
       }
+    }
+    // |s_i| and the unit phase u_i = s_i / |s_i| (1 for s_i = 0) of every i at
+    // once, then the running product ph_{i+1} = ph_i u_i in one lane: the same
+    // operations as one serial loop over i (ph u with u = 1 is exact)
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (valid && t == 0) {
+      double* sd = side + k * side_len(n);
+      const zd* ug = sub + g * NMAX;
+      zd ph = zd{1.0, 0.0};
+      sd[3 * n] = 1.0;
+      sd[3 * n + 1] = 0.0;
+      for (int i = 0; i + 1 < n; ++i) {
+        ph = zmul(ph, ug[i]);
+        sd[3 * n + 2 * (i + 1)] = ph.re;
+        sd[3 * n + 2 * (i + 1) + 1] = ph.im;
+      }
+      sd[n + n - 1] = 0.0;
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -1341,15 +1368,22 @@ __global__ __launch_bounds__(256) void eigh_house_wide_kernel(int n, int64_t nk,
       sd[t] = Lw[lo_off(t, n) + t].re;
       sd[2 * n + t] = (t < n - 1) ? tauv[t] : 0.0;
     }
+    // |s_i| and the unit phase u_i = s_i / |s_i| (1 for s_i = 0) of every i at
+    // once, then the running product ph_{i+1} = ph_i u_i in one lane: the same
+    // operations as one serial loop over i (ph u with u = 1 is exact)
+    if (t + 1 < n) {
+      const zd s_ = sub[t];
+      const double as = sqrt(zabs2(s_));
+      sd[n + t] = as;
+      sub[t] = (as > 0.0) ? zscale(s_, 1.0 / as) : zd{1.0, 0.0};
+    }
+    __syncthreads();
     if (t == 0) {
       zd ph = zd{1.0, 0.0};
       sd[3 * n] = 1.0;
       sd[3 * n + 1] = 0.0;
       for (int i = 0; i + 1 < n; ++i) {
-        const zd s_ = sub[i];
-        const double as = sqrt(zabs2(s_));
-        sd[n + i] = as;
-        if (as > 0.0) ph = zmul(ph, zscale(s_, 1.0 / as));
+        ph = zmul(ph, sub[i]);
         sd[3 * n + 2 * (i + 1)] = ph.re;
         sd[3 * n + 2 * (i + 1) + 1] = ph.im;
       }
@@ -1486,15 +1520,22 @@ __global__ __launch_bounds__(64) void eigh_house_lds_kernel(int n, int64_t nk,
       sd[t] = A[t + (int64_t)n * t].re;
       sd[2 * n + t] = (t < n - 1) ? tauv[t] : 0.0;
     }
+    // |s_i| and the unit phase u_i = s_i / |s_i| (1 for s_i = 0) of every i at
+    // once, then the running product ph_{i+1} = ph_i u_i in one lane: the same
+    // operations as one serial loop over i (ph u with u = 1 is exact)
+    if (t + 1 < n) {
+      const zd s_ = sub[t];
+      const double as = sqrt(zabs2(s_));
+      sd[n + t] = as;
+      sub[t] = (as > 0.0) ? zscale(s_, 1.0 / as) : zd{1.0, 0.0};
+    }
+    sync();
     if (t == 0) {
       zd ph = zd{1.0, 0.0};
       sd[3 * n] = 1.0;
       sd[3 * n + 1] = 0.0;
       for (int i = 0; i + 1 < n; ++i) {
-        const zd s_ = sub[i];
-        const double as = sqrt(zabs2(s_));
-        sd[n + i] = as;
-        if (as > 0.0) ph = zmul(ph, zscale(s_, 1.0 / as));
+        ph = zmul(ph, sub[i]);
         sd[3 * n + 2 * (i + 1)] = ph.re;
         sd[3 * n + 2 * (i + 1) + 1] = ph.im;
       }
