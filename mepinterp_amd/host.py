@@ -106,6 +106,9 @@ def write_outputs(out: dict, out_dir: str, p: dict):
               hw_list_ev(p.get("n_hw", 1), p.get("hw_min", 0.0), p.get("hw_max", 0.0)))
     for key in ("mep_bands", "mep_ic", "mep_lc", "mep_cs", "mep_tens", "kubo_mep_ic",
                 "kubo_mep_lc", "kubo_mep_cs", "kubo_mep_tens", "ahc_tens"):
+        # mep_bands.npy only with MEP%do_write_mep_bands (core.f90:586-599)
+        if key == "mep_bands" and not p.get("do_write_mep_bands", False):
+            continue
         if key in out:
             write_npy(os.path.join(out_dir, key + ".npy"), out[key])
     if "ahcVELO" in out:
@@ -125,7 +128,8 @@ def params_from_cfg(cfg: dict) -> dict:
                 do_opt=cfg.get("do_opt", False), do_gyro=cfg.get("do_gyro", False),
                 n_eF=cfg["N_eF"], eF_min=cfg["eF_min"], eF_max=cfg["eF_max"],
                 n_hw=cfg["N_hw"], hw_min=cfg["hw_min"], hw_max=cfg["hw_max"],
-                T_kelvin=cfg["T_kelvin"], eta=cfg["eta"], valence_bands=cfg["valence_bands"])
+                T_kelvin=cfg["T_kelvin"], eta=cfg["eta"], valence_bands=cfg["valence_bands"],
+                do_write_mep_bands=cfg.get("do_write_mep_bands", False))
 
 
 def run_params(p: dict) -> capi.RunParams:

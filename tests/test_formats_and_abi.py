@@ -261,3 +261,18 @@ def test_tb_dat_roundtrip_and_preferred_over_hr(tmp_path):
     R2, H2, r2, c2 = read_tb_basis(base)
     assert r2 is None and c2 is None
     np.testing.assert_allclose(H2[big], 2.0 * m["H"][big], rtol=1e-15)
+
+
+def test_mep_bands_written_only_with_its_flag(tmp_path):
+    """write_outputs writes mep_bands.npy only with MEP%do_write_mep_bands
+    (core.f90:586-599); the other MEP tensors are written either way."""
+    import numpy as np
+    from mepinterp_amd import host
+    out = dict(mep_bands=np.zeros((3, 3, 2)), mep_tens=np.ones((3, 3)), ef_lst=np.array([0.0]),
+               Ne=np.array([2.0]))
+    for flag in (False, True):
+        d = tmp_path / f"flag{int(flag)}"
+        host.write_outputs(out, str(d), dict(n_hw=1, hw_min=0.0, hw_max=0.0,
+                                             do_write_mep_bands=flag))
+        assert (d / "mep_tens.npy").exists()
+        assert (d / "mep_bands.npy").exists() == flag
