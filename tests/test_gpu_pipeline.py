@@ -472,3 +472,44 @@ def test_optical_and_gyrotropic_sums_beyond_32_bands_and_16_levels(n, n_eF, with
     for key in keys:
         assert np.abs(o[key]).max() > 0, key
         assert _rel(g[key], o[key]) < 1e-9, f"{key}: {_rel(g[key], o[key]):.2e}"
+
+
+@pytest.mark.parametrize("n0", [12, 32])
+def test_pairs_just_above_the_separation_threshold(n0):
+    """Eigenvectors from the direct (twisted-factorization) path where it is
+    least accurate: two uncoupled copies of a model, the second shifted by
+    delta = 2.5e-5 max|E|, so every eigenvalue has a partner 2.5 threshold
+    units away (the verdict sends a k-point to the rotation replay only below
+    1e-5 max|E|).  Eigenvalues, V (through the shared gauge convention) and
+    every band sum against the oracle; n = 24 runs the narrow eigensolver,
+    n = 64 the wide one."""
+    import oracle as O
+    from mepinterp_amd import capi, models
+    base = models.make_random(seed=500 + n0, n=n0, L=1)
+    R, H0 = base["R_frac"], base["H"]
+    i0 = int(np.argmin(np.abs(R).sum(axis=0)))
+    k = rand_kpts(12, seed=n0)
+    # max|E| over the test k-points, from the oracle's eigenvalues of one copy
+    om0 = oracle_model(base)
+    rho = max(np.max(np.abs(O.get_wann_interp(om0, k[:, i])[0])) for i in range(k.shape[1]))
+    delta = 2.5e-5 * rho
+    n = 2 * n0
+    H = np.zeros((n, n, R.shape[1]), dtype=np.complex128)
+    H[:n0, :n0] = H0
+    H[n0:, n0:] = H0
+    H[n0:, n0:, i0] += delta * np.eye(n0)
+    m = dict(base, H=np.asfortranarray(H))
+    om = oracle_model(m)
+    with _engine(m) as eng:
+        e, V, _, _ = eng.interp(k)
+        jobs = capi.MEP_JOB_AHC | capi.MEP_JOB_KUBO | capi.MEP_JOB_MEP
+        kw = dict(n_eF=2, eF_min=-0.01, eF_max=0.01, n_hw=2, hw_min=0.0, hw_max=0.05,
+                  T_kelvin=300.0, eta=0.004, valence_bands=n0)
+        g = eng.run_kpts(k, _params(jobs, **kw))
+    for ik in range(k.shape[1]):
+        eo, Vo, _, _ = O.get_wann_interp(om, k[:, ik])
+        gaps = np.diff(np.sort(eo))
+        assert gaps.min() > 1e-5 * np.max(np.abs(eo)), "a pair fell below the threshold"
+        assert np.max(np.abs(e[:, ik] - eo)) < E_TOL
+        assert _rel(V[..., ik], Vo) < 1e-8, (ik, _rel(V[..., ik], Vo))
+    _compare_sums(g, _oracle_sums(om, k, jobs, **kw))
