@@ -246,8 +246,13 @@ int mep_set_workspace_limit(mep_ctx *ctx, double bytes);
                    2 = two k-points per wavefront, one row per lane.
                    32 < n: 1 = one 256-thread workgroup per k-point (default
                    for n > 52); 2 = one wavefront per k-point with the matrix
-                   in LDS (n <= 64; default for n <= 52). */
+                   in LDS (n <= 64; default for n <= 52).
+     MEP_VAR_GEMM32  the FP32 Fourier GEMM of the mixed path on k lists:
+                   1 = FP32 MFMA (v_mfma_f32_16x16x4_f32); 2 = split bfloat16
+                   (three bf16 parts per operand, six products accumulated in
+                   FP32 on bf16 MFMA: FP32-equivalent) (default). */
 #define MEP_VAR_EIGH 1
+#define MEP_VAR_GEMM32 2
 int mep_set_variant(mep_ctx *ctx, int32_t what, int32_t value);
 
 #ifdef __cplusplus

@@ -28,6 +28,7 @@ MEP_JOB_GYRO = 16
 MEP_PREC_FP64 = 0
 MEP_PREC_MIXED = 1
 MEP_VAR_EIGH = 1
+MEP_VAR_GEMM32 = 2
 
 # every symbol include/mepinterp.h declares
 EXPORTS = ("mep_ctx_create", "mep_ctx_destroy", "mep_last_error", "mep_interp_batch",

@@ -66,10 +66,11 @@ def run_c1(q: int = 32):
                 sigma_xy=float(out["ahc_tens"][0, 1]))
 
 
-def run_c2k(nk: int = 262144):
+def run_c2k(nk: int = 262144, mixed: bool = False):
     """C2's model (n_wf = 32, NR = 343) on nk seeded points of its 64^3 mesh
     as an explicit k list: the Fourier GEMM route (phase table + FP64 MFMA
-    GEMM) instead of the separable mesh sums."""
+    GEMM) instead of the separable mesh sums.  mixed: the mixed-precision
+    path instead, timed with each FP32 GEMM (FP32 MFMA and split bfloat16)."""
     import numpy as np_
     from .refine import mp_mesh
     m = models.make_c2(seed=2)
@@ -79,16 +80,25 @@ def run_c2k(nk: int = 262144):
     p = dict(do_ahc=True, do_kubo=True, do_mep=True, n_eF=1, eF_min=0.0, eF_max=0.0, n_hw=1,
              hw_min=0.0, hw_max=0.0, T_kelvin=300.0, eta=1e-3 / AU_TO_EV, valence_bands=16)
     rp = host.run_params(p)
+    if mixed:
+        rp.precision = capi.MEP_PREC_MIXED
+    res = {}
     with capi.Engine(m["R_frac"], m["H"], a_latt=m["a_latt"]) as eng:
-        eng.run_kpts(k[:, :64], rp)
-        eng.set_profiling(True)
-        eng.reset_times()
-        t0 = time.perf_counter()
-        eng.run_kpts(k, rp)
-        dt = time.perf_counter() - t0
-        split = {s_: eng.kernel_time(s_)[0] for s_ in STAGES}
+        for var in ((1, 2) if mixed else (0,)):
+            if mixed:
+                eng.set_variant(capi.MEP_VAR_GEMM32, var)
+            eng.run_kpts(k[:, :64], rp)
+            eng.set_profiling(True)
+            eng.reset_times()
+            t0 = time.perf_counter()
+            eng.run_kpts(k, rp)
+            dt = time.perf_counter() - t0
+            eng.set_profiling(False)
+            split = {s_: eng.kernel_time(s_)[0] for s_ in STAGES + ("refine",)}
+            res[{0: "fp64", 1: "mixed_fp32_gemm", 2: "mixed_bf16x3_gemm"}[var]] = dict(
+                seconds=dt, k_pts_s=nk / dt, kernel_ms=split)
     return dict(config="c2k", workload=f"n_wf=32 NR=343, {nk} seeded points of the 64^3 mesh as a "
-                "k list (Fourier GEMM)", n_k=nk, seconds=dt, k_pts_s=nk / dt, kernel_ms=split)
+                "k list (Fourier GEMM)" + (", mixed precision" if mixed else ""), n_k=nk, **res)
 
 
 def run_c3(scale: int = 1):
@@ -203,6 +213,7 @@ def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("config", choices=("c1", "c2k", "c3", "c4", "c5"))
     ap.add_argument("--kpts", type=int, default=262144, help="c2k: number of k-points")
+    ap.add_argument("--mixed", action="store_true", help="c2k: the mixed-precision path")
     ap.add_argument("--mesh", type=int, default=None, help="Q for a Q^3 mesh (c1, c4, c5)")
     ap.add_argument("--scale", type=int, default=1, help="divide the c3 meshes by this")
     ap.add_argument("--box", type=int, default=9, help="c4: cells per box edge (odd)")
@@ -211,7 +222,7 @@ def main(argv=None):
     if a.config == "c1":
         r = run_c1(a.mesh or 32)
     elif a.config == "c2k":
-        r = run_c2k(a.kpts)
+        r = run_c2k(a.kpts, a.mixed)
     elif a.config == "c3":
         r = run_c3(a.scale)
     elif a.config == "c4":

@@ -80,6 +80,7 @@ struct mep_ctx {
   // k-block workspace
   int64_t kcap = 0, kblock_req = 0;
   int var_eigh = 0;  // MEP_VAR_EIGH
+  int var_gemm32 = 0;  // MEP_VAR_GEMM32
   // bumped whenever a device buffer is (re)allocated or a kernel variant
   // changes: a captured batch graph holds the pointers and kernels of one
   // generation
@@ -456,6 +457,12 @@ extern "C" int mep_set_variant(mep_ctx* c, int32_t what, int32_t value) {
     c->var_eigh = value;
     return MEP_OK;
   }
+  if (what == MEP_VAR_GEMM32) {
+    if (value < 0 || value > 2) return fail(MEP_ERR_ARG, "mep_set_variant: MEP_VAR_GEMM32 takes 0, 1 or 2");
+    if (value != c->var_gemm32) ++c->gen;
+    c->var_gemm32 = value;
+    return MEP_OK;
+  }
   return fail(MEP_ERR_ARG, "mep_set_variant: unknown stage");
 }
 
@@ -610,8 +617,12 @@ static int interp_block(mep_ctx* c, int64_t nb, bool velo, int prec = MEP_PREC_F
       timed(c, K_PHASE,
             [&] { launch_phase_f32(c->st, nb, c->Kh, c->has_zero, c->dk, c->dR, P32); });
       timed(c, K_GEMM, [&] {
-        launch_ft_gemm_f32(c->st, (int)nb, (int)(2 * c->ncols), (int)c->K, P32, c->dX32,
-                           reinterpret_cast<float*>(H32));
+        if (c->var_gemm32 == 1)
+          launch_ft_gemm_f32(c->st, (int)nb, (int)(2 * c->ncols), (int)c->K, P32, c->dX32,
+                             reinterpret_cast<float*>(H32));
+        else  // default: split bfloat16, FP32-equivalent
+          launch_ft_gemm_bf16x3(c->st, (int)nb, (int)(2 * c->ncols), (int)c->K, P32, c->dX32,
+                                reinterpret_cast<float*>(H32));
       });
     }
     timed(c, K_EIGH, [&] {

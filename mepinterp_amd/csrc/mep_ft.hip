@@ -281,6 +281,147 @@ void launch_phase(hipStream_t s, int64_t nk, int Kh, int has_zero, const double*
                      Rh, Pt);
 }
 
+// --------------------------------------------------------------------------
+// The FP32 Fourier GEMM C = Pt^T X in split bfloat16 ("BF16x3"): every FP32
+// operand is x = hi + mid + lo with three round-to-nearest bf16 parts (8
+// significand bits each, 24 together), and C accumulates in FP32 the six
+// products whose size reaches 2^-24 of hi*hi (lo*hi, hi*lo, mid*mid,
+// mid*hi, hi*mid, hi*hi, smallest first) on v_mfma_f32_16x16x16_bf16.  The
+// dropped mid*lo, lo*mid, lo*lo lie below that, so the result matches the
+// FP32 GEMM to its own rounding.  The split happens once per element, when
+// the K-chunk is staged: LDS holds the three planes m-major with the chunk's
+// 16 k contiguous (pitch 20 bf16 = 40 B: a 16-lane fragment read of 8 B per
+// lane covers the 32 banks once).  Tiles, XCD order and double buffering
+// as ft_gemm_f32_kernel.
+// --------------------------------------------------------------------------
+typedef short bf_v4 __attribute__((ext_vector_type(4)));
+constexpr int B3_P = 20;  // bf16 per staged row: 16 k + 4 pad
+
+__device__ inline unsigned short bf16_rn(float x) {
+  const unsigned u = __float_as_uint(x);
+  return (unsigned short)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+__device__ inline float bf16_f(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
+
+__global__ __launch_bounds__(256, 2) void ft_gemm_bf16x3_kernel(int M, int N, int K,
+                                                                const float* __restrict__ Pt,
+                                                                const float* __restrict__ X,
+                                                                float* __restrict__ C, int tiles_n,
+                                                                int tiles_total) {
+  // [buffer][plane hi / mid / lo][row m or n of the tile][k]
+  __shared__ __attribute__((aligned(16))) unsigned short As[2][3][DG_BM * B3_P];
+  __shared__ __attribute__((aligned(16))) unsigned short Bs[2][3][DG_BN * B3_P];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wr = wave >> 1, wc = wave & 1;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int b = blockIdx.x;
+  const int xcd = b & 7, loc = b >> 3;
+  const int base = tiles_total >> 3, rem = tiles_total & 7;
+  const int L = xcd * base + (xcd < rem ? xcd : rem) + loc;
+  const int64_t m0 = (int64_t)(L / tiles_n) * DG_BM;
+  const int64_t n0 = (int64_t)(L % tiles_n) * DG_BN;
+
+  v4f acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+
+  // staging: thread (sc, sk) takes the eight k = 8 sk .. 8 sk + 7 of tile row sc
+  const int sc = tid & 127, sk = tid >> 7;
+  float ra[8], rb[8];
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int it = 0; it < 8; ++it) {
+      const int kk = k0 + 8 * sk + it;
+      ra[it] = (kk < K && m0 + sc < M) ? Pt[(int64_t)kk * M + m0 + sc] : 0.0f;
+      rb[it] = (kk < K && n0 + sc < N) ? X[(int64_t)kk * N + n0 + sc] : 0.0f;
+    }
+  };
+  auto split_store = [&](const float* r, unsigned short* dst0, int buf_stride) {
+    unsigned short h[3][8];
+#pragma unroll
+    for (int it = 0; it < 8; ++it) {
+      const float x = r[it];
+      const unsigned short a0 = bf16_rn(x);
+      const float r1 = x - bf16_f(a0);
+      const unsigned short a1 = bf16_rn(r1);
+      const float r2 = r1 - bf16_f(a1);
+      h[0][it] = a0;
+      h[1][it] = a1;
+      h[2][it] = bf16_rn(r2);
+    }
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      // row sc, k 8 sk .. 8 sk + 7 at byte 40 sc + 16 sk: two 8-byte stores
+      // (the offset is 8-byte aligned, not 16)
+      uint2 v0, v1;
+      v0.x = (unsigned)h[q][0] | ((unsigned)h[q][1] << 16);
+      v0.y = (unsigned)h[q][2] | ((unsigned)h[q][3] << 16);
+      v1.x = (unsigned)h[q][4] | ((unsigned)h[q][5] << 16);
+      v1.y = (unsigned)h[q][6] | ((unsigned)h[q][7] << 16);
+      unsigned short* d = dst0 + q * buf_stride + sc * B3_P + 8 * sk;
+      *reinterpret_cast<uint2*>(d) = v0;
+      *reinterpret_cast<uint2*>(d + 4) = v1;
+    }
+  };
+  auto lstore = [&](int buf) {
+    split_store(ra, &As[buf][0][0], DG_BM * B3_P);
+    split_store(rb, &Bs[buf][0][0], DG_BN * B3_P);
+  };
+  const int nch = (K + DG_KC - 1) / DG_KC;
+  gload(0);
+  lstore(0);
+  __syncthreads();
+  for (int ch = 0; ch < nch; ++ch) {
+    const int buf = ch & 1;
+    if (ch + 1 < nch) gload((ch + 1) * DG_KC);
+    // fragments: row 16 i + lr of the wave's 64, k = 4 lk .. 4 lk + 3
+    bf_v4 a[3][4], bb[3][4];
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        a[q][i] = *reinterpret_cast<const bf_v4*>(&As[buf][q][(64 * wr + 16 * i + lr) * B3_P + 4 * lk]);
+        bb[q][i] = *reinterpret_cast<const bf_v4*>(&Bs[buf][q][(64 * wc + 16 * i + lr) * B3_P + 4 * lk]);
+      }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        v4f c = acc[i][j];
+        c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[2][i], bb[0][j], c, 0, 0, 0);  // lo hi
+        c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[0][i], bb[2][j], c, 0, 0, 0);  // hi lo
+        c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[1][i], bb[1][j], c, 0, 0, 0);  // mid mid
+        c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[1][i], bb[0][j], c, 0, 0, 0);  // mid hi
+        c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[0][i], bb[1][j], c, 0, 0, 0);  // hi mid
+        c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[0][i], bb[0][j], c, 0, 0, 0);  // hi hi
+        acc[i][j] = c;
+      }
+    if (ch + 1 < nch) lstore(buf ^ 1);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int64_t row = m0 + 64 * wr + 16 * i + 4 * lk + v;
+        const int64_t col = n0 + 64 * wc + 16 * j + lr;
+        if (row < M && col < N) C[row * (int64_t)N + col] = acc[i][j][v];
+      }
+}
+
+void launch_ft_gemm_bf16x3(hipStream_t s, int M, int N, int K, const float* Pt, const float* X,
+                           float* C) {
+  const int tiles_m = (M + DG_BM - 1) / DG_BM, tiles_n = (N + DG_BN - 1) / DG_BN;
+  const int tiles = tiles_m * tiles_n;
+  hipLaunchKernelGGL(ft_gemm_bf16x3_kernel, dim3((unsigned)tiles), dim3(256), 0, s, M, N, K, Pt, X,
+                     C, tiles_n, tiles);
+}
+
 void launch_ft_gemm_f64(hipStream_t s, int M, int N, int K, const double* Pt, const double* X,
                         double* C) {
   const int tiles_m = (M + DG_BM - 1) / DG_BM, tiles_n = (N + DG_BN - 1) / DG_BN;

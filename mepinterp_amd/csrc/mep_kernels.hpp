@@ -36,6 +36,10 @@ struct SepFT {
 void launch_sep_ft(hipStream_t s, const SepFT& f, int64_t k0, int64_t nk, zd* H64, zf* H32);
 void launch_phase_f32(hipStream_t s, int64_t nk, int Kh, int has_zero, const double* kpts,
                       const double* Rh, float* Pt);
+// the same product in split bfloat16 (three bf16 parts per operand, six
+// products accumulated in FP32 on bf16 MFMA): FP32-equivalent
+void launch_ft_gemm_bf16x3(hipStream_t s, int M, int N, int K, const float* Pt, const float* X,
+                           float* C);
 void launch_ft_gemm_f32(hipStream_t s, int M, int N, int K, const float* Pt, const float* X,
                         float* C);
 

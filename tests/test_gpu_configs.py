@@ -225,3 +225,37 @@ def test_wide_householder_variants_match_oracle(n):
             eng.set_variant(capi.MEP_VAR_EIGH, var)
             g = eng.run_kpts(k, host.run_params(p))
             _compare(g, o, what=f"n={n} eigh variant {var}")
+
+
+def test_mixed_kpoint_list_gemm_variants():
+    """The mixed path on an explicit k list (the FP32 Fourier GEMM route):
+    the FP32-MFMA GEMM and the split-bfloat16 GEMM (MEP_VAR_GEMM32) against
+    the FP64 oracle on 1,024 seeded points of C2's 64^3 mesh, and against
+    each other.  Both carry FP32 rounding of H(k) accumulated over NR = 343
+    terms; the split-bfloat16 products reproduce FP32 products to 2^-24."""
+    import oracle as O
+    from mepinterp_amd import capi, host, models
+    from mepinterp_amd.refine import mp_mesh
+    m = models.make_c2(seed=2)
+    mesh = mp_mesh((64, 64, 64))
+    rng = np.random.default_rng(1024)
+    k = np.asfortranarray(mesh[:, rng.choice(mesh.shape[1], size=1024, replace=False)])
+    p = dict(ALL_JOBS, n_eF=1, eF_min=0.0, eF_max=0.0, n_hw=1, hw_min=0.0, hw_max=0.0,
+             T_kelvin=300.0, eta=1e-3 / AU_TO_EV, valence_bands=16)
+    pk = {kk: v for kk, v in p.items() if kk in O.RunParams.__init__.__code__.co_varnames}
+    o = O.accumulate(oracle_model(m), O.RunParams(**pk), k)
+    rp = host.run_params(dict(p))
+    rp.precision = capi.MEP_PREC_MIXED
+    got = {}
+    with _engine(m) as eng:
+        for var in (1, 2):
+            eng.set_variant(capi.MEP_VAR_GEMM32, var)
+            got[var] = eng.run_kpts(k, rp)
+            for key in ("ahc", "kubo_ic", "kubo_lc", "niu_ic", "niu_lc"):
+                err = _rel(got[var][key], o[key])
+                print(f"GEMM32 variant {var} {key}: rel err vs FP64 oracle {err:.2e}", flush=True)
+                assert err < 2e-4, (var, key, err)
+        with pytest.raises(RuntimeError):
+            eng.set_variant(capi.MEP_VAR_GEMM32, 5)
+    for key in ("ahc", "kubo_ic", "niu_ic"):
+        assert _rel(got[2][key], got[1][key]) < 1e-4, key
