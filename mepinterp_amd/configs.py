@@ -101,6 +101,31 @@ def run_c2k(nk: int = 262144, mixed: bool = False):
                 "k list (Fourier GEMM)" + (", mixed precision" if mixed else ""), n_k=nk, **res)
 
 
+def run_c3_models(only):
+    """One run of each listed C3 model (indexes into models.make_c3()), each
+    after a 4^3 warm-up and apart from the others by a pause, so that a
+    kernel trace separates them."""
+    jobs = dict(do_ahc=True, do_kubo=True, do_mep=True, n_eF=1, eF_min=0.0, eF_max=0.0, n_hw=1,
+                hw_min=0.0, hw_max=0.0, T_kelvin=300.0, eta=1e-3 / AU_TO_EV)
+    c3 = models.make_c3()
+    out = []
+    for i in only:
+        m, mp = c3[i]
+        rp = host.run_params(dict(jobs, valence_bands=m["H"].shape[0] // 2))
+        eng = capi.Engine(m["R_frac"], m["H"], a_latt=m["a_latt"])
+        try:
+            eng.run_mesh((4, 4, 4), rp)
+            time.sleep(0.3)
+            t0 = time.perf_counter()
+            eng.run_mesh(mp, rp)
+            dt = time.perf_counter() - t0
+        finally:
+            eng.close()
+        time.sleep(0.3)
+        out.append(dict(model=m["name"], n_k=mp[0] ** 3, seconds=dt, k_pts_s=mp[0] ** 3 / dt))
+    return dict(config="c3", workload="selected C3 models, one run each", models=out)
+
+
 def run_c3(scale: int = 1):
     from . import tune
     jobs = dict(do_ahc=True, do_kubo=True, do_mep=True, n_eF=1, eF_min=0.0, eF_max=0.0, n_hw=1,
@@ -216,6 +241,8 @@ def main(argv=None):
     ap.add_argument("--mixed", action="store_true", help="c2k: the mixed-precision path")
     ap.add_argument("--mesh", type=int, default=None, help="Q for a Q^3 mesh (c1, c4, c5)")
     ap.add_argument("--scale", type=int, default=1, help="divide the c3 meshes by this")
+    ap.add_argument("--only", type=str, default=None,
+                    help="c3: comma-separated model indexes, one run each (for kernel traces)")
     ap.add_argument("--box", type=int, default=9, help="c4: cells per box edge (odd)")
     ap.add_argument("--flat", type=float, default=1.0, help="c4: half-width of the window's flat top")
     a = ap.parse_args(argv)
@@ -224,7 +251,7 @@ def main(argv=None):
     elif a.config == "c2k":
         r = run_c2k(a.kpts, a.mixed)
     elif a.config == "c3":
-        r = run_c3(a.scale)
+        r = run_c3_models([int(x) for x in a.only.split(",")]) if a.only else run_c3(a.scale)
     elif a.config == "c4":
         r = run_c4(a.mesh or 48, a.box, a.flat)
     else:

@@ -27,6 +27,8 @@
 #include "mep_kernels.hpp"
 #include "mep_lane.hpp"
 
+typedef double mep_v4d __attribute__((ext_vector_type(4)));
+
 // NMAX lanes form a team that owns one k-point; a wavefront runs 64/NMAX
 // teams side by side.  Team reductions and broadcasts run on the VALU
 // (team_sum, team_bcast of mep_lane.hpp).
@@ -935,6 +937,146 @@ __global__ __launch_bounds__(32) void eigh_ql_wide_kernel(int n, int64_t nk,
   }
 }
 
+// The eigenvectors of the k-point's real tridiagonal T into
+// Zt[col * (NMAX + 1) + row]: the listed k-points (separation verdict not
+// set) replay their QL rotations on the identity, lane t <-> row t; the
+// separated ones take lane t <-> eigenvector t from T directly.  csb and ixb
+// are the team's staging slots.
+template <int NMAX, class R>
+__device__ __attribute__((always_inline)) inline void tri_vectors(
+    int n, int t, bool valid, bool direct, int64_t k, int64_t cap, const R* sd,
+    const R* __restrict__ rot, const uint8_t* __restrict__ rcol, const double* __restrict__ E,
+    R* Zt, R* csb, int* ixb) {
+  constexpr int LDZ = NMAX + 1;
+  if (!direct) {
+#pragma unroll
+    for (int c = 0; c < NMAX; ++c) Zt[c * LDZ + t] = (c == t) ? R(1) : R(0);
+  }
+  const int64_t Q = (valid && !direct) ? (int64_t)sd[5 * n] : 0;
+  const R* rk = rot + (valid ? k : 0) * 2 * cap;
+  const uint8_t* ck = rcol + (valid ? k : 0) * cap;
+  R nc = R(0), ns = R(0);
+  int ni = 0;
+  if (t < Q) {
+    nc = rk[2 * t];
+    ns = rk[2 * t + 1];
+    ni = ck[t];
+  }
+  int prev = -2;
+  R carry = R(0);
+  for (int64_t q0 = 0; q0 < Q; q0 += NMAX) {
+    __builtin_amdgcn_wave_barrier();
+    csb[2 * t] = nc;
+    csb[2 * t + 1] = ns;
+    ixb[t] = ni;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int64_t qn = q0 + NMAX + t;
+    if (qn < Q) {
+      nc = rk[2 * qn];
+      ns = rk[2 * qn + 1];
+      ni = ck[qn];
+    }
+    const int cnt = (Q - q0 < NMAX) ? (int)(Q - q0) : NMAX;
+    for (int r = 0; r < cnt; ++r) {
+      const R c = csb[2 * r], s = csb[2 * r + 1];
+      const int i = ixb[r];
+      const R zi = Zt[i * LDZ + t];
+      const R zi1 = (prev == i + 1) ? carry : Zt[(i + 1) * LDZ + t];
+      Zt[(i + 1) * LDZ + t] = s * zi + c * zi1;
+      carry = c * zi - s * zi1;
+      Zt[i * LDZ + t] = carry;
+      prev = i;
+    }
+  }
+  if (direct) {
+    // lane t: eigenvector of T = tridiag(e, d, e) for lambda_t (ascending),
+    // from the twisted factorization at the index r of smallest |gamma_r|:
+    //   dp_i = d_i - l - e_{i-1}^2 / dp_{i-1}   (top down)
+    //   um_i = d_i - l - e_i^2 / um_{i+1}       (bottom up)
+    //   gamma_i = dp_i + um_i - (d_i - l)
+    //   z_r = 1, z_i = -e_i z_{i+1} / dp_i (i < r), z_i = -e_{i-1} z_{i-1} / um_i (i > r)
+    // (Dhillon and Parlett's twisted factorizations); pivots below
+    // eps |T| are clamped to +-eps |T|.
+    const R lam = (t < n) ? (R)E[k * n + t] : R(0);
+    // d and e of the k-point in the team's rotation-staging slot (unused
+    // on this path): one coalesced load, then LDS broadcasts
+    csb[t] = (t < n) ? sd[t] : R(0);
+    csb[NMAX + t] = (t < n) ? sd[n + t] : R(0);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const R* dl = csb;
+    const R* el = csb + NMAX;
+    R tn = R(0);
+#pragma unroll
+    for (int i = 0; i < NMAX; ++i)
+      if (i < n) tn = fmax(tn, fabs(dl[i]) + R(2) * fabs(el[i]));
+    const R pivmin = cplx_of<R>::eps * (tn > R(0) ? tn : R(1));
+    R um[NMAX], dp[NMAX];
+    R u = R(1);
+#pragma unroll
+    for (int i = NMAX - 1; i >= 0; --i) {
+      R v = R(1);
+      if (i < n) {
+        const R ei = (i < n - 1) ? el[i] : R(0);
+        v = dl[i] - lam - ((i < n - 1) ? ei * ei * rcp_nr(u) : R(0));
+        if (fabs(v) < pivmin) v = (v < R(0)) ? -pivmin : pivmin;
+        u = v;
+      }
+      um[i] = v;
+    }
+    R dd = R(1);
+#pragma unroll
+    for (int i = 0; i < NMAX; ++i) {
+      R v = R(1);
+      if (i < n) {
+        const R em = (i > 0) ? el[i - 1] : R(0);
+        v = dl[i] - lam - ((i > 0) ? em * em * rcp_nr(dd) : R(0));
+        if (fabs(v) < pivmin) v = (v < R(0)) ? -pivmin : pivmin;
+        dd = v;
+      }
+      dp[i] = v;
+    }
+    int r = 0;
+    R gmin = R(0);
+#pragma unroll
+    for (int i = 0; i < NMAX; ++i)
+      if (i < n) {
+        const R g = fabs(dp[i] + um[i] - (dl[i] - lam));
+        if (i == 0 || g < gmin) {
+          gmin = g;
+          r = i;
+        }
+      }
+    R zp = R(1);
+#pragma unroll
+    for (int i = NMAX - 1; i >= 0; --i)
+      if (i < r) {
+        zp = -(el[i] * rcp_nr(dp[i])) * zp;
+        dp[i] = zp;
+      }
+    zp = R(1);
+#pragma unroll
+    for (int i = 0; i < NMAX; ++i)
+      if (i > r && i < n) {
+        zp = -(el[i - 1] * rcp_nr(um[i])) * zp;
+        um[i] = zp;
+      }
+    R nz = R(0);
+#pragma unroll
+    for (int i = 0; i < NMAX; ++i) {
+      const R zi = (i >= n) ? R(0) : (i < r ? dp[i] : (i == r ? R(1) : um[i]));
+      dp[i] = zi;
+      nz += zi * zi;
+    }
+    const R inv = R(1) / sqrt(nz);
+#pragma unroll
+    for (int i = 0; i < NMAX; ++i) Zt[t * LDZ + i] = dp[i] * inv;
+  }
+}
+
 // A team per k-point.  Z starts as the identity in LDS (lane r <-> row r);
 // the rotation stream is staged through LDS one team-width chunk at a time,
 // the next chunk's loads in flight while the current one is applied.  Then
@@ -975,133 +1117,7 @@ __global__ __launch_bounds__(64) void eigh_vec_kernel(int n, int64_t nk,
     // count slot -1: eigenvalues separated, no rotations recorded; the
     // eigenvectors of the real tridiagonal come straight from T below
     const bool direct = valid && sd[5 * n] < R(0);
-    if (!direct) {
-#pragma unroll
-      for (int c = 0; c < NMAX; ++c) Zt[c * LDZ + t] = (c == t) ? R(1) : R(0);
-    }
-    const int64_t Q = (valid && !direct) ? (int64_t)sd[5 * n] : 0;
-    const R* rk = rot + (valid ? k : 0) * 2 * cap;
-    const uint8_t* ck = rcol + (valid ? k : 0) * cap;
-    R nc = R(0), ns = R(0);
-    int ni = 0;
-    if (t < Q) {
-      nc = rk[2 * t];
-      ns = rk[2 * t + 1];
-      ni = ck[t];
-    }
-    int prev = -2;
-    R carry = R(0);
-    for (int64_t q0 = 0; q0 < Q; q0 += NMAX) {
-      __builtin_amdgcn_wave_barrier();
-      csb[2 * t] = nc;
-      csb[2 * t + 1] = ns;
-      ixb[t] = ni;
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      const int64_t qn = q0 + NMAX + t;
-      if (qn < Q) {
-        nc = rk[2 * qn];
-        ns = rk[2 * qn + 1];
-        ni = ck[qn];
-      }
-      const int cnt = (Q - q0 < NMAX) ? (int)(Q - q0) : NMAX;
-      for (int r = 0; r < cnt; ++r) {
-        const R c = csb[2 * r], s = csb[2 * r + 1];
-        const int i = ixb[r];
-        const R zi = Zt[i * LDZ + t];
-        const R zi1 = (prev == i + 1) ? carry : Zt[(i + 1) * LDZ + t];
-        Zt[(i + 1) * LDZ + t] = s * zi + c * zi1;
-        carry = c * zi - s * zi1;
-        Zt[i * LDZ + t] = carry;
-        prev = i;
-      }
-    }
-    if (direct) {
-      // lane t: eigenvector of T = tridiag(e, d, e) for lambda_t (ascending),
-      // from the twisted factorization at the index r of smallest |gamma_r|:
-      //   dp_i = d_i - l - e_{i-1}^2 / dp_{i-1}   (top down)
-      //   um_i = d_i - l - e_i^2 / um_{i+1}       (bottom up)
-      //   gamma_i = dp_i + um_i - (d_i - l)
-      //   z_r = 1, z_i = -e_i z_{i+1} / dp_i (i < r), z_i = -e_{i-1} z_{i-1} / um_i (i > r)
-      // (Dhillon and Parlett's twisted factorizations); pivots below
-      // eps |T| are clamped to +-eps |T|.
-      const R lam = (t < n) ? (R)E[k * n + t] : R(0);
-      // d and e of the k-point in the team's rotation-staging slot (unused
-      // on this path): one coalesced load, then LDS broadcasts
-      csb[t] = (t < n) ? sd[t] : R(0);
-      csb[NMAX + t] = (t < n) ? sd[n + t] : R(0);
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      const R* dl = csb;
-      const R* el = csb + NMAX;
-      R tn = R(0);
-#pragma unroll
-      for (int i = 0; i < NMAX; ++i)
-        if (i < n) tn = fmax(tn, fabs(dl[i]) + R(2) * fabs(el[i]));
-      const R pivmin = cplx_of<R>::eps * (tn > R(0) ? tn : R(1));
-      R um[NMAX], dp[NMAX];
-      R u = R(1);
-#pragma unroll
-      for (int i = NMAX - 1; i >= 0; --i) {
-        R v = R(1);
-        if (i < n) {
-          const R ei = (i < n - 1) ? el[i] : R(0);
-          v = dl[i] - lam - ((i < n - 1) ? ei * ei * rcp_nr(u) : R(0));
-          if (fabs(v) < pivmin) v = (v < R(0)) ? -pivmin : pivmin;
-          u = v;
-        }
-        um[i] = v;
-      }
-      R dd = R(1);
-#pragma unroll
-      for (int i = 0; i < NMAX; ++i) {
-        R v = R(1);
-        if (i < n) {
-          const R em = (i > 0) ? el[i - 1] : R(0);
-          v = dl[i] - lam - ((i > 0) ? em * em * rcp_nr(dd) : R(0));
-          if (fabs(v) < pivmin) v = (v < R(0)) ? -pivmin : pivmin;
-          dd = v;
-        }
-        dp[i] = v;
-      }
-      int r = 0;
-      R gmin = R(0);
-#pragma unroll
-      for (int i = 0; i < NMAX; ++i)
-        if (i < n) {
-          const R g = fabs(dp[i] + um[i] - (dl[i] - lam));
-          if (i == 0 || g < gmin) {
-            gmin = g;
-            r = i;
-          }
-        }
-      R zp = R(1);
-#pragma unroll
-      for (int i = NMAX - 1; i >= 0; --i)
-        if (i < r) {
-          zp = -(el[i] * rcp_nr(dp[i])) * zp;
-          dp[i] = zp;
-        }
-      zp = R(1);
-#pragma unroll
-      for (int i = 0; i < NMAX; ++i)
-        if (i > r && i < n) {
-          zp = -(el[i - 1] * rcp_nr(um[i])) * zp;
-          um[i] = zp;
-        }
-      R nz = R(0);
-#pragma unroll
-      for (int i = 0; i < NMAX; ++i) {
-        const R zi = (i >= n) ? R(0) : (i < r ? dp[i] : (i == r ? R(1) : um[i]));
-        dp[i] = zi;
-        nz += zi * zi;
-      }
-      const R inv = R(1) / sqrt(nz);
-#pragma unroll
-      for (int i = 0; i < NMAX; ++i) Zt[t * LDZ + i] = dp[i] * inv;
-    }
+    tri_vectors<NMAX, R>(n, t, valid, direct, k, cap, sd, rot, rcol, E, Zt, csb, ixb);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -1701,105 +1717,243 @@ size_t eigh_replay_wide_lds(int n) {
 
 // columns per LDS block of the back-transformation (about 73 KB of LDS, so
 // that two workgroups fit a CU with the reflector buffers and tau)
-static inline int back_cb(int n) {
-  int cb = (73 * 1024) / (16 * (n + 1));
-  return cb < n ? cb : n;
-}
-
-// Back-transformation U = Q W for 32 < n <= 128: a block of CB columns of W
-// in LDS, two threads per column (even / odd rows, combined by a lane
-// shuffle), the reflectors read from U's slot (the same address across a
-// wave).  Each block goes back to the rotation record with the column's
-// gauge phase applied; U is written once every block is done.
-__global__ __launch_bounds__(256) void eigh_back_wide_kernel(int n, int64_t nk, int CB,
-                                                             const double* __restrict__ side,
-                                                             double* __restrict__ rot,
-                                                             zd* __restrict__ U) {
-  extern __shared__ double bw_smem[];
-  const int LDW = CB + 1;
-  zd* Ws = reinterpret_cast<zd*>(bw_smem);  // [n][LDW]
-  zd* Vb = Ws + (int64_t)n * LDW;           // [2][n] reflector columns (double buffer)
-  double* taus = reinterpret_cast<double*>(Vb + 2 * n);  // [n]
-  const int t = threadIdx.x, jj = t >> 1, hf = t & 1;  // blockDim.x >= n
+// Back-transformation U = Q W for 32 < n <= 128 on FP64 MFMA.  The n - 2
+// reflectors form blocks of 16; block J (reflectors 16J .. 16J + 15) is
+// Q_J = H_16J ... H_16J+15 = I - V T V^H (compact WY: T upper triangular,
+// T(i, i) = tau_i, T(0:i, i) = -tau_i T(0:i, 0:i) G(0:i, i), G = V^H V), and
+// W <- Q_J W = W - V (T (V^H W)) runs for J from the last block to the first.
+// One wavefront per 16-column strip of W (NS = ceil(n / 16) waves per
+// k-point); the strip stays in the wave's registers as NS accumulator tiles
+// of 16 x 16 in the MFMA layout (lane l holds rows l / 16 + 4 v, v < 4, of
+// column l % 16), and serves directly as the B operand of V^H W with the
+// sum order over rows permuted to match.  Complex products take three real
+// products each.  Per k-point: prologue, wave s computes T of the blocks
+// J = s, s + NS, ... (G on MFMA from the reflectors in L2, the recurrence
+// one row of T per lane); then per block the reflectors are staged in LDS
+// (the next block's loads in flight) and every wave applies Q_J to its strip.
+// Then each column takes the gauge convention (largest-modulus component,
+// first on ties, real > 0) and goes to U's column of its eigenvalue rank.
+template <int NSMAX>
+__global__ __launch_bounds__(64 * NSMAX, NSMAX <= 4 ? 3 : 2) void eigh_back_mfma_kernel(int n, int64_t nk,
+                                                                   const double* __restrict__ side,
+                                                                   const double* __restrict__ rot,
+                                                                   zd* __restrict__ U) {
+  extern __shared__ __attribute__((aligned(16))) double bm_smem[];
+  constexpr int VP = 17;  // LDS pitch (complex) of the staged block, G and T
+  const int NS = (n + 15) >> 4;      // strips = waves = 16-row tiles
+  const int NJ = (n - 2 + 15) >> 4;  // blocks of 16 reflectors
+  zd* Vs = reinterpret_cast<zd*>(bm_smem);  // [16 NS][VP]; in the prologue G per wave
+  zd* Ts = Vs + 16 * NS * VP;               // [NJ][16][VP]
   const int64_t cap = rot_cap(n);
+  const zd zero = zd{0.0, 0.0};
   for (int64_t k = blockIdx.x; k < nk; k += gridDim.x) {
+    // the thread index is made opaque per k-point: otherwise the compiler
+    // hoists every lane-dependent address out of this loop and spills them
+    int t = threadIdx.x;
+    asm volatile("" : "+v"(t));
+    const int lane = t & 63;
+    const int s = __builtin_amdgcn_readfirstlane(t >> 6);
+    const int q = lane >> 4, c = lane & 15;
+    const int j = 16 * s + c;  // the lane's column of W
     const double* sd = side + k * side_len(n);
-    zd* W = reinterpret_cast<zd*>(rot + k * 2 * cap);
+    const zd* Wg = reinterpret_cast<const zd*>(rot + k * 2 * cap);
     const zd* Uk = U + k * (int64_t)n * n;
-    for (int c0 = 0; c0 < n; c0 += CB) {
-      const int cbn = (n - c0 < CB) ? n - c0 : CB;
-      __syncthreads();
-      for (int idx = t; idx < n * cbn; idx += blockDim.x) {
-        const int i = idx / cbn, c = idx % cbn;
-        Ws[i * LDW + c] = W[(int64_t)i * n + c0 + c];
+    mep_v4d wr[NSMAX], wi[NSMAX];
+#pragma unroll
+    for (int rt = 0; rt < NSMAX; ++rt)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = 16 * rt + q + 4 * v;
+        const zd x = (rt < NS && row < n && j < n) ? Wg[(int64_t)row * n + j] : zero;
+        wr[rt][v] = x.re;
+        wi[rt][v] = x.im;
       }
-      // tau in LDS; the reflector columns pass through LDS one at a time,
-      // the next one's load in flight while the current one is applied
-      if (t < n) {
-        taus[t] = sd[2 * n + t];
-        Vb[t] = (n >= 3) ? Uk[(int64_t)n * (n - 3) + t] : zd{0.0, 0.0};
-      }
-      __syncthreads();
-      const bool act = jj < cbn;
-      int cur = 0;
-      for (int kc = n - 3; kc >= 0; --kc) {
-        const zd nxt = (kc > 0 && t < n) ? Uk[(int64_t)n * (kc - 1) + t] : zd{0.0, 0.0};
-        const double tk = taus[kc];
-        if (act && tk != 0.0) {
-          const zd* vcol = Vb + cur * n;
-          const int i0 = kc + 1 + hf;
-          zd s0 = zd{0.0, 0.0}, s1 = zd{0.0, 0.0};
-          int i = i0;
-          for (; i + 2 < n; i += 4) {
-            s0 = zadd(s0, zcmul(vcol[i], Ws[i * LDW + jj]));
-            s1 = zadd(s1, zcmul(vcol[i + 2], Ws[(i + 2) * LDW + jj]));
-          }
-          if (i < n) s0 = zadd(s0, zcmul(vcol[i], Ws[i * LDW + jj]));
-          zd sacc = zadd(s0, s1);
-          sacc.re += __shfl_xor(sacc.re, 1, 64);
-          sacc.im += __shfl_xor(sacc.im, 1, 64);
-          sacc = zscale(sacc, tk);
-          for (int i2 = i0; i2 < n; i2 += 2)
-            Ws[i2 * LDW + jj] = zsub(Ws[i2 * LDW + jj], zmul(vcol[i2], sacc));
+    // ---- prologue: T of the blocks J = s, s + NS, ... ---------------------
+    zd* G = Vs + s * 16 * VP;
+    __syncthreads();  // the previous k-point's reads of Vs and Ts are done
+    for (int J = s; J < NJ; J += NS) {
+      const int kc = 16 * J + c;
+      const bool colok = kc <= n - 3;
+      mep_v4d g1 = {0, 0, 0, 0}, g2 = {0, 0, 0, 0}, g3 = {0, 0, 0, 0};
+      for (int rt = J; rt < NS; ++rt) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int row = 16 * rt + q + 4 * v;
+          const zd x = (colok && row > kc && row < n) ? Uk[(int64_t)n * kc + row] : zero;
+          // G = V^H V: A = conj(V)^T and B = V hold the same element here
+          g1 = __builtin_amdgcn_mfma_f64_16x16x4f64(x.re, x.re, g1, 0, 0, 0);
+          g2 = __builtin_amdgcn_mfma_f64_16x16x4f64(-x.im, x.im, g2, 0, 0, 0);
+          g3 = __builtin_amdgcn_mfma_f64_16x16x4f64(x.re - x.im, x.re + x.im, g3, 0, 0, 0);
         }
-        if (t < n) Vb[(cur ^ 1) * n + t] = nxt;
-        __syncthreads();
-        cur ^= 1;
       }
-      // gauge: largest-modulus component (first on ties) real > 0
-      double best = -1.0;
-      int bi = n;
-      if (act)
-        for (int i = hf; i < n; i += 2) {
-          const double m2 = zabs2(Ws[i * LDW + jj]);
-          if (m2 > best) {
-            best = m2;
-            bi = i;
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        stz(G + (q + 4 * v) * VP + c, zd{g1[v] - g2[v], g3[v] - g1[v] - g2[v]});
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      // lane r < 16: row r of T, T(r, i) = -tau_i sum_{r <= m < i} T(r, m) G(m, i),
+      // the row kept in LDS (each lane reads back only what it wrote)
+      zd* TJ = Ts + J * 16 * VP;
+      if (lane < 16) {
+        const int r = lane;
+#pragma unroll 1
+        for (int i = 0; i < 16; ++i) {
+          const double ti = (16 * J + i <= n - 3) ? sd[2 * n + 16 * J + i] : 0.0;
+          zd x0 = zero, x1 = zero;
+          int m = r;
+          for (; m + 1 < i; m += 2) {
+            const zd g0 = ldz(G + m * VP + i), t0 = ldz(TJ + r * VP + m);
+            const zd g1_ = ldz(G + (m + 1) * VP + i), t1 = ldz(TJ + r * VP + m + 1);
+            x0.re = fma(t0.re, g0.re, fma(-t0.im, g0.im, x0.re));
+            x0.im = fma(t0.re, g0.im, fma(t0.im, g0.re, x0.im));
+            x1.re = fma(t1.re, g1_.re, fma(-t1.im, g1_.im, x1.re));
+            x1.im = fma(t1.re, g1_.im, fma(t1.im, g1_.re, x1.im));
           }
+          if (m < i) {
+            const zd g0 = ldz(G + m * VP + i), t0 = ldz(TJ + r * VP + m);
+            x0.re = fma(t0.re, g0.re, fma(-t0.im, g0.im, x0.re));
+            x0.im = fma(t0.re, g0.im, fma(t0.im, g0.re, x0.im));
+          }
+          const zd x = zadd(x0, x1);
+          stz(TJ + r * VP + i, (i < r) ? zero : (i == r) ? zd{ti, 0.0} : zd{-ti * x.re, -ti * x.im});
         }
-      const double ob = __shfl_xor(best, 1, 64);
-      const int oi = __shfl_xor(bi, 1, 64);
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    // ---- blocks from the last to the first ---------------------------------
+    // staging: thread t takes elements t + 64 NS u (u < 4) of the block,
+    // consecutive threads on consecutive rows of one reflector
+    const int nrow = 16 * NS;
+    zd pv[4];
+    auto fetch = [&](int J) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int idx = t + 64 * NS * u;
+        const int row = idx % nrow, cc = idx / nrow, kc = 16 * J + cc;
+        pv[u] = (kc <= n - 3 && row > kc && row < n) ? Uk[(int64_t)n * kc + row] : zero;
+      }
+    };
+    fetch(NJ - 1);
+    for (int J = NJ - 1; J >= 0; --J) {
+      __syncthreads();  // T is complete; the previous block's reads of Vs are done
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int idx = t + 64 * NS * u;
+        stz(Vs + (idx % nrow) * VP + idx / nrow, pv[u]);
+      }
+      __syncthreads();
+      if (J > 0) fetch(J - 1);
+      // Y = V^H W (rows above 16 J hold zeros in V)
+      mep_v4d p1 = {0, 0, 0, 0}, p2 = {0, 0, 0, 0}, p3 = {0, 0, 0, 0};
+#pragma unroll
+      for (int rt = 0; rt < NSMAX; ++rt) {
+        if (rt < J || rt >= NS) continue;  // uniform
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const zd a = ldz(Vs + (16 * rt + q + 4 * v) * VP + c);  // conj V(row, 16 J + c)
+          p1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a.re, wr[rt][v], p1, 0, 0, 0);
+          p2 = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.im, wi[rt][v], p2, 0, 0, 0);
+          p3 = __builtin_amdgcn_mfma_f64_16x16x4f64(a.re - a.im, wr[rt][v] + wi[rt][v], p3, 0, 0, 0);
+        }
+        __builtin_amdgcn_sched_barrier(0);  // one row tile's operands in flight at a time
+      }
+      mep_v4d yr, yi;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        yr[v] = p1[v] - p2[v];
+        yi[v] = p3[v] - p1[v] - p2[v];
+      }
+      // Y' = T Y
+      const zd* TJ = Ts + J * 16 * VP;
+      p1 = p2 = p3 = mep_v4d{0, 0, 0, 0};
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const zd tt = ldz(TJ + c * VP + q + 4 * v);  // T(c, q + 4v)
+        p1 = __builtin_amdgcn_mfma_f64_16x16x4f64(tt.re, yr[v], p1, 0, 0, 0);
+        p2 = __builtin_amdgcn_mfma_f64_16x16x4f64(tt.im, yi[v], p2, 0, 0, 0);
+        p3 = __builtin_amdgcn_mfma_f64_16x16x4f64(tt.re + tt.im, yr[v] + yi[v], p3, 0, 0, 0);
+      }
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        yr[v] = p1[v] - p2[v];
+        yi[v] = p3[v] - p1[v] - p2[v];
+      }
+      // W -= V Y'
+#pragma unroll
+      for (int rt = 0; rt < NSMAX; ++rt) {
+        if (rt < J || rt >= NS) continue;  // uniform
+        p1 = p2 = p3 = mep_v4d{0, 0, 0, 0};
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const zd a = ldz(Vs + (16 * rt + c) * VP + q + 4 * v);  // V(16 rt + c, 16 J + q + 4v)
+          p1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a.re, yr[v], p1, 0, 0, 0);
+          p2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a.im, yi[v], p2, 0, 0, 0);
+          p3 = __builtin_amdgcn_mfma_f64_16x16x4f64(a.re + a.im, yr[v] + yi[v], p3, 0, 0, 0);
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          wr[rt][v] -= p1[v] - p2[v];
+          wi[rt][v] -= p3[v] - p1[v] - p2[v];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+    // ---- gauge: largest-modulus component (first on ties) real > 0 ---------
+    double best = -1.0;
+    int bi = n;
+    double ur = 1.0, ui = 0.0;
+#pragma unroll
+    for (int rt = 0; rt < NSMAX; ++rt)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = 16 * rt + q + 4 * v;
+        const double m2 = wr[rt][v] * wr[rt][v] + wi[rt][v] * wi[rt][v];
+        if (rt < NS && row < n && m2 > best) {
+          best = m2;
+          bi = row;
+          ur = wr[rt][v];
+          ui = wi[rt][v];
+        }
+      }
+#pragma unroll
+    for (int off = 16; off <= 32; off *= 2) {
+      const double ob = __shfl_xor(best, off, 64);
+      const int oi = __shfl_xor(bi, off, 64);
+      const double our = __shfl_xor(ur, off, 64), oui = __shfl_xor(ui, off, 64);
       if (ob > best || (ob == best && oi < bi)) {
         best = ob;
         bi = oi;
+        ur = our;
+        ui = oui;
       }
-      zd phs = zd{1.0, 0.0};
-      if (act && bi < n) {
-        const zd u = Ws[bi * LDW + jj];
-        const double au = sqrt(zabs2(u));
-        if (au > 0.0) phs = zd{u.re / au, -u.im / au};
-      }
-      if (act)
-        for (int i = hf; i < n; i += 2) W[(int64_t)i * n + c0 + jj] = zmul(Ws[i * LDW + jj], phs);
     }
-    __syncthreads();  // every reflector read before U is overwritten
-    for (int idx = t; idx < n * n; idx += blockDim.x) {
-      const int i = idx % n, j = idx / n;
+    zd phs = zd{1.0, 0.0};
+    {
+      const double au = sqrt(ur * ur + ui * ui);
+      if (bi < n && au > 0.0) phs = zd{ur / au, -ui / au};
+    }
+    // every wave has passed the last staging barrier, so no reflector of
+    // this k-point is read after this point
+    if (j < n) {
       const int rnk = (int)sd[n + j];
-      U[k * (int64_t)n * n + (int64_t)n * rnk + i] = W[(int64_t)i * n + j];
+      zd* Uo = U + k * (int64_t)n * n + (int64_t)n * rnk;
+#pragma unroll
+      for (int rt = 0; rt < NSMAX; ++rt)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int row = 16 * rt + q + 4 * v;
+          if (rt < NS && row < n) Uo[row] = zmul(zd{wr[rt][v], wi[rt][v]}, phs);
+        }
     }
-    __syncthreads();
   }
+}
+
+size_t eigh_back_mfma_lds(int n) {
+  const size_t ns = (size_t)(n + 15) / 16, nj = (size_t)(n - 2 + 15) / 16;
+  return sizeof(zd) * 17 * 16 * (ns + nj);
 }
 
 static size_t eigh_list_offset(int n, int64_t nk);
@@ -1809,16 +1963,12 @@ void launch_eigh_wide(hipStream_t s, int n, int64_t nk, const zd* Hk, int64_t st
   double* side = scratch;
   double* rot = side + nk * side_len(n);
   uint8_t* rcol = reinterpret_cast<uint8_t*>(rot + nk * 2 * rot_cap(n));
-  const int CB = back_cb(n);
   const size_t lh = eigh_house_wide_lds(n), lr = eigh_replay_wide_lds(n);
-  const size_t lb = sizeof(zd) * ((size_t)n * (CB + 1) + 2 * (size_t)n) + sizeof(double) * (size_t)n;
   const size_t lq = sizeof(double) * 2 * (size_t)n * 32;
   (void)hipFuncSetAttribute((const void*)eigh_house_wide_kernel,
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lh);
   (void)hipFuncSetAttribute((const void*)eigh_replay_wide_kernel,
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lr);
-  (void)hipFuncSetAttribute((const void*)eigh_back_wide_kernel,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lb);
   (void)hipFuncSetAttribute((const void*)eigh_ql_wide_kernel<double>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lq);
   unsigned grid = (unsigned)(nk < 4096 ? nk : 4096);
@@ -1846,10 +1996,25 @@ void launch_eigh_wide(hipStream_t s, int n, int64_t nk, const zd* Hk, int64_t st
   const int rthreads = n <= 64 ? 64 : 128;
   hipLaunchKernelGGL(eigh_replay_wide_kernel, dim3(grid), dim3(rthreads), lr, s, n, nk, side, rot,
                      rcol, E);
-  int bthreads = ((2 * CB + 63) / 64) * 64;
-  if (bthreads < n) bthreads = ((n + 63) / 64) * 64;  // one thread per reflector row
-  hipLaunchKernelGGL(eigh_back_wide_kernel, dim3(grid), dim3(bthreads), lb, s, n, nk, CB, side, rot,
-                     U);
+  // back-transformation on MFMA: one wavefront per 16-column strip
+  const size_t lm = eigh_back_mfma_lds(n);
+  const int ns = (n + 15) / 16;
+  const unsigned mgrid = (unsigned)(nk < 65536 ? nk : 65536);
+#define MEP_BACKM_LAUNCH(NSM)                                                                     \
+  (void)hipFuncSetAttribute((const void*)eigh_back_mfma_kernel<NSM>,                             \
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lm);                \
+  hipLaunchKernelGGL(eigh_back_mfma_kernel<NSM>, dim3(mgrid), dim3(64 * ns), lm, s, n, nk, side, \
+                     rot, U);
+  if (ns <= 3) {
+    MEP_BACKM_LAUNCH(3)
+  } else if (ns <= 4) {
+    MEP_BACKM_LAUNCH(4)
+  } else if (ns <= 6) {  // five strips: the six-strip build (no register spills)
+    MEP_BACKM_LAUNCH(6)
+  } else {
+    MEP_BACKM_LAUNCH(8)
+  }
+#undef MEP_BACKM_LAUNCH
 }
 
 // side records, rotation records, rotation columns, then (256-byte
