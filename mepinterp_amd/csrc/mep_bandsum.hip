@@ -1227,11 +1227,12 @@ size_t bandsum_wide_lds_bytes(const BandArgs& b) {
 // x != y, whose occupations differ at some E_F are compacted, in pair order,
 // into LDS chunks of up to 256 records (dE, f_x - f_y per E_F, the 9
 // Im(V_i(x,y) V_j(y,x)) and the 9 complex V_i(y,x) V_j(x,y)); thread
-// (item, split) walks every split-th record for its item = (hw, E_F),
+// (slot, split) walks every split-th record for its IPT items = (hw, E_F),
 // accumulating in registers over all k; the splits are combined once at the
-// end.  Items beyond 256 are taken in further passes over the k-points.
+// end.  Items beyond 256 IPT are taken in further passes over the k-points.
 // ===========================================================================
 constexpr int HW_CH = 256;  // pair candidates per chunk (one per thread)
+constexpr int HW_IPT = 2;   // items per thread, Hermitian V
 
 
 // Records per pair (x, y) of differing occupation: dE, w_k (f_x - f_y) for
@@ -1245,8 +1246,10 @@ __host__ __device__ constexpr int hw_vals() { return HERM ? 12 : 27; }
 template <bool HERM>
 __host__ __device__ constexpr int hw_accs() { return HERM ? 24 : 36; }
 
-template <bool HERM>
-__global__ __launch_bounds__(256, 3) void hwsum_kernel(BandArgs b, double Tsmear, AccLayout L,
+// IPT items per thread: every record read from LDS feeds IPT items (the
+// LDS return path, not the VALU, limits one item per thread).
+template <bool HERM, int IPT>
+__global__ __launch_bounds__(256, IPT > 1 ? 2 : 3) void hwsum_kernel(BandArgs b, double Tsmear, AccLayout L,
                                                        int64_t nk, const double* __restrict__ E,
                                                        const zd* __restrict__ V,
                                                        const double* __restrict__ w,
@@ -1264,17 +1267,30 @@ __global__ __launch_bounds__(256, 3) void hwsum_kernel(BandArgs b, double Tsmear
   const bool finiteT = b.T_kelvin > 1e-2;
   const int items = nhw * nef;
   double* accp = part + (int64_t)blockIdx.x * L.total;
-  for (int g0 = 0; g0 < items; g0 += 256) {
-    const int IW = (items - g0 < 256) ? items - g0 : 256;
-    const int S = 256 / IW;
-    const int it = g0 + t % IW, split = t / IW;
+  for (int g0 = 0; g0 < items; g0 += 256 * IPT) {
+    // this pass: IW items, IWT thread slots per split, each slot holding the
+    // items slot + IWT u (u < IPT), S splits over the records
+    const int IW = (items - g0 < 256 * IPT) ? items - g0 : 256 * IPT;
+    const int IWT = (IW + IPT - 1) / IPT;
+    const int S = 256 / IWT;
+    const int slot = t % IWT, split = t / IWT;
     const bool act = split < S;
-    const int h = it % nhw, ef = it / nhw;
-    const zd wh = zd{b.hw[act ? h : 0], b.eta};
-    const zd wh2 = zmul(wh, wh);
-    double acc[NA];
+    bool iok[IPT];
+    int ef[IPT];
+    zd wh[IPT], wh2[IPT];
 #pragma unroll
-    for (int q = 0; q < NA; ++q) acc[q] = 0.0;
+    for (int u = 0; u < IPT; ++u) {
+      iok[u] = act && slot + IWT * u < IW;
+      const int it = g0 + (iok[u] ? slot + IWT * u : 0);
+      ef[u] = it / nhw;
+      wh[u] = zd{b.hw[it % nhw], b.eta};
+      wh2[u] = zmul(wh[u], wh[u]);
+    }
+    double acc[IPT][NA];
+#pragma unroll
+    for (int u = 0; u < IPT; ++u)
+#pragma unroll
+      for (int q = 0; q < NA; ++q) acc[u][q] = 0.0;
     for (int64_t k = blockIdx.x; k < nk; k += gridDim.x) {
       const double wk = w ? w[k] : 1.0;
       const zd* Vk = V + k * 3 * (int64_t)n2;
@@ -1344,46 +1360,49 @@ __global__ __launch_bounds__(256, 3) void hwsum_kernel(BandArgs b, double Tsmear
         }
         __syncthreads();
         if (act)
-#pragma unroll 4
+#pragma unroll 2
           for (int j = split; j < cnt; j += S) {  // several records' LDS loads in flight
             const double* sp = st + (int64_t)j * PS;
             const double dE = sp[0];
-            const double f = sp[1 + ef];
             const double* pq = sp + 1 + nef;
-            const zd denv = zd{dE * dE - wh2.re, -wh2.im};
-            const zd deno = zmul(zd{dE, 0.0}, zd{dE - wh.re, -wh.im});
-            const double av = zabs2(denv), ao = zabs2(deno);
-            // one reciprocal for both denominators; f = 0 records (another
-            // E_F differs) contribute exactly zero
-            const double fi = f * rcp_nr(av * ao);
-            const double iv = fi * ao, io = fi * av;
-            const zd rv = zd{denv.re * iv, -denv.im * iv};
-            const zd ro = zd{deno.re * io, -deno.im * io};
-            if constexpr (HERM) {
-              // acc: [0..5] rv Im P_c, [6..11] Re(ro) |x_a|^2 then Im(ro) |x_a|^2,
-              // [12..17] Re(ro) P_c, [18..23] Im(ro) P_c
 #pragma unroll
-              for (int c = 0; c < 3; ++c) {
-                acc[2 * c] += pq[c] * rv.re;
-                acc[2 * c + 1] += pq[c] * rv.im;
-                acc[6 + c] += ro.re * pq[9 + c];
-                acc[9 + c] += ro.im * pq[9 + c];
-                acc[12 + 2 * c] += ro.re * pq[3 + 2 * c];
-                acc[13 + 2 * c] += ro.re * pq[4 + 2 * c];
-                acc[18 + 2 * c] += ro.im * pq[3 + 2 * c];
-                acc[19 + 2 * c] += ro.im * pq[4 + 2 * c];
-              }
-            } else {
-              // acc: [0..17] velo_q (re, im), [18..35] ohc_q (re, im)
+            for (int u = 0; u < IPT; ++u) {
+              const double f = sp[1 + ef[u]];
+              const zd denv = zd{dE * dE - wh2[u].re, -wh2[u].im};
+              const zd deno = zmul(zd{dE, 0.0}, zd{dE - wh[u].re, -wh[u].im});
+              const double av = zabs2(denv), ao = zabs2(deno);
+              // one reciprocal for both denominators; f = 0 records (another
+              // E_F differs) contribute exactly zero
+              const double fi = f * rcp_nr(av * ao);
+              const double iv = fi * ao, io = fi * av;
+              const zd rv = zd{denv.re * iv, -denv.im * iv};
+              const zd ro = zd{deno.re * io, -deno.im * io};
+              if constexpr (HERM) {
+                // acc: [0..5] rv Im P_c, [6..11] Re(ro) |x_a|^2 then Im(ro) |x_a|^2,
+                // [12..17] Re(ro) P_c, [18..23] Im(ro) P_c
 #pragma unroll
-              for (int q = 0; q < 9; ++q) {
-                const double vim = pq[q];
-                acc[2 * q] += vim * rv.re;
-                acc[2 * q + 1] += vim * rv.im;
-                // i * (q2o * ro)
-                const double qr = pq[9 + 2 * q], qi = pq[9 + 2 * q + 1];
-                acc[18 + 2 * q] -= qr * ro.im + qi * ro.re;
-                acc[18 + 2 * q + 1] += qr * ro.re - qi * ro.im;
+                for (int c = 0; c < 3; ++c) {
+                  acc[u][2 * c] += pq[c] * rv.re;
+                  acc[u][2 * c + 1] += pq[c] * rv.im;
+                  acc[u][6 + c] += ro.re * pq[9 + c];
+                  acc[u][9 + c] += ro.im * pq[9 + c];
+                  acc[u][12 + 2 * c] += ro.re * pq[3 + 2 * c];
+                  acc[u][13 + 2 * c] += ro.re * pq[4 + 2 * c];
+                  acc[u][18 + 2 * c] += ro.im * pq[3 + 2 * c];
+                  acc[u][19 + 2 * c] += ro.im * pq[4 + 2 * c];
+                }
+              } else {
+                // acc: [0..17] velo_q (re, im), [18..35] ohc_q (re, im)
+#pragma unroll
+                for (int q = 0; q < 9; ++q) {
+                  const double vim = pq[q];
+                  acc[u][2 * q] += vim * rv.re;
+                  acc[u][2 * q + 1] += vim * rv.im;
+                  // i * (q2o * ro)
+                  const double qr = pq[9 + 2 * q], qi = pq[9 + 2 * q + 1];
+                  acc[u][18 + 2 * q] -= qr * ro.im + qi * ro.re;
+                  acc[u][18 + 2 * q + 1] += qr * ro.re - qi * ro.im;
+                }
               }
             }
           }
@@ -1396,7 +1415,11 @@ __global__ __launch_bounds__(256, 3) void hwsum_kernel(BandArgs b, double Tsmear
       __syncthreads();
       if (act)
 #pragma unroll
-        for (int q = 0; q < HA; ++q) st[(split * IW + t % IW) * HA + q] = acc[half * HA + q];
+        for (int u = 0; u < IPT; ++u)
+          if (iok[u])
+#pragma unroll
+            for (int q = 0; q < HA; ++q)
+              st[(split * IW + slot + IWT * u) * HA + q] = acc[u][half * HA + q];
       __syncthreads();
       auto red = [&](int i2, int q) {
         double sR = 0.0;
@@ -1452,7 +1475,8 @@ size_t hwsum_lds_bytes(const BandArgs& b) {
   const size_t nv = b.v_herm ? hw_vals<true>() : hw_vals<false>();
   const size_t ha = (b.v_herm ? hw_accs<true>() : hw_accs<false>()) / 2;
   const size_t ps = 1 + (size_t)b.n_ef + nv;
-  const size_t stage = (size_t)HW_CH * ps > 256 * ha ? (size_t)HW_CH * ps : 256 * ha;
+  const size_t ipt = b.v_herm ? HW_IPT : 1;
+  const size_t stage = (size_t)HW_CH * ps > 256 * ipt * ha ? (size_t)HW_CH * ps : 256 * ipt * ha;
   return sizeof(double) * ((size_t)b.n + (size_t)b.n_ef * b.n + 4 + stage);
 }
 
@@ -1573,14 +1597,14 @@ void launch_bandsum(hipStream_t s, const BandArgs& b, int64_t nk, const double* 
     if (b.jobs & MEP_JOB_AHC) {
       const size_t lh = hwsum_lds_bytes(b);
       if (b.v_herm) {
-        (void)hipFuncSetAttribute((const void*)hwsum_kernel<true>,
+        (void)hipFuncSetAttribute((const void*)hwsum_kernel<true, HW_IPT>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lh);
-        hipLaunchKernelGGL(hwsum_kernel<true>, dim3(nblocks), dim3(256), lh, s, b, Tsmear, L, nk,
+        hipLaunchKernelGGL((hwsum_kernel<true, HW_IPT>), dim3(nblocks), dim3(256), lh, s, b, Tsmear, L, nk,
                            E, V, w, part);
       } else {
-        (void)hipFuncSetAttribute((const void*)hwsum_kernel<false>,
+        (void)hipFuncSetAttribute((const void*)hwsum_kernel<false, 1>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lh);
-        hipLaunchKernelGGL(hwsum_kernel<false>, dim3(nblocks), dim3(256), lh, s, b, Tsmear, L, nk,
+        hipLaunchKernelGGL((hwsum_kernel<false, 1>), dim3(nblocks), dim3(256), lh, s, b, Tsmear, L, nk,
                            E, V, w, part);
       }
     }
