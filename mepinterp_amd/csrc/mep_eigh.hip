@@ -1099,7 +1099,7 @@ __global__ __launch_bounds__(64) void eigh_vec_kernel(int n, int64_t nk,
   constexpr int RCH = 8;  // reflectors staged per chunk
   constexpr int ZS = (NMAX * LDZ > 2 * RCH * NMAX) ? NMAX * LDZ : 2 * RCH * NMAX;
   __shared__ R Zt_s[TEAMS][ZS];
-  __shared__ R cs_s[TEAMS][2 * NMAX];
+  __shared__ R cs_s[TEAMS][4 * NMAX];  // staging slots; [2 NMAX, 4 NMAX): the phases
   __shared__ int ix_s[TEAMS][NMAX];
   const int64_t cap = rot_cap(n);
   for (int64_t kb = (int64_t)blockIdx.x * TEAMS; kb < nk; kb += (int64_t)gridDim.x * TEAMS) {
@@ -1117,6 +1117,12 @@ __global__ __launch_bounds__(64) void eigh_vec_kernel(int n, int64_t nk,
     // count slot -1: eigenvalues separated, no rotations recorded; the
     // eigenvectors of the real tridiagonal come straight from T below
     const bool direct = valid && sd[5 * n] < R(0);
+    // the phases of P (one coalesced load, then LDS broadcasts)
+    R* phb = csb + 2 * NMAX;
+    if (t < n) {
+      phb[2 * t] = sd[3 * n + 2 * t];
+      phb[2 * t + 1] = sd[3 * n + 2 * t + 1];
+    }
     tri_vectors<NMAX, R>(n, t, valid, direct, k, cap, sd, rot, rcol, E, Zt, csb, ixb);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -1126,7 +1132,7 @@ __global__ __launch_bounds__(64) void eigh_vec_kernel(int n, int64_t nk,
 #pragma unroll
     for (int i = 0; i < NMAX; ++i) {
       const int ic = i < n ? i : n - 1;
-      const Z ph = Z{sd[3 * n + 2 * ic], sd[3 * n + 2 * ic + 1]};
+      const Z ph = Z{phb[2 * ic], phb[2 * ic + 1]};
       W[i] = (valid && i < n && t < n) ? zscale(ph, Zt[t * LDZ + i]) : Z{R(0), R(0)};
     }
     // tau of the reflectors into the (now free) rotation staging area
@@ -1220,12 +1226,32 @@ __global__ __launch_bounds__(64) void eigh_vec_kernel(int n, int64_t nk,
         for (int i = 0; i < NMAX; ++i) W[i] = zmul(W[i], phs);
       }
     }
+    // columns to U's column of their rank, through the team's LDS area in
+    // two halves (NMAX / 2 columns, pitch NMAX + 1), so that each store
+    // writes one contiguous column per team instead of one element per column
     const int rnk = (valid && t < n) ? (direct ? t : (int)sd[n + t]) : 0;
-    if (valid && t < n) {
-      Z* Uo = U + k * (int64_t)n * n + (int64_t)n * rnk;
+    ixb[t] = rnk;
+    Z* Wt = reinterpret_cast<Z*>(Zt);
+    constexpr int HC = NMAX / 2;
 #pragma unroll
-      for (int i = 0; i < NMAX; ++i)
-        if (i < n) Uo[i] = W[i];
+    for (int hh = 0; hh < 2; ++hh) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      if (t >= hh * HC && t < (hh + 1) * HC)
+#pragma unroll
+        for (int i = 0; i < NMAX; ++i) Wt[(t - hh * HC) * (NMAX + 1) + i] = W[i];
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      if (valid && t < n) {
+        Z* Uk_ = U + k * (int64_t)n * n + t;
+#pragma unroll
+        for (int s2 = 0; s2 < HC; ++s2) {
+          const int col = hh * HC + s2;
+          if (col < n) Uk_[(int64_t)n * ixb[col]] = Wt[s2 * (NMAX + 1) + t];
+        }
+      }
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
