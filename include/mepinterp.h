@@ -244,9 +244,12 @@ int mep_set_workspace_limit(mep_ctx *ctx, double bytes);
      MEP_VAR_EIGH  FP64 Householder reduction.  16 < n <= 32:
                    1 = one k-point per wavefront, half a row per lane (default);
                    2 = two k-points per wavefront, one row per lane.
-                   32 < n: 1 = one 256-thread workgroup per k-point (default
-                   for n > 52); 2 = one wavefront per k-point with the matrix
-                   in LDS (n <= 64; default for n <= 52).
+                   32 < n: 1 = one 256-thread workgroup per k-point with the
+                   matrix in LDS (default for 64 < n <= 88); 2 = one
+                   wavefront per k-point with the matrix in LDS (n <= 64;
+                   default for n <= 44); 3 = n / 16 wavefronts per k-point
+                   with the matrix in registers (default for 44 < n <= 64
+                   and n > 88).
      MEP_VAR_GEMM32  the FP32 Fourier GEMM of the mixed path on k lists:
                    1 = FP32 MFMA (v_mfma_f32_16x16x4_f32); 2 = split bfloat16
                    (three bf16 parts per operand, six products accumulated in

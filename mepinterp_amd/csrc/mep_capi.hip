@@ -452,7 +452,7 @@ extern "C" int mep_set_kblock(mep_ctx* c, int64_t kb) {
 extern "C" int mep_set_variant(mep_ctx* c, int32_t what, int32_t value) {
   if (!c) return fail(MEP_ERR_ARG, "mep_set_variant: null context");
   if (what == MEP_VAR_EIGH) {
-    if (value < 0 || value > 2) return fail(MEP_ERR_ARG, "mep_set_variant: MEP_VAR_EIGH takes 0, 1 or 2");
+    if (value < 0 || value > 3) return fail(MEP_ERR_ARG, "mep_set_variant: MEP_VAR_EIGH takes 0 ... 3");
     if (value != c->var_eigh) ++c->gen;
     c->var_eigh = value;
     return MEP_OK;

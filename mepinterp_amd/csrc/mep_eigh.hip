@@ -1587,6 +1587,201 @@ __global__ __launch_bounds__(64) void eigh_house_lds_kernel(int n, int64_t nk,
   }
 }
 
+// a or b by value (a conditional on class lvalues selects addresses, which
+// keeps register arrays in scratch)
+__device__ inline zd zsel(bool c, zd a, zd b) { return zd{c ? a.re : b.re, c ? a.im : b.im}; }
+
+// Householder reduction for 32 < n <= NMAX (NMAX = 64, 80, 96, 128) with the
+// matrix in registers: a workgroup of NMAX / 16 wavefronts per k-point,
+// lane 4 r + q of wave w holding row i = 16 w + r, columns j = 4 (jj + s) + q
+// (jj < NMAX / 4), where s = kc / 4 for the column kc being reduced: every
+// four columns the row arrays shift down one entry, so column kc always sits
+// at entry 0 of the lanes q = kc % 4 and every register index stays a
+// compile-time constant.  Per column: the owners write x = A(., kc) to LDS
+// with the tail norm (block sum), every thread forms v and tau, the four
+// lanes of a row sum their parts of p = tau A v (DPP), a block sum gives
+// v^H p, and the update A -= v w^H + w v^H runs over the entries that still
+// hold columns > kc - 4 (uniform skip); rows of finished waves skip it too.
+// v goes to U's column kc as it is made.  Three barriers per column (LDS
+// buffers alternate between columns).  Same outputs as the other kernels.
+template <int NMAX>
+__global__ __launch_bounds__(4 * NMAX, NMAX <= 96 ? 3 : 2) void eigh_house_rows_kernel(int n, int64_t nk,
+                                                                  const zd* __restrict__ Hk,
+                                                                  int64_t strideHk, int packed,
+                                                                  const double* __restrict__ kpts,
+                                                                  const double* __restrict__ tau,
+                                                                  zd* __restrict__ U,
+                                                                  double* __restrict__ side) {
+  constexpr int NW = NMAX / 16, NC = NMAX / 4;
+  __shared__ __attribute__((aligned(16))) zd vv[2][NMAX];
+  __shared__ __attribute__((aligned(16))) zd ww[2][NMAX];
+  __shared__ __attribute__((aligned(16))) zd sub[NMAX];
+  __shared__ __attribute__((aligned(16))) zd fph[NMAX];
+  __shared__ double red[2][2][NW];
+  const double two_pi = 6.283185307179586476925;
+  const zd zero = zd{0.0, 0.0};
+  for (int64_t k = blockIdx.x; k < nk; k += gridDim.x) {
+    int t = threadIdx.x;
+    asm volatile("" : "+v"(t));
+    const int lane = t & 63, wv = __builtin_amdgcn_readfirstlane(t >> 6);
+    const int q = lane & 3, i = 16 * wv + (lane >> 2);
+    double* sd = side + k * side_len(n);
+    zd* Uk = U + k * (int64_t)n * n;
+    __syncthreads();  // the previous k-point's reads of sub and fph are done
+    if (tau) {
+      if (t < n) {
+        zd f;
+        const double ang = two_pi * (kpts[3 * k] * tau[3 * t] + kpts[3 * k + 1] * tau[3 * t + 1] +
+                                     kpts[3 * k + 2] * tau[3 * t + 2]);
+        sin_cos(ang, &f.im, &f.re);
+        fph[t] = f;
+      }
+      __syncthreads();
+    }
+    zd row[NC];
+    {
+      const zd fi = zsel(tau && i < n, zconj(fph[i < n ? i : 0]), zd{1.0, 0.0});
+#pragma unroll
+      for (int jj = 0; jj < NC; ++jj) {
+        const int j = 4 * jj + q;
+        zd x = zero;
+        if (i < n && j < n) {
+          const int i0 = j >= i ? i : j, j0 = j >= i ? j : i;  // (i0 <= j0) element
+          const int64_t o = packed ? i0 + ((int64_t)j0 * (j0 + 1)) / 2 : i0 + (int64_t)n * j0;
+          x = Hk[k * strideHk + o];
+          if (j < i) x = zconj(x);
+          if (tau && j != i) x = zmul(x, zmul(fph[j], fi));
+          if (j == i) x.im = 0.0;
+        }
+        row[jj] = x;
+      }
+    }
+    for (int kc = 0; kc < n; ++kc) {
+      const int s = kc >> 2, m = kc & 3, b = kc & 1;
+      if (m == 0 && kc > 0) {
+        // the empty asm keeps the copies element-wise (as a block move the
+        // array would leave the registers)
+#pragma unroll
+        for (int jj = 0; jj + 1 < NC; ++jj) {
+          zd x = row[jj + 1];
+          asm volatile("" : "+v"(x.re), "+v"(x.im));
+          row[jj] = x;
+        }
+        row[NC - 1] = zero;
+      }
+      const bool own = q == m;  // this lane holds A(i, kc) in row[0]
+      const zd xo = zsel(own, row[0], zero);
+      if (own && i == kc) sd[kc] = xo.re;  // the diagonal is final here
+      if (kc == n - 2 && own && i == n - 1) sub[n - 2] = xo;
+      if (kc + 2 >= n) continue;  // uniform
+      const bool below = i > kc && i < n;
+      // x (rows > kc) to LDS, the tail norm over rows > kc + 1
+      if (own) vv[b][i] = zsel(below, xo, zero);
+      {
+        double part = (own && i > kc + 1 && i < n) ? zabs2(xo) : 0.0;
+        part = rows4_sum(row16_sum(part));
+        if (lane == 0) red[b][0][wv] = part;
+      }
+      __syncthreads();  // (1)
+      double tail = 0.0;
+#pragma unroll
+      for (int w2 = 0; w2 < NW; ++w2) tail += red[b][0][w2];
+      const zd x0 = vv[b][kc + 1];
+      const bool refl = tail > 0.0;
+      const double ax02 = zabs2(x0);
+      const double xn = sqrt(tail + ax02);
+      const double ax0 = sqrt(ax02);
+      const zd alpha = zsel(!refl, x0, zsel(ax0 > 0.0, zscale(x0, -xn * rcp_nr(ax0)), zd{-xn, 0.0}));
+      const zd v0 = zsub(x0, alpha);
+      const double vn2 = 2.0 * xn * (xn + ax0);
+      const double tk = refl ? 2.0 * rcp_nr(vn2) : 0.0;
+      auto vget = [&](int j) {  // v_j (zero at and above kc, and without a reflection)
+        const zd x = vv[b][j];
+        return zsel(!refl, zero, zsel(j == kc + 1, v0, x));
+      };
+      const zd vi = zsel(i < n, vget(i), zero);
+      const bool active = 16 * wv + 15 > kc;  // uniform: the wave has rows > kc
+      zd p = zero;
+      if (active) {
+        zd p2[2] = {zero, zero};
+#pragma unroll
+        for (int jj = 0; jj < NC; ++jj) {
+          if (jj >= NC - s) continue;  // uniform: entries past the shifted end
+          const int j = 4 * (jj + s) + q;
+          const zd a_ = row[jj], b_ = vget(j);
+          zd& c = p2[jj & 1];
+          c.re = fma(a_.re, b_.re, fma(-a_.im, b_.im, c.re));
+          c.im = fma(a_.re, b_.im, fma(a_.im, b_.re, c.im));
+        }
+        p = zadd(p2[0], p2[1]);
+      }
+      p.re += dpp_d<0xB1>(p.re);  // the four lanes of the row
+      p.im += dpp_d<0xB1>(p.im);
+      p.re += dpp_d<0x4E>(p.re);
+      p.im += dpp_d<0x4E>(p.im);
+      p = zsel(below, zscale(p, tk), zero);
+      {
+        double part = (q == 0) ? vi.re * p.re + vi.im * p.im : 0.0;
+        part = rows4_sum(row16_sum(part));
+        if (lane == 0) red[b][1][wv] = part;
+      }
+      __syncthreads();  // (2)
+      double vp = 0.0;
+#pragma unroll
+      for (int w2 = 0; w2 < NW; ++w2) vp += red[b][1][w2];
+      const double K = 0.5 * tk * vp;
+      const zd wi_ = zsub(p, zscale(vi, K));
+      if (q == 0) ww[b][i] = wi_;
+      // v to U's column kc (rows > kc), as the other kernels leave it
+      if (own && below) Uk[i + (int64_t)n * kc] = vi;
+      if (t == 0) {
+        sub[kc] = alpha;
+        sd[2 * n + kc] = tk;
+      }
+      __syncthreads();  // (3)
+      if (refl && active) {
+#pragma unroll
+        for (int jj = 0; jj < NC; ++jj) {
+          if (jj >= NC - s) continue;  // uniform
+          const int j = 4 * (jj + s) + q;
+          // A(i, j) -= v_i conj(w_j) + w_i conj(v_j)  (zero for j <= kc or i <= kc)
+          const zd vj = vget(j), wj = ww[b][j];
+          zd r_ = row[jj];
+          r_.re = fma(-vi.re, wj.re, fma(-vi.im, wj.im, fma(-wi_.re, vj.re, fma(-wi_.im, vj.im, r_.re))));
+          r_.im = fma(-vi.im, wj.re, fma(vi.re, wj.im, fma(-wi_.im, vj.re, fma(wi_.re, vj.im, r_.im))));
+          row[jj] = r_;
+        }
+      }
+    }
+    __syncthreads();  // sub complete
+    if (t == 0) {
+      sd[2 * n + n - 2] = 0.0;
+      sd[2 * n + n - 1] = 0.0;
+    }
+    // |s_i| and the unit phase u_i = s_i / |s_i| (1 for s_i = 0) of every i at
+    // once, then the running product ph_{i+1} = ph_i u_i in one lane: the same
+    // operations as one serial loop over i (ph u with u = 1 is exact)
+    if (t + 1 < n) {
+      const zd s_ = sub[t];
+      const double as = sqrt(zabs2(s_));
+      sd[n + t] = as;
+      sub[t] = (as > 0.0) ? zscale(s_, 1.0 / as) : zd{1.0, 0.0};
+    }
+    __syncthreads();
+    if (t == 0) {
+      zd ph = zd{1.0, 0.0};
+      sd[3 * n] = 1.0;
+      sd[3 * n + 1] = 0.0;
+      for (int i2 = 0; i2 + 1 < n; ++i2) {
+        ph = zmul(ph, sub[i2]);
+        sd[3 * n + 2 * (i2 + 1)] = ph.re;
+        sd[3 * n + 2 * (i2 + 1) + 1] = ph.im;
+      }
+      sd[n + n - 1] = 0.0;
+    }
+  }
+}
+
 size_t eigh_house_lds_bytes(int n) {
   return sizeof(zd) * ((size_t)n * n + 4 * 64) + sizeof(double) * 64;
 }
@@ -1998,8 +2193,27 @@ void launch_eigh_wide(hipStream_t s, int n, int64_t nk, const zd* Hk, int64_t st
   (void)hipFuncSetAttribute((const void*)eigh_ql_wide_kernel<double>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lq);
   unsigned grid = (unsigned)(nk < 4096 ? nk : 4096);
-  if (house == 0) house = n <= 52 ? 2 : 1;  // measured crossover (C3 models n = 36 ... 64)
-  if (house == 2 && n <= 64) {
+  // measured crossovers (C3 models n = 36 ... 68, C5, seeded models n = 72 ... 128):
+  // one wavefront with the matrix in LDS up to 44, registers to 64, the LDS
+  // workgroup to 88 (five or six wavefronts of registers sit unevenly on the
+  // four SIMDs), registers above
+  if (house == 0) house = n <= 44 ? 2 : n <= 64 ? 3 : n <= 88 ? 1 : 3;
+  if (house == 3) {
+    // the matrix in registers, NMAX / 16 wavefronts per k-point
+    const unsigned rgrid = (unsigned)(nk < 65536 ? nk : 65536);
+    if (n <= 64)
+      hipLaunchKernelGGL(eigh_house_rows_kernel<64>, dim3(rgrid), dim3(256), 0, s, n, nk, Hk,
+                         strideHk, packed, g.kpts, g.tau, U, side);
+    else if (n <= 80)
+      hipLaunchKernelGGL(eigh_house_rows_kernel<80>, dim3(rgrid), dim3(320), 0, s, n, nk, Hk,
+                         strideHk, packed, g.kpts, g.tau, U, side);
+    else if (n <= 96)
+      hipLaunchKernelGGL(eigh_house_rows_kernel<96>, dim3(rgrid), dim3(384), 0, s, n, nk, Hk,
+                         strideHk, packed, g.kpts, g.tau, U, side);
+    else
+      hipLaunchKernelGGL(eigh_house_rows_kernel<128>, dim3(rgrid), dim3(512), 0, s, n, nk, Hk,
+                         strideHk, packed, g.kpts, g.tau, U, side);
+  } else if (house == 2 && n <= 64) {
     const size_t ll = eigh_house_lds_bytes(n);
     (void)hipFuncSetAttribute((const void*)eigh_house_lds_kernel,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)ll);

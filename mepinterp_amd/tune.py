@@ -4,8 +4,9 @@ For a model of (n_wf, NR) at a given precision, the tuner times the full run
 of the model's own mesh
   1. for each eigensolver variant the library offers at this n_wf (the FP64
      Householder reduction, MEP_VAR_EIGH: for 16 < n_wf <= 32 one or two
-     k-points per wavefront, for 32 < n_wf <= 64 a workgroup or a wavefront
-     per k-point), at the default k-block, and keeps the faster;
+     k-points per wavefront; for 32 < n_wf <= 128 the matrix in a workgroup's
+     LDS, in one wavefront's LDS (n_wf <= 64) or in a workgroup's registers),
+     at the default k-block, and keeps the faster;
   2. then for a few k-block sizes (0 = the library's default: the largest
      block within the context's workspace limit) with that variant.
 Choices are cached in a JSON file keyed by "n_wf/NR/precision/mesh size", so a
@@ -26,8 +27,14 @@ KBLOCKS = (0, 8192, 32768, 131072)
 
 def eigh_variants(n_wf: int, precision: int) -> tuple:
     """MEP_VAR_EIGH values that select different kernels at this size."""
-    if precision == capi.MEP_PREC_FP64 and 16 < n_wf <= 64:
+    if precision != capi.MEP_PREC_FP64:
+        return (0,)
+    if 16 < n_wf <= 32:
         return (1, 2)
+    if 32 < n_wf <= 64:
+        return (1, 2, 3)
+    if 64 < n_wf <= 128:
+        return (1, 3)
     return (0,)
 
 

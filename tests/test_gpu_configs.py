@@ -178,7 +178,7 @@ def test_c2_householder_variants_match_oracle(centres):
             assert g["n_k"] == 301
             _compare(g, o, what=f"C2 eigh variant {var} centres={centres}")
         with pytest.raises(RuntimeError):
-            eng.set_variant(capi.MEP_VAR_EIGH, 3)
+            eng.set_variant(capi.MEP_VAR_EIGH, 4)
         with pytest.raises(RuntimeError):
             eng.set_variant(99, 1)
 
@@ -206,11 +206,12 @@ def test_tuner_picks_and_caches(tmp_path):
         assert s["n_k"] == 16 ** 3
 
 
-@pytest.mark.parametrize("n", [40, 60])
+@pytest.mark.parametrize("n", [40, 60, 72, 90, 100])
 def test_wide_householder_variants_match_oracle(n):
-    """Both Householder kernels for 32 < n <= 64 (a 256-thread workgroup
-    per k-point; one wavefront per k-point with the matrix in LDS) against
-    the oracle on random k-points."""
+    """Every Householder kernel for 32 < n <= 128 (the matrix in a 256-thread
+    workgroup's LDS; in one wavefront's LDS for n <= 64; in the registers of
+    n / 16 wavefronts, 64-, 80-, 96- and 128-row builds) against the oracle on
+    random k-points."""
     import oracle as O
     from mepinterp_amd import capi, host, models
     m = models.make_random(seed=90 + n, n=n, L=1)
@@ -220,8 +221,9 @@ def test_wide_householder_variants_match_oracle(n):
              T_kelvin=300.0, eta=1e-3 / AU_TO_EV, valence_bands=n // 2)
     pk = {kk: v for kk, v in p.items() if kk in O.RunParams.__init__.__code__.co_varnames}
     o = O.accumulate(oracle_model(m), O.RunParams(**pk), k)
+    from mepinterp_amd import tune
     with _engine(m) as eng:
-        for var in (1, 2):
+        for var in tune.eigh_variants(n, capi.MEP_PREC_FP64):
             eng.set_variant(capi.MEP_VAR_EIGH, var)
             g = eng.run_kpts(k, host.run_params(p))
             _compare(g, o, what=f"n={n} eigh variant {var}")
