@@ -1471,11 +1471,18 @@ __global__ __launch_bounds__(256, IPT > 1 ? 2 : 3) void hwsum_kernel(BandArgs b,
   }
 }
 
+// items per thread: two for Hermitian V with many items (C5's 256
+// frequencies: 94.5 -> 83.4 ms per 32^3); few items (the single (hw, E_F)
+// pair sums) keep one, at the higher occupancy
+static int hwsum_ipt(const BandArgs& b) {
+  return (b.v_herm && (int64_t)b.n_hw * b.n_ef >= 128) ? HW_IPT : 1;
+}
+
 size_t hwsum_lds_bytes(const BandArgs& b) {
   const size_t nv = b.v_herm ? hw_vals<true>() : hw_vals<false>();
   const size_t ha = (b.v_herm ? hw_accs<true>() : hw_accs<false>()) / 2;
   const size_t ps = 1 + (size_t)b.n_ef + nv;
-  const size_t ipt = b.v_herm ? HW_IPT : 1;
+  const size_t ipt = (size_t)hwsum_ipt(b);
   const size_t stage = (size_t)HW_CH * ps > 256 * ipt * ha ? (size_t)HW_CH * ps : 256 * ipt * ha;
   return sizeof(double) * ((size_t)b.n + (size_t)b.n_ef * b.n + 4 + stage);
 }
@@ -1596,10 +1603,15 @@ void launch_bandsum(hipStream_t s, const BandArgs& b, int64_t nk, const double* 
                        V, w, part, wide_scr, A, Om);
     if (b.jobs & MEP_JOB_AHC) {
       const size_t lh = hwsum_lds_bytes(b);
-      if (b.v_herm) {
+      if (b.v_herm && hwsum_ipt(b) == HW_IPT) {
         (void)hipFuncSetAttribute((const void*)hwsum_kernel<true, HW_IPT>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lh);
         hipLaunchKernelGGL((hwsum_kernel<true, HW_IPT>), dim3(nblocks), dim3(256), lh, s, b, Tsmear, L, nk,
+                           E, V, w, part);
+      } else if (b.v_herm) {
+        (void)hipFuncSetAttribute((const void*)hwsum_kernel<true, 1>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lh);
+        hipLaunchKernelGGL((hwsum_kernel<true, 1>), dim3(nblocks), dim3(256), lh, s, b, Tsmear, L, nk,
                            E, V, w, part);
       } else {
         (void)hipFuncSetAttribute((const void*)hwsum_kernel<false, 1>,
