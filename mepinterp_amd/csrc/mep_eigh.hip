@@ -2229,8 +2229,19 @@ void launch_eigh_wide(hipStream_t s, int n, int64_t nk, const zd* Hk, int64_t st
   int64_t qb = (nk + 31) / 32;
   unsigned qgrid = (unsigned)(qb < 65536 ? qb : 65536);
   hipLaunchKernelGGL(zero_count_kernel, dim3(1), dim3(1), 0, s, count);
-  hipLaunchKernelGGL(eigh_ql_wide_kernel<double>, dim3(qgrid), dim3(32), lq, s, n, nk, side, rot,
-                     rcol, E, info, 1, list, count);
+  // eigenvalues and the separation verdict: d and e in registers up to 48
+  // bands (more k-points in flight than LDS columns allow), else in LDS
+  const int64_t qb64 = (nk + 63) / 64;
+  const unsigned qgrid64 = (unsigned)(qb64 < 65536 ? qb64 : 65536);
+  if (n <= 40)
+    hipLaunchKernelGGL((eigh_ql_reg_kernel<40, double>), dim3(qgrid64), dim3(64), 0, s, n, nk, side,
+                       E, info, list, count);
+  else if (n <= 48)
+    hipLaunchKernelGGL((eigh_ql_reg_kernel<48, double>), dim3(qgrid64), dim3(64), 0, s, n, nk, side,
+                       E, info, list, count);
+  else
+    hipLaunchKernelGGL(eigh_ql_wide_kernel<double>, dim3(qgrid), dim3(32), lq, s, n, nk, side, rot,
+                       rcol, E, info, 1, list, count);
   hipLaunchKernelGGL(eigh_ql_wide_kernel<double>, dim3(qgrid), dim3(32), lq, s, n, nk, side, rot,
                      rcol, E, info, 2, list, count);
   const int rthreads = n <= 64 ? 64 : 128;
