@@ -390,7 +390,7 @@ def test_fused_velocity_band_sums_equal_the_two_kernel_path(n, kblock):
         assert np.array_equal(out[0][key], out[1][key]), (key, _rel(out[0][key], out[1][key]))
 
 
-@pytest.mark.parametrize("n", [24, 32, 40, 64])
+@pytest.mark.parametrize("n", [24, 32, 40, 64, 96])
 def test_near_degenerate_k_points_take_the_rotation_path(n):
     """Two uncoupled copies of one model, the second shifted by 1e-9 Hartree:
     every eigenvalue has a partner 1e-9 away, below the separation threshold,
