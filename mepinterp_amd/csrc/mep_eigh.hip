@@ -15,14 +15,23 @@
 //      P that make the subdiagonal real (T = P T_r P^H), d, e and tau in a
 //      side record.
 //   eigh_ql (one LANE per k-point, 64 k-points per wavefront): the implicit
-//      QL recurrence on T_r (EISPACK tql2 method) with d, e in LDS columns;
-//      every sweep's Givens rotations (c_i, s_i) are appended to a per-k
-//      record in HBM; eigenvalues are ranked and written in ascending order.
-//   eigh_vec (a team per k): replays the rotations on Z (LDS, lane r <-> row
-//      r), then U = Q P Z: lane j keeps column j of P Z in registers and
-//      applies the reflectors in reverse order.
+//      QL recurrence on T_r (EISPACK tql2 method).  A first pass with d, e in
+//      registers (eigh_ql_reg) gives the ascending eigenvalues and a
+//      separation verdict; only k-points with close eigenvalues run the
+//      second pass, which appends every sweep's Givens rotations (c_i, s_i)
+//      to a per-k record in HBM.
+//   eigh_vec (a team per k): the vectors of the real tridiagonal, directly
+//      from twisted factorizations for separated k-points or by replaying
+//      the rotations on Z (LDS, lane r <-> row r), then U = Q P Z: lane j
+//      keeps column j of P Z in registers and applies the reflectors in
+//      reverse order.
 //   Gauge convention shared with the oracle: each eigenvector's
 //   largest-modulus component is made real and positive.
+// 32 < n <= 128: Householder with the matrix in one wavefront's LDS, a
+// workgroup's LDS or a workgroup's registers (eigh_house_lds / _wide /
+// _rows, MEP_VAR_EIGH), the QL passes (registers up to 48 bands, else LDS
+// columns), the tridiagonal vectors (eigh_replay_wide) and the blocked
+// compact-WY back-transformation on FP64 MFMA (eigh_back_mfma).
 #include "mep_common.hpp"
 #include "mep_kernels.hpp"
 #include "mep_lane.hpp"
