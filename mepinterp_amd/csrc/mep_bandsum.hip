@@ -1359,15 +1359,12 @@ __global__ __launch_bounds__(256, IPT > 1 ? 2 : 3) void hwsum_kernel(BandArgs b,
           }
         }
         __syncthreads();
-        if (act)
-#pragma unroll 2
-          for (int j = split; j < cnt; j += S) {  // several records' LDS loads in flight
-            const double* sp = st + (int64_t)j * PS;
-            const double dE = sp[0];
-            const double* pq = sp + 1 + nef;
+        // one record (dE, the weight f of each item, the products pq) into
+        // the items' sums
+        auto consume = [&](const double dE, const double* fv, const double* pq) {
 #pragma unroll
             for (int u = 0; u < IPT; ++u) {
-              const double f = sp[1 + ef[u]];
+              const double f = fv[u];
               const zd denv = zd{dE * dE - wh2[u].re, -wh2[u].im};
               const zd deno = zmul(zd{dE, 0.0}, zd{dE - wh[u].re, -wh[u].im});
               const double av = zabs2(denv), ao = zabs2(deno);
@@ -1405,7 +1402,47 @@ __global__ __launch_bounds__(256, IPT > 1 ? 2 : 3) void hwsum_kernel(BandArgs b,
                 }
               }
             }
+        };
+        if constexpr (HERM && IPT > 1) {
+          // the m records j = split, split + S, ... taken in pairs in two
+          // register sets, so one record's LDS loads are in flight while the
+          // other is summed.  No exit between a load and its use (the
+          // compiler would sink the loads to the use): past the last record
+          // the loads repeat it, and an odd m's extra record adds with
+          // weight 0 (exactly nothing: dE != 0 in every record)
+          if (act && split < cnt) {
+            constexpr int RW = 1 + IPT + NV;
+            const int m = (cnt - 1 - split) / S + 1;
+            auto rec = [&](int i) { return st + (int64_t)(split + (i < m ? i : m - 1) * S) * PS; };
+            auto ld = [&](const double* p, double* r) {
+              r[0] = p[0];
+#pragma unroll
+              for (int u = 0; u < IPT; ++u) r[1 + u] = p[1 + ef[u]];
+#pragma unroll
+              for (int q = 0; q < NV; ++q) r[1 + IPT + q] = p[1 + nef + q];
+            };
+            double ra[RW], rb[RW];
+            ld(rec(0), ra);
+            for (int i = 0; i < m; i += 2) {
+              ld(rec(i + 1), rb);
+              consume(ra[0], ra + 1, ra + 1 + IPT);
+              ld(rec(i + 2), ra);
+              double fb[IPT];
+#pragma unroll
+              for (int u = 0; u < IPT; ++u) fb[u] = (i + 1 < m) ? rb[1 + u] : 0.0;
+              consume(rb[0], fb, rb + 1 + IPT);
+            }
           }
+        } else if (act) {
+#pragma unroll 2
+          for (int j = split; j < cnt; j += S) {  // several records' LDS loads in flight
+            const double* sp = st + (int64_t)j * PS;
+            double fv[IPT];
+#pragma unroll
+            for (int u = 0; u < IPT; ++u) fv[u] = sp[1 + ef[u]];
+            consume(sp[0], fv, sp + 1 + nef);
+          }
+        }
       }
     }
     // combine the splits of each item in two halves of the accumulators
