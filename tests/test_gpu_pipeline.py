@@ -206,11 +206,15 @@ def test_run_mesh_matches_oracle(name):
 @pytest.mark.parametrize("n,nef,nhw,mp", [(20, 1, 1, (4, 3, 3)), (32, 1, 1, (4, 3, 3)),
                                            (29, 17, 3, (4, 3, 3)), (40, 3, 4, (3, 3, 2)),
                                            (72, 1, 1, (2, 2, 2)), (96, 2, 3, (2, 2, 1)),
-                                           (128, 1, 2, (2, 1, 1))])
+                                           (128, 1, 2, (2, 1, 1)), (40, 5, 30, (2, 1, 1)),
+                                           (40, 3, 200, (2, 1, 1))])
 def test_run_mesh_large_bands_matches_oracle(n, nef, nhw, mp):
     """n_wf > 16 (32-band kernel variants), a single (hw, E_F) pair-sum path,
     more E_F points than one pass of the workgroup covers, and the wide
-    kernels for 32 < n_wf <= 128."""
+    kernels for 32 < n_wf <= 128.  The last two cases take the omega-resolved
+    kernel's two-items-per-thread path: 150 items (3 record splits, so lanes
+    of one wavefront walk different record counts, odd and even) and 600
+    items (two passes, the second with 5 splits)."""
     import oracle as O
     from mepinterp_amd import capi, models
     m = models.make_random(seed=40 + n, n=n, L=1)
