@@ -18,7 +18,8 @@
 //                 ...; wave 0 sums the Berry curvature of n0 (and finds the
 //                 Niu exclusion windows), wave 1 the local (LC) MEP terms,
 //                 waves 2-3 (single (hw, E_F)) the velo/ohc pair sums; the
-//                 band sums are wavefront shuffle reductions;
+//                 band sums are wavefront shuffle reductions (several
+//                 (hw, E_F) items: hwsum_kernel, launched after this one);
 //   itinerant     C_j = V_k B_l - V_l B_k, (j,k,l) cyclic, with
 //                 B_l(m, n0) = V_l(m, n0) r(m, n0): three complex n x n
 //                 products on FP64 MFMA (three real products per complex
@@ -587,42 +588,8 @@ __global__ __launch_bounds__(BS_THREADS, 2) void bandsum_kernel(BandArgs b, doub
       PW[wv * 24 + lane] = 0.0;
     }
     }
-    // ---- hw-resolved Hall and optical conductivity for several (hw, E_F) ---
-    if (doAHC && !pmode) {
-      // thread h owns every accumulator entry of frequency h
-      for (int h = t; h < nhw; h += BS_THREADS) {
-        const zd wh = zd{b.hw[h], b.eta};
-        const zd wh2 = zmul(wh, wh);
-        for (int x = 0; x < n; ++x)
-          for (int y = 0; y < n; ++y) {
-            if (x == y) continue;
-            const double dE = Es[x] - Es[y];
-            const zd denv = zd{dE * dE - wh2.re, -wh2.im};
-            const zd deno = zmul(zd{dE, 0.0}, zd{dE - wh.re, -wh.im});
-            const double iv = 1.0 / zabs2(denv), io = 1.0 / zabs2(deno);
-            const zd rv = zd{denv.re * iv, -denv.im * iv};
-            const zd ro = zd{deno.re * io, -deno.im * io};
-#pragma unroll
-            for (int j = 0; j < 3; ++j)
-#pragma unroll
-              for (int i = 0; i < 3; ++i) {
-                const int q = i + 3 * j;
-                const double vim = zmul(ldz(Vs + i * PL + x + LD * y), ldz(Vs + j * PL + y + LD * x)).im;
-                const zd q2o = zmul(zmul(ldz(Vs + i * PL + y + LD * x), ldz(Vs + j * PL + x + LD * y)), ro);
-                for (int ef = 0; ef < nef; ++ef) {
-                  const double dfv = wk * (fd[ef + nef * x] - fd[ef + nef * y]);
-                  double* av = acc + L.velo + 2 * 9 * (h + (int64_t)nhw * ef);
-                  double* ao = acc + L.ohc + 2 * 9 * (h + (int64_t)nhw * ef);
-                  av[2 * q] += vim * rv.re * dfv;
-                  av[2 * q + 1] += vim * rv.im * dfv;
-                  const zd q2 = zmuli(zscale(q2o, dfv));
-                  ao[2 * q] += q2.re;
-                  ao[2 * q + 1] += q2.im;
-                }
-              }
-          }
-      }
-    }
+    // (the Hall and optical conductivities of several (hw, E_F) items run in
+    // hwsum_kernel after this kernel)
 
     // ==== itinerant (IC) terms on MFMA ======================================
     // Wave item: one 16 x 16 block (tr, tc) of all three C_j when the
@@ -1614,16 +1581,52 @@ static void launch_bs(hipStream_t s, const BandArgs& b, double Tsmear, const Acc
                      f ? f->strideHk : (int64_t)0, f ? f->U : nullptr);
 }
 
+// n <= 32: the Hall and optical conductivities of several (hw, E_F) items
+// (velo_ahc_tens / kubo_ohc_tens over a frequency list or several Fermi
+// levels) run in hwsum_kernel, every item on its own threads, reading V
+// from HBM; the single-item sums stay in the band-sum kernel's pair phase
+static bool hw_many(const BandArgs& b) {
+  return (b.jobs & MEP_JOB_AHC) && (int64_t)b.n_hw * b.n_ef > 1;
+}
+
 size_t bandsum_lds_bytes(const BandArgs& b) {
   AccLayout L = acc_layout(b);
   if (b.n > 32) {
     const size_t a = bandsum_wide_lds_bytes(b), h = hwsum_lds_bytes(b);
     return a > h ? a : h;
   }
-  return b.n <= 16 ? BsLds<16>::bytes(b.n_ef, L.total) : BsLds<32>::bytes(b.n_ef, L.total);
+  const size_t a = b.n <= 16 ? BsLds<16>::bytes(b.n_ef, L.total) : BsLds<32>::bytes(b.n_ef, L.total);
+  const size_t h = hw_many(b) ? hwsum_lds_bytes(b) : 0;
+  return a > h ? a : h;
 }
 
-bool bandsum_can_fuse(const BandArgs& b) { return b.n > 16 && b.n <= 32 && b.v_herm && !b.pos_op; }
+// the velocities formed inside the band-sum kernel never reach HBM, which
+// hwsum_kernel reads
+bool bandsum_can_fuse(const BandArgs& b) {
+  return b.n > 16 && b.n <= 32 && b.v_herm && !b.pos_op && !hw_many(b);
+}
+
+static void launch_hwsum(hipStream_t s, const BandArgs& b, double Tsmear, const AccLayout& L,
+                         int64_t nk, const double* E, const zd* V, const double* w, int nblocks,
+                         double* part) {
+  const size_t lh = hwsum_lds_bytes(b);
+  if (b.v_herm && hwsum_ipt(b) == HW_IPT) {
+    (void)hipFuncSetAttribute((const void*)hwsum_kernel<true, HW_IPT>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lh);
+    hipLaunchKernelGGL((hwsum_kernel<true, HW_IPT>), dim3(nblocks), dim3(256), lh, s, b, Tsmear, L,
+                       nk, E, V, w, part);
+  } else if (b.v_herm) {
+    (void)hipFuncSetAttribute((const void*)hwsum_kernel<true, 1>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lh);
+    hipLaunchKernelGGL((hwsum_kernel<true, 1>), dim3(nblocks), dim3(256), lh, s, b, Tsmear, L, nk,
+                       E, V, w, part);
+  } else {
+    (void)hipFuncSetAttribute((const void*)hwsum_kernel<false, 1>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lh);
+    hipLaunchKernelGGL((hwsum_kernel<false, 1>), dim3(nblocks), dim3(256), lh, s, b, Tsmear, L, nk,
+                       E, V, w, part);
+  }
+}
 
 void launch_bandsum(hipStream_t s, const BandArgs& b, int64_t nk, const double* E, const zd* V,
                     const zd* A, const zd* Om, const double* w, int nblocks, double* part,
@@ -1638,31 +1641,15 @@ void launch_bandsum(hipStream_t s, const BandArgs& b, int64_t nk, const double* 
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     hipLaunchKernelGGL(bandsum_wide_kernel, dim3(nblocks), dim3(256), lds, s, b, Tsmear, L, nk, E,
                        V, w, part, wide_scr, A, Om);
-    if (b.jobs & MEP_JOB_AHC) {
-      const size_t lh = hwsum_lds_bytes(b);
-      if (b.v_herm && hwsum_ipt(b) == HW_IPT) {
-        (void)hipFuncSetAttribute((const void*)hwsum_kernel<true, HW_IPT>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lh);
-        hipLaunchKernelGGL((hwsum_kernel<true, HW_IPT>), dim3(nblocks), dim3(256), lh, s, b, Tsmear, L, nk,
-                           E, V, w, part);
-      } else if (b.v_herm) {
-        (void)hipFuncSetAttribute((const void*)hwsum_kernel<true, 1>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lh);
-        hipLaunchKernelGGL((hwsum_kernel<true, 1>), dim3(nblocks), dim3(256), lh, s, b, Tsmear, L, nk,
-                           E, V, w, part);
-      } else {
-        (void)hipFuncSetAttribute((const void*)hwsum_kernel<false, 1>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lh);
-        hipLaunchKernelGGL((hwsum_kernel<false, 1>), dim3(nblocks), dim3(256), lh, s, b, Tsmear, L, nk,
-                           E, V, w, part);
-      }
-    }
-  } else if (b.n <= 16) {
-    launch_bs<16, false>(s, b, Tsmear, L, nk, E, V, A, Om, w, nblocks, part, nullptr);
-  } else if (fuse) {
-    launch_bs<32, true>(s, b, Tsmear, L, nk, E, V, A, Om, w, nblocks, part, fuse);
+    if (b.jobs & MEP_JOB_AHC) launch_hwsum(s, b, Tsmear, L, nk, E, V, w, nblocks, part);
   } else {
-    launch_bs<32, false>(s, b, Tsmear, L, nk, E, V, A, Om, w, nblocks, part, nullptr);
+    if (b.n <= 16)
+      launch_bs<16, false>(s, b, Tsmear, L, nk, E, V, A, Om, w, nblocks, part, nullptr);
+    else if (fuse)
+      launch_bs<32, true>(s, b, Tsmear, L, nk, E, V, A, Om, w, nblocks, part, fuse);
+    else
+      launch_bs<32, false>(s, b, Tsmear, L, nk, E, V, A, Om, w, nblocks, part, nullptr);
+    if (hw_many(b)) launch_hwsum(s, b, Tsmear, L, nk, E, V, w, nblocks, part);
   }
 }
 

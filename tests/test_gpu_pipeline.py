@@ -207,14 +207,16 @@ def test_run_mesh_matches_oracle(name):
                                            (29, 17, 3, (4, 3, 3)), (40, 3, 4, (3, 3, 2)),
                                            (72, 1, 1, (2, 2, 2)), (96, 2, 3, (2, 2, 1)),
                                            (128, 1, 2, (2, 1, 1)), (40, 5, 30, (2, 1, 1)),
-                                           (40, 3, 200, (2, 1, 1))])
+                                           (40, 3, 200, (2, 1, 1)), (24, 2, 70, (3, 2, 2)),
+                                           (12, 4, 1, (3, 3, 2))])
 def test_run_mesh_large_bands_matches_oracle(n, nef, nhw, mp):
     """n_wf > 16 (32-band kernel variants), a single (hw, E_F) pair-sum path,
     more E_F points than one pass of the workgroup covers, and the wide
-    kernels for 32 < n_wf <= 128.  The last two cases take the omega-resolved
-    kernel's two-items-per-thread path: 150 items (3 record splits, so lanes
-    of one wavefront walk different record counts, odd and even) and 600
-    items (two passes, the second with 5 splits)."""
+    kernels for 32 < n_wf <= 128.  The omega-resolved kernel's
+    two-items-per-thread path: 150 items (3 record splits, so lanes of one
+    wavefront walk different record counts, odd and even), 600 items (two
+    passes, the second with 5 splits), and 140 items at n_wf = 24; four
+    Fermi levels at n_wf = 12 take its one-item path."""
     import oracle as O
     from mepinterp_amd import capi, models
     m = models.make_random(seed=40 + n, n=n, L=1)
@@ -381,8 +383,10 @@ def test_fused_velocity_band_sums_equal_the_two_kernel_path(n, kblock):
     from mepinterp_amd import capi, models
     m = models.make_random(seed=40 + n, n=n, L=1, tb=False)
     mp = (6, 5, 4)
-    p = _params(capi.MEP_JOB_AHC | capi.MEP_JOB_KUBO | capi.MEP_JOB_MEP, n_eF=3, eF_min=-0.02,
-                eF_max=0.02, n_hw=4, hw_min=0.0, hw_max=0.3, T_kelvin=300.0,
+    # one (hw, E_F) item: several go to the omega-resolved kernel, which
+    # reads V from HBM, so neither run would fuse
+    p = _params(capi.MEP_JOB_AHC | capi.MEP_JOB_KUBO | capi.MEP_JOB_MEP, n_eF=1, eF_min=0.01,
+                eF_max=0.01, n_hw=1, hw_min=0.1, hw_max=0.1, T_kelvin=300.0,
                 eta=0.1 / AU_TO_EV, valence_bands=n // 2)
     out = []
     for separate in (False, True):
