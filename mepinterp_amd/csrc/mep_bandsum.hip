@@ -85,7 +85,8 @@ struct BsLds {
 // convention.
 template <int NMAX, bool FUSED>
 __global__ __launch_bounds__(BS_THREADS, 2) void bandsum_kernel(BandArgs b, double Tsmear,
-                                                                AccLayout L, int64_t nk,
+                                                                AccLayout L, int64_t gap,
+                                                                int64_t nk,
                                                                 const double* __restrict__ E,
                                                                 const zd* __restrict__ V,
                                                                 const zd* __restrict__ A,
@@ -846,8 +847,10 @@ __global__ __launch_bounds__(BS_THREADS, 2) void bandsum_kernel(BandArgs b, doub
     }
   }
   __syncthreads();
-  double* out = part + (int64_t)blockIdx.x * L.total;
-  for (int64_t c = t; c < L.total; c += BS_THREADS) out[c] += acc[c];
+  // the record in HBM holds the velo / ohc block (gap entries from L.velo)
+  // that this layout leaves out
+  double* out = part + (int64_t)blockIdx.x * (L.total + gap);
+  for (int64_t c = t; c < L.total; c += BS_THREADS) out[c < L.velo ? c : c + gap] += acc[c];
 }
 
 // ===========================================================================
@@ -1569,18 +1572,6 @@ __global__ __launch_bounds__(64 * RP_WAVES) void reduce_parts_kernel(int nblocks
   }
 }
 
-template <int NMAX, bool FUSED>
-static void launch_bs(hipStream_t s, const BandArgs& b, double Tsmear, const AccLayout& L,
-                      int64_t nk, const double* E, const zd* V, const zd* A, const zd* Om,
-                      const double* w, int nblocks, double* part, const VeloFuse* f) {
-  size_t lds = BsLds<NMAX>::bytes(b.n_ef, L.total);
-  (void)hipFuncSetAttribute((const void*)bandsum_kernel<NMAX, FUSED>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  hipLaunchKernelGGL((bandsum_kernel<NMAX, FUSED>), dim3(nblocks), dim3(BS_THREADS), lds, s, b,
-                     Tsmear, L, nk, E, V, A, Om, w, part, f ? f->Hk : nullptr,
-                     f ? f->strideHk : (int64_t)0, f ? f->U : nullptr);
-}
-
 // n <= 32: the Hall and optical conductivities of several (hw, E_F) items
 // (velo_ahc_tens / kubo_ohc_tens over a frequency list or several Fermi
 // levels) run in hwsum_kernel, every item on its own threads, reading V
@@ -1589,13 +1580,45 @@ static bool hw_many(const BandArgs& b) {
   return (b.jobs & MEP_JOB_AHC) && (int64_t)b.n_hw * b.n_ef > 1;
 }
 
+// The n <= 32 kernel's accumulators in LDS: the record's layout without the
+// velo / ohc block when hwsum_kernel fills it (then N_hw does not bound the
+// LDS); *gap is the length of the block left out
+static AccLayout lds_layout(const BandArgs& b, const AccLayout& L, int64_t* gap) {
+  AccLayout S = L;
+  *gap = 0;
+  if (hw_many(b)) {
+    *gap = L.kic - L.velo;
+    S.ohc = L.velo;
+    S.kic -= *gap; S.klc -= *gap; S.kcs -= *gap;
+    S.nic -= *gap; S.nlc -= *gap; S.ncs -= *gap;
+    S.total -= *gap;
+  }
+  return S;
+}
+
+template <int NMAX, bool FUSED>
+static void launch_bs(hipStream_t s, const BandArgs& b, double Tsmear, const AccLayout& L,
+                      int64_t nk, const double* E, const zd* V, const zd* A, const zd* Om,
+                      const double* w, int nblocks, double* part, const VeloFuse* f) {
+  int64_t gap;
+  const AccLayout S = lds_layout(b, L, &gap);
+  size_t lds = BsLds<NMAX>::bytes(b.n_ef, S.total);
+  (void)hipFuncSetAttribute((const void*)bandsum_kernel<NMAX, FUSED>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipLaunchKernelGGL((bandsum_kernel<NMAX, FUSED>), dim3(nblocks), dim3(BS_THREADS), lds, s, b,
+                     Tsmear, S, gap, nk, E, V, A, Om, w, part, f ? f->Hk : nullptr,
+                     f ? f->strideHk : (int64_t)0, f ? f->U : nullptr);
+}
+
 size_t bandsum_lds_bytes(const BandArgs& b) {
   AccLayout L = acc_layout(b);
   if (b.n > 32) {
     const size_t a = bandsum_wide_lds_bytes(b), h = hwsum_lds_bytes(b);
     return a > h ? a : h;
   }
-  const size_t a = b.n <= 16 ? BsLds<16>::bytes(b.n_ef, L.total) : BsLds<32>::bytes(b.n_ef, L.total);
+  int64_t gap;
+  const AccLayout S = lds_layout(b, L, &gap);
+  const size_t a = b.n <= 16 ? BsLds<16>::bytes(b.n_ef, S.total) : BsLds<32>::bytes(b.n_ef, S.total);
   const size_t h = hw_many(b) ? hwsum_lds_bytes(b) : 0;
   return a > h ? a : h;
 }

@@ -208,7 +208,7 @@ def test_run_mesh_matches_oracle(name):
                                            (72, 1, 1, (2, 2, 2)), (96, 2, 3, (2, 2, 1)),
                                            (128, 1, 2, (2, 1, 1)), (40, 5, 30, (2, 1, 1)),
                                            (40, 3, 200, (2, 1, 1)), (24, 2, 70, (3, 2, 2)),
-                                           (12, 4, 1, (3, 3, 2))])
+                                           (12, 4, 1, (3, 3, 2)), (20, 1, 2000, (2, 1, 1))])
 def test_run_mesh_large_bands_matches_oracle(n, nef, nhw, mp):
     """n_wf > 16 (32-band kernel variants), a single (hw, E_F) pair-sum path,
     more E_F points than one pass of the workgroup covers, and the wide
@@ -216,7 +216,8 @@ def test_run_mesh_large_bands_matches_oracle(n, nef, nhw, mp):
     two-items-per-thread path: 150 items (3 record splits, so lanes of one
     wavefront walk different record counts, odd and even), 600 items (two
     passes, the second with 5 splits), and 140 items at n_wf = 24; four
-    Fermi levels at n_wf = 12 take its one-item path."""
+    Fermi levels at n_wf = 12 take its one-item path; 2,000 frequencies at
+    n_wf = 20, whose accumulators (576 KB per workgroup) live in HBM."""
     import oracle as O
     from mepinterp_amd import capi, models
     m = models.make_random(seed=40 + n, n=n, L=1)
