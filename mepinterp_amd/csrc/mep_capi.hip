@@ -723,8 +723,8 @@ static int band_plan(mep_ctx* c, const mep_run_params* p, int64_t kblock, BandSe
   b.hw = nullptr;
   b.fd_in = nullptr;
   if (bandsum_lds_bytes(b) > 160 * 1024)
-    return fail(MEP_ERR_ARG, "band-sum accumulators exceed the 160 KiB LDS of one workgroup "
-                             "(reduce N_hw * N_eF)");
+    return fail(MEP_ERR_ARG, "band-sum tables exceed the 160 KiB LDS of one workgroup "
+                             "(reduce N_eF)");
   bs.L = acc_layout(b);
   bs.ext = (p->jobs & (MEP_JOB_OPT | MEP_JOB_GYRO)) != 0;
   bs.X = ext_layout(b);
