@@ -9,7 +9,9 @@
 // One k-point at a time per 256-thread workgroup; every output element is
 // owned by one thread, which adds its per-k value to the workgroup's record
 // (fixed order, deterministic).  For n <= 32 V is staged in LDS (planar,
-// padded); above that it is read from L2 / HBM.  A and Omega are read in the
+// padded) where it fits beside the occupation tables; otherwise it is read
+// from L2 / HBM.  photo_2nd_cond's m sum is a product per (b, c) shared by
+// every (a, hw, E_F) item when V is in LDS.  A and Omega are read in the
 // reference layout (position operator only).  The occupations, their
 // derivative and the per-band curvature sit in dynamic LDS, sized by n and
 // n_eF (ext_lds_bytes).
@@ -41,9 +43,13 @@ __device__ inline int levi3(int a, int b, int c) {
 }
 
 // dynamic LDS: [V planes if LDSV] | E (128) | omega (3 x 128 complex) |
-// fd (n_ef x n) | fd_stat_deriv (n_ef x n)
-__host__ __device__ inline size_t ext_lds_doubles(int n, int nef, bool ldsv) {
-  return (ldsv ? 2 * 3 * (size_t)XS_PL : 0) + 128 + 2 * 3 * 128 + 2 * (size_t)nef * n;
+// fd (n_ef x n) | fd_stat_deriv (n_ef x n) | [if LDSV: photo's W and P
+// (XS_NMAX^2 complex each) and split sums (XS_THREADS complex), with
+// MEP_JOB_OPT]
+__host__ __device__ inline size_t ext_lds_doubles(int n, int nef, bool ldsv, bool photo) {
+  return (ldsv ? 2 * 3 * (size_t)XS_PL : 0) +
+         (ldsv && photo ? 2 * 2 * XS_NMAX * XS_NMAX + 2 * XS_THREADS : 0) + 128 + 2 * 3 * 128 +
+         2 * (size_t)nef * n;
 }
 
 template <bool LDSV>
@@ -156,31 +162,94 @@ __global__ __launch_bounds__(XS_THREADS) void extsum_kernel(BandArgs b, double T
         rec[X.opt_a + o + 1] += wk * herm.im;
       }
     // ---- photo_2nd_cond: item (a, b, c, hw, ef) ------------------------------
-    if (doOpt)
-      for (int it = t; it < 27 * nhw * nef; it += XS_THREADS) {
-        const int q = it % 27, h = (it / 27) % nhw, ef = it / (27 * nhw);
-        const int a = q % 3, bb = (q / 3) % 3, c = q / 9;
-        const double hwv = b.hw[h];
-        zd acc = zd{0.0, 0.0};
-        for (int nn = 0; nn < n; ++nn)
-          for (int m = 0; m < n; ++m) {
-            const zd dEnm = zd{Es[nn] - Es[m], -eta};
-            for (int l = 0; l < n; ++l) {
-              if (l == nn) continue;
-              const double df = fd[ef + nef * l] - fd[ef + nef * nn];
-              const zd vvv = zmul(Vat(a, nn, l), zmul(Vat(bb, l, m), Vat(c, m, nn)));
-              zd g = zd{0.0, 0.0};
-              for (int om = -1; om <= 1; om += 2) {
-                const zd dEhw = zd{Es[nn] - Es[l] + om * hwv, -2.0 * eta};
-                g = zadd(g, zdiv(vvv, zmul(dEnm, dEhw)));
+    // sum_{n, m, l != n} (f_l - f_n) / hw^2 V_a(n,l) V_b(l,m) V_c(m,n)
+    //   / (E_n - E_m - i eta) sum_{+-} 1 / (E_n - E_l +- hw - 2 i eta)
+    if (doOpt) {
+      if constexpr (LDSV) {
+        // the m sum does not depend on a, hw or E_F: per (b, c) the product
+        // P(l, n) = sum_m V_b(l, m) W_c(m, n), W_c(m, n) = V_c(m, n) /
+        // (E_n - E_m - i eta), in LDS; then each item (a, hw, E_F) is the
+        // O(n^2) contraction with V_a(n, l) (n^3 per (b, c) instead of per
+        // item).  Items are split over the threads like the omega-resolved
+        // sums: thread (slot, split), splits combined in a fixed order.
+        zd* Wm = reinterpret_cast<zd*>(fd + 2 * (int64_t)nef * n);  // [XS_NMAX^2]
+        zd* Pm = Wm + XS_NMAX * XS_NMAX;                              // [XS_NMAX^2]
+        zd* red = Pm + XS_NMAX * XS_NMAX;                             // [XS_THREADS]
+        const int NI = 3 * nhw * nef;
+        for (int c = 0; c < 3; ++c) {
+          __syncthreads();  // the previous (b, c)'s readers of W are done
+          for (int e = t; e < n2; e += XS_THREADS) {
+            const int m = e % n, nn = e / n;
+            const double x = Es[nn] - Es[m], iv = 1.0 / (x * x + eta * eta);
+            Wm[e] = zmul(Vat(c, m, nn), zd{x * iv, eta * iv});
+          }
+          for (int bb = 0; bb < 3; ++bb) {
+            __syncthreads();  // W written; the previous P's readers are done
+            for (int e = t; e < n2; e += XS_THREADS) {
+              const int l = e % n, nn = e / n;
+              zd sp = zd{0.0, 0.0};
+              for (int m = 0; m < n; ++m) sp = zadd(sp, zmul(Vat(bb, l, m), Wm[m + n * nn]));
+              Pm[e] = sp;
+            }
+            __syncthreads();
+            for (int g0 = 0; g0 < NI; g0 += XS_THREADS) {
+              const int IW = (NI - g0 < XS_THREADS) ? NI - g0 : XS_THREADS;
+              const int S = XS_THREADS / IW, slot = t % IW, split = t / IW;
+              const int it = g0 + slot, a = it % 3, h = (it / 3) % nhw, ef = it / (3 * nhw);
+              const double hwv = b.hw[h];
+              zd acc = zd{0.0, 0.0};
+              if (split < S)
+                for (int pp = split; pp < n2; pp += S) {
+                  const int l = pp % n, nn = pp / n;
+                  if (l == nn) continue;
+                  const double df = fd[ef + nef * l] - fd[ef + nef * nn];
+                  const double x = Es[nn] - Es[l];
+                  const double xp = x + hwv, xm = x - hwv, e2 = 4.0 * eta * eta;
+                  const double ip = 1.0 / (xp * xp + e2), im = 1.0 / (xm * xm + e2);
+                  const zd sh = zd{xp * ip + xm * im, 2.0 * eta * (ip + im)};
+                  acc = zadd(acc, zscale(zmul(zmul(Vat(a, nn, l), Pm[pp]), sh), df));
+                }
+              red[t] = acc;
+              __syncthreads();
+              if (split == 0) {
+                zd sum = red[slot];
+                for (int u = 1; u < S; ++u) sum = zadd(sum, red[slot + IW * u]);
+                const int q = a + 3 * bb + 9 * c;
+                const int64_t o = 2 * (q + 27 * (h + (int64_t)nhw * ef));
+                rec[X.photo + o] += wk * (sum.re / (hwv * hwv));
+                rec[X.photo + o + 1] += wk * (sum.im / (hwv * hwv));
               }
-              acc = zadd(acc, zscale(g, df / (hwv * hwv)));
+              __syncthreads();  // red is free again
             }
           }
-        const int64_t o = 2 * (q + 27 * (h + (int64_t)nhw * ef));
-        rec[X.photo + o] += wk * acc.re;
-        rec[X.photo + o + 1] += wk * acc.im;
+        }
+      } else {
+        for (int it = t; it < 27 * nhw * nef; it += XS_THREADS) {
+          const int q = it % 27, h = (it / 27) % nhw, ef = it / (27 * nhw);
+          const int a = q % 3, bb = (q / 3) % 3, c = q / 9;
+          const double hwv = b.hw[h];
+          zd acc = zd{0.0, 0.0};
+          for (int nn = 0; nn < n; ++nn)
+            for (int m = 0; m < n; ++m) {
+              const zd dEnm = zd{Es[nn] - Es[m], -eta};
+              for (int l = 0; l < n; ++l) {
+                if (l == nn) continue;
+                const double df = fd[ef + nef * l] - fd[ef + nef * nn];
+                const zd vvv = zmul(Vat(a, nn, l), zmul(Vat(bb, l, m), Vat(c, m, nn)));
+                zd g = zd{0.0, 0.0};
+                for (int om = -1; om <= 1; om += 2) {
+                  const zd dEhw = zd{Es[nn] - Es[l] + om * hwv, -2.0 * eta};
+                  g = zadd(g, zdiv(vvv, zmul(dEnm, dEhw)));
+                }
+                acc = zadd(acc, zscale(g, df / (hwv * hwv)));
+              }
+            }
+          const int64_t o = 2 * (q + 27 * (h + (int64_t)nhw * ef));
+          rec[X.photo + o] += wk * acc.re;
+          rec[X.photo + o + 1] += wk * acc.im;
+        }
       }
+    }
     // ---- gyrotropic C, D: item (a, b, ef); Dw: item (a, b, hw, ef) -----------
     if (doGyro) {
       for (int it = t; it < 9 * nef; it += XS_THREADS) {
@@ -225,8 +294,15 @@ __global__ __launch_bounds__(XS_THREADS) void extsum_kernel(BandArgs b, double T
   }
 }
 
+// V and photo's tables in LDS for n <= 32 where they fit beside the
+// occupation tables; otherwise V is read from L2 / HBM
+static bool ext_ldsv(const BandArgs& b) {
+  const bool photo = (b.jobs & MEP_JOB_OPT) != 0;
+  return b.n <= XS_NMAX && sizeof(double) * ext_lds_doubles(b.n, b.n_ef, true, photo) <= 160 * 1024;
+}
+
 size_t ext_lds_bytes(const BandArgs& b) {
-  return sizeof(double) * ext_lds_doubles(b.n, b.n_ef, b.n <= XS_NMAX);
+  return sizeof(double) * ext_lds_doubles(b.n, b.n_ef, ext_ldsv(b), (b.jobs & MEP_JOB_OPT) != 0);
 }
 
 void launch_extsum(hipStream_t s, const BandArgs& b, int64_t nk, const double* E, const zd* V,
@@ -234,7 +310,7 @@ void launch_extsum(hipStream_t s, const BandArgs& b, int64_t nk, const double* E
   ExtLayout X = ext_layout(b);
   const double kB = 8.6173303 * 1e-5 / 27.211385;  // kBoltz_Eh_K (constants.f90:22)
   const size_t lds = ext_lds_bytes(b);
-  if (b.n <= XS_NMAX) {
+  if (ext_ldsv(b)) {
     (void)hipFuncSetAttribute((const void*)extsum_kernel<true>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     hipLaunchKernelGGL(extsum_kernel<true>, dim3(nblocks), dim3(XS_THREADS), lds, s, b,
