@@ -1280,7 +1280,9 @@ __global__ __launch_bounds__(256, IPT > 1 ? 2 : 3) void hwsum_kernel(BandArgs b,
         __syncthreads();  // fd ready; the previous chunk consumed
         const int pidx = c0 + t;
         const int x = pidx % n, y = pidx / n;
-        bool keep = pidx < n2 && x != y;
+        // Hermitian V: one record per unordered pair x < y, summed for both
+        // orders below; general V: every ordered pair
+        bool keep = pidx < n2 && (HERM ? x < y : x != y);
         if (keep) {
           bool any = false;
           for (int e_ = 0; e_ < nef; ++e_) any |= fd[e_ + nef * x] != fd[e_ + nef * y];
@@ -1335,30 +1337,46 @@ __global__ __launch_bounds__(256, IPT > 1 ? 2 : 3) void hwsum_kernel(BandArgs b,
 #pragma unroll
             for (int u = 0; u < IPT; ++u) {
               const double f = fv[u];
-              const zd denv = zd{dE * dE - wh2[u].re, -wh2[u].im};
-              const zd deno = zmul(zd{dE, 0.0}, zd{dE - wh[u].re, -wh[u].im});
-              const double av = zabs2(denv), ao = zabs2(deno);
-              // one reciprocal for both denominators; f = 0 records (another
-              // E_F differs) contribute exactly zero
-              const double fi = f * rcp_nr(av * ao);
-              const double iv = fi * ao, io = fi * av;
-              const zd rv = zd{denv.re * iv, -denv.im * iv};
-              const zd ro = zd{deno.re * io, -deno.im * io};
               if constexpr (HERM) {
+                // the record of x < y stands for (x, y) and (y, x): (y, x) has
+                // -dE, -f and conj P, so its velo term equals (x, y)'s and its
+                // ohc weight is ro2 = -f / (dE (dE + w)).  One reciprocal for
+                // the three denominators; f = 0 records (another E_F differs)
+                // contribute exactly zero.
+                const zd denv = zd{dE * dE - wh2[u].re, -wh2[u].im};
+                const double di = dE * wh[u].im;
+                const zd deno = zd{dE * (dE - wh[u].re), -di};  // dE (dE - w)
+                const zd den2 = zd{dE * (dE + wh[u].re), di};   // dE (dE + w)
+                const double av = zabs2(denv), ao = zabs2(deno), a2 = zabs2(den2);
+                const double o2 = ao * a2;
+                const double fi = f * rcp_nr(av * o2);
+                const double iv = 2.0 * fi * o2, io = fi * av * a2, i2 = fi * av * ao;
+                const zd rv = zd{denv.re * iv, -denv.im * iv};  // both orders: 2 f / denv
+                const zd ro = zd{deno.re * io, -deno.im * io};  // f / deno
+                const zd r2 = zd{-den2.re * i2, den2.im * i2};  // -f / den2
+                const zd sp = zadd(ro, r2), sm = zsub(ro, r2);
                 // acc: [0..5] rv Im P_c, [6..11] Re(ro) |x_a|^2 then Im(ro) |x_a|^2,
-                // [12..17] Re(ro) P_c, [18..23] Im(ro) P_c
+                // [12..17] Re(ro) P_c, [18..23] Im(ro) P_c, over both orders
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
                   acc[u][2 * c] += pq[c] * rv.re;
                   acc[u][2 * c + 1] += pq[c] * rv.im;
-                  acc[u][6 + c] += ro.re * pq[9 + c];
-                  acc[u][9 + c] += ro.im * pq[9 + c];
-                  acc[u][12 + 2 * c] += ro.re * pq[3 + 2 * c];
-                  acc[u][13 + 2 * c] += ro.re * pq[4 + 2 * c];
-                  acc[u][18 + 2 * c] += ro.im * pq[3 + 2 * c];
-                  acc[u][19 + 2 * c] += ro.im * pq[4 + 2 * c];
+                  acc[u][6 + c] += sp.re * pq[9 + c];
+                  acc[u][9 + c] += sp.im * pq[9 + c];
+                  acc[u][12 + 2 * c] += sp.re * pq[3 + 2 * c];
+                  acc[u][13 + 2 * c] += sm.re * pq[4 + 2 * c];
+                  acc[u][18 + 2 * c] += sp.im * pq[3 + 2 * c];
+                  acc[u][19 + 2 * c] += sm.im * pq[4 + 2 * c];
                 }
               } else {
+                const zd denv = zd{dE * dE - wh2[u].re, -wh2[u].im};
+                const zd deno = zmul(zd{dE, 0.0}, zd{dE - wh[u].re, -wh[u].im});
+                const double av = zabs2(denv), ao = zabs2(deno);
+                // one reciprocal for both denominators
+                const double fi = f * rcp_nr(av * ao);
+                const double iv = fi * ao, io = fi * av;
+                const zd rv = zd{denv.re * iv, -denv.im * iv};
+                const zd ro = zd{deno.re * io, -deno.im * io};
                 // acc: [0..17] velo_q (re, im), [18..35] ohc_q (re, im)
 #pragma unroll
                 for (int q = 0; q < 9; ++q) {
