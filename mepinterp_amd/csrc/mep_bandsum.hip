@@ -506,7 +506,10 @@ __global__ __launch_bounds__(BS_THREADS, 2) void bandsum_kernel(BandArgs b, doub
           const double dfv = f0 - fd[nn];
           const zd denv = zd{dE * dE - w2.re, -w2.im};
           const zd deno = zmul(zd{dE, 0.0}, zd{dE - wq.re, -wq.im});
-          const double iv = 1.0 / zabs2(denv), io = 1.0 / zabs2(deno);
+          // one reciprocal for both denominators
+          const double av = zabs2(denv), ao = zabs2(deno);
+          const double ia = rcp_nr(av * ao);
+          const double iv = ia * ao, io = ia * av;
           // selects, not a branch, drop n == n0 (where 1/deno is not finite)
           const bool off = nn != n0;
           const zd rv = off ? zd{denv.re * iv * dfv, -denv.im * iv * dfv} : zd{0.0, 0.0};
