@@ -259,15 +259,30 @@ __global__ __launch_bounds__(256, 2) void velo_mfma_kernel(int n, int64_t nk,
 typedef double velo_v4d __attribute__((ext_vector_type(4)));
 constexpr int VW_CH = 8;
 
+// lower-triangle tile p (row-major: (0,0), (1,0), (1,1), (2,0), ...) of an
+// edge of tiles: V_a = U^H T_a is Hermitian, so the second product forms
+// the tiles on and below the diagonal and writes each off-diagonal one
+// twice (V(c, r) = conj V(r, c))
+__device__ inline void lower_tile(int p, int* tr, int* tc) {
+  int r = (int)((sqrtf(8.0f * (float)p + 1.0f) - 1.0f) * 0.5f);
+  if ((r + 1) * (r + 2) / 2 <= p) ++r;
+  if (r * (r + 1) / 2 > p) --r;
+  *tr = r;
+  *tc = p - r * (r + 1) / 2;
+}
+
 template <int PASS>
 __device__ inline void velo_wide_pass(int n, int wv, int lane, const zd* __restrict__ X, int packed,
                                       const TBPhase& tb, const zd* __restrict__ Uk,
                                       zd* __restrict__ T, zd* __restrict__ Vk) {
   const int lr = lane & 15, kq = lane >> 4;
-  const int n2 = n * n, nt = (n + 15) / 16, ntile = nt * nt;
-  for (int item = wv; item < 3 * ntile; item += 4) {
-    const int a = item / ntile, tile = item % ntile;
-    const int r0 = 16 * (tile / nt), c0 = 16 * (tile % nt);
+  const int n2 = n * n, nt = (n + 15) / 16;
+  const int nit = PASS == 0 ? nt * nt : nt * (nt + 1) / 2;  // tiles per component
+  for (int item = wv; item < 3 * nit; item += 4) {
+    const int a = item / nit, ti = item % nit;
+    int tr = ti / nt, tc = ti % nt;
+    if (PASS == 1) lower_tile(ti, &tr, &tc);
+    const int r0 = 16 * tr, c0 = 16 * tc;
     const int ar = r0 + lr, bc = c0 + lr;
     const zd* Ta = T + (int64_t)a * n2;
     velo_v4d p1 = {0, 0, 0, 0}, p2 = {0, 0, 0, 0}, p3 = {0, 0, 0, 0};
@@ -295,7 +310,11 @@ __device__ inline void velo_wide_pass(int n, int wv, int lane, const zd* __restr
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const int row = r0 + kq + 4 * v;
-      if (row < n && bc < n) out[row + (int64_t)n * bc] = zd{p1[v] - p2[v], p3[v] - p1[v] - p2[v]};
+      if (row < n && bc < n) {
+        const zd z = zd{p1[v] - p2[v], p3[v] - p1[v] - p2[v]};
+        out[row + (int64_t)n * bc] = z;
+        if (PASS == 1 && tr != tc) out[bc + (int64_t)n * row] = zconj(z);
+      }
     }
   }
 }
@@ -399,13 +418,17 @@ __global__ __launch_bounds__(256) void velo_lds_kernel(int n, int64_t nk, const 
         }
       }
       __syncthreads();  // T complete
-      // V_a = U^H T, straight to HBM
+      // V_a = U^H T, straight to HBM: the tiles on and below the diagonal
+      // (V_a is Hermitian), each off-diagonal one written twice
+      const int nlt = nt * (nt + 1) / 2;
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
+      for (int u = 0; u < 3; ++u) {
         const int tile = wv + 4 * u;
-        if (tile >= ntile) break;
-        const int r0 = 16 * (tile / nt);
-        const int ar = r0 + lr, bc = 16 * (tile % nt) + lr;
+        if (tile >= nlt) break;
+        int tr, tc;
+        lower_tile(tile, &tr, &tc);
+        const int r0 = 16 * tr;
+        const int ar = r0 + lr, bc = 16 * tc + lr;
         const bool aok = ar < n, bok = bc < n;
         const int arc = aok ? ar : 0, bcc = bok ? bc : 0;
         velo_v4d p1 = {0, 0, 0, 0}, p2 = {0, 0, 0, 0}, p3 = {0, 0, 0, 0};
@@ -422,8 +445,11 @@ __global__ __launch_bounds__(256) void velo_lds_kernel(int n, int64_t nk, const 
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           const int row = r0 + kq + 4 * v;
-          if (row < n && bc < n)
-            Vk[a * (int64_t)n2 + row + (int64_t)n * bc] = zd{p1[v] - p2[v], p3[v] - p1[v] - p2[v]};
+          if (row < n && bc < n) {
+            const zd z = zd{p1[v] - p2[v], p3[v] - p1[v] - p2[v]};
+            Vk[a * (int64_t)n2 + row + (int64_t)n * bc] = z;
+            if (tr != tc) Vk[a * (int64_t)n2 + bc + (int64_t)n * row] = zconj(z);
+          }
         }
       }
       __syncthreads();  // T read before the next component's dH overwrites it
