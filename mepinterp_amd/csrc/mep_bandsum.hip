@@ -1346,18 +1346,19 @@ __global__ __launch_bounds__(256, IPT > 1 ? 2 : 3) void hwsum_kernel(BandArgs b,
                 // ohc weight is ro2 = -f / (dE (dE + w)).  One reciprocal for
                 // the three denominators; f = 0 records (another E_F differs)
                 // contribute exactly zero.
-                const zd denv = zd{dE * dE - wh2[u].re, -wh2[u].im};
-                const double di = dE * wh[u].im;
-                const zd deno = zd{dE * (dE - wh[u].re), -di};  // dE (dE - w)
-                const zd den2 = zd{dE * (dE + wh[u].re), di};   // dE (dE + w)
-                const double av = zabs2(denv), ao = zabs2(deno), a2 = zabs2(den2);
-                const double o2 = ao * a2;
-                const double fi = f * rcp_nr(av * o2);
-                const double iv = 2.0 * fi * o2, io = fi * av * a2, i2 = fi * av * ao;
-                const zd rv = zd{denv.re * iv, -denv.im * iv};  // both orders: 2 f / denv
-                const zd ro = zd{deno.re * io, -deno.im * io};  // f / deno
-                const zd r2 = zd{-den2.re * i2, den2.im * i2};  // -f / den2
-                const zd sp = zadd(ro, r2), sm = zsub(ro, r2);
+                // with p = dE - w, q = dE + w: denv = p q, deno = dE p and
+                // the (y, x) denominator dE q, so R = f / (dE |p|^2 |q|^2)
+                // gives rv = 2 f / denv = 2 dE R conj(p q),
+                // ro = f / deno = |q|^2 R conj(p), ro2 = -|p|^2 R conj(q)
+                const double wr = wh[u].re, wi = wh[u].im;
+                const double pr = dE - wr, qr = dE + wr;
+                const double ap = pr * pr + wi * wi, aq = qr * qr + wi * wi;
+                const double R = f * rcp_nr(dE * ap * aq);
+                const double Rq = R * aq, Rp = R * ap, Rd = 2.0 * R * dE;
+                const zd rv = zd{(dE * dE - wh2[u].re) * Rd, wh2[u].im * Rd};
+                const double t1 = pr * Rq, t2 = qr * Rp;
+                const zd sp = zd{t1 - t2, wi * (Rq + Rp)};  // ro + ro2
+                const zd sm = zd{t1 + t2, wi * (Rq - Rp)};  // ro - ro2
                 // acc: [0..5] rv Im P_c, [6..11] Re(ro) |x_a|^2 then Im(ro) |x_a|^2,
                 // [12..17] Re(ro) P_c, [18..23] Im(ro) P_c, over both orders
 #pragma unroll
