@@ -94,6 +94,12 @@ __device__ inline double rsq_nr(double x) {
   e = fma(-x * y, y, 1.0);
   return fma(0.5 * y, e, y);
 }
+// sqrt(x), x >= 0: x / sqrt(x) from rsq_nr in the normal range (about half
+// the instructions of the IEEE sequence), the IEEE sqrt below it and at 0
+__device__ inline double sqrt_nr(double x) {
+  if (x > 1e-290) return x * rsq_nr(x);
+  return sqrt(x);
+}
 __device__ inline float rcp_nr(float x) {
   float r = __builtin_amdgcn_rcpf(x);
   r = fmaf(r, fmaf(-x, r, 1.0f), r);

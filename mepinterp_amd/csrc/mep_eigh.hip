@@ -268,8 +268,8 @@ __global__ __launch_bounds__(64, 4) void eigh_house_half_kernel(int n, int64_t n
       const zd xt = zd{swap32_sum(xo.re, xo.re), swap32_sum(xo.im, xo.im)};  // x_t in both halves
       const bool refl = tail > 0.0;
       const double ax02 = zabs2(x0);
-      const double xn = sqrt(tail + ax02);
-      const double ax0 = sqrt(ax02);
+      const double xn = sqrt_nr(tail + ax02);
+      const double ax0 = sqrt_nr(ax02);
       const zd alpha = !refl ? x0 : ((ax0 > 0.0) ? zscale(x0, -xn * rcp_nr(ax0)) : zd{-xn, 0.0});
       zd v = (refl && below) ? xt : zero;
       if (t == kc + 1 && refl) v = zsub(xt, alpha);
