@@ -456,8 +456,8 @@ __global__ __launch_bounds__(64, 2) void eigh_house_row2_kernel(int n, int64_t n
       const double tail = team_sum<32, double>(t > kc + 1 ? a2 : 0.0);
       const double ax02 = team_bcast<32>(a2, kc + 1);  // |A(kc+1, kc)|^2 of this k-point
       const bool refl = tail > 0.0;
-      const double xn = sqrt(tail + ax02);
-      const double ax0 = sqrt(ax02);
+      const double xn = sqrt_nr(tail + ax02);
+      const double ax0 = sqrt_nr(ax02);
       // alpha is formed in every lane from its own A(t, kc); lane kc + 1's
       // (x0 = A(kc+1, kc)) is the one kept
       const zd alpha = !refl ? xo : ((ax0 > 0.0) ? zscale(xo, -xn * rcp_nr(ax0)) : zd{-xn, 0.0});
@@ -1336,8 +1336,8 @@ __global__ __launch_bounds__(256) void eigh_house_wide_kernel(int n, int64_t nk,
       const zd x0 = col[0];
       const bool refl = tail2 > 0.0;
       const double ax02 = zabs2(x0);
-      const double xn = sqrt(tail2 + ax02);
-      const double ax0 = sqrt(ax02);
+      const double xn = sqrt_nr(tail2 + ax02);
+      const double ax0 = sqrt_nr(ax02);
       const zd alpha = !refl ? x0 : ((ax0 > 0.0) ? zscale(x0, -xn / ax0) : zd{-xn, 0.0});
       const zd v0 = zsub(x0, alpha);
       const double vn2 = tail2 + zabs2(v0);
@@ -1502,8 +1502,8 @@ __global__ __launch_bounds__(64) void eigh_house_lds_kernel(int n, int64_t nk,
       const double tail = wave_sum(t > kc + 1 ? a2 : 0.0);
       const double ax02 = readlane_d(a2, kc + 1);
       const bool refl = tail > 0.0;
-      const double xn = sqrt(tail + ax02);
-      const double ax0 = sqrt(ax02);
+      const double xn = sqrt_nr(tail + ax02);
+      const double ax0 = sqrt_nr(ax02);
       // lane kc + 1 (x0 = A(kc+1, kc)) keeps its alpha
       const zd alpha = !refl ? xo : ((ax0 > 0.0) ? zscale(xo, -xn * rcp_nr(ax0)) : zd{-xn, 0.0});
       zd v = (refl && below) ? xo : zero;
@@ -1698,8 +1698,8 @@ __global__ __launch_bounds__(4 * NMAX, NMAX <= 96 ? 3 : 2) void eigh_house_rows_
       const zd x0 = vv[b][kc + 1];
       const bool refl = tail > 0.0;
       const double ax02 = zabs2(x0);
-      const double xn = sqrt(tail + ax02);
-      const double ax0 = sqrt(ax02);
+      const double xn = sqrt_nr(tail + ax02);
+      const double ax0 = sqrt_nr(ax02);
       const zd alpha = zsel(!refl, x0, zsel(ax0 > 0.0, zscale(x0, -xn * rcp_nr(ax0)), zd{-xn, 0.0}));
       const zd v0 = zsub(x0, alpha);
       const double vn2 = 2.0 * xn * (xn + ax0);
