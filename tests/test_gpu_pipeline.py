@@ -326,6 +326,25 @@ def test_optical_and_gyrotropic_sums_match_oracle(name):
             assert not np.abs(g[key]).any() and not np.abs(o[key]).any()
 
 
+def test_photogalvanic_sums_with_many_items_match_oracle():
+    """photo_2nd_cond with 300 (a, hw, E_F) items per (b, c): more items than
+    threads, so the kernel's item loop takes two chunks (the second with
+    splits of the pair range), against the oracle's k-loop."""
+    import oracle as O
+    from mepinterp_amd import capi, models
+    m = models.make_random(seed=12, n=12, L=1)
+    mp = (2, 2, 1)
+    jobs = capi.MEP_JOB_AHC | capi.MEP_JOB_OPT
+    kw = dict(n_eF=2, eF_min=-0.01, eF_max=0.01, n_hw=50, hw_min=0.01, hw_max=0.2,
+              T_kelvin=300.0, eta=0.004, valence_bands=4)
+    with _engine(m) as eng:
+        g = eng.run_mesh(mp, _params(jobs, **kw))
+    p = O.RunParams(do_ahc=True, do_opt=True, **kw)
+    o = O.accumulate(oracle_model(m), p, O.mp_mesh(mp))
+    assert np.abs(o["photo"]).max() > 0
+    assert _rel(g["photo"], o["photo"]) < 1e-9, f"photo: {_rel(g['photo'], o['photo']):.2e}"
+
+
 def test_run_batch_equals_per_model_runs():
     """mep_run_batch over models of uneven n_wf and NR (both kernel
     families) gives each model exactly its own mep_run_mesh result."""
