@@ -11,7 +11,7 @@
 // (fixed order, deterministic).  For n <= 32 V is staged in LDS (planar,
 // padded) where it fits beside the occupation tables; otherwise it is read
 // from L2 / HBM.  photo_2nd_cond's m sum is a product per (b, c) shared by
-// every (a, hw, E_F) item when V is in LDS.  A and Omega are read in the
+// every (a, hw, E_F) item, kept in LDS (n <= 64 where it fits).  A and Omega are read in the
 // reference layout (position operator only).  The occupations, their
 // derivative and the per-band curvature sit in dynamic LDS, sized by n and
 // n_eF (ext_lds_bytes).
@@ -43,13 +43,19 @@ __device__ inline int levi3(int a, int b, int c) {
 }
 
 // dynamic LDS: [V planes if LDSV] | E (128) | omega (3 x 128 complex) |
-// fd (n_ef x n) | fd_stat_deriv (n_ef x n) | [if LDSV: photo's W and P
-// (XS_NMAX^2 complex each) and split sums (XS_THREADS complex), with
-// MEP_JOB_OPT]
+// fd (n_ef x n) | fd_stat_deriv (n_ef x n) | [with MEP_JOB_OPT: photo's W
+// and P (XS_NMAX^2 complex each) if LDSV, else P (n^2 complex) where it
+// fits; then split sums (XS_THREADS complex)]
 __host__ __device__ inline size_t ext_lds_doubles(int n, int nef, bool ldsv, bool photo) {
-  return (ldsv ? 2 * 3 * (size_t)XS_PL : 0) +
-         (ldsv && photo ? 2 * 2 * XS_NMAX * XS_NMAX + 2 * XS_THREADS : 0) + 128 + 2 * 3 * 128 +
-         2 * (size_t)nef * n;
+  const size_t base = (ldsv ? 2 * 3 * (size_t)XS_PL : 0) + 128 + 2 * 3 * 128 + 2 * (size_t)nef * n;
+  if (!photo) return base;
+  return base + (ldsv ? 2 * 2 * XS_NMAX * XS_NMAX : 2 * (size_t)n * n) + 2 * XS_THREADS;
+}
+// V from L2: photo_2nd_cond's per-(b, c) product P (n^2 complex) in LDS
+// where it fits beside the occupation tables (n <= 64)
+constexpr int XS_PMAX = 64;
+__host__ __device__ inline bool ext_photo_p(int n, int nef) {
+  return n <= XS_PMAX && sizeof(double) * ext_lds_doubles(n, nef, false, true) <= 160 * 1024;
 }
 
 template <bool LDSV>
@@ -165,30 +171,40 @@ __global__ __launch_bounds__(XS_THREADS) void extsum_kernel(BandArgs b, double T
     // sum_{n, m, l != n} (f_l - f_n) / hw^2 V_a(n,l) V_b(l,m) V_c(m,n)
     //   / (E_n - E_m - i eta) sum_{+-} 1 / (E_n - E_l +- hw - 2 i eta)
     if (doOpt) {
-      if constexpr (LDSV) {
+      if (LDSV || ext_photo_p(n, nef)) {
         // the m sum does not depend on a, hw or E_F: per (b, c) the product
         // P(l, n) = sum_m V_b(l, m) W_c(m, n), W_c(m, n) = V_c(m, n) /
         // (E_n - E_m - i eta), in LDS; then each item (a, hw, E_F) is the
         // O(n^2) contraction with V_a(n, l) (n^3 per (b, c) instead of per
         // item).  Items are split over the threads like the omega-resolved
         // sums: thread (slot, split), splits combined in a fixed order.
-        zd* Wm = reinterpret_cast<zd*>(fd + 2 * (int64_t)nef * n);  // [XS_NMAX^2]
-        zd* Pm = Wm + XS_NMAX * XS_NMAX;                              // [XS_NMAX^2]
-        zd* red = Pm + XS_NMAX * XS_NMAX;                             // [XS_THREADS]
+        // V in LDS: W (XS_NMAX^2) | P (XS_NMAX^2) | red; V from L2: P (n^2) |
+        // red, and W's factor 1 / (E_n - E_m - i eta) formed in the m loop
+        zd* Wm = reinterpret_cast<zd*>(fd + 2 * (int64_t)nef * n);
+        zd* Pm = LDSV ? Wm + XS_NMAX * XS_NMAX : Wm;
+        zd* red = Pm + (LDSV ? XS_NMAX * XS_NMAX : n2);             // [XS_THREADS]
         const int NI = 3 * nhw * nef;
         for (int c = 0; c < 3; ++c) {
           __syncthreads();  // the previous (b, c)'s readers of W are done
-          for (int e = t; e < n2; e += XS_THREADS) {
-            const int m = e % n, nn = e / n;
-            const double x = Es[nn] - Es[m], iv = 1.0 / (x * x + eta * eta);
-            Wm[e] = zmul(Vat(c, m, nn), zd{x * iv, eta * iv});
-          }
+          if (LDSV)
+            for (int e = t; e < n2; e += XS_THREADS) {
+              const int m = e % n, nn = e / n;
+              const double x = Es[nn] - Es[m], iv = 1.0 / (x * x + eta * eta);
+              Wm[e] = zmul(Vat(c, m, nn), zd{x * iv, eta * iv});
+            }
           for (int bb = 0; bb < 3; ++bb) {
             __syncthreads();  // W written; the previous P's readers are done
             for (int e = t; e < n2; e += XS_THREADS) {
               const int l = e % n, nn = e / n;
               zd sp = zd{0.0, 0.0};
-              for (int m = 0; m < n; ++m) sp = zadd(sp, zmul(Vat(bb, l, m), Wm[m + n * nn]));
+              if (LDSV) {
+                for (int m = 0; m < n; ++m) sp = zadd(sp, zmul(Vat(bb, l, m), Wm[m + n * nn]));
+              } else {
+                for (int m = 0; m < n; ++m) {
+                  const double x = Es[nn] - Es[m], iv = 1.0 / (x * x + eta * eta);
+                  sp = zadd(sp, zmul(Vat(bb, l, m), zmul(Vat(c, m, nn), zd{x * iv, eta * iv})));
+                }
+              }
               Pm[e] = sp;
             }
             __syncthreads();
@@ -302,7 +318,10 @@ static bool ext_ldsv(const BandArgs& b) {
 }
 
 size_t ext_lds_bytes(const BandArgs& b) {
-  return sizeof(double) * ext_lds_doubles(b.n, b.n_ef, ext_ldsv(b), (b.jobs & MEP_JOB_OPT) != 0);
+  const bool photo = (b.jobs & MEP_JOB_OPT) != 0;
+  const bool ldsv = ext_ldsv(b);
+  return sizeof(double) *
+         ext_lds_doubles(b.n, b.n_ef, ldsv, photo && (ldsv || ext_photo_p(b.n, b.n_ef)));
 }
 
 void launch_extsum(hipStream_t s, const BandArgs& b, int64_t nk, const double* E, const zd* V,
