@@ -801,7 +801,8 @@ static int band_block(mep_ctx* c, BandSetup& bs, int64_t nb, const double* w, bo
                    fuse ? &vf : nullptr, reinterpret_cast<zd*>(c->dscr));
     if (bs.ext)
       launch_extsum(c->st, bs.b, nb, c->dE, c->dV, c->dA, c->dOm, w, nblk,
-                    c->dpart + (int64_t)bs.nblocks * bs.L.total);
+                    c->dpart + (int64_t)bs.nblocks * bs.L.total,
+                    c->n > 32 ? reinterpret_cast<zd*>(c->dscr) : nullptr);
   });
   HIPCHK(hipGetLastError());
   return MEP_OK;

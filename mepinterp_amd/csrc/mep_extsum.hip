@@ -46,10 +46,12 @@ __device__ inline int levi3(int a, int b, int c) {
 // fd (n_ef x n) | fd_stat_deriv (n_ef x n) | [with MEP_JOB_OPT: photo's W
 // and P (XS_NMAX^2 complex each) if LDSV, else P (n^2 complex) where it
 // fits; then split sums (XS_THREADS complex)]
-__host__ __device__ inline size_t ext_lds_doubles(int n, int nef, bool ldsv, bool photo) {
+__host__ __device__ inline size_t ext_lds_doubles(int n, int nef, bool ldsv, bool photo,
+                                                  bool plds = true) {
   const size_t base = (ldsv ? 2 * 3 * (size_t)XS_PL : 0) + 128 + 2 * 3 * 128 + 2 * (size_t)nef * n;
   if (!photo) return base;
-  return base + (ldsv ? 2 * 2 * XS_NMAX * XS_NMAX : 2 * (size_t)n * n) + 2 * XS_THREADS;
+  return base + (ldsv ? 2 * 2 * XS_NMAX * XS_NMAX : plds ? 2 * (size_t)n * n : 0) +
+         2 * XS_THREADS;
 }
 // V from L2: photo_2nd_cond's per-(b, c) product P (n^2 complex) in LDS
 // where it fits beside the occupation tables (n <= 64)
@@ -65,7 +67,8 @@ __global__ __launch_bounds__(XS_THREADS) void extsum_kernel(BandArgs b, double T
                                                            const zd* __restrict__ A,
                                                            const zd* __restrict__ Om,
                                                            const double* __restrict__ w,
-                                                           double* __restrict__ part) {
+                                                           double* __restrict__ part,
+                                                           zd* __restrict__ scr) {
   extern __shared__ __attribute__((aligned(16))) double xs_smem[];
   zd* Vs = reinterpret_cast<zd*>(xs_smem);                         // LDSV only
   double* Es = xs_smem + (LDSV ? 2 * 3 * XS_PL : 0);
@@ -171,7 +174,9 @@ __global__ __launch_bounds__(XS_THREADS) void extsum_kernel(BandArgs b, double T
     // sum_{n, m, l != n} (f_l - f_n) / hw^2 V_a(n,l) V_b(l,m) V_c(m,n)
     //   / (E_n - E_m - i eta) sum_{+-} 1 / (E_n - E_l +- hw - 2 i eta)
     if (doOpt) {
-      if (LDSV || ext_photo_p(n, nef)) {
+      // P in LDS (V in LDS, or n <= 64), else in the workgroup's slot of scr
+      const bool pl = LDSV || ext_photo_p(n, nef);
+      if (pl || scr) {
         // the m sum does not depend on a, hw or E_F: per (b, c) the product
         // P(l, n) = sum_m V_b(l, m) W_c(m, n), W_c(m, n) = V_c(m, n) /
         // (E_n - E_m - i eta), in LDS; then each item (a, hw, E_F) is the
@@ -181,8 +186,8 @@ __global__ __launch_bounds__(XS_THREADS) void extsum_kernel(BandArgs b, double T
         // V in LDS: W (XS_NMAX^2) | P (XS_NMAX^2) | red; V from L2: P (n^2) |
         // red, and W's factor 1 / (E_n - E_m - i eta) formed in the m loop
         zd* Wm = reinterpret_cast<zd*>(fd + 2 * (int64_t)nef * n);
-        zd* Pm = LDSV ? Wm + XS_NMAX * XS_NMAX : Wm;
-        zd* red = Pm + (LDSV ? XS_NMAX * XS_NMAX : n2);             // [XS_THREADS]
+        zd* Pm = LDSV ? Wm + XS_NMAX * XS_NMAX : pl ? Wm : scr + (int64_t)blockIdx.x * n2;
+        zd* red = LDSV ? Pm + XS_NMAX * XS_NMAX : pl ? Pm + n2 : Wm;  // [XS_THREADS]
         const int NI = 3 * nhw * nef;
         for (int c = 0; c < 3; ++c) {
           __syncthreads();  // the previous (b, c)'s readers of W are done
@@ -317,15 +322,18 @@ static bool ext_ldsv(const BandArgs& b) {
   return b.n <= XS_NMAX && sizeof(double) * ext_lds_doubles(b.n, b.n_ef, true, photo) <= 160 * 1024;
 }
 
+// (with MEP_JOB_OPT and P outside LDS, the split sums still take
+// XS_THREADS complex of LDS)
 size_t ext_lds_bytes(const BandArgs& b) {
   const bool photo = (b.jobs & MEP_JOB_OPT) != 0;
   const bool ldsv = ext_ldsv(b);
   return sizeof(double) *
-         ext_lds_doubles(b.n, b.n_ef, ldsv, photo && (ldsv || ext_photo_p(b.n, b.n_ef)));
+         ext_lds_doubles(b.n, b.n_ef, ldsv, photo, ldsv || ext_photo_p(b.n, b.n_ef));
 }
 
 void launch_extsum(hipStream_t s, const BandArgs& b, int64_t nk, const double* E, const zd* V,
-                   const zd* A, const zd* Om, const double* w, int nblocks, double* part) {
+                   const zd* A, const zd* Om, const double* w, int nblocks, double* part,
+                   zd* scr) {
   ExtLayout X = ext_layout(b);
   const double kB = 8.6173303 * 1e-5 / 27.211385;  // kBoltz_Eh_K (constants.f90:22)
   const size_t lds = ext_lds_bytes(b);
@@ -333,11 +341,11 @@ void launch_extsum(hipStream_t s, const BandArgs& b, int64_t nk, const double* E
     (void)hipFuncSetAttribute((const void*)extsum_kernel<true>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     hipLaunchKernelGGL(extsum_kernel<true>, dim3(nblocks), dim3(XS_THREADS), lds, s, b,
-                       kB * b.T_kelvin, X, nk, E, V, A, Om, w, part);
+                       kB * b.T_kelvin, X, nk, E, V, A, Om, w, part, scr);
   } else {
     (void)hipFuncSetAttribute((const void*)extsum_kernel<false>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     hipLaunchKernelGGL(extsum_kernel<false>, dim3(nblocks), dim3(XS_THREADS), lds, s, b,
-                       kB * b.T_kelvin, X, nk, E, V, A, Om, w, part);
+                       kB * b.T_kelvin, X, nk, E, V, A, Om, w, part, scr);
   }
 }

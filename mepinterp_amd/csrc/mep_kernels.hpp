@@ -135,8 +135,11 @@ struct ExtLayout {
 ExtLayout ext_layout(const BandArgs& b);
 // dynamic LDS of the optical / gyrotropic kernel (must stay within 160 KiB)
 size_t ext_lds_bytes(const BandArgs& b);
+// scr: nblocks slots of n^2 complex (photo_2nd_cond's per-(b, c) product
+// where it does not fit in LDS), or null to keep the per-item loop there
 void launch_extsum(hipStream_t s, const BandArgs& b, int64_t nk, const double* E, const zd* V,
-                   const zd* A, const zd* Om, const double* w, int nblocks, double* part);
+                   const zd* A, const zd* Om, const double* w, int nblocks, double* part,
+                   zd* scr);
 
 // per-k Berry curvature summed over occupied bands at one E_F (V planar):
 // out (nk, 9), element i + 3j

@@ -479,12 +479,13 @@ def test_position_operator_beyond_16_bands_matches_oracle(n, textbook):
     _compare_sums(g, o)
 
 
-@pytest.mark.parametrize("n,n_eF,with_r", [(40, 3, True), (24, 20, False)])
+@pytest.mark.parametrize("n,n_eF,with_r", [(40, 3, True), (24, 20, False), (72, 2, False)])
 def test_optical_and_gyrotropic_sums_beyond_32_bands_and_16_levels(n, n_eF, with_r):
     """The optical / photogalvanic / gyrotropic sums above the n_wf = 32 and
     N_eF = 16 of the LDS-sized kernel: V read from L2 / HBM for n = 40 (with
     a position operator, so kubo_opt_tens and get_gyro_Dw run), 20 Fermi
-    levels at n = 24, against the oracle's k-loop."""
+    levels at n = 24, and n = 72, whose photogalvanic product lives in HBM
+    scratch, against the oracle's k-loop."""
     import oracle as O
     from mepinterp_amd import capi, models
     m = models.make_random(seed=300 + n, n=n, L=1, with_r=with_r)
