@@ -890,7 +890,7 @@ __global__ __launch_bounds__(256) void bandsum_wide_kernel(BandArgs b, double Ts
   double* F2m = F2 + 9 * n;           // [9][n]
   double* red = F2m + 9 * n;          // [4][36]
   uint8_t* win = reinterpret_cast<uint8_t*>(red + 4 * 36);  // [nval][n] lo, hi
-  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const int t = threadIdx.x, lane = t & 63, wv = __builtin_amdgcn_readfirstlane(t >> 6);
   const int n2 = n * n;
   const bool finiteT = b.T_kelvin > 1e-2;
   const bool doAHC = (b.jobs & MEP_JOB_AHC) != 0;
@@ -1038,15 +1038,18 @@ __global__ __launch_bounds__(256) void bandsum_wide_kernel(BandArgs b, double Ts
     // B_a(m, n0) = V_a(m, n0) r(m, n0), r(m, n0) = 1 / (E_n0 - E_m) (0 on the
     // diagonal): one division per pair, into the workgroup's slot of Bscr,
     // so the MFMA loop below only loads (in chunks of BW_CH steps whose
-    // loads are all issued before the chunk's products)
+    // loads are all issued before the chunk's products).  Bscr holds B
+    // transposed (n0 + n m), so the 16 columns n0 of a tile load as one run
     if (doK || doNjob) {
       const int nt = (n + 15) / 16;
       const int lr = lane & 15, kq = lane >> 4;
       zd* Bw = Bscr + (int64_t)blockIdx.x * 3 * n2;
       for (int idx = t; idx < n2; idx += 256) {
-        const double rr = rinv(Es, idx % n, idx / n);
+        const int n0 = idx % n, m = idx / n;
+        const double rr = rinv(Es, m, n0);
 #pragma unroll
-        for (int a = 0; a < 3; ++a) Bw[a * (int64_t)n2 + idx] = zscale(Vk[a * (int64_t)n2 + idx], rr);
+        for (int a = 0; a < 3; ++a)
+          Bw[a * (int64_t)n2 + idx] = zscale(Vk[a * (int64_t)n2 + m + (int64_t)n * n0], rr);
       }
       __syncthreads();
       for (int item = wv; item < 3 * nt; item += 4) {
@@ -1075,8 +1078,11 @@ __global__ __launch_bounds__(256) void bandsum_wide_kernel(BandArgs b, double Ts
                   a1[i] = Vl_[ar + (int64_t)n * m];
                 }
                 if (colok) {
-                  b0[i] = Bl_[m + (int64_t)n * n0];
-                  const zd bk = Bk_[m + (int64_t)n * n0];
+                  // uniform row base per step, one 32-bit lane offset
+                  const int64_t rb = (int64_t)n * (m0 + 4 * i);
+                  const int lo = n0 + n * kq;
+                  b0[i] = (Bl_ + rb)[lo];
+                  const zd bk = (Bk_ + rb)[lo];
                   b1[i] = zd{-bk.re, -bk.im};
                 }
               }
